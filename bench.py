@@ -1,0 +1,157 @@
+"""Benchmark: Tmrt+UTCI pixel-hours/s of the batched day forward on one MI355X.
+
+A step is one day (24 hourly steps) of the batched forward over the 2000x2000 c2 workload
+(seeded synthetic morphology, its 30-day forcing cycled): per step the host builds the day's
+tables, uploads them and the device runs 24 x (shadow pass + pixel pass); Tmrt and UTCI stay in
+device memory. Static preparation (sky view) runs before the timed region and is reported apart.
+
+Prints one JSON line (see the fields below). ``--gpus N`` with N > 1 is refused: this build runs
+on one GPU.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, 'solweig-cuda_amd'))
+
+import numpy as np  # noqa: E402
+
+B_ALG = 157.5          # algorithmic bytes per pixel-hour of the forward (static inputs + outputs)
+HBM_PEAK = 8.0e12      # MI355X HBM3E bytes/s
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(hours=(10, 11, 12, 13), size=200):
+    """The FP64 NumPy oracle (a port of the UMEP step) on the 200x200 c0 workload, a bounded
+    sample of daytime hours after its static preparation, on one core."""
+    os.environ.setdefault('OMP_NUM_THREADS', '1')
+    try:
+        os.sched_setaffinity(0, {sorted(os.sched_getaffinity(0))[0]})
+    except (AttributeError, OSError):
+        pass
+    from solweig_amd import synthetic, sun as psun
+    from oracle import day as oday, umep
+    d = synthetic.make_c0(size=size)
+    s = psun.met_sun(d.met, d.location, d.utc)
+    t0 = time.perf_counter()
+    p = oday.prepare(d.dsm, d.trees, d.dem, d.walls, d.aspect, d.latitude)
+    t_static = time.perf_counter() - t0
+    hrs = list(hours)
+    met = d.met
+    sub = {k: np.asarray(v)[hrs[0]:hrs[-1] + 1] for k, v in s.items()}
+    t0 = time.perf_counter()
+    oday.run_day(p, met[hrs[0]:hrs[-1] + 1], sub)
+    dt = time.perf_counter() - t0
+    px_h = size * size * len(hrs)
+    return dict(value=px_h / dt, unit='pixel-hours/s', cores=1, kind='port',
+                sample=f'oracle FP64 NumPy UMEP step on c0 {size}x{size}, hours {hrs[0]}-{hrs[-1]} '
+                       f'({px_h} px-h in {dt:.1f} s; static SVF prep {t_static:.1f} s excluded)')
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument('--gpus', type=int, default=1)
+    ap.add_argument('--steps', type=int, default=3)
+    ap.add_argument('--warmup', type=int, default=1)
+    ap.add_argument('--size', type=int, default=2000)
+    ap.add_argument('--graph', action='store_true', help='replay the day as a captured hipGraph')
+    ap.add_argument('--no-cpu', action='store_true')
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f'bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU sharding is a '
+              f'design note, not part of this build', file=sys.stderr)
+        sys.exit(2)
+    from solweig_amd import runtime as rt, synthetic, batched
+    days = 30 if args.size == 2000 else 3
+    log(f'[bench] generating c2 morphology {args.size}x{args.size}')
+    t0 = time.perf_counter()
+    dom = synthetic.make_c1(seed=2, size=args.size, days=days, start=__import__('datetime').date(2025, 6, 6),
+                            name='c2')
+    log(f'[bench] generated in {time.perf_counter() - t0:.1f} s; static preparation')
+    t0 = time.perf_counter()
+    eng = batched.Engine([dom], n_hours=24)
+    rt.synchronize()
+    t_static = time.perf_counter() - t0
+    log(f'[bench] static prep {t_static:.2f} s')
+    P = eng.domains[0].P
+    ev = [rt.Event() for _ in range(3 * 24 + 1)]
+
+    def one_day(k, timed):
+        tabs = eng.tables(day_index=k % days)
+        eng.load(tabs)
+        eng.reset_state()
+        if args.graph and not timed:
+            eng.forward_day(use_graph=True)
+            return None
+        if args.graph:
+            ev[0].record(eng.stream)
+            eng.forward_day(use_graph=True)
+            ev[1].record(eng.stream)
+            return None
+        for h in range(24):
+            ev[3 * h].record(eng.stream)
+            rt.call('sw_hour_shadow', eng.dom_dev.ptr, eng.block_start.ptr, 1, eng.total_blocks, h, 0,
+                    eng.stream.handle)
+            ev[3 * h + 1].record(eng.stream)
+            rt.call('sw_hour_pixels', eng.dom_dev.ptr, eng.block_start.ptr, 1, eng.total_blocks, h,
+                    eng.gvf_off.ptr, eng.gvf_bounds.ptr, eng.first, eng.second, 0, eng.stream.handle)
+            ev[3 * h + 2].record(eng.stream)
+        return tabs
+
+    for k in range(args.warmup):
+        one_day(k, False)
+        eng.stream.sync()
+        log(f'[bench] warmup day {k} done')
+    rt.synchronize()
+    t_shadow = t_pixel = 0.0
+    n_day_hours = 0
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        tabs = one_day(args.warmup + k, True)
+        eng.stream.sync()
+        if tabs is not None:
+            for h in range(24):
+                if tabs[0].hours[h, 0]:
+                    n_day_hours += 1
+                t_shadow += ev[3 * h].elapsed_ms(ev[3 * h + 1])
+                t_pixel += ev[3 * h + 1].elapsed_ms(ev[3 * h + 2])
+        log(f'[bench] day {k} done')
+    rt.synchronize()
+    dt = time.perf_counter() - t0
+    px_h = P * 24 * args.steps
+    value = px_h / dt
+    launches = 24 * args.steps
+    pair_s = (t_shadow + t_pixel) / 1e3 / launches if launches and t_pixel else None
+    tmrt, utci = eng.outputs(0)
+    log(f'[bench] Tmrt range {np.nanmin(tmrt):.2f}..{np.nanmax(tmrt):.2f}, UTCI {np.nanmin(utci):.2f}..{np.nanmax(utci):.2f}')
+    roof = None
+    if pair_s:
+        achieved = B_ALG * P / pair_s
+        roof = dict(bound='hbm', achieved=achieved / 1e9, peak=HBM_PEAK / 1e9, unit='GB/s',
+                    frac=achieved / HBM_PEAK, traffic=None,
+                    kernel='k_hour_shadow + k_hour_pixels (one hour of the forward)',
+                    avg_ms=dict(k_hour_shadow=t_shadow / launches, k_hour_pixels=t_pixel / launches,
+                                daytime_hours=n_day_hours / max(args.steps, 1)))
+    cpu = None if args.no_cpu else cpu_baseline()
+    out = dict(metric='Tmrt+UTCI pixel-hours/s at 2000^2 x24h, 1 GPU; batched forward',
+               value=value, unit='pixel-hours/s', n_gpus=1, steps=args.steps, warmup=args.warmup,
+               ms_per_step=dt / args.steps * 1e3, higher_is_better=True, scaling='weak', vs_baseline=None,
+               dtype='f32 (f64 masks/lag/wedge/UTCI)', data='synthetic',
+               config=dict(workload=f'c2 batched forward {args.size}x{args.size}, 24 h per step '
+                                    f'(30-day forcing cycled), host day tables inside the timed region',
+                           graph=bool(args.graph), static_prep_s=t_static),
+               roofline=roof, cpu_baseline=cpu)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == '__main__':
+    main()

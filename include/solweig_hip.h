@@ -1,0 +1,160 @@
+/* C ABI of libsolweig_hip.so - the MI355X (gfx950) SOLWEIG 2022a radiation engine.
+ *
+ * Every function returns 0 on success and a negative code on failure; sw_last_error() gives
+ * the message of the last failure on the calling thread. Pointers named *_dev are device
+ * pointers (from sw_malloc or any HIP allocation); ``stream`` is a hipStream_t (NULL = the
+ * null stream). Rasters are row-major [rows][cols]; planes are stacked [plane][rows*cols].
+ *
+ * Each entry point names the reference interface it replaces (zetingli/SOLWEIG-CUDA).
+ */
+#ifndef SOLWEIG_HIP_H
+#define SOLWEIG_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    SW_OK = 0,
+    SW_ERR_ARGUMENT = -1,
+    SW_ERR_HIP = -2,
+    SW_ERR_LIMIT = -3,
+};
+
+/* ----- per-hour parameter record: SW_HOUR_STRIDE doubles per (domain, hour) ------------- */
+enum {
+    SWH_DAY = 0,          /* 1 when the sun altitude > 0 (model.py:96)                        */
+    SWH_ALTITUDE, SWH_AZIMUTH, SWH_TANZEN,
+    SWH_TA, SWH_RADG, SWH_RADD, SWH_RADI, SWH_PSI,
+    SWH_SINALT, SWH_COSALT,                  /* sin/cos(altitude*pi/180)                      */
+    SWH_ALTMAX, SWH_FRAC, SWH_SNUP, SWH_CITGG, /* surface temperature wave (model.py:141-142)  */
+    SWH_LAG_WEIGHT, SWH_LAG_FIRST, SWH_LAG_UPDATE, /* TsWaveDelay of the Lup maps (742-756)  */
+    SWH_FACE_LOW, SWH_FACE_HIGH, SWH_FACE_MIDDLE,  /* self-shadowed wall bounds (678-684)    */
+    SWH_RAY_START, SWH_RAY_LEN,              /* sun ray slice in the ray tables               */
+    SWH_LWALL,            /* SBC*ewall*((Tgwall+Ta+273.15)^4-(Ta+273.15)^4), gvf.cu field 7 */
+    SWH_FLUX4,            /* SBC*(Ta+273.15)^4                                             */
+    SWH_LW_AMBIENT, SWH_LW_SUNLIT, SWH_LW_REFL,  /* ewall*SBC*T^4/pi, (1-ewall)*0.5/pi     */
+    SWH_SW_SUNLIT, SWH_SW_SHADED,            /* (alb*radI*cos+radD/2)/pi, alb*radD/2/pi        */
+    SWH_TA32,             /* float32(Ta) as used by the UTCI path (series.py:466)           */
+    SWH_UTCI0,            /* 7 coefficients of UTCI in powers of dTmrt                     */
+    SWH_PATCH_TABLE = SWH_UTCI0 + 7,         /* row of the patch tables for this hour         */
+    SWH_TGOUT_WEIGHT, SWH_TGOUT_UPDATE,      /* TsWaveDelay of TgOut (model.py:161-163)       */
+    SWH_ALBEDO_B, SWH_ABSK, SWH_ABSL, SWH_FUP, SWH_FSIDE, SWH_FCYL,
+    SWH_DIFF_VEG,         /* diffsh of a vegetation-blocked patch: 1-(1-0.03f) in FP32       */
+    SWH_TGWALL,
+    SW_HOUR_STRIDE = 64
+};
+
+/* patch tables per hour row: SW_PATCH_F floats and SW_PATCH_D doubles per patch */
+enum {
+    SWP_LV = 0, SWP_KD, SWP_LDN, SWP_LSD, SWP_WCOS, SWP_WSIN, SWP_DE, SWP_DW, SWP_DN, SWP_DS,
+    SWP_ELIGIBLE, SW_PATCH_F = 12
+};
+enum { SWP_YI = 0, SWP_SUN_BELOW, SWP_SHADE_FROM, SW_PATCH_D = 4 };
+enum { SW_PATCHES = 153, SW_VIS_WORDS = 5 };
+
+/* FULL-mode output planes of sw_hour_pixels (the per-step 39-tuple's per-pixel members) */
+enum {
+    SWF_KDOWN = 0, SWF_KUP, SWF_LDOWN, SWF_LUP, SWF_TG, SWF_KEAST, SWF_KSOUTH, SWF_KWEST,
+    SWF_KNORTH, SWF_LEAST, SWF_LSOUTH, SWF_LWEST, SWF_LNORTH, SWF_KSIDEI, SWF_TGOUT, SWF_LSIDE,
+    SWF_KSIDED, SWF_DRAD, SWF_KSIDE, SWF_SHADOW, SWF_WALLSUN, SWF_FSH, SW_FULL_PLANES
+};
+
+/* Device-side description of one domain. All fields are 8 bytes wide; pointers are device
+ * pointers. Optional outputs may be NULL. */
+typedef struct sw_domain {
+    int64_t rows, cols, pixels, landcover;
+    double dsm_raise, reserved0;
+    const float *dsm, *trees, *walls, *aspect;               /* [P]; trees clipped >= 0       */
+    const float *emis, *alb, *tgk, *tstart, *tmaxlst, *svfbuveg; /* [P]                      */
+    const double *tan_asvf, *tan_svfalfa;                    /* [P]                           */
+    const uint8_t *ground;                                   /* [P] 1 where dsm-dem < 2        */
+    const uint32_t *vis;                                     /* [3][5][P] sh, veg, vb bits     */
+    void *rec;                                               /* [P] float4 hour scratch         */
+    float *lag;                                              /* [6][P] Tgmap1, E, S, W, N, TgOut1 */
+    float *tmrt, *utci;                                      /* [n_hours][P] (may be NULL)     */
+    const double *hours;                                     /* [n_hours][SW_HOUR_STRIDE]     */
+    const float *ptab_f;                                     /* [rows][153][SW_PATCH_F]       */
+    const double *ptab_d;                                    /* [rows][153][SW_PATCH_D]       */
+    const int32_t *ray_off;                                  /* [sum L][2] (row, col) shifts  */
+    const double *ray_dz;                                    /* [sum L] ray heights            */
+    uint8_t *masks;                                          /* [P] bits of the last shadow pass */
+    double *shadow_planes;                                   /* [8][P] cast_shadow planes      */
+    float *full;                                             /* [SW_FULL_PLANES][P]           */
+} sw_domain;
+
+/* ----- runtime ------------------------------------------------------------------------ */
+int sw_version(void);
+const char *sw_last_error(void);
+int sw_device_count(int *count);
+int sw_set_device(int device);
+int sw_malloc(void **ptr_dev, size_t bytes);
+int sw_free(void *ptr_dev);
+int sw_memcpy_h2d(void *dst_dev, const void *src, size_t bytes, void *stream);
+int sw_memcpy_d2h(void *dst, const void *src_dev, size_t bytes, void *stream);
+int sw_memcpy_d2d(void *dst_dev, const void *src_dev, size_t bytes, void *stream);
+int sw_memset(void *dst_dev, int value, size_t bytes, void *stream);
+int sw_stream_create(void **stream);
+int sw_stream_destroy(void *stream);
+int sw_stream_sync(void *stream);
+int sw_event_create(void **event);
+int sw_event_destroy(void *event);
+int sw_event_record(void *event, void *stream);
+int sw_event_elapsed_ms(void *start, void *stop, float *ms);
+int sw_device_sync(void);
+
+/* ----- static sky view (replaces svf.svf_calculator, svf.py:81-114; geometry.cu:77-159) -- */
+/* Visibility bits of 153 patches for every pixel: bit k of word w of mask m is patch 32w+k;
+ * m = 0 sky not blocked by buildings (shmat), 1 not vegetation-blocked (vegmat), 2 vbmat.   */
+int sw_svf_visibility(const float *dsm_dev, const float *trees_dev, double dsm_raise,
+                      int rows, int cols, const int32_t *offsets_dev, const double *heights_dev,
+                      const int32_t *lengths_dev, int capacity, int patches,
+                      uint32_t *bits_dev, void *stream);
+/* Band-weighted reduction to 15 SVF planes + svftotal, FP64 [16][P] in the kernel order
+ * (total, E, S, W, N for building, vegetation, vegetation-blocking-building).              */
+int sw_svf_reduce(const uint32_t *bits_dev, const float *dsm_dev, const float *trees_dev,
+                  double dsm_raise, int rows, int cols, const uint8_t *sectors_dev,
+                  const int32_t *band_offsets_dev, const double *weights_dev, int bands,
+                  int patches, double transmissivity, double *out_dev, void *stream);
+
+/* ----- per-hour path ----------------------------------------------------------------- */
+/* Sun-ray shadow march for hour ``hour`` of every domain (replaces model.cast_shadow,
+ * model.py:689-708, and geometry.cu:185-253). ``domains_dev`` is a device array of n_domains
+ * sw_domain; ``block_start_dev`` [n_domains+1] the cumulative 32x8 tile counts.
+ * full_planes != 0 also writes the 8 FP64 cast_shadow planes to shadow_planes.             */
+int sw_hour_shadow(const sw_domain *domains_dev, const int32_t *block_start_dev, int n_domains,
+                   int total_blocks, int hour, int full_planes, void *stream);
+/* Ground view factors, lag recursion, patch sums, Kdown/Kup/Kside/Ldown/Lside, Tmrt and UTCI
+ * for hour ``hour`` (replaces Solweig_2022a_calc's per-pixel stages, model.py:141-251, with
+ * gvf.cu, shortwave.cu, longwave.cu, flux.cu and utci.cu). full != 0 writes the FULL planes.
+ * gvf_offsets_dev [18][second][2], gvf_bounds_dev [18][3] as prepare_gvf_geometry builds them. */
+int sw_hour_pixels(const sw_domain *domains_dev, const int32_t *block_start_dev, int n_domains,
+                   int total_blocks, int hour, const int32_t *gvf_offsets_dev,
+                   const double *gvf_bounds_dev, int first_steps, int second_steps, int full,
+                   void *stream);
+/* Every hour in [hour_begin, hour_end): the two passes above back to back on one stream
+ * (the day loop of series.py:445-469 without host round trips).                           */
+int sw_hours_forward(const sw_domain *domains_dev, const int32_t *block_start_dev, int n_domains,
+                     int total_blocks, int hour_begin, int hour_end, const int32_t *gvf_offsets_dev,
+                     const double *gvf_bounds_dev, int first_steps, int second_steps, void *stream);
+
+/* UTCI of a Tmrt plane for scalar weather (replaces utci.utci_calculator's scalar-weather
+ * path, utci.py:22-33, utci.cu:243-523): ``coefficients`` are 7 host FP64 values c_k with
+ * UTCI = sum_k c_k (Tmrt - ta32)^k; sentinel != 0 writes -999; ground_dev (nullable) NaNs
+ * pixels where it is 0 (series.py:468-469).                                                */
+int sw_utci_scalar(const float *tmrt_dev, long long n, const double *coefficients, double ta32,
+                   int sentinel, const uint8_t *ground_dev, float *out_dev, void *stream);
+
+/* hipGraph capture of a launch sequence on ``stream`` (the captured day loop).            */
+int sw_graph_begin(void *stream);
+int sw_graph_end(void *stream, void **graph_exec);
+int sw_graph_launch(void *graph_exec, void *stream);
+int sw_graph_destroy(void *graph_exec);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

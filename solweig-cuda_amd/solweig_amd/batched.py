@@ -1,0 +1,128 @@
+"""Batched day forward: every hour of a day, for one or many uneven domains, on one stream.
+
+Per day the host builds each domain's hour tables (sky.build_day), uploads them into buffers of
+fixed capacity, and the device runs, per hour, one shadow launch and one pixel launch that cover
+the tiles of all domains together (sw_hours_forward). Lag state stays on the device across the
+hours of a day and is cleared at each day start (series.py:425-436). Because buffer addresses
+never change, the 2 x n_hours launches of a day can be captured once as a hipGraph and replayed
+for every further day after the tables are refreshed.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import runtime as rt
+from .sky import build_day, gvf_geometry
+from .static import PreparedDomain
+from .sun import met_sun
+
+
+class Engine:
+    def __init__(self, domains, n_hours=24, stream=None, keep_outputs=True):
+        self.domains = [d if isinstance(d, PreparedDomain) else PreparedDomain(d) for d in domains]
+        self.n_hours = int(n_hours)
+        self.stream = stream if stream is not None else rt.Stream()
+        off, bounds, self.first, self.second = gvf_geometry(self.domains[0].domain.scale)
+        self.gvf_off = rt.DeviceArray.from_host(off, np.int32)
+        self.gvf_bounds = rt.DeviceArray.from_host(bounds, np.float64)
+        self.buf = []
+        for d in self.domains:
+            cap = self.n_hours * (max(d.shape) + 2)
+            b = dict(hours=rt.DeviceArray((self.n_hours, rt.SW_HOUR_STRIDE), np.float64),
+                     ptab_f=rt.DeviceArray((self.n_hours, rt.SW_PATCHES, rt.SW_PATCH_F), np.float32),
+                     ptab_d=rt.DeviceArray((self.n_hours, rt.SW_PATCHES, rt.SW_PATCH_D), np.float64),
+                     ray_off=rt.DeviceArray((cap, 2), np.int32), ray_dz=rt.DeviceArray((cap,), np.float64),
+                     cap=cap)
+            if keep_outputs:
+                b['tmrt'] = rt.DeviceArray((self.n_hours, d.P), np.float32)
+                b['utci'] = rt.DeviceArray((self.n_hours, d.P), np.float32)
+            self.buf.append(b)
+        starts = np.zeros(len(self.domains) + 1, dtype=np.int32)
+        starts[1:] = np.cumsum([d.tiles for d in self.domains])
+        self.total_blocks = int(starts[-1])
+        self.block_start = rt.DeviceArray.from_host(starts)
+        self.structs = (rt.SwDomain * len(self.domains))()
+        for i, (d, b) in enumerate(zip(self.domains, self.buf)):
+            self.structs[i] = d.struct(hours=b['hours'].ptr, ptab_f=b['ptab_f'].ptr, ptab_d=b['ptab_d'].ptr,
+                                       ray_off=b['ray_off'].ptr, ray_dz=b['ray_dz'].ptr,
+                                       tmrt=b['tmrt'].ptr if 'tmrt' in b else 0,
+                                       utci=b['utci'].ptr if 'utci' in b else 0)
+        self.dom_dev = rt.DeviceArray((ctypes.sizeof(self.structs),), np.uint8)
+        rt.call('sw_memcpy_h2d', self.dom_dev.ptr, ctypes.addressof(self.structs),
+                ctypes.sizeof(self.structs), self.stream.handle)
+        self.stream.sync()
+        self.graph = None
+
+    # ------------------------------------------------------------------ tables
+    def tables(self, day_index=0, met=None, sun=None):
+        """Host tables of one day for every domain. ``met`` / ``sun`` override per domain."""
+        out = []
+        for k, d in enumerate(self.domains):
+            dm = d.domain
+            rows = met[k] if met is not None else dm.met[day_index * self.n_hours:(day_index + 1) * self.n_hours]
+            s = sun[k] if sun is not None else met_sun(rows, dm.location, dm.utc)
+            out.append(build_day(rows, s, dm.latitude, d.shape, dm.scale, d.amaxvalue,
+                                 landcover=d.landcover, wall_params=d.wall_params))
+        return out
+
+    def load(self, tables):
+        """Upload one day's tables (fixed addresses, so a captured graph stays valid)."""
+        for t, b in zip(tables, self.buf):
+            if t.n_hours != self.n_hours:
+                raise ValueError(f'expected {self.n_hours} hours, got {t.n_hours}')
+            if t.ray_dz.shape[0] > b['cap']:
+                raise ValueError('ray tables exceed their buffer')
+            b['hours'].upload(t.hours, self.stream)
+            b['ptab_f'].upload(t.ptab_f, self.stream)
+            b['ptab_d'].upload(t.ptab_d, self.stream)
+            n = t.ray_dz.shape[0]
+            rt.call('sw_memcpy_h2d', b['ray_off'].ptr, t.ray_off.ctypes.data, t.ray_off.nbytes, self.stream.handle)
+            rt.call('sw_memcpy_h2d', b['ray_dz'].ptr, t.ray_dz.ctypes.data, n * 8, self.stream.handle)
+            self.stream.sync()
+
+    def reset_state(self):
+        for d in self.domains:
+            d.dev['lag'].zero(self.stream)
+
+    # ------------------------------------------------------------------ launches
+    def _launch_hours(self, h0, h1):
+        rt.call('sw_hours_forward', self.dom_dev.ptr, self.block_start.ptr, len(self.domains),
+                self.total_blocks, h0, h1, self.gvf_off.ptr, self.gvf_bounds.ptr, self.first,
+                self.second, self.stream.handle)
+
+    def forward_day(self, use_graph=False):
+        """Enqueue all hours of the loaded day (state must be reset by the caller per day)."""
+        if use_graph:
+            if self.graph is None:
+                self.graph = rt.Graph.capture(self.stream, lambda: self._launch_hours(0, self.n_hours))
+            self.graph.launch(self.stream)
+        else:
+            self._launch_hours(0, self.n_hours)
+
+    def hour(self, h, full=False):
+        """Both passes of one hour (for step-wise use and tests)."""
+        rt.call('sw_hour_shadow', self.dom_dev.ptr, self.block_start.ptr, len(self.domains),
+                self.total_blocks, h, int(full), self.stream.handle)
+        rt.call('sw_hour_pixels', self.dom_dev.ptr, self.block_start.ptr, len(self.domains),
+                self.total_blocks, h, self.gvf_off.ptr, self.gvf_bounds.ptr, self.first, self.second,
+                int(full), self.stream.handle)
+
+    def outputs(self, k=0):
+        """Host (Tmrt, UTCI) [n_hours, rows, cols] of domain k for the last day run."""
+        self.stream.sync()
+        b, d = self.buf[k], self.domains[k]
+        return (b['tmrt'].to_host().reshape(self.n_hours, *d.shape),
+                b['utci'].to_host().reshape(self.n_hours, *d.shape))
+
+
+def run_day(domain, met=None, sun=None, use_graph=False):
+    """One day of Tmrt/UTCI for a synthetic.Domain (host arrays out)."""
+    met = domain.met if met is None else met
+    eng = Engine([domain], n_hours=met.shape[0])
+    t = eng.tables(met=[met], sun=None if sun is None else [sun])
+    eng.load(t)
+    eng.reset_state()
+    eng.forward_day(use_graph)
+    return eng.outputs(0)

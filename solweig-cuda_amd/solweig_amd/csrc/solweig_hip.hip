@@ -1,0 +1,869 @@
+// MI355X (gfx950) kernels of the SOLWEIG 2022a per-pixel-hour radiation path and their C ABI.
+//
+// Layout: every raster is a row-major plane [rows][cols]; a domain is described by an
+// sw_domain record in device memory; per-(domain, hour) scalars live in an FP64 table of
+// SW_HOUR_STRIDE values built on the host. Work is tiled 32 columns x 8 rows per 256-thread
+// workgroup (a wave covers two 32-pixel row segments), tiles of all domains of a launch are
+// concatenated, and the workgroup index is remapped so that consecutive tiles stay on one XCD
+// (one L2) of the 8.
+//
+// Exactness: the building/vegetation march (shadows and sky-view visibility) compares FP64
+// values formed exactly as the NumPy reference forms them (float32 rasters widened to FP64,
+// vegetation DSMs as FP64 sums, no contraction), so the masks are bit-identical to
+// umep_solweig.py. Radiation sums run in FP32; wedge, lag blend, surface temperature and the
+// UTCI polynomial evaluation run in FP64.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+
+#include "../../../include/solweig_hip.h"
+
+namespace {
+
+constexpr int TILE_X = 32;
+constexpr int TILE_Y = 8;
+constexpr int BLOCK = TILE_X * TILE_Y;
+constexpr int NXCD = 8;
+constexpr double SBC = 5.67051e-8;
+constexpr int MAX_SECOND = 64;
+
+thread_local std::string g_error;
+
+int fail(int code, const std::string &msg) {
+    g_error = msg;
+    return code;
+}
+
+int hip_check(hipError_t e, const char *what) {
+    if (e == hipSuccess) return SW_OK;
+    return fail(SW_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+// Workgroup i is dispatched to XCD i % 8: give each XCD a contiguous run of tiles.
+__device__ __forceinline__ int xcd_remap(int i, int total) {
+    const int q = total / NXCD, r = total % NXCD;
+    const int x = i % NXCD, k = i / NXCD;
+    return (x < r) ? x * (q + 1) + k : r * (q + 1) + (x - r) * q + k;
+}
+
+__device__ __forceinline__ int find_domain(const int32_t *bs, int nd, int b) {
+    int lo = 0, hi = nd - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (bs[mid] <= b) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
+struct PixelAt {
+    int d, r, c;
+    long long p;
+    bool valid;
+};
+
+__device__ __forceinline__ PixelAt locate(const sw_domain *doms, const int32_t *bs, int nd,
+                                          int total, const sw_domain *&D) {
+    PixelAt at;
+    const int b = xcd_remap((int)blockIdx.x, total);
+    at.d = find_domain(bs, nd, b);
+    D = doms + at.d;
+    const int t = b - bs[at.d];
+    const int cols = (int)D->cols, rows = (int)D->rows;
+    const int tpr = (cols + TILE_X - 1) / TILE_X;
+    at.r = (t / tpr) * TILE_Y + (int)threadIdx.x / TILE_X;
+    at.c = (t % tpr) * TILE_X + (int)threadIdx.x % TILE_X;
+    at.valid = at.r < rows && at.c < cols;
+    at.p = (long long)at.r * cols + at.c;
+    return at;
+}
+
+// FP64 heights of a pixel as series.py:252-287 forms them: a = dsm (+ raise), vegetation
+// canopy DSM = trees + a and trunk-zone DSM = 0.25 trees + a, each zeroed where it equals a.
+struct Heights {
+    double a, top, bot;
+};
+
+__device__ __forceinline__ Heights heights_at(const float *dsm, const float *trees, double raise,
+                                              long long q) {
+#pragma clang fp contract(off)
+    Heights h;
+    h.a = (double)dsm[q] + raise;
+    const double t = (double)trees[q];
+    const double v1 = t + h.a;
+    h.top = (v1 == h.a) ? 0.0 : v1;
+    const double v2 = t * 0.25 + h.a;
+    h.bot = (v2 == h.a) ? 0.0 : v2;
+    return h;
+}
+
+// =====================================================================================
+// Static sky-view visibility: thread per (pixel, 32-patch word); geometry.cu:77-119.
+// =====================================================================================
+__global__ __launch_bounds__(BLOCK) void k_svf_visibility(
+    const float *__restrict__ dsm, const float *__restrict__ trees, double raise, int rows,
+    int cols, const int32_t *__restrict__ offsets, const double *__restrict__ heights,
+    const int32_t *__restrict__ lengths, int capacity, int patches, uint32_t *__restrict__ bits,
+    int tiles) {
+#pragma clang fp contract(off)
+    const int tpr = (cols + TILE_X - 1) / TILE_X;
+    const int b = xcd_remap((int)blockIdx.x, tiles);
+    const int r = (b / tpr) * TILE_Y + (int)threadIdx.x / TILE_X;
+    const int c = (b % tpr) * TILE_X + (int)threadIdx.x % TILE_X;
+    const int w = (int)blockIdx.y;
+    if (r >= rows || c >= cols) return;
+    const long long P = (long long)rows * cols;
+    const long long p = (long long)r * cols + c;
+    const Heights me = heights_at(dsm, trees, raise, p);
+    const bool bushplant = (me.bot == 0.0) && me.top > 1.0;
+    uint32_t wsh = 0, wveg = 0, wvb = 0;
+    const int k0 = w * 32;
+    const int k1 = min(patches, k0 + 32);
+    for (int k = k0; k < k1; ++k) {
+        const int32_t *off = offsets + (long long)k * capacity * 2;
+        const double *dz = heights + (long long)k * capacity;
+        const int L = lengths[k];
+        bool sh = false, veg = bushplant, ever = false;
+        for (int n = 0; n < L; ++n) {
+            const int rr = r + off[2 * n], cc = c + off[2 * n + 1];
+            if ((unsigned)rr < (unsigned)rows && (unsigned)cc < (unsigned)cols) {
+                const Heights q = heights_at(dsm, trees, raise, (long long)rr * cols + cc);
+                const double z = dz[n], zp = dz[n > 0 ? n - 1 : 0];
+                if (q.a - z > me.a) { sh = true; veg = false; break; }
+                const int cnt = (int)(q.top - z > me.a) + (int)(q.bot - z > me.a) +
+                                (int)(q.top - zp > me.a) + (int)(q.bot - zp > me.a);
+                veg = veg || (cnt > 0 && cnt < 4);
+                ever = ever || veg;
+            } else {
+                sh = me.a < 0.0;
+                if (sh) veg = false;
+                ever = ever || veg;
+                break;
+            }
+        }
+        const uint32_t bit = 1u << (k - k0);
+        if (!sh) wsh |= bit;                    // shmat = 1 - sh
+        if (!veg) wveg |= bit;                  // vegmat = 1 - veg
+        if (!(ever && !veg)) wvb |= bit;        // vbmat = 1 - (ever - veg)
+    }
+    bits[(0 * SW_VIS_WORDS + w) * P + p] = wsh;
+    bits[(1 * SW_VIS_WORDS + w) * P + p] = wveg;
+    bits[(2 * SW_VIS_WORDS + w) * P + p] = wvb;
+}
+
+// Band-weighted counts -> 15 SVF planes + svftotal (geometry.cu:121-159).
+__global__ __launch_bounds__(256) void k_svf_reduce(
+    const uint32_t *__restrict__ bits, const float *__restrict__ dsm,
+    const float *__restrict__ trees, double raise, long long P, const uint8_t *__restrict__ sectors,
+    const int32_t *__restrict__ band_offsets, const double *__restrict__ weights, int bands,
+    double trans, double *__restrict__ out) {
+#pragma clang fp contract(off)
+    const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= P) return;
+    uint32_t word[3][SW_VIS_WORDS];
+    for (int m = 0; m < 3; ++m)
+        for (int w = 0; w < SW_VIS_WORDS; ++w) word[m][w] = bits[(m * SW_VIS_WORDS + w) * P + p];
+    double v[15];
+    for (int i = 0; i < 15; ++i) v[i] = 0.0;
+    for (int b = 0; b < bands; ++b) {
+        int cnt[15];
+        for (int i = 0; i < 15; ++i) cnt[i] = 0;
+        for (int j = band_offsets[b]; j < band_offsets[b + 1]; ++j) {
+            const uint8_t sec = sectors[j];
+            for (int m = 0; m < 3; ++m) {
+                const int on = (word[m][j >> 5] >> (j & 31)) & 1;
+                cnt[m * 5] += on;
+                for (int d = 0; d < 4; ++d)
+                    if (sec & (1 << d)) cnt[m * 5 + 1 + d] += on;
+            }
+        }
+        const double wf = weights[2 * b], wa = weights[2 * b + 1];
+        for (int g = 0; g < 3; ++g) {
+            v[g * 5] += wf * (double)cnt[g * 5];
+            for (int d = 1; d < 5; ++d) v[g * 5 + d] += wa * (double)cnt[g * 5 + d];
+        }
+    }
+    const double eps = 3.0459e-4;
+    v[2] += eps;
+    v[3] += eps;
+    const Heights me = heights_at(dsm, trees, raise, p);
+    const double last = me.bot == 0.0 ? eps : 0.0;
+    v[7] += last; v[8] += last; v[12] += last; v[13] += last;
+    for (int i = 0; i < 15; ++i) {
+        if (v[i] > 1.0) v[i] = 1.0;
+        out[i * P + p] = v[i];
+    }
+    out[15 * P + p] = v[0] - (1.0 - v[5]) * (1.0 - trans);
+}
+
+// =====================================================================================
+// Per-hour sun-ray shadow march (shadowingfunction_wallheight_23, umep_solweig.py:2201-2406;
+// geometry.cu:185-253) + the per-pixel record the ground-view gather reads.
+// rec = (shadow, Lup field of sunonsurface, albedo, ground | sunwall << 1).
+// =====================================================================================
+template <bool FULL>
+__global__ __launch_bounds__(BLOCK) void k_hour_shadow(const sw_domain *__restrict__ doms,
+                                                       const int32_t *__restrict__ bs, int nd,
+                                                       int total, int hour) {
+#pragma clang fp contract(off)
+    const sw_domain *D;
+    const PixelAt at = locate(doms, bs, nd, total, D);
+    const double *H = D->hours + (long long)hour * SW_HOUR_STRIDE;
+    if (H[SWH_DAY] == 0.0 || !at.valid) return;
+    const int rows = (int)D->rows, cols = (int)D->cols;
+    const long long P = D->pixels, p = at.p;
+    const float *dsm = D->dsm, *trees = D->trees;
+    const double raise = D->dsm_raise;
+    const Heights me = heights_at(dsm, trees, raise, p);
+    const int start = (int)H[SWH_RAY_START], L = (int)H[SWH_RAY_LEN];
+    const int32_t *off = D->ray_off + 2LL * start;
+    const double *dzs = D->ray_dz + start;
+    double f = me.a, topmax = me.top, prev = 0.0;
+    bool veg = (me.bot == 0.0) && me.top > 1.0;
+    bool ever = false;
+    for (int k = 0; k < L; ++k) {
+        const int rr = at.r + off[2 * k], cc = at.c + off[2 * k + 1];
+        const double dz = dzs[k];
+        double A = 0.0, T = 0.0, B = 0.0, Tp = 0.0, Bp = 0.0;
+        if ((unsigned)rr < (unsigned)rows && (unsigned)cc < (unsigned)cols) {
+            const Heights q = heights_at(dsm, trees, raise, (long long)rr * cols + cc);
+            A = q.a - dz;
+            T = q.top - dz;
+            B = q.bot - dz;
+            Tp = q.top - prev;
+            Bp = q.bot - prev;
+        }
+        f = fmax(f, A);
+        topmax = fmax(topmax, T);
+        const int n = (int)(T > me.a) + (int)(B > me.a) + (int)(Tp > me.a) + (int)(Bp > me.a);
+        prev = dz;
+        veg = veg || (n > 0 && n < 4);
+        if (f > me.a) veg = false;
+        ever = ever || veg;
+    }
+    const bool blocked = f > me.a;
+    const double wall = (double)D->walls[p];
+    const double wallbol = wall > 0.0 ? 1.0 : 0.0;
+    const double aspect = ((double)D->aspect[p] * 3.141592653589793) / 180.0;
+    const double low = H[SWH_FACE_LOW], high = H[SWH_FACE_HIGH];
+    double facesh;
+    if (H[SWH_FACE_MIDDLE] != 0.0)
+        facesh = (double)(aspect < low || aspect >= high) - wallbol + 1.0;
+    else
+        facesh = 1.0 - (double)(aspect > low || aspect <= high);
+    const double vegv = veg ? 1.0 : 0.0;
+    const double shvoveg = (topmax - me.a) * vegv;
+    double wallsun = wall - (f - me.a);
+    if (wallsun < 0.0) wallsun = 0.0;
+    if (facesh == 1.0) wallsun = 0.0;
+    const double wallsh = wall - wallsun;
+    double wallshve = shvoveg * wallbol - wallsh;
+    if (wallshve < 0.0) wallshve = 0.0;
+    if (wallshve > wall) wallshve = wall;
+    wallsun = wallsun - wallshve;
+    if (wallsun < 0.0) { wallshve = 0.0; wallsun = 0.0; }
+    const double sh = blocked ? 0.0 : 1.0;
+    const double vegsh = 1.0 - vegv;
+    const double shadow = sh - (1.0 - vegsh) * (1.0 - H[SWH_PSI]);
+    const double ground = D->ground[p] ? 1.0 : 0.0;
+    const bool sunwall = (wallsun / wall * ground) == 1.0;
+    // Surface temperature of the hour (umep_solweig.py:144-170) and the sunonsurface Lup field.
+    const double snup = H[SWH_SNUP];
+    const double amp = (double)D->tgk[p] * H[SWH_ALTMAX] + (double)D->tstart[p];
+    double tg = amp * sin(((H[SWH_FRAC] - snup / 24.0) / ((double)D->tmaxlst[p] / 24.0 - snup / 24.0)) *
+                          3.141592653589793 / 2.0);
+    tg = tg * H[SWH_CITGG];
+    if (D->landcover == 1 && tg < 0.0) tg = 0.0;
+    const double ta = H[SWH_TA], emis = (double)D->emis[p];
+    const double tk = tg * shadow + ta + 273.15, ta4 = (ta + 273.15);
+    const double lup = SBC * emis * (tk * tk) * (tk * tk) - SBC * emis * (ta4 * ta4) * (ta4 * ta4);
+    float4 rec;
+    rec.x = (float)shadow;
+    rec.y = (float)lup;
+    rec.z = D->alb[p];
+    rec.w = (float)((D->ground[p] ? 1 : 0) | (sunwall ? 2 : 0));
+    reinterpret_cast<float4 *>(D->rec)[p] = rec;
+    if (D->masks)
+        D->masks[p] = (uint8_t)((blocked ? 0 : 1) | (veg ? 0 : 2) | ((ever && !veg) ? 0 : 4) |
+                                (sunwall ? 8 : 0) | (facesh == 1.0 ? 16 : 0));
+    if (FULL && D->shadow_planes) {
+        double *o = D->shadow_planes;
+        const double facesun = ((facesh + wallbol == 1.0) && wall > 0.0) ? 1.0 : 0.0;
+        o[p] = vegsh;
+        o[P + p] = sh;
+        o[2 * P + p] = 1.0 - ((ever ? 1.0 : 0.0) - vegv);
+        o[3 * P + p] = wallsh;
+        o[4 * P + p] = wallsun;
+        o[5 * P + p] = wallshve;
+        o[6 * P + p] = facesh;
+        o[7 * P + p] = facesun;
+    }
+}
+
+// =====================================================================================
+// Per-hour pixel pass: ground view factors (sunonsurface_2018a / gvf_2018a), lag recursion,
+// wedge, Kdown/Kup, anisotropic Kside and longwave patch sums, Tmrt, UTCI.
+// =====================================================================================
+struct Triple {
+    float lup, alb, nosh;
+};
+
+__device__ __forceinline__ float influence(float wall_sum, float ground_sum, float wall_div,
+                                           float ground_div, bool has_wall) {
+    return has_wall ? (wall_sum + ground_sum) / wall_div : ground_sum / ground_div;
+}
+
+__device__ __forceinline__ float distance_weight(float near_v, float far_v) {
+    return (near_v * 0.5f + far_v * 0.4f) / 0.9f;
+}
+
+struct GvfOut {
+    Triple all, e, s, w, n;
+};
+
+__device__ __forceinline__ GvfOut ground_view(const sw_domain *D, const PixelAt &at, const float4 me,
+                                              const double *H, const int32_t *__restrict__ gofs,
+                                              const double *__restrict__ gbounds, int first,
+                                              int second) {
+    const int rows = (int)D->rows, cols = (int)D->cols;
+    const float4 *rec = reinterpret_cast<const float4 *>(D->rec);
+    const float building = ((int)me.w & 1) ? 1.0f : 0.0f;
+    const float lwall = (float)H[SWH_LWALL];
+    const float albedo_b = (float)H[SWH_ALBEDO_B];
+    const float wallbol = D->walls[at.p] > 0.0f ? 1.0f : 0.0f;
+    const double aspect = ((double)D->aspect[at.p] * 3.141592653589793) / 180.0;
+    const float firstf = (float)first, secondf = (float)second;
+    const float local_lup = me.y * (1.0f - building);
+    const float local_albsh = me.z * (1.0f - building) * me.x;
+    const float local_nosh = me.z * (1.0f - building);
+    GvfOut o = {};
+    for (int dir = 0; dir < 18; ++dir) {
+        const int32_t *off = gofs + dir * second * 2;
+        float f = building;
+        float t_bu = 0.f, t_sh = 0.f, t_lup = 0.f, t_alb = 0.f, t_sw = 0.f;
+        float wall_seen = 0.f, any_wall = 0.f;
+        float s_sh = 0.f, s_wall = 0.f, s_lup = 0.f, s_lwall = 0.f, s_albsh = 0.f;
+        float s_albwall = 0.f, s_alb = 0.f, s_albwallnosh = 0.f;
+        float f_lwall = 0.f, f_lup = 0.f, f_albwall = 0.f, f_albsh = 0.f, f_albwallnosh = 0.f,
+              f_alb = 0.f;
+        bool f_sunin = false;
+        for (int st = 0; st < second; ++st) {
+            const int rr = at.r + off[2 * st], cc = at.c + off[2 * st + 1];
+            if ((unsigned)rr < (unsigned)rows && (unsigned)cc < (unsigned)cols) {
+                const float4 q = rec[(long long)rr * cols + cc];
+                const int qb = (int)q.w;
+                t_bu = (float)(qb & 1);
+                t_sw = (float)((qb >> 1) & 1);
+                t_sh = q.x;
+                t_lup = q.y;
+                t_alb = q.z;
+            }
+            f = fminf(f, t_bu);
+            s_sh += t_sh * f;
+            s_lup += t_lup * f;
+            s_albsh += (t_alb * t_sh) * f;
+            s_alb += t_alb * f;
+            wall_seen = (t_sw * f + wall_seen) > 0.f ? 1.f : 0.f;
+            any_wall = ((1.f - f) + any_wall) > 0.f ? 1.f : 0.f;
+            s_lwall += wall_seen * lwall;
+            s_albwall += wall_seen * albedo_b;
+            s_wall += wall_seen;
+            s_albwallnosh += any_wall * albedo_b;
+            if (st + 1 <= first) {
+                f_sunin = s_wall > 0.f;
+                f_lwall = s_lwall;
+                f_lup = s_lup;
+                f_albwall = s_albwall;
+                f_albsh = s_albsh;
+                f_albwallnosh = s_albwallnosh;
+                f_alb = s_alb;
+            }
+        }
+        const bool sunin2 = s_wall > 0.f;
+        const bool f_wallin = f_albwallnosh > 0.f;
+        const bool wallin2 = s_albwallnosh > 0.f;
+        const double *bd = gbounds + dir * 3;
+        float facesh;
+        if (bd[2] == 0.0)
+            facesh = (float)(aspect < bd[0] || aspect >= bd[1]) - wallbol + 1.0f;
+        else
+            facesh = 1.0f - (float)(aspect > bd[0] || aspect <= bd[1]);
+        float keep = (s_wall == secondf ? 1.0f : 0.0f) - facesh;
+        if (keep == 1.0f) {
+            s_lwall = 0.f;
+            s_albwall = 0.f;
+        }
+        const float lup_d = distance_weight(influence(f_lwall, f_lup, firstf + 1.f, firstf, f_sunin),
+                                            influence(s_lwall, s_lup, secondf + 1.f, secondf, sunin2)) +
+                            local_lup;
+        const float alb_d = distance_weight(influence(f_albwall, f_albsh, firstf + 1.f, firstf, f_sunin),
+                                            influence(s_albwall, s_albsh, secondf + 1.f, secondf, sunin2)) +
+                            local_albsh;
+        const float nosh_d = distance_weight(influence(f_albwallnosh, f_alb, firstf + 1.f, firstf, f_wallin),
+                                             influence(s_albwallnosh, s_alb, secondf, secondf, wallin2)) *
+                                 building +
+                             local_nosh;
+        o.all.lup += lup_d; o.all.alb += alb_d; o.all.nosh += nosh_d;
+        const int az = 5 + 20 * dir;
+        if (az < 180) { o.e.lup += lup_d; o.e.alb += alb_d; o.e.nosh += nosh_d; }
+        if (az >= 90 && az < 270) { o.s.lup += lup_d; o.s.alb += alb_d; o.s.nosh += nosh_d; }
+        if (az >= 180) { o.w.lup += lup_d; o.w.alb += alb_d; o.w.nosh += nosh_d; }
+        if (az >= 270 || az < 90) { o.n.lup += lup_d; o.n.alb += alb_d; o.n.nosh += nosh_d; }
+    }
+    return o;
+}
+
+// Cylinder wedge: fraction of sunlit walls seen by a standing cylinder (umep_solweig.py:383-426).
+__device__ __forceinline__ float wedge(double ta, double tb) {
+    const double PI = 3.141592653589793;
+    const double xa = 1.0 - 2.0 / (ta * tb), ha = 2.0 / (ta * tb), ba = 1.0 / ta;
+    const double hkil = 2.0 * ba * ha;
+    double ukil = 0.0;
+    if (xa < 0.0) {
+        const double qa = tb / 2.0;
+        const double za = sqrt(ba * ba - (qa * qa) / 4.0);
+        const double phi = atan(za / qa);
+        const double area = (sin(phi) - phi * cos(phi)) / (1.0 - cos(phi));
+        ukil = 2.0 * ba * xa * area;
+    }
+    const double F = (2.0 * PI * ba - (hkil + ukil)) / (2.0 * PI * ba);
+    return isnan(F) ? 0.5f : (float)F;
+}
+
+__device__ __forceinline__ float lag(float current, float &state, double w, bool first,
+                                     bool update) {
+    const double cur = (double)current;
+    const double prev = first ? cur : (double)state;
+    const double delayed = cur * (1.0 - w) + prev * w;
+    state = (float)(update ? delayed : prev);
+    return (float)delayed;
+}
+
+template <bool FULL>
+__global__ __launch_bounds__(BLOCK) void k_hour_pixels(const sw_domain *__restrict__ doms,
+                                                       const int32_t *__restrict__ bs, int nd,
+                                                       int total, int hour,
+                                                       const int32_t *__restrict__ gofs,
+                                                       const double *__restrict__ gbounds,
+                                                       int first, int second) {
+    const sw_domain *D;
+    const PixelAt at = locate(doms, bs, nd, total, D);
+    if (!at.valid) return;
+    const double *H = D->hours + (long long)hour * SW_HOUR_STRIDE;
+    const long long P = D->pixels, p = at.p;
+    const bool day = H[SWH_DAY] != 0.0;
+    const float emis = D->emis[p];
+    const float radI = (float)H[SWH_RADI], radD = (float)H[SWH_RADD], radG = (float)H[SWH_RADG];
+    const float albedo_b = (float)H[SWH_ALBEDO_B];
+    const float flux4 = (float)H[SWH_FLUX4];
+    float Kdown = 0.f, Kup = 0.f, KupE = 0.f, KupS = 0.f, KupW = 0.f, KupN = 0.f;
+    float KsideI = 0.f, Lup, LupE, LupS, LupW, LupN, shadow = 0.f, Fsh = 0.f, tgout = 0.f, tg = 0.f;
+    float4 me = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (day) {
+        me = reinterpret_cast<const float4 *>(D->rec)[p];
+        shadow = me.x;
+        const GvfOut g = ground_view(D, at, me, H, gofs, gbounds, first, second);
+        const float amb = flux4 * emis;
+        const double w = H[SWH_LAG_WEIGHT];
+        const bool lf = H[SWH_LAG_FIRST] != 0.0, lu = H[SWH_LAG_UPDATE] != 0.0;
+        float *lg = D->lag;
+        float s0 = lg[p], s1 = lg[P + p], s2 = lg[2 * P + p], s3 = lg[3 * P + p], s4 = lg[4 * P + p];
+        Lup = lag(g.all.lup / 18.0f + amb, s0, w, lf, lu);
+        LupE = lag(g.e.lup / 9.0f + amb, s1, w, lf, lu);
+        LupS = lag(g.s.lup / 9.0f + amb, s2, w, lf, lu);
+        LupW = lag(g.w.lup / 9.0f + amb, s3, w, lf, lu);
+        LupN = lag(g.n.lup / 9.0f + amb, s4, w, lf, lu);
+        lg[p] = s0; lg[P + p] = s1; lg[2 * P + p] = s2; lg[3 * P + p] = s3; lg[4 * P + p] = s4;
+        if (FULL) {
+            const double snup = H[SWH_SNUP];
+            const double amp = (double)D->tgk[p] * H[SWH_ALTMAX] + (double)D->tstart[p];
+            double t = amp * sin(((H[SWH_FRAC] - snup / 24.0) / ((double)D->tmaxlst[p] / 24.0 - snup / 24.0)) *
+                                 3.141592653589793 / 2.0) * H[SWH_CITGG];
+            if (D->landcover == 1 && t < 0.0) t = 0.0;
+            tg = (float)t;
+            float s5 = lg[5 * P + p];
+            tgout = lag((float)(t * (double)shadow + H[SWH_TA]), s5, w, lf, lu);
+            lg[5 * P + p] = s5;
+        }
+        Fsh = wedge(D->tan_svfalfa[p], H[SWH_TANZEN]);
+        const float svfbuveg = D->svfbuveg[p];
+        const float sinalt = (float)H[SWH_SINALT];
+        const float around = albedo_b * (1.0f - svfbuveg) * (radG * (1.0f - Fsh) + radD * Fsh);
+        const float refl = radD * svfbuveg + around;
+        Kup = (g.all.alb / 18.0f) * radI * sinalt + refl * (g.all.nosh / 18.0f);
+        KupE = (g.e.alb / 9.0f) * radI * sinalt + refl * (g.e.nosh / 9.0f);
+        KupS = (g.s.alb / 9.0f) * radI * sinalt + refl * (g.s.nosh / 9.0f);
+        KupW = (g.w.alb / 9.0f) * radI * sinalt + refl * (g.w.nosh / 9.0f);
+        KupN = (g.n.alb / 9.0f) * radI * sinalt + refl * (g.n.nosh / 9.0f);
+        Kdown = radI * shadow * sinalt + around;  // + dRad below
+        KsideI = shadow * radI * (float)H[SWH_COSALT];
+    } else {
+        Lup = flux4 * emis;
+        LupE = LupS = LupW = LupN = Lup;
+        if (FULL) tgout = (float)H[SWH_TA];
+    }
+    // ---- anisotropic patch sums over the 153 sky patches --------------------------------
+    const int row = (int)H[SWH_PATCH_TABLE];
+    const float *pf = D->ptab_f + (long long)row * SW_PATCHES * SW_PATCH_F;
+    const double *pd = D->ptab_d + (long long)row * SW_PATCHES * SW_PATCH_D;
+    const double tan_asvf = D->tan_asvf[p];
+    const float dveg = (float)H[SWH_DIFF_VEG];
+    float a_lv = 0.f, a_kd = 0.f, a_ldn = 0.f, a_lsd = 0.f, a_vc = 0.f, a_vs = 0.f;
+    float a_swsun = 0.f, a_swsh = 0.f, a_lsunc = 0.f, a_lsuns = 0.f, a_lshc = 0.f, a_lshs = 0.f;
+    float a_rc = 0.f, a_rs = 0.f;
+    float cE = 0.f, cW = 0.f, cN = 0.f, cS = 0.f;       // FULL: cardinal sky+veg+walls
+    float rE = 0.f, rW = 0.f, rN = 0.f, rS = 0.f;       // FULL: cardinal reflected weights
+    const float lw_amb = (float)H[SWH_LW_AMBIENT], lw_sun = (float)H[SWH_LW_SUNLIT];
+    const uint32_t *vis = D->vis;
+    for (int w = 0; w < SW_VIS_WORDS; ++w) {
+        const uint32_t vsh = vis[(0 * SW_VIS_WORDS + w) * P + p];
+        const uint32_t vveg = vis[(1 * SW_VIS_WORDS + w) * P + p];
+        const uint32_t vvb = vis[(2 * SW_VIS_WORDS + w) * P + p];
+        const uint32_t sky = vsh & vveg, vegonly = vsh & ~vveg;
+        const uint32_t vegf = ~vveg | ~vvb, bld = ~vsh & vvb, obst = ~vsh | ~vveg | ~vvb;
+        const int k0 = w * 32, k1 = min(SW_PATCHES, k0 + 32);
+        for (int k = k0; k < k1; ++k) {
+            const int j = k - k0;
+            const float *c = pf + k * SW_PATCH_F;
+            const double *cd = pd + k * SW_PATCH_D;
+            const float wcos = c[SWP_WCOS], wsin = c[SWP_WSIN];
+            const float dsh = ((sky >> j) & 1) ? 1.0f : (((vegonly >> j) & 1) ? dveg : 0.0f);
+            a_lv += dsh * c[SWP_LV];
+            a_kd += dsh * c[SWP_KD];
+            if ((sky >> j) & 1) {
+                a_ldn += c[SWP_LDN];
+                a_lsd += c[SWP_LSD];
+                if (FULL) {
+                    cE += c[SWP_LSD] * c[SWP_DE]; cW += c[SWP_LSD] * c[SWP_DW];
+                    cN += c[SWP_LSD] * c[SWP_DN]; cS += c[SWP_LSD] * c[SWP_DS];
+                }
+            }
+            if ((vegf >> j) & 1) {
+                a_vc += wcos;
+                a_vs += wsin;
+                if (FULL) {
+                    const float v = lw_amb * wcos;
+                    cE += v * c[SWP_DE]; cW += v * c[SWP_DW]; cN += v * c[SWP_DN]; cS += v * c[SWP_DS];
+                }
+            }
+            if ((bld >> j) & 1) {
+                const double x = tan_asvf + cd[SWP_YI];
+                const bool sunlit = x < cd[SWP_SUN_BELOW];
+                const bool shaded = x >= cd[SWP_SHADE_FROM];
+                if (sunlit) a_swsun += wcos;
+                if (shaded) a_swsh += wcos;
+                const bool elig = c[SWP_ELIGIBLE] != 0.0f;
+                const bool lsun = elig && sunlit, lsh = elig ? shaded : true;
+                if (lsun) { a_lsunc += wcos; a_lsuns += wsin; }
+                if (lsh) { a_lshc += wcos; a_lshs += wsin; }
+                if (FULL) {
+                    const float v = (lsun ? lw_sun * wcos : 0.f) + (lsh ? lw_amb * wcos : 0.f);
+                    cE += v * c[SWP_DE]; cW += v * c[SWP_DW]; cN += v * c[SWP_DN]; cS += v * c[SWP_DS];
+                }
+            }
+            if ((obst >> j) & 1) {
+                a_rc += wcos;
+                a_rs += wsin;
+                if (FULL) {
+                    rE += wcos * c[SWP_DE]; rW += wcos * c[SWP_DW];
+                    rN += wcos * c[SWP_DN]; rS += wcos * c[SWP_DS];
+                }
+            }
+        }
+    }
+    const float dRad = a_lv * radD;
+    Kdown += day ? dRad : 0.f;
+    const float sw_sun = (float)H[SWH_SW_SUNLIT], sw_sh = (float)H[SWH_SW_SHADED];
+    const float KsideD = day ? a_kd : 0.f;
+    const float Kside = day ? KsideI + KsideD + sw_sun * a_swsun + sw_sh * a_swsh + sw_sh * a_vc : 0.f;
+    const float reflsurf = (a_ldn + Lup) * (float)H[SWH_LW_REFL];
+    const float Ldown = a_ldn + lw_amb * a_vs + lw_amb * a_lshs + lw_sun * a_lsuns + reflsurf * a_rs;
+    const float Lside = a_lsd + lw_amb * a_vc + lw_amb * a_lshc + lw_sun * a_lsunc + reflsurf * a_rc;
+    const float absK = (float)H[SWH_ABSK], absL = (float)H[SWH_ABSL];
+    const float Fup = (float)H[SWH_FUP], Fside = (float)H[SWH_FSIDE], Fcyl = (float)H[SWH_FCYL];
+    const float Ksum = (KupN + KupE + KupS + KupW) * 0.5f;
+    const float Lsum = (LupN + LupE + LupS + LupW) * 0.5f;
+    const float Sstr = absK * (Kside * Fcyl + (Kdown + Kup) * Fup + Ksum * Fside) +
+                       absL * ((Ldown + Lup) * Fup + Lside * Fcyl + Lsum * Fside);
+    const float Tmrt = sqrtf(sqrtf(Sstr / (absL * (float)SBC))) - 273.2f;
+    // UTCI with the hour's scalar weather (utci.cu:500-523), NaN off the ground mask.
+    float u;
+    if (!D->ground[p]) {
+        u = __builtin_nanf("");
+    } else if (H[SWH_UTCI0 + 6] == -999.0 || Tmrt <= -999.0f) {
+        u = -999.0f;
+    } else {
+        const double dT = (double)(Tmrt - (float)H[SWH_TA32]);
+        double acc = H[SWH_UTCI0 + 6];
+        for (int k = 5; k >= 0; --k) acc = acc * dT + H[SWH_UTCI0 + k];
+        u = (float)acc;
+    }
+    if (D->tmrt) D->tmrt[(long long)hour * P + p] = Tmrt;
+    if (D->utci) D->utci[(long long)hour * P + p] = u;
+    if (FULL && D->full) {
+        float *o = D->full;
+        const float refl = reflsurf;
+        o[SWF_KDOWN * P + p] = Kdown;
+        o[SWF_KUP * P + p] = Kup;
+        o[SWF_LDOWN * P + p] = Ldown;
+        o[SWF_LUP * P + p] = Lup;
+        o[SWF_TG * P + p] = tg;
+        o[SWF_KEAST * P + p] = KupE * 0.5f;
+        o[SWF_KSOUTH * P + p] = KupS * 0.5f;
+        o[SWF_KWEST * P + p] = KupW * 0.5f;
+        o[SWF_KNORTH * P + p] = KupN * 0.5f;
+        o[SWF_LEAST * P + p] = LupE * 0.5f + cE + refl * rE;
+        o[SWF_LSOUTH * P + p] = LupS * 0.5f + cS + refl * rS;
+        o[SWF_LWEST * P + p] = LupW * 0.5f + cW + refl * rW;
+        o[SWF_LNORTH * P + p] = LupN * 0.5f + cN + refl * rN;
+        o[SWF_KSIDEI * P + p] = KsideI;
+        o[SWF_TGOUT * P + p] = tgout;
+        o[SWF_LSIDE * P + p] = Lside;
+        o[SWF_KSIDED * P + p] = KsideD;
+        o[SWF_DRAD * P + p] = day ? dRad : 0.f;
+        o[SWF_KSIDE * P + p] = Kside;
+        o[SWF_SHADOW * P + p] = shadow;
+        o[SWF_WALLSUN * P + p] = 0.f;
+        o[SWF_FSH * P + p] = Fsh;
+    }
+}
+
+struct Coef7 {
+    double c[7];
+};
+
+__global__ void k_utci_scalar(const float *__restrict__ tmrt, long long n, Coef7 cf, float ta,
+                              int sentinel, const uint8_t *__restrict__ ground, float *__restrict__ out) {
+    const double *c = cf.c;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (long long)blockDim.x * gridDim.x) {
+        const float t = tmrt[i];
+        if (ground && !ground[i]) { out[i] = __builtin_nanf(""); continue; }
+        if (sentinel || t <= -999.0f) { out[i] = -999.0f; continue; }
+        const double dT = (double)(t - ta);
+        double acc = c[6];
+        for (int k = 5; k >= 0; --k) acc = acc * dT + c[k];
+        out[i] = (float)acc;
+    }
+}
+
+int launch_hour_shadow(const sw_domain *doms, const int32_t *bs, int nd, int total, int hour,
+                       bool full, hipStream_t s) {
+    if (full)
+        hipLaunchKernelGGL(k_hour_shadow<true>, dim3(total), dim3(BLOCK), 0, s, doms, bs, nd, total, hour);
+    else
+        hipLaunchKernelGGL(k_hour_shadow<false>, dim3(total), dim3(BLOCK), 0, s, doms, bs, nd, total, hour);
+    return hip_check(hipGetLastError(), "k_hour_shadow launch");
+}
+
+int launch_hour_pixels(const sw_domain *doms, const int32_t *bs, int nd, int total, int hour,
+                       const int32_t *gofs, const double *gb, int first, int second, bool full,
+                       hipStream_t s) {
+    if (full)
+        hipLaunchKernelGGL(k_hour_pixels<true>, dim3(total), dim3(BLOCK), 0, s, doms, bs, nd, total,
+                           hour, gofs, gb, first, second);
+    else
+        hipLaunchKernelGGL(k_hour_pixels<false>, dim3(total), dim3(BLOCK), 0, s, doms, bs, nd, total,
+                           hour, gofs, gb, first, second);
+    return hip_check(hipGetLastError(), "k_hour_pixels launch");
+}
+
+int check_launch(const void *doms, const void *bs, int nd, int total) {
+    if (!doms || !bs || nd <= 0 || total <= 0) return fail(SW_ERR_ARGUMENT, "empty domain table");
+    return SW_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int sw_version(void) { return 100; }
+
+const char *sw_last_error(void) { return g_error.c_str(); }
+
+int sw_device_count(int *count) {
+    if (!count) return fail(SW_ERR_ARGUMENT, "count is NULL");
+    return hip_check(hipGetDeviceCount(count), "hipGetDeviceCount");
+}
+
+int sw_set_device(int device) { return hip_check(hipSetDevice(device), "hipSetDevice"); }
+
+int sw_malloc(void **ptr, size_t bytes) {
+    if (!ptr) return fail(SW_ERR_ARGUMENT, "ptr is NULL");
+    *ptr = nullptr;
+    if (bytes == 0) return SW_OK;
+    return hip_check(hipMalloc(ptr, bytes), "hipMalloc");
+}
+
+int sw_free(void *ptr) { return ptr ? hip_check(hipFree(ptr), "hipFree") : SW_OK; }
+
+int sw_memcpy_h2d(void *dst, const void *src, size_t bytes, void *stream) {
+    if (!bytes) return SW_OK;
+    return hip_check(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, (hipStream_t)stream),
+                     "hipMemcpyAsync H2D");
+}
+
+int sw_memcpy_d2h(void *dst, const void *src, size_t bytes, void *stream) {
+    if (!bytes) return SW_OK;
+    int rc = hip_check(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, (hipStream_t)stream),
+                       "hipMemcpyAsync D2H");
+    if (rc) return rc;
+    return hip_check(hipStreamSynchronize((hipStream_t)stream), "hipStreamSynchronize");
+}
+
+int sw_memcpy_d2d(void *dst, const void *src, size_t bytes, void *stream) {
+    if (!bytes) return SW_OK;
+    return hip_check(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, (hipStream_t)stream),
+                     "hipMemcpyAsync D2D");
+}
+
+int sw_memset(void *dst, int value, size_t bytes, void *stream) {
+    if (!bytes) return SW_OK;
+    return hip_check(hipMemsetAsync(dst, value, bytes, (hipStream_t)stream), "hipMemsetAsync");
+}
+
+int sw_stream_create(void **stream) {
+    if (!stream) return fail(SW_ERR_ARGUMENT, "stream is NULL");
+    return hip_check(hipStreamCreateWithFlags((hipStream_t *)stream, hipStreamNonBlocking),
+                     "hipStreamCreate");
+}
+
+int sw_stream_destroy(void *stream) {
+    return stream ? hip_check(hipStreamDestroy((hipStream_t)stream), "hipStreamDestroy") : SW_OK;
+}
+
+int sw_stream_sync(void *stream) {
+    return hip_check(hipStreamSynchronize((hipStream_t)stream), "hipStreamSynchronize");
+}
+
+int sw_event_create(void **event) {
+    if (!event) return fail(SW_ERR_ARGUMENT, "event is NULL");
+    return hip_check(hipEventCreate((hipEvent_t *)event), "hipEventCreate");
+}
+
+int sw_event_destroy(void *event) {
+    return event ? hip_check(hipEventDestroy((hipEvent_t)event), "hipEventDestroy") : SW_OK;
+}
+
+int sw_event_record(void *event, void *stream) {
+    return hip_check(hipEventRecord((hipEvent_t)event, (hipStream_t)stream), "hipEventRecord");
+}
+
+int sw_event_elapsed_ms(void *start, void *stop, float *ms) {
+    int rc = hip_check(hipEventSynchronize((hipEvent_t)stop), "hipEventSynchronize");
+    if (rc) return rc;
+    return hip_check(hipEventElapsedTime(ms, (hipEvent_t)start, (hipEvent_t)stop), "hipEventElapsedTime");
+}
+
+int sw_device_sync(void) { return hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize"); }
+
+int sw_svf_visibility(const float *dsm, const float *trees, double raise, int rows, int cols,
+                      const int32_t *offsets, const double *heights, const int32_t *lengths,
+                      int capacity, int patches, uint32_t *bits, void *stream) {
+    if (!dsm || !trees || !offsets || !heights || !lengths || !bits)
+        return fail(SW_ERR_ARGUMENT, "sw_svf_visibility: NULL pointer");
+    if (rows <= 0 || cols <= 0 || capacity <= 0 || patches <= 0 || patches > 32 * SW_VIS_WORDS)
+        return fail(SW_ERR_ARGUMENT, "sw_svf_visibility: bad sizes");
+    const int tiles = ((cols + TILE_X - 1) / TILE_X) * ((rows + TILE_Y - 1) / TILE_Y);
+    const int words = (patches + 31) / 32;
+    hipLaunchKernelGGL(k_svf_visibility, dim3(tiles, words), dim3(BLOCK), 0, (hipStream_t)stream,
+                       dsm, trees, raise, rows, cols, offsets, heights, lengths, capacity, patches,
+                       bits, tiles);
+    return hip_check(hipGetLastError(), "k_svf_visibility launch");
+}
+
+int sw_svf_reduce(const uint32_t *bits, const float *dsm, const float *trees, double raise, int rows,
+                  int cols, const uint8_t *sectors, const int32_t *band_offsets,
+                  const double *weights, int bands, int patches, double trans, double *out,
+                  void *stream) {
+    if (!bits || !dsm || !trees || !sectors || !band_offsets || !weights || !out)
+        return fail(SW_ERR_ARGUMENT, "sw_svf_reduce: NULL pointer");
+    if (rows <= 0 || cols <= 0 || bands <= 0 || patches != SW_PATCHES)
+        return fail(SW_ERR_ARGUMENT, "sw_svf_reduce: bad sizes");
+    const long long P = (long long)rows * cols;
+    hipLaunchKernelGGL(k_svf_reduce, dim3((unsigned)((P + 255) / 256)), dim3(256), 0,
+                       (hipStream_t)stream, bits, dsm, trees, raise, P, sectors, band_offsets,
+                       weights, bands, trans, out);
+    return hip_check(hipGetLastError(), "k_svf_reduce launch");
+}
+
+int sw_hour_shadow(const sw_domain *doms, const int32_t *bs, int nd, int total, int hour,
+                   int full_planes, void *stream) {
+    int rc = check_launch(doms, bs, nd, total);
+    if (rc) return rc;
+    if (hour < 0) return fail(SW_ERR_ARGUMENT, "sw_hour_shadow: negative hour");
+    return launch_hour_shadow(doms, bs, nd, total, hour, full_planes != 0, (hipStream_t)stream);
+}
+
+int sw_hour_pixels(const sw_domain *doms, const int32_t *bs, int nd, int total, int hour,
+                   const int32_t *gofs, const double *gb, int first, int second, int full,
+                   void *stream) {
+    int rc = check_launch(doms, bs, nd, total);
+    if (rc) return rc;
+    if (!gofs || !gb) return fail(SW_ERR_ARGUMENT, "sw_hour_pixels: NULL GVF geometry");
+    if (first < 1 || second < 1 || second > MAX_SECOND || first > second)
+        return fail(SW_ERR_ARGUMENT, "sw_hour_pixels: bad GVF step counts");
+    if (hour < 0) return fail(SW_ERR_ARGUMENT, "sw_hour_pixels: negative hour");
+    return launch_hour_pixels(doms, bs, nd, total, hour, gofs, gb, first, second, full != 0,
+                              (hipStream_t)stream);
+}
+
+int sw_hours_forward(const sw_domain *doms, const int32_t *bs, int nd, int total, int h0, int h1,
+                     const int32_t *gofs, const double *gb, int first, int second, void *stream) {
+    int rc = check_launch(doms, bs, nd, total);
+    if (rc) return rc;
+    if (!gofs || !gb || first < 1 || second < 1 || second > MAX_SECOND || first > second || h0 < 0 ||
+        h1 < h0)
+        return fail(SW_ERR_ARGUMENT, "sw_hours_forward: bad arguments");
+    for (int h = h0; h < h1; ++h) {
+        rc = launch_hour_shadow(doms, bs, nd, total, h, false, (hipStream_t)stream);
+        if (rc) return rc;
+        rc = launch_hour_pixels(doms, bs, nd, total, h, gofs, gb, first, second, false,
+                                (hipStream_t)stream);
+        if (rc) return rc;
+    }
+    return SW_OK;
+}
+
+int sw_utci_scalar(const float *tmrt, long long n, const double *coefficients, double ta32,
+                   int sentinel, const uint8_t *ground, float *out, void *stream) {
+    if (!tmrt || !out || !coefficients) return fail(SW_ERR_ARGUMENT, "sw_utci_scalar: NULL pointer");
+    if (n <= 0) return SW_OK;
+    Coef7 cf;
+    for (int k = 0; k < 7; ++k) cf.c[k] = coefficients[k];
+    const long long blocks = (n + 255) / 256;
+    hipLaunchKernelGGL(k_utci_scalar, dim3((unsigned)(blocks < 65536 ? blocks : 65536)), dim3(256), 0,
+                       (hipStream_t)stream, tmrt, n, cf, (float)ta32, sentinel, ground, out);
+    return hip_check(hipGetLastError(), "k_utci_scalar launch");
+}
+
+int sw_graph_begin(void *stream) {
+    return hip_check(hipStreamBeginCapture((hipStream_t)stream, hipStreamCaptureModeThreadLocal),
+                     "hipStreamBeginCapture");
+}
+
+int sw_graph_end(void *stream, void **graph_exec) {
+    if (!graph_exec) return fail(SW_ERR_ARGUMENT, "graph_exec is NULL");
+    hipGraph_t g = nullptr;
+    int rc = hip_check(hipStreamEndCapture((hipStream_t)stream, &g), "hipStreamEndCapture");
+    if (rc) return rc;
+    hipGraphExec_t e = nullptr;
+    rc = hip_check(hipGraphInstantiate(&e, g, nullptr, nullptr, 0), "hipGraphInstantiate");
+    hipGraphDestroy(g);
+    if (rc) return rc;
+    *graph_exec = (void *)e;
+    return SW_OK;
+}
+
+int sw_graph_launch(void *graph_exec, void *stream) {
+    return hip_check(hipGraphLaunch((hipGraphExec_t)graph_exec, (hipStream_t)stream), "hipGraphLaunch");
+}
+
+int sw_graph_destroy(void *graph_exec) {
+    return graph_exec ? hip_check(hipGraphExecDestroy((hipGraphExec_t)graph_exec), "hipGraphExecDestroy")
+                      : SW_OK;
+}
+
+}  // extern "C"

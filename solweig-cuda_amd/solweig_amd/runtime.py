@@ -1,0 +1,239 @@
+"""ctypes binding of libsolweig_hip.so (the C ABI declared in include/solweig_hip.h).
+
+This is the only place that talks to the device: device buffers, streams, events, graphs and the
+kernel entry points. There is no CPU fallback - if the library cannot be loaded every call raises.
+Replaces the reference's CuPy/NVRTC runtime (runtime.py:11-76).
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBRARY = os.path.join(HERE, 'libsolweig_hip.so')
+
+SW_HOUR_STRIDE = 64
+SW_PATCH_F = 12
+SW_PATCH_D = 4
+SW_PATCHES = 153
+SW_VIS_WORDS = 5
+TILE_X, TILE_Y = 32, 8
+
+# hour record fields (enum order of solweig_hip.h)
+_HOUR_FIELDS = ('DAY ALTITUDE AZIMUTH TANZEN TA RADG RADD RADI PSI SINALT COSALT ALTMAX FRAC SNUP '
+                'CITGG LAG_WEIGHT LAG_FIRST LAG_UPDATE FACE_LOW FACE_HIGH FACE_MIDDLE RAY_START RAY_LEN '
+                'LWALL FLUX4 LW_AMBIENT LW_SUNLIT LW_REFL SW_SUNLIT SW_SHADED TA32 UTCI0').split()
+H = {name: i for i, name in enumerate(_HOUR_FIELDS)}
+H['PATCH_TABLE'] = H['UTCI0'] + 7
+for _k, _name in enumerate('TGOUT_WEIGHT TGOUT_UPDATE ALBEDO_B ABSK ABSL FUP FSIDE FCYL DIFF_VEG TGWALL'.split()):
+    H[_name] = H['PATCH_TABLE'] + 1 + _k
+PF = {n: i for i, n in enumerate('LV KD LDN LSD WCOS WSIN DE DW DN DS ELIGIBLE'.split())}
+PD = {n: i for i, n in enumerate('YI SUN_BELOW SHADE_FROM'.split())}
+FULL_PLANES = ('Kdown Kup Ldown Lup Tg Keast Ksouth Kwest Knorth Least Lsouth Lwest Lnorth KsideI '
+               'TgOut Lside KsideD dRad Kside shadow wallsun F_sh').split()
+
+
+class SwDomain(ctypes.Structure):
+    """Mirror of sw_domain: every field 8 bytes."""
+    _fields_ = [(n, ctypes.c_int64) for n in ('rows', 'cols', 'pixels', 'landcover')] + \
+        [('dsm_raise', ctypes.c_double), ('reserved0', ctypes.c_double)] + \
+        [(n, ctypes.c_uint64) for n in ('dsm', 'trees', 'walls', 'aspect', 'emis', 'alb', 'tgk', 'tstart',
+                                        'tmaxlst', 'svfbuveg', 'tan_asvf', 'tan_svfalfa', 'ground', 'vis',
+                                        'rec', 'lag', 'tmrt', 'utci', 'hours', 'ptab_f', 'ptab_d',
+                                        'ray_off', 'ray_dz', 'masks', 'shadow_planes', 'full')]
+
+
+class SolweigHipError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def lib():
+    """The loaded library; raises if it is missing (build it with __graft_entry__.build())."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIBRARY):
+        raise SolweigHipError(f'{LIBRARY} is not built; run __graft_entry__.build()')
+    L = ctypes.CDLL(LIBRARY)
+    vp, i32, i64, f64, sz = ctypes.c_void_p, ctypes.c_int, ctypes.c_longlong, ctypes.c_double, ctypes.c_size_t
+    sig = {
+        'sw_version': ([], i32), 'sw_last_error': ([], ctypes.c_char_p),
+        'sw_device_count': ([ctypes.POINTER(i32)], i32), 'sw_set_device': ([i32], i32),
+        'sw_malloc': ([ctypes.POINTER(vp), sz], i32), 'sw_free': ([vp], i32),
+        'sw_memcpy_h2d': ([vp, vp, sz, vp], i32), 'sw_memcpy_d2h': ([vp, vp, sz, vp], i32),
+        'sw_memcpy_d2d': ([vp, vp, sz, vp], i32), 'sw_memset': ([vp, i32, sz, vp], i32),
+        'sw_stream_create': ([ctypes.POINTER(vp)], i32), 'sw_stream_destroy': ([vp], i32),
+        'sw_stream_sync': ([vp], i32), 'sw_event_create': ([ctypes.POINTER(vp)], i32),
+        'sw_event_destroy': ([vp], i32), 'sw_event_record': ([vp, vp], i32),
+        'sw_event_elapsed_ms': ([vp, vp, ctypes.POINTER(ctypes.c_float)], i32),
+        'sw_device_sync': ([], i32),
+        'sw_svf_visibility': ([vp, vp, f64, i32, i32, vp, vp, vp, i32, i32, vp, vp], i32),
+        'sw_svf_reduce': ([vp, vp, vp, f64, i32, i32, vp, vp, vp, i32, i32, f64, vp, vp], i32),
+        'sw_hour_shadow': ([vp, vp, i32, i32, i32, i32, vp], i32),
+        'sw_hour_pixels': ([vp, vp, i32, i32, i32, vp, vp, i32, i32, i32, vp], i32),
+        'sw_hours_forward': ([vp, vp, i32, i32, i32, i32, vp, vp, i32, i32, vp], i32),
+        'sw_utci_scalar': ([vp, i64, vp, f64, i32, vp, vp, vp], i32),
+        'sw_graph_begin': ([vp], i32), 'sw_graph_end': ([vp, ctypes.POINTER(vp)], i32),
+        'sw_graph_launch': ([vp, vp], i32), 'sw_graph_destroy': ([vp], i32),
+    }
+    for name, (args, res) in sig.items():
+        fn = getattr(L, name)
+        fn.argtypes = args
+        fn.restype = res
+    _lib = L
+    return L
+
+
+EXPORTED = ('sw_version', 'sw_last_error', 'sw_device_count', 'sw_set_device', 'sw_malloc', 'sw_free',
+            'sw_memcpy_h2d', 'sw_memcpy_d2h', 'sw_memcpy_d2d', 'sw_memset', 'sw_stream_create',
+            'sw_stream_destroy', 'sw_stream_sync', 'sw_event_create', 'sw_event_destroy',
+            'sw_event_record', 'sw_event_elapsed_ms', 'sw_device_sync', 'sw_svf_visibility',
+            'sw_svf_reduce', 'sw_hour_shadow', 'sw_hour_pixels', 'sw_hours_forward', 'sw_utci_scalar',
+            'sw_graph_begin', 'sw_graph_end', 'sw_graph_launch', 'sw_graph_destroy')
+
+
+def check(rc):
+    if rc != 0:
+        msg = lib().sw_last_error()
+        raise SolweigHipError(f'libsolweig_hip error {rc}: {msg.decode() if msg else ""}')
+
+
+def call(name, *args):
+    check(getattr(lib(), name)(*args))
+
+
+class DeviceArray:
+    """A device allocation with a NumPy dtype and shape (row-major)."""
+
+    def __init__(self, shape, dtype):
+        self.shape = tuple(int(s) for s in (shape if np.ndim(shape) else (shape,)))
+        self.dtype = np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape, dtype=np.int64)) * self.dtype.itemsize
+        self._ptr = ctypes.c_void_p()
+        call('sw_malloc', ctypes.byref(self._ptr), self.nbytes)
+
+    @property
+    def ptr(self):
+        return self._ptr.value or 0
+
+    @classmethod
+    def from_host(cls, array, dtype=None, stream=None):
+        a = np.ascontiguousarray(array, dtype=dtype)
+        d = cls(a.shape, a.dtype)
+        d.upload(a, stream)
+        return d
+
+    def upload(self, array, stream=None):
+        a = np.ascontiguousarray(array, dtype=self.dtype)
+        if a.nbytes > self.nbytes:
+            raise ValueError('upload larger than the allocation')
+        call('sw_memcpy_h2d', self.ptr, a.ctypes.data, a.nbytes, _s(stream))
+        call('sw_stream_sync', _s(stream))  # the host array may be pageable and short-lived
+
+    def to_host(self, stream=None):
+        out = np.empty(self.shape, dtype=self.dtype)
+        call('sw_memcpy_d2h', out.ctypes.data, self.ptr, self.nbytes, _s(stream))
+        return out
+
+    def zero(self, stream=None):
+        call('sw_memset', self.ptr, 0, self.nbytes, _s(stream))
+
+    def free(self):
+        if self._ptr.value:
+            lib().sw_free(self._ptr)
+            self._ptr = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def _s(stream):
+    if stream is None:
+        return None
+    return stream.handle if isinstance(stream, Stream) else stream
+
+
+class Stream:
+    def __init__(self):
+        self._h = ctypes.c_void_p()
+        call('sw_stream_create', ctypes.byref(self._h))
+
+    @property
+    def handle(self):
+        return self._h.value
+
+    def sync(self):
+        call('sw_stream_sync', self.handle)
+
+    def __del__(self):
+        try:
+            if self._h.value:
+                lib().sw_stream_destroy(self._h)
+        except Exception:
+            pass
+
+
+class Event:
+    def __init__(self):
+        self._h = ctypes.c_void_p()
+        call('sw_event_create', ctypes.byref(self._h))
+
+    def record(self, stream=None):
+        call('sw_event_record', self._h, _s(stream))
+
+    def elapsed_ms(self, later):
+        ms = ctypes.c_float()
+        call('sw_event_elapsed_ms', self._h, later._h, ctypes.byref(ms))
+        return float(ms.value)
+
+    def __del__(self):
+        try:
+            if self._h.value:
+                lib().sw_event_destroy(self._h)
+        except Exception:
+            pass
+
+
+class Graph:
+    """A captured launch sequence (hipGraphExec)."""
+
+    def __init__(self, handle):
+        self._h = handle
+
+    @classmethod
+    def capture(cls, stream, body):
+        call('sw_graph_begin', _s(stream))
+        try:
+            body()
+        finally:
+            h = ctypes.c_void_p()
+            call('sw_graph_end', _s(stream), ctypes.byref(h))
+        return cls(h)
+
+    def launch(self, stream):
+        call('sw_graph_launch', self._h, _s(stream))
+
+    def __del__(self):
+        try:
+            if self._h.value:
+                lib().sw_graph_destroy(self._h)
+        except Exception:
+            pass
+
+
+def device_count():
+    n = ctypes.c_int()
+    check(lib().sw_device_count(ctypes.byref(n)))
+    return n.value
+
+
+def synchronize():
+    call('sw_device_sync')

@@ -1,0 +1,124 @@
+"""Static preparation of a domain: host derivations, sky view on the device, device planes.
+
+Follows series._prepare_spatial (series.py:252-349): DSM/DEM raised to >= 0, trees clipped at 0,
+ground mask from DSM - DEM < 2 m, land-cover maps (Tgmaps_v1) or the default constants, amaxvalue,
+the SVF pass and the derived planes svfbuveg, asvf, svfalfa. The device keeps only what the hour
+kernels read: FP32 rasters, the visibility bits, FP64 tan(asvf)/tan(svfalfa) and FP32 maps.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from . import runtime as rt
+from .sky import CONSTANTS
+from .svf import sky_view
+
+LANDCOVER = {  # code: (albedo, emissivity, TgK, Tstart, TmaxLST) of landcoverclasses_2016a.txt
+    2: (0.18, 0.95, 0.58, -9.78, 15.0), 1: (0.18, 0.95, 0.58, -9.78, 15.0),
+    0: (0.20, 0.95, 0.37, -3.41, 15.0), 7: (0.05, 0.98, 0.00, 0.00, 12.0),
+    5: (0.16, 0.94, 0.21, -3.38, 14.0), 6: (0.25, 0.94, 0.33, -3.01, 14.0),
+    99: (0.20, 0.90, 0.58, -3.41, 15.0)}
+
+
+def normalise_landcover(values):
+    """series.py:505-513: non-integer or out-of-range codes -> 6, codes 3/4 -> 5."""
+    src = np.asarray(values, dtype=float)
+    r = np.rint(src)
+    bad = (~np.isfinite(src)) | (~np.isclose(src, r)) | (r < 1) | (r > 7)
+    r[bad] = 6
+    r[(r == 3) | (r == 4)] = 5
+    return r.astype(np.float32)
+
+
+class PreparedDomain:
+    """Device-resident static state of one raster domain."""
+
+    def __init__(self, domain, *, stream=None, keep_host=False):
+        self.domain = domain
+        dsm = np.asarray(domain.dsm, dtype=np.float32)
+        dem = np.asarray(domain.dem, dtype=np.float64)
+        rows, cols = dsm.shape
+        self.shape = (rows, cols)
+        self.P = rows * cols
+        a = dsm.astype(np.float64)
+        raise_ = max(0.0, -float(a.min()))
+        dem_raise = max(0.0, -float(dem.min()))
+        if raise_:
+            a = a + raise_
+        if dem_raise:
+            dem = dem + dem_raise
+        self.dsm_raise = raise_
+        trees = np.asarray(domain.trees, dtype=np.float32).copy()
+        trees[trees < 0] = 0
+        self.amaxvalue = float(max(a.max() - a.min(), float(trees.max())))
+        ground = (a - dem) < 2.0
+        z = np.zeros((rows, cols), dtype=np.float32)
+        if domain.landcover is not None:
+            lc = normalise_landcover(domain.landcover)
+            maps = {k: np.copy(lc) for k in ('alb', 'emis', 'tgk', 'tstart', 'tmax')}
+            for code in np.unique(lc):
+                v = LANDCOVER[int(code)]
+                sel = lc == code
+                for key, val in zip(('alb', 'emis', 'tgk', 'tstart', 'tmax'), v):
+                    maps[key][sel] = val
+            w = LANDCOVER[99]
+            self.landcover = 1
+            self.wall_params = (float(np.float32(w[2])), float(np.float32(w[3])), float(np.float32(w[4])))
+        else:
+            maps = dict(alb=z + np.float32(CONSTANTS['albedo_g']), emis=z + np.float32(CONSTANTS['eground']),
+                        tgk=z + np.float32(0.37), tstart=z + np.float32(-3.41), tmax=z + np.float32(15.))
+            self.landcover = 0
+            self.wall_params = (0.37, -3.41, 15.0)
+        if domain.alb_grid is not None:
+            maps['alb'] = np.asarray(domain.alb_grid, dtype=np.float32)
+        if domain.emis_grid is not None:
+            maps['emis'] = np.asarray(domain.emis_grid, dtype=np.float32)
+        up = lambda x, t=np.float32: rt.DeviceArray.from_host(np.ascontiguousarray(x, dtype=t), stream=stream)
+        self.dev = {}
+        self.dev['dsm'] = up(dsm)
+        self.dev['trees'] = up(trees)
+        self.dev['walls'] = up(domain.walls)
+        self.dev['aspect'] = up(domain.aspect)
+        self.dev['ground'] = up(ground, np.uint8)
+        for key in ('alb', 'emis', 'tgk', 'tstart', 'tmax'):
+            self.dev[key] = up(maps[key])
+        sv = sky_view(self.dev['dsm'], self.dev['trees'], raise_, self.shape, self.amaxvalue,
+                      domain.scale, CONSTANTS['trans'], stream)
+        self.skyview = sv
+        self.dev['vis'] = sv.bits
+        # The derived planes are formed on the host with NumPy so that tan(asvf) - which decides
+        # the sunlit/shaded wall split - is the same number the NumPy model uses.
+        planes = sv.planes.to_host().reshape(16, rows, cols)
+        svf = planes[0].astype(np.float32).astype(np.float64)
+        svfveg = planes[5].astype(np.float32).astype(np.float64)
+        svfbuveg = svf - (1.0 - svfveg) * (1.0 - CONSTANTS['trans'])
+        asvf = np.arccos(np.sqrt(svf))
+        tmp = svf + svfveg - 1.0
+        tmp[tmp < 0.0] = 0.0
+        with np.errstate(divide='ignore'):
+            svfalfa = np.arcsin(np.exp(np.log(1.0 - tmp) / 2.0))
+        self.dev['svfbuveg'] = up(svfbuveg)
+        self.dev['tan_asvf'] = up(np.tan(asvf), np.float64)
+        self.dev['tan_svfalfa'] = up(np.tan(svfalfa), np.float64)
+        self.dev['rec'] = rt.DeviceArray((self.P, 4), np.float32)
+        self.dev['lag'] = rt.DeviceArray((6, self.P), np.float32)
+        if keep_host:
+            self.host = dict(a=a, trees=trees, ground=ground, svf_planes=planes, svfbuveg=svfbuveg,
+                             asvf=asvf, svfalfa=svfalfa, maps=maps)
+        self.tiles = ((cols + rt.TILE_X - 1) // rt.TILE_X) * ((rows + rt.TILE_Y - 1) // rt.TILE_Y)
+
+    def struct(self, **extra):
+        """The sw_domain record for this domain (extra: hours, ptab_f, ... device pointers)."""
+        d = self.dev
+        s = rt.SwDomain()
+        s.rows, s.cols, s.pixels, s.landcover = self.shape[0], self.shape[1], self.P, self.landcover
+        s.dsm_raise = self.dsm_raise
+        for field, key in (('dsm', 'dsm'), ('trees', 'trees'), ('walls', 'walls'), ('aspect', 'aspect'),
+                           ('emis', 'emis'), ('alb', 'alb'), ('tgk', 'tgk'), ('tstart', 'tstart'),
+                           ('tmaxlst', 'tmax'), ('svfbuveg', 'svfbuveg'), ('tan_asvf', 'tan_asvf'),
+                           ('tan_svfalfa', 'tan_svfalfa'), ('ground', 'ground'), ('vis', 'vis'),
+                           ('rec', 'rec'), ('lag', 'lag')):
+            setattr(s, field, d[key].ptr)
+        for k, v in extra.items():
+            setattr(s, k, v or 0)
+        return s

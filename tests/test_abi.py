@@ -1,0 +1,57 @@
+"""C ABI surface: the library loads, exports every declared symbol, and the Python mirrors of
+the header's enums and struct match it (no GPU needed)."""
+import ctypes
+import os
+import re
+
+import pytest
+
+from solweig_amd import runtime as rt
+
+HEADER = os.path.join(os.path.dirname(__file__), '..', 'include', 'solweig_hip.h')
+
+
+def header_text():
+    return open(HEADER).read()
+
+
+def test_library_exports_every_declared_symbol():
+    if not os.path.exists(rt.LIBRARY):
+        pytest.skip('library not built')
+    L = ctypes.CDLL(rt.LIBRARY)
+    declared = re.findall(r'^\s*(?:int|const char \*)\s*(sw_\w+)\(', header_text(), flags=re.M)
+    assert declared, 'no declarations parsed'
+    for name in declared:
+        assert hasattr(L, name), name
+    assert set(declared) == set(rt.EXPORTED)
+    assert rt.lib().sw_version() >= 100
+
+
+def _enum_values(block_start):
+    text = header_text()
+    i = text.rfind('enum {', 0, text.index(block_start))
+    body = text[i:text.index('};', i)]
+    body = re.sub(r'/\*.*?\*/', '', body, flags=re.S)
+    vals, cur = {}, -1
+    for tok in [t.strip() for t in body[body.index('{') + 1:].split(',') if t.strip()]:
+        if '=' in tok:
+            name, expr = (x.strip() for x in tok.split('='))
+            cur = eval(expr, {}, {k.replace('SWH_', ''): v for k, v in vals.items()} | vals)
+        else:
+            name, cur = tok, cur + 1
+        vals[name] = cur
+    return vals
+
+
+def test_hour_record_layout_matches_header():
+    vals = _enum_values('SWH_DAY = 0')
+    for name, idx in rt.H.items():
+        assert vals['SWH_' + name] == idx, name
+    assert vals['SW_HOUR_STRIDE'] == rt.SW_HOUR_STRIDE
+
+
+def test_domain_struct_is_256_bytes():
+    assert ctypes.sizeof(rt.SwDomain) == 256
+    fields = re.findall(r'\*(\w+)', header_text()[header_text().index('typedef struct sw_domain'):])
+    names = [f[0] for f in rt.SwDomain._fields_][6:]
+    assert fields[:len(names)] == names
