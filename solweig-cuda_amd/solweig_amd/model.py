@@ -1,0 +1,205 @@
+"""Per-step drop-in of the reference model API (model.py:15-256, 961-1016).
+
+``prepare_model`` uploads the static fields once; ``Solweig_2022a_calc`` runs one SOLWEIG 2022a
+step with the reference's signature and returns its 39-tuple in ``OUTPUT_NAMES`` order. A step
+is the plain per-step form: the host computes the hour's scalars and tables (sky.hour_tables),
+uploads them, and two device launches write every per-pixel output plane (the shadow pass in
+full mode and the pixel pass in full mode); the lag maps are read from and written back to the
+caller's state arrays. Per-pixel planes come back as device views (``.get()`` copies to host).
+"""
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import runtime as rt
+from . import sky
+from .static import prepared_from_arrays
+
+SBC = 5.67051e-8
+
+OUTPUT_NAMES = ('Tmrt', 'Kdown', 'Kup', 'Ldown', 'Lup', 'Tg', 'ea', 'esky', 'I0', 'CI',
+                'shadow', 'firstdaytime', 'timestepdec', 'timeadd', 'Tgmap1', 'Tgmap1E', 'Tgmap1S',
+                'Tgmap1W', 'Tgmap1N', 'Keast', 'Ksouth', 'Kwest', 'Knorth', 'Least', 'Lsouth', 'Lwest',
+                'Lnorth', 'KsideI', 'TgOut1', 'TgOut', 'radI', 'radD', 'Lside', 'L_patches', 'CI_Tg',
+                'CI_TgG', 'KsideD', 'dRad', 'Kside')
+STATE_NAMES = ('CI', 'firstdaytime', 'timeadd', 'Tgmap1', 'Tgmap1E', 'Tgmap1S', 'Tgmap1W', 'Tgmap1N', 'TgOut1')
+FIELD_NAMES = ('svf', 'svfN', 'svfW', 'svfE', 'svfS', 'svfveg', 'svfNveg', 'svfEveg', 'svfSveg', 'svfWveg',
+               'svfaveg', 'svfEaveg', 'svfSaveg', 'svfWaveg', 'svfNaveg', 'lc_grid', 'TgK', 'Tstart',
+               'alb_grid', 'emis_grid', 'TmaxLST', 'svfbuveg', 'asvf', 'svfalfa')
+LAG_NAMES = ('Tgmap1', 'Tgmap1E', 'Tgmap1S', 'Tgmap1W', 'Tgmap1N', 'TgOut1')
+
+
+@dataclass
+class PreparedModel:
+    domain: object
+    hours: object
+    ptab_f: object
+    ptab_d: object
+    ray_off: object
+    ray_dz: object
+    full: object
+    shadow_planes: object
+    tmrt: object
+    dom_dev: object
+    block_start: object
+    gvf_off: object
+    gvf_bounds: object
+    first: int
+    second: int
+    shape: tuple
+    dtype: np.dtype
+    stream: object
+    location: dict
+    scale: float
+
+
+def _host(v):
+    if isinstance(v, (rt.DeviceArray, rt.DeviceView)):
+        return v.to_host()
+    return np.asarray(v.get() if hasattr(v, 'get') else v)
+
+
+def prepare_model(*, dtype=np.float32, **inputs):
+    """Upload the static fields of series._prepare_spatial's dict (model.py:987-1016)."""
+    dtype = np.dtype(dtype)
+    if dtype != np.float32:
+        raise ValueError('the device path computes radiation in float32')
+    if not int(inputs.get('anisotropic_sky', 1)) or int(inputs.get('patch_option', 2)) != 2 \
+            or not int(inputs.get('usevegdem', 1)) or not int(inputs.get('cyl', 1)):
+        raise ValueError('supported configuration: cyl=1, anisotropic_sky=1, usevegdem=1, patch_option=2')
+    rows, cols = int(inputs['rows']), int(inputs['cols'])
+    g = {k: _host(inputs[k]) for k in ('dsm', 'vegdem', 'walls', 'dirwalls', 'buildings', 'svfbuveg', 'asvf',
+                                       'svfalfa', 'shmat', 'vegshmat', 'vbshvegshmat', 'alb_grid', 'emis_grid',
+                                       'TgK', 'Tstart', 'TmaxLST')}
+    stream = rt.Stream()
+    dom = prepared_from_arrays(
+        dsm=g['dsm'], vegdem=g['vegdem'], walls=g['walls'], dirwalls=g['dirwalls'], buildings=g['buildings'],
+        svfbuveg=g['svfbuveg'], asvf=g['asvf'], svfalfa=g['svfalfa'], shmat=g['shmat'], vegshmat=g['vegshmat'],
+        vbshvegshmat=g['vbshvegshmat'], alb_grid=g['alb_grid'], emis_grid=g['emis_grid'], TgK=g['TgK'],
+        Tstart=g['Tstart'], TmaxLST=g['TmaxLST'], landcover=int(inputs.get('landcover', 0)),
+        wall_params=(float(inputs['TgK_wall']), float(inputs['Tstart_wall']), float(inputs['TmaxLST_wall'])),
+        amaxvalue=float(inputs['amaxvalue']), scale=float(inputs['scale']), stream=stream)
+    P = rows * cols
+    cap = max(rows, cols) + 2
+    bufs = dict(hours=rt.DeviceArray((1, rt.SW_HOUR_STRIDE), np.float64),
+                ptab_f=rt.DeviceArray((1, rt.SW_PATCHES, rt.SW_PATCH_F), np.float32),
+                ptab_d=rt.DeviceArray((1, rt.SW_PATCHES, rt.SW_PATCH_D), np.float64),
+                ray_off=rt.DeviceArray((cap, 2), np.int32), ray_dz=rt.DeviceArray((cap,), np.float64),
+                full=rt.DeviceArray((len(rt.FULL_PLANES), P), np.float32),
+                shadow_planes=rt.DeviceArray((8, P), np.float64), tmrt=rt.DeviceArray((1, P), np.float32))
+    s = dom.struct(hours=bufs['hours'].ptr, ptab_f=bufs['ptab_f'].ptr, ptab_d=bufs['ptab_d'].ptr,
+                   ray_off=bufs['ray_off'].ptr, ray_dz=bufs['ray_dz'].ptr, tmrt=bufs['tmrt'].ptr,
+                   full=bufs['full'].ptr, shadow_planes=bufs['shadow_planes'].ptr)
+    dom_dev = rt.DeviceArray((ctypes.sizeof(s),), np.uint8)
+    rt.call('sw_memcpy_h2d', dom_dev.ptr, ctypes.addressof(s), ctypes.sizeof(s), stream.handle)
+    stream.sync()
+    bs = rt.DeviceArray.from_host(np.array([0, dom.tiles], dtype=np.int32))
+    off, bounds, first, second = sky.gvf_geometry(float(inputs['scale']), float(inputs.get('first', 1.0)),
+                                                  float(inputs.get('second', 22.0)))
+    loc = inputs.get('location') or {'latitude': 0.0}
+    return PreparedModel(dom, bufs['hours'], bufs['ptab_f'], bufs['ptab_d'], bufs['ray_off'], bufs['ray_dz'],
+                         bufs['full'], bufs['shadow_planes'], bufs['tmrt'], dom_dev, bs,
+                         rt.DeviceArray.from_host(off, np.int32), rt.DeviceArray.from_host(bounds, np.float64),
+                         first, second, (rows, cols), dtype, stream, loc, float(inputs['scale']))
+
+
+def _set_lag(prep, k, value):
+    P = prep.shape[0] * prep.shape[1]
+    lag = prep.domain.dev['lag']
+    dst = lag.ptr + k * P * 4
+    if isinstance(value, (rt.DeviceArray, rt.DeviceView)) and value.dtype == np.float32:
+        rt.call('sw_memcpy_d2d', dst, value.ptr, P * 4, prep.stream.handle)
+        return
+    a = np.ascontiguousarray(np.broadcast_to(np.asarray(_host(value), dtype=np.float32), prep.shape))
+    rt.call('sw_memcpy_h2d', dst, a.ctypes.data, a.nbytes, prep.stream.handle)
+    prep.stream.sync()
+
+
+def _lag_copy(prep, k):
+    P = prep.shape[0] * prep.shape[1]
+    out = rt.DeviceArray(prep.shape, np.float32)
+    rt.call('sw_memcpy_d2d', out.ptr, prep.domain.dev['lag'].ptr + k * P * 4, P * 4, prep.stream.handle)
+    return out
+
+
+def run_step(prep, *, altitude, azimuth, zen, jday, dectime, altmax, Ta, RH, radG, P, psi, CI,
+             firstdaytime, timeadd, timestepdec, lags, Ws=-999.0, shadow_planes=True):
+    """Host tables + the two full-mode launches of one step. Returns (meta, new state)."""
+    dom = prep.domain
+    h, pf, pd, off, dz, st, meta = sky.hour_tables(
+        float(altitude), float(azimuth), float(zen), float(jday), float(dectime), float(altmax), float(Ta),
+        float(RH), float(radG), float(P), float(Ws), float(jday), float(prep.location['latitude']),
+        prep.shape, prep.scale, dom.amaxvalue, (float(CI), int(firstdaytime), float(timeadd), float(timestepdec)),
+        wall_params=dom.wall_params, psi=float(psi))
+    h[rt.H['RAY_START']] = 0
+    h[rt.H['PATCH_TABLE']] = 0
+    prep.hours.upload(h[None], prep.stream)
+    prep.ptab_f.upload(pf[None], prep.stream)
+    prep.ptab_d.upload(pd[None], prep.stream)
+    if len(dz):
+        off = np.ascontiguousarray(off, np.int32)
+        dz = np.ascontiguousarray(dz, np.float64)
+        rt.call('sw_memcpy_h2d', prep.ray_off.ptr, off.ctypes.data, off.nbytes, prep.stream.handle)
+        rt.call('sw_memcpy_h2d', prep.ray_dz.ptr, dz.ctypes.data, dz.nbytes, prep.stream.handle)
+    for k, v in enumerate(lags):
+        _set_lag(prep, k, v)
+    rt.call('sw_hour_shadow', prep.dom_dev.ptr, prep.block_start.ptr, 1, dom.tiles, 0, int(shadow_planes),
+            prep.stream.handle)
+    rt.call('sw_hour_pixels', prep.dom_dev.ptr, prep.block_start.ptr, 1, dom.tiles, 0, prep.gvf_off.ptr,
+            prep.gvf_bounds.ptr, prep.first, prep.second, 1, prep.stream.handle)
+    return meta, st
+
+
+def Solweig_2022a_calc(i, dsm, scale, rows, cols, svf, svfN, svfW, svfE, svfS, svfveg, svfNveg, svfEveg, svfSveg,
+                       svfWveg, svfaveg, svfEaveg, svfSaveg, svfWaveg, svfNaveg, vegdem, vegdem2, albedo_b, absK,
+                       absL, ewall, Fside, Fup, Fcyl, altitude, azimuth, zen, jday, usevegdem, onlyglobal,
+                       buildings, location, psi, landcover, lc_grid, dectime, altmax, dirwalls, walls, cyl,
+                       elvis, Ta, RH, radG, radD, radI, P, amaxvalue, bush, Twater, TgK, Tstart, alb_grid,
+                       emis_grid, TgK_wall, Tstart_wall, TmaxLST, TmaxLST_wall, first, second, svfalfa,
+                       svfbuveg, firstdaytime, timeadd, timestepdec, Tgmap1, Tgmap1E, Tgmap1S, Tgmap1W,
+                       Tgmap1N, CI, TgOut1, diffsh, shmat, vegshmat, vbshvegshmat, anisotropic_sky, asvf,
+                       patch_option, *, dtype=np.float32, prepared=None):
+    """One SOLWEIG 2022a step (model.py:15-256): returns the 39-tuple of OUTPUT_NAMES."""
+    consts = (float(albedo_b), float(ewall), float(absK), float(absL), float(Fside), float(Fup), float(Fcyl))
+    if consts != (0.2, 0.9, 0.7, 0.95, 0.22, 0.06, 0.28) or float(elvis) != 0 or int(onlyglobal) != 1:
+        raise ValueError('the device path is built for the series.py constants (albedo_b 0.2, ewall 0.9, '
+                         'absK 0.7, absL 0.95, Fside 0.22, Fup 0.06, Fcyl 0.28, elvis 0, onlyglobal 1)')
+    if prepared is None:
+        args = {k: v for k, v in locals().items() if k not in ('prepared', 'consts')}
+        prepared = prepare_model(**args)
+    prep = prepared
+    prep.location = location
+    meta, (CI2, fdt2, timeadd2, _) = run_step(
+        prep, altitude=altitude, azimuth=azimuth, zen=zen, jday=jday, dectime=dectime, altmax=altmax, Ta=Ta,
+        RH=RH, radG=radG, P=P, psi=psi, CI=CI, firstdaytime=firstdaytime, timeadd=timeadd,
+        timestepdec=timestepdec, lags=(Tgmap1, Tgmap1E, Tgmap1S, Tgmap1W, Tgmap1N, TgOut1))
+    F = {name: prep.full.plane(k, prep.shape) for k, name in enumerate(rt.FULL_PLANES)}
+    lags = [_lag_copy(prep, k) for k in range(6)]
+    Tmrt = prep.tmrt.plane(0, prep.shape)
+    day = float(altitude) > 0
+    L_patches = None
+    if day:
+        alt, azi, lv = sky.perez(float(zen) * (180 / np.pi), float(azimuth), meta['radD'], meta['radI'], float(jday))
+        L_patches = np.column_stack((alt, azi, lv))
+    else:
+        a, z = sky.patches()
+        L_patches = np.column_stack((a, z, np.zeros_like(a)))
+    out = dict(Tmrt=Tmrt, ea=meta['ea'], esky=meta['esky'], I0=meta['I0'], CI=CI2, firstdaytime=fdt2,
+               timestepdec=timestepdec, timeadd=timeadd2, TgOut1=lags[5], radI=meta['radI'] if day else radI,
+               radD=meta['radD'] if day else radD, L_patches=L_patches, CI_Tg=meta['CI_Tg'],
+               CI_TgG=meta['CI_TgG'])
+    for k, name in enumerate(LAG_NAMES[:5]):
+        out[name] = lags[k]
+    for name in ('Kdown', 'Kup', 'Ldown', 'Lup', 'Tg', 'shadow', 'Keast', 'Ksouth', 'Kwest', 'Knorth', 'Least',
+                 'Lsouth', 'Lwest', 'Lnorth', 'KsideI', 'TgOut', 'Lside', 'KsideD', 'dRad', 'Kside'):
+        out[name] = F[name]
+    return tuple(out[n] for n in OUTPUT_NAMES)
+
+
+def cast_shadow(prepared, azimuth, altitude):
+    """The 8 FP64 planes of the last step's shadow pass (vegsh, sh, vbshvegsh, wallsh, wallsun,
+    wallshve, facesh, facesun; model.py:689-708) as device views."""
+    return tuple(prepared.shadow_planes.plane(k, prepared.shape) for k in range(8))

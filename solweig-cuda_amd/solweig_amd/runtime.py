@@ -107,6 +107,27 @@ def call(name, *args):
     check(getattr(lib(), name)(*args))
 
 
+class DeviceView:
+    """A typed window [offset, offset + shape) into another device allocation."""
+
+    def __init__(self, owner, offset_bytes, shape, dtype):
+        self.owner, self.offset = owner, int(offset_bytes)
+        self.shape, self.dtype = tuple(shape), np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape, dtype=np.int64)) * self.dtype.itemsize
+
+    @property
+    def ptr(self):
+        return self.owner.ptr + self.offset
+
+    def to_host(self, stream=None):
+        out = np.empty(self.shape, dtype=self.dtype)
+        call('sw_memcpy_d2h', out.ctypes.data, self.ptr, self.nbytes, _s(stream))
+        return out
+
+    def get(self):
+        return self.to_host()
+
+
 class DeviceArray:
     """A device allocation with a NumPy dtype and shape (row-major)."""
 
@@ -142,6 +163,15 @@ class DeviceArray:
 
     def zero(self, stream=None):
         call('sw_memset', self.ptr, 0, self.nbytes, _s(stream))
+
+    def get(self):
+        return self.to_host()
+
+    def plane(self, k, shape=None):
+        """View of plane k of a [planes, ...] array, optionally reshaped."""
+        inner = self.shape[1:]
+        n = int(np.prod(inner, dtype=np.int64)) * self.dtype.itemsize
+        return DeviceView(self, k * n, shape or inner, self.dtype)
 
     def free(self):
         if self._ptr.value:

@@ -1,0 +1,204 @@
+"""Time-series driver (drop-in for series.run_timeseries / load_meteorology, series.py:91-194,
+and run.load_case, run.py:70-86).
+
+Rasters are read from .npy files or passed as arrays (GeoTIFF needs GDAL, which this build does
+not use); walls and aspect come from the DSM (synthetic.wall_heights / wall_aspect). The day loop
+runs on the batched engine: one upload of the day's tables and 24 x 2 launches per day, the lag
+state reset at each day start, UTCI NaN outside the ground mask (series.py:466-469). With
+``plain=True`` every hour goes through the per-step API (model.Solweig_2022a_calc) instead.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass
+from datetime import datetime, timedelta
+import json
+import os
+from pathlib import Path
+
+import numpy as np
+
+from . import batched, synthetic
+from .sun import met_sun
+
+
+@dataclass(frozen=True)
+class DayForcing:
+    date: str
+    values: np.ndarray
+    timestamps: tuple
+
+
+@dataclass(frozen=True)
+class Meteorology:
+    values: np.ndarray
+    timestamps: tuple
+    days: tuple
+    timestep_minutes: float
+
+
+def _read_values(source):
+    if isinstance(source, np.ndarray):
+        values = np.array(source, dtype=np.float64, order='C', copy=True)
+        return values.reshape(1, -1) if values.ndim == 1 else values
+    paths = [Path(source)] if isinstance(source, (str, os.PathLike)) else [Path(p) for p in source]
+    arrays = [np.loadtxt(p.expanduser().resolve(), skiprows=1, ndmin=2, dtype=np.float64, encoding='utf-8-sig')
+              for p in paths]
+    return np.ascontiguousarray(arrays[0] if len(arrays) == 1 else np.concatenate(arrays), dtype=np.float64)
+
+
+def load_meteorology(source) -> Meteorology:
+    """Forcing table grouped by calendar date (series.py:175-194)."""
+    if isinstance(source, Meteorology):
+        return source
+    values = _read_values(source)
+    stamps = [datetime(int(y), 1, 1) + timedelta(days=int(d) - 1, hours=int(h), minutes=int(m))
+              for y, d, h, m in values[:, :4].astype(np.int64)]
+    timestep = (stamps[1] - stamps[0]).total_seconds() / 60 if len(stamps) > 1 else 0.0
+    values.setflags(write=False)
+    groups, start = [], 0
+    for stop in range(1, len(stamps) + 1):
+        if stop == len(stamps) or stamps[stop].date() != stamps[start].date():
+            groups.append(DayForcing(stamps[start].date().isoformat(), values[start:stop], tuple(stamps[start:stop])))
+            start = stop
+    return Meteorology(values, tuple(stamps), tuple(groups), timestep)
+
+
+def _raster(src):
+    if src is None:
+        return None
+    if isinstance(src, np.ndarray):
+        return src
+    p = str(src)
+    if p.endswith('.npy'):
+        return np.load(p, allow_pickle=False)
+    raise ValueError(f'{p}: only .npy rasters are read (GeoTIFF input needs GDAL)')
+
+
+def run_timeseries(building_dsm_path, tree_path, dem_path, met_file, output_path, longitude, latitude, *,
+                   landcover_path=None, utc_offset_hours=None, site_altitude_m=3., wall_method='dsm',
+                   dsm_wall_threshold_m=3., save_tmrt=True, dtype=np.float32, no_write=False,
+                   output_core=None, output_chunk_steps=24, async_output=True, output_queue_size=2,
+                   output_valid_mask=None, progress=False, pixel_size_m=1.0, plain=False, **unused):
+    """Run every day of the forcing; returns {'static': {...}, 'days': {date: {var: [paths]}}}
+    (series.py:91-156). Outputs per day: UTCI (and TMRT) stacks [hours, rows, cols] as .npy."""
+    if wall_method != 'dsm':
+        raise ValueError("only wall_method='dsm' is supported")
+    forcing = load_meteorology(met_file)
+    dsm = np.asarray(_raster(building_dsm_path), dtype=np.float32)
+    trees = np.asarray(_raster(tree_path), dtype=np.float32)
+    dem = np.asarray(_raster(dem_path), dtype=np.float32)
+    walls = synthetic.wall_heights(dsm, dsm_wall_threshold_m).astype(np.float32)
+    aspect = synthetic.wall_aspect(dsm, walls, pixel_size_m).astype(np.float32)
+    utc = float(longitude) / 15.0 if utc_offset_hours is None else float(utc_offset_hours)
+    lc = _raster(landcover_path)
+    dom = synthetic.Domain(dsm, trees, dem, walls, aspect, forcing.values, float(latitude), float(longitude),
+                           utc, float(site_altitude_m), scale=1.0 / float(pixel_size_m), landcover=lc)
+    out = Path(output_path).resolve()
+    files = {'static': {}, 'days': {}}
+    variables = ['UTCI'] + (['TMRT'] if save_tmrt else [])
+    crop = (slice(None), slice(None)) if output_core is None else \
+        (slice(output_core[0], output_core[0] + output_core[2]), slice(output_core[1], output_core[1] + output_core[3]))
+    engine = None
+    for day in forcing.days:
+        n = day.values.shape[0]
+        if progress:
+            print(f'[{datetime.now():%H:%M:%S}] {day.date}: {n} steps', flush=True)
+        sun = met_sun(day.values, dom.location, utc)
+        if plain:
+            tmrt, utci = _plain_day(dom, day.values, sun)
+        else:
+            if engine is None or engine.n_hours != n:
+                engine = batched.Engine([dom], n_hours=n)
+            engine.load(engine.tables(met=[day.values], sun=[sun]))
+            engine.reset_state()
+            engine.forward_day()
+            tmrt, utci = engine.outputs(0)
+        if output_valid_mask is not None:
+            utci = np.where(np.asarray(output_valid_mask)[None], utci, np.nan)
+        day_files = {v: [] for v in variables}
+        if not no_write:
+            folder = out / day.date
+            folder.mkdir(parents=True, exist_ok=True)
+            for name, stack in (('UTCI', utci), ('TMRT', tmrt)):
+                if name in variables:
+                    p = folder / f'{name}.npy'
+                    np.save(p, stack[(slice(None),) + crop])
+                    day_files[name].append(str(p))
+        else:
+            day_files = {'UTCI': utci, 'TMRT': tmrt}
+        files['days'][day.date] = day_files
+    return files
+
+
+def _plain_day(dom, met, sun):
+    """One day through the per-step API (the plain form, series.py:445-469)."""
+    from . import model, utci as u
+    from .static import PreparedDomain
+    pd = PreparedDomain(dom, keep_host=True)
+    h = pd.host
+    rows, cols = pd.shape
+    inputs = dict(rows=rows, cols=cols, scale=dom.scale, dsm=h['a'], vegdem=np.where(h['trees'] > 0, h['trees'] + h['a'], 0.0),
+                  walls=dom.walls, dirwalls=dom.aspect, buildings=h['ground'].astype(np.float64),
+                  svfbuveg=h['svfbuveg'], asvf=h['asvf'], svfalfa=h['svfalfa'],
+                  shmat=None, vegshmat=None, vbshvegshmat=None, alb_grid=h['maps']['alb'],
+                  emis_grid=h['maps']['emis'], TgK=h['maps']['tgk'], Tstart=h['maps']['tstart'],
+                  TmaxLST=h['maps']['tmax'], landcover=pd.landcover, TgK_wall=pd.wall_params[0],
+                  Tstart_wall=pd.wall_params[1], TmaxLST_wall=pd.wall_params[2], amaxvalue=pd.amaxvalue,
+                  location=dom.location)
+    shm, vgm, vbm = pd.skyview.masks()
+    inputs.update(shmat=shm, vegshmat=vgm, vbshvegshmat=vbm)
+    prep = model.prepare_model(**inputs)
+    return plain_day(prep, met, sun, ground=pd.dev['ground'])
+
+
+def plain_day(prep, met, sun, ground=None):
+    """Hour loop of series._run_day over model.Solweig_2022a_calc-style steps; returns host
+    (Tmrt, UTCI) stacks."""
+    from . import model, utci as u
+    n = met.shape[0]
+    dectime = sun['dectime']
+    timestepdec = (dectime[1] - dectime[0]) if n > 1 else 1 / 24.
+    CI, fdt, timeadd = 1., 1, 0.
+    lags = [0.0] * 6
+    tm, ut = [], []
+    for i in range(n):
+        if dectime[i] - np.floor(dectime[i]) == 0:
+            CI = 1.
+        meta, (CI, fdt, timeadd, _) = model.run_step(
+            prep, altitude=sun['altitude'][i], azimuth=sun['azimuth'][i], zen=sun['zen'][i], jday=sun['jday'][i],
+            dectime=dectime[i], altmax=sun['altmax'][i], Ta=met[i, 11], RH=met[i, 10], radG=met[i, 14],
+            P=met[i, 12], psi=0.03 if 97 < met[i, 1] < 300 else 0.5, CI=CI, firstdaytime=fdt, timeadd=timeadd,
+            timestepdec=timestepdec, lags=lags, Ws=met[i, 9])
+        lags = [prep.domain.dev['lag'].plane(k) for k in range(6)]
+        tmrt = prep.tmrt.plane(0, prep.shape)
+        utci = u.utci_calculator(met[i, 11], met[i, 10], tmrt, met[i, 9], ground=ground, stream=prep.stream)
+        prep.stream.sync()
+        tm.append(tmrt.to_host())
+        ut.append(utci.to_host())
+    return np.stack(tm), np.stack(ut)
+
+
+PATH_KEYS = {'building_dsm_path', 'tree_path', 'dem_path', 'met_file', 'building_geometry_path',
+             'building_height_path', 'landcover_path', 'output_path'}
+PREPARATION_KEYS = {'case_name', 'case_type', 'case_size_m', 'core_area_m', 'buffer_area_m', 'resolution_m',
+                    'start_date', 'end_date'}
+
+
+def load_case(path):
+    """case.json -> (run_timeseries kwargs, preparation settings) (run.py:70-86)."""
+    path = Path(path).resolve()
+    case = json.loads(path.read_text(encoding='utf-8-sig'))
+    settings = {k: case.pop(k) for k in PREPARATION_KEYS if k in case}
+    settings.setdefault('case_type', 'local')
+    settings.setdefault('resolution_m', 1)
+    for key in PATH_KEYS & case.keys():
+        value = case[key]
+        if value is None:
+            continue
+        entries = value if isinstance(value, list) else [value]
+        resolved = [str((path.parent / Path(e).expanduser()).resolve()) for e in entries]
+        case[key] = resolved if isinstance(value, list) else resolved[0]
+    case.setdefault('async_output', True)
+    case.setdefault('output_queue_size', 2)
+    case.setdefault('output_chunk_steps', 24)
+    return case, settings

@@ -368,105 +368,122 @@ class DayTables:
         return self.hours.shape[0]
 
 
+def hour_tables(alt, azi, zen, jday, dectime, altmax, Ta, RH, radG, P, Ws, doy, latitude, shape, scale,
+                amaxvalue, state, *, wall_params=(0.37, -3.41, 15.0), constants=CONSTANTS, psi=None):
+    """One step's hour record, patch rows and sun ray (Solweig_2022a_calc's host part,
+    model.py:73-218). ``state`` = (CI, firstdaytime, timeadd, timestepdec) before the step; the
+    returned state is the one after it (the CI reset at midnight rows is the caller's)."""
+    CI, firstdaytime, timeadd, timestepdec = state
+    c = constants
+    tgk_w, tstart_w, tmax_w = wall_params
+    h = np.zeros(SW_HOUR_STRIDE)
+    psi = leaf_psi(doy) if psi is None else psi
+    snup = daylen(jday, latitude)
+    ea = 6.107 * 10 ** ((7.5 * Ta) / (237.3 + Ta)) * (RH / 100.)
+    msteg = 46.5 * (ea / (Ta + 273.15))
+    esky = (1 - (1 + msteg) * np.exp(-((1.2 + 3.0 * msteg) ** 0.5))) + c['elvis']
+    radI = radD = 0.0
+    tgwall = 0.0
+    lv = None
+    I0, citg, citgg = 0.0, CI, CI
+    off = np.zeros((0, 2), np.int32)
+    dz = np.zeros(0)
+    if alt > 0:
+        I0, CI, Kt = clearness(zen, jday, Ta, RH / 100., radG, latitude, P)
+        if (CI > 1) or (CI == np.inf):
+            CI = 1
+        radI, radD = diffuse_fraction(radG, alt, Kt, Ta, RH)
+        _, _, lv = perez(zen * (180 / np.pi), azi, radD, radI, jday)
+        radI0, radD0 = diffuse_fraction(I0, alt, 1., Ta, RH)
+        corr = 0.1473 * np.log(90 - (zen / np.pi * 180)) + 0.3454
+        citg = (radG / radI0) + (1 - corr)
+        if (citg > 1) or (citg == np.inf):
+            citg = 1
+        radG0 = radI0 * np.sin(alt * DEG) + radD0
+        citgg = (radG / radG0) + (1 - corr)
+        if (citgg > 1) or (citgg == np.inf):
+            citgg = 1
+        frac = dectime - np.floor(dectime)
+        tgwall = (tgk_w * altmax + tstart_w) * np.sin(((frac - snup / 24) / (tmax_w / 24 - snup / 24)) * np.pi / 2)
+        if tgwall < 0:
+            tgwall = 0
+        tgwall = tgwall * citgg
+        # the five Lup delays use the incoming timing; TgOut's call advances it (model.py:154-163)
+        first = 1 if firstdaytime == 1 else 0
+        if timeadd >= 59 / 1440:
+            update, weight = 1, math.exp(-33.27 * timeadd)
+            timeadd = timestepdec if timestepdec > 59 / 1440 else 0.0
+        else:
+            update = 0
+            timeadd = timeadd + timestepdec
+            weight = math.exp(-33.27 * timeadd)
+        firstdaytime = 0
+        off, dz, low, high, middle = shadow_ray(azi, alt, scale, amaxvalue, shape)
+        h[H['DAY']] = 1
+        h[H['CITGG']], h[H['FRAC']] = citgg, frac
+        h[H['LAG_WEIGHT']], h[H['LAG_FIRST']], h[H['LAG_UPDATE']] = weight, first, update
+        h[H['FACE_LOW']], h[H['FACE_HIGH']], h[H['FACE_MIDDLE']] = low, high, float(middle)
+        h[H['RAY_LEN']] = len(dz)
+    else:
+        timeadd = 0.
+        firstdaytime = 1
+    if CI < 0.95:
+        esky = CI * esky + (1 - CI) * 1.
+    h[H['ALTITUDE']], h[H['AZIMUTH']], h[H['TANZEN']] = alt, azi, np.tan(zen)
+    h[H['TA']], h[H['RADG']], h[H['RADD']], h[H['RADI']], h[H['PSI']] = Ta, radG, radD, radI, psi
+    h[H['SINALT']], h[H['COSALT']] = np.sin(alt * (np.pi / 180)), np.cos(alt * DEG)
+    h[H['ALTMAX']], h[H['SNUP']] = altmax, snup
+    h[H['LWALL']] = SBC * c['ewall'] * (tgwall + Ta + 273.15) ** 4 - SBC * c['ewall'] * (Ta + 273.15) ** 4
+    h[H['FLUX4']] = SBC * (Ta + 273.15) ** 4
+    h[H['LW_AMBIENT']] = (c['ewall'] * SBC * ((Ta + 273.15) ** 4)) / np.pi
+    h[H['LW_SUNLIT']] = (c['ewall'] * SBC * ((Ta + tgwall + 273.15) ** 4)) / np.pi
+    h[H['LW_REFL']] = (1 - c['ewall']) * 0.5 / np.pi
+    h[H['SW_SUNLIT']] = (c['albedo_b'] * (radI * np.cos(alt * DEG)) + (radD * 0.5)) / np.pi
+    h[H['SW_SHADED']] = (c['albedo_b'] * radD * 0.5) / np.pi
+    ta32, rh32, va32 = np.float32(Ta), np.float32(RH), np.float32(Ws)
+    h[H['TA32']] = float(ta32)
+    if ta32 <= -999 or rh32 <= -999 or va32 <= -999:
+        h[H['UTCI0']:H['UTCI0'] + 7] = -999.0
+    else:
+        h[H['UTCI0']:H['UTCI0'] + 7] = _utci.coefficients(float(ta32), float(rh32), float(va32))
+    h[H['ALBEDO_B']], h[H['ABSK']], h[H['ABSL']] = c['albedo_b'], c['absK'], c['absL']
+    h[H['FUP']], h[H['FSIDE']], h[H['FCYL']] = c['Fup'], c['Fside'], c['Fcyl']
+    h[H['DIFF_VEG']], h[H['TGWALL']] = diff_veg(c['trans']), tgwall
+    pf, pd = patch_rows(alt > 0, esky, Ta, alt, azi, lv, radD)
+    meta = dict(CI=CI, esky=esky, radI=radI, radD=radD, tgwall=tgwall, I0=I0, ea=ea, CI_Tg=citg,
+                CI_TgG=citgg, lv=lv, firstdaytime=firstdaytime, timeadd=timeadd)
+    return h, pf, pd, off, dz, (CI, firstdaytime, timeadd, timestepdec), meta
+
+
 def build_day(met, sun, latitude, shape, scale, amaxvalue, *, landcover=0, wall_params=(0.37, -3.41, 15.0),
               constants=CONSTANTS, state=None):
     """Tables for the forcing rows of one day (series._run_day, series.py:396-469)."""
     n = met.shape[0]
     dectime = sun['dectime']
     timestepdec = (dectime[1] - dectime[0]) if n > 1 else 1 / 24.
-    CI, firstdaytime, timeadd = 1., 1, 0.
-    if state is not None:
-        CI, firstdaytime, timeadd = state
+    st = (1., 1, 0., timestepdec) if state is None else tuple(state) + (timestepdec,)
     hours = np.zeros((n, SW_HOUR_STRIDE))
     pf = np.zeros((n, SW_PATCHES, SW_PATCH_F), dtype=np.float32)
     pd = np.zeros((n, SW_PATCHES, SW_PATCH_D))
-    offs, dzs = [], []
+    offs, dzs, meta = [], [], []
     start = 0
-    c = constants
-    tgk_w, tstart_w, tmax_w = wall_params
-    dveg = diff_veg(c['trans'])
-    meta = []
     for i in range(n):
-        h = hours[i]
         if dectime[i] - np.floor(dectime[i]) == 0:
-            CI = 1.
-        alt, azi, zen = float(sun['altitude'][i]), float(sun['azimuth'][i]), float(sun['zen'][i])
-        jday, altmax = float(sun['jday'][i]), float(sun['altmax'][i])
-        Ta, RH, radG, P, Ws = (float(met[i, k]) for k in (11, 10, 14, 12, 9))
-        psi = leaf_psi(met[i, 1])
-        snup = daylen(jday, latitude)
-        ea = 6.107 * 10 ** ((7.5 * Ta) / (237.3 + Ta)) * (RH / 100.)
-        msteg = 46.5 * (ea / (Ta + 273.15))
-        esky = (1 - (1 + msteg) * np.exp(-((1.2 + 3.0 * msteg) ** 0.5))) + c['elvis']
-        radI = radD = 0.0
-        tgwall = 0.0
-        lv = None
-        if alt > 0:
-            I0, CI, Kt = clearness(zen, jday, Ta, RH / 100., radG, latitude, P)
-            if (CI > 1) or (CI == np.inf):
-                CI = 1
-            radI, radD = diffuse_fraction(radG, alt, Kt, Ta, RH)
-            _, _, lv = perez(zen * (180 / np.pi), azi, radD, radI, jday)
-            radI0, radD0 = diffuse_fraction(I0, alt, 1., Ta, RH)
-            corr = 0.1473 * np.log(90 - (zen / np.pi * 180)) + 0.3454
-            radG0 = radI0 * np.sin(alt * DEG) + radD0
-            citgg = (radG / radG0) + (1 - corr)
-            if (citgg > 1) or (citgg == np.inf):
-                citgg = 1
-            frac = dectime[i] - np.floor(dectime[i])
-            tgwall = (tgk_w * altmax + tstart_w) * np.sin(((frac - snup / 24) / (tmax_w / 24 - snup / 24)) * np.pi / 2)
-            if tgwall < 0:
-                tgwall = 0
-            tgwall = tgwall * citgg
-            # wave delay of the Lup maps uses the incoming timing; TgOut's call advances it
-            first = 1 if firstdaytime == 1 else 0
-            if timeadd >= 59 / 1440:
-                update, weight = 1, math.exp(-33.27 * timeadd)
-                timeadd = timestepdec if timestepdec > 59 / 1440 else 0.0
-            else:
-                update = 0
-                timeadd = timeadd + timestepdec
-                weight = math.exp(-33.27 * timeadd)
-            firstdaytime = 0
-            off, dz, low, high, middle = shadow_ray(azi, alt, scale, amaxvalue, shape)
-            h[H['DAY']] = 1
-            h[H['CITGG']], h[H['FRAC']] = citgg, frac
-            h[H['LAG_WEIGHT']], h[H['LAG_FIRST']], h[H['LAG_UPDATE']] = weight, first, update
-            h[H['FACE_LOW']], h[H['FACE_HIGH']], h[H['FACE_MIDDLE']] = low, high, float(middle)
-            h[H['RAY_START']], h[H['RAY_LEN']] = start, len(dz)
+            st = (1.,) + st[1:]
+        h, pf[i], pd[i], off, dz, st, m = hour_tables(
+            float(sun['altitude'][i]), float(sun['azimuth'][i]), float(sun['zen'][i]), float(sun['jday'][i]),
+            float(dectime[i]), float(sun['altmax'][i]), *(float(met[i, k]) for k in (11, 10, 14, 12, 9)),
+            met[i, 1], latitude, shape, scale, amaxvalue, st, wall_params=wall_params, constants=constants)
+        h[H['RAY_START']] = start
+        h[H['PATCH_TABLE']] = i
+        hours[i] = h
+        if len(dz):
             offs.append(off)
             dzs.append(dz)
             start += len(dz)
-        else:
-            timeadd = 0.
-            firstdaytime = 1
-        if CI < 0.95:
-            esky = CI * esky + (1 - CI) * 1.
-        h[H['ALTITUDE']], h[H['AZIMUTH']], h[H['TANZEN']] = alt, azi, np.tan(zen)
-        h[H['TA']], h[H['RADG']], h[H['RADD']], h[H['RADI']], h[H['PSI']] = Ta, radG, radD, radI, psi
-        h[H['SINALT']], h[H['COSALT']] = np.sin(alt * (np.pi / 180)), np.cos(alt * DEG)
-        h[H['ALTMAX']], h[H['SNUP']] = altmax, snup
-        h[H['LWALL']] = SBC * c['ewall'] * (tgwall + Ta + 273.15) ** 4 - SBC * c['ewall'] * (Ta + 273.15) ** 4
-        h[H['FLUX4']] = SBC * (Ta + 273.15) ** 4
-        h[H['LW_AMBIENT']] = (c['ewall'] * SBC * ((Ta + 273.15) ** 4)) / np.pi
-        h[H['LW_SUNLIT']] = (c['ewall'] * SBC * ((Ta + tgwall + 273.15) ** 4)) / np.pi
-        h[H['LW_REFL']] = (1 - c['ewall']) * 0.5 / np.pi
-        h[H['SW_SUNLIT']] = (c['albedo_b'] * (radI * np.cos(alt * DEG)) + (radD * 0.5)) / np.pi
-        h[H['SW_SHADED']] = (c['albedo_b'] * radD * 0.5) / np.pi
-        ta32, rh32, va32 = np.float32(Ta), np.float32(RH), np.float32(Ws)
-        h[H['TA32']] = float(ta32)
-        if ta32 <= -999 or rh32 <= -999 or va32 <= -999:
-            h[H['UTCI0']:H['UTCI0'] + 7] = -999.0
-        else:
-            h[H['UTCI0']:H['UTCI0'] + 7] = _utci.coefficients(float(ta32), float(rh32), float(va32))
-        h[H['PATCH_TABLE']] = i
-        h[H['ALBEDO_B']], h[H['ABSK']], h[H['ABSL']] = c['albedo_b'], c['absK'], c['absL']
-        h[H['FUP']], h[H['FSIDE']], h[H['FCYL']] = c['Fup'], c['Fside'], c['Fcyl']
-        h[H['DIFF_VEG']], h[H['TGWALL']] = dveg, tgwall
-        pf[i], pd[i] = patch_rows(alt > 0, esky, Ta, alt, azi, lv, radD)
-        meta.append(dict(CI=CI, esky=esky, radI=radI, radD=radD, tgwall=tgwall))
+        meta.append(m)
     ray_off = np.concatenate(offs) if offs else np.zeros((1, 2), np.int32)
     ray_dz = np.concatenate(dzs) if dzs else np.zeros(1)
     t = DayTables(hours, pf, pd, ray_off, ray_dz, meta)
-    t.end_state = (CI, firstdaytime, timeadd)
+    t.end_state = st[:3]
     return t

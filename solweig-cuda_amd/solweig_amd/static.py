@@ -122,3 +122,53 @@ class PreparedDomain:
         for k, v in extra.items():
             setattr(s, k, v or 0)
         return s
+
+
+def pack_visibility(shmat, vegshmat, vbshvegshmat):
+    """[rows, cols, 153] 0/1 masks -> uint32 [3, 5, P] bit words (bit k of word w = patch 32w+k)."""
+    out = []
+    for m in (shmat, vegshmat, vbshvegshmat):
+        m = np.asarray(m)
+        rows, cols, n = m.shape
+        bits = np.zeros((rows * cols, rt.SW_VIS_WORDS * 32), dtype=np.uint8)
+        bits[:, :n] = (m.reshape(rows * cols, n) > 0.5)
+        words = np.packbits(bits, axis=1, bitorder='little').view('<u4')
+        out.append(words.T)
+    return np.ascontiguousarray(np.stack(out), dtype=np.uint32)
+
+
+def prepared_from_arrays(*, dsm, vegdem, walls, dirwalls, buildings, svfbuveg, asvf, svfalfa, shmat,
+                         vegshmat, vbshvegshmat, alb_grid, emis_grid, TgK, Tstart, TmaxLST, landcover,
+                         wall_params, amaxvalue, scale=1.0, stream=None):
+    """A PreparedDomain from the static dict series._prepare_spatial builds (series.py:350-365),
+    without recomputing the sky view. DSM and canopy heights must be FP32-exact."""
+    a = np.asarray(dsm, dtype=np.float64)
+    top = np.asarray(vegdem, dtype=np.float64)
+    rows, cols = a.shape
+    dsm32 = a.astype(np.float32)
+    trees32 = np.where(top != 0, top - a, 0.0).astype(np.float32)
+    if not np.array_equal(dsm32.astype(np.float64), a):
+        raise ValueError('dsm must be FP32-exact')
+    self = PreparedDomain.__new__(PreparedDomain)
+    from types import SimpleNamespace
+    self.domain = SimpleNamespace(scale=float(scale), dsm=dsm32, trees=trees32, walls=walls, aspect=dirwalls,
+                                  latitude=None, met=None)
+    self.shape, self.P = (rows, cols), rows * cols
+    self.dsm_raise = 0.0
+    self.amaxvalue = float(amaxvalue)
+    self.landcover = int(landcover)
+    self.wall_params = tuple(float(v) for v in wall_params)
+    plane = lambda v: np.broadcast_to(np.asarray(v, dtype=np.float32), (rows, cols))
+    up = lambda x, t=np.float32: rt.DeviceArray.from_host(np.ascontiguousarray(x, dtype=t), stream=stream)
+    self.dev = dict(dsm=up(dsm32), trees=up(trees32), walls=up(plane(walls)), aspect=up(plane(dirwalls)),
+                    ground=up(np.asarray(buildings) == 1, np.uint8), alb=up(plane(alb_grid)),
+                    emis=up(plane(emis_grid)), tgk=up(plane(TgK)), tstart=up(plane(Tstart)),
+                    tmax=up(plane(TmaxLST)), svfbuveg=up(plane(svfbuveg)),
+                    tan_asvf=up(np.tan(np.asarray(asvf, dtype=np.float64)), np.float64),
+                    tan_svfalfa=up(np.tan(np.asarray(svfalfa, dtype=np.float64)), np.float64),
+                    vis=up(pack_visibility(shmat, vegshmat, vbshvegshmat), np.uint32),
+                    rec=rt.DeviceArray((rows * cols, 4), np.float32),
+                    lag=rt.DeviceArray((6, rows * cols), np.float32))
+    self.skyview = None
+    self.tiles = ((cols + rt.TILE_X - 1) // rt.TILE_X) * ((rows + rt.TILE_Y - 1) // rt.TILE_Y)
+    return self

@@ -85,14 +85,19 @@ class SkyView:
 
     def masks(self):
         """Host (shmat, vegmat, vbmat) as float32 [rows, cols, 153] (for parity checks)."""
-        b = self.bits.to_host().reshape(3, rt.SW_VIS_WORDS, -1)
-        out = []
-        for m in range(3):
-            words = b[m].T.astype('<u4')
-            bits = np.unpackbits(words.view(np.uint8).reshape(-1, rt.SW_VIS_WORDS * 4), axis=1,
-                                 bitorder='little')[:, :rt.SW_PATCHES]
-            out.append(bits.reshape(*self.shape, rt.SW_PATCHES).astype(np.float32))
-        return tuple(out)
+        return unpack_visibility(self.bits.to_host(), self.shape)
+
+
+def unpack_visibility(bits, shape, patches=153):
+    """uint32 [3, 5, P] bit words -> three float32 [rows, cols, patches] 0/1 masks."""
+    b = np.asarray(bits, dtype=np.uint32).reshape(3, rt.SW_VIS_WORDS, -1)
+    out = []
+    for m in range(3):
+        words = np.ascontiguousarray(b[m].T, dtype='<u4')
+        u8 = words.view(np.uint8).reshape(words.shape[0], rt.SW_VIS_WORDS * 4)
+        bits_m = np.unpackbits(u8, axis=1, bitorder='little')[:, :patches]
+        out.append(bits_m.reshape(*shape, patches).astype(np.float32))
+    return tuple(out)
 
 
 def sky_view(dsm_dev, trees_dev, dsm_raise, shape, amaxvalue, scale, transmissivity=0.03, stream=None):

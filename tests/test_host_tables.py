@@ -1,0 +1,77 @@
+"""Host-side helpers of the device path (CPU): bit packing, thresholds, tables vs the oracle."""
+import numpy as np
+import pytest
+
+from solweig_amd import sky, static, svf as psvf, sun as psun, synthetic
+from solweig_amd.runtime import H, PD, PF
+from oracle import day as oday, svf as osvf, umep
+
+
+def test_visibility_pack_round_trip():
+    rng = np.random.default_rng(0)
+    masks = [(rng.random((7, 5, 153)) > 0.5).astype(np.float32) for _ in range(3)]
+    words = static.pack_visibility(*masks)
+    assert words.shape == (3, 5, 35) and words.dtype == np.uint32
+    back = psvf.unpack_visibility(words, (7, 5))
+    for a, b in zip(masks, back):
+        np.testing.assert_array_equal(a, b)
+    # bit k of word w is patch 32 w + k
+    one = np.zeros((1, 1, 153), np.float32)
+    one[0, 0, 37] = 1
+    w = static.pack_visibility(one, one, one)
+    assert w[0, 1, 0] == 1 << 5 and w[0, 0, 0] == 0
+
+
+def test_svf_ray_tables_match_oracle():
+    a = psvf.ray_tables(50, 40, 23.5, 1.0)
+    b = osvf.ray_tables(50, 40, 23.5, 1.0)
+    np.testing.assert_array_equal(a['lengths'], b['lengths'])
+    cap = a['capacity']
+    np.testing.assert_array_equal(a['offsets'], b['offsets'][:, :cap])
+    np.testing.assert_array_equal(a['heights'], b['heights'][:, :cap])
+    np.testing.assert_array_equal(a['weights'], b['weights'])
+    np.testing.assert_array_equal(a['sectors'], b['sectors'])
+
+
+@pytest.mark.parametrize('alt', [6.0, 18.0, 42.0, 78.0, 90.0, 6.000000000000001])
+def test_sunlit_thresholds_split_numpy_arctan(alt):
+    lo, hi = sky.sunlit_thresholds(alt)
+    assert lo <= hi
+    f = lambda x: np.arctan(np.full(16, x)) * (180 / np.pi)
+    if np.isfinite(lo) and lo > 0:
+        below = np.nextafter(lo, 0)
+        assert f(below)[0] < alt and not (f(lo)[0] < alt)
+    if np.isfinite(hi) and hi > 0:
+        assert f(hi)[0] > alt and not (f(np.nextafter(hi, 0))[0] > alt)
+
+
+def test_sun_ray_matches_umep_loop():
+    from oracle import brute
+    for az, al in ((135.0, 30.0), (250.0, 3.0), (10.0, 55.0), (300.0, 8.0), (90.0, 20.0), (45.0, 45.0)):
+        off, dz, *_ = sky.shadow_ray(az, al, 1.0, 25.0, (60, 70))
+        steps = brute.ray(az, al, 1.0, 25.0, (60, 70))
+        assert len(steps) == len(dz)
+        np.testing.assert_array_equal(off, np.array([(s[0], s[1]) for s in steps]))
+        np.testing.assert_array_equal(dz, np.array([s[2] for s in steps]))
+
+
+def test_day_tables_follow_the_oracle_scalars():
+    d = synthetic.small_domain(seed=3, size=24)
+    s = psun.met_sun(d.met, d.location, d.utc)
+    t = sky.build_day(d.met, s, d.latitude, d.shape, 1.0, 20.0)
+    p = oday.prepare(d.dsm, d.trees, d.dem, d.walls, d.aspect, d.latitude)
+    state = dict(CI=1., firstdaytime=1., timeadd=0., timestepdec=1 / 24, Tgmap1=0., Tgmap1E=0.,
+                 Tgmap1S=0., Tgmap1W=0., Tgmap1N=0., TgOut1=0.)
+    for i in range(24):
+        hour = dict(altitude=s['altitude'][i], azimuth=s['azimuth'][i], zen=s['zen'][i], jday=s['jday'][i],
+                    dectime=s['dectime'][i], altmax=s['altmax'][i], Ta=d.met[i, 11], RH=d.met[i, 10],
+                    radG=d.met[i, 14], P=d.met[i, 12], psi=0.03)
+        if s['dectime'][i] % 1 == 0:
+            state['CI'] = 1.
+        out, state = umep.step(p, hour, state)
+        m = t.meta[i]
+        assert m['CI'] == state['CI']
+        assert m['esky'] == out['esky']
+        if s['altitude'][i] > 0:
+            assert m['radI'] == out['radI'] and m['radD'] == out['radD']
+            assert m['tgwall'] == out['Tgwall']
