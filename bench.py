@@ -65,6 +65,10 @@ def main():
     ap.add_argument('--size', type=int, default=2000)
     ap.add_argument('--graph', action='store_true', help='replay the day as a captured hipGraph')
     ap.add_argument('--no-cpu', action='store_true')
+    ap.add_argument('--copy-calibration', action='store_true',
+                    help='run a 256 MiB streaming copy (sw_copy) 3 times for HBM counter scaling')
+    ap.add_argument('--plain-days', type=int, default=1,
+                    help='also time the plain per-step forward for this many days (0 = skip)')
     args = ap.parse_args()
     if args.gpus != 1:
         print(f'bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU sharding is a '
@@ -141,6 +145,33 @@ def main():
                     kernel='k_hour_shadow + k_hour_pixels (one hour of the forward)',
                     avg_ms=dict(k_hour_shadow=t_shadow / launches, k_hour_pixels=t_pixel / launches,
                                 daytime_hours=n_day_hours / max(args.steps, 1)))
+    plain = None
+    if args.plain_days > 0:
+        from solweig_amd import model, series
+        from solweig_amd.sun import met_sun
+        prep = model.prepare_from_domain(eng.domains[0], dom.location, dom.scale, stream=eng.stream)
+        mets = [dom.met[k * 24:(k + 1) * 24] for k in range(days)]
+        series.plain_day(prep, mets[0], met_sun(mets[0], dom.location, dom.utc),
+                         ground=eng.domains[0].dev['ground'], keep=False)
+        rt.synchronize()
+        t0 = time.perf_counter()
+        for k in range(args.plain_days):
+            series.plain_day(prep, mets[k + 1], met_sun(mets[k + 1], dom.location, dom.utc),
+                             ground=eng.domains[0].dev['ground'], keep=False)
+        rt.synchronize()
+        pdt = time.perf_counter() - t0
+        plain = dict(value=P * 24 * args.plain_days / pdt, ms_per_day=pdt / args.plain_days * 1e3,
+                     batched_over_plain=value / (P * 24 * args.plain_days / pdt),
+                     note='per-step API: host tables + H2D + full-output shadow and pixel passes + UTCI '
+                          'kernel + stream sync every hour, outputs kept on the device')
+        log(f'[bench] plain per-step forward {plain["value"]:.3e} px-h/s ({plain["ms_per_day"]:.1f} ms/day)')
+    if args.copy_calibration:
+        src = rt.DeviceArray((64 << 20,), np.float32)
+        dst = rt.DeviceArray((64 << 20,), np.float32)
+        for _ in range(3):
+            rt.call('sw_copy', dst.ptr, src.ptr, src.nbytes, eng.stream.handle)
+        eng.stream.sync()
+        log('[bench] copy calibration: 3 x 256 MiB read + 256 MiB written')
     cpu = None if args.no_cpu else cpu_baseline()
     out = dict(metric='Tmrt+UTCI pixel-hours/s at 2000^2 x24h, 1 GPU; batched forward',
                value=value, unit='pixel-hours/s', n_gpus=1, steps=args.steps, warmup=args.warmup,
@@ -148,7 +179,7 @@ def main():
                dtype='f32 (f64 masks/lag/wedge/UTCI)', data='synthetic',
                config=dict(workload=f'c2 batched forward {args.size}x{args.size}, 24 h per step '
                                     f'(30-day forcing cycled), host day tables inside the timed region',
-                           graph=bool(args.graph), static_prep_s=t_static),
+                           graph=bool(args.graph), static_prep_s=t_static, plain_forward=plain),
                roofline=roof, cpu_baseline=cpu)
     print(json.dumps(out), flush=True)
 

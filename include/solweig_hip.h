@@ -148,6 +148,10 @@ int sw_hours_forward(const sw_domain *domains_dev, const int32_t *block_start_de
 int sw_utci_scalar(const float *tmrt_dev, long long n, const double *coefficients, double ta32,
                    int sentinel, const uint8_t *ground_dev, float *out_dev, void *stream);
 
+/* Streaming device copy (float4 loads/stores, bytes and addresses 16-byte aligned): the plain
+ * kernel of known traffic used to calibrate HBM byte counters. No reference counterpart.       */
+int sw_copy(void *dst_dev, const void *src_dev, size_t bytes, void *stream);
+
 /* hipGraph capture of a launch sequence on ``stream`` (the captured day loop).            */
 int sw_graph_begin(void *stream);
 int sw_graph_end(void *stream, void **graph_exec);

@@ -21,6 +21,8 @@
 
 #include "../../../include/solweig_hip.h"
 
+static_assert(sizeof(sw_domain) == 256, "sw_domain must stay 32 eight-byte fields");
+
 namespace {
 
 constexpr int TILE_X = 32;
@@ -649,6 +651,14 @@ __global__ void k_utci_scalar(const float *__restrict__ tmrt, long long n, Coef7
     }
 }
 
+// Plain streaming copy: the reference point for HBM byte counters.
+__global__ __launch_bounds__(256) void k_copy(const float4 *__restrict__ src, float4 *__restrict__ dst,
+                                              long long n) {
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (long long)blockDim.x * gridDim.x)
+        dst[i] = src[i];
+}
+
 int launch_hour_shadow(const sw_domain *doms, const int32_t *bs, int nd, int total, int hour,
                        bool full, hipStream_t s) {
     if (full)
@@ -839,6 +849,16 @@ int sw_utci_scalar(const float *tmrt, long long n, const double *coefficients, d
     return hip_check(hipGetLastError(), "k_utci_scalar launch");
 }
 
+int sw_copy(void *dst, const void *src, size_t bytes, void *stream) {
+    if (!dst || !src) return fail(SW_ERR_ARGUMENT, "sw_copy: NULL pointer");
+    if (bytes % 16 != 0 || ((uintptr_t)dst | (uintptr_t)src) % 16 != 0)
+        return fail(SW_ERR_ARGUMENT, "sw_copy: size and addresses must be 16-byte multiples");
+    const long long n = (long long)(bytes / 16);
+    hipLaunchKernelGGL(k_copy, dim3(2048), dim3(256), 0, (hipStream_t)stream, (const float4 *)src,
+                       (float4 *)dst, n);
+    return hip_check(hipGetLastError(), "k_copy launch");
+}
+
 int sw_graph_begin(void *stream) {
     return hip_check(hipStreamBeginCapture((hipStream_t)stream, hipStreamCaptureModeThreadLocal),
                      "hipStreamBeginCapture");
@@ -851,7 +871,7 @@ int sw_graph_end(void *stream, void **graph_exec) {
     if (rc) return rc;
     hipGraphExec_t e = nullptr;
     rc = hip_check(hipGraphInstantiate(&e, g, nullptr, nullptr, 0), "hipGraphInstantiate");
-    hipGraphDestroy(g);
+    (void)hipGraphDestroy(g);
     if (rc) return rc;
     *graph_exec = (void *)e;
     return SW_OK;

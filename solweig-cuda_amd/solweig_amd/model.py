@@ -82,6 +82,15 @@ def prepare_model(*, dtype=np.float32, **inputs):
         Tstart=g['Tstart'], TmaxLST=g['TmaxLST'], landcover=int(inputs.get('landcover', 0)),
         wall_params=(float(inputs['TgK_wall']), float(inputs['Tstart_wall']), float(inputs['TmaxLST_wall'])),
         amaxvalue=float(inputs['amaxvalue']), scale=float(inputs['scale']), stream=stream)
+    loc = inputs.get('location') or {'latitude': 0.0}
+    return prepare_from_domain(dom, loc, float(inputs['scale']), float(inputs.get('first', 1.0)),
+                               float(inputs.get('second', 22.0)), stream=stream)
+
+
+def prepare_from_domain(dom, location, scale=1.0, first=1.0, second=22.0, stream=None):
+    """Per-step buffers around an already prepared domain (static.PreparedDomain)."""
+    stream = stream or rt.Stream()
+    rows, cols = dom.shape
     P = rows * cols
     cap = max(rows, cols) + 2
     bufs = dict(hours=rt.DeviceArray((1, rt.SW_HOUR_STRIDE), np.float64),
@@ -97,13 +106,11 @@ def prepare_model(*, dtype=np.float32, **inputs):
     rt.call('sw_memcpy_h2d', dom_dev.ptr, ctypes.addressof(s), ctypes.sizeof(s), stream.handle)
     stream.sync()
     bs = rt.DeviceArray.from_host(np.array([0, dom.tiles], dtype=np.int32))
-    off, bounds, first, second = sky.gvf_geometry(float(inputs['scale']), float(inputs.get('first', 1.0)),
-                                                  float(inputs.get('second', 22.0)))
-    loc = inputs.get('location') or {'latitude': 0.0}
+    off, bounds, f_steps, s_steps = sky.gvf_geometry(scale, first, second)
     return PreparedModel(dom, bufs['hours'], bufs['ptab_f'], bufs['ptab_d'], bufs['ray_off'], bufs['ray_dz'],
                          bufs['full'], bufs['shadow_planes'], bufs['tmrt'], dom_dev, bs,
                          rt.DeviceArray.from_host(off, np.int32), rt.DeviceArray.from_host(bounds, np.float64),
-                         first, second, (rows, cols), dtype, stream, loc, float(inputs['scale']))
+                         f_steps, s_steps, (rows, cols), np.dtype(np.float32), stream, location, float(scale))
 
 
 def _set_lag(prep, k, value):
@@ -111,7 +118,8 @@ def _set_lag(prep, k, value):
     lag = prep.domain.dev['lag']
     dst = lag.ptr + k * P * 4
     if isinstance(value, (rt.DeviceArray, rt.DeviceView)) and value.dtype == np.float32:
-        rt.call('sw_memcpy_d2d', dst, value.ptr, P * 4, prep.stream.handle)
+        if value.ptr != dst:
+            rt.call('sw_memcpy_d2d', dst, value.ptr, P * 4, prep.stream.handle)
         return
     a = np.ascontiguousarray(np.broadcast_to(np.asarray(_host(value), dtype=np.float32), prep.shape))
     rt.call('sw_memcpy_h2d', dst, a.ctypes.data, a.nbytes, prep.stream.handle)

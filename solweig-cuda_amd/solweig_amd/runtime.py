@@ -78,6 +78,7 @@ def lib():
         'sw_hour_pixels': ([vp, vp, i32, i32, i32, vp, vp, i32, i32, i32, vp], i32),
         'sw_hours_forward': ([vp, vp, i32, i32, i32, i32, vp, vp, i32, i32, vp], i32),
         'sw_utci_scalar': ([vp, i64, vp, f64, i32, vp, vp, vp], i32),
+        'sw_copy': ([vp, vp, sz, vp], i32),
         'sw_graph_begin': ([vp], i32), 'sw_graph_end': ([vp, ctypes.POINTER(vp)], i32),
         'sw_graph_launch': ([vp, vp], i32), 'sw_graph_destroy': ([vp], i32),
     }
@@ -94,7 +95,7 @@ EXPORTED = ('sw_version', 'sw_last_error', 'sw_device_count', 'sw_set_device', '
             'sw_stream_destroy', 'sw_stream_sync', 'sw_event_create', 'sw_event_destroy',
             'sw_event_record', 'sw_event_elapsed_ms', 'sw_device_sync', 'sw_svf_visibility',
             'sw_svf_reduce', 'sw_hour_shadow', 'sw_hour_pixels', 'sw_hours_forward', 'sw_utci_scalar',
-            'sw_graph_begin', 'sw_graph_end', 'sw_graph_launch', 'sw_graph_destroy')
+            'sw_copy', 'sw_graph_begin', 'sw_graph_end', 'sw_graph_launch', 'sw_graph_destroy')
 
 
 def check(rc):

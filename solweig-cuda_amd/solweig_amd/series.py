@@ -131,29 +131,17 @@ def run_timeseries(building_dsm_path, tree_path, dem_path, met_file, output_path
 
 
 def _plain_day(dom, met, sun):
-    """One day through the per-step API (the plain form, series.py:445-469)."""
-    from . import model, utci as u
+    """One day through the per-step path (the plain form, series.py:445-469)."""
+    from . import model
     from .static import PreparedDomain
-    pd = PreparedDomain(dom, keep_host=True)
-    h = pd.host
-    rows, cols = pd.shape
-    inputs = dict(rows=rows, cols=cols, scale=dom.scale, dsm=h['a'], vegdem=np.where(h['trees'] > 0, h['trees'] + h['a'], 0.0),
-                  walls=dom.walls, dirwalls=dom.aspect, buildings=h['ground'].astype(np.float64),
-                  svfbuveg=h['svfbuveg'], asvf=h['asvf'], svfalfa=h['svfalfa'],
-                  shmat=None, vegshmat=None, vbshvegshmat=None, alb_grid=h['maps']['alb'],
-                  emis_grid=h['maps']['emis'], TgK=h['maps']['tgk'], Tstart=h['maps']['tstart'],
-                  TmaxLST=h['maps']['tmax'], landcover=pd.landcover, TgK_wall=pd.wall_params[0],
-                  Tstart_wall=pd.wall_params[1], TmaxLST_wall=pd.wall_params[2], amaxvalue=pd.amaxvalue,
-                  location=dom.location)
-    shm, vgm, vbm = pd.skyview.masks()
-    inputs.update(shmat=shm, vegshmat=vgm, vbshvegshmat=vbm)
-    prep = model.prepare_model(**inputs)
+    pd = PreparedDomain(dom)
+    prep = model.prepare_from_domain(pd, dom.location, dom.scale)
     return plain_day(prep, met, sun, ground=pd.dev['ground'])
 
 
-def plain_day(prep, met, sun, ground=None):
+def plain_day(prep, met, sun, ground=None, keep=True):
     """Hour loop of series._run_day over model.Solweig_2022a_calc-style steps; returns host
-    (Tmrt, UTCI) stacks."""
+    (Tmrt, UTCI) stacks (``keep``) or leaves the last hour on the device."""
     from . import model, utci as u
     n = met.shape[0]
     dectime = sun['dectime']
@@ -173,9 +161,10 @@ def plain_day(prep, met, sun, ground=None):
         tmrt = prep.tmrt.plane(0, prep.shape)
         utci = u.utci_calculator(met[i, 11], met[i, 10], tmrt, met[i, 9], ground=ground, stream=prep.stream)
         prep.stream.sync()
-        tm.append(tmrt.to_host())
-        ut.append(utci.to_host())
-    return np.stack(tm), np.stack(ut)
+        if keep:
+            tm.append(tmrt.to_host())
+            ut.append(utci.to_host())
+    return (np.stack(tm), np.stack(ut)) if keep else (tmrt, utci)
 
 
 PATH_KEYS = {'building_dsm_path', 'tree_path', 'dem_path', 'met_file', 'building_geometry_path',
