@@ -123,11 +123,16 @@ def main():
         tabs = one_day(args.warmup + k, True)
         eng.stream.sync()
         if tabs is not None:
+            per_hour = []
             for h in range(24):
                 if tabs[0].hours[h, 0]:
                     n_day_hours += 1
-                t_shadow += ev[3 * h].elapsed_ms(ev[3 * h + 1])
-                t_pixel += ev[3 * h + 1].elapsed_ms(ev[3 * h + 2])
+                a = ev[3 * h].elapsed_ms(ev[3 * h + 1])
+                b = ev[3 * h + 1].elapsed_ms(ev[3 * h + 2])
+                t_shadow += a
+                t_pixel += b
+                per_hour.append(f'{h}:{a:.2f}/{b:.2f}(alt {tabs[0].hours[h, 1]:.0f})')
+            log('[bench] per hour shadow/pixel ms: ' + ' '.join(per_hour))
         log(f'[bench] day {k} done')
     rt.synchronize()
     dt = time.perf_counter() - t0
