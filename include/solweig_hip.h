@@ -45,6 +45,8 @@ enum {
     SWH_ALBEDO_B, SWH_ABSK, SWH_ABSL, SWH_FUP, SWH_FSIDE, SWH_FCYL,
     SWH_DIFF_VEG,         /* diffsh of a vegetation-blocked patch: 1-(1-0.03f) in FP32       */
     SWH_TGWALL,
+    SWH_SKY_A, SWH_SKY_B, /* sky patch flux = A*omega*trig - B*g*omega*trig, A = SBC*T^4/pi,
+                             B = (1 - esky)*A, g = exp(0.308*(1.7 - 1/cos(zenith))) per band */
     SW_HOUR_STRIDE = 64
 };
 
@@ -63,7 +65,7 @@ enum {
     SWF_KSIDED, SWF_DRAD, SWF_KSIDE, SWF_SHADOW, SWF_WALLSUN, SWF_FSH, SW_FULL_PLANES
 };
 
-/* Device-side description of one domain. All fields are 8 bytes wide; pointers are device
+/* Device-side description of one domain (264 bytes). All fields are 8 bytes wide; pointers are device
  * pointers. Optional outputs may be NULL. */
 typedef struct sw_domain {
     int64_t rows, cols, pixels, landcover;
@@ -84,7 +86,18 @@ typedef struct sw_domain {
     uint8_t *masks;                                          /* [P] bits of the last shadow pass */
     double *shadow_planes;                                   /* [8][P] cast_shadow planes      */
     float *full;                                             /* [SW_FULL_PLANES][P]           */
+    const float *skysums;                                    /* [SW_SKY_SUMS][P] static sums   */
 } sw_domain;
+
+/* Static per-pixel sums over the patches of each visibility class (day patch geometry), the
+ * hour-independent part of the longwave patch sums: */
+enum {
+    SWS_SKY_SIN = 0, SWS_SKY_GSIN, SWS_SKY_COS, SWS_SKY_GCOS,   /* sky patches: w sin, g w sin, ... */
+    SWS_VEG_COS, SWS_VEG_SIN,                                 /* vegetation patches              */
+    SWS_OBS_COS, SWS_OBS_SIN,                                 /* any obstruction (reflection)    */
+    SWS_BLD_COS, SWS_BLD_SIN,                                 /* building patches                */
+    SW_SKY_SUMS = 10
+};
 
 /* ----- runtime ------------------------------------------------------------------------ */
 int sw_version(void);
@@ -119,6 +132,11 @@ int sw_svf_reduce(const uint32_t *bits_dev, const float *dsm_dev, const float *t
                   double dsm_raise, int rows, int cols, const uint8_t *sectors_dev,
                   const int32_t *band_offsets_dev, const double *weights_dev, int bands,
                   int patches, double transmissivity, double *out_dev, void *stream);
+
+/* Static patch sums per pixel from the visibility bits: table_dev [153][4] floats per patch
+ * (w sin, w cos, g w sin, g w cos; w the steradian) -> out_dev [SW_SKY_SUMS][P].            */
+int sw_sky_sums(const uint32_t *bits_dev, long long pixels, const float *table_dev, float *out_dev,
+                void *stream);
 
 /* ----- per-hour path ----------------------------------------------------------------- */
 /* Sun-ray shadow march for hour ``hour`` of every domain (replaces model.cast_shadow,

@@ -21,7 +21,7 @@
 
 #include "../../../include/solweig_hip.h"
 
-static_assert(sizeof(sw_domain) == 256, "sw_domain must stay 32 eight-byte fields");
+static_assert(sizeof(sw_domain) == 264, "sw_domain must stay 33 eight-byte fields");
 
 namespace {
 
@@ -202,9 +202,15 @@ __global__ __launch_bounds__(256) void k_svf_reduce(
 
 // =====================================================================================
 // Per-hour sun-ray shadow march (shadowingfunction_wallheight_23, umep_solweig.py:2201-2406;
-// geometry.cu:185-253) + the per-pixel record the ground-view gather reads.
-// rec = (shadow, Lup field of sunonsurface, albedo, ground | sunwall << 1).
+// geometry.cu:185-253) and the 8-byte record the ground-view walk gathers:
+//   rec.x = Lup field of sunonsurface (umep_solweig.py:1418), rec.y = albedo with its four low
+//   mantissa bits replaced by flags: bit 0 ground, bit 1 sunlit wall, bits 2-3 shadow class
+//   (0: building shadow, 1: canopy shadow, 2: sunlit). The albedo loses < 2^-19 relatively.
 // =====================================================================================
+__device__ __forceinline__ float pack_albedo(float alb, uint32_t flags) {
+    return __uint_as_float((__float_as_uint(alb) & ~15u) | flags);
+}
+
 template <bool FULL>
 __global__ __launch_bounds__(BLOCK) void k_hour_shadow(const sw_domain *__restrict__ doms,
                                                        const int32_t *__restrict__ bs, int nd,
@@ -269,8 +275,8 @@ __global__ __launch_bounds__(BLOCK) void k_hour_shadow(const sw_domain *__restri
     const double sh = blocked ? 0.0 : 1.0;
     const double vegsh = 1.0 - vegv;
     const double shadow = sh - (1.0 - vegsh) * (1.0 - H[SWH_PSI]);
-    const double ground = D->ground[p] ? 1.0 : 0.0;
-    const bool sunwall = (wallsun / wall * ground) == 1.0;
+    const bool ground = D->ground[p] != 0;
+    const bool sunwall = (wallsun / wall * (ground ? 1.0 : 0.0)) == 1.0;
     // Surface temperature of the hour (umep_solweig.py:144-170) and the sunonsurface Lup field.
     const double snup = H[SWH_SNUP];
     const double amp = (double)D->tgk[p] * H[SWH_ALTMAX] + (double)D->tstart[p];
@@ -281,12 +287,11 @@ __global__ __launch_bounds__(BLOCK) void k_hour_shadow(const sw_domain *__restri
     const double ta = H[SWH_TA], emis = (double)D->emis[p];
     const double tk = tg * shadow + ta + 273.15, ta4 = (ta + 273.15);
     const double lup = SBC * emis * (tk * tk) * (tk * tk) - SBC * emis * (ta4 * ta4) * (ta4 * ta4);
-    float4 rec;
-    rec.x = (float)shadow;
-    rec.y = (float)lup;
-    rec.z = D->alb[p];
-    rec.w = (float)((D->ground[p] ? 1 : 0) | (sunwall ? 2 : 0));
-    reinterpret_cast<float4 *>(D->rec)[p] = rec;
+    const uint32_t cls = blocked ? 0u : (veg ? 1u : 2u);
+    float2 rec;
+    rec.x = (float)lup;
+    rec.y = pack_albedo(D->alb[p], (ground ? 1u : 0u) | (sunwall ? 2u : 0u) | (cls << 2));
+    reinterpret_cast<float2 *>(D->rec)[p] = rec;
     if (D->masks)
         D->masks[p] = (uint8_t)((blocked ? 0 : 1) | (veg ? 0 : 2) | ((ever && !veg) ? 0 : 4) |
                                 (sunwall ? 8 : 0) | (facesh == 1.0 ? 16 : 0));
@@ -302,6 +307,31 @@ __global__ __launch_bounds__(BLOCK) void k_hour_shadow(const sw_domain *__restri
         o[6 * P + p] = facesh;
         o[7 * P + p] = facesun;
     }
+}
+
+// Static per-pixel patch sums of the hour-independent longwave terms (from the visibility bits).
+__global__ __launch_bounds__(256) void k_sky_sums(const uint32_t *__restrict__ bits, long long P,
+                                                  const float *__restrict__ tab, float *__restrict__ out) {
+    const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= P) return;
+    float s[SW_SKY_SUMS];
+    for (int i = 0; i < SW_SKY_SUMS; ++i) s[i] = 0.f;
+    for (int w = 0; w < SW_VIS_WORDS; ++w) {
+        const uint32_t vsh = bits[(0 * SW_VIS_WORDS + w) * P + p];
+        const uint32_t vveg = bits[(1 * SW_VIS_WORDS + w) * P + p];
+        const uint32_t vvb = bits[(2 * SW_VIS_WORDS + w) * P + p];
+        const uint32_t sky = vsh & vveg, vegf = ~vveg | ~vvb, bld = ~vsh & vvb, obst = ~vsh | ~vveg | ~vvb;
+        const int k0 = w * 32, k1 = min(SW_PATCHES, k0 + 32);
+        for (int k = k0; k < k1; ++k) {
+            const int j = k - k0;
+            const float *c = tab + 4 * k;
+            if ((sky >> j) & 1) { s[SWS_SKY_SIN] += c[0]; s[SWS_SKY_COS] += c[1]; s[SWS_SKY_GSIN] += c[2]; s[SWS_SKY_GCOS] += c[3]; }
+            if ((vegf >> j) & 1) { s[SWS_VEG_SIN] += c[0]; s[SWS_VEG_COS] += c[1]; }
+            if ((obst >> j) & 1) { s[SWS_OBS_SIN] += c[0]; s[SWS_OBS_COS] += c[1]; }
+            if ((bld >> j) & 1) { s[SWS_BLD_SIN] += c[0]; s[SWS_BLD_COS] += c[1]; }
+        }
+    }
+    for (int i = 0; i < SW_SKY_SUMS; ++i) out[i * P + p] = s[i];
 }
 
 // =====================================================================================
@@ -325,78 +355,102 @@ struct GvfOut {
     Triple all, e, s, w, n;
 };
 
-__device__ __forceinline__ GvfOut ground_view(const sw_domain *D, const PixelAt &at, const float4 me,
+struct Sample {
+    float lup, alb, shv;
+    bool ground, sunwall;
+};
+
+__device__ __forceinline__ Sample decode(float2 q, float shv_canopy) {
+    Sample s;
+    const uint32_t u = __float_as_uint(q.y);
+    s.lup = q.x;
+    s.alb = __uint_as_float(u & ~15u);
+    s.ground = (u & 1u) != 0;
+    s.sunwall = (u & 2u) != 0;
+    const uint32_t cls = (u >> 2) & 3u;
+    s.shv = cls == 2u ? 1.0f : (cls == 1u ? shv_canopy : 0.0f);
+    return s;
+}
+
+// One pixel's walk over the 18 directions x ``second`` steps. The running minimum f of the
+// ground mask only falls from 1 to 0; once it is 0 every later step adds nothing to the ground
+// sums, keeps the sunlit-wall flag and sets the any-wall flag, so the rest of the walk is added
+// in closed form. Wall sums are counts times the per-hour wall flux / wall albedo.
+template <bool CHECK>
+__device__ __forceinline__ GvfOut ground_view(const sw_domain *D, const PixelAt &at, const Sample me,
                                               const double *H, const int32_t *__restrict__ gofs,
-                                              const double *__restrict__ gbounds, int first,
-                                              int second) {
+                                              const double *__restrict__ gbounds, int first, int second,
+                                              float shv_canopy) {
     const int rows = (int)D->rows, cols = (int)D->cols;
-    const float4 *rec = reinterpret_cast<const float4 *>(D->rec);
-    const float building = ((int)me.w & 1) ? 1.0f : 0.0f;
+    const float2 *rec = reinterpret_cast<const float2 *>(D->rec);
     const float lwall = (float)H[SWH_LWALL];
     const float albedo_b = (float)H[SWH_ALBEDO_B];
-    const float wallbol = D->walls[at.p] > 0.0f ? 1.0f : 0.0f;
+    const float building = me.ground ? 1.0f : 0.0f;
+    const bool wallbol = D->walls[at.p] > 0.0f;
     const double aspect = ((double)D->aspect[at.p] * 3.141592653589793) / 180.0;
     const float firstf = (float)first, secondf = (float)second;
-    const float local_lup = me.y * (1.0f - building);
-    const float local_albsh = me.z * (1.0f - building) * me.x;
-    const float local_nosh = me.z * (1.0f - building);
+    const float local_lup = me.lup * (1.0f - building);
+    const float local_albsh = me.alb * (1.0f - building) * me.shv;
+    const float local_nosh = me.alb * (1.0f - building);
     GvfOut o = {};
     for (int dir = 0; dir < 18; ++dir) {
         const int32_t *off = gofs + dir * second * 2;
-        float f = building;
-        float t_bu = 0.f, t_sh = 0.f, t_lup = 0.f, t_alb = 0.f, t_sw = 0.f;
-        float wall_seen = 0.f, any_wall = 0.f;
-        float s_sh = 0.f, s_wall = 0.f, s_lup = 0.f, s_lwall = 0.f, s_albsh = 0.f;
-        float s_albwall = 0.f, s_alb = 0.f, s_albwallnosh = 0.f;
-        float f_lwall = 0.f, f_lup = 0.f, f_albwall = 0.f, f_albsh = 0.f, f_albwallnosh = 0.f,
-              f_alb = 0.f;
-        bool f_sunin = false;
-        for (int st = 0; st < second; ++st) {
-            const int rr = at.r + off[2 * st], cc = at.c + off[2 * st + 1];
-            if ((unsigned)rr < (unsigned)rows && (unsigned)cc < (unsigned)cols) {
-                const float4 q = rec[(long long)rr * cols + cc];
-                const int qb = (int)q.w;
-                t_bu = (float)(qb & 1);
-                t_sw = (float)((qb >> 1) & 1);
-                t_sh = q.x;
-                t_lup = q.y;
-                t_alb = q.z;
-            }
-            f = fminf(f, t_bu);
-            s_sh += t_sh * f;
-            s_lup += t_lup * f;
-            s_albsh += (t_alb * t_sh) * f;
-            s_alb += t_alb * f;
-            wall_seen = (t_sw * f + wall_seen) > 0.f ? 1.f : 0.f;
-            any_wall = ((1.f - f) + any_wall) > 0.f ? 1.f : 0.f;
-            s_lwall += wall_seen * lwall;
-            s_albwall += wall_seen * albedo_b;
-            s_wall += wall_seen;
-            s_albwallnosh += any_wall * albedo_b;
-            if (st + 1 <= first) {
-                f_sunin = s_wall > 0.f;
-                f_lwall = s_lwall;
-                f_lup = s_lup;
-                f_albwall = s_albwall;
-                f_albsh = s_albsh;
-                f_albwallnosh = s_albwallnosh;
-                f_alb = s_alb;
+        float s_lup = 0.f, s_albsh = 0.f, s_alb = 0.f;
+        float f_lup = 0.f, f_albsh = 0.f, f_alb = 0.f;
+        int cnt_wall = 0, cnt_any = 0, f_cnt_wall = 0, f_cnt_any = 0;
+        if (!me.ground) {
+            cnt_any = second;                         // f = 0 from the start: all walls "any"
+            f_cnt_any = first;
+        } else {
+            Sample t = me;
+            bool wall_seen = false;
+            for (int st = 0; st < second; ++st) {
+                const int dr = off[2 * st], dc = off[2 * st + 1];
+                if (!CHECK) {
+                    t = decode(rec[at.p + (long long)dr * cols + dc], shv_canopy);
+                } else {
+                    const int rr = at.r + dr, cc = at.c + dc;
+                    if ((unsigned)rr < (unsigned)rows && (unsigned)cc < (unsigned)cols)
+                        t = decode(rec[(long long)rr * cols + cc], shv_canopy);
+                }
+                if (t.ground) {
+                    s_lup += t.lup;
+                    s_albsh += t.alb * t.shv;
+                    s_alb += t.alb;
+                    wall_seen = wall_seen || t.sunwall;
+                    cnt_wall += wall_seen ? 1 : 0;
+                    if (st + 1 <= first) {
+                        f_lup = s_lup; f_albsh = s_albsh; f_alb = s_alb;
+                        f_cnt_wall = cnt_wall; f_cnt_any = cnt_any;
+                    }
+                } else {
+                    const int rest = second - st;     // this step and the remaining ones
+                    const int w = wall_seen ? 1 : 0;
+                    if (st + 1 <= first) {
+                        const int upto = first - st;
+                        f_lup = s_lup; f_albsh = s_albsh; f_alb = s_alb;
+                        f_cnt_wall = cnt_wall + w * upto;
+                        f_cnt_any = cnt_any + upto;
+                    }
+                    cnt_wall += w * rest;
+                    cnt_any += rest;
+                    break;
+                }
             }
         }
-        const bool sunin2 = s_wall > 0.f;
-        const bool f_wallin = f_albwallnosh > 0.f;
-        const bool wallin2 = s_albwallnosh > 0.f;
+        const bool sunin2 = cnt_wall > 0, f_sunin = f_cnt_wall > 0;
+        const bool wallin2 = cnt_any > 0, f_wallin = f_cnt_any > 0;
         const double *bd = gbounds + dir * 3;
-        float facesh;
+        int facesh;
         if (bd[2] == 0.0)
-            facesh = (float)(aspect < bd[0] || aspect >= bd[1]) - wallbol + 1.0f;
+            facesh = (int)(aspect < bd[0] || aspect >= bd[1]) - (wallbol ? 1 : 0) + 1;
         else
-            facesh = 1.0f - (float)(aspect > bd[0] || aspect <= bd[1]);
-        float keep = (s_wall == secondf ? 1.0f : 0.0f) - facesh;
-        if (keep == 1.0f) {
-            s_lwall = 0.f;
-            s_albwall = 0.f;
-        }
+            facesh = 1 - (int)(aspect > bd[0] || aspect <= bd[1]);
+        const bool keep = (cnt_wall == second) && facesh == 0;
+        const float s_lwall = keep ? 0.f : (float)cnt_wall * lwall;
+        const float s_albwall = keep ? 0.f : (float)cnt_wall * albedo_b;
+        const float f_lwall = (float)f_cnt_wall * lwall, f_albwall = (float)f_cnt_wall * albedo_b;
+        const float s_albwallnosh = (float)cnt_any * albedo_b, f_albwallnosh = (float)f_cnt_any * albedo_b;
         const float lup_d = distance_weight(influence(f_lwall, f_lup, firstf + 1.f, firstf, f_sunin),
                                             influence(s_lwall, s_lup, secondf + 1.f, secondf, sunin2)) +
                             local_lup;
@@ -462,11 +516,14 @@ __global__ __launch_bounds__(BLOCK) void k_hour_pixels(const sw_domain *__restri
     const float flux4 = (float)H[SWH_FLUX4];
     float Kdown = 0.f, Kup = 0.f, KupE = 0.f, KupS = 0.f, KupW = 0.f, KupN = 0.f;
     float KsideI = 0.f, Lup, LupE, LupS, LupW, LupN, shadow = 0.f, Fsh = 0.f, tgout = 0.f, tg = 0.f;
-    float4 me = make_float4(0.f, 0.f, 0.f, 0.f);
     if (day) {
-        me = reinterpret_cast<const float4 *>(D->rec)[p];
-        shadow = me.x;
-        const GvfOut g = ground_view(D, at, me, H, gofs, gbounds, first, second);
+        const float shv_canopy = (float)(1.0 - (1.0 - H[SWH_PSI]));
+        const Sample me = decode(reinterpret_cast<const float2 *>(D->rec)[p], shv_canopy);
+        shadow = me.shv;
+        const int m = second - 1;
+        const bool interior = at.r >= m && at.c >= m && at.r < (int)D->rows - m && at.c < (int)D->cols - m;
+        const GvfOut g = interior ? ground_view<false>(D, at, me, H, gofs, gbounds, first, second, shv_canopy)
+                                  : ground_view<true>(D, at, me, H, gofs, gbounds, first, second, shv_canopy);
         const float amb = flux4 * emis;
         const double w = H[SWH_LAG_WEIGHT];
         const bool lf = H[SWH_LAG_FIRST] != 0.0, lu = H[SWH_LAG_UPDATE] != 0.0;
@@ -519,61 +576,79 @@ __global__ __launch_bounds__(BLOCK) void k_hour_pixels(const sw_domain *__restri
     float rE = 0.f, rW = 0.f, rN = 0.f, rS = 0.f;       // FULL: cardinal reflected weights
     const float lw_amb = (float)H[SWH_LW_AMBIENT], lw_sun = (float)H[SWH_LW_SUNLIT];
     const uint32_t *vis = D->vis;
-    for (int w = 0; w < SW_VIS_WORDS; ++w) {
-        const uint32_t vsh = vis[(0 * SW_VIS_WORDS + w) * P + p];
-        const uint32_t vveg = vis[(1 * SW_VIS_WORDS + w) * P + p];
-        const uint32_t vvb = vis[(2 * SW_VIS_WORDS + w) * P + p];
-        const uint32_t sky = vsh & vveg, vegonly = vsh & ~vveg;
-        const uint32_t vegf = ~vveg | ~vvb, bld = ~vsh & vvb, obst = ~vsh | ~vveg | ~vvb;
-        const int k0 = w * 32, k1 = min(SW_PATCHES, k0 + 32);
-        for (int k = k0; k < k1; ++k) {
-            const int j = k - k0;
-            const float *c = pf + k * SW_PATCH_F;
-            const double *cd = pd + k * SW_PATCH_D;
-            const float wcos = c[SWP_WCOS], wsin = c[SWP_WSIN];
-            const float dsh = ((sky >> j) & 1) ? 1.0f : (((vegonly >> j) & 1) ? dveg : 0.0f);
-            a_lv += dsh * c[SWP_LV];
-            a_kd += dsh * c[SWP_KD];
-            if ((sky >> j) & 1) {
-                a_ldn += c[SWP_LDN];
-                a_lsd += c[SWP_LSD];
+    if (!FULL) {
+        // hour-independent parts from the static sums; sky flux linear in the sky emissivity
+        const float *ss = D->skysums;
+        const float A = (float)H[SWH_SKY_A], B = (float)H[SWH_SKY_B];
+        a_ldn = A * ss[SWS_SKY_SIN * P + p] - B * ss[SWS_SKY_GSIN * P + p];
+        a_lsd = A * ss[SWS_SKY_COS * P + p] - B * ss[SWS_SKY_GCOS * P + p];
+        a_vc = ss[SWS_VEG_COS * P + p];
+        a_vs = ss[SWS_VEG_SIN * P + p];
+        a_rc = ss[SWS_OBS_COS * P + p];
+        a_rs = ss[SWS_OBS_SIN * P + p];
+        if (!day) {
+            a_lshc = ss[SWS_BLD_COS * P + p];
+            a_lshs = ss[SWS_BLD_SIN * P + p];
+        }
+    }
+    if (FULL || day) {
+        float lv_sky = 0.f, lv_veg = 0.f, kd_sky = 0.f, kd_veg = 0.f;
+        for (int w = 0; w < SW_VIS_WORDS; ++w) {
+            const uint32_t vsh = vis[(0 * SW_VIS_WORDS + w) * P + p];
+            const uint32_t vveg = vis[(1 * SW_VIS_WORDS + w) * P + p];
+            const uint32_t vvb = vis[(2 * SW_VIS_WORDS + w) * P + p];
+            const uint32_t sky = vsh & vveg, vegonly = vsh & ~vveg;
+            const uint32_t vegf = ~vveg | ~vvb, bld = ~vsh & vvb, obst = ~vsh | ~vveg | ~vvb;
+            const int k0 = w * 32, k1 = min(SW_PATCHES, k0 + 32);
+            for (int k = k0; k < k1; ++k) {
+                const int j = k - k0;
+                const float *c = pf + k * SW_PATCH_F;
+                const double *cd = pd + k * SW_PATCH_D;
+                const float wcos = c[SWP_WCOS], wsin = c[SWP_WSIN];
+                if ((sky >> j) & 1) { lv_sky += c[SWP_LV]; kd_sky += c[SWP_KD]; }
+                if ((vegonly >> j) & 1) { lv_veg += c[SWP_LV]; kd_veg += c[SWP_KD]; }
                 if (FULL) {
-                    cE += c[SWP_LSD] * c[SWP_DE]; cW += c[SWP_LSD] * c[SWP_DW];
-                    cN += c[SWP_LSD] * c[SWP_DN]; cS += c[SWP_LSD] * c[SWP_DS];
+                    if ((sky >> j) & 1) {
+                        a_ldn += c[SWP_LDN];
+                        a_lsd += c[SWP_LSD];
+                        cE += c[SWP_LSD] * c[SWP_DE]; cW += c[SWP_LSD] * c[SWP_DW];
+                        cN += c[SWP_LSD] * c[SWP_DN]; cS += c[SWP_LSD] * c[SWP_DS];
+                    }
+                    if ((vegf >> j) & 1) {
+                        a_vc += wcos;
+                        a_vs += wsin;
+                        const float v = lw_amb * wcos;
+                        cE += v * c[SWP_DE]; cW += v * c[SWP_DW]; cN += v * c[SWP_DN]; cS += v * c[SWP_DS];
+                    }
+                    if ((obst >> j) & 1) {
+                        a_rc += wcos;
+                        a_rs += wsin;
+                        rE += wcos * c[SWP_DE]; rW += wcos * c[SWP_DW];
+                        rN += wcos * c[SWP_DN]; rS += wcos * c[SWP_DS];
+                    }
                 }
-            }
-            if ((vegf >> j) & 1) {
-                a_vc += wcos;
-                a_vs += wsin;
-                if (FULL) {
-                    const float v = lw_amb * wcos;
-                    cE += v * c[SWP_DE]; cW += v * c[SWP_DW]; cN += v * c[SWP_DN]; cS += v * c[SWP_DS];
-                }
-            }
-            if ((bld >> j) & 1) {
-                const double x = tan_asvf + cd[SWP_YI];
-                const bool sunlit = x < cd[SWP_SUN_BELOW];
-                const bool shaded = x >= cd[SWP_SHADE_FROM];
-                if (sunlit) a_swsun += wcos;
-                if (shaded) a_swsh += wcos;
-                const bool elig = c[SWP_ELIGIBLE] != 0.0f;
-                const bool lsun = elig && sunlit, lsh = elig ? shaded : true;
-                if (lsun) { a_lsunc += wcos; a_lsuns += wsin; }
-                if (lsh) { a_lshc += wcos; a_lshs += wsin; }
-                if (FULL) {
-                    const float v = (lsun ? lw_sun * wcos : 0.f) + (lsh ? lw_amb * wcos : 0.f);
-                    cE += v * c[SWP_DE]; cW += v * c[SWP_DW]; cN += v * c[SWP_DN]; cS += v * c[SWP_DS];
-                }
-            }
-            if ((obst >> j) & 1) {
-                a_rc += wcos;
-                a_rs += wsin;
-                if (FULL) {
-                    rE += wcos * c[SWP_DE]; rW += wcos * c[SWP_DW];
-                    rN += wcos * c[SWP_DN]; rS += wcos * c[SWP_DS];
+                if ((bld >> j) & 1) {
+                    bool sunlit = false, shaded = true;
+                    if (day) {
+                        const double x = tan_asvf + cd[SWP_YI];
+                        sunlit = x < cd[SWP_SUN_BELOW];
+                        shaded = x >= cd[SWP_SHADE_FROM];
+                    }
+                    if (sunlit) a_swsun += wcos;
+                    if (shaded) a_swsh += wcos;
+                    const bool elig = c[SWP_ELIGIBLE] != 0.0f;
+                    const bool lsun = elig && sunlit, lsh = elig ? shaded : true;
+                    if (lsun) { a_lsunc += wcos; a_lsuns += wsin; }
+                    if (lsh) { a_lshc += wcos; a_lshs += wsin; }
+                    if (FULL) {
+                        const float v = (lsun ? lw_sun * wcos : 0.f) + (lsh ? lw_amb * wcos : 0.f);
+                        cE += v * c[SWP_DE]; cW += v * c[SWP_DW]; cN += v * c[SWP_DN]; cS += v * c[SWP_DS];
+                    }
                 }
             }
         }
+        a_lv = lv_sky + dveg * lv_veg;
+        a_kd = kd_sky + dveg * kd_veg;
     }
     const float dRad = a_lv * radD;
     Kdown += day ? dRad : 0.f;
@@ -797,6 +872,13 @@ int sw_svf_reduce(const uint32_t *bits, const float *dsm, const float *trees, do
                        (hipStream_t)stream, bits, dsm, trees, raise, P, sectors, band_offsets,
                        weights, bands, trans, out);
     return hip_check(hipGetLastError(), "k_svf_reduce launch");
+}
+
+int sw_sky_sums(const uint32_t *bits, long long P, const float *tab, float *out, void *stream) {
+    if (!bits || !tab || !out || P <= 0) return fail(SW_ERR_ARGUMENT, "sw_sky_sums: bad arguments");
+    hipLaunchKernelGGL(k_sky_sums, dim3((unsigned)((P + 255) / 256)), dim3(256), 0, (hipStream_t)stream, bits,
+                       P, tab, out);
+    return hip_check(hipGetLastError(), "k_sky_sums launch");
 }
 
 int sw_hour_shadow(const sw_domain *doms, const int32_t *bs, int nd, int total, int hour,

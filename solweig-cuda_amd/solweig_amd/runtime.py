@@ -27,10 +27,12 @@ _HOUR_FIELDS = ('DAY ALTITUDE AZIMUTH TANZEN TA RADG RADD RADI PSI SINALT COSALT
                 'LWALL FLUX4 LW_AMBIENT LW_SUNLIT LW_REFL SW_SUNLIT SW_SHADED TA32 UTCI0').split()
 H = {name: i for i, name in enumerate(_HOUR_FIELDS)}
 H['PATCH_TABLE'] = H['UTCI0'] + 7
-for _k, _name in enumerate('TGOUT_WEIGHT TGOUT_UPDATE ALBEDO_B ABSK ABSL FUP FSIDE FCYL DIFF_VEG TGWALL'.split()):
+for _k, _name in enumerate('TGOUT_WEIGHT TGOUT_UPDATE ALBEDO_B ABSK ABSL FUP FSIDE FCYL DIFF_VEG TGWALL '
+                           'SKY_A SKY_B'.split()):
     H[_name] = H['PATCH_TABLE'] + 1 + _k
 PF = {n: i for i, n in enumerate('LV KD LDN LSD WCOS WSIN DE DW DN DS ELIGIBLE'.split())}
 PD = {n: i for i, n in enumerate('YI SUN_BELOW SHADE_FROM'.split())}
+SKY_SUMS = 10
 FULL_PLANES = ('Kdown Kup Ldown Lup Tg Keast Ksouth Kwest Knorth Least Lsouth Lwest Lnorth KsideI '
                'TgOut Lside KsideD dRad Kside shadow wallsun F_sh').split()
 
@@ -42,7 +44,7 @@ class SwDomain(ctypes.Structure):
         [(n, ctypes.c_uint64) for n in ('dsm', 'trees', 'walls', 'aspect', 'emis', 'alb', 'tgk', 'tstart',
                                         'tmaxlst', 'svfbuveg', 'tan_asvf', 'tan_svfalfa', 'ground', 'vis',
                                         'rec', 'lag', 'tmrt', 'utci', 'hours', 'ptab_f', 'ptab_d',
-                                        'ray_off', 'ray_dz', 'masks', 'shadow_planes', 'full')]
+                                        'ray_off', 'ray_dz', 'masks', 'shadow_planes', 'full', 'skysums')]
 
 
 class SolweigHipError(RuntimeError):
@@ -79,6 +81,7 @@ def lib():
         'sw_hours_forward': ([vp, vp, i32, i32, i32, i32, vp, vp, i32, i32, vp], i32),
         'sw_utci_scalar': ([vp, i64, vp, f64, i32, vp, vp, vp], i32),
         'sw_copy': ([vp, vp, sz, vp], i32),
+        'sw_sky_sums': ([vp, i64, vp, vp, vp], i32),
         'sw_graph_begin': ([vp], i32), 'sw_graph_end': ([vp, ctypes.POINTER(vp)], i32),
         'sw_graph_launch': ([vp, vp], i32), 'sw_graph_destroy': ([vp], i32),
     }
@@ -94,7 +97,7 @@ EXPORTED = ('sw_version', 'sw_last_error', 'sw_device_count', 'sw_set_device', '
             'sw_memcpy_h2d', 'sw_memcpy_d2h', 'sw_memcpy_d2d', 'sw_memset', 'sw_stream_create',
             'sw_stream_destroy', 'sw_stream_sync', 'sw_event_create', 'sw_event_destroy',
             'sw_event_record', 'sw_event_elapsed_ms', 'sw_device_sync', 'sw_svf_visibility',
-            'sw_svf_reduce', 'sw_hour_shadow', 'sw_hour_pixels', 'sw_hours_forward', 'sw_utci_scalar',
+            'sw_svf_reduce', 'sw_sky_sums', 'sw_hour_shadow', 'sw_hour_pixels', 'sw_hours_forward', 'sw_utci_scalar',
             'sw_copy', 'sw_graph_begin', 'sw_graph_end', 'sw_graph_launch', 'sw_graph_destroy')
 
 

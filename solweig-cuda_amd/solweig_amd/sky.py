@@ -449,6 +449,8 @@ def hour_tables(alt, azi, zen, jday, dectime, altmax, Ta, RH, radG, P, Ws, doy, 
     h[H['ALBEDO_B']], h[H['ABSK']], h[H['ABSL']] = c['albedo_b'], c['absK'], c['absL']
     h[H['FUP']], h[H['FSIDE']], h[H['FCYL']] = c['Fup'], c['Fside'], c['Fcyl']
     h[H['DIFF_VEG']], h[H['TGWALL']] = diff_veg(c['trans']), tgwall
+    h[H['SKY_A']] = SBC * ((Ta + 273.15) ** 4) / np.pi
+    h[H['SKY_B']] = (1 - esky) * h[H['SKY_A']]
     pf, pd = patch_rows(alt > 0, esky, Ta, alt, azi, lv, radD)
     meta = dict(CI=CI, esky=esky, radI=radI, radD=radD, tgwall=tgwall, I0=I0, ea=ea, CI_Tg=citg,
                 CI_TgG=citgg, lv=lv, firstdaytime=firstdaytime, timeadd=timeadd)

@@ -10,7 +10,7 @@ from __future__ import annotations
 import numpy as np
 
 from . import runtime as rt
-from .sky import CONSTANTS
+from .sky import CONSTANTS, DEG, _geometry
 from .svf import sky_view
 
 LANDCOVER = {  # code: (albedo, emissivity, TgK, Tstart, TmaxLST) of landcoverclasses_2016a.txt
@@ -28,6 +28,23 @@ def normalise_landcover(values):
     r[bad] = 6
     r[(r == 3) | (r == 4)] = 5
     return r.astype(np.float32)
+
+
+def sky_sum_table():
+    """Per patch (w sin, w cos, g w sin, g w cos) of the day patch geometry, w the steradian and
+    g = exp(0.308 (1.7 - 1/cos(zenith))) the Martin-Berdahl band factor (umep_solweig.py:526-527)."""
+    alt, _, ster, wcos, wsin, _, _ = _geometry(True)
+    g = np.exp(0.308 * (1.7 - (1 / np.cos((90 - alt) * DEG))))
+    return np.ascontiguousarray(np.column_stack((wsin, wcos, g * wsin, g * wcos)), dtype=np.float32)
+
+
+def sky_sums(vis, P, stream=None):
+    """Static per-pixel longwave patch sums [SKY_SUMS, P] from the visibility bits."""
+    tab = rt.DeviceArray.from_host(sky_sum_table())
+    out = rt.DeviceArray((rt.SKY_SUMS, P), np.float32)
+    rt.call('sw_sky_sums', vis.ptr, int(P), tab.ptr, out.ptr, rt._s(stream))
+    rt.call('sw_stream_sync', rt._s(stream))
+    return out
 
 
 class PreparedDomain:
@@ -100,8 +117,9 @@ class PreparedDomain:
         self.dev['svfbuveg'] = up(svfbuveg)
         self.dev['tan_asvf'] = up(np.tan(asvf), np.float64)
         self.dev['tan_svfalfa'] = up(np.tan(svfalfa), np.float64)
-        self.dev['rec'] = rt.DeviceArray((self.P, 4), np.float32)
+        self.dev['rec'] = rt.DeviceArray((self.P, 2), np.float32)
         self.dev['lag'] = rt.DeviceArray((6, self.P), np.float32)
+        self.dev['skysums'] = sky_sums(self.dev['vis'], self.P, stream)
         if keep_host:
             self.host = dict(a=a, trees=trees, ground=ground, svf_planes=planes, svfbuveg=svfbuveg,
                              asvf=asvf, svfalfa=svfalfa, maps=maps)
@@ -117,7 +135,7 @@ class PreparedDomain:
                            ('emis', 'emis'), ('alb', 'alb'), ('tgk', 'tgk'), ('tstart', 'tstart'),
                            ('tmaxlst', 'tmax'), ('svfbuveg', 'svfbuveg'), ('tan_asvf', 'tan_asvf'),
                            ('tan_svfalfa', 'tan_svfalfa'), ('ground', 'ground'), ('vis', 'vis'),
-                           ('rec', 'rec'), ('lag', 'lag')):
+                           ('rec', 'rec'), ('lag', 'lag'), ('skysums', 'skysums')):
             setattr(s, field, d[key].ptr)
         for k, v in extra.items():
             setattr(s, k, v or 0)
@@ -167,8 +185,9 @@ def prepared_from_arrays(*, dsm, vegdem, walls, dirwalls, buildings, svfbuveg, a
                     tan_asvf=up(np.tan(np.asarray(asvf, dtype=np.float64)), np.float64),
                     tan_svfalfa=up(np.tan(np.asarray(svfalfa, dtype=np.float64)), np.float64),
                     vis=up(pack_visibility(shmat, vegshmat, vbshvegshmat), np.uint32),
-                    rec=rt.DeviceArray((rows * cols, 4), np.float32),
+                    rec=rt.DeviceArray((rows * cols, 2), np.float32),
                     lag=rt.DeviceArray((6, rows * cols), np.float32))
+    self.dev['skysums'] = sky_sums(self.dev['vis'], rows * cols, stream)
     self.skyview = None
     self.tiles = ((cols + rt.TILE_X - 1) // rt.TILE_X) * ((rows + rt.TILE_Y - 1) // rt.TILE_Y)
     return self

@@ -50,8 +50,8 @@ def test_hour_record_layout_matches_header():
     assert vals['SW_HOUR_STRIDE'] == rt.SW_HOUR_STRIDE
 
 
-def test_domain_struct_is_256_bytes():
-    assert ctypes.sizeof(rt.SwDomain) == 256
+def test_domain_struct_is_264_bytes():
+    assert ctypes.sizeof(rt.SwDomain) == 264
     fields = re.findall(r'\*(\w+)', header_text()[header_text().index('typedef struct sw_domain'):])
     names = [f[0] for f in rt.SwDomain._fields_][6:]
     assert fields[:len(names)] == names
