@@ -66,14 +66,71 @@ struct PixelAt {
     bool valid;
 };
 
-__device__ __forceinline__ PixelAt locate(const sw_domain *doms, const int32_t *bs, int nd,
-                                          int total, const sw_domain *&D) {
+// Address spaces: per-pixel planes are read and written through global (1) pointers; every
+// value that is the same for the whole launch (the domain record, hour record, ray, patch and
+// ground-view tables) is read through constant (4) pointers so it lands in scalar registers.
+#define AS1 __attribute__((address_space(1)))
+#define AS4 __attribute__((address_space(4)))
+typedef float v2f __attribute__((ext_vector_type(2)));
+
+struct DomainG {
+    int64_t rows, cols, pixels, landcover;
+    double dsm_raise;
+    const AS1 float *dsm, *trees, *walls, *aspect, *emis, *alb, *tgk, *tstart, *tmaxlst, *svfbuveg;
+    const AS1 double *tan_asvf, *tan_svfalfa;
+    const AS1 uint8_t *ground;
+    const AS1 uint32_t *vis;
+    AS1 v2f *rec;
+    AS1 float *lag, *tmrt, *utci;
+    const AS4 double *hours;
+    const AS4 float *ptab_f;
+    const AS4 double *ptab_d;
+    const AS4 int32_t *ray_off;
+    const AS4 double *ray_dz;
+    AS1 uint8_t *masks;
+    AS1 double *shadow_planes;
+    AS1 float *full;
+    const AS1 float *skysums;
+};
+
+__device__ __forceinline__ DomainG domain_at(const sw_domain *D) {
+    const AS4 sw_domain *d = (const AS4 sw_domain *)D;
+    DomainG g;
+    g.rows = d->rows; g.cols = d->cols; g.pixels = d->pixels; g.landcover = d->landcover;
+    g.dsm_raise = d->dsm_raise;
+    g.dsm = (const AS1 float *)d->dsm; g.trees = (const AS1 float *)d->trees;
+    g.walls = (const AS1 float *)d->walls; g.aspect = (const AS1 float *)d->aspect;
+    g.emis = (const AS1 float *)d->emis; g.alb = (const AS1 float *)d->alb;
+    g.tgk = (const AS1 float *)d->tgk; g.tstart = (const AS1 float *)d->tstart;
+    g.tmaxlst = (const AS1 float *)d->tmaxlst; g.svfbuveg = (const AS1 float *)d->svfbuveg;
+    g.tan_asvf = (const AS1 double *)d->tan_asvf; g.tan_svfalfa = (const AS1 double *)d->tan_svfalfa;
+    g.ground = (const AS1 uint8_t *)d->ground; g.vis = (const AS1 uint32_t *)d->vis;
+    g.rec = (AS1 v2f *)d->rec; g.lag = (AS1 float *)d->lag;
+    g.tmrt = (AS1 float *)d->tmrt; g.utci = (AS1 float *)d->utci;
+    g.hours = (const AS4 double *)d->hours;
+    g.ptab_f = (const AS4 float *)d->ptab_f; g.ptab_d = (const AS4 double *)d->ptab_d;
+    g.ray_off = (const AS4 int32_t *)d->ray_off; g.ray_dz = (const AS4 double *)d->ray_dz;
+    g.masks = (AS1 uint8_t *)d->masks; g.shadow_planes = (AS1 double *)d->shadow_planes;
+    g.full = (AS1 float *)d->full; g.skysums = (const AS1 float *)d->skysums;
+    return g;
+}
+
+constexpr int CHUNK = 8;   // march / walk steps whose loads are issued together
+
+__device__ __forceinline__ PixelAt locate(const sw_domain *doms, const int32_t *bs_, int nd,
+                                          int total, DomainG &G) {
+    const AS4 int32_t *bs = (const AS4 int32_t *)bs_;
     PixelAt at;
     const int b = xcd_remap((int)blockIdx.x, total);
-    at.d = find_domain(bs, nd, b);
-    D = doms + at.d;
+    int lo = 0, hi = nd - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (bs[mid] <= b) lo = mid; else hi = mid - 1;
+    }
+    at.d = lo;
+    G = domain_at(doms + at.d);
     const int t = b - bs[at.d];
-    const int cols = (int)D->cols, rows = (int)D->rows;
+    const int cols = (int)G.cols, rows = (int)G.rows;
     const int tpr = (cols + TILE_X - 1) / TILE_X;
     at.r = (t / tpr) * TILE_Y + (int)threadIdx.x / TILE_X;
     at.c = (t % tpr) * TILE_X + (int)threadIdx.x % TILE_X;
@@ -88,12 +145,24 @@ struct Heights {
     double a, top, bot;
 };
 
-__device__ __forceinline__ Heights heights_at(const float *dsm, const float *trees, double raise,
-                                              long long q) {
+template <typename PF>
+__device__ __forceinline__ Heights heights_at(PF dsm, PF trees, double raise, long long q) {
 #pragma clang fp contract(off)
     Heights h;
     h.a = (double)dsm[q] + raise;
     const double t = (double)trees[q];
+    const double v1 = t + h.a;
+    h.top = (v1 == h.a) ? 0.0 : v1;
+    const double v2 = t * 0.25 + h.a;
+    h.bot = (v2 == h.a) ? 0.0 : v2;
+    return h;
+}
+
+__device__ __forceinline__ Heights heights_of(float dsm, float tree, double raise) {
+#pragma clang fp contract(off)
+    Heights h;
+    h.a = (double)dsm + raise;
+    const double t = (double)tree;
     const double v1 = t + h.a;
     h.top = (v1 == h.a) ? 0.0 : v1;
     const double v2 = t * 0.25 + h.a;
@@ -216,45 +285,59 @@ __global__ __launch_bounds__(BLOCK) void k_hour_shadow(const sw_domain *__restri
                                                        const int32_t *__restrict__ bs, int nd,
                                                        int total, int hour) {
 #pragma clang fp contract(off)
-    const sw_domain *D;
-    const PixelAt at = locate(doms, bs, nd, total, D);
-    const double *H = D->hours + (long long)hour * SW_HOUR_STRIDE;
+    DomainG G;
+    const PixelAt at = locate(doms, bs, nd, total, G);
+    const AS4 double *H = G.hours + (long long)hour * SW_HOUR_STRIDE;
     if (H[SWH_DAY] == 0.0 || !at.valid) return;
-    const int rows = (int)D->rows, cols = (int)D->cols;
-    const long long P = D->pixels, p = at.p;
-    const float *dsm = D->dsm, *trees = D->trees;
-    const double raise = D->dsm_raise;
+    const int rows = (int)G.rows, cols = (int)G.cols;
+    const long long P = G.pixels, p = at.p;
+    const AS1 float *dsm = G.dsm, *trees = G.trees;
+    const double raise = G.dsm_raise;
     const Heights me = heights_at(dsm, trees, raise, p);
     const int start = (int)H[SWH_RAY_START], L = (int)H[SWH_RAY_LEN];
-    const int32_t *off = D->ray_off + 2LL * start;
-    const double *dzs = D->ray_dz + start;
+    const AS4 int32_t *off = G.ray_off + 2LL * start;
+    const AS4 double *dzs = G.ray_dz + start;
     double f = me.a, topmax = me.top, prev = 0.0;
     bool veg = (me.bot == 0.0) && me.top > 1.0;
     bool ever = false;
-    for (int k = 0; k < L; ++k) {
-        const int rr = at.r + off[2 * k], cc = at.c + off[2 * k + 1];
-        const double dz = dzs[k];
-        double A = 0.0, T = 0.0, B = 0.0, Tp = 0.0, Bp = 0.0;
-        if ((unsigned)rr < (unsigned)rows && (unsigned)cc < (unsigned)cols) {
-            const Heights q = heights_at(dsm, trees, raise, (long long)rr * cols + cc);
-            A = q.a - dz;
-            T = q.top - dz;
-            B = q.bot - dz;
-            Tp = q.top - prev;
-            Bp = q.bot - prev;
+    for (int k0 = 0; k0 < L; k0 += CHUNK) {
+        float av[CHUNK], tv[CHUNK];
+        bool in[CHUNK];
+#pragma unroll
+        for (int i = 0; i < CHUNK; ++i) {
+            const int k = k0 + i < L ? k0 + i : L - 1;
+            const int rr = at.r + off[2 * k], cc = at.c + off[2 * k + 1];
+            in[i] = (unsigned)rr < (unsigned)rows && (unsigned)cc < (unsigned)cols;
+            const long long q = in[i] ? (long long)rr * cols + cc : p;
+            av[i] = dsm[q];
+            tv[i] = trees[q];
         }
-        f = fmax(f, A);
-        topmax = fmax(topmax, T);
-        const int n = (int)(T > me.a) + (int)(B > me.a) + (int)(Tp > me.a) + (int)(Bp > me.a);
-        prev = dz;
-        veg = veg || (n > 0 && n < 4);
-        if (f > me.a) veg = false;
-        ever = ever || veg;
+#pragma unroll
+        for (int i = 0; i < CHUNK; ++i) {
+            if (k0 + i >= L) break;
+            const double dz = dzs[k0 + i];
+            double A = 0.0, T = 0.0, B = 0.0, Tp = 0.0, Bp = 0.0;
+            if (in[i]) {
+                const Heights q = heights_of(av[i], tv[i], raise);
+                A = q.a - dz;
+                T = q.top - dz;
+                B = q.bot - dz;
+                Tp = q.top - prev;
+                Bp = q.bot - prev;
+            }
+            f = fmax(f, A);
+            topmax = fmax(topmax, T);
+            const int n = (int)(T > me.a) + (int)(B > me.a) + (int)(Tp > me.a) + (int)(Bp > me.a);
+            prev = dz;
+            veg = veg || (n > 0 && n < 4);
+            if (f > me.a) veg = false;
+            ever = ever || veg;
+        }
     }
     const bool blocked = f > me.a;
-    const double wall = (double)D->walls[p];
+    const double wall = (double)G.walls[p];
     const double wallbol = wall > 0.0 ? 1.0 : 0.0;
-    const double aspect = ((double)D->aspect[p] * 3.141592653589793) / 180.0;
+    const double aspect = ((double)G.aspect[p] * 3.141592653589793) / 180.0;
     const double low = H[SWH_FACE_LOW], high = H[SWH_FACE_HIGH];
     double facesh;
     if (H[SWH_FACE_MIDDLE] != 0.0)
@@ -275,28 +358,28 @@ __global__ __launch_bounds__(BLOCK) void k_hour_shadow(const sw_domain *__restri
     const double sh = blocked ? 0.0 : 1.0;
     const double vegsh = 1.0 - vegv;
     const double shadow = sh - (1.0 - vegsh) * (1.0 - H[SWH_PSI]);
-    const bool ground = D->ground[p] != 0;
+    const bool ground = G.ground[p] != 0;
     const bool sunwall = (wallsun / wall * (ground ? 1.0 : 0.0)) == 1.0;
     // Surface temperature of the hour (umep_solweig.py:144-170) and the sunonsurface Lup field.
     const double snup = H[SWH_SNUP];
-    const double amp = (double)D->tgk[p] * H[SWH_ALTMAX] + (double)D->tstart[p];
-    double tg = amp * sin(((H[SWH_FRAC] - snup / 24.0) / ((double)D->tmaxlst[p] / 24.0 - snup / 24.0)) *
+    const double amp = (double)G.tgk[p] * H[SWH_ALTMAX] + (double)G.tstart[p];
+    double tg = amp * sin(((H[SWH_FRAC] - snup / 24.0) / ((double)G.tmaxlst[p] / 24.0 - snup / 24.0)) *
                           3.141592653589793 / 2.0);
     tg = tg * H[SWH_CITGG];
-    if (D->landcover == 1 && tg < 0.0) tg = 0.0;
-    const double ta = H[SWH_TA], emis = (double)D->emis[p];
+    if (G.landcover == 1 && tg < 0.0) tg = 0.0;
+    const double ta = H[SWH_TA], emis = (double)G.emis[p];
     const double tk = tg * shadow + ta + 273.15, ta4 = (ta + 273.15);
     const double lup = SBC * emis * (tk * tk) * (tk * tk) - SBC * emis * (ta4 * ta4) * (ta4 * ta4);
     const uint32_t cls = blocked ? 0u : (veg ? 1u : 2u);
-    float2 rec;
+    v2f rec;
     rec.x = (float)lup;
-    rec.y = pack_albedo(D->alb[p], (ground ? 1u : 0u) | (sunwall ? 2u : 0u) | (cls << 2));
-    reinterpret_cast<float2 *>(D->rec)[p] = rec;
-    if (D->masks)
-        D->masks[p] = (uint8_t)((blocked ? 0 : 1) | (veg ? 0 : 2) | ((ever && !veg) ? 0 : 4) |
+    rec.y = pack_albedo(G.alb[p], (ground ? 1u : 0u) | (sunwall ? 2u : 0u) | (cls << 2));
+    G.rec[p] = rec;
+    if (G.masks)
+        G.masks[p] = (uint8_t)((blocked ? 0 : 1) | (veg ? 0 : 2) | ((ever && !veg) ? 0 : 4) |
                                 (sunwall ? 8 : 0) | (facesh == 1.0 ? 16 : 0));
-    if (FULL && D->shadow_planes) {
-        double *o = D->shadow_planes;
+    if (FULL && G.shadow_planes) {
+        AS1 double *o = G.shadow_planes;
         const double facesun = ((facesh + wallbol == 1.0) && wall > 0.0) ? 1.0 : 0.0;
         o[p] = vegsh;
         o[P + p] = sh;
@@ -360,7 +443,7 @@ struct Sample {
     bool ground, sunwall;
 };
 
-__device__ __forceinline__ Sample decode(float2 q, float shv_canopy) {
+__device__ __forceinline__ Sample decode(v2f q, float shv_canopy) {
     Sample s;
     const uint32_t u = __float_as_uint(q.y);
     s.lup = q.x;
@@ -377,24 +460,24 @@ __device__ __forceinline__ Sample decode(float2 q, float shv_canopy) {
 // sums, keeps the sunlit-wall flag and sets the any-wall flag, so the rest of the walk is added
 // in closed form. Wall sums are counts times the per-hour wall flux / wall albedo.
 template <bool CHECK>
-__device__ __forceinline__ GvfOut ground_view(const sw_domain *D, const PixelAt &at, const Sample me,
-                                              const double *H, const int32_t *__restrict__ gofs,
-                                              const double *__restrict__ gbounds, int first, int second,
+__device__ __forceinline__ GvfOut ground_view(const DomainG &G, const PixelAt &at, const Sample me,
+                                              const AS4 double *H, const AS4 int32_t *gofs,
+                                              const AS4 double *gbounds, int first, int second,
                                               float shv_canopy) {
-    const int rows = (int)D->rows, cols = (int)D->cols;
-    const float2 *rec = reinterpret_cast<const float2 *>(D->rec);
+    const int rows = (int)G.rows, cols = (int)G.cols;
+    const AS1 v2f *rec = G.rec;
     const float lwall = (float)H[SWH_LWALL];
     const float albedo_b = (float)H[SWH_ALBEDO_B];
     const float building = me.ground ? 1.0f : 0.0f;
-    const bool wallbol = D->walls[at.p] > 0.0f;
-    const double aspect = ((double)D->aspect[at.p] * 3.141592653589793) / 180.0;
+    const bool wallbol = G.walls[at.p] > 0.0f;
+    const double aspect = ((double)G.aspect[at.p] * 3.141592653589793) / 180.0;
     const float firstf = (float)first, secondf = (float)second;
     const float local_lup = me.lup * (1.0f - building);
     const float local_albsh = me.alb * (1.0f - building) * me.shv;
     const float local_nosh = me.alb * (1.0f - building);
     GvfOut o = {};
     for (int dir = 0; dir < 18; ++dir) {
-        const int32_t *off = gofs + dir * second * 2;
+        const AS4 int32_t *off = gofs + dir * second * 2;
         float s_lup = 0.f, s_albsh = 0.f, s_alb = 0.f;
         float f_lup = 0.f, f_albsh = 0.f, f_alb = 0.f;
         int cnt_wall = 0, cnt_any = 0, f_cnt_wall = 0, f_cnt_any = 0;
@@ -403,44 +486,61 @@ __device__ __forceinline__ GvfOut ground_view(const sw_domain *D, const PixelAt 
             f_cnt_any = first;
         } else {
             Sample t = me;
-            bool wall_seen = false;
-            for (int st = 0; st < second; ++st) {
-                const int dr = off[2 * st], dc = off[2 * st + 1];
-                if (!CHECK) {
-                    t = decode(rec[at.p + (long long)dr * cols + dc], shv_canopy);
-                } else {
-                    const int rr = at.r + dr, cc = at.c + dc;
-                    if ((unsigned)rr < (unsigned)rows && (unsigned)cc < (unsigned)cols)
-                        t = decode(rec[(long long)rr * cols + cc], shv_canopy);
+            bool wall_seen = false, alive = true;
+            for (int s0 = 0; s0 < second; s0 += CHUNK) {
+                if (__ballot(alive) == 0) break;
+                v2f buf[CHUNK];
+                bool in[CHUNK];
+#pragma unroll
+                for (int i = 0; i < CHUNK; ++i) {
+                    const int st = s0 + i < second ? s0 + i : second - 1;
+                    const int dr = off[2 * st], dc = off[2 * st + 1];
+                    long long q;
+                    if (!CHECK) {
+                        in[i] = true;
+                        q = at.p + (long long)dr * cols + dc;
+                    } else {
+                        const int rr = at.r + dr, cc = at.c + dc;
+                        in[i] = (unsigned)rr < (unsigned)rows && (unsigned)cc < (unsigned)cols;
+                        q = in[i] ? (long long)rr * cols + cc : at.p;
+                    }
+                    if (alive) buf[i] = rec[q];
                 }
-                if (t.ground) {
-                    s_lup += t.lup;
-                    s_albsh += t.alb * t.shv;
-                    s_alb += t.alb;
-                    wall_seen = wall_seen || t.sunwall;
-                    cnt_wall += wall_seen ? 1 : 0;
-                    if (st + 1 <= first) {
-                        f_lup = s_lup; f_albsh = s_albsh; f_alb = s_alb;
-                        f_cnt_wall = cnt_wall; f_cnt_any = cnt_any;
+#pragma unroll
+                for (int i = 0; i < CHUNK; ++i) {
+                    const int st = s0 + i;
+                    if (st >= second) break;
+                    if (!alive) continue;
+                    if (in[i]) t = decode(buf[i], shv_canopy);
+                    if (t.ground) {
+                        s_lup += t.lup;
+                        s_albsh += t.alb * t.shv;
+                        s_alb += t.alb;
+                        wall_seen = wall_seen || t.sunwall;
+                        cnt_wall += wall_seen ? 1 : 0;
+                        if (st + 1 <= first) {
+                            f_lup = s_lup; f_albsh = s_albsh; f_alb = s_alb;
+                            f_cnt_wall = cnt_wall; f_cnt_any = cnt_any;
+                        }
+                    } else {
+                        const int rest = second - st;     // this step and the remaining ones
+                        const int w = wall_seen ? 1 : 0;
+                        if (st + 1 <= first) {
+                            const int upto = first - st;
+                            f_lup = s_lup; f_albsh = s_albsh; f_alb = s_alb;
+                            f_cnt_wall = cnt_wall + w * upto;
+                            f_cnt_any = cnt_any + upto;
+                        }
+                        cnt_wall += w * rest;
+                        cnt_any += rest;
+                        alive = false;
                     }
-                } else {
-                    const int rest = second - st;     // this step and the remaining ones
-                    const int w = wall_seen ? 1 : 0;
-                    if (st + 1 <= first) {
-                        const int upto = first - st;
-                        f_lup = s_lup; f_albsh = s_albsh; f_alb = s_alb;
-                        f_cnt_wall = cnt_wall + w * upto;
-                        f_cnt_any = cnt_any + upto;
-                    }
-                    cnt_wall += w * rest;
-                    cnt_any += rest;
-                    break;
                 }
             }
         }
         const bool sunin2 = cnt_wall > 0, f_sunin = f_cnt_wall > 0;
         const bool wallin2 = cnt_any > 0, f_wallin = f_cnt_any > 0;
-        const double *bd = gbounds + dir * 3;
+        const AS4 double *bd = gbounds + dir * 3;
         int facesh;
         if (bd[2] == 0.0)
             facesh = (int)(aspect < bd[0] || aspect >= bd[1]) - (wallbol ? 1 : 0) + 1;
@@ -501,16 +601,18 @@ template <bool FULL>
 __global__ __launch_bounds__(BLOCK) void k_hour_pixels(const sw_domain *__restrict__ doms,
                                                        const int32_t *__restrict__ bs, int nd,
                                                        int total, int hour,
-                                                       const int32_t *__restrict__ gofs,
-                                                       const double *__restrict__ gbounds,
+                                                       const int32_t *__restrict__ gofs_,
+                                                       const double *__restrict__ gbounds_,
                                                        int first, int second) {
-    const sw_domain *D;
-    const PixelAt at = locate(doms, bs, nd, total, D);
+    const AS4 int32_t *gofs = (const AS4 int32_t *)gofs_;
+    const AS4 double *gbounds = (const AS4 double *)gbounds_;
+    DomainG G;
+    const PixelAt at = locate(doms, bs, nd, total, G);
     if (!at.valid) return;
-    const double *H = D->hours + (long long)hour * SW_HOUR_STRIDE;
-    const long long P = D->pixels, p = at.p;
+    const AS4 double *H = G.hours + (long long)hour * SW_HOUR_STRIDE;
+    const long long P = G.pixels, p = at.p;
     const bool day = H[SWH_DAY] != 0.0;
-    const float emis = D->emis[p];
+    const float emis = G.emis[p];
     const float radI = (float)H[SWH_RADI], radD = (float)H[SWH_RADD], radG = (float)H[SWH_RADG];
     const float albedo_b = (float)H[SWH_ALBEDO_B];
     const float flux4 = (float)H[SWH_FLUX4];
@@ -518,16 +620,16 @@ __global__ __launch_bounds__(BLOCK) void k_hour_pixels(const sw_domain *__restri
     float KsideI = 0.f, Lup, LupE, LupS, LupW, LupN, shadow = 0.f, Fsh = 0.f, tgout = 0.f, tg = 0.f;
     if (day) {
         const float shv_canopy = (float)(1.0 - (1.0 - H[SWH_PSI]));
-        const Sample me = decode(reinterpret_cast<const float2 *>(D->rec)[p], shv_canopy);
+        const Sample me = decode(G.rec[p], shv_canopy);
         shadow = me.shv;
         const int m = second - 1;
-        const bool interior = at.r >= m && at.c >= m && at.r < (int)D->rows - m && at.c < (int)D->cols - m;
-        const GvfOut g = interior ? ground_view<false>(D, at, me, H, gofs, gbounds, first, second, shv_canopy)
-                                  : ground_view<true>(D, at, me, H, gofs, gbounds, first, second, shv_canopy);
+        const bool interior = at.r >= m && at.c >= m && at.r < (int)G.rows - m && at.c < (int)G.cols - m;
+        const GvfOut g = interior ? ground_view<false>(G, at, me, H, gofs, gbounds, first, second, shv_canopy)
+                                  : ground_view<true>(G, at, me, H, gofs, gbounds, first, second, shv_canopy);
         const float amb = flux4 * emis;
         const double w = H[SWH_LAG_WEIGHT];
         const bool lf = H[SWH_LAG_FIRST] != 0.0, lu = H[SWH_LAG_UPDATE] != 0.0;
-        float *lg = D->lag;
+        AS1 float *lg = G.lag;
         float s0 = lg[p], s1 = lg[P + p], s2 = lg[2 * P + p], s3 = lg[3 * P + p], s4 = lg[4 * P + p];
         Lup = lag(g.all.lup / 18.0f + amb, s0, w, lf, lu);
         LupE = lag(g.e.lup / 9.0f + amb, s1, w, lf, lu);
@@ -537,17 +639,17 @@ __global__ __launch_bounds__(BLOCK) void k_hour_pixels(const sw_domain *__restri
         lg[p] = s0; lg[P + p] = s1; lg[2 * P + p] = s2; lg[3 * P + p] = s3; lg[4 * P + p] = s4;
         if (FULL) {
             const double snup = H[SWH_SNUP];
-            const double amp = (double)D->tgk[p] * H[SWH_ALTMAX] + (double)D->tstart[p];
-            double t = amp * sin(((H[SWH_FRAC] - snup / 24.0) / ((double)D->tmaxlst[p] / 24.0 - snup / 24.0)) *
+            const double amp = (double)G.tgk[p] * H[SWH_ALTMAX] + (double)G.tstart[p];
+            double t = amp * sin(((H[SWH_FRAC] - snup / 24.0) / ((double)G.tmaxlst[p] / 24.0 - snup / 24.0)) *
                                  3.141592653589793 / 2.0) * H[SWH_CITGG];
-            if (D->landcover == 1 && t < 0.0) t = 0.0;
+            if (G.landcover == 1 && t < 0.0) t = 0.0;
             tg = (float)t;
             float s5 = lg[5 * P + p];
             tgout = lag((float)(t * (double)shadow + H[SWH_TA]), s5, w, lf, lu);
             lg[5 * P + p] = s5;
         }
-        Fsh = wedge(D->tan_svfalfa[p], H[SWH_TANZEN]);
-        const float svfbuveg = D->svfbuveg[p];
+        Fsh = wedge(G.tan_svfalfa[p], H[SWH_TANZEN]);
+        const float svfbuveg = G.svfbuveg[p];
         const float sinalt = (float)H[SWH_SINALT];
         const float around = albedo_b * (1.0f - svfbuveg) * (radG * (1.0f - Fsh) + radD * Fsh);
         const float refl = radD * svfbuveg + around;
@@ -565,9 +667,9 @@ __global__ __launch_bounds__(BLOCK) void k_hour_pixels(const sw_domain *__restri
     }
     // ---- anisotropic patch sums over the 153 sky patches --------------------------------
     const int row = (int)H[SWH_PATCH_TABLE];
-    const float *pf = D->ptab_f + (long long)row * SW_PATCHES * SW_PATCH_F;
-    const double *pd = D->ptab_d + (long long)row * SW_PATCHES * SW_PATCH_D;
-    const double tan_asvf = D->tan_asvf[p];
+    const AS4 float *pf = G.ptab_f + (long long)row * SW_PATCHES * SW_PATCH_F;
+    const AS4 double *pd = G.ptab_d + (long long)row * SW_PATCHES * SW_PATCH_D;
+    const double tan_asvf = G.tan_asvf[p];
     const float dveg = (float)H[SWH_DIFF_VEG];
     float a_lv = 0.f, a_kd = 0.f, a_ldn = 0.f, a_lsd = 0.f, a_vc = 0.f, a_vs = 0.f;
     float a_swsun = 0.f, a_swsh = 0.f, a_lsunc = 0.f, a_lsuns = 0.f, a_lshc = 0.f, a_lshs = 0.f;
@@ -575,10 +677,10 @@ __global__ __launch_bounds__(BLOCK) void k_hour_pixels(const sw_domain *__restri
     float cE = 0.f, cW = 0.f, cN = 0.f, cS = 0.f;       // FULL: cardinal sky+veg+walls
     float rE = 0.f, rW = 0.f, rN = 0.f, rS = 0.f;       // FULL: cardinal reflected weights
     const float lw_amb = (float)H[SWH_LW_AMBIENT], lw_sun = (float)H[SWH_LW_SUNLIT];
-    const uint32_t *vis = D->vis;
+    const AS1 uint32_t *vis = G.vis;
     if (!FULL) {
         // hour-independent parts from the static sums; sky flux linear in the sky emissivity
-        const float *ss = D->skysums;
+        const AS1 float *ss = G.skysums;
         const float A = (float)H[SWH_SKY_A], B = (float)H[SWH_SKY_B];
         a_ldn = A * ss[SWS_SKY_SIN * P + p] - B * ss[SWS_SKY_GSIN * P + p];
         a_lsd = A * ss[SWS_SKY_COS * P + p] - B * ss[SWS_SKY_GCOS * P + p];
@@ -602,8 +704,8 @@ __global__ __launch_bounds__(BLOCK) void k_hour_pixels(const sw_domain *__restri
             const int k0 = w * 32, k1 = min(SW_PATCHES, k0 + 32);
             for (int k = k0; k < k1; ++k) {
                 const int j = k - k0;
-                const float *c = pf + k * SW_PATCH_F;
-                const double *cd = pd + k * SW_PATCH_D;
+                const AS4 float *c = pf + k * SW_PATCH_F;
+                const AS4 double *cd = pd + k * SW_PATCH_D;
                 const float wcos = c[SWP_WCOS], wsin = c[SWP_WSIN];
                 if ((sky >> j) & 1) { lv_sky += c[SWP_LV]; kd_sky += c[SWP_KD]; }
                 if ((vegonly >> j) & 1) { lv_veg += c[SWP_LV]; kd_veg += c[SWP_KD]; }
@@ -667,7 +769,7 @@ __global__ __launch_bounds__(BLOCK) void k_hour_pixels(const sw_domain *__restri
     const float Tmrt = sqrtf(sqrtf(Sstr / (absL * (float)SBC))) - 273.2f;
     // UTCI with the hour's scalar weather (utci.cu:500-523), NaN off the ground mask.
     float u;
-    if (!D->ground[p]) {
+    if (!G.ground[p]) {
         u = __builtin_nanf("");
     } else if (H[SWH_UTCI0 + 6] == -999.0 || Tmrt <= -999.0f) {
         u = -999.0f;
@@ -677,10 +779,10 @@ __global__ __launch_bounds__(BLOCK) void k_hour_pixels(const sw_domain *__restri
         for (int k = 5; k >= 0; --k) acc = acc * dT + H[SWH_UTCI0 + k];
         u = (float)acc;
     }
-    if (D->tmrt) D->tmrt[(long long)hour * P + p] = Tmrt;
-    if (D->utci) D->utci[(long long)hour * P + p] = u;
-    if (FULL && D->full) {
-        float *o = D->full;
+    if (G.tmrt) G.tmrt[(long long)hour * P + p] = Tmrt;
+    if (G.utci) G.utci[(long long)hour * P + p] = u;
+    if (FULL && G.full) {
+        AS1 float *o = G.full;
         const float refl = reflsurf;
         o[SWF_KDOWN * P + p] = Kdown;
         o[SWF_KUP * P + p] = Kup;
