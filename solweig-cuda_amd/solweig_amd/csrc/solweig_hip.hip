@@ -504,7 +504,9 @@ __device__ __forceinline__ GvfOut ground_view(const DomainG &G, const PixelAt &a
                         in[i] = (unsigned)rr < (unsigned)rows && (unsigned)cc < (unsigned)cols;
                         q = in[i] ? (long long)rr * cols + cc : at.p;
                     }
-                    if (alive) buf[i] = rec[q];
+                    // unconditional so the gathers issue back to back; finished lanes re-read
+                    // their own record
+                    buf[i] = rec[alive ? q : at.p];
                 }
 #pragma unroll
                 for (int i = 0; i < CHUNK; ++i) {
