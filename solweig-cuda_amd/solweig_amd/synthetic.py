@@ -245,7 +245,7 @@ def make_c4(seed=4, count=64, side=(256, 1536), max_side=None):
 
 
 def small_domain(seed=7, size=48, trees=True, boxes=5, height=(6.0, 20.0), hours=None,
-                 landcover=False, maps=False):
+                 landcover=False, maps=False, latitude=DUBLIN[0]):
     """A small domain for fast parity tests (CPU oracle finishes in seconds)."""
     rng = np.random.default_rng(seed)
     dsm = boxes_fixed(rng, size, size, count=boxes, side=(4, max(5, size // 5)), height=height)
@@ -253,11 +253,11 @@ def small_domain(seed=7, size=48, trees=True, boxes=5, height=(6.0, 20.0), hours
     if trees:
         canopy = canopy_to_cover(rng, dsm > 0, cover=0.12, sigma=(1.5, 3.0), spread=4.0)
     dsm, tr, dem, walls, aspect = _finish_morphology(dsm, canopy)
-    met = day_met(datetime.date(2025, 6, 21), 1, *DUBLIN, 0.0,
+    met = day_met(datetime.date(2025, 6, 21), 1, latitude, DUBLIN[1], 0.0,
                   lambda d: np.random.default_rng(seed * 100 + d), cloudy=True)
     if hours is not None:
         met = met[list(hours)]
-    d = Domain(dsm, tr, dem, walls, aspect, met, DUBLIN[0], DUBLIN[1], 0.0, name=f"small_{size}")
+    d = Domain(dsm, tr, dem, walls, aspect, met, latitude, DUBLIN[1], 0.0, name=f"small_{size}")
     if landcover:
         d.landcover = np.ones_like(dsm)
     if maps:
