@@ -75,7 +75,7 @@ typedef struct sw_domain {
     const double *tan_asvf, *tan_svfalfa;                    /* [P]                           */
     const uint8_t *ground;                                   /* [P] 1 where dsm-dem < 2        */
     const uint32_t *vis;                                     /* [3][5][P] sh, veg, vb bits     */
-    void *rec;                                               /* [P] float4 hour scratch         */
+    void *rec;                                               /* [P] 16-byte hour records        */
     float *lag;                                              /* [6][P] Tgmap1, E, S, W, N, TgOut1 */
     float *tmrt, *utci;                                      /* [n_hours][P] (may be NULL)     */
     const double *hours;                                     /* [n_hours][SW_HOUR_STRIDE]     */
@@ -148,7 +148,9 @@ int sw_hour_shadow(const sw_domain *domains_dev, const int32_t *block_start_dev,
 /* Ground view factors, lag recursion, patch sums, Kdown/Kup/Kside/Ldown/Lside, Tmrt and UTCI
  * for hour ``hour`` (replaces Solweig_2022a_calc's per-pixel stages, model.py:141-251, with
  * gvf.cu, shortwave.cu, longwave.cu, flux.cu and utci.cu). full != 0 writes the FULL planes.
- * gvf_offsets_dev [18][second][2], gvf_bounds_dev [18][3] as prepare_gvf_geometry builds them. */
+ * gvf_offsets_dev [18][S][2] (row, col) step shifts with S = second rounded up to a multiple of
+ * 8, entries past ``second`` repeating the last step; gvf_bounds_dev [18][3] facesh bounds, as
+ * prepare_gvf_geometry builds them (model.py:533-566). The neighbour record rec is [P] x 16 B. */
 int sw_hour_pixels(const sw_domain *domains_dev, const int32_t *block_start_dev, int n_domains,
                    int total_blocks, int hour, const int32_t *gvf_offsets_dev,
                    const double *gvf_bounds_dev, int first_steps, int second_steps, int full,

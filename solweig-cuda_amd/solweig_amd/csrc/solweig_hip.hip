@@ -72,6 +72,7 @@ struct PixelAt {
 #define AS1 __attribute__((address_space(1)))
 #define AS4 __attribute__((address_space(4)))
 typedef float v2f __attribute__((ext_vector_type(2)));
+typedef float v4f __attribute__((ext_vector_type(4)));
 
 struct DomainG {
     int64_t rows, cols, pixels, landcover;
@@ -80,7 +81,7 @@ struct DomainG {
     const AS1 double *tan_asvf, *tan_svfalfa;
     const AS1 uint8_t *ground;
     const AS1 uint32_t *vis;
-    AS1 v2f *rec;
+    AS1 v4f *rec;
     AS1 float *lag, *tmrt, *utci;
     const AS4 double *hours;
     const AS4 float *ptab_f;
@@ -105,7 +106,7 @@ __device__ __forceinline__ DomainG domain_at(const sw_domain *D) {
     g.tmaxlst = (const AS1 float *)d->tmaxlst; g.svfbuveg = (const AS1 float *)d->svfbuveg;
     g.tan_asvf = (const AS1 double *)d->tan_asvf; g.tan_svfalfa = (const AS1 double *)d->tan_svfalfa;
     g.ground = (const AS1 uint8_t *)d->ground; g.vis = (const AS1 uint32_t *)d->vis;
-    g.rec = (AS1 v2f *)d->rec; g.lag = (AS1 float *)d->lag;
+    g.rec = (AS1 v4f *)d->rec; g.lag = (AS1 float *)d->lag;
     g.tmrt = (AS1 float *)d->tmrt; g.utci = (AS1 float *)d->utci;
     g.hours = (const AS4 double *)d->hours;
     g.ptab_f = (const AS4 float *)d->ptab_f; g.ptab_d = (const AS4 double *)d->ptab_d;
@@ -271,13 +272,13 @@ __global__ __launch_bounds__(256) void k_svf_reduce(
 
 // =====================================================================================
 // Per-hour sun-ray shadow march (shadowingfunction_wallheight_23, umep_solweig.py:2201-2406;
-// geometry.cu:185-253) and the 8-byte record the ground-view walk gathers:
-//   rec.x = Lup field of sunonsurface (umep_solweig.py:1418), rec.y = albedo with its four low
-//   mantissa bits replaced by flags: bit 0 ground, bit 1 sunlit wall, bits 2-3 shadow class
-//   (0: building shadow, 1: canopy shadow, 2: sunlit). The albedo loses < 2^-19 relatively.
+// geometry.cu:185-253) and the 16-byte record the ground-view walk gathers:
+//   rec.x = Lup field of sunonsurface (umep_solweig.py:1418), rec.y = albedo, rec.z = albedo x
+//   shadow, rec.w = flag word: byte 0 ground (0/1), byte 1 sunlit wall (0/1), byte 2 shadow
+//   class (0: building shadow, 1: canopy shadow, 2: sunlit).
 // =====================================================================================
-__device__ __forceinline__ float pack_albedo(float alb, uint32_t flags) {
-    return __uint_as_float((__float_as_uint(alb) & ~15u) | flags);
+__device__ __forceinline__ float shadow_of_class(uint32_t cls, float shv_canopy) {
+    return cls == 2u ? 1.0f : (cls == 1u ? shv_canopy : 0.0f);
 }
 
 template <bool FULL>
@@ -371,9 +372,12 @@ __global__ __launch_bounds__(BLOCK) void k_hour_shadow(const sw_domain *__restri
     const double tk = tg * shadow + ta + 273.15, ta4 = (ta + 273.15);
     const double lup = SBC * emis * (tk * tk) * (tk * tk) - SBC * emis * (ta4 * ta4) * (ta4 * ta4);
     const uint32_t cls = blocked ? 0u : (veg ? 1u : 2u);
-    v2f rec;
+    const float alb = G.alb[p];
+    v4f rec;
     rec.x = (float)lup;
-    rec.y = pack_albedo(G.alb[p], (ground ? 1u : 0u) | (sunwall ? 2u : 0u) | (cls << 2));
+    rec.y = alb;
+    rec.z = alb * shadow_of_class(cls, (float)(1.0 - (1.0 - H[SWH_PSI])));
+    rec.w = __uint_as_float((ground ? 1u : 0u) | (sunwall ? 0x100u : 0u) | (cls << 16));
     G.rec[p] = rec;
     if (G.masks)
         G.masks[p] = (uint8_t)((blocked ? 0 : 1) | (veg ? 0 : 2) | ((ever && !veg) ? 0 : 4) |
@@ -438,129 +442,101 @@ struct GvfOut {
     Triple all, e, s, w, n;
 };
 
-struct Sample {
-    float lup, alb, shv;
-    bool ground, sunwall;
+// Running state of one direction of the walk. ``af`` is 1 while the running minimum of the
+// ground mask is 1 (umep_solweig.py:1498) and 0 from the first non-ground sample on; every
+// term is multiplied by it, so the walk needs no per-lane branches. ``L`` counts the ground
+// steps, ``wf`` is 1 once a sunlit wall was passed, ``cw`` counts the ground steps from there.
+struct Walk {
+    float af, lup, alb, albsh, L, wf, cw;
 };
 
-__device__ __forceinline__ Sample decode(v2f q, float shv_canopy) {
-    Sample s;
-    const uint32_t u = __float_as_uint(q.y);
-    s.lup = q.x;
-    s.alb = __uint_as_float(u & ~15u);
-    s.ground = (u & 1u) != 0;
-    s.sunwall = (u & 2u) != 0;
-    const uint32_t cls = (u >> 2) & 3u;
-    s.shv = cls == 2u ? 1.0f : (cls == 1u ? shv_canopy : 0.0f);
-    return s;
-}
-
-// One pixel's walk over the 18 directions x ``second`` steps. The running minimum f of the
-// ground mask only falls from 1 to 0; once it is 0 every later step adds nothing to the ground
-// sums, keeps the sunlit-wall flag and sets the any-wall flag, so the rest of the walk is added
-// in closed form. Wall sums are counts times the per-hour wall flux / wall albedo.
+// One pixel's walk over the 18 directions x ``second`` steps (sunonsurface_2018a). After the
+// walk, the steps from the first non-ground sample to the end are added in closed form: each of
+// them adds one "any wall" count and, if a sunlit wall was passed, one sunlit-wall count
+// (gvf.cu:209-239 sums the same terms step by step). The snapshot after ``first`` steps gives
+// the near-field terms. On the domain edge a step whose shift leaves the raster repeats the
+// previous sample, as the shifted-window assignment of the reference leaves it.
 template <bool CHECK>
-__device__ __forceinline__ GvfOut ground_view(const DomainG &G, const PixelAt &at, const Sample me,
+__device__ __forceinline__ GvfOut ground_view(const DomainG &G, const PixelAt &at, const v4f me,
                                               const AS4 double *H, const AS4 int32_t *gofs,
-                                              const AS4 double *gbounds, int first, int second,
-                                              float shv_canopy) {
+                                              const AS4 double *gbounds, int first, int second) {
     const int rows = (int)G.rows, cols = (int)G.cols;
-    const AS1 v2f *rec = G.rec;
+    const AS1 v4f *rec = G.rec;
     const float lwall = (float)H[SWH_LWALL];
     const float albedo_b = (float)H[SWH_ALBEDO_B];
-    const float building = me.ground ? 1.0f : 0.0f;
+    const float building = (float)(__float_as_uint(me.w) & 0xffu);
     const bool wallbol = G.walls[at.p] > 0.0f;
     const double aspect = ((double)G.aspect[at.p] * 3.141592653589793) / 180.0;
     const float firstf = (float)first, secondf = (float)second;
-    const float local_lup = me.lup * (1.0f - building);
-    const float local_albsh = me.alb * (1.0f - building) * me.shv;
-    const float local_nosh = me.alb * (1.0f - building);
+    const int stride = (second + CHUNK - 1) / CHUNK * CHUNK;   // padded step axis of gofs
+    const float local_lup = me.x * (1.0f - building);
+    const float local_albsh = me.z * (1.0f - building);
+    const float local_nosh = me.y * (1.0f - building);
     GvfOut o = {};
     for (int dir = 0; dir < 18; ++dir) {
-        const AS4 int32_t *off = gofs + dir * second * 2;
-        float s_lup = 0.f, s_albsh = 0.f, s_alb = 0.f;
-        float f_lup = 0.f, f_albsh = 0.f, f_alb = 0.f;
-        int cnt_wall = 0, cnt_any = 0, f_cnt_wall = 0, f_cnt_any = 0;
-        if (!me.ground) {
-            cnt_any = second;                         // f = 0 from the start: all walls "any"
-            f_cnt_any = first;
-        } else {
-            Sample t = me;
-            bool wall_seen = false, alive = true;
-            for (int s0 = 0; s0 < second; s0 += CHUNK) {
-                if (__ballot(alive) == 0) break;
-                v2f buf[CHUNK];
-                bool in[CHUNK];
+        const AS4 int32_t *off = gofs + dir * stride * 2;
+        Walk w = {building, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+        Walk f = w;
+        bool snapped = false;
+        long long qprev = at.p;
+        for (int s0 = 0; s0 < second; s0 += CHUNK) {
+            if (__ballot(w.af != 0.0f) == 0) break;    // every lane of the wave has stopped
+            v4f buf[CHUNK];
 #pragma unroll
-                for (int i = 0; i < CHUNK; ++i) {
-                    const int st = s0 + i < second ? s0 + i : second - 1;
-                    const int dr = off[2 * st], dc = off[2 * st + 1];
-                    long long q;
-                    if (!CHECK) {
-                        in[i] = true;
-                        q = at.p + (long long)dr * cols + dc;
-                    } else {
-                        const int rr = at.r + dr, cc = at.c + dc;
-                        in[i] = (unsigned)rr < (unsigned)rows && (unsigned)cc < (unsigned)cols;
-                        q = in[i] ? (long long)rr * cols + cc : at.p;
-                    }
-                    // unconditional so the gathers issue back to back; finished lanes re-read
-                    // their own record
-                    buf[i] = rec[alive ? q : at.p];
+            for (int i = 0; i < CHUNK; ++i) {
+                const int dr = off[2 * (s0 + i)], dc = off[2 * (s0 + i) + 1];
+                long long q;
+                if (!CHECK) {
+                    q = at.p + (long long)(dr * cols + dc);
+                } else {
+                    const int rr = at.r + dr, cc = at.c + dc;
+                    const bool in = (unsigned)rr < (unsigned)rows && (unsigned)cc < (unsigned)cols;
+                    q = in ? (long long)rr * cols + cc : qprev;
+                    qprev = q;
                 }
+                buf[i] = rec[q];
+            }
 #pragma unroll
-                for (int i = 0; i < CHUNK; ++i) {
-                    const int st = s0 + i;
-                    if (st >= second) break;
-                    if (!alive) continue;
-                    if (in[i]) t = decode(buf[i], shv_canopy);
-                    if (t.ground) {
-                        s_lup += t.lup;
-                        s_albsh += t.alb * t.shv;
-                        s_alb += t.alb;
-                        wall_seen = wall_seen || t.sunwall;
-                        cnt_wall += wall_seen ? 1 : 0;
-                        if (st + 1 <= first) {
-                            f_lup = s_lup; f_albsh = s_albsh; f_alb = s_alb;
-                            f_cnt_wall = cnt_wall; f_cnt_any = cnt_any;
-                        }
-                    } else {
-                        const int rest = second - st;     // this step and the remaining ones
-                        const int w = wall_seen ? 1 : 0;
-                        if (st + 1 <= first) {
-                            const int upto = first - st;
-                            f_lup = s_lup; f_albsh = s_albsh; f_alb = s_alb;
-                            f_cnt_wall = cnt_wall + w * upto;
-                            f_cnt_any = cnt_any + upto;
-                        }
-                        cnt_wall += w * rest;
-                        cnt_any += rest;
-                        alive = false;
-                    }
-                }
+            for (int i = 0; i < CHUNK; ++i) {
+                const int st = s0 + i;
+                if (st >= second) break;
+                const uint32_t fl = __float_as_uint(buf[i].w);
+                w.af *= (float)(fl & 0xffu);
+                w.lup = fmaf(w.af, buf[i].x, w.lup);
+                w.alb = fmaf(w.af, buf[i].y, w.alb);
+                w.albsh = fmaf(w.af, buf[i].z, w.albsh);
+                w.L += w.af;
+                // wf is 0 or 1: the larger bit pattern is the larger value
+                w.wf = __uint_as_float(max(__float_as_uint(w.wf),
+                                           __float_as_uint(w.af * (float)((fl >> 8) & 0xffu))));
+                w.cw = fmaf(w.wf, w.af, w.cw);
+                if (st == first - 1) { f = w; snapped = true; }
             }
         }
-        const bool sunin2 = cnt_wall > 0, f_sunin = f_cnt_wall > 0;
-        const bool wallin2 = cnt_any > 0, f_wallin = f_cnt_any > 0;
+        if (!snapped) f = w;                          // the wave stopped before step ``first``
+        const float cnt_wall = w.cw + w.wf * (secondf - w.L), cnt_any = secondf - w.L;
+        const float f_cnt_wall = f.cw + f.wf * (firstf - f.L), f_cnt_any = firstf - f.L;
+        const bool sunin2 = cnt_wall > 0.f, f_sunin = f_cnt_wall > 0.f;
+        const bool wallin2 = cnt_any > 0.f, f_wallin = f_cnt_any > 0.f;
         const AS4 double *bd = gbounds + dir * 3;
         int facesh;
         if (bd[2] == 0.0)
             facesh = (int)(aspect < bd[0] || aspect >= bd[1]) - (wallbol ? 1 : 0) + 1;
         else
             facesh = 1 - (int)(aspect > bd[0] || aspect <= bd[1]);
-        const bool keep = (cnt_wall == second) && facesh == 0;
-        const float s_lwall = keep ? 0.f : (float)cnt_wall * lwall;
-        const float s_albwall = keep ? 0.f : (float)cnt_wall * albedo_b;
-        const float f_lwall = (float)f_cnt_wall * lwall, f_albwall = (float)f_cnt_wall * albedo_b;
-        const float s_albwallnosh = (float)cnt_any * albedo_b, f_albwallnosh = (float)f_cnt_any * albedo_b;
-        const float lup_d = distance_weight(influence(f_lwall, f_lup, firstf + 1.f, firstf, f_sunin),
-                                            influence(s_lwall, s_lup, secondf + 1.f, secondf, sunin2)) +
+        const bool keep = (cnt_wall == secondf) && facesh == 0;
+        const float s_lwall = keep ? 0.f : cnt_wall * lwall;
+        const float s_albwall = keep ? 0.f : cnt_wall * albedo_b;
+        const float f_lwall = f_cnt_wall * lwall, f_albwall = f_cnt_wall * albedo_b;
+        const float s_albwallnosh = cnt_any * albedo_b, f_albwallnosh = f_cnt_any * albedo_b;
+        const float lup_d = distance_weight(influence(f_lwall, f.lup, firstf + 1.f, firstf, f_sunin),
+                                            influence(s_lwall, w.lup, secondf + 1.f, secondf, sunin2)) +
                             local_lup;
-        const float alb_d = distance_weight(influence(f_albwall, f_albsh, firstf + 1.f, firstf, f_sunin),
-                                            influence(s_albwall, s_albsh, secondf + 1.f, secondf, sunin2)) +
+        const float alb_d = distance_weight(influence(f_albwall, f.albsh, firstf + 1.f, firstf, f_sunin),
+                                            influence(s_albwall, w.albsh, secondf + 1.f, secondf, sunin2)) +
                             local_albsh;
-        const float nosh_d = distance_weight(influence(f_albwallnosh, f_alb, firstf + 1.f, firstf, f_wallin),
-                                             influence(s_albwallnosh, s_alb, secondf, secondf, wallin2)) *
+        const float nosh_d = distance_weight(influence(f_albwallnosh, f.alb, firstf + 1.f, firstf, f_wallin),
+                                             influence(s_albwallnosh, w.alb, secondf, secondf, wallin2)) *
                                  building +
                              local_nosh;
         o.all.lup += lup_d; o.all.alb += alb_d; o.all.nosh += nosh_d;
@@ -599,6 +575,67 @@ __device__ __forceinline__ float lag(float current, float &state, double w, bool
     return (float)delayed;
 }
 
+// Patches per altitude band (create_patches, umep_solweig.py:1737-1786): band b holds patches
+// [BAND_START[b], BAND_START[b+1]); the steradian-weighted trig terms are equal within a band.
+constexpr int BAND_START[9] = {0, 31, 61, 89, 113, 132, 145, 152, 153};
+
+// Daytime patch sums of one pixel (shortwave.cu and longwave.cu day terms, umep_solweig.py:666-911
+// and 946-1356). Diffuse: sum of diffsh_k x (luminance, lumChi cos omega) over the patches, with
+// diffsh 1 for sky, 1 - (1 - 0.03f) for vegetation-only and 0 for building patches. Building
+// patches are split into sunlit and shaded walls by the exact thresholds of the hour's table;
+// since the trig weights are per band, those sums are band counts times the band weight.
+__device__ __forceinline__ void day_patch_sums(const AS1 uint32_t *vis, long long P, long long p,
+                                               const AS4 float *pf, const AS4 double *pd,
+                                               double tan_asvf, float dveg, float &a_lv, float &a_kd,
+                                               float &a_swsun, float &a_swsh, float &a_lsunc,
+                                               float &a_lsuns, float &a_lshc, float &a_lshs) {
+#pragma unroll
+    for (int w = 0; w < SW_VIS_WORDS; ++w) {
+        const uint32_t vsh = vis[(0 * SW_VIS_WORDS + w) * P + p];
+        const uint32_t vveg = vis[(1 * SW_VIS_WORDS + w) * P + p];
+        const uint32_t vvb = vis[(2 * SW_VIS_WORDS + w) * P + p];
+        const uint32_t sky = vsh & vveg, vegonly = vsh & ~vveg, bld = ~vsh & vvb;
+        const int k0 = w * 32, n = SW_PATCHES - k0 < 32 ? SW_PATCHES - k0 : 32;
+#pragma unroll
+        for (int j = 0; j < n; ++j) {
+            const AS4 float *c = pf + (k0 + j) * SW_PATCH_F;
+            const float ds = ((sky >> j) & 1u) ? 1.0f : (((vegonly >> j) & 1u) ? dveg : 0.0f);
+            a_lv = fmaf(ds, c[SWP_LV], a_lv);
+            a_kd = fmaf(ds, c[SWP_KD], a_kd);
+        }
+        if (__ballot(bld != 0u) == 0) continue;       // no building patch in this word
+        uint32_t sun = 0u, shd = 0u, elig = 0u;
+#pragma unroll
+        for (int j = 0; j < n; ++j) {
+            const AS4 double *cd = pd + (k0 + j) * SW_PATCH_D;
+            const double x = tan_asvf + cd[SWP_YI];
+            sun |= (x < cd[SWP_SUN_BELOW] ? 1u : 0u) << j;
+            shd |= (x >= cd[SWP_SHADE_FROM] ? 1u : 0u) << j;
+            elig |= (pf[(k0 + j) * SW_PATCH_F + SWP_ELIGIBLE] != 0.0f ? 1u : 0u) << j;
+        }
+#pragma unroll
+        for (int b = 0; b < 8; ++b) {
+            const int lo = BAND_START[b] > k0 ? BAND_START[b] : k0;
+            const int hi = BAND_START[b + 1] < k0 + n ? BAND_START[b + 1] : k0 + n;
+            if (lo >= hi) continue;
+            const uint32_t span = (uint32_t)(hi - lo);
+            const uint32_t mask = (span == 32u ? 0xffffffffu : ((1u << span) - 1u)) << (lo - k0);
+            const uint32_t bm = bld & mask;
+            const float wc = pf[BAND_START[b] * SW_PATCH_F + SWP_WCOS];
+            const float wsn = pf[BAND_START[b] * SW_PATCH_F + SWP_WSIN];
+            const float c_sun = (float)__popc(bm & sun), c_sh = (float)__popc(bm & shd);
+            const float c_lsun = (float)__popc(bm & elig & sun);
+            const float c_lsh = (float)__popc(bm & (~elig | shd));
+            a_swsun = fmaf(wc, c_sun, a_swsun);
+            a_swsh = fmaf(wc, c_sh, a_swsh);
+            a_lsunc = fmaf(wc, c_lsun, a_lsunc);
+            a_lsuns = fmaf(wsn, c_lsun, a_lsuns);
+            a_lshc = fmaf(wc, c_lsh, a_lshc);
+            a_lshs = fmaf(wsn, c_lsh, a_lshs);
+        }
+    }
+}
+
 template <bool FULL>
 __global__ __launch_bounds__(BLOCK) void k_hour_pixels(const sw_domain *__restrict__ doms,
                                                        const int32_t *__restrict__ bs, int nd,
@@ -621,13 +658,12 @@ __global__ __launch_bounds__(BLOCK) void k_hour_pixels(const sw_domain *__restri
     float Kdown = 0.f, Kup = 0.f, KupE = 0.f, KupS = 0.f, KupW = 0.f, KupN = 0.f;
     float KsideI = 0.f, Lup, LupE, LupS, LupW, LupN, shadow = 0.f, Fsh = 0.f, tgout = 0.f, tg = 0.f;
     if (day) {
-        const float shv_canopy = (float)(1.0 - (1.0 - H[SWH_PSI]));
-        const Sample me = decode(G.rec[p], shv_canopy);
-        shadow = me.shv;
+        const v4f me = G.rec[p];
+        shadow = shadow_of_class((__float_as_uint(me.w) >> 16) & 3u, (float)(1.0 - (1.0 - H[SWH_PSI])));
         const int m = second - 1;
         const bool interior = at.r >= m && at.c >= m && at.r < (int)G.rows - m && at.c < (int)G.cols - m;
-        const GvfOut g = interior ? ground_view<false>(G, at, me, H, gofs, gbounds, first, second, shv_canopy)
-                                  : ground_view<true>(G, at, me, H, gofs, gbounds, first, second, shv_canopy);
+        const GvfOut g = interior ? ground_view<false>(G, at, me, H, gofs, gbounds, first, second)
+                                  : ground_view<true>(G, at, me, H, gofs, gbounds, first, second);
         const float amb = flux4 * emis;
         const double w = H[SWH_LAG_WEIGHT];
         const bool lf = H[SWH_LAG_FIRST] != 0.0, lu = H[SWH_LAG_UPDATE] != 0.0;
@@ -695,7 +731,11 @@ __global__ __launch_bounds__(BLOCK) void k_hour_pixels(const sw_domain *__restri
             a_lshs = ss[SWS_BLD_SIN * P + p];
         }
     }
-    if (FULL || day) {
+    if (!FULL && day) {
+        day_patch_sums(vis, P, p, pf, pd, tan_asvf, dveg, a_lv, a_kd, a_swsun, a_swsh, a_lsunc,
+                       a_lsuns, a_lshc, a_lshs);
+    }
+    if (FULL) {
         float lv_sky = 0.f, lv_veg = 0.f, kd_sky = 0.f, kd_veg = 0.f;
         for (int w = 0; w < SW_VIS_WORDS; ++w) {
             const uint32_t vsh = vis[(0 * SW_VIS_WORDS + w) * P + p];

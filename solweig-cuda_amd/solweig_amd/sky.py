@@ -279,10 +279,15 @@ def shadow_ray(azimuth, altitude, scale, amaxvalue, shape, first=0):
 
 def gvf_geometry(scale=1.0, first=1.0, second=22.0):
     """18 search directions x second_steps offsets and facesh bounds (prepare_gvf_geometry,
-    model.py:533-566; sunonsurface_2018a, umep_solweig.py:1466-1563)."""
+    model.py:533-566; sunonsurface_2018a, umep_solweig.py:1466-1563).
+
+    The step axis of ``offsets`` is padded to a multiple of 8 by repeating the last step (the
+    device loads the steps of a direction 8 at a time); steps past ``second_steps`` are never
+    summed."""
     first_steps = max(1, int(np.round(first * scale)))
     second_steps = int(np.round(second * scale))
-    offsets = np.empty((18, second_steps, 2), dtype=np.int32)
+    stride = -(-second_steps // 8) * 8
+    offsets = np.empty((18, stride, 2), dtype=np.int32)
     bounds = np.empty((18, 3))
     for j, degrees in enumerate(np.arange(5, 359, 20)):
         az = degrees * (np.pi / 180)
@@ -294,6 +299,7 @@ def gvf_geometry(scale=1.0, first=1.0, second=22.0):
             else:
                 dy, dx = np.sign(sa) * abs(round(n * ta)), -np.sign(ca) * n
             offsets[j, n] = int(dx), int(dy)
+        offsets[j, second_steps:] = offsets[j, second_steps - 1]
         low, high = az - np.pi / 2, az + np.pi / 2
         if low >= 0 and high < 2 * np.pi:
             bounds[j] = low, high, 0

@@ -117,7 +117,7 @@ class PreparedDomain:
         self.dev['svfbuveg'] = up(svfbuveg)
         self.dev['tan_asvf'] = up(np.tan(asvf), np.float64)
         self.dev['tan_svfalfa'] = up(np.tan(svfalfa), np.float64)
-        self.dev['rec'] = rt.DeviceArray((self.P, 2), np.float32)
+        self.dev['rec'] = rt.DeviceArray((self.P, 4), np.float32)
         self.dev['lag'] = rt.DeviceArray((6, self.P), np.float32)
         self.dev['skysums'] = sky_sums(self.dev['vis'], self.P, stream)
         if keep_host:
@@ -185,7 +185,7 @@ def prepared_from_arrays(*, dsm, vegdem, walls, dirwalls, buildings, svfbuveg, a
                     tan_asvf=up(np.tan(np.asarray(asvf, dtype=np.float64)), np.float64),
                     tan_svfalfa=up(np.tan(np.asarray(svfalfa, dtype=np.float64)), np.float64),
                     vis=up(pack_visibility(shmat, vegshmat, vbshvegshmat), np.uint32),
-                    rec=rt.DeviceArray((rows * cols, 2), np.float32),
+                    rec=rt.DeviceArray((rows * cols, 4), np.float32),
                     lag=rt.DeviceArray((6, rows * cols), np.float32))
     self.dev['skysums'] = sky_sums(self.dev['vis'], rows * cols, stream)
     self.skyview = None
