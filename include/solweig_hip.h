@@ -106,6 +106,9 @@ int sw_device_count(int *count);
 int sw_set_device(int device);
 int sw_malloc(void **ptr_dev, size_t bytes);
 int sw_free(void *ptr_dev);
+/* Page-locked host memory (hipHostMalloc) for asynchronous, stream-ordered uploads. */
+int sw_host_alloc(void **ptr, size_t bytes);
+int sw_host_free(void *ptr);
 int sw_memcpy_h2d(void *dst_dev, const void *src, size_t bytes, void *stream);
 int sw_memcpy_d2h(void *dst, const void *src_dev, size_t bytes, void *stream);
 int sw_memcpy_d2d(void *dst_dev, const void *src_dev, size_t bytes, void *stream);
@@ -117,6 +120,7 @@ int sw_event_create(void **event);
 int sw_event_destroy(void *event);
 int sw_event_record(void *event, void *stream);
 int sw_event_elapsed_ms(void *start, void *stop, float *ms);
+int sw_event_sync(void *event);
 int sw_device_sync(void);
 
 /* ----- static sky view (replaces svf.svf_calculator, svf.py:81-114; geometry.cu:77-159) -- */

@@ -14,7 +14,7 @@ import ctypes
 import numpy as np
 
 from . import runtime as rt
-from .sky import build_day, gvf_geometry
+from .sky import STEP_BLOCK, build_day, gvf_geometry
 from .static import PreparedDomain
 from .sun import met_sun
 
@@ -29,7 +29,7 @@ class Engine:
         self.gvf_bounds = rt.DeviceArray.from_host(bounds, np.float64)
         self.buf = []
         for d in self.domains:
-            cap = self.n_hours * (max(d.shape) + 2)
+            cap = self.n_hours * (max(d.shape) + 2 + STEP_BLOCK)
             b = dict(hours=rt.DeviceArray((self.n_hours, rt.SW_HOUR_STRIDE), np.float64),
                      ptab_f=rt.DeviceArray((self.n_hours, rt.SW_PATCHES, rt.SW_PATCH_F), np.float32),
                      ptab_d=rt.DeviceArray((self.n_hours, rt.SW_PATCHES, rt.SW_PATCH_D), np.float64),

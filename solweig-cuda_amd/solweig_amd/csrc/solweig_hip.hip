@@ -118,6 +118,17 @@ __device__ __forceinline__ DomainG domain_at(const sw_domain *D) {
 
 constexpr int CHUNK = 8;   // march / walk steps whose loads are issued together
 
+// A non-negative integer-valued FP64 field of a uniform record (a table offset or length) as an
+// int, with integer operations only, so the value stays in scalar registers (a float-to-int
+// conversion would run on the vector unit and force the table addresses into vector registers).
+__device__ __forceinline__ int uniform_index(double d) {
+    const uint64_t b = (uint64_t)__double_as_longlong(d);
+    const int e = (int)((b >> 52) & 0x7ffu) - 1023;
+    if (e < 0) return 0;
+    const uint64_t m = (b & ((1ull << 52) - 1)) | (1ull << 52);
+    return (int)(m >> (52 - e));
+}
+
 __device__ __forceinline__ PixelAt locate(const sw_domain *doms, const int32_t *bs_, int nd,
                                           int total, DomainG &G) {
     const AS4 int32_t *bs = (const AS4 int32_t *)bs_;
@@ -281,6 +292,58 @@ __device__ __forceinline__ float shadow_of_class(uint32_t cls, float shv_canopy)
     return cls == 2u ? 1.0f : (cls == 1u ? shv_canopy : 0.0f);
 }
 
+struct March {
+    double f, topmax, prev;
+    bool veg, ever;
+};
+
+// CHUNK steps [k0, k0 + CHUNK) of the sun-ray march of one pixel. MASKED: per-lane bounds
+// tests (a step outside the raster is made a no-op by an infinite dz) and only steps < L count;
+// otherwise all CHUNK steps are inside the raster for every lane and below L.
+template <bool MASKED>
+__device__ __forceinline__ void march_chunk(March &m, const AS1 float *dsm, const AS1 float *trees,
+                                            double raise, double me_a, const PixelAt &at, int rows,
+                                            int cols, const AS4 int32_t *off, const AS4 double *dzs,
+                                            int k0, int L) {
+#pragma clang fp contract(off)
+    float av[CHUNK], tv[CHUNK];
+    bool in[CHUNK];
+#pragma unroll
+    for (int i = 0; i < CHUNK; ++i) {
+        const int dr = off[2 * (k0 + i)], dc = off[2 * (k0 + i) + 1];
+        long long q;
+        if (!MASKED) {
+            in[i] = true;
+            q = at.p + (long long)(dr * cols + dc);
+        } else {
+            const int rr = at.r + dr, cc = at.c + dc;
+            in[i] = (unsigned)rr < (unsigned)rows && (unsigned)cc < (unsigned)cols;
+            q = in[i] ? (long long)rr * cols + cc : at.p;
+        }
+        av[i] = dsm[q];
+        tv[i] = trees[q];
+    }
+#pragma unroll
+    for (int i = 0; i < CHUNK; ++i) {
+        if (MASKED && k0 + i >= L) break;
+        const double z = dzs[k0 + i];
+        const double dz = (!MASKED || in[i]) ? z : __builtin_huge_val();
+        const double dzp = (!MASKED || in[i]) ? m.prev : __builtin_huge_val();
+        const double a = (double)av[i] + raise;
+        const double t = (double)tv[i];
+        const double v1 = t + a;
+        const double top = (v1 == a) ? 0.0 : v1;
+        const double v2 = __builtin_fma(t, 0.25, a);   // t * 0.25 is exact: one rounding, as NumPy
+        const double bot = (v2 == a) ? 0.0 : v2;
+        m.f = fmax(m.f, a - dz);
+        m.topmax = fmax(m.topmax, top - dz);
+        const bool vstep = (top - dzp > me_a) && !(bot - dz > me_a);
+        m.prev = z;
+        m.veg = (m.veg || vstep) && !(m.f > me_a);
+        m.ever = m.ever || m.veg;
+    }
+}
+
 template <bool FULL>
 __global__ __launch_bounds__(BLOCK) void k_hour_shadow(const sw_domain *__restrict__ doms,
                                                        const int32_t *__restrict__ bs, int nd,
@@ -295,46 +358,38 @@ __global__ __launch_bounds__(BLOCK) void k_hour_shadow(const sw_domain *__restri
     const AS1 float *dsm = G.dsm, *trees = G.trees;
     const double raise = G.dsm_raise;
     const Heights me = heights_at(dsm, trees, raise, p);
-    const int start = (int)H[SWH_RAY_START], L = (int)H[SWH_RAY_LEN];
+    const int start = uniform_index(H[SWH_RAY_START]);
+    const int L = uniform_index(H[SWH_RAY_LEN]);
     const AS4 int32_t *off = G.ray_off + 2LL * start;
     const AS4 double *dzs = G.ray_dz + start;
-    double f = me.a, topmax = me.top, prev = 0.0;
-    bool veg = (me.bot == 0.0) && me.top > 1.0;
-    bool ever = false;
+    // The march follows the reference's array form: f = running max of a - dz, the canopy
+    // pergola test on (top, bot) x (dz, previous dz), and topmax of top - dz. Because trees >= 0,
+    // bot <= top, so of the four canopy differences B = bot - dz is the smallest and Tp = top - prev
+    // the largest: "1 to 3 of the 4 above the receiver" is Tp > a && !(B > a).
+    // A shift that leaves the raster leaves it for good (row and column shifts grow in magnitude
+    // with the step), and such a step changes nothing (the zero fill of the shifted arrays is never
+    // above a >= 0), so each chunk of steps is either inside the raster for every lane of the wave
+    // (no bounds tests) or is evaluated with per-lane tests; the wave stops when no lane is inside.
+    March m;
+    m.f = me.a; m.topmax = me.top; m.prev = 0.0;
+    m.veg = (me.bot == 0.0) && me.top > 1.0;
+    m.ever = false;
+    const bool all_lanes = true;
+    const uint64_t active = __ballot(all_lanes);
     for (int k0 = 0; k0 < L; k0 += CHUNK) {
-        float av[CHUNK], tv[CHUNK];
-        bool in[CHUNK];
-#pragma unroll
-        for (int i = 0; i < CHUNK; ++i) {
-            const int k = k0 + i < L ? k0 + i : L - 1;
-            const int rr = at.r + off[2 * k], cc = at.c + off[2 * k + 1];
-            in[i] = (unsigned)rr < (unsigned)rows && (unsigned)cc < (unsigned)cols;
-            const long long q = in[i] ? (long long)rr * cols + cc : p;
-            av[i] = dsm[q];
-            tv[i] = trees[q];
-        }
-#pragma unroll
-        for (int i = 0; i < CHUNK; ++i) {
-            if (k0 + i >= L) break;
-            const double dz = dzs[k0 + i];
-            double A = 0.0, T = 0.0, B = 0.0, Tp = 0.0, Bp = 0.0;
-            if (in[i]) {
-                const Heights q = heights_of(av[i], tv[i], raise);
-                A = q.a - dz;
-                T = q.top - dz;
-                B = q.bot - dz;
-                Tp = q.top - prev;
-                Bp = q.bot - prev;
-            }
-            f = fmax(f, A);
-            topmax = fmax(topmax, T);
-            const int n = (int)(T > me.a) + (int)(B > me.a) + (int)(Tp > me.a) + (int)(Bp > me.a);
-            prev = dz;
-            veg = veg || (n > 0 && n < 4);
-            if (f > me.a) veg = false;
-            ever = ever || veg;
-        }
+        const int kl = k0 + CHUNK - 1 < L ? k0 + CHUNK - 1 : L - 1;
+        const bool in_last = (unsigned)(at.r + off[2 * kl]) < (unsigned)rows &&
+                             (unsigned)(at.c + off[2 * kl + 1]) < (unsigned)cols;
+        const bool in_first = (unsigned)(at.r + off[2 * k0]) < (unsigned)rows &&
+                              (unsigned)(at.c + off[2 * k0 + 1]) < (unsigned)cols;
+        if (__ballot(in_first) == 0) break;          // every lane's ray has left the raster
+        if (k0 + CHUNK <= L && __ballot(in_last) == active)
+            march_chunk<false>(m, dsm, trees, raise, me.a, at, rows, cols, off, dzs, k0, L);
+        else
+            march_chunk<true>(m, dsm, trees, raise, me.a, at, rows, cols, off, dzs, k0, L);
     }
+    const double f = m.f, topmax = m.topmax;
+    const bool veg = m.veg, ever = m.ever;
     const bool blocked = f > me.a;
     const double wall = (double)G.walls[p];
     const double wallbol = wall > 0.0 ? 1.0 : 0.0;
@@ -704,7 +759,7 @@ __global__ __launch_bounds__(BLOCK) void k_hour_pixels(const sw_domain *__restri
         if (FULL) tgout = (float)H[SWH_TA];
     }
     // ---- anisotropic patch sums over the 153 sky patches --------------------------------
-    const int row = (int)H[SWH_PATCH_TABLE];
+    const int row = uniform_index(H[SWH_PATCH_TABLE]);
     const AS4 float *pf = G.ptab_f + (long long)row * SW_PATCHES * SW_PATCH_F;
     const AS4 double *pd = G.ptab_d + (long long)row * SW_PATCHES * SW_PATCH_D;
     const double tan_asvf = G.tan_asvf[p];
@@ -927,6 +982,20 @@ int sw_malloc(void **ptr, size_t bytes) {
 }
 
 int sw_free(void *ptr) { return ptr ? hip_check(hipFree(ptr), "hipFree") : SW_OK; }
+
+int sw_host_alloc(void **ptr, size_t bytes) {
+    if (!ptr) return fail(SW_ERR_ARGUMENT, "ptr is NULL");
+    *ptr = nullptr;
+    if (bytes == 0) return SW_OK;
+    return hip_check(hipHostMalloc(ptr, bytes, hipHostMallocDefault), "hipHostMalloc");
+}
+
+int sw_host_free(void *ptr) { return ptr ? hip_check(hipHostFree(ptr), "hipHostFree") : SW_OK; }
+
+int sw_event_sync(void *event) {
+    if (!event) return fail(SW_ERR_ARGUMENT, "event is NULL");
+    return hip_check(hipEventSynchronize((hipEvent_t)event), "hipEventSynchronize");
+}
 
 int sw_memcpy_h2d(void *dst, const void *src, size_t bytes, void *stream) {
     if (!bytes) return SW_OK;

@@ -92,7 +92,7 @@ def prepare_from_domain(dom, location, scale=1.0, first=1.0, second=22.0, stream
     stream = stream or rt.Stream()
     rows, cols = dom.shape
     P = rows * cols
-    cap = max(rows, cols) + 2
+    cap = max(rows, cols) + 2 + sky.STEP_BLOCK
     bufs = dict(hours=rt.DeviceArray((1, rt.SW_HOUR_STRIDE), np.float64),
                 ptab_f=rt.DeviceArray((1, rt.SW_PATCHES, rt.SW_PATCH_F), np.float32),
                 ptab_d=rt.DeviceArray((1, rt.SW_PATCHES, rt.SW_PATCH_D), np.float64),
@@ -148,10 +148,9 @@ def run_step(prep, *, altitude, azimuth, zen, jday, dectime, altmax, Ta, RH, rad
     prep.ptab_f.upload(pf[None], prep.stream)
     prep.ptab_d.upload(pd[None], prep.stream)
     if len(dz):
-        off = np.ascontiguousarray(off, np.int32)
-        dz = np.ascontiguousarray(dz, np.float64)
-        rt.call('sw_memcpy_h2d', prep.ray_off.ptr, off.ctypes.data, off.nbytes, prep.stream.handle)
-        rt.call('sw_memcpy_h2d', prep.ray_dz.ptr, dz.ctypes.data, dz.nbytes, prep.stream.handle)
+        off, dz = sky.pad_steps(off, dz)
+        prep.ray_off.upload(off, prep.stream)
+        prep.ray_dz.upload(dz, prep.stream)
     for k, v in enumerate(lags):
         _set_lag(prep, k, v)
     rt.call('sw_hour_shadow', prep.dom_dev.ptr, prep.block_start.ptr, 1, dom.tiles, 0, int(shadow_planes),

@@ -67,6 +67,8 @@ def lib():
         'sw_version': ([], i32), 'sw_last_error': ([], ctypes.c_char_p),
         'sw_device_count': ([ctypes.POINTER(i32)], i32), 'sw_set_device': ([i32], i32),
         'sw_malloc': ([ctypes.POINTER(vp), sz], i32), 'sw_free': ([vp], i32),
+        'sw_host_alloc': ([ctypes.POINTER(vp), sz], i32), 'sw_host_free': ([vp], i32),
+        'sw_event_sync': ([vp], i32),
         'sw_memcpy_h2d': ([vp, vp, sz, vp], i32), 'sw_memcpy_d2h': ([vp, vp, sz, vp], i32),
         'sw_memcpy_d2d': ([vp, vp, sz, vp], i32), 'sw_memset': ([vp, i32, sz, vp], i32),
         'sw_stream_create': ([ctypes.POINTER(vp)], i32), 'sw_stream_destroy': ([vp], i32),
@@ -94,6 +96,7 @@ def lib():
 
 
 EXPORTED = ('sw_version', 'sw_last_error', 'sw_device_count', 'sw_set_device', 'sw_malloc', 'sw_free',
+            'sw_host_alloc', 'sw_host_free', 'sw_event_sync',
             'sw_memcpy_h2d', 'sw_memcpy_d2h', 'sw_memcpy_d2d', 'sw_memset', 'sw_stream_create',
             'sw_stream_destroy', 'sw_stream_sync', 'sw_event_create', 'sw_event_destroy',
             'sw_event_record', 'sw_event_elapsed_ms', 'sw_device_sync', 'sw_svf_visibility',

@@ -277,6 +277,23 @@ def shadow_ray(azimuth, altitude, scale, amaxvalue, shape, first=0):
     return off, dz[:n].copy(), low, high, middle
 
 
+STEP_BLOCK = 8   # the device reads ray and ground-view steps 8 at a time
+
+
+def pad_steps(off, dz):
+    """Ray offsets/heights padded to a multiple of STEP_BLOCK by repeating the last step (the
+    hour record keeps the true length; padded steps are loaded but never evaluated)."""
+    n = len(dz)
+    m = -(-n // STEP_BLOCK) * STEP_BLOCK
+    if m == n:
+        return np.ascontiguousarray(off, np.int32), np.ascontiguousarray(dz, np.float64)
+    o = np.empty((m, 2), np.int32)
+    z = np.empty(m, np.float64)
+    o[:n], z[:n] = off, dz
+    o[n:], z[n:] = off[n - 1], dz[n - 1]
+    return o, z
+
+
 def gvf_geometry(scale=1.0, first=1.0, second=22.0):
     """18 search directions x second_steps offsets and facesh bounds (prepare_gvf_geometry,
     model.py:533-566; sunonsurface_2018a, umep_solweig.py:1466-1563).
@@ -299,7 +316,7 @@ def gvf_geometry(scale=1.0, first=1.0, second=22.0):
             else:
                 dy, dx = np.sign(sa) * abs(round(n * ta)), -np.sign(ca) * n
             offsets[j, n] = int(dx), int(dy)
-        offsets[j, second_steps:] = offsets[j, second_steps - 1]
+        offsets[j, second_steps:] = offsets[j, second_steps - 1]      # see pad_steps
         low, high = az - np.pi / 2, az + np.pi / 2
         if low >= 0 and high < 2 * np.pi:
             bounds[j] = low, high, 0
@@ -486,6 +503,7 @@ def build_day(met, sun, latitude, shape, scale, amaxvalue, *, landcover=0, wall_
         h[H['PATCH_TABLE']] = i
         hours[i] = h
         if len(dz):
+            off, dz = pad_steps(off, dz)
             offs.append(off)
             dzs.append(dz)
             start += len(dz)
