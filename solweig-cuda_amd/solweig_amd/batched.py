@@ -49,6 +49,17 @@ class Engine:
                                        ray_off=b['ray_off'].ptr, ray_dz=b['ray_dz'].ptr,
                                        tmrt=b['tmrt'].ptr if 'tmrt' in b else 0,
                                        utci=b['utci'].ptr if 'utci' in b else 0)
+        # two page-locked staging slots per domain: the host fills one while the copies from the
+        # other may still be queued behind the previous day's launches
+        self._stage = [[dict(hours=rt.PinnedArray(b['hours'].shape, np.float64),
+                             ptab_f=rt.PinnedArray(b['ptab_f'].shape, np.float32),
+                             ptab_d=rt.PinnedArray(b['ptab_d'].shape, np.float64),
+                             ray_off=rt.PinnedArray((b['cap'], 2), np.int32),
+                             ray_dz=rt.PinnedArray((b['cap'],), np.float64)) for b in self.buf]
+                       for _ in range(2)]
+        self._stage_event = [rt.Event(), rt.Event()]
+        self._stage_used = [False, False]
+        self._slot = 0
         self.dom_dev = rt.DeviceArray((ctypes.sizeof(self.structs),), np.uint8)
         rt.call('sw_memcpy_h2d', self.dom_dev.ptr, ctypes.addressof(self.structs),
                 ctypes.sizeof(self.structs), self.stream.handle)
@@ -68,19 +79,32 @@ class Engine:
         return out
 
     def load(self, tables):
-        """Upload one day's tables (fixed addresses, so a captured graph stays valid)."""
+        """Queue the upload of one day's tables on the engine's stream (fixed device addresses, so
+        a captured graph stays valid). The copies run after every launch queued before them, so
+        the host may build and load the next day while the device still runs this one."""
         for t, b in zip(tables, self.buf):
             if t.n_hours != self.n_hours:
                 raise ValueError(f'expected {self.n_hours} hours, got {t.n_hours}')
             if t.ray_dz.shape[0] > b['cap']:
                 raise ValueError('ray tables exceed their buffer')
-            b['hours'].upload(t.hours, self.stream)
-            b['ptab_f'].upload(t.ptab_f, self.stream)
-            b['ptab_d'].upload(t.ptab_d, self.stream)
+        slot = self._slot
+        self._slot ^= 1
+        if self._stage_used[slot]:
+            self._stage_event[slot].sync()        # the copies out of this slot have finished
+        h = self.stream.handle
+        for t, b, st in zip(tables, self.buf, self._stage[slot]):
             n = t.ray_dz.shape[0]
-            rt.call('sw_memcpy_h2d', b['ray_off'].ptr, t.ray_off.ctypes.data, t.ray_off.nbytes, self.stream.handle)
-            rt.call('sw_memcpy_h2d', b['ray_dz'].ptr, t.ray_dz.ctypes.data, n * 8, self.stream.handle)
-            self.stream.sync()
+            st['hours'].array[...] = t.hours
+            st['ptab_f'].array[...] = t.ptab_f
+            st['ptab_d'].array[...] = t.ptab_d
+            st['ray_off'].array[:n] = t.ray_off
+            st['ray_dz'].array[:n] = t.ray_dz
+            for key in ('hours', 'ptab_f', 'ptab_d'):
+                rt.call('sw_memcpy_h2d', b[key].ptr, st[key].ptr, st[key].nbytes, h)
+            rt.call('sw_memcpy_h2d', b['ray_off'].ptr, st['ray_off'].ptr, n * 8, h)
+            rt.call('sw_memcpy_h2d', b['ray_dz'].ptr, st['ray_dz'].ptr, n * 8, h)
+        self._stage_event[slot].record(self.stream)
+        self._stage_used[slot] = True
 
     def reset_state(self):
         for d in self.domains:

@@ -192,6 +192,34 @@ class DeviceArray:
             pass
 
 
+class PinnedArray:
+    """Page-locked host memory (sw_host_alloc) exposed as a NumPy array, for uploads that run
+    asynchronously in stream order. The caller must not overwrite it while a copy from it is
+    still pending (wait on an event recorded after that copy)."""
+
+    def __init__(self, shape, dtype):
+        self.shape = tuple(int(s) for s in (shape if np.ndim(shape) else (shape,)))
+        self.dtype = np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape, dtype=np.int64)) * self.dtype.itemsize
+        self._ptr = ctypes.c_void_p()
+        call('sw_host_alloc', ctypes.byref(self._ptr), max(self.nbytes, 1))
+        buf = (ctypes.c_char * max(self.nbytes, 1)).from_address(self._ptr.value)
+        self.array = np.frombuffer(buf, dtype=self.dtype, count=int(np.prod(self.shape))).reshape(self.shape)
+
+    @property
+    def ptr(self):
+        return self._ptr.value or 0
+
+    def __del__(self):
+        try:
+            if self._ptr.value:
+                self.array = None
+                lib().sw_host_free(self._ptr)
+                self._ptr = ctypes.c_void_p()
+        except Exception:
+            pass
+
+
 def _s(stream):
     if stream is None:
         return None
@@ -225,6 +253,9 @@ class Event:
 
     def record(self, stream=None):
         call('sw_event_record', self._h, _s(stream))
+
+    def sync(self):
+        call('sw_event_sync', self._h)
 
     def elapsed_ms(self, later):
         ms = ctypes.c_float()
