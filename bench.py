@@ -24,6 +24,9 @@ import numpy as np  # noqa: E402
 
 B_ALG = 157.5          # algorithmic bytes per pixel-hour of the forward (static inputs + outputs)
 HBM_PEAK = 8.0e12      # MI355X HBM3E bytes/s
+# HBM bytes of one hour (shadow + pixel launch, mean over a day) of the 2000x2000 workload, from
+# rocprofv3 FETCH_SIZE (x 1.9999 copy calibration) + WRITE_SIZE; profiles/k_hour_pair_c2_2000x2000.md
+TRAFFIC_PER_HOUR_2000 = 1.054e9
 
 
 def log(msg):
@@ -150,7 +153,9 @@ def main():
     if pair_s:
         achieved = B_ALG * P / pair_s
         roof = dict(bound='hbm', achieved=achieved / 1e9, peak=HBM_PEAK / 1e9, unit='GB/s',
-                    frac=achieved / HBM_PEAK, traffic=None,
+                    frac=achieved / HBM_PEAK,
+                    traffic=TRAFFIC_PER_HOUR_2000 if args.size == 2000 else None,
+                    algorithmic_bytes_per_hour=B_ALG * P,
                     kernel='k_hour_shadow + k_hour_pixels (one hour of the forward)',
                     avg_ms=dict(k_hour_shadow=t_shadow / launches, k_hour_pixels=t_pixel / launches,
                                 daytime_hours=n_day_hours / max(args.steps, 1),
