@@ -165,6 +165,55 @@ int sw_hours_forward(const sw_domain *domains_dev, const int32_t *block_start_de
                      int total_blocks, int hour_begin, int hour_end, const int32_t *gvf_offsets_dev,
                      const double *gvf_bounds_dev, int first_steps, int second_steps, void *stream);
 
+/* ----- adjoint of J = mean over ground pixels of UTCI at one objective hour -------------------
+ * (no counterpart in the reference: the differentiable form of the per-step chain
+ * model.py:141-251 + utci.cu, with shadow and sky-view masks held fixed). Per hour the device
+ * returns dJ/d(hour-record value) for the values below, per-patch sums for the luminance and
+ * diffuse tables, and accumulates dJ/d(albedo map) and dJ/d(emissivity map).                  */
+enum {
+    SWA_TA = 0, SWA_PSI, SWA_CITGG, SWA_LWALL, SWA_ALBEDO_B, SWA_RADI, SWA_RADD, SWA_RADG,
+    SWA_FLUX4, SWA_SW_SUNLIT, SWA_SW_SHADED, SWA_SKY_A, SWA_SKY_B, SWA_LW_AMBIENT, SWA_LW_SUNLIT,
+    SWA_TA32, SWA_UTCI0,                      /* 7 UTCI coefficients                                */
+    SWA_TRANS_STATIC = SWA_UTCI0 + 7,         /* dJ/d transmissivity through svfbuveg           */
+    SWA_SLOTS = 24
+};
+/* per-patch sums at the objective hour: sum over pixels of lambda_Kdown (resp. lambda_Kside)
+ * over the pixels that see patch k as sky (resp. through vegetation only)                   */
+enum { SWA_PATCH_SKY_KDOWN = 0, SWA_PATCH_VEG_KDOWN, SWA_PATCH_SKY_KSIDE, SWA_PATCH_VEG_KSIDE,
+       SWA_PATCH_SUMS = 4 };
+
+/* Device buffers of the adjoint of one domain (all device pointers; 16 x 8 bytes). */
+typedef struct sw_adjoint {
+    float *lam_state;      /* [5][P] adjoint of the Lup lag states (all, E, S, W, N)        */
+    float *lam_rec;        /* [3][P] adjoint of the neighbour-record Lup, albedo x shadow, albedo */
+    float *coef;           /* [18][P][4] per-direction ground-view coefficients of the hour  */
+    float *grad_alb;       /* [P] dJ/d albedo map, accumulated                                 */
+    float *grad_emis;      /* [P] dJ/d emissivity map, accumulated                             */
+    float *lam_patch;      /* [2][P] lambda_Kdown, lambda_Kside at the objective hour           */
+    const float *svfveg;   /* [P] vegetation sky view factor (FP32)                           */
+    double *scalars;       /* [n_hours][SWA_SLOTS]                                             */
+    double *patch;         /* [SWA_PATCH_SUMS][153]                                            */
+    double *objective;     /* [2] sum of UTCI over ground pixels at the objective hour, count */
+    int64_t reserved[6];
+} sw_adjoint;
+
+/* One hour of the backward sweep for a single domain: the shadow pass of ``hour`` must have
+ * run (its neighbour records are read). At hour == objective_hour the forward of the hour is
+ * recomputed (the lag states must hold the previous hour's) and the adjoint seeded with
+ * 1/count per ground pixel (inv_count); earlier hours receive the lag-state adjoint only.
+ * Requires first_steps == 1.                                                                 */
+int sw_adjoint_hour(const sw_domain *domain_dev, const int32_t *block_start_dev, int total_blocks,
+                    int hour, int objective_hour, const sw_adjoint *adjoint_dev,
+                    const int32_t *gvf_offsets_dev, const double *gvf_bounds_dev, int first_steps,
+                    int second_steps, double inv_count, void *stream);
+/* Forward of hours [0, objective_hour) then the backward sweep objective_hour .. first_hour
+ * (the first daytime hour, where the lag resets), recomputing each hour's shadow pass.
+ * Buffers of adjoint_dev must be zeroed by the caller; lag states reset by the caller.      */
+int sw_day_adjoint(const sw_domain *domain_dev, const int32_t *block_start_dev, int total_blocks,
+                   int objective_hour, int first_hour, const sw_adjoint *adjoint_dev,
+                   const int32_t *gvf_offsets_dev, const double *gvf_bounds_dev, int first_steps,
+                   int second_steps, double inv_count, void *stream);
+
 /* UTCI of a Tmrt plane for scalar weather (replaces utci.utci_calculator's scalar-weather
  * path, utci.py:22-33, utci.cu:243-523): ``coefficients`` are 7 host FP64 values c_k with
  * UTCI = sum_k c_k (Tmrt - ta32)^k; sentinel != 0 writes -999; ground_dev (nullable) NaNs

@@ -115,17 +115,27 @@ def prepare(dsm, trees, dem, walls, aspect, latitude, *, scale=1.0, landcover=No
     return p
 
 
+def with_transmissivity(p, trans):
+    """p with canopy transmissivity ``trans`` in svfbuveg and diffsh (series.py:343-349)."""
+    q = dict(p)
+    q['trans'] = float(trans)
+    q['svfbuveg'] = p['svf'] - (1.0 - p['svfveg']) * (1.0 - q['trans'])
+    q['diffsh'] = osvf.diffsh(p['shmat'], p['vegshmat'], q['trans'])
+    return q
+
+
 def leaf_psi(doy):
     """series.py:416-422 (deciduous, leaf-on strictly between DOY 97 and 300)."""
     return np.where((doy > 97) & (doy < 300), 3. / 100., .5)
 
 
-def run_day(p, met, sun, *, keep=('Tmrt',)):
+def run_day(p, met, sun, *, keep=('Tmrt',), psi=None):
     """One day of hourly steps (series.py:396-495). ``sun`` holds per-row altitude, azimuth,
-    zen, jday, dectime, altmax. Returns a list of per-hour dicts with ``keep`` outputs + UTCI."""
+    zen, jday, dectime, altmax. Returns a list of per-hour dicts with ``keep`` outputs + UTCI.
+    ``psi`` overrides the seasonal canopy transmissivity of series.py:416-422."""
     rows, cols = p['dsm'].shape
     n = met.shape[0]
-    psi = leaf_psi(met[:, 1])
+    psi = leaf_psi(met[:, 1]) if psi is None else np.full(n, float(psi))
     dectime = sun['dectime']
     timestepdec = (dectime[1] - dectime[0]) if n > 1 else 1 / 24.
     zero = np.zeros((rows, cols))

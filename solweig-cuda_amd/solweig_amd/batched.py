@@ -14,7 +14,7 @@ import ctypes
 import numpy as np
 
 from . import runtime as rt
-from .sky import STEP_BLOCK, build_day, gvf_geometry
+from .sky import CONSTANTS, STEP_BLOCK, build_day, gvf_geometry
 from .static import PreparedDomain
 from .sun import met_sun
 
@@ -67,15 +67,18 @@ class Engine:
         self.graph = None
 
     # ------------------------------------------------------------------ tables
-    def tables(self, day_index=0, met=None, sun=None):
-        """Host tables of one day for every domain. ``met`` / ``sun`` override per domain."""
+    def tables(self, day_index=0, met=None, sun=None, constants=None, psi=None):
+        """Host tables of one day for every domain. ``met`` / ``sun`` override per domain;
+        ``constants`` (sky.CONSTANTS-like) and ``psi`` (canopy transmissivity of the hour's
+        shadow, default by season) override the model constants."""
         out = []
         for k, d in enumerate(self.domains):
             dm = d.domain
             rows = met[k] if met is not None else dm.met[day_index * self.n_hours:(day_index + 1) * self.n_hours]
             s = sun[k] if sun is not None else met_sun(rows, dm.location, dm.utc)
             out.append(build_day(rows, s, dm.latitude, d.shape, dm.scale, d.amaxvalue,
-                                 landcover=d.landcover, wall_params=d.wall_params))
+                                 landcover=d.landcover, wall_params=d.wall_params,
+                                 constants=constants or CONSTANTS, psi=psi))
         return out
 
     def load(self, tables):

@@ -13,7 +13,9 @@ SOURCES = [os.path.join(HERE, 'csrc', 'solweig_hip.hip')]
 OUTPUT = os.path.join(HERE, 'libsolweig_hip.so')
 FLAGS = ['--offload-arch=gfx950', '-O3', '-std=c++17', '-fPIC', '-shared',
          # packed FP32 VALU forms stay disabled (they have returned wrong values on these parts)
-         '-Xclang', '-target-feature', '-Xclang', '-packed-fp32-ops']
+         '-Xclang', '-target-feature', '-Xclang', '-packed-fp32-ops',
+         # hardware float atomics (the adjoint's accumulations into device-memory buffers)
+         '-munsafe-fp-atomics']
 
 
 def build(verbose=False):

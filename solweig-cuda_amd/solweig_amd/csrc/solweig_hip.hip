@@ -959,6 +959,488 @@ int check_launch(const void *doms, const void *bs, int nd, int total) {
     return SW_OK;
 }
 
+
+// =====================================================================================
+// Adjoint of J = (1/N) sum over ground pixels of UTCI at the objective hour, with the shadow
+// and sky-view masks held fixed. Per hour (backward from the objective hour):
+//   k_adj_prep   per receiver pixel: at the objective hour the forward of the hour is
+//                recomputed and the chain UTCI -> Tmrt -> Sstr -> radiation terms is reversed;
+//                every hour reverses the lag recursion and the ground-view sums, writing the
+//                per-direction coefficients of the walk and adding its own terms;
+//   k_adj_patch  (objective hour) per-patch sums of lambda_Kdown / lambda_Kside;
+//   k_adj_gather per source pixel: the transposed walk (the 18 x second samples that reach it)
+//                and the chain through the neighbour record to the albedo/emissivity maps.
+// Per-hour scalar derivatives are summed per wave in FP32 and across waves in FP64.
+// =====================================================================================
+struct AdjG {
+    AS1 float *lam_state, *lam_rec, *grad_alb, *grad_emis, *lam_patch;
+    AS1 v4f *coef;
+    const AS1 float *svfveg;
+    AS1 double *scalars, *patch, *objective;
+};
+
+__device__ __forceinline__ AdjG adjoint_at(const sw_adjoint *a_) {
+    const AS4 sw_adjoint *a = (const AS4 sw_adjoint *)a_;
+    AdjG g;
+    g.lam_state = (AS1 float *)a->lam_state; g.lam_rec = (AS1 float *)a->lam_rec;
+    g.coef = (AS1 v4f *)a->coef; g.grad_alb = (AS1 float *)a->grad_alb;
+    g.grad_emis = (AS1 float *)a->grad_emis; g.lam_patch = (AS1 float *)a->lam_patch;
+    g.svfveg = (const AS1 float *)a->svfveg; g.scalars = (AS1 double *)a->scalars;
+    g.patch = (AS1 double *)a->patch; g.objective = (AS1 double *)a->objective;
+    return g;
+}
+
+__device__ __forceinline__ float wave_sum(float v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// Adds the per-lane values v[0..n) of every lane of the wave to dst[0..n) (every lane must call).
+template <int N>
+__device__ __forceinline__ void add_wave(AS1 double *dst, const float (&v)[N]) {
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+        const float t = wave_sum(v[i]);
+        if ((threadIdx.x & 63) == 0 && t != 0.0f) atomicAdd((double *)(dst + i), (double)t);
+    }
+}
+
+__device__ __forceinline__ bool dir_in(int dir, int card) {
+    const int az = 5 + 20 * dir;
+    switch (card) {
+    case 1: return az < 180;                    // east half
+    case 2: return az >= 90 && az < 270;        // south half
+    case 3: return az >= 180;                   // west half
+    default: return az >= 270 || az < 90;       // north half
+    }
+}
+
+// Reverse of ground_view for one receiver: for each direction the walk is recomputed, its
+// coefficients (dJ per unit of a sample's Lup / albedo x shadow / albedo, and the walk length)
+// are stored for the gather, and the receiver's own terms (near field at step 0, local terms),
+// the wall-flux and wall-albedo derivatives are added. lg, la, ln: adjoints of the ground-view
+// sums (all, E, S, W, N) of Lup, albedo x shadow and albedo.
+template <bool CHECK>
+__device__ __forceinline__ void adj_walk(const DomainG &G, const AdjG &A, const PixelAt &at,
+                                         const v4f me, const AS4 double *H, const AS4 int32_t *gofs,
+                                         const AS4 double *gbounds, int first, int second,
+                                         const float (&lg)[5], const float (&la)[5],
+                                         const float (&ln)[5], float &d_lwall, float &d_albedo_b) {
+    const int rows = (int)G.rows, cols = (int)G.cols;
+    const long long P = G.pixels, p = at.p;
+    const AS1 v4f *rec = G.rec;
+    const float b = (float)(__float_as_uint(me.w) & 0xffu);
+    const bool wallbol = G.walls[p] > 0.0f;
+    const double aspect = ((double)G.aspect[p] * 3.141592653589793) / 180.0;
+    const float firstf = (float)first, secondf = (float)second;
+    const int stride = (second + CHUNK - 1) / CHUNK * CHUNK;
+    const float k4 = 0.4f / 0.9f, k5 = 0.5f / 0.9f;
+    float self_lup = 0.f, self_albsh = 0.f, self_alb = 0.f;
+    for (int dir = 0; dir < 18; ++dir) {
+        float mu = lg[0], nu = la[0], ka = ln[0];
+#pragma unroll
+        for (int c = 1; c < 5; ++c)
+            if (dir_in(dir, c)) { mu += lg[c]; nu += la[c]; ka += ln[c]; }
+        const AS4 int32_t *off = gofs + dir * stride * 2;
+        Walk w = {b, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+        Walk f = w;
+        bool snapped = false;
+        long long qprev = p;
+        float rep = 0.f;                              // alive steps that repeat qprev off the raster
+        for (int s0 = 0; s0 < second; s0 += CHUNK) {
+            if (__ballot(w.af != 0.0f) == 0) break;
+            v4f buf[CHUNK];
+            bool in[CHUNK];
+#pragma unroll
+            for (int i = 0; i < CHUNK; ++i) {
+                const int dr = off[2 * (s0 + i)], dc = off[2 * (s0 + i) + 1];
+                long long q;
+                if (!CHECK) {
+                    in[i] = true;
+                    q = p + (long long)(dr * cols + dc);
+                } else {
+                    const int rr = at.r + dr, cc = at.c + dc;
+                    in[i] = (unsigned)rr < (unsigned)rows && (unsigned)cc < (unsigned)cols;
+                    q = in[i] ? (long long)rr * cols + cc : qprev;
+                    qprev = q;
+                }
+                buf[i] = rec[q];
+            }
+#pragma unroll
+            for (int i = 0; i < CHUNK; ++i) {
+                const int st = s0 + i;
+                if (st >= second) break;
+                const uint32_t fl = __float_as_uint(buf[i].w);
+                w.af *= (float)(fl & 0xffu);
+                w.L += w.af;
+                w.wf = __uint_as_float(max(__float_as_uint(w.wf),
+                                           __float_as_uint(w.af * (float)((fl >> 8) & 0xffu))));
+                w.cw = fmaf(w.wf, w.af, w.cw);
+                if (CHECK && !in[i]) rep += w.af;
+                if (st == first - 1) { f = w; snapped = true; }
+            }
+        }
+        if (!snapped) f = w;
+        const float cnt_wall = w.cw + w.wf * (secondf - w.L), cnt_any = secondf - w.L;
+        const float f_cnt_wall = f.cw + f.wf * (firstf - f.L), f_cnt_any = firstf - f.L;
+        const bool sunin2 = cnt_wall > 0.f, f_sunin = f_cnt_wall > 0.f;
+        const bool wallin2 = cnt_any > 0.f, f_wallin = f_cnt_any > 0.f;
+        const AS4 double *bd = gbounds + dir * 3;
+        int facesh;
+        if (bd[2] == 0.0)
+            facesh = (int)(aspect < bd[0] || aspect >= bd[1]) - (wallbol ? 1 : 0) + 1;
+        else
+            facesh = 1 - (int)(aspect > bd[0] || aspect <= bd[1]);
+        const bool keep = (cnt_wall == secondf) && facesh == 0;
+        const float cs = k4 / (sunin2 ? secondf + 1.f : secondf);
+        const float cf = k5 / (f_sunin ? firstf + 1.f : firstf);
+        const float a_lup = mu * cs, a_albsh = nu * cs, a_nosh = ka * b * (k4 / secondf);
+        v4f cv;
+        cv.x = a_lup; cv.y = a_albsh; cv.z = a_nosh; cv.w = w.L;
+        A.coef[(long long)dir * P + p] = cv;
+        if (f.L >= 1.0f) {                             // near field: step 0 is the receiver itself
+            self_lup += mu * cf;
+            self_albsh += nu * cf;
+            self_alb += ka * b * (k5 / (f_wallin ? firstf + 1.f : firstf));
+        }
+        self_lup += mu * (1.0f - b);                  // local terms of a non-ground receiver
+        self_albsh += nu * (1.0f - b);
+        self_alb += ka * (1.0f - b);
+        const float wall_s = (sunin2 && !keep) ? cnt_wall / (secondf + 1.f) : 0.f;
+        const float wall_f = f_sunin ? f_cnt_wall / (firstf + 1.f) : 0.f;
+        d_lwall += mu * (k4 * wall_s + k5 * wall_f);
+        d_albedo_b += nu * (k4 * wall_s + k5 * wall_f) +
+                      ka * b * (k4 * (wallin2 ? cnt_any / secondf : 0.f) +
+                                k5 * (f_wallin ? f_cnt_any / (firstf + 1.f) : 0.f));
+        if (CHECK && rep > 0.f) {                      // repeated off-raster samples land on qprev
+            atomicAdd((float *)(A.lam_rec + qprev), a_lup * rep);
+            atomicAdd((float *)(A.lam_rec + P + qprev), a_albsh * rep);
+            atomicAdd((float *)(A.lam_rec + 2 * P + qprev), a_nosh * rep);
+        }
+    }
+    atomicAdd((float *)(A.lam_rec + p), self_lup);
+    atomicAdd((float *)(A.lam_rec + P + p), self_albsh);
+    atomicAdd((float *)(A.lam_rec + 2 * P + p), self_alb);
+}
+
+template <bool TOP>
+__global__ __launch_bounds__(BLOCK) void k_adj_prep(const sw_domain *__restrict__ doms,
+                                                    const int32_t *__restrict__ bs, int total, int hour,
+                                                    const sw_adjoint *__restrict__ adj_,
+                                                    const int32_t *__restrict__ gofs_,
+                                                    const double *__restrict__ gbounds_, int first,
+                                                    int second, double inv_n) {
+    const AS4 int32_t *gofs = (const AS4 int32_t *)gofs_;
+    const AS4 double *gbounds = (const AS4 double *)gbounds_;
+    DomainG G;
+    const PixelAt at = locate(doms, bs, 1, total, G);
+    const AdjG A = adjoint_at(adj_);
+    const AS4 double *H = G.hours + (long long)hour * SW_HOUR_STRIDE;
+    float sl[SWA_SLOTS];
+#pragma unroll
+    for (int i = 0; i < SWA_SLOTS; ++i) sl[i] = 0.f;
+    float obj[2] = {0.f, 0.f};
+    if (at.valid && H[SWH_DAY] != 0.0) {
+        const long long P = G.pixels, p = at.p;
+        const float emis = G.emis[p];
+        const v4f me = G.rec[p];
+        const uint32_t cls = (__float_as_uint(me.w) >> 16) & 3u;
+        const float shadow = shadow_of_class(cls, (float)(1.0 - (1.0 - H[SWH_PSI])));
+        const float flux4 = (float)H[SWH_FLUX4];
+        const float amb = flux4 * emis;
+        const double w = H[SWH_LAG_WEIGHT];
+        const bool lf = H[SWH_LAG_FIRST] != 0.0, lu = H[SWH_LAG_UPDATE] != 0.0;
+        const int m = second - 1;
+        const bool interior = at.r >= m && at.c >= m && at.r < (int)G.rows - m && at.c < (int)G.cols - m;
+        float ld[5] = {0.f, 0.f, 0.f, 0.f, 0.f};      // adjoint of the delayed Lup (all, E, S, W, N)
+        float la[5] = {0.f, 0.f, 0.f, 0.f, 0.f}, ln[5] = {0.f, 0.f, 0.f, 0.f, 0.f};
+        if (TOP) {
+            // ---- forward of the hour (as k_hour_pixels, day branch) ----------------------
+            const GvfOut g = interior ? ground_view<false>(G, at, me, H, gofs, gbounds, first, second)
+                                      : ground_view<true>(G, at, me, H, gofs, gbounds, first, second);
+            const AS1 float *lg = G.lag;
+            float s0 = lg[p], s1 = lg[P + p], s2 = lg[2 * P + p], s3 = lg[3 * P + p], s4 = lg[4 * P + p];
+            const float Lup = lag(g.all.lup / 18.0f + amb, s0, w, lf, lu);
+            const float LupE = lag(g.e.lup / 9.0f + amb, s1, w, lf, lu);
+            const float LupS = lag(g.s.lup / 9.0f + amb, s2, w, lf, lu);
+            const float LupW = lag(g.w.lup / 9.0f + amb, s3, w, lf, lu);
+            const float LupN = lag(g.n.lup / 9.0f + amb, s4, w, lf, lu);
+            const float radI = (float)H[SWH_RADI], radD = (float)H[SWH_RADD], radG = (float)H[SWH_RADG];
+            const float albedo_b = (float)H[SWH_ALBEDO_B];
+            const float Fsh = wedge(G.tan_svfalfa[p], H[SWH_TANZEN]);
+            const float svfbuveg = G.svfbuveg[p];
+            const float sinalt = (float)H[SWH_SINALT], cosalt = (float)H[SWH_COSALT];
+            const float X = radG * (1.0f - Fsh) + radD * Fsh;
+            const float around = albedo_b * (1.0f - svfbuveg) * X;
+            const float refl = radD * svfbuveg + around;
+            const float Kup = (g.all.alb / 18.0f) * radI * sinalt + refl * (g.all.nosh / 18.0f);
+            const float KupE = (g.e.alb / 9.0f) * radI * sinalt + refl * (g.e.nosh / 9.0f);
+            const float KupS = (g.s.alb / 9.0f) * radI * sinalt + refl * (g.s.nosh / 9.0f);
+            const float KupW = (g.w.alb / 9.0f) * radI * sinalt + refl * (g.w.nosh / 9.0f);
+            const float KupN = (g.n.alb / 9.0f) * radI * sinalt + refl * (g.n.nosh / 9.0f);
+            const float KsideI = shadow * radI * cosalt;
+            const int row = uniform_index(H[SWH_PATCH_TABLE]);
+            const AS4 float *pf = G.ptab_f + (long long)row * SW_PATCHES * SW_PATCH_F;
+            const AS4 double *pd = G.ptab_d + (long long)row * SW_PATCHES * SW_PATCH_D;
+            float a_lv = 0.f, a_kd = 0.f, a_swsun = 0.f, a_swsh = 0.f, a_lsunc = 0.f, a_lsuns = 0.f;
+            float a_lshc = 0.f, a_lshs = 0.f;
+            day_patch_sums(G.vis, P, p, pf, pd, G.tan_asvf[p], (float)H[SWH_DIFF_VEG], a_lv, a_kd,
+                           a_swsun, a_swsh, a_lsunc, a_lsuns, a_lshc, a_lshs);
+            const AS1 float *ss = G.skysums;
+            const float SA = (float)H[SWH_SKY_A], SB = (float)H[SWH_SKY_B];
+            const float ss0 = ss[SWS_SKY_SIN * P + p], ss1 = ss[SWS_SKY_GSIN * P + p];
+            const float ss2 = ss[SWS_SKY_COS * P + p], ss3 = ss[SWS_SKY_GCOS * P + p];
+            const float a_ldn = SA * ss0 - SB * ss1, a_lsd = SA * ss2 - SB * ss3;
+            const float a_vc = ss[SWS_VEG_COS * P + p], a_vs = ss[SWS_VEG_SIN * P + p];
+            const float a_rc = ss[SWS_OBS_COS * P + p], a_rs = ss[SWS_OBS_SIN * P + p];
+            const float dRad = a_lv * radD;
+            const float Kdown = radI * shadow * sinalt + around + dRad;
+            const float sw_sun = (float)H[SWH_SW_SUNLIT], sw_sh = (float)H[SWH_SW_SHADED];
+            const float Kside = KsideI + a_kd + sw_sun * a_swsun + sw_sh * a_swsh + sw_sh * a_vc;
+            const float lw_amb = (float)H[SWH_LW_AMBIENT], lw_sun = (float)H[SWH_LW_SUNLIT];
+            const float lw_refl = (float)H[SWH_LW_REFL];
+            const float reflsurf = (a_ldn + Lup) * lw_refl;
+            const float Ldown = a_ldn + lw_amb * a_vs + lw_amb * a_lshs + lw_sun * a_lsuns + reflsurf * a_rs;
+            const float Lside = a_lsd + lw_amb * a_vc + lw_amb * a_lshc + lw_sun * a_lsunc + reflsurf * a_rc;
+            const float absK = (float)H[SWH_ABSK], absL = (float)H[SWH_ABSL];
+            const float Fup = (float)H[SWH_FUP], Fside = (float)H[SWH_FSIDE], Fcyl = (float)H[SWH_FCYL];
+            const float Ksum = (KupN + KupE + KupS + KupW) * 0.5f;
+            const float Lsum = (LupN + LupE + LupS + LupW) * 0.5f;
+            const float Sstr = absK * (Kside * Fcyl + (Kdown + Kup) * Fup + Ksum * Fside) +
+                               absL * ((Ldown + Lup) * Fup + Lside * Fcyl + Lsum * Fside);
+            const float Tmrt = sqrtf(sqrtf(Sstr / (absL * (float)SBC))) - 273.2f;
+            const bool ground = G.ground[p] != 0;
+            // ---- reverse ---------------------------------------------------------------------
+            float lU = 0.f, lT = 0.f;
+            if (ground && H[SWH_UTCI0 + 6] != -999.0 && Tmrt > -999.0f) {
+                const double dT = (double)(Tmrt - (float)H[SWH_TA32]);
+                double acc = H[SWH_UTCI0 + 6], der = 6.0 * H[SWH_UTCI0 + 6];
+                for (int k = 5; k >= 0; --k) {
+                    acc = acc * dT + H[SWH_UTCI0 + k];
+                    if (k >= 1) der = der * dT + (double)k * H[SWH_UTCI0 + k];
+                }
+                obj[0] = (float)acc;
+                obj[1] = 1.0f;
+                lU = (float)inv_n;
+                lT = (float)(inv_n * der);
+                double pw = 1.0;
+                for (int k = 0; k < 7; ++k) { sl[SWA_UTCI0 + k] = (float)(inv_n * pw); pw *= dT; }
+                sl[SWA_TA32] = -lT;
+            }
+            (void)lU;
+            const float lS = lT * (Tmrt + 273.2f) / (4.0f * Sstr);
+            const float lKside = lS * absK * Fcyl, lKdown = lS * absK * Fup, lKup = lS * absK * Fup;
+            const float lKupC = lS * absK * Fside * 0.5f;
+            const float lLdown = lS * absL * Fup, lLside = lS * absL * Fcyl;
+            const float lLupC = lS * absL * Fside * 0.5f;
+            const float lrefls = lLdown * a_rs + lLside * a_rc;
+            ld[0] = lS * absL * Fup + lrefls * lw_refl;
+            ld[1] = ld[2] = ld[3] = ld[4] = lLupC;
+            const float l_ldn = lLdown + lrefls * lw_refl;
+            sl[SWA_SKY_A] = l_ldn * ss0 + lLside * ss2;
+            sl[SWA_SKY_B] = -(l_ldn * ss1 + lLside * ss3);
+            sl[SWA_LW_AMBIENT] = lLdown * (a_vs + a_lshs) + lLside * (a_vc + a_lshc);
+            sl[SWA_LW_SUNLIT] = lLdown * a_lsuns + lLside * a_lsunc;
+            sl[SWA_SW_SUNLIT] = lKside * a_swsun;
+            sl[SWA_SW_SHADED] = lKside * (a_swsh + a_vc);
+            A.lam_patch[p] = lKdown;
+            A.lam_patch[P + p] = lKside;
+            // Kup and the ground-view albedo sums
+            const float gsum_alb = g.all.alb / 18.0f, gsum_nosh = g.all.nosh / 18.0f;
+            float lrefl = lKup * gsum_nosh;
+            float d_radI = lKup * sinalt * gsum_alb;
+            la[0] = lKup * radI * sinalt / 18.0f;
+            ln[0] = lKup * refl / 18.0f;
+            const Triple gc[4] = {g.e, g.s, g.w, g.n};
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                la[c + 1] = lKupC * radI * sinalt / 9.0f;
+                ln[c + 1] = lKupC * refl / 9.0f;
+                lrefl += lKupC * (gc[c].nosh / 9.0f);
+                d_radI += lKupC * sinalt * (gc[c].alb / 9.0f);
+            }
+            const float laround = lKdown + lrefl;
+            d_radI += lKdown * shadow * sinalt + lKside * shadow * cosalt;
+            sl[SWA_RADI] = d_radI;
+            sl[SWA_RADD] = lKdown * a_lv + lrefl * svfbuveg + laround * albedo_b * (1.0f - svfbuveg) * Fsh;
+            sl[SWA_RADG] = laround * albedo_b * (1.0f - svfbuveg) * (1.0f - Fsh);
+            sl[SWA_ALBEDO_B] = laround * (1.0f - svfbuveg) * X;
+            const float lsvfbuveg = lrefl * radD - laround * albedo_b * X;
+            sl[SWA_TRANS_STATIC] = lsvfbuveg * (1.0f - A.svfveg[p]);
+            const float lshadow = lKdown * radI * sinalt + lKside * radI * cosalt;
+            sl[SWA_PSI] = cls == 1u ? lshadow : 0.f;
+        }
+        // ---- lag recursion, reversed (the delayed value is used this hour, the state by the next)
+        AS1 float *ls = A.lam_state;
+        float lcur[5];
+        const float wf = (float)w;
+#pragma unroll
+        for (int k = 0; k < 5; ++k) {
+            const float lso = ls[k * P + p];
+            const float ldt = ld[k] + (lu ? lso : 0.f);
+            const float lprev = ldt * wf + (lu ? 0.f : lso);
+            lcur[k] = ldt * (1.0f - wf) + (lf ? lprev : 0.f);
+            ls[k * P + p] = lf ? 0.f : lprev;
+        }
+        // current Lup = ground-view sum / 18 (or / 9 per half) + flux4 * emis
+        const float lg5[5] = {lcur[0] / 18.0f, lcur[1] / 9.0f, lcur[2] / 9.0f, lcur[3] / 9.0f, lcur[4] / 9.0f};
+        const float lamb = lcur[0] + lcur[1] + lcur[2] + lcur[3] + lcur[4];
+        A.grad_emis[p] += lamb * flux4;
+        sl[SWA_FLUX4] = lamb * emis;
+        float d_lwall = 0.f, d_albedo_b = 0.f;
+        if (interior) adj_walk<false>(G, A, at, me, H, gofs, gbounds, first, second, lg5, la, ln, d_lwall, d_albedo_b);
+        else adj_walk<true>(G, A, at, me, H, gofs, gbounds, first, second, lg5, la, ln, d_lwall, d_albedo_b);
+        sl[SWA_LWALL] = d_lwall;
+        sl[SWA_ALBEDO_B] += d_albedo_b;
+    }
+    add_wave(A.scalars + (long long)hour * SWA_SLOTS, sl);
+    if (TOP) add_wave(A.objective, obj);
+}
+
+// Per-patch sums of lambda_Kdown and lambda_Kside over the pixels that see the patch as sky or
+// through vegetation only (objective hour).
+__global__ __launch_bounds__(256) void k_adj_patch(const uint32_t *__restrict__ bits,
+                                                   const float *__restrict__ lam, long long P,
+                                                   double *__restrict__ out) {
+    const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const bool valid = p < P;
+    const float lK = valid ? lam[p] : 0.f, lS = valid ? lam[P + p] : 0.f;
+    for (int w = 0; w < SW_VIS_WORDS; ++w) {
+        const uint32_t vsh = valid ? bits[(0 * SW_VIS_WORDS + w) * P + p] : 0u;
+        const uint32_t vveg = valid ? bits[(1 * SW_VIS_WORDS + w) * P + p] : 0u;
+        const uint32_t sky = vsh & vveg, vegonly = vsh & ~vveg;
+        const int k0 = w * 32, n = SW_PATCHES - k0 < 32 ? SW_PATCHES - k0 : 32;
+        for (int j = 0; j < n; ++j) {
+            const bool a = (sky >> j) & 1u, v = (vegonly >> j) & 1u;
+            const uint64_t any = __ballot(a || v);
+            if (any == 0) continue;
+            const float t0 = wave_sum(a ? lK : 0.f), t1 = wave_sum(v ? lK : 0.f);
+            const float t2 = wave_sum(a ? lS : 0.f), t3 = wave_sum(v ? lS : 0.f);
+            if ((threadIdx.x & 63) == 0) {
+                const int k = k0 + j;
+                if (t0 != 0.f) atomicAdd(out + SWA_PATCH_SKY_KDOWN * SW_PATCHES + k, (double)t0);
+                if (t1 != 0.f) atomicAdd(out + SWA_PATCH_VEG_KDOWN * SW_PATCHES + k, (double)t1);
+                if (t2 != 0.f) atomicAdd(out + SWA_PATCH_SKY_KSIDE * SW_PATCHES + k, (double)t2);
+                if (t3 != 0.f) atomicAdd(out + SWA_PATCH_VEG_KSIDE * SW_PATCHES + k, (double)t3);
+            }
+        }
+    }
+}
+
+// Transposed walk for each source pixel q: the sample of receiver p = q - shift(d, s) at step s
+// reaches q when p's walk is alive there (s < L_d(p)). Then the neighbour record of q is
+// reversed to the albedo and emissivity maps and the hour's Ta, CI_TgG and canopy shadow.
+__global__ __launch_bounds__(BLOCK) void k_adj_gather(const sw_domain *__restrict__ doms,
+                                                      const int32_t *__restrict__ bs, int total,
+                                                      int hour, const sw_adjoint *__restrict__ adj_,
+                                                      const int32_t *__restrict__ gofs_, int second) {
+#pragma clang fp contract(off)
+    const AS4 int32_t *gofs = (const AS4 int32_t *)gofs_;
+    DomainG G;
+    const PixelAt at = locate(doms, bs, 1, total, G);
+    const AdjG A = adjoint_at(adj_);
+    const AS4 double *H = G.hours + (long long)hour * SW_HOUR_STRIDE;
+    float sl[3] = {0.f, 0.f, 0.f};                    // Ta, CI_TgG, psi
+    if (at.valid && H[SWH_DAY] != 0.0) {
+        const int rows = (int)G.rows, cols = (int)G.cols;
+        const long long P = G.pixels, q = at.p;
+        const int stride = (second + CHUNK - 1) / CHUNK * CHUNK;
+        const int m = second - 1;
+        const bool interior = at.r >= m && at.c >= m && at.r < rows - m && at.c < cols - m;
+        AS1 float *lr = A.lam_rec;
+        float l_lup = lr[q], l_albsh = lr[P + q], l_alb = lr[2 * P + q];
+        lr[q] = 0.f; lr[P + q] = 0.f; lr[2 * P + q] = 0.f;
+        for (int dir = 0; dir < 18; ++dir) {
+            const AS4 int32_t *off = gofs + dir * stride * 2;
+            const AS1 v4f *cd = A.coef + (long long)dir * P;
+            for (int s0 = 0; s0 < second; s0 += CHUNK) {
+                v4f buf[CHUNK];
+                bool in[CHUNK];
+#pragma unroll
+                for (int i = 0; i < CHUNK; ++i) {
+                    const int dr = off[2 * (s0 + i)], dc = off[2 * (s0 + i) + 1];
+                    long long pr;
+                    if (interior) {
+                        in[i] = true;
+                        pr = q - (long long)(dr * cols + dc);
+                    } else {
+                        const int rr = at.r - dr, cc = at.c - dc;
+                        in[i] = (unsigned)rr < (unsigned)rows && (unsigned)cc < (unsigned)cols;
+                        pr = in[i] ? (long long)rr * cols + cc : q;
+                    }
+                    buf[i] = cd[pr];
+                }
+#pragma unroll
+                for (int i = 0; i < CHUNK; ++i) {
+                    const int st = s0 + i;
+                    if (st >= second) break;
+                    const float on = (in[i] && (float)st < buf[i].w) ? 1.0f : 0.0f;
+                    l_lup = fmaf(on, buf[i].x, l_lup);
+                    l_albsh = fmaf(on, buf[i].y, l_albsh);
+                    l_alb = fmaf(on, buf[i].z, l_alb);
+                }
+            }
+        }
+        // the record of q: Lup = SBC emis ((Tg sh + Ta + 273.15)^4 - (Ta + 273.15)^4), alb, alb sh
+        const v4f r = G.rec[q];
+        const uint32_t cls = (__float_as_uint(r.w) >> 16) & 3u;
+        const float sh = shadow_of_class(cls, (float)(1.0 - (1.0 - H[SWH_PSI])));
+        const double snup = H[SWH_SNUP];
+        const double amp = (double)G.tgk[q] * H[SWH_ALTMAX] + (double)G.tstart[q];
+        const double base = amp * sin(((H[SWH_FRAC] - snup / 24.0) / ((double)G.tmaxlst[q] / 24.0 - snup / 24.0)) *
+                                      3.141592653589793 / 2.0);
+        double tg = base * H[SWH_CITGG];
+        const bool clamped = G.landcover == 1 && tg < 0.0;
+        if (clamped) tg = 0.0;
+        const double ta = H[SWH_TA], emis = (double)G.emis[q];
+        const double tk = tg * (double)sh + ta + 273.15, ta4 = ta + 273.15;
+        const double ll = (double)l_lup;
+        A.grad_emis[q] += (float)(ll * SBC * ((tk * tk) * (tk * tk) - (ta4 * ta4) * (ta4 * ta4)));
+        A.grad_alb[q] += l_alb + l_albsh * sh;
+        const double dtk = ll * SBC * emis * 4.0 * (tk * tk * tk);
+        sl[0] = (float)(dtk - ll * SBC * emis * 4.0 * (ta4 * ta4 * ta4));
+        sl[1] = clamped ? 0.f : (float)(dtk * (double)sh * base);
+        sl[2] = cls == 1u ? (float)(dtk * tg) + l_albsh * G.alb[q] : 0.f;
+    }
+    AS1 double *d = A.scalars + (long long)hour * SWA_SLOTS;
+    const float v0[1] = {sl[0]}, v1[1] = {sl[1]}, v2[1] = {sl[2]};
+    add_wave(d + SWA_TA, v0);
+    add_wave(d + SWA_CITGG, v1);
+    add_wave(d + SWA_PSI, v2);
+}
+
+int launch_adjoint_hour(const sw_domain *dom, const int32_t *bs, int total, int hour, bool top,
+                        const sw_adjoint *adj, const int32_t *gofs, const double *gb, int first,
+                        int second, double inv_n, hipStream_t s) {
+    if (top)
+        hipLaunchKernelGGL(k_adj_prep<true>, dim3(total), dim3(BLOCK), 0, s, dom, bs, total, hour, adj,
+                           gofs, gb, first, second, inv_n);
+    else
+        hipLaunchKernelGGL(k_adj_prep<false>, dim3(total), dim3(BLOCK), 0, s, dom, bs, total, hour, adj,
+                           gofs, gb, first, second, inv_n);
+    int rc = hip_check(hipGetLastError(), "k_adj_prep launch");
+    if (rc) return rc;
+    if (top) {
+        // the per-patch sums need the pixel count and the visibility bits: read the records
+        sw_domain d;
+        sw_adjoint a;
+        rc = hip_check(hipMemcpyAsync(&d, dom, sizeof d, hipMemcpyDeviceToHost, s), "domain record");
+        if (rc) return rc;
+        rc = hip_check(hipMemcpyAsync(&a, adj, sizeof a, hipMemcpyDeviceToHost, s), "adjoint record");
+        if (rc) return rc;
+        rc = hip_check(hipStreamSynchronize(s), "adjoint record sync");
+        if (rc) return rc;
+        const long long P = d.pixels;
+        hipLaunchKernelGGL(k_adj_patch, dim3((unsigned)((P + 255) / 256)), dim3(256), 0, s, d.vis,
+                           a.lam_patch, P, a.patch);
+        rc = hip_check(hipGetLastError(), "k_adj_patch launch");
+        if (rc) return rc;
+    }
+    hipLaunchKernelGGL(k_adj_gather, dim3(total), dim3(BLOCK), 0, s, dom, bs, total, hour, adj, gofs, second);
+    return hip_check(hipGetLastError(), "k_adj_gather launch");
+}
+
 }  // namespace
 
 extern "C" {
@@ -1127,6 +1609,44 @@ int sw_hours_forward(const sw_domain *doms, const int32_t *bs, int nd, int total
         if (rc) return rc;
         rc = launch_hour_pixels(doms, bs, nd, total, h, gofs, gb, first, second, false,
                                 (hipStream_t)stream);
+        if (rc) return rc;
+    }
+    return SW_OK;
+}
+
+int sw_adjoint_hour(const sw_domain *dom, const int32_t *bs, int total, int hour, int objective_hour,
+                    const sw_adjoint *adj, const int32_t *gofs, const double *gb, int first, int second,
+                    double inv_n, void *stream) {
+    int rc = check_launch(dom, bs, 1, total);
+    if (rc) return rc;
+    if (!adj || !gofs || !gb) return fail(SW_ERR_ARGUMENT, "sw_adjoint_hour: NULL pointer");
+    if (first != 1) return fail(SW_ERR_ARGUMENT, "sw_adjoint_hour: first_steps must be 1");
+    if (second < 1 || second > MAX_SECOND || hour < 0 || hour > objective_hour)
+        return fail(SW_ERR_ARGUMENT, "sw_adjoint_hour: bad arguments");
+    return launch_adjoint_hour(dom, bs, total, hour, hour == objective_hour, adj, gofs, gb, first, second,
+                               inv_n, (hipStream_t)stream);
+}
+
+int sw_day_adjoint(const sw_domain *dom, const int32_t *bs, int total, int objective_hour, int first_hour,
+                   const sw_adjoint *adj, const int32_t *gofs, const double *gb, int first, int second,
+                   double inv_n, void *stream) {
+    int rc = check_launch(dom, bs, 1, total);
+    if (rc) return rc;
+    if (!adj || !gofs || !gb || first != 1 || second < 1 || second > MAX_SECOND || first_hour < 0 ||
+        first_hour > objective_hour)
+        return fail(SW_ERR_ARGUMENT, "sw_day_adjoint: bad arguments");
+    const hipStream_t s = (hipStream_t)stream;
+    for (int h = 0; h < objective_hour; ++h) {
+        rc = launch_hour_shadow(dom, bs, 1, total, h, false, s);
+        if (rc) return rc;
+        rc = launch_hour_pixels(dom, bs, 1, total, h, gofs, gb, first, second, false, s);
+        if (rc) return rc;
+    }
+    for (int h = objective_hour; h >= first_hour; --h) {
+        rc = launch_hour_shadow(dom, bs, 1, total, h, false, s);
+        if (rc) return rc;
+        rc = launch_adjoint_hour(dom, bs, total, h, h == objective_hour, adj, gofs, gb, first, second,
+                                 inv_n, s);
         if (rc) return rc;
     }
     return SW_OK;

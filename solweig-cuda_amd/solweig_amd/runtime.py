@@ -47,6 +47,21 @@ class SwDomain(ctypes.Structure):
                                         'ray_off', 'ray_dz', 'masks', 'shadow_planes', 'full', 'skysums')]
 
 
+class SwAdjoint(ctypes.Structure):
+    """Mirror of sw_adjoint: device pointers of the adjoint buffers."""
+    _fields_ = [(n, ctypes.c_uint64) for n in ('lam_state', 'lam_rec', 'coef', 'grad_alb', 'grad_emis',
+                                               'lam_patch', 'svfveg', 'scalars', 'patch', 'objective')] + \
+        [('reserved', ctypes.c_int64 * 6)]
+
+
+# slots of the per-hour adjoint record (enum SWA_* of solweig_hip.h)
+_ADJ_FIELDS = 'TA PSI CITGG LWALL ALBEDO_B RADI RADD RADG FLUX4 SW_SUNLIT SW_SHADED SKY_A SKY_B LW_AMBIENT LW_SUNLIT TA32 UTCI0'.split()
+ADJ = {name: i for i, name in enumerate(_ADJ_FIELDS)}
+ADJ['TRANS_STATIC'] = ADJ['UTCI0'] + 7
+ADJ_SLOTS = 24
+ADJ_PATCH = {'SKY_KDOWN': 0, 'VEG_KDOWN': 1, 'SKY_KSIDE': 2, 'VEG_KSIDE': 3}
+
+
 class SolweigHipError(RuntimeError):
     pass
 
@@ -84,6 +99,8 @@ def lib():
         'sw_utci_scalar': ([vp, i64, vp, f64, i32, vp, vp, vp], i32),
         'sw_copy': ([vp, vp, sz, vp], i32),
         'sw_sky_sums': ([vp, i64, vp, vp, vp], i32),
+        'sw_adjoint_hour': ([vp, vp, i32, i32, i32, vp, vp, vp, i32, i32, f64, vp], i32),
+        'sw_day_adjoint': ([vp, vp, i32, i32, i32, vp, vp, vp, i32, i32, f64, vp], i32),
         'sw_graph_begin': ([vp], i32), 'sw_graph_end': ([vp, ctypes.POINTER(vp)], i32),
         'sw_graph_launch': ([vp, vp], i32), 'sw_graph_destroy': ([vp], i32),
     }
@@ -101,7 +118,8 @@ EXPORTED = ('sw_version', 'sw_last_error', 'sw_device_count', 'sw_set_device', '
             'sw_stream_destroy', 'sw_stream_sync', 'sw_event_create', 'sw_event_destroy',
             'sw_event_record', 'sw_event_elapsed_ms', 'sw_device_sync', 'sw_svf_visibility',
             'sw_svf_reduce', 'sw_sky_sums', 'sw_hour_shadow', 'sw_hour_pixels', 'sw_hours_forward', 'sw_utci_scalar',
-            'sw_copy', 'sw_graph_begin', 'sw_graph_end', 'sw_graph_launch', 'sw_graph_destroy')
+            'sw_copy', 'sw_graph_begin', 'sw_graph_end', 'sw_graph_launch', 'sw_graph_destroy',
+            'sw_adjoint_hour', 'sw_day_adjoint')
 
 
 def check(rc):

@@ -346,8 +346,9 @@ def _geometry(round_trip):
     return alt, azi, ster, wcos, wsin, dirs, thr
 
 
-def patch_rows(day, esky, Ta, altitude, azimuth, lv=None, radD=0.0):
-    """One hour's patch table: float [153, SW_PATCH_F] and double [153, SW_PATCH_D]."""
+def patch_rows(day, esky, Ta, altitude, azimuth, lv=None, radD=0.0, f64=False):
+    """One hour's patch table: float [153, SW_PATCH_F] and double [153, SW_PATCH_D]
+    (``f64``: the float table left in FP64, for derivatives)."""
     alt, azi, ster, wcos, wsin, dirs, thr = _geometry(bool(day))
     f = np.zeros((SW_PATCHES, SW_PATCH_F))
     d = np.zeros((SW_PATCHES, SW_PATCH_D))
@@ -374,7 +375,7 @@ def patch_rows(day, esky, Ta, altitude, azimuth, lv=None, radD=0.0):
         f[:, PF['ELIGIBLE']] = ((diff > 90) & (diff < 270) & (altitude > 0)).astype(np.float64)
         yi = 2 * np.cos(diff * DEG) * np.tan(altitude * DEG)
         d[:, PD['YI']] = np.where(yi > 0, 0.0, yi)
-    return f.astype(np.float32), d
+    return (f if f64 else f.astype(np.float32)), d
 
 
 # ----------------------------------------------------------------------------- a day
@@ -392,10 +393,13 @@ class DayTables:
 
 
 def hour_tables(alt, azi, zen, jday, dectime, altmax, Ta, RH, radG, P, Ws, doy, latitude, shape, scale,
-                amaxvalue, state, *, wall_params=(0.37, -3.41, 15.0), constants=CONSTANTS, psi=None):
+                amaxvalue, state, *, wall_params=(0.37, -3.41, 15.0), constants=CONSTANTS, psi=None,
+                f64=False):
     """One step's hour record, patch rows and sun ray (Solweig_2022a_calc's host part,
     model.py:73-218). ``state`` = (CI, firstdaytime, timeadd, timestepdec) before the step; the
-    returned state is the one after it (the CI reset at midnight rows is the caller's)."""
+    returned state is the one after it (the CI reset at midnight rows is the caller's).
+    ``f64``: every value as a smooth FP64 function of the forcing (no float32 rounding of the
+    UTCI inputs or the patch table), for the derivatives of the adjoint."""
     CI, firstdaytime, timeadd, timestepdec = state
     c = constants
     tgk_w, tstart_w, tmax_w = wall_params
@@ -463,7 +467,7 @@ def hour_tables(alt, azi, zen, jday, dectime, altmax, Ta, RH, radG, P, Ws, doy, 
     h[H['LW_REFL']] = (1 - c['ewall']) * 0.5 / np.pi
     h[H['SW_SUNLIT']] = (c['albedo_b'] * (radI * np.cos(alt * DEG)) + (radD * 0.5)) / np.pi
     h[H['SW_SHADED']] = (c['albedo_b'] * radD * 0.5) / np.pi
-    ta32, rh32, va32 = np.float32(Ta), np.float32(RH), np.float32(Ws)
+    ta32, rh32, va32 = (Ta, RH, Ws) if f64 else (np.float32(Ta), np.float32(RH), np.float32(Ws))
     h[H['TA32']] = float(ta32)
     if ta32 <= -999 or rh32 <= -999 or va32 <= -999:
         h[H['UTCI0']:H['UTCI0'] + 7] = -999.0
@@ -471,17 +475,18 @@ def hour_tables(alt, azi, zen, jday, dectime, altmax, Ta, RH, radG, P, Ws, doy, 
         h[H['UTCI0']:H['UTCI0'] + 7] = _utci.coefficients(float(ta32), float(rh32), float(va32))
     h[H['ALBEDO_B']], h[H['ABSK']], h[H['ABSL']] = c['albedo_b'], c['absK'], c['absL']
     h[H['FUP']], h[H['FSIDE']], h[H['FCYL']] = c['Fup'], c['Fside'], c['Fcyl']
-    h[H['DIFF_VEG']], h[H['TGWALL']] = diff_veg(c['trans']), tgwall
+    h[H['DIFF_VEG']] = (1.0 - (1.0 - c['trans'])) if f64 else diff_veg(c['trans'])
+    h[H['TGWALL']] = tgwall
     h[H['SKY_A']] = SBC * ((Ta + 273.15) ** 4) / np.pi
     h[H['SKY_B']] = (1 - esky) * h[H['SKY_A']]
-    pf, pd = patch_rows(alt > 0, esky, Ta, alt, azi, lv, radD)
+    pf, pd = patch_rows(alt > 0, esky, Ta, alt, azi, lv, radD, f64=f64)
     meta = dict(CI=CI, esky=esky, radI=radI, radD=radD, tgwall=tgwall, I0=I0, ea=ea, CI_Tg=citg,
                 CI_TgG=citgg, lv=lv, firstdaytime=firstdaytime, timeadd=timeadd)
     return h, pf, pd, off, dz, (CI, firstdaytime, timeadd, timestepdec), meta
 
 
 def build_day(met, sun, latitude, shape, scale, amaxvalue, *, landcover=0, wall_params=(0.37, -3.41, 15.0),
-              constants=CONSTANTS, state=None):
+              constants=CONSTANTS, state=None, psi=None):
     """Tables for the forcing rows of one day (series._run_day, series.py:396-469)."""
     n = met.shape[0]
     dectime = sun['dectime']
@@ -495,10 +500,13 @@ def build_day(met, sun, latitude, shape, scale, amaxvalue, *, landcover=0, wall_
     for i in range(n):
         if dectime[i] - np.floor(dectime[i]) == 0:
             st = (1.,) + st[1:]
+        st_in = st
         h, pf[i], pd[i], off, dz, st, m = hour_tables(
             float(sun['altitude'][i]), float(sun['azimuth'][i]), float(sun['zen'][i]), float(sun['jday'][i]),
             float(dectime[i]), float(sun['altmax'][i]), *(float(met[i, k]) for k in (11, 10, 14, 12, 9)),
-            met[i, 1], latitude, shape, scale, amaxvalue, st, wall_params=wall_params, constants=constants)
+            met[i, 1], latitude, shape, scale, amaxvalue, st, wall_params=wall_params, constants=constants,
+            psi=psi)
+        m['state_in'] = st_in
         h[H['RAY_START']] = start
         h[H['PATCH_TABLE']] = i
         hours[i] = h

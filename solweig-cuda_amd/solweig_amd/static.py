@@ -50,8 +50,9 @@ def sky_sums(vis, P, stream=None):
 class PreparedDomain:
     """Device-resident static state of one raster domain."""
 
-    def __init__(self, domain, *, stream=None, keep_host=False):
+    def __init__(self, domain, *, stream=None, keep_host=False, trans=None):
         self.domain = domain
+        self.trans = float(CONSTANTS['trans'] if trans is None else trans)
         dsm = np.asarray(domain.dsm, dtype=np.float32)
         dem = np.asarray(domain.dem, dtype=np.float64)
         rows, cols = dsm.shape
@@ -100,7 +101,7 @@ class PreparedDomain:
         for key in ('alb', 'emis', 'tgk', 'tstart', 'tmax'):
             self.dev[key] = up(maps[key])
         sv = sky_view(self.dev['dsm'], self.dev['trees'], raise_, self.shape, self.amaxvalue,
-                      domain.scale, CONSTANTS['trans'], stream)
+                      domain.scale, self.trans, stream)
         self.skyview = sv
         self.dev['vis'] = sv.bits
         # The derived planes are formed on the host with NumPy so that tan(asvf) - which decides
@@ -108,13 +109,15 @@ class PreparedDomain:
         planes = sv.planes.to_host().reshape(16, rows, cols)
         svf = planes[0].astype(np.float32).astype(np.float64)
         svfveg = planes[5].astype(np.float32).astype(np.float64)
-        svfbuveg = svf - (1.0 - svfveg) * (1.0 - CONSTANTS['trans'])
+        self._svf, self._svfveg = svf, svfveg
+        svfbuveg = svf - (1.0 - svfveg) * (1.0 - self.trans)
         asvf = np.arccos(np.sqrt(svf))
         tmp = svf + svfveg - 1.0
         tmp[tmp < 0.0] = 0.0
         with np.errstate(divide='ignore'):
             svfalfa = np.arcsin(np.exp(np.log(1.0 - tmp) / 2.0))
         self.dev['svfbuveg'] = up(svfbuveg)
+        self.dev['svfveg'] = up(svfveg)
         self.dev['tan_asvf'] = up(np.tan(asvf), np.float64)
         self.dev['tan_svfalfa'] = up(np.tan(svfalfa), np.float64)
         self.dev['rec'] = rt.DeviceArray((self.P, 4), np.float32)
@@ -124,6 +127,22 @@ class PreparedDomain:
             self.host = dict(a=a, trees=trees, ground=ground, svf_planes=planes, svfbuveg=svfbuveg,
                              asvf=asvf, svfalfa=svfalfa, maps=maps)
         self.tiles = ((cols + rt.TILE_X - 1) // rt.TILE_X) * ((rows + rt.TILE_Y - 1) // rt.TILE_Y)
+
+    def set_transmissivity(self, trans, stream=None):
+        """Canopy transmissivity of the static svfbuveg plane (series.py:343)."""
+        self.trans = float(trans)
+        svfbuveg = self._svf - (1.0 - self._svfveg) * (1.0 - self.trans)
+        self.dev['svfbuveg'].upload(svfbuveg, stream)
+
+    def set_maps(self, alb=None, emis=None, stream=None):
+        """Replace the ground albedo / emissivity maps (host arrays or device arrays)."""
+        for key, v in (('alb', alb), ('emis', emis)):
+            if v is None:
+                continue
+            if isinstance(v, (rt.DeviceArray, rt.DeviceView)):
+                rt.call('sw_memcpy_d2d', self.dev[key].ptr, v.ptr, self.dev[key].nbytes, rt._s(stream))
+            else:
+                self.dev[key].upload(np.asarray(v, dtype=np.float32).reshape(self.shape), stream)
 
     def struct(self, **extra):
         """The sw_domain record for this domain (extra: hours, ptab_f, ... device pointers)."""

@@ -55,3 +55,18 @@ def test_domain_struct_is_264_bytes():
     fields = re.findall(r'\*(\w+)', header_text()[header_text().index('typedef struct sw_domain'):])
     names = [f[0] for f in rt.SwDomain._fields_][6:]
     assert fields[:len(names)] == names
+
+
+def test_adjoint_slots_and_struct_match_header():
+    vals = _enum_values('SWA_TA = 0')
+    for name, idx in rt.ADJ.items():
+        assert vals['SWA_' + name] == idx, name
+    assert vals['SWA_SLOTS'] == rt.ADJ_SLOTS
+    pv = _enum_values('SWA_PATCH_SKY_KDOWN = 0')
+    for name, idx in rt.ADJ_PATCH.items():
+        assert pv['SWA_PATCH_' + name] == idx
+    assert ctypes.sizeof(rt.SwAdjoint) == 128
+    text = header_text()
+    block = text[text.index('typedef struct sw_adjoint'):text.index('} sw_adjoint;')]
+    fields = re.findall(r'\*(\w+);', block)
+    assert fields == [f[0] for f in rt.SwAdjoint._fields_][:-1]

@@ -1,0 +1,113 @@
+"""Adjoint of J = mean over ground pixels of UTCI at 14:00 (GPU).
+
+The device gradient (FP32 chain on the device, FP64 host chain through the hour tables) is
+checked against central differences of the FP64 oracle (the FP64 reference gradient) on a small
+domain: per-pixel albedo and emissivity at sampled pixels, wall albedo, canopy transmissivity
+and the hourly Ta, RH, wind and global radiation at several hours. J itself must equal the
+oracle's within the forward tolerance.
+"""
+import numpy as np
+import pytest
+
+from oracle import day as oday
+
+pytestmark = pytest.mark.gpu
+
+H_OBJ = 14
+RTOL = 2e-2      # relative agreement of the gradient with the FP64 central differences
+
+
+@pytest.fixture(scope='module')
+def rt():
+    from solweig_amd import runtime
+    if runtime.device_count() < 1:
+        pytest.skip('no GPU')
+    return runtime
+
+
+@pytest.fixture(scope='module')
+def case(rt):
+    from solweig_amd import adjoint, synthetic, sun as psun
+    d = synthetic.small_domain(seed=21, size=30, maps=True)
+    print(f'adjoint case: {d.shape}, objective hour {H_OBJ}', flush=True)
+    prob = adjoint.UtciGradient(d, objective_hour=H_OBJ)
+    g = prob.gradient()
+    s = psun.met_sun(d.met, d.location, d.utc)
+    p = oday.prepare(d.dsm, d.trees, d.dem, d.walls, d.aspect, d.latitude, alb_grid=d.alb_grid,
+                     emis_grid=d.emis_grid)
+    return d, prob, g, s, p
+
+
+def _oracle_J(p, met, s, trans=None):
+    q = p if trans is None else oday.with_transmissivity(p, trans)
+    psi = None if trans is None else trans
+    sub = {k: np.asarray(v)[:H_OBJ + 1] for k, v in s.items()}
+    out = oday.run_day(q, met[:H_OBJ + 1], sub, keep=(), psi=psi if psi is not None else 0.03)
+    return float(np.nanmean(out[H_OBJ]['UTCI']))
+
+
+def _close(dev, ref, scale, what):
+    err = abs(dev - ref)
+    print(f'  {what}: device {dev:+.6e}  FP64 FD {ref:+.6e}  |diff| {err:.2e}', flush=True)
+    assert err <= RTOL * abs(ref) + 1e-3 * scale, what
+
+
+def test_objective_matches_oracle(case):
+    d, prob, g, s, p = case
+    J_ref = _oracle_J(p, d.met, s)
+    print(f'J device {g["J"]:.6f}  oracle {J_ref:.6f}', flush=True)
+    assert abs(g['J'] - J_ref) < 1e-3
+    assert abs(prob.objective() - g['J']) < 1e-4
+
+
+def test_scalar_gradients_match_fp64_differences(case):
+    d, prob, g, s, p = case
+    scale = max(abs(g['albedo_b']), abs(g['trans']), np.abs(g['Ta']).max(), np.abs(g['radG']).max())
+    for name, step in (('albedo_b', 1e-4), ('trans', 1e-4)):
+        if name == 'albedo_b':
+            Jp = _oracle_J(dict(p, albedo_b=p['albedo_b'] + step), d.met, s)
+            Jm = _oracle_J(dict(p, albedo_b=p['albedo_b'] - step), d.met, s)
+        else:
+            Jp = _oracle_J(p, d.met, s, trans=0.03 + step)
+            Jm = _oracle_J(p, d.met, s, trans=0.03 - step)
+        _close(g[name], (Jp - Jm) / (2 * step), scale, name)
+    for name, col, step in (('Ta', 11, 1e-2), ('RH', 10, 1e-2), ('Ws', 9, 1e-3), ('radG', 14, 1e-2)):
+        for h in (H_OBJ, H_OBJ - 2, H_OBJ - 5):
+            mp, mm = d.met.copy(), d.met.copy()
+            mp[h, col] += step
+            mm[h, col] -= step
+            ref = (_oracle_J(p, mp, s) - _oracle_J(p, mm, s)) / (2 * step)
+            _close(g[name][h], ref, scale, f'{name}[{h}]')
+    assert np.all(g['Ta'][H_OBJ + 1:] == 0)
+
+
+def test_map_gradients_match_fp64_differences(case):
+    d, prob, g, s, p = case
+    rng = np.random.default_rng(33)
+    ground = np.argwhere(p['buildings'] == 1)
+    picks = ground[rng.choice(len(ground), size=6, replace=False)]
+    for key, grid in (('alb', 'alb_grid'), ('emis', 'emis_grid')):
+        scale = float(np.abs(g[key]).max())
+        for r, c in picks:
+            step = 1e-3
+            up, dn = p[grid].copy(), p[grid].copy()
+            up[r, c] += step
+            dn[r, c] -= step
+            ref = (_oracle_J(dict(p, **{grid: up}), d.met, s) - _oracle_J(dict(p, **{grid: dn}), d.met, s)) / (2 * step)
+            _close(float(g[key][r, c]), ref, scale, f'{key}[{r},{c}]')
+
+
+def test_torch_autograd_wrapper(case):
+    torch = pytest.importorskip('torch')
+    from solweig_amd.autograd import utci_objective
+    d, prob, g, s, p = case
+    dev = torch.device('cuda') if torch.cuda.is_available() else torch.device('cpu')
+    alb = torch.tensor(d.alb_grid, device=dev, requires_grad=True)
+    emis = torch.tensor(d.emis_grid, device=dev, requires_grad=True)
+    Ta = torch.tensor(d.met[:, 11], dtype=torch.float64, requires_grad=True)
+    J = utci_objective(prob, alb, emis, 0.2, 0.03, Ta, d.met[:, 10], d.met[:, 9], d.met[:, 14])
+    J.backward()
+    assert abs(float(J) - g['J']) < 1e-5
+    np.testing.assert_allclose(alb.grad.cpu().numpy(), g['alb'], rtol=1e-5, atol=1e-9)
+    np.testing.assert_allclose(emis.grad.cpu().numpy(), g['emis'], rtol=1e-5, atol=1e-9)
+    np.testing.assert_allclose(Ta.grad.numpy(), g['Ta'], rtol=1e-6, atol=1e-12)
