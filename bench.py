@@ -60,6 +60,42 @@ def cpu_baseline(hours=(10, 11, 12, 13), size=200):
                        f'({px_h} px-h in {dt:.1f} s; static SVF prep {t_static:.1f} s excluded)')
 
 
+def adjoint_leg(size=1000, repeats=3):
+    """Forward vs forward + adjoint on the 1000x1000 differentiation workload (C3 morphology with
+    per-pixel albedo U(0.1, 0.4) and emissivity U(0.90, 0.98) maps; J = mean ground UTCI at 14:00)."""
+    from solweig_amd import adjoint, synthetic
+    d = synthetic.make_c3(size=size)
+    prob = adjoint.UtciGradient(d, objective_hour=14)
+    eng = prob.eng
+
+    def forward_day():
+        _, tabs = prob._tables()
+        eng.load(tabs)
+        eng.reset_state()
+        eng.forward_day()
+        eng.stream.sync()
+
+    def timed(fn):
+        fn()
+        ts = []
+        for _ in range(repeats):
+            t0 = time.perf_counter()
+            out = fn()
+            ts.append(time.perf_counter() - t0)
+        return float(np.median(ts)) * 1e3, out
+
+    t_fwd, _ = timed(forward_day)
+    t_obj, J = timed(prob.objective)
+    t_grad, g = timed(prob.gradient)
+    log(f'[bench] adjoint leg: forward day {t_fwd:.1f} ms, objective {t_obj:.1f} ms, gradient {t_grad:.1f} ms')
+    return dict(workload=f'c3 {size}x{size}, 24 h forcing, J = mean UTCI over ground pixels at 14:00; '
+                         f'gradient w.r.t. albedo and emissivity maps, wall albedo, transmissivity and hourly '
+                         f'Ta, RH, wind, radG',
+                forward_day_ms=t_fwd, objective_ms=t_obj, gradient_ms=t_grad,
+                gradient_over_forward=t_grad / t_fwd, forward_plus_adjoint_over_forward=(t_obj + t_grad) / t_fwd,
+                J=g['J'], hours=list(g['hours']))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
@@ -70,6 +106,7 @@ def main():
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--copy-calibration', action='store_true',
                     help='run a 256 MiB streaming copy (sw_copy) 3 times for HBM counter scaling')
+    ap.add_argument('--no-adjoint', action='store_true', help='skip the forward + adjoint leg')
     ap.add_argument('--plain-days', type=int, default=1,
                     help='also time the plain per-step forward for this many days (0 = skip)')
     args = ap.parse_args()
@@ -187,6 +224,7 @@ def main():
             rt.call('sw_copy', dst.ptr, src.ptr, src.nbytes, eng.stream.handle)
         eng.stream.sync()
         log('[bench] copy calibration: 3 x 256 MiB read + 256 MiB written')
+    adj = None if args.no_adjoint else adjoint_leg()
     cpu = None if args.no_cpu else cpu_baseline()
     out = dict(metric='Tmrt+UTCI pixel-hours/s at 2000^2 x24h, 1 GPU; batched forward',
                value=value, unit='pixel-hours/s', n_gpus=1, steps=args.steps, warmup=args.warmup,
@@ -195,7 +233,8 @@ def main():
                config=dict(workload=f'c2 batched forward {args.size}x{args.size}, 24 h per step '
                                     f'(30-day forcing cycled), host day tables inside the timed region '
                                     f'(built for day k+1 while the device runs day k)',
-                           graph=bool(args.graph), static_prep_s=t_static, plain_forward=plain),
+                           graph=bool(args.graph), static_prep_s=t_static, plain_forward=plain,
+                           forward_adjoint=adj),
                roofline=roof, cpu_baseline=cpu)
     print(json.dumps(out), flush=True)
 

@@ -132,9 +132,11 @@ class UtciGradient:
                 self.eng.first, self.eng.second, 1.0 / self.count, self.stream.handle)
 
     def gradient(self):
-        """J and dJ/d every parameter (host arrays / floats)."""
+        """J and dJ/d every parameter (host arrays / floats). The derivatives of the hour tables
+        are taken on the host while the device runs the sweep."""
         self.run_device()
         sun, t, first = self._last
+        jac = {h: self._jacobian(sun, t, h, h == self.h_obj) for h in range(first, self.h_obj + 1)}
         self.stream.sync()
         obj = self.buf['objective'].to_host()
         scal = self.buf['scalars'].to_host()
@@ -149,11 +151,41 @@ class UtciGradient:
         out['trans'] = float(scal[:, rt.ADJ['TRANS_STATIC']].sum())
         for h in range(first, self.h_obj + 1):
             dv = self._value_gradient(scal[h], patch if h == self.h_obj else None, t, h)
-            for name in FORCING:
-                out[name][h] += self._chain(sun, t, h, dv, name)
-            out['albedo_b'] += self._chain(sun, t, h, dv, 'albedo_b')
-            out['trans'] += self._chain(sun, t, h, dv, 'trans')
+            for name, col in jac[h].items():
+                if name in FORCING:
+                    out[name][h] += float(dv @ col)
+                else:
+                    out[name] += float(dv @ col)
         return out
+
+    def _jacobian(self, sun, t, h, top):
+        """d(value vector)/d(parameter) of hour h. Ta, RH, radG (and Ws at the objective hour,
+        the only hour whose UTCI counts) by central differences of the FP64 table builder; the
+        wall albedo and the transmissivity enter the values linearly and are written out."""
+        row = self.met[h]
+        base = dict(Ta=float(row[11]), RH=float(row[10]), Ws=float(row[9]), radG=float(row[14]),
+                    albedo_b=self.albedo_b, trans=self.trans)
+        names = ['Ta', 'RH', 'radG'] + (['Ws'] if top else [])
+        cols = {}
+        for name in names:
+            step = STEPS[name]
+            hi, lo = dict(base), dict(base)
+            hi[name] += step
+            lo[name] -= step
+            vh = self._vector(sun, t, h, hi['Ta'], hi['RH'], hi['Ws'], hi['radG'], hi['albedo_b'], hi['trans'], top)
+            vl = self._vector(sun, t, h, lo['Ta'], lo['RH'], lo['Ws'], lo['radG'], lo['albedo_b'], lo['trans'], top)
+            cols[name] = (vh - vl) / (2.0 * step)
+        n = len(VALUES) + 7 + (2 * rt.SW_PATCHES + 1 if top else 0)
+        hh = t.hours[h]
+        ab, tr = np.zeros(n), np.zeros(n)
+        ab[VALUES.index('ALBEDO_B')] = 1.0
+        ab[VALUES.index('SW_SUNLIT')] = hh[rt.H['RADI']] * hh[rt.H['COSALT']] / np.pi
+        ab[VALUES.index('SW_SHADED')] = hh[rt.H['RADD']] * 0.5 / np.pi
+        tr[VALUES.index('PSI')] = 1.0
+        if top:
+            tr[-1] = 1.0                                   # DIFF_VEG = 1 - (1 - trans)
+        cols['albedo_b'], cols['trans'] = ab, tr
+        return cols
 
     # ------------------------------------------------------------------ host chain
     def _vector(self, sun, t, h, Ta, RH, Ws, radG, albedo_b, trans, with_patches):
@@ -164,7 +196,8 @@ class UtciGradient:
             float(sun['altitude'][h]), float(sun['azimuth'][h]), float(sun['zen'][h]), float(sun['jday'][h]),
             float(sun['dectime'][h]), float(sun['altmax'][h]), Ta, RH, radG, float(row[12]), Ws, row[1],
             self.domain.latitude, self.pd.shape, self.domain.scale, -1.0, m['state_in'],
-            wall_params=self.pd.wall_params, constants=self._constants(albedo_b, trans), psi=trans, f64=True)
+            wall_params=self.pd.wall_params, constants=self._constants(albedo_b, trans), psi=trans, f64=True,
+            rows=with_patches)
         v = [hh[rt.H[k]] for k in VALUES] + list(hh[rt.H['UTCI0']:rt.H['UTCI0'] + 7])
         if with_patches:
             v += list(pf[:, rt.PF['LV']]) + list(pf[:, rt.PF['KD']]) + [hh[rt.H['DIFF_VEG']]]
@@ -184,17 +217,3 @@ class UtciGradient:
             g_dv = radD * float(lv @ patch[P['VEG_KDOWN']]) + float(kd @ patch[P['VEG_KSIDE']])
             g += list(g_lv) + list(g_kd) + [g_dv]
         return np.asarray(g, dtype=np.float64)
-
-    def _chain(self, sun, t, h, dv, name):
-        """sum_i dJ/dv_i dv_i/d(name) at hour h, dv/d(name) by central differences in FP64."""
-        row = self.met[h]
-        base = dict(Ta=float(row[11]), RH=float(row[10]), Ws=float(row[9]), radG=float(row[14]),
-                    albedo_b=self.albedo_b, trans=self.trans)
-        step = STEPS[name]
-        with_p = len(dv) > len(VALUES) + 7
-        hi, lo = dict(base), dict(base)
-        hi[name] += step
-        lo[name] -= step
-        vh = self._vector(sun, t, h, hi['Ta'], hi['RH'], hi['Ws'], hi['radG'], hi['albedo_b'], hi['trans'], with_p)
-        vl = self._vector(sun, t, h, lo['Ta'], lo['RH'], lo['Ws'], lo['radG'], lo['albedo_b'], lo['trans'], with_p)
-        return float(dv @ ((vh - vl) / (2.0 * step)))

@@ -394,12 +394,13 @@ class DayTables:
 
 def hour_tables(alt, azi, zen, jday, dectime, altmax, Ta, RH, radG, P, Ws, doy, latitude, shape, scale,
                 amaxvalue, state, *, wall_params=(0.37, -3.41, 15.0), constants=CONSTANTS, psi=None,
-                f64=False):
+                f64=False, rows=True):
     """One step's hour record, patch rows and sun ray (Solweig_2022a_calc's host part,
     model.py:73-218). ``state`` = (CI, firstdaytime, timeadd, timestepdec) before the step; the
     returned state is the one after it (the CI reset at midnight rows is the caller's).
     ``f64``: every value as a smooth FP64 function of the forcing (no float32 rounding of the
-    UTCI inputs or the patch table), for the derivatives of the adjoint."""
+    UTCI inputs or the patch table), for the derivatives of the adjoint; ``rows=False`` skips the
+    Perez luminance and the patch rows (returned as None)."""
     CI, firstdaytime, timeadd, timestepdec = state
     c = constants
     tgk_w, tstart_w, tmax_w = wall_params
@@ -420,7 +421,8 @@ def hour_tables(alt, azi, zen, jday, dectime, altmax, Ta, RH, radG, P, Ws, doy, 
         if (CI > 1) or (CI == np.inf):
             CI = 1
         radI, radD = diffuse_fraction(radG, alt, Kt, Ta, RH)
-        _, _, lv = perez(zen * (180 / np.pi), azi, radD, radI, jday)
+        if rows:
+            _, _, lv = perez(zen * (180 / np.pi), azi, radD, radI, jday)
         radI0, radD0 = diffuse_fraction(I0, alt, 1., Ta, RH)
         corr = 0.1473 * np.log(90 - (zen / np.pi * 180)) + 0.3454
         citg = (radG / radI0) + (1 - corr)
@@ -479,7 +481,7 @@ def hour_tables(alt, azi, zen, jday, dectime, altmax, Ta, RH, radG, P, Ws, doy, 
     h[H['TGWALL']] = tgwall
     h[H['SKY_A']] = SBC * ((Ta + 273.15) ** 4) / np.pi
     h[H['SKY_B']] = (1 - esky) * h[H['SKY_A']]
-    pf, pd = patch_rows(alt > 0, esky, Ta, alt, azi, lv, radD, f64=f64)
+    pf, pd = patch_rows(alt > 0, esky, Ta, alt, azi, lv, radD, f64=f64) if rows else (None, None)
     meta = dict(CI=CI, esky=esky, radI=radI, radD=radD, tgwall=tgwall, I0=I0, ea=ea, CI_Tg=citg,
                 CI_TgG=citgg, lv=lv, firstdaytime=firstdaytime, timeadd=timeadd)
     return h, pf, pd, off, dz, (CI, firstdaytime, timeadd, timestepdec), meta
