@@ -107,6 +107,7 @@ def main():
     ap.add_argument('--copy-calibration', action='store_true',
                     help='run a 256 MiB streaming copy (sw_copy) 3 times for HBM counter scaling')
     ap.add_argument('--no-adjoint', action='store_true', help='skip the forward + adjoint leg')
+    ap.add_argument('--no-tune', action='store_true', help='keep the default 32x8 tile (no autotuning)')
     ap.add_argument('--plain-days', type=int, default=1,
                     help='also time the plain per-step forward for this many days (0 = skip)')
     args = ap.parse_args()
@@ -127,6 +128,11 @@ def main():
     t_static = time.perf_counter() - t0
     log(f'[bench] static prep {t_static:.2f} s')
     P = eng.domains[0].P
+    tuned = None
+    if not args.no_tune:
+        from solweig_amd import autotune
+        tuned = autotune.tune(eng, eng.tables(day_index=0), use_cache=False)
+        log(f'[bench] autotuned tile width {tuned["tile_cols"]} (ms over hours {tuned.get("hours")}: {tuned["ms"]})')
 
     def launch_day(evs):
         """Queue the 24 hours of the loaded day; with events, one per pass boundary."""
@@ -233,7 +239,7 @@ def main():
                config=dict(workload=f'c2 batched forward {args.size}x{args.size}, 24 h per step '
                                     f'(30-day forcing cycled), host day tables inside the timed region '
                                     f'(built for day k+1 while the device runs day k)',
-                           graph=bool(args.graph), static_prep_s=t_static, plain_forward=plain,
+                           graph=bool(args.graph), static_prep_s=t_static, tile=tuned, plain_forward=plain,
                            forward_adjoint=adj),
                roofline=roof, cpu_baseline=cpu)
     print(json.dumps(out), flush=True)

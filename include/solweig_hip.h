@@ -69,7 +69,9 @@ enum {
  * pointers. Optional outputs may be NULL. */
 typedef struct sw_domain {
     int64_t rows, cols, pixels, landcover;
-    double dsm_raise, reserved0;
+    double dsm_raise;
+    int64_t tile_cols;    /* workgroup tile: tile_cols x (256 / tile_cols) pixels, tile_cols one of
+                             16, 32, 64, 128 (0 means 32); block_start counts these tiles       */
     const float *dsm, *trees, *walls, *aspect;               /* [P]; trees clipped >= 0       */
     const float *emis, *alb, *tgk, *tstart, *tmaxlst, *svfbuveg; /* [P]                      */
     const double *tan_asvf, *tan_svfalfa;                    /* [P]                           */

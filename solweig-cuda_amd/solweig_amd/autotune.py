@@ -1,0 +1,93 @@
+"""Tile-width autotuner of the hour kernels.
+
+Every hour kernel maps a 256-thread workgroup to a tile of W x (256 / W) pixels, W in
+runtime.TILE_CHOICES; a wave then covers 64 / W rows of W pixels. The width trades the
+coalescing of the neighbour gathers (wide rows) against the L1/L2 reuse of the walks and rays
+between the rows of a wave (tall tiles). ``tune`` times the shadow + pixel pass of representative
+hours of the loaded day (the lowest and the highest daytime sun, where the ray march and the
+ground-view walk respectively dominate) for every width on the actual domains, keeps the fastest,
+and remembers it per domain shape in a small JSON file.
+"""
+from __future__ import annotations
+
+import json
+import os
+
+import numpy as np
+
+from . import runtime as rt
+
+CACHE = os.environ.get('SOLWEIG_AMD_TUNE_CACHE',
+                       os.path.join(os.path.expanduser('~'), '.cache', 'solweig_amd', 'tiles.json'))
+
+
+def _key(engine):
+    return ';'.join(f'{d.shape[0]}x{d.shape[1]}' for d in engine.domains) + f'|h{engine.n_hours}'
+
+
+def _load_cache():
+    try:
+        with open(CACHE) as f:
+            return json.load(f)
+    except (OSError, ValueError):
+        return {}
+
+
+def _store_cache(key, value):
+    data = _load_cache()
+    data[key] = value
+    try:
+        os.makedirs(os.path.dirname(CACHE), exist_ok=True)
+        with open(CACHE, 'w') as f:
+            json.dump(data, f, indent=1)
+    except OSError:
+        pass
+
+
+def representative_hours(tables):
+    """Daytime hours with the lowest and the highest sun of the first domain's day."""
+    h = tables[0].hours
+    day = np.flatnonzero(h[:, rt.H['DAY']] != 0)
+    if len(day) == 0:
+        return []
+    alt = h[day, rt.H['ALTITUDE']]
+    return sorted({int(day[np.argmin(alt)]), int(day[np.argmax(alt)])})
+
+
+def tune(engine, tables=None, candidates=rt.TILE_CHOICES, repeats=3, use_cache=True):
+    """Pick the tile width of ``engine`` (all domains alike); returns {'tile_cols', 'ms'}.
+    The day's tables are loaded into the engine; the lag state is cleared afterwards."""
+    key = _key(engine)
+    if use_cache:
+        hit = _load_cache().get(key)
+        if hit and hit.get('tile_cols') in candidates:
+            engine.set_tiles(hit['tile_cols'])
+            return dict(hit, cached=True)
+    tables = tables if tables is not None else engine.tables(day_index=0)
+    engine.load(tables)
+    hours = representative_hours(tables)
+    if not hours:
+        return dict(tile_cols=engine.domains[0].tile_cols, ms={}, cached=False)
+    ev = [rt.Event() for _ in range(2)]
+    times = {}
+    for w in candidates:
+        engine.set_tiles(w)
+        engine.reset_state()
+        for h in hours:                                  # warm-up of this launch geometry
+            engine.hour(h)
+        best = float('inf')
+        for _ in range(repeats):
+            ev[0].record(engine.stream)
+            for h in hours:
+                engine.hour(h)
+            ev[1].record(engine.stream)
+            engine.stream.sync()
+            best = min(best, ev[0].elapsed_ms(ev[1]))
+        times[int(w)] = best
+    choice = min(times, key=times.get)
+    engine.set_tiles(choice)
+    engine.reset_state()
+    result = dict(tile_cols=int(choice), ms=times, hours=hours)
+    if use_cache:
+        _store_cache(key, result)
+    return dict(result, cached=False)

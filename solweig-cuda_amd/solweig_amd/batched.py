@@ -39,16 +39,10 @@ class Engine:
                 b['tmrt'] = rt.DeviceArray((self.n_hours, d.P), np.float32)
                 b['utci'] = rt.DeviceArray((self.n_hours, d.P), np.float32)
             self.buf.append(b)
-        starts = np.zeros(len(self.domains) + 1, dtype=np.int32)
-        starts[1:] = np.cumsum([d.tiles for d in self.domains])
-        self.total_blocks = int(starts[-1])
-        self.block_start = rt.DeviceArray.from_host(starts)
+        self.block_start = rt.DeviceArray((len(self.domains) + 1,), np.int32)
         self.structs = (rt.SwDomain * len(self.domains))()
-        for i, (d, b) in enumerate(zip(self.domains, self.buf)):
-            self.structs[i] = d.struct(hours=b['hours'].ptr, ptab_f=b['ptab_f'].ptr, ptab_d=b['ptab_d'].ptr,
-                                       ray_off=b['ray_off'].ptr, ray_dz=b['ray_dz'].ptr,
-                                       tmrt=b['tmrt'].ptr if 'tmrt' in b else 0,
-                                       utci=b['utci'].ptr if 'utci' in b else 0)
+        self.dom_dev = rt.DeviceArray((ctypes.sizeof(self.structs),), np.uint8)
+        self._write_domains()
         # two page-locked staging slots per domain: the host fills one while the copies from the
         # other may still be queued behind the previous day's launches
         self._stage = [[dict(hours=rt.PinnedArray(b['hours'].shape, np.float64),
@@ -60,11 +54,32 @@ class Engine:
         self._stage_event = [rt.Event(), rt.Event()]
         self._stage_used = [False, False]
         self._slot = 0
-        self.dom_dev = rt.DeviceArray((ctypes.sizeof(self.structs),), np.uint8)
+        self.graph = None
+
+    def _write_domains(self):
+        """Upload the domain records and the tile prefix (after a change of tile widths)."""
+        starts = np.zeros(len(self.domains) + 1, dtype=np.int32)
+        starts[1:] = np.cumsum([d.tiles for d in self.domains])
+        self.total_blocks = int(starts[-1])
+        self.block_start.upload(starts, self.stream)
+        for i, (d, b) in enumerate(zip(self.domains, self.buf)):
+            self.structs[i] = d.struct(hours=b['hours'].ptr, ptab_f=b['ptab_f'].ptr, ptab_d=b['ptab_d'].ptr,
+                                       ray_off=b['ray_off'].ptr, ray_dz=b['ray_dz'].ptr,
+                                       tmrt=b['tmrt'].ptr if 'tmrt' in b else 0,
+                                       utci=b['utci'].ptr if 'utci' in b else 0)
         rt.call('sw_memcpy_h2d', self.dom_dev.ptr, ctypes.addressof(self.structs),
                 ctypes.sizeof(self.structs), self.stream.handle)
         self.stream.sync()
-        self.graph = None
+        self.graph = None                                # launch geometry changed
+
+    def set_tiles(self, tile_cols):
+        """Tile width per domain (one value for all, or a list), from rt.TILE_CHOICES."""
+        widths = [tile_cols] * len(self.domains) if np.ndim(tile_cols) == 0 else list(tile_cols)
+        for d, w in zip(self.domains, widths):
+            if int(w) not in rt.TILE_CHOICES:
+                raise ValueError(f'tile width {w} not in {rt.TILE_CHOICES}')
+            d.tile_cols = int(w)
+        self._write_domains()
 
     # ------------------------------------------------------------------ tables
     def tables(self, day_index=0, met=None, sun=None, constants=None, psi=None):

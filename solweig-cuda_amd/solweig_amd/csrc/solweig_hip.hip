@@ -77,6 +77,7 @@ typedef float v4f __attribute__((ext_vector_type(4)));
 struct DomainG {
     int64_t rows, cols, pixels, landcover;
     double dsm_raise;
+    int tile_shift;                 // log2 of the tile width
     const AS1 float *dsm, *trees, *walls, *aspect, *emis, *alb, *tgk, *tstart, *tmaxlst, *svfbuveg;
     const AS1 double *tan_asvf, *tan_svfalfa;
     const AS1 uint8_t *ground;
@@ -99,6 +100,8 @@ __device__ __forceinline__ DomainG domain_at(const sw_domain *D) {
     DomainG g;
     g.rows = d->rows; g.cols = d->cols; g.pixels = d->pixels; g.landcover = d->landcover;
     g.dsm_raise = d->dsm_raise;
+    const int64_t tc = d->tile_cols;
+    g.tile_shift = tc == 16 ? 4 : (tc == 64 ? 6 : (tc == 128 ? 7 : 5));
     g.dsm = (const AS1 float *)d->dsm; g.trees = (const AS1 float *)d->trees;
     g.walls = (const AS1 float *)d->walls; g.aspect = (const AS1 float *)d->aspect;
     g.emis = (const AS1 float *)d->emis; g.alb = (const AS1 float *)d->alb;
@@ -143,9 +146,10 @@ __device__ __forceinline__ PixelAt locate(const sw_domain *doms, const int32_t *
     G = domain_at(doms + at.d);
     const int t = b - bs[at.d];
     const int cols = (int)G.cols, rows = (int)G.rows;
-    const int tpr = (cols + TILE_X - 1) / TILE_X;
-    at.r = (t / tpr) * TILE_Y + (int)threadIdx.x / TILE_X;
-    at.c = (t % tpr) * TILE_X + (int)threadIdx.x % TILE_X;
+    const int sh = G.tile_shift, tx = 1 << sh, ty = BLOCK >> sh;
+    const int tpr = (cols + tx - 1) >> sh;
+    at.r = (t / tpr) * ty + ((int)threadIdx.x >> sh);
+    at.c = (t % tpr) * tx + ((int)threadIdx.x & (tx - 1));
     at.valid = at.r < rows && at.c < cols;
     at.p = (long long)at.r * cols + at.c;
     return at;

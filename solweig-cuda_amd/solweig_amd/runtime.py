@@ -19,7 +19,14 @@ SW_PATCH_F = 12
 SW_PATCH_D = 4
 SW_PATCHES = 153
 SW_VIS_WORDS = 5
-TILE_X, TILE_Y = 32, 8
+TILE_X, TILE_Y = 32, 8          # default workgroup tile (256 threads)
+TILE_CHOICES = (16, 32, 64, 128)  # tile widths the hour kernels accept (height = 256 / width)
+
+
+def tile_count(shape, tile_cols=TILE_X):
+    rows, cols = shape
+    ty = 256 // tile_cols
+    return ((cols + tile_cols - 1) // tile_cols) * ((rows + ty - 1) // ty)
 
 # hour record fields (enum order of solweig_hip.h)
 _HOUR_FIELDS = ('DAY ALTITUDE AZIMUTH TANZEN TA RADG RADD RADI PSI SINALT COSALT ALTMAX FRAC SNUP '
@@ -40,7 +47,7 @@ FULL_PLANES = ('Kdown Kup Ldown Lup Tg Keast Ksouth Kwest Knorth Least Lsouth Lw
 class SwDomain(ctypes.Structure):
     """Mirror of sw_domain: every field 8 bytes."""
     _fields_ = [(n, ctypes.c_int64) for n in ('rows', 'cols', 'pixels', 'landcover')] + \
-        [('dsm_raise', ctypes.c_double), ('reserved0', ctypes.c_double)] + \
+        [('dsm_raise', ctypes.c_double), ('tile_cols', ctypes.c_int64)] + \
         [(n, ctypes.c_uint64) for n in ('dsm', 'trees', 'walls', 'aspect', 'emis', 'alb', 'tgk', 'tstart',
                                         'tmaxlst', 'svfbuveg', 'tan_asvf', 'tan_svfalfa', 'ground', 'vis',
                                         'rec', 'lag', 'tmrt', 'utci', 'hours', 'ptab_f', 'ptab_d',

@@ -126,7 +126,12 @@ class PreparedDomain:
         if keep_host:
             self.host = dict(a=a, trees=trees, ground=ground, svf_planes=planes, svfbuveg=svfbuveg,
                              asvf=asvf, svfalfa=svfalfa, maps=maps)
-        self.tiles = ((cols + rt.TILE_X - 1) // rt.TILE_X) * ((rows + rt.TILE_Y - 1) // rt.TILE_Y)
+        self.tile_cols = rt.TILE_X
+
+    @property
+    def tiles(self):
+        """Workgroup tiles of this domain at its tile width."""
+        return rt.tile_count(self.shape, self.tile_cols)
 
     def set_transmissivity(self, trans, stream=None):
         """Canopy transmissivity of the static svfbuveg plane (series.py:343)."""
@@ -150,6 +155,7 @@ class PreparedDomain:
         s = rt.SwDomain()
         s.rows, s.cols, s.pixels, s.landcover = self.shape[0], self.shape[1], self.P, self.landcover
         s.dsm_raise = self.dsm_raise
+        s.tile_cols = self.tile_cols
         for field, key in (('dsm', 'dsm'), ('trees', 'trees'), ('walls', 'walls'), ('aspect', 'aspect'),
                            ('emis', 'emis'), ('alb', 'alb'), ('tgk', 'tgk'), ('tstart', 'tstart'),
                            ('tmaxlst', 'tmax'), ('svfbuveg', 'svfbuveg'), ('tan_asvf', 'tan_asvf'),
@@ -208,5 +214,5 @@ def prepared_from_arrays(*, dsm, vegdem, walls, dirwalls, buildings, svfbuveg, a
                     lag=rt.DeviceArray((6, rows * cols), np.float32))
     self.dev['skysums'] = sky_sums(self.dev['vis'], rows * cols, stream)
     self.skyview = None
-    self.tiles = ((cols + rt.TILE_X - 1) // rt.TILE_X) * ((rows + rt.TILE_Y - 1) // rt.TILE_Y)
+    self.tile_cols = rt.TILE_X
     return self

@@ -72,3 +72,39 @@ def test_graph_replays_match_eager(rt):
         t1, u1 = eng.outputs(0)
         assert np.array_equal(t0, t1, equal_nan=True) and np.array_equal(u0, u1, equal_nan=True)
     assert eng.graph is not None
+
+
+def test_tile_widths_give_identical_days(rt):
+    """Every tile width the autotuner may pick gives the same bits (37 x 53 raster: partial tiles)."""
+    from solweig_amd import batched, synthetic
+    d = synthetic.small_domain(seed=61, size=53, maps=True)
+    d.dsm, d.trees, d.dem = d.dsm[:37], d.trees[:37], d.dem[:37]
+    d.walls, d.aspect = d.walls[:37], d.aspect[:37]
+    d.alb_grid, d.emis_grid = d.alb_grid[:37], d.emis_grid[:37]
+    eng = batched.Engine([d], n_hours=24)
+    tabs = eng.tables()
+    ref = None
+    for w in rt.TILE_CHOICES:
+        print(f'tile width {w}', flush=True)
+        eng.set_tiles(w)
+        eng.load(tabs)
+        eng.reset_state()
+        eng.forward_day()
+        out = eng.outputs(0)
+        if ref is None:
+            ref = out
+        else:
+            assert np.array_equal(out[0], ref[0], equal_nan=True) and np.array_equal(out[1], ref[1], equal_nan=True), w
+
+
+def test_autotuner_picks_a_measured_width(rt, tmp_path, monkeypatch):
+    from solweig_amd import autotune, batched, synthetic
+    monkeypatch.setattr(autotune, 'CACHE', str(tmp_path / 'tiles.json'))
+    d = synthetic.make_c1(seed=13, size=96, days=1)
+    eng = batched.Engine([d], n_hours=24)
+    res = autotune.tune(eng, repeats=2)
+    print(f'autotune: {res}', flush=True)
+    assert res['tile_cols'] in rt.TILE_CHOICES and eng.domains[0].tile_cols == res['tile_cols']
+    assert set(res['ms']) == set(rt.TILE_CHOICES)
+    again = autotune.tune(eng)
+    assert again['cached'] and again['tile_cols'] == res['tile_cols']
