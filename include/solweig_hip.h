@@ -65,7 +65,7 @@ enum {
     SWF_KSIDED, SWF_DRAD, SWF_KSIDE, SWF_SHADOW, SWF_WALLSUN, SWF_FSH, SW_FULL_PLANES
 };
 
-/* Device-side description of one domain (264 bytes). All fields are 8 bytes wide; pointers are device
+/* Device-side description of one domain (280 bytes). All fields are 8 bytes wide; pointers are device
  * pointers. Optional outputs may be NULL. */
 typedef struct sw_domain {
     int64_t rows, cols, pixels, landcover;
@@ -89,6 +89,9 @@ typedef struct sw_domain {
     double *shadow_planes;                                   /* [8][P] cast_shadow planes      */
     float *full;                                             /* [SW_FULL_PLANES][P]           */
     const float *skysums;                                    /* [SW_SKY_SUMS][P] static sums   */
+    const float *hmax;    /* [ceil(rows/16)][hmax_cols] upper bound of max(dsm, canopy top) per
+                             16 x 16 cell (FP32 rounded up), lets the sun-ray march skip steps */
+    int64_t hmax_cols;    /* ceil(cols/16)                                                    */
 } sw_domain;
 
 /* Static per-pixel sums over the patches of each visibility class (day patch geometry), the

@@ -21,7 +21,7 @@
 
 #include "../../../include/solweig_hip.h"
 
-static_assert(sizeof(sw_domain) == 264, "sw_domain must stay 33 eight-byte fields");
+static_assert(sizeof(sw_domain) == 280, "sw_domain must stay 35 eight-byte fields");
 
 namespace {
 
@@ -64,6 +64,7 @@ struct PixelAt {
     int d, r, c;
     long long p;
     bool valid;
+    int wr0, wr1, wc0, wc1;     // rows and columns spanned by this pixel's wave (uniform)
 };
 
 // Address spaces: per-pixel planes are read and written through global (1) pointers; every
@@ -93,6 +94,8 @@ struct DomainG {
     AS1 double *shadow_planes;
     AS1 float *full;
     const AS1 float *skysums;
+    const AS4 float *hmax;
+    int hmax_cols;
 };
 
 __device__ __forceinline__ DomainG domain_at(const sw_domain *D) {
@@ -116,6 +119,7 @@ __device__ __forceinline__ DomainG domain_at(const sw_domain *D) {
     g.ray_off = (const AS4 int32_t *)d->ray_off; g.ray_dz = (const AS4 double *)d->ray_dz;
     g.masks = (AS1 uint8_t *)d->masks; g.shadow_planes = (AS1 double *)d->shadow_planes;
     g.full = (AS1 float *)d->full; g.skysums = (const AS1 float *)d->skysums;
+    g.hmax = (const AS4 float *)d->hmax; g.hmax_cols = (int)d->hmax_cols;
     return g;
 }
 
@@ -148,8 +152,14 @@ __device__ __forceinline__ PixelAt locate(const sw_domain *doms, const int32_t *
     const int cols = (int)G.cols, rows = (int)G.rows;
     const int sh = G.tile_shift, tx = 1 << sh, ty = BLOCK >> sh;
     const int tpr = (cols + tx - 1) >> sh;
-    at.r = (t / tpr) * ty + ((int)threadIdx.x >> sh);
-    at.c = (t % tpr) * tx + ((int)threadIdx.x & (tx - 1));
+    const int tr0 = (t / tpr) * ty, tc0 = (t % tpr) * tx;
+    at.r = tr0 + ((int)threadIdx.x >> sh);
+    at.c = tc0 + ((int)threadIdx.x & (tx - 1));
+    const int w64 = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6) * 64;
+    at.wr0 = tr0 + (w64 >> sh);
+    at.wr1 = tr0 + ((w64 + 63) >> sh);
+    at.wc0 = tx > 64 ? tc0 + (w64 & (tx - 1)) : tc0;
+    at.wc1 = tx > 64 ? at.wc0 + 63 : tc0 + tx - 1;
     at.valid = at.r < rows && at.c < cols;
     at.p = (long long)at.r * cols + at.c;
     return at;
@@ -301,6 +311,39 @@ struct March {
     bool veg, ever;
 };
 
+constexpr int SKIP_WINDOW = 32;   // steps per test of the coarse height bound (multiple of CHUNK)
+constexpr int HCELL_SHIFT = 4;    // 16 x 16 cells of the coarse height bound
+
+// Minimum over the wave's lanes (every lane calls), returned as a uniform value.
+__device__ __forceinline__ double wave_min_uniform(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o, 64));
+    const uint64_t b = (uint64_t)__double_as_longlong(v);
+    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)b);
+    const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(b >> 32));
+    return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+}
+
+// True when every sample of steps [k0, ke] of every lane of the wave is at most a_min above
+// the ray: max over the 16 x 16 cells covering the samples' bounding box (the shifts grow
+// monotonically, so the box spans the wave's pixels shifted by the first and the last step)
+// minus the height of the step before k0. Such steps change nothing: a - dz, top - dz,
+// top - dz_prev and bot - dz all stay <= the receiver's height.
+__device__ __forceinline__ bool window_below(const DomainG &G, const PixelAt &at, const AS4 int32_t *off,
+                                             int k0, int ke, int rows, int cols, double dz_before,
+                                             double a_min) {
+    const int dr0 = off[2 * k0], dr1 = off[2 * ke], dc0 = off[2 * k0 + 1], dc1 = off[2 * ke + 1];
+    int r0 = at.wr0 + min(dr0, dr1), r1 = at.wr1 + max(dr0, dr1);
+    int c0 = at.wc0 + min(dc0, dc1), c1 = at.wc1 + max(dc0, dc1);
+    r0 = max(r0, 0); c0 = max(c0, 0); r1 = min(r1, rows - 1); c1 = min(c1, cols - 1);
+    if (r0 > r1 || c0 > c1) return true;             // every sample is off the raster
+    float hm = -__builtin_huge_valf();
+    for (int cr = r0 >> HCELL_SHIFT; cr <= (r1 >> HCELL_SHIFT); ++cr)
+        for (int cc = c0 >> HCELL_SHIFT; cc <= (c1 >> HCELL_SHIFT); ++cc)
+            hm = fmaxf(hm, G.hmax[cr * G.hmax_cols + cc]);
+    return (double)hm - dz_before <= a_min;
+}
+
 // CHUNK steps [k0, k0 + CHUNK) of the sun-ray march of one pixel. MASKED: per-lane bounds
 // tests (a step outside the raster is made a no-op by an infinite dz) and only steps < L count;
 // otherwise all CHUNK steps are inside the raster for every lane and below L.
@@ -356,12 +399,17 @@ __global__ __launch_bounds__(BLOCK) void k_hour_shadow(const sw_domain *__restri
     DomainG G;
     const PixelAt at = locate(doms, bs, nd, total, G);
     const AS4 double *H = G.hours + (long long)hour * SW_HOUR_STRIDE;
-    if (H[SWH_DAY] == 0.0 || !at.valid) return;
+    if (H[SWH_DAY] == 0.0) return;
     const int rows = (int)G.rows, cols = (int)G.cols;
     const long long P = G.pixels, p = at.p;
     const AS1 float *dsm = G.dsm, *trees = G.trees;
     const double raise = G.dsm_raise;
-    const Heights me = heights_at(dsm, trees, raise, p);
+    Heights me = {0.0, 0.0, 0.0};
+    if (at.valid) me = heights_at(dsm, trees, raise, p);
+    // lowest receiver of the wave: a window of steps whose samples all stay at or below it
+    // (bounded through the coarse height map) cannot change any lane's march
+    const double a_min = wave_min_uniform(at.valid ? me.a : __builtin_huge_val());
+    if (!at.valid) return;
     const int start = uniform_index(H[SWH_RAY_START]);
     const int L = uniform_index(H[SWH_RAY_LEN]);
     const AS4 int32_t *off = G.ray_off + 2LL * start;
@@ -381,6 +429,14 @@ __global__ __launch_bounds__(BLOCK) void k_hour_shadow(const sw_domain *__restri
     const bool all_lanes = true;
     const uint64_t active = __ballot(all_lanes);
     for (int k0 = 0; k0 < L; k0 += CHUNK) {
+        if (k0 > 0 && (k0 & (SKIP_WINDOW - 1)) == 0) {
+            const int ke = (k0 + SKIP_WINDOW < L ? k0 + SKIP_WINDOW : L) - 1;
+            if (window_below(G, at, off, k0, ke, rows, cols, dzs[k0 - 1], a_min)) {
+                m.prev = dzs[ke];
+                k0 = ke + 1 - CHUNK;
+                continue;
+            }
+        }
         const int kl = k0 + CHUNK - 1 < L ? k0 + CHUNK - 1 : L - 1;
         const bool in_last = (unsigned)(at.r + off[2 * kl]) < (unsigned)rows &&
                              (unsigned)(at.c + off[2 * kl + 1]) < (unsigned)cols;

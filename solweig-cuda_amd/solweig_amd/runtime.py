@@ -51,7 +51,9 @@ class SwDomain(ctypes.Structure):
         [(n, ctypes.c_uint64) for n in ('dsm', 'trees', 'walls', 'aspect', 'emis', 'alb', 'tgk', 'tstart',
                                         'tmaxlst', 'svfbuveg', 'tan_asvf', 'tan_svfalfa', 'ground', 'vis',
                                         'rec', 'lag', 'tmrt', 'utci', 'hours', 'ptab_f', 'ptab_d',
-                                        'ray_off', 'ray_dz', 'masks', 'shadow_planes', 'full', 'skysums')]
+                                        'ray_off', 'ray_dz', 'masks', 'shadow_planes', 'full', 'skysums',
+                                        'hmax')] + [('hmax_cols', ctypes.c_int64)]
+HMAX_CELL = 16
 
 
 class SwAdjoint(ctypes.Structure):

@@ -47,6 +47,25 @@ def sky_sums(vis, P, stream=None):
     return out
 
 
+def height_bound(a, trees):
+    """Per 16 x 16 cell, an FP32 upper bound of max(a, canopy top) with a the FP64 DSM (raised
+    to >= 0) and top = trees + a (series.py:282-287): no sample of the sun-ray march in the cell
+    can exceed it."""
+    a = np.asarray(a, dtype=np.float64)
+    t = np.asarray(trees, dtype=np.float64)
+    h = np.maximum(a, t + a)
+    rows, cols = h.shape
+    c = rt.HMAX_CELL
+    R, C = -(-rows // c), -(-cols // c)
+    pad = np.full((R * c, C * c), -np.inf)
+    pad[:rows, :cols] = h
+    m = pad.reshape(R, c, C, c).max(axis=(1, 3))
+    m32 = m.astype(np.float32)
+    low = m32.astype(np.float64) < m
+    m32[low] = np.nextafter(m32[low], np.float32(np.inf))
+    return np.ascontiguousarray(m32)
+
+
 class PreparedDomain:
     """Device-resident static state of one raster domain."""
 
@@ -123,6 +142,7 @@ class PreparedDomain:
         self.dev['rec'] = rt.DeviceArray((self.P, 4), np.float32)
         self.dev['lag'] = rt.DeviceArray((6, self.P), np.float32)
         self.dev['skysums'] = sky_sums(self.dev['vis'], self.P, stream)
+        self.dev['hmax'] = up(height_bound(a, trees))
         if keep_host:
             self.host = dict(a=a, trees=trees, ground=ground, svf_planes=planes, svfbuveg=svfbuveg,
                              asvf=asvf, svfalfa=svfalfa, maps=maps)
@@ -160,8 +180,9 @@ class PreparedDomain:
                            ('emis', 'emis'), ('alb', 'alb'), ('tgk', 'tgk'), ('tstart', 'tstart'),
                            ('tmaxlst', 'tmax'), ('svfbuveg', 'svfbuveg'), ('tan_asvf', 'tan_asvf'),
                            ('tan_svfalfa', 'tan_svfalfa'), ('ground', 'ground'), ('vis', 'vis'),
-                           ('rec', 'rec'), ('lag', 'lag'), ('skysums', 'skysums')):
+                           ('rec', 'rec'), ('lag', 'lag'), ('skysums', 'skysums'), ('hmax', 'hmax')):
             setattr(s, field, d[key].ptr)
+        s.hmax_cols = -(-self.shape[1] // rt.HMAX_CELL)
         for k, v in extra.items():
             setattr(s, k, v or 0)
         return s
@@ -192,6 +213,8 @@ def prepared_from_arrays(*, dsm, vegdem, walls, dirwalls, buildings, svfbuveg, a
     trees32 = np.where(top != 0, top - a, 0.0).astype(np.float32)
     if not np.array_equal(dsm32.astype(np.float64), a):
         raise ValueError('dsm must be FP32-exact')
+    if np.any(trees32 < 0):
+        raise ValueError('vegdem below the DSM: canopy heights must be >= 0 (series.py:282-287)')
     self = PreparedDomain.__new__(PreparedDomain)
     from types import SimpleNamespace
     self.domain = SimpleNamespace(scale=float(scale), dsm=dsm32, trees=trees32, walls=walls, aspect=dirwalls,
@@ -213,6 +236,7 @@ def prepared_from_arrays(*, dsm, vegdem, walls, dirwalls, buildings, svfbuveg, a
                     rec=rt.DeviceArray((rows * cols, 4), np.float32),
                     lag=rt.DeviceArray((6, rows * cols), np.float32))
     self.dev['skysums'] = sky_sums(self.dev['vis'], rows * cols, stream)
+    self.dev['hmax'] = up(height_bound(a, trees32))
     self.skyview = None
     self.tile_cols = rt.TILE_X
     return self

@@ -50,11 +50,12 @@ def test_hour_record_layout_matches_header():
     assert vals['SW_HOUR_STRIDE'] == rt.SW_HOUR_STRIDE
 
 
-def test_domain_struct_is_264_bytes():
-    assert ctypes.sizeof(rt.SwDomain) == 264
+def test_domain_struct_is_280_bytes():
+    assert ctypes.sizeof(rt.SwDomain) == 280
     fields = re.findall(r'\*(\w+)', header_text()[header_text().index('typedef struct sw_domain'):])
-    names = [f[0] for f in rt.SwDomain._fields_][6:]
+    names = [f[0] for f in rt.SwDomain._fields_][6:-1]
     assert fields[:len(names)] == names
+    assert rt.SwDomain._fields_[-1][0] == 'hmax_cols'
 
 
 def test_adjoint_slots_and_struct_match_header():
