@@ -544,14 +544,21 @@ struct Triple {
     float lup, alb, nosh;
 };
 
-__device__ __forceinline__ float influence(float wall_sum, float ground_sum, float wall_div,
-                                           float ground_div, bool has_wall) {
-    return has_wall ? (wall_sum + ground_sum) / wall_div : ground_sum / ground_div;
+// influence (near or far field): with a sunlit / any wall in the field, walls and ground share
+// the divisor ``n + 1``, else the ground alone is divided by ``n`` (gvf.cu:209-239); divisors
+// enter as reciprocals.
+__device__ __forceinline__ float influence(float wall_sum, float ground_sum, float inv_wall_div,
+                                           float inv_ground_div, bool has_wall) {
+    return has_wall ? (wall_sum + ground_sum) * inv_wall_div : ground_sum * inv_ground_div;
 }
 
+// (near * 0.5 + far * 0.4) / 0.9 as two products
+constexpr float NEAR_W = 0.5f / 0.9f, FAR_W = 0.4f / 0.9f;
 __device__ __forceinline__ float distance_weight(float near_v, float far_v) {
-    return (near_v * 0.5f + far_v * 0.4f) / 0.9f;
+    return near_v * NEAR_W + far_v * FAR_W;
 }
+
+constexpr float INV18 = 1.0f / 18.0f, INV9 = 1.0f / 9.0f;
 
 struct GvfOut {
     Triple all, e, s, w, n;
@@ -583,6 +590,8 @@ __device__ __forceinline__ GvfOut ground_view(const DomainG &G, const PixelAt &a
     const bool wallbol = G.walls[at.p] > 0.0f;
     const double aspect = ((double)G.aspect[at.p] * 3.141592653589793) / 180.0;
     const float firstf = (float)first, secondf = (float)second;
+    const float inv_f = 1.0f / firstf, inv_f1 = 1.0f / (firstf + 1.0f);
+    const float inv_s = 1.0f / secondf, inv_s1 = 1.0f / (secondf + 1.0f);
     const int stride = (second + CHUNK - 1) / CHUNK * CHUNK;   // padded step axis of gofs
     const float local_lup = me.x * (1.0f - building);
     const float local_albsh = me.z * (1.0f - building);
@@ -644,14 +653,14 @@ __device__ __forceinline__ GvfOut ground_view(const DomainG &G, const PixelAt &a
         const float s_albwall = keep ? 0.f : cnt_wall * albedo_b;
         const float f_lwall = f_cnt_wall * lwall, f_albwall = f_cnt_wall * albedo_b;
         const float s_albwallnosh = cnt_any * albedo_b, f_albwallnosh = f_cnt_any * albedo_b;
-        const float lup_d = distance_weight(influence(f_lwall, f.lup, firstf + 1.f, firstf, f_sunin),
-                                            influence(s_lwall, w.lup, secondf + 1.f, secondf, sunin2)) +
+        const float lup_d = distance_weight(influence(f_lwall, f.lup, inv_f1, inv_f, f_sunin),
+                                            influence(s_lwall, w.lup, inv_s1, inv_s, sunin2)) +
                             local_lup;
-        const float alb_d = distance_weight(influence(f_albwall, f.albsh, firstf + 1.f, firstf, f_sunin),
-                                            influence(s_albwall, w.albsh, secondf + 1.f, secondf, sunin2)) +
+        const float alb_d = distance_weight(influence(f_albwall, f.albsh, inv_f1, inv_f, f_sunin),
+                                            influence(s_albwall, w.albsh, inv_s1, inv_s, sunin2)) +
                             local_albsh;
-        const float nosh_d = distance_weight(influence(f_albwallnosh, f.alb, firstf + 1.f, firstf, f_wallin),
-                                             influence(s_albwallnosh, w.alb, secondf, secondf, wallin2)) *
+        const float nosh_d = distance_weight(influence(f_albwallnosh, f.alb, inv_f1, inv_f, f_wallin),
+                                             influence(s_albwallnosh, w.alb, inv_s, inv_s, wallin2)) *
                                  building +
                              local_nosh;
         o.all.lup += lup_d; o.all.alb += alb_d; o.all.nosh += nosh_d;
@@ -784,11 +793,11 @@ __global__ __launch_bounds__(BLOCK) void k_hour_pixels(const sw_domain *__restri
         const bool lf = H[SWH_LAG_FIRST] != 0.0, lu = H[SWH_LAG_UPDATE] != 0.0;
         AS1 float *lg = G.lag;
         float s0 = lg[p], s1 = lg[P + p], s2 = lg[2 * P + p], s3 = lg[3 * P + p], s4 = lg[4 * P + p];
-        Lup = lag(g.all.lup / 18.0f + amb, s0, w, lf, lu);
-        LupE = lag(g.e.lup / 9.0f + amb, s1, w, lf, lu);
-        LupS = lag(g.s.lup / 9.0f + amb, s2, w, lf, lu);
-        LupW = lag(g.w.lup / 9.0f + amb, s3, w, lf, lu);
-        LupN = lag(g.n.lup / 9.0f + amb, s4, w, lf, lu);
+        Lup = lag(g.all.lup * INV18 + amb, s0, w, lf, lu);
+        LupE = lag(g.e.lup * INV9 + amb, s1, w, lf, lu);
+        LupS = lag(g.s.lup * INV9 + amb, s2, w, lf, lu);
+        LupW = lag(g.w.lup * INV9 + amb, s3, w, lf, lu);
+        LupN = lag(g.n.lup * INV9 + amb, s4, w, lf, lu);
         lg[p] = s0; lg[P + p] = s1; lg[2 * P + p] = s2; lg[3 * P + p] = s3; lg[4 * P + p] = s4;
         if (FULL) {
             const double snup = H[SWH_SNUP];
@@ -806,11 +815,11 @@ __global__ __launch_bounds__(BLOCK) void k_hour_pixels(const sw_domain *__restri
         const float sinalt = (float)H[SWH_SINALT];
         const float around = albedo_b * (1.0f - svfbuveg) * (radG * (1.0f - Fsh) + radD * Fsh);
         const float refl = radD * svfbuveg + around;
-        Kup = (g.all.alb / 18.0f) * radI * sinalt + refl * (g.all.nosh / 18.0f);
-        KupE = (g.e.alb / 9.0f) * radI * sinalt + refl * (g.e.nosh / 9.0f);
-        KupS = (g.s.alb / 9.0f) * radI * sinalt + refl * (g.s.nosh / 9.0f);
-        KupW = (g.w.alb / 9.0f) * radI * sinalt + refl * (g.w.nosh / 9.0f);
-        KupN = (g.n.alb / 9.0f) * radI * sinalt + refl * (g.n.nosh / 9.0f);
+        Kup = (g.all.alb * INV18) * radI * sinalt + refl * (g.all.nosh * INV18);
+        KupE = (g.e.alb * INV9) * radI * sinalt + refl * (g.e.nosh * INV9);
+        KupS = (g.s.alb * INV9) * radI * sinalt + refl * (g.s.nosh * INV9);
+        KupW = (g.w.alb * INV9) * radI * sinalt + refl * (g.w.nosh * INV9);
+        KupN = (g.n.alb * INV9) * radI * sinalt + refl * (g.n.nosh * INV9);
         Kdown = radI * shadow * sinalt + around;  // + dRad below
         KsideI = shadow * radI * (float)H[SWH_COSALT];
     } else {
@@ -1095,7 +1104,9 @@ __device__ __forceinline__ void adj_walk(const DomainG &G, const AdjG &A, const 
     const double aspect = ((double)G.aspect[p] * 3.141592653589793) / 180.0;
     const float firstf = (float)first, secondf = (float)second;
     const int stride = (second + CHUNK - 1) / CHUNK * CHUNK;
-    const float k4 = 0.4f / 0.9f, k5 = 0.5f / 0.9f;
+    const float k4 = FAR_W, k5 = NEAR_W;
+    const float inv_f = 1.0f / firstf, inv_f1 = 1.0f / (firstf + 1.0f);
+    const float inv_s = 1.0f / secondf, inv_s1 = 1.0f / (secondf + 1.0f);
     float self_lup = 0.f, self_albsh = 0.f, self_alb = 0.f;
     for (int dir = 0; dir < 18; ++dir) {
         float mu = lg[0], nu = la[0], ka = ln[0];
@@ -1153,26 +1164,26 @@ __device__ __forceinline__ void adj_walk(const DomainG &G, const AdjG &A, const 
         else
             facesh = 1 - (int)(aspect > bd[0] || aspect <= bd[1]);
         const bool keep = (cnt_wall == secondf) && facesh == 0;
-        const float cs = k4 / (sunin2 ? secondf + 1.f : secondf);
-        const float cf = k5 / (f_sunin ? firstf + 1.f : firstf);
-        const float a_lup = mu * cs, a_albsh = nu * cs, a_nosh = ka * b * (k4 / secondf);
+        const float cs = k4 * (sunin2 ? inv_s1 : inv_s);
+        const float cf = k5 * (f_sunin ? inv_f1 : inv_f);
+        const float a_lup = mu * cs, a_albsh = nu * cs, a_nosh = ka * b * (k4 * inv_s);
         v4f cv;
         cv.x = a_lup; cv.y = a_albsh; cv.z = a_nosh; cv.w = w.L;
         A.coef[(long long)dir * P + p] = cv;
         if (f.L >= 1.0f) {                             // near field: step 0 is the receiver itself
             self_lup += mu * cf;
             self_albsh += nu * cf;
-            self_alb += ka * b * (k5 / (f_wallin ? firstf + 1.f : firstf));
+            self_alb += ka * b * (k5 * (f_wallin ? inv_f1 : inv_f));
         }
         self_lup += mu * (1.0f - b);                  // local terms of a non-ground receiver
         self_albsh += nu * (1.0f - b);
         self_alb += ka * (1.0f - b);
-        const float wall_s = (sunin2 && !keep) ? cnt_wall / (secondf + 1.f) : 0.f;
-        const float wall_f = f_sunin ? f_cnt_wall / (firstf + 1.f) : 0.f;
+        const float wall_s = (sunin2 && !keep) ? cnt_wall * inv_s1 : 0.f;
+        const float wall_f = f_sunin ? f_cnt_wall * inv_f1 : 0.f;
         d_lwall += mu * (k4 * wall_s + k5 * wall_f);
         d_albedo_b += nu * (k4 * wall_s + k5 * wall_f) +
-                      ka * b * (k4 * (wallin2 ? cnt_any / secondf : 0.f) +
-                                k5 * (f_wallin ? f_cnt_any / (firstf + 1.f) : 0.f));
+                      ka * b * (k4 * (wallin2 ? cnt_any * inv_s : 0.f) +
+                                k5 * (f_wallin ? f_cnt_any * inv_f1 : 0.f));
         if (CHECK && rep > 0.f) {                      // repeated off-raster samples land on qprev
             atomicAdd((float *)(A.lam_rec + qprev), a_lup * rep);
             atomicAdd((float *)(A.lam_rec + P + qprev), a_albsh * rep);
@@ -1221,11 +1232,11 @@ __global__ __launch_bounds__(BLOCK) void k_adj_prep(const sw_domain *__restrict_
                                       : ground_view<true>(G, at, me, H, gofs, gbounds, first, second);
             const AS1 float *lg = G.lag;
             float s0 = lg[p], s1 = lg[P + p], s2 = lg[2 * P + p], s3 = lg[3 * P + p], s4 = lg[4 * P + p];
-            const float Lup = lag(g.all.lup / 18.0f + amb, s0, w, lf, lu);
-            const float LupE = lag(g.e.lup / 9.0f + amb, s1, w, lf, lu);
-            const float LupS = lag(g.s.lup / 9.0f + amb, s2, w, lf, lu);
-            const float LupW = lag(g.w.lup / 9.0f + amb, s3, w, lf, lu);
-            const float LupN = lag(g.n.lup / 9.0f + amb, s4, w, lf, lu);
+            const float Lup = lag(g.all.lup * INV18 + amb, s0, w, lf, lu);
+            const float LupE = lag(g.e.lup * INV9 + amb, s1, w, lf, lu);
+            const float LupS = lag(g.s.lup * INV9 + amb, s2, w, lf, lu);
+            const float LupW = lag(g.w.lup * INV9 + amb, s3, w, lf, lu);
+            const float LupN = lag(g.n.lup * INV9 + amb, s4, w, lf, lu);
             const float radI = (float)H[SWH_RADI], radD = (float)H[SWH_RADD], radG = (float)H[SWH_RADG];
             const float albedo_b = (float)H[SWH_ALBEDO_B];
             const float Fsh = wedge(G.tan_svfalfa[p], H[SWH_TANZEN]);
@@ -1234,11 +1245,11 @@ __global__ __launch_bounds__(BLOCK) void k_adj_prep(const sw_domain *__restrict_
             const float X = radG * (1.0f - Fsh) + radD * Fsh;
             const float around = albedo_b * (1.0f - svfbuveg) * X;
             const float refl = radD * svfbuveg + around;
-            const float Kup = (g.all.alb / 18.0f) * radI * sinalt + refl * (g.all.nosh / 18.0f);
-            const float KupE = (g.e.alb / 9.0f) * radI * sinalt + refl * (g.e.nosh / 9.0f);
-            const float KupS = (g.s.alb / 9.0f) * radI * sinalt + refl * (g.s.nosh / 9.0f);
-            const float KupW = (g.w.alb / 9.0f) * radI * sinalt + refl * (g.w.nosh / 9.0f);
-            const float KupN = (g.n.alb / 9.0f) * radI * sinalt + refl * (g.n.nosh / 9.0f);
+            const float Kup = (g.all.alb * INV18) * radI * sinalt + refl * (g.all.nosh * INV18);
+            const float KupE = (g.e.alb * INV9) * radI * sinalt + refl * (g.e.nosh * INV9);
+            const float KupS = (g.s.alb * INV9) * radI * sinalt + refl * (g.s.nosh * INV9);
+            const float KupW = (g.w.alb * INV9) * radI * sinalt + refl * (g.w.nosh * INV9);
+            const float KupN = (g.n.alb * INV9) * radI * sinalt + refl * (g.n.nosh * INV9);
             const float KsideI = shadow * radI * cosalt;
             const int row = uniform_index(H[SWH_PATCH_TABLE]);
             const AS4 float *pf = G.ptab_f + (long long)row * SW_PATCHES * SW_PATCH_F;
@@ -1307,18 +1318,18 @@ __global__ __launch_bounds__(BLOCK) void k_adj_prep(const sw_domain *__restrict_
             A.lam_patch[p] = lKdown;
             A.lam_patch[P + p] = lKside;
             // Kup and the ground-view albedo sums
-            const float gsum_alb = g.all.alb / 18.0f, gsum_nosh = g.all.nosh / 18.0f;
+            const float gsum_alb = g.all.alb * INV18, gsum_nosh = g.all.nosh * INV18;
             float lrefl = lKup * gsum_nosh;
             float d_radI = lKup * sinalt * gsum_alb;
-            la[0] = lKup * radI * sinalt / 18.0f;
-            ln[0] = lKup * refl / 18.0f;
+            la[0] = lKup * radI * sinalt * INV18;
+            ln[0] = lKup * refl * INV18;
             const Triple gc[4] = {g.e, g.s, g.w, g.n};
 #pragma unroll
             for (int c = 0; c < 4; ++c) {
-                la[c + 1] = lKupC * radI * sinalt / 9.0f;
-                ln[c + 1] = lKupC * refl / 9.0f;
-                lrefl += lKupC * (gc[c].nosh / 9.0f);
-                d_radI += lKupC * sinalt * (gc[c].alb / 9.0f);
+                la[c + 1] = lKupC * radI * sinalt * INV9;
+                ln[c + 1] = lKupC * refl * INV9;
+                lrefl += lKupC * (gc[c].nosh * INV9);
+                d_radI += lKupC * sinalt * (gc[c].alb * INV9);
             }
             const float laround = lKdown + lrefl;
             d_radI += lKdown * shadow * sinalt + lKside * shadow * cosalt;
@@ -1344,7 +1355,7 @@ __global__ __launch_bounds__(BLOCK) void k_adj_prep(const sw_domain *__restrict_
             ls[k * P + p] = lf ? 0.f : lprev;
         }
         // current Lup = ground-view sum / 18 (or / 9 per half) + flux4 * emis
-        const float lg5[5] = {lcur[0] / 18.0f, lcur[1] / 9.0f, lcur[2] / 9.0f, lcur[3] / 9.0f, lcur[4] / 9.0f};
+        const float lg5[5] = {lcur[0] * INV18, lcur[1] * INV9, lcur[2] * INV9, lcur[3] * INV9, lcur[4] * INV9};
         const float lamb = lcur[0] + lcur[1] + lcur[2] + lcur[3] + lcur[4];
         A.grad_emis[p] += lamb * flux4;
         sl[SWA_FLUX4] = lamb * emis;
