@@ -219,6 +219,41 @@ int sw_day_adjoint(const sw_domain *domain_dev, const int32_t *block_start_dev, 
                    const int32_t *gvf_offsets_dev, const double *gvf_bounds_dev, int first_steps,
                    int second_steps, double inv_count, void *stream);
 
+/* ----- per-step sub-models as standalone elementwise calls ------------------------------------
+ * fields[k] is a device pointer to counts[k] values (1 = broadcast, or n); the working precision
+ * is real_bytes (4: float, 8: double) for every field except the FP64 svfalfa of the wedge;
+ * params are host FP64 values; out_dev receives n_out planes of n values (working precision).
+ *   sw_cylindric_wedge     fields svfalfa (FP64);                 params zen;            1 plane
+ *                          (model.cylindric_wedge, model.py:736-739; umep_solweig.py:383-426)
+ *   sw_wave_delay          fields current, state;                 params first, update,
+ *                          weight; planes delayed, next state  (model.TsWaveDelay_2015a, 742-756)
+ *   sw_kup                 fields svfbuveg, F_sh, gvfalb (all, E, S, W, N), gvfalbnosh (all, E,
+ *                          S, W, N); params radI, radD, radG, altitude, albedo_b; 5 planes Kup,
+ *                          KupE, KupS, KupW, KupN       (model.Kup_veg_2015a, model.py:759-768)
+ *   sw_lside               fields svf E/S/W/N, svfveg E/S/W/N, svfaveg E/S/W/N, Tw, Ldown, F_sh,
+ *                          Lup E/S/W/N; params azimuth, altitude, Ta, SBC, ewall, esky, t, CI,
+ *                          anisotropic; planes Least, Lsouth, Lwest, Lnorth
+ *                                                       (model.Lside_veg_v2022a, model.py:771-781)
+ *   sw_surface_temperature fields TgK, Tstart, TgK_wall, Tstart_wall, TmaxLST, TmaxLST_wall;
+ *                          params altmax, frac(dectime), SNUP, CI_TgG, landcover; planes Tg,
+ *                          Tgwall                   (model.surface_temperature, model.py:784-793)
+ *   sw_assemble_tmrt       fields Kdown, Kup, Knorth, Keast, Ksouth, Kwest, Ldown, Lup, Lnorth,
+ *                          Least, Lsouth, Lwest, KsideI, Kside, Lside; params absK, absL, Fside,
+ *                          Fup, Fcyl, cyl, anisotropic, SBC; planes Tmrt, Sstr
+ *                                                         (model.assemble_tmrt, model.py:796-805) */
+int sw_cylindric_wedge(const void *const *fields, const long long *counts, int n_fields, long long n,
+                       const double *params, int n_params, int real_bytes, void *out_dev, void *stream);
+int sw_wave_delay(const void *const *fields, const long long *counts, int n_fields, long long n,
+                  const double *params, int n_params, int real_bytes, void *out_dev, void *stream);
+int sw_kup(const void *const *fields, const long long *counts, int n_fields, long long n,
+           const double *params, int n_params, int real_bytes, void *out_dev, void *stream);
+int sw_lside(const void *const *fields, const long long *counts, int n_fields, long long n,
+             const double *params, int n_params, int real_bytes, void *out_dev, void *stream);
+int sw_surface_temperature(const void *const *fields, const long long *counts, int n_fields, long long n,
+                           const double *params, int n_params, int real_bytes, void *out_dev, void *stream);
+int sw_assemble_tmrt(const void *const *fields, const long long *counts, int n_fields, long long n,
+                     const double *params, int n_params, int real_bytes, void *out_dev, void *stream);
+
 /* UTCI of a Tmrt plane for scalar weather (replaces utci.utci_calculator's scalar-weather
  * path, utci.py:22-33, utci.cu:243-523): ``coefficients`` are 7 host FP64 values c_k with
  * UTCI = sum_k c_k (Tmrt - ta32)^k; sentinel != 0 writes -999; ground_dev (nullable) NaNs

@@ -779,3 +779,86 @@ def step(p, hour, state):
                TgOut=TgOut, Sstr=Sstr)
     new_state = dict(CI=CI, firstdaytime=firstdaytime, timeadd=timeadd, timestepdec=timestepdec, **Tgmap)
     return out, new_state
+
+
+# ----------------------------------------------------------------------------- per-step sub-models
+
+def lside_weights(svf, svfveg, svfaveg, total=4.4897):
+    """View-factor weights of wall, vegetation, sky and reflection for one cardinal direction
+    (Lvikt_veg, umep_solweig.py:1183-1198): (veg, wall, sky, refl)."""
+    def p(x):
+        return 63.227 * x ** 6 - 161.51 * x ** 5 + 156.91 * x ** 4 - 70.424 * x ** 3 + 16.773 * x ** 2 - 0.4863 * x
+    wall = (total - p(svf)) / total - (total - p(svfaveg)) / total
+    both = svfveg + svf - 1
+    sky = p(both) / total
+    refl = (total - p(both)) / total
+    veg = refl - wall
+    return veg, wall, sky, refl
+
+
+def lside(svfS, svfW, svfN, svfE, svfEveg, svfSveg, svfWveg, svfNveg, svfEaveg, svfSaveg, svfWaveg,
+          svfNaveg, azimuth, altitude, Ta, Tw, SBC, ewall, Ldown, esky, t, F_sh, CI, LupE, LupS, LupW,
+          LupN, anisotropic_longwave):
+    """Longwave on the four cardinal faces of a standing body (Lside_veg_v2022a,
+    umep_solweig.py:1032-1180): returns (Least, Lsouth, Lwest, Lnorth)."""
+    Fs = 2 * F_sh - 1
+    c = 1 - CI
+    T4 = (Ta + 273.15) ** 4
+    allsky = esky * SBC * T4 * (1 - c) + c * SBC * T4
+    faces = (  # (svf, veg, aveg, Lup, lit(azimuth), azimuth offset of the face)
+        (svfE, svfEveg, svfEaveg, LupE, (azimuth > (180 - t)) and (azimuth <= (360 - t)), azimuth - 180 + t),
+        (svfS, svfSveg, svfSaveg, LupS, (azimuth <= (90 - t)) or (azimuth > (270 - t)), azimuth - 270 + t),
+        (svfW, svfWveg, svfWaveg, LupW, (azimuth > (360 - t)) or (azimuth <= (180 - t)), azimuth + t),
+        (svfN, svfNveg, svfNaveg, LupN, (azimuth > (90 - t)) and (azimuth <= (270 - t)), azimuth - 90 + t))
+    out = []
+    for svf, veg, aveg, up, lit, azi in faces:
+        if anisotropic_longwave == 1:
+            out.append(up * 0.5)
+            continue
+        wveg, wwall, wsky, wrefl = lside_weights(svf, veg, aveg)
+        if altitude > 0 and lit:
+            alfa = np.arcsin(np.exp(np.log(1 - svf) / 2))
+            a_b = np.arctan(alfa)
+            b_b = np.arctan(np.tan(alfa * Fs))
+            betasun = (a_b - b_b) / 2 + b_b
+            sun = SBC * ewall * ((Ta + 273.15 + Tw * np.sin(azi * (np.pi / 180))) ** 4) * wwall * (1 - Fs) * \
+                np.cos(betasun) * 0.5
+            shade = SBC * ewall * T4 * wwall * Fs * 0.5
+        else:
+            sun = 0
+            shade = SBC * ewall * T4 * wwall * 0.5
+        sky = ((svf + veg - 1) * allsky) * wsky * 0.5
+        vegl = SBC * ewall * T4 * wveg * 0.5
+        refl = (Ldown + up) * wrefl * (1 - ewall) * 0.5
+        out.append(sky + sun + shade + vegl + up * 0.5 + refl)
+    return tuple(out)
+
+
+def surface_temperature(TgK, Tstart, TgK_wall, Tstart_wall, TmaxLST, TmaxLST_wall, altmax, dectime, SNUP,
+                        CI_TgG, landcover=0):
+    """Ground and wall temperature wave (umep_solweig.py:144-170, as in step): (Tg, Tgwall)."""
+    frac = dectime - np.floor(dectime)
+    Tg = (TgK * altmax + Tstart) * np.sin(((frac - SNUP / 24) / (TmaxLST / 24 - SNUP / 24)) * np.pi / 2)
+    Tgw = (TgK_wall * altmax + Tstart_wall) * np.sin(((frac - SNUP / 24) / (TmaxLST_wall / 24 - SNUP / 24)) *
+                                                    np.pi / 2)
+    Tgw = np.where(Tgw < 0, 0.0, Tgw)
+    Tg = Tg * CI_TgG
+    Tgw = Tgw * CI_TgG
+    if landcover == 1:
+        Tg = np.where(Tg < 0, 0.0, Tg)
+    return Tg, Tgw
+
+
+def assemble_tmrt(Kdown, Kup, Knorth, Keast, Ksouth, Kwest, Ldown, Lup, Lnorth, Least, Lsouth, Lwest, KsideI,
+                  Kside, Lside, absK, absL, Fside, Fup, Fcyl, cyl, anisotropic_sky, SBC=5.67051e-8):
+    """Mean radiant temperature from the absorbed flux (umep_solweig.py:345-360): (Tmrt, Sstr)."""
+    K4 = Knorth + Keast + Ksouth + Kwest
+    L4 = Lnorth + Least + Lsouth + Lwest
+    if cyl == 1 and anisotropic_sky == 1:
+        S = absK * (Kside * Fcyl + (Kdown + Kup) * Fup + K4 * Fside) + \
+            absL * ((Ldown + Lup) * Fup + Lside * Fcyl + L4 * Fside)
+    elif cyl == 1:
+        S = absK * (KsideI * Fcyl + (Kdown + Kup) * Fup + K4 * Fside) + absL * ((Ldown + Lup) * Fup + L4 * Fside)
+    else:
+        S = absK * ((Kdown + Kup) * Fup + K4 * Fside) + absL * ((Ldown + Lup) * Fup + L4 * Fside)
+    return np.sqrt(np.sqrt(S / (absL * SBC))) - 273.2, S

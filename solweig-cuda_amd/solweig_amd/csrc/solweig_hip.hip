@@ -1481,6 +1481,202 @@ __global__ __launch_bounds__(BLOCK) void k_adj_gather(const sw_domain *__restric
     add_wave(d + SWA_PSI, v2);
 }
 
+// =====================================================================================
+// Per-step sub-models as standalone elementwise kernels (model.py's cylindric_wedge,
+// TsWaveDelay_2015a, Kup_veg_2015a, Lside_veg_v2022a, surface_temperature, assemble_tmrt).
+// Each input is a plane of n values or a single value broadcast to all pixels; the field table
+// travels by value in the kernel arguments. R is the working precision (float or double).
+// =====================================================================================
+constexpr int SUB_FIELDS = 20;
+struct FieldSet {
+    const void *ptr[SUB_FIELDS];
+    long long count[SUB_FIELDS];
+};
+
+template <typename T>
+__device__ __forceinline__ T fld(const FieldSet &F, int k, long long i) {
+    return ((const T *)F.ptr[k])[F.count[k] == 1 ? 0 : i];
+}
+
+// Sunlit wall fraction of a cylinder, umep_solweig.py:383-426, in FP64 without the NaN
+// replacement of the step (the caller of the sub-model sees NaN where the reference does).
+__device__ __forceinline__ double wedge64(double alfa, double zen) {
+    const double PI = 3.141592653589793;
+    const double ta = tan(alfa), tb = tan(zen);
+    const double xa = 1.0 - 2.0 / (ta * tb), ha = 2.0 / (ta * tb), ba = 1.0 / ta;
+    double ukil = 0.0;
+    if (xa < 0.0) {
+        const double qa = tb / 2.0;
+        const double za = sqrt(ba * ba - (qa * qa) / 4.0);
+        const double phi = atan(za / qa);
+        ukil = 2.0 * ba * xa * ((sin(phi) - phi * cos(phi)) / (1.0 - cos(phi)));
+    }
+    return (2.0 * PI * ba - (2.0 * ba * ha + ukil)) / (2.0 * PI * ba);
+}
+
+template <typename R>
+__global__ void k_sub_wedge(FieldSet F, long long n, double zen, R *out) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = (R)wedge64(fld<double>(F, 0, i), zen);
+}
+
+template <typename R>
+__global__ void k_sub_delay(FieldSet F, long long n, int first, int update, R weight, R *out) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const R cur = fld<R>(F, 0, i);
+    const R prev = first ? cur : fld<R>(F, 1, i);
+    const R d = cur * (R(1) - weight) + prev * weight;
+    out[i] = d;
+    out[n + i] = update ? d : prev;
+}
+
+// fields: svfbuveg, F_sh, gvfalb (all, E, S, W, N), gvfalbnosh (all, E, S, W, N)
+template <typename R>
+__global__ void k_sub_kup(FieldSet F, long long n, R radI, R radD, R radG, double altitude, R albedo_b, R *out) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const R svfb = fld<R>(F, 0, i), fsh = fld<R>(F, 1, i);
+    const R s = (R)sin(altitude * (3.141592653589793 / 180.0));
+    const R refl = radD * svfb + albedo_b * (R(1) - svfb) * (radG * (R(1) - fsh) + radD * fsh);
+    for (int c = 0; c < 5; ++c) out[c * n + i] = fld<R>(F, 2 + c, i) * radI * s + refl * fld<R>(F, 7 + c, i);
+}
+
+template <typename R>
+__device__ __forceinline__ R view_poly(R x) {
+    return R(63.227) * pow(x, R(6)) - R(161.51) * pow(x, R(5)) + R(156.91) * pow(x, R(4)) -
+           R(70.424) * pow(x, R(3)) + R(16.773) * pow(x, R(2)) - R(0.4863) * x;
+}
+
+// fields (faces E, S, W, N): svf[4], svfveg[4], svfaveg[4], Tw, Ldown, F_sh, Lup[4]
+template <typename R>
+__global__ void k_sub_lside(FieldSet F, long long n, double azimuth, double altitude, R Ta, R SBC, R ewall,
+                            R esky, double t, R CI, int anisotropic, R *out) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const R total = R(4.4897);
+    const R T4 = pow(Ta + R(273.15), R(4));
+    const R c = R(1) - CI;
+    const R allsky = esky * SBC * T4 * (R(1) - c) + c * SBC * T4;
+    const R Fs = R(2) * fld<R>(F, 14, i) - R(1);
+    const R Tw = fld<R>(F, 12, i), Ldown = fld<R>(F, 13, i);
+    for (int f = 0; f < 4; ++f) {
+        const R up = fld<R>(F, 15 + f, i);
+        if (anisotropic == 1) { out[f * n + i] = up * R(0.5); continue; }
+        const R svf = fld<R>(F, f, i), veg = fld<R>(F, 4 + f, i), aveg = fld<R>(F, 8 + f, i);
+        const R wwall = (total - view_poly(svf)) / total - (total - view_poly(aveg)) / total;
+        const R both = veg + svf - R(1);
+        const R wsky = view_poly(both) / total, wrefl = (total - view_poly(both)) / total;
+        const R wveg = wrefl - wwall;
+        // face f lit by the sun, and the sun's azimuth relative to the face
+        bool lit;
+        double rel;
+        switch (f) {
+        case 0: lit = azimuth > 180.0 - t && azimuth <= 360.0 - t; rel = azimuth - 180.0 + t; break;
+        case 1: lit = azimuth <= 90.0 - t || azimuth > 270.0 - t; rel = azimuth - 270.0 + t; break;
+        case 2: lit = azimuth > 360.0 - t || azimuth <= 180.0 - t; rel = azimuth + t; break;
+        default: lit = azimuth > 90.0 - t && azimuth <= 270.0 - t; rel = azimuth - 90.0 + t; break;
+        }
+        R sun = R(0), shade = SBC * ewall * T4 * wwall * R(0.5);
+        if (altitude > 0.0 && lit) {
+            const R alfa = asin(exp(log(R(1) - svf) / R(2)));
+            const R ab = atan(alfa), bb = atan(tan(alfa * Fs));
+            const R betasun = (ab - bb) / R(2) + bb;
+            const R tw = Ta + R(273.15) + Tw * (R)sin(rel * (3.141592653589793 / 180.0));
+            sun = SBC * ewall * pow(tw, R(4)) * wwall * (R(1) - Fs) * cos(betasun) * R(0.5);
+            shade = SBC * ewall * T4 * wwall * Fs * R(0.5);
+        }
+        const R sky = ((svf + veg - R(1)) * allsky) * wsky * R(0.5);
+        const R vegl = SBC * ewall * T4 * wveg * R(0.5);
+        const R refl = (Ldown + up) * wrefl * (R(1) - ewall) * R(0.5);
+        out[f * n + i] = sky + sun + shade + vegl + up * R(0.5) + refl;
+    }
+}
+
+// fields: TgK, Tstart, TgK_wall, Tstart_wall, TmaxLST, TmaxLST_wall
+template <typename R>
+__global__ void k_sub_surface(FieldSet F, long long n, double altmax, double frac, double snup, R citgg,
+                              int landcover, R *out) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const R PI = R(3.141592653589793);
+    const R x0 = R(frac) - R(snup) / R(24);
+    R tg = (fld<R>(F, 0, i) * R(altmax) + fld<R>(F, 1, i)) *
+           sin((x0 / (fld<R>(F, 4, i) / R(24) - R(snup) / R(24))) * PI / R(2));
+    R tw = (fld<R>(F, 2, i) * R(altmax) + fld<R>(F, 3, i)) *
+           sin((x0 / (fld<R>(F, 5, i) / R(24) - R(snup) / R(24))) * PI / R(2));
+    if (tw < R(0)) tw = R(0);
+    tg = tg * citgg;
+    tw = tw * citgg;
+    if (landcover == 1 && tg < R(0)) tg = R(0);
+    out[i] = tg;
+    out[n + i] = tw;
+}
+
+// fields: Kdown, Kup, Knorth, Keast, Ksouth, Kwest, Ldown, Lup, Lnorth, Least, Lsouth, Lwest,
+//         KsideI, Kside, Lside
+template <typename R>
+__global__ void k_sub_assemble(FieldSet F, long long n, R absK, R absL, R Fside, R Fup, R Fcyl, int cyl,
+                               int anisotropic, R SBC, R *out) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const R K4 = fld<R>(F, 2, i) + fld<R>(F, 3, i) + fld<R>(F, 4, i) + fld<R>(F, 5, i);
+    const R L4 = fld<R>(F, 8, i) + fld<R>(F, 9, i) + fld<R>(F, 10, i) + fld<R>(F, 11, i);
+    const R KdKu = fld<R>(F, 0, i) + fld<R>(F, 1, i), LdLu = fld<R>(F, 6, i) + fld<R>(F, 7, i);
+    R S;
+    if (cyl == 1 && anisotropic == 1)
+        S = absK * (fld<R>(F, 13, i) * Fcyl + KdKu * Fup + K4 * Fside) +
+            absL * (LdLu * Fup + fld<R>(F, 14, i) * Fcyl + L4 * Fside);
+    else if (cyl == 1)
+        S = absK * (fld<R>(F, 12, i) * Fcyl + KdKu * Fup + K4 * Fside) + absL * (LdLu * Fup + L4 * Fside);
+    else
+        S = absK * (KdKu * Fup + K4 * Fside) + absL * (LdLu * Fup + L4 * Fside);
+    out[i] = sqrt(sqrt(S / (absL * SBC))) - R(273.2);
+    out[n + i] = S;
+}
+
+// Host side: one launcher per sub-model, dispatching on the working precision.
+template <typename R>
+int launch_sub(int which, const FieldSet &F, long long n, const double *q, void *out, hipStream_t s) {
+    const dim3 grid((unsigned)((n + 255) / 256)), block(256);
+    R *o = (R *)out;
+    switch (which) {
+    case 0: hipLaunchKernelGGL(k_sub_wedge<R>, grid, block, 0, s, F, n, q[0], o); break;
+    case 1: hipLaunchKernelGGL(k_sub_delay<R>, grid, block, 0, s, F, n, (int)q[0], (int)q[1], (R)q[2], o); break;
+    case 2: hipLaunchKernelGGL(k_sub_kup<R>, grid, block, 0, s, F, n, (R)q[0], (R)q[1], (R)q[2], q[3], (R)q[4], o);
+        break;
+    case 3: hipLaunchKernelGGL(k_sub_lside<R>, grid, block, 0, s, F, n, q[0], q[1], (R)q[2], (R)q[3], (R)q[4],
+                               (R)q[5], q[6], (R)q[7], (int)q[8], o); break;
+    case 4: hipLaunchKernelGGL(k_sub_surface<R>, grid, block, 0, s, F, n, q[0], q[1], q[2], (R)q[3], (int)q[4], o);
+        break;
+    default: hipLaunchKernelGGL(k_sub_assemble<R>, grid, block, 0, s, F, n, (R)q[0], (R)q[1], (R)q[2], (R)q[3],
+                                (R)q[4], (int)q[5], (int)q[6], (R)q[7], o); break;
+    }
+    return hip_check(hipGetLastError(), "sub-model launch");
+}
+
+// fields expected and parameters expected per sub-model
+constexpr int SUB_NFIELDS[6] = {1, 2, 12, 19, 6, 15};
+constexpr int SUB_NPARAMS[6] = {1, 3, 5, 9, 5, 8};
+
+int run_sub(int which, const void *const *fields, const long long *counts, int n_fields, long long n,
+            const double *params, int n_params, int real_bytes, void *out, void *stream) {
+    if (!fields || !counts || !params || !out) return fail(SW_ERR_ARGUMENT, "sub-model: NULL pointer");
+    if (n_fields != SUB_NFIELDS[which] || n_params != SUB_NPARAMS[which])
+        return fail(SW_ERR_ARGUMENT, "sub-model: wrong number of fields or parameters");
+    if (real_bytes != 4 && real_bytes != 8) return fail(SW_ERR_ARGUMENT, "sub-model: precision is 4 or 8 bytes");
+    if (n <= 0) return SW_OK;
+    FieldSet F = {};
+    for (int k = 0; k < n_fields; ++k) {
+        if (!fields[k] || (counts[k] != 1 && counts[k] != n))
+            return fail(SW_ERR_ARGUMENT, "sub-model: a field is NULL or neither 1 nor n values");
+        F.ptr[k] = fields[k];
+        F.count[k] = counts[k];
+    }
+    return real_bytes == 8 ? launch_sub<double>(which, F, n, params, out, (hipStream_t)stream)
+                           : launch_sub<float>(which, F, n, params, out, (hipStream_t)stream);
+}
+
 int launch_adjoint_hour(const sw_domain *dom, const int32_t *bs, int total, int hour, bool top,
                         const sw_adjoint *adj, const int32_t *gofs, const double *gb, int first,
                         int second, double inv_n, hipStream_t s) {
@@ -1722,6 +1918,19 @@ int sw_day_adjoint(const sw_domain *dom, const int32_t *bs, int total, int objec
     }
     return SW_OK;
 }
+
+#define SW_SUB_ENTRY(name, which)                                                                   \
+    int name(const void *const *fields, const long long *counts, int n_fields, long long n,          \
+             const double *params, int n_params, int real_bytes, void *out_dev, void *stream) {      \
+        return run_sub(which, fields, counts, n_fields, n, params, n_params, real_bytes, out_dev, stream); \
+    }
+SW_SUB_ENTRY(sw_cylindric_wedge, 0)
+SW_SUB_ENTRY(sw_wave_delay, 1)
+SW_SUB_ENTRY(sw_kup, 2)
+SW_SUB_ENTRY(sw_lside, 3)
+SW_SUB_ENTRY(sw_surface_temperature, 4)
+SW_SUB_ENTRY(sw_assemble_tmrt, 5)
+#undef SW_SUB_ENTRY
 
 int sw_utci_scalar(const float *tmrt, long long n, const double *coefficients, double ta32,
                    int sentinel, const uint8_t *ground, float *out, void *stream) {

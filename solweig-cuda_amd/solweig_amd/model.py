@@ -206,7 +206,112 @@ def Solweig_2022a_calc(i, dsm, scale, rows, cols, svf, svfN, svfW, svfE, svfS, s
     return tuple(out[n] for n in OUTPUT_NAMES)
 
 
-def cast_shadow(prepared, azimuth, altitude):
-    """The 8 FP64 planes of the last step's shadow pass (vegsh, sh, vbshvegsh, wallsh, wallsun,
-    wallshve, facesh, facesun; model.py:689-708) as device views."""
-    return tuple(prepared.shadow_planes.plane(k, prepared.shape) for k in range(8))
+@dataclass
+class PreparedShadow:
+    """Device state of the sun-ray march of one raster (model.py:626-652)."""
+    dev: dict
+    shape: tuple
+    building_max: float
+    vegetation_max: float
+    scale: float
+    usevegdem: bool
+    hours: object
+    ray_off: object
+    ray_dz: object
+    planes: object
+    dom_dev: object
+    block_start: object
+    tiles: int
+    stream: object
+
+
+def prepare_shadow(a, walls, aspect, scale, *, vegdem=None, vegdem2=None, bush=None, amaxvalue=None):
+    """Static state of cast_shadow (model.py:638-652). ``aspect`` in radians as the step passes it
+    (float32 degrees x pi/180 in FP64); ``a`` and the canopy top ``vegdem`` FP32-exact;
+    ``vegdem2``/``bush`` must be the ones series.py:282-287 derives from them."""
+    from .static import height_bound
+    a = np.asarray(_host(a), dtype=np.float64)
+    rows, cols = a.shape
+    P = rows * cols
+    dsm32 = a.astype(np.float32)
+    if not np.array_equal(dsm32.astype(np.float64), a):
+        raise ValueError('prepare_shadow: the DSM must be FP32-exact')
+    use = vegdem is not None
+    if use:
+        top = np.asarray(_host(vegdem), dtype=np.float64)
+        trees = np.where(top != 0, top - a, 0.0)
+        trees32 = trees.astype(np.float32)
+        if np.any(trees32 < 0) or not np.array_equal(np.where(top != 0, trees32.astype(np.float64) + a, 0.0), top):
+            raise ValueError('prepare_shadow: vegdem must be an FP32-exact canopy top above the DSM')
+        v2 = trees32.astype(np.float64) * 0.25 + a
+        v2[v2 == a] = 0
+        if vegdem2 is not None and not np.array_equal(np.asarray(_host(vegdem2), dtype=np.float64), v2):
+            raise ValueError('prepare_shadow: vegdem2 differs from 0.25 trees + dsm (series.py:285-286)')
+        if bush is not None and not np.array_equal(np.asarray(_host(bush), dtype=np.float64),
+                                                   np.logical_not(v2 * top) * top):
+            raise ValueError('prepare_shadow: bush differs from not(vegdem2 vegdem) vegdem (series.py:287)')
+    else:
+        trees32 = np.zeros((rows, cols), np.float32)
+    asp = np.asarray(_host(aspect), dtype=np.float64)
+    deg32 = (asp * (180.0 / np.pi)).astype(np.float32)
+    if not np.array_equal(deg32.astype(np.float64) * np.pi / 180.0, asp):
+        raise ValueError('prepare_shadow: aspect must be float32 degrees times pi/180 (model.py:995-996)')
+    stream = rt.Stream()
+    up = lambda x, t=np.float32: rt.DeviceArray.from_host(np.ascontiguousarray(np.broadcast_to(x, (rows, cols)), dtype=t))
+    zero = up(np.float32(0))
+    dev = dict(dsm=up(dsm32), trees=up(trees32), walls=up(np.asarray(_host(walls), np.float32)), aspect=up(deg32),
+               ground=up(np.uint8(1), np.uint8), zero=zero, one=up(np.float32(1)),
+               tan64=rt.DeviceArray((P,), np.float64), rec=rt.DeviceArray((P, 4), np.float32),
+               hmax=rt.DeviceArray.from_host(height_bound(a, trees32)))
+    building_max = float(a.max())
+    veg_max = float(amaxvalue) if (use and amaxvalue is not None) else float(max(a.max() - a.min(), trees32.max()))
+    cap = max(rows, cols) + 2 + sky.STEP_BLOCK
+    hours = rt.DeviceArray((1, rt.SW_HOUR_STRIDE), np.float64)
+    ray_off, ray_dz = rt.DeviceArray((cap, 2), np.int32), rt.DeviceArray((cap,), np.float64)
+    planes = rt.DeviceArray((8, rows, cols), np.float64)
+    s = rt.SwDomain()
+    s.rows, s.cols, s.pixels, s.landcover, s.tile_cols = rows, cols, P, 0, rt.TILE_X
+    for f in ('emis', 'alb', 'tgk', 'tstart', 'svfbuveg'):
+        setattr(s, f, dev['zero'].ptr)
+    s.tmaxlst = dev['one'].ptr
+    s.dsm, s.trees, s.walls, s.aspect = dev['dsm'].ptr, dev['trees'].ptr, dev['walls'].ptr, dev['aspect'].ptr
+    s.ground, s.rec, s.hmax, s.hmax_cols = dev['ground'].ptr, dev['rec'].ptr, dev['hmax'].ptr, -(-cols // rt.HMAX_CELL)
+    s.tan_asvf = s.tan_svfalfa = dev['tan64'].ptr
+    s.hours, s.ray_off, s.ray_dz, s.shadow_planes = hours.ptr, ray_off.ptr, ray_dz.ptr, planes.ptr
+    dom_dev = rt.DeviceArray((ctypes.sizeof(s),), np.uint8)
+    rt.call('sw_memcpy_h2d', dom_dev.ptr, ctypes.addressof(s), ctypes.sizeof(s), stream.handle)
+    tiles = rt.tile_count((rows, cols))
+    bs = rt.DeviceArray.from_host(np.array([0, tiles], dtype=np.int32))
+    stream.sync()
+    return PreparedShadow(dev, (rows, cols), building_max, veg_max, float(scale), use, hours, ray_off, ray_dz,
+                          planes, dom_dev, bs, tiles, stream)
+
+
+def cast_shadow(geometry, azimuth, altitude):
+    """Shadows of one sun position (model.py:689-708): with vegetation the 8 FP64 planes (vegsh,
+    sh, vbshvegsh, wallsh, wallsun, wallshve, facesh, facesun), else (sh, wallsh, wallsun, facesh,
+    facesun), as device views. Masks are bit-identical to shadowingfunction_wallheight_23."""
+    if isinstance(geometry, PreparedModel):
+        raise TypeError('cast_shadow takes the state of prepare_shadow')
+    g = geometry
+    amax = g.vegetation_max if g.usevegdem else g.building_max
+    off, dz, low, high, middle = sky.shadow_ray(float(azimuth), float(altitude), g.scale, amax, g.shape)
+    h = np.zeros((1, rt.SW_HOUR_STRIDE))
+    h[0, rt.H['DAY']] = 1.0
+    h[0, rt.H['RAY_START']], h[0, rt.H['RAY_LEN']] = 0, len(dz)
+    h[0, rt.H['FACE_LOW']], h[0, rt.H['FACE_HIGH']], h[0, rt.H['FACE_MIDDLE']] = low, high, float(middle)
+    h[0, rt.H['PSI']] = 1.0
+    g.hours.upload(h, g.stream)
+    if len(dz):
+        off, dz = sky.pad_steps(off, dz)
+        g.ray_off.upload(off, g.stream)
+        g.ray_dz.upload(dz, g.stream)
+    rt.call('sw_hour_shadow', g.dom_dev.ptr, g.block_start.ptr, 1, g.tiles, 0, 1, g.stream.handle)
+    g.stream.sync()
+    planes = tuple(g.planes.plane(k, g.shape) for k in range(8))
+    return planes if g.usevegdem else tuple(planes[k] for k in (1, 3, 4, 6, 7))
+
+
+# per-step sub-models with the reference's names (model.py:736-958)
+from .flux import (Kup_veg_2015a, Lside_veg_v2022a, Perez_v3, TsWaveDelay_2015a, assemble_tmrt,  # noqa: E402,F401
+                   cylindric_wedge, surface_temperature)

@@ -190,10 +190,12 @@ _PEREZ = np.array([  # rows a1..a4, b1..b4, c1..c4, d1..d4, e1..e4; columns = 8 
 _CLEAR_EDGES = (1.065, 1.230, 1.500, 1.950, 2.800, 4.500, 6.200)
 
 
-def perez(zen_deg, azimuth, radD, radI, jday):
+def perez(zen_deg, azimuth, radD, radI, jday, clip=False, full=False):
     """Perez et al. (1993) relative luminance over the 153 patches (Perez_v3,
     umep_solweig.py:1836-2057): returns (alt, azi, lv) with the patch angles round-tripped
-    deg -> rad -> deg as the original returns them."""
+    deg -> rad -> deg as the original returns them; ``full`` also returns the sky clearness and
+    brightness. ``clip`` limits the cosine of the patch-sun angle to [-1, 1] (model.py:933; the
+    UMEP step does not clip)."""
     zen = zen_deg * DEG
     az = azimuth * DEG
     altitude = (90 - zen_deg) * DEG
@@ -229,8 +231,12 @@ def perez(zen_deg, azimuth, radD, radI, jday):
     alt, azi = patches()
     vzen, valt, vazi = (90 - alt) * DEG, alt * DEG, azi * DEG
     cosg = np.sin(valt) * np.sin(altitude) + np.cos(altitude) * np.cos(valt) * np.cos(np.abs(vazi - az))
+    if clip:
+        cosg = np.clip(cosg, -1.0, 1.0)
     lv = (1 + ma * np.exp(mb / np.cos(vzen))) * ((1 + mc * np.exp(md * np.arccos(cosg)) + me * cosg * cosg))
     lv = lv / np.sum(lv)
+    if full:
+        return valt * (180 / np.pi), vazi * (180 / np.pi), lv, clear, bright
     return valt * (180 / np.pi), vazi * (180 / np.pi), lv
 
 
