@@ -254,6 +254,20 @@ int sw_surface_temperature(const void *const *fields, const long long *counts, i
 int sw_assemble_tmrt(const void *const *fields, const long long *counts, int n_fields, long long n,
                      const double *params, int n_params, int real_bytes, void *out_dev, void *stream);
 
+/* Ground view factors of one step for arbitrary caller fields (replaces model.gvf,
+ * model.py:569-622, and its gvf_pack / gvf_compact / gvf kernels, gvf.cu:79-292; UMEP
+ * gvf_2018a + sunonsurface_2018a, umep_solweig.py:583-663, 1400-1600). fields as above, 10 of
+ * them: buildings, walls, aspect (radians), wallsun (FP64), shadow, Tg, Tgwall, emis, albedo
+ * (working precision), lc_grid (FP64). params: Ta, ewall, SBC, albedo_b, landcover, Twater.
+ * offsets_dev [18][step_stride][2] (row, col) and bounds_dev [18][3] as prepare_gvf_geometry
+ * builds them (model.py:533-566). work_dev holds rows*cols*8 values of the working precision.
+ * out_dev receives 18 planes: gvfLup, gvfalb, gvfalbnosh, the E, S, W, N triples, gvfSum,
+ * gvfNorm, then Tg with water pixels (lc_grid == 3) replaced by Twater - Ta.            */
+int sw_gvf(const void *const *fields, const long long *counts, int n_fields, int rows, int cols,
+           const int32_t *offsets_dev, const double *bounds_dev, int step_stride, int first_steps,
+           int second_steps, const double *params, int n_params, int real_bytes, void *work_dev,
+           void *out_dev, void *stream);
+
 /* UTCI of a Tmrt plane for scalar weather (replaces utci.utci_calculator's scalar-weather
  * path, utci.py:22-33, utci.cu:243-523): ``coefficients`` are 7 host FP64 values c_k with
  * UTCI = sum_k c_k (Tmrt - ta32)^k; sentinel != 0 writes -999; ground_dev (nullable) NaNs

@@ -1659,6 +1659,16 @@ int launch_sub(int which, const FieldSet &F, long long n, const double *q, void 
 constexpr int SUB_NFIELDS[6] = {1, 2, 12, 19, 6, 15};
 constexpr int SUB_NPARAMS[6] = {1, 3, 5, 9, 5, 8};
 
+int make_fieldset(const void *const *fields, const long long *counts, int n_fields, long long n, FieldSet &F) {
+    for (int k = 0; k < n_fields; ++k) {
+        if (!fields[k] || (counts[k] != 1 && counts[k] != n))
+            return fail(SW_ERR_ARGUMENT, "sub-model: a field is NULL or neither 1 nor n values");
+        F.ptr[k] = fields[k];
+        F.count[k] = counts[k];
+    }
+    return SW_OK;
+}
+
 int run_sub(int which, const void *const *fields, const long long *counts, int n_fields, long long n,
             const double *params, int n_params, int real_bytes, void *out, void *stream) {
     if (!fields || !counts || !params || !out) return fail(SW_ERR_ARGUMENT, "sub-model: NULL pointer");
@@ -1667,14 +1677,178 @@ int run_sub(int which, const void *const *fields, const long long *counts, int n
     if (real_bytes != 4 && real_bytes != 8) return fail(SW_ERR_ARGUMENT, "sub-model: precision is 4 or 8 bytes");
     if (n <= 0) return SW_OK;
     FieldSet F = {};
-    for (int k = 0; k < n_fields; ++k) {
-        if (!fields[k] || (counts[k] != 1 && counts[k] != n))
-            return fail(SW_ERR_ARGUMENT, "sub-model: a field is NULL or neither 1 nor n values");
-        F.ptr[k] = fields[k];
-        F.count[k] = counts[k];
-    }
+    const int rc = make_fieldset(fields, counts, n_fields, n, F);
+    if (rc) return rc;
     return real_bytes == 8 ? launch_sub<double>(which, F, n, params, out, (hipStream_t)stream)
                            : launch_sub<float>(which, F, n, params, out, (hipStream_t)stream);
+}
+
+// =====================================================================================
+// Standalone ground view factors for arbitrary caller fields (model.gvf, model.py:569-622;
+// gvf_2018a / sunonsurface_2018a, umep_solweig.py:583-663, 1400-1600). Unlike the fused hour
+// pass, the ground mask may hold any value (the walk keeps its running minimum) and the shadow
+// may be fractional, so the walk follows the reference step by step in the working precision
+// R, with contraction off so the FP64 form rounds as NumPy does.
+// Fields: 0 buildings, 1 walls, 2 wall aspect (radians), 3 wallsun, 9 land cover (all FP64);
+//         4 shadow, 5 Tg, 6 Tgwall, 7 emis, 8 albedo (R).
+// =====================================================================================
+template <typename R>
+struct GvfSample {
+    R bu, sh, sunwall, lup0, lup1, albsh, alb, pad;
+};
+
+// Per pixel: the neighbour sample of every walk and the Tg after the water replacement
+// (umep_solweig.py:1419-1421). lup0 uses the caller's Tg (the first direction, which runs before
+// the in-place replacement), lup1 the replaced one (the other 17 directions).
+template <typename R>
+__global__ void k_gvf_pack(FieldSet F, long long n, R ta, R t273, R sbc, double t4a, int landcover,
+                           double twater_minus_ta, GvfSample<R> *rec, R *tg_out) {
+#pragma clang fp contract(off)
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double bu = fld<double>(F, 0, i);
+    const R sh = fld<R>(F, 4, i), tg = fld<R>(F, 5, i), emis = fld<R>(F, 7, i), alb = fld<R>(F, 8, i);
+    R tg1 = tg;
+    if (landcover == 1 && fld<double>(F, 9, i) == 3.0) tg1 = (R)twater_minus_ta;
+    const R se = sbc * emis, amb = se * (R)t4a;
+    GvfSample<R> s;
+    s.bu = (R)bu;
+    s.sh = sh;
+    s.sunwall = (fld<double>(F, 3, i) / fld<double>(F, 1, i)) * bu == 1.0 ? R(1) : R(0);
+    s.lup0 = se * pow(tg * sh + ta + t273, R(4)) - amb;
+    s.lup1 = se * pow(tg1 * sh + ta + t273, R(4)) - amb;
+    s.albsh = alb * sh;
+    s.alb = alb;
+    s.pad = R(0);
+    rec[i] = s;
+    tg_out[i] = tg1;
+}
+
+// Sums of one direction's walk; ``sn`` is the snapshot after the first ``first`` steps.
+template <typename R>
+struct GvfSums {
+    R wall, sh, lwall, lupsh, albwall, albsh, albwallnosh, albnosh;
+};
+
+template <typename R>
+__device__ __forceinline__ R gvf_mix(R wall_sum, R ground_sum, R wall_div, R ground_div, R has_wall) {
+#pragma clang fp contract(off)
+    return ((wall_sum + ground_sum) / wall_div) * has_wall + ground_sum / ground_div * (has_wall * R(-1) + R(1));
+}
+
+// One thread per receiver pixel: 18 directions x ``second`` steps, a step whose shift leaves the
+// raster repeating the previous sample (the shifted-window assignment of the reference).
+// Output planes (17 + updated Tg written by the pack kernel): gvfLup, gvfalb, gvfalbnosh, then
+// the E, S, W, N triples, gvfSum, gvfNorm.
+template <typename R>
+__global__ void __launch_bounds__(256) k_gvf_walk(FieldSet F, const GvfSample<R> *rec, int rows, int cols,
+                                                  const int32_t *offsets, int stride, const double *bounds,
+                                                  int first, int second, R ta, R t273, R sbc_ewall,
+                                                  R sbc_ewall_t4a, R albedo_b, R ambient_sbc, R t4a, R *out) {
+#pragma clang fp contract(off)
+    const long long n = (long long)rows * cols;
+    const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const int r = (int)(p / cols), c = (int)(p % cols);
+    const GvfSample<R> me = rec[p];
+    const bool wallbol = fld<double>(F, 1, p) > 0.0;
+    const double aspect = fld<double>(F, 2, p);
+    const R lwall = sbc_ewall * pow(fld<R>(F, 6, p) + ta + t273, R(4)) - sbc_ewall_t4a;
+    const R emis = fld<R>(F, 7, p);
+    const R firstf = (R)first, secondf = (R)second;
+    R acc_lup = 0, acc_alb = 0, acc_nosh = 0, acc_sum = 0;
+    R card[4][3] = {};
+    for (int dir = 0; dir < 18; ++dir) {
+        const int32_t *off = offsets + (long long)dir * stride * 2;
+        const bool d0 = dir == 0;
+        GvfSample<R> t = me;
+        R f = me.bu;
+        GvfSums<R> w = {}, s = {};
+        R bub = 0, bubwall = 0;
+        for (int st = 0; st < second; ++st) {
+            const int rr = r + off[2 * st], cc = c + off[2 * st + 1];
+            if ((unsigned)rr < (unsigned)rows && (unsigned)cc < (unsigned)cols) t = rec[(long long)rr * cols + cc];
+            if (t.bu < f) f = t.bu;
+            w.sh = w.sh + t.sh * f;
+            w.lupsh = w.lupsh + (d0 ? t.lup0 : t.lup1) * f;
+            w.albsh = w.albsh + t.albsh * f;
+            w.albnosh = w.albnosh + t.alb * f;
+            const R tempb = t.sunwall * f;
+            const R tempbwall = f * R(-1) + R(1);
+            bub = (tempb + bub) > R(0) ? R(1) : R(0);
+            bubwall = (tempbwall + bubwall) > R(0) ? R(1) : R(0);
+            w.lwall = w.lwall + bub * lwall;
+            w.albwall = w.albwall + bub * albedo_b;
+            w.wall = w.wall + bub;
+            w.albwallnosh = w.albwallnosh + bubwall * albedo_b;
+            if (st < first) s = w;
+        }
+        const R sunin = s.wall > R(0) ? R(1) : R(0), wallin = s.albwallnosh > R(0) ? R(1) : R(0);
+        const R sunin2 = w.wall > R(0) ? R(1) : R(0), wallin2 = w.albwallnosh > R(0) ? R(1) : R(0);
+        const double *bd = bounds + dir * 3;
+        int facesh;
+        if (bd[2] == 0.0)
+            facesh = (int)(aspect < bd[0] || aspect >= bd[1]) - (wallbol ? 1 : 0) + 1;
+        else
+            facesh = 1 - (int)(aspect > bd[0] || aspect <= bd[1]);
+        const bool keep = (w.wall == secondf) && facesh == 0;
+        const R f1 = firstf + R(1), s1 = secondf + R(1);
+        if (keep) { w.wall = 0; w.lwall = 0; w.albwall = 0; }
+        R gvf2 = gvf_mix(w.wall, w.sh, s1, secondf, sunin2);
+        if (gvf2 > R(1)) gvf2 = R(1);
+        const R lup1 = gvf_mix(s.lwall, s.lupsh, f1, firstf, sunin);
+        const R lup2 = gvf_mix(w.lwall, w.lupsh, s1, secondf, sunin2);
+        const R alb1 = gvf_mix(s.albwall, s.albsh, f1, firstf, sunin);
+        const R alb2 = gvf_mix(w.albwall, w.albsh, s1, secondf, sunin2);
+        const R nosh1 = gvf_mix(s.albwallnosh, s.albnosh, f1, firstf, wallin);
+        const R nosh2 = gvf_mix(w.albwallnosh, w.albnosh, secondf, secondf, wallin2);
+        const R notbu = me.bu * R(-1) + R(1);
+        const R lup = (lup1 * R(0.5) + lup2 * R(0.4)) / R(0.9) + me.lup1 * notbu;
+        const R alb = (alb1 * R(0.5) + alb2 * R(0.4)) / R(0.9) + me.alb * notbu * me.sh;
+        const R nosh = (nosh1 * R(0.5) + nosh2 * R(0.4)) / R(0.9) * me.bu + me.alb * notbu;
+        acc_lup = acc_lup + lup;
+        acc_alb = acc_alb + alb;
+        acc_nosh = acc_nosh + nosh;
+        acc_sum = acc_sum + gvf2;
+        const int az = 5 + 20 * dir;
+        const bool in_card[4] = {az < 180, az >= 90 && az < 270, az >= 180, az >= 270 || az < 90};
+        for (int k = 0; k < 4; ++k)
+            if (in_card[k]) {
+                card[k][0] = card[k][0] + lup;
+                card[k][1] = card[k][1] + alb;
+                card[k][2] = card[k][2] + nosh;
+            }
+    }
+    const R ambient = ambient_sbc * emis * t4a;
+    out[p] = acc_lup / R(18) + ambient;
+    out[n + p] = acc_alb / R(18);
+    out[2 * n + p] = acc_nosh / R(18);
+    for (int k = 0; k < 4; ++k) {
+        out[(3 + 3 * k) * n + p] = card[k][0] / R(9) + ambient;
+        out[(4 + 3 * k) * n + p] = card[k][1] / R(9);
+        out[(5 + 3 * k) * n + p] = card[k][2] / R(9);
+    }
+    out[15 * n + p] = acc_sum;
+    out[16 * n + p] = me.bu == R(0) ? R(1) : acc_sum / R(18);
+}
+
+// params: Ta, ewall, SBC, albedo_b, landcover, Twater
+template <typename R>
+int launch_gvf(const FieldSet &F, int rows, int cols, const int32_t *offsets, int stride, const double *bounds,
+               int first, int second, const double *q, void *work, void *out, hipStream_t s) {
+    const long long n = (long long)rows * cols;
+    const double ta = q[0], ewall = q[1], sbc = q[2], albedo_b = q[3], twater = q[5];
+    const int landcover = (int)q[4];
+    const double t4a = pow(ta + 273.15, 4.0);
+    GvfSample<R> *rec = (GvfSample<R> *)work;
+    R *o = (R *)out;
+    const dim3 grid((unsigned)((n + 255) / 256)), block(256);
+    hipLaunchKernelGGL(k_gvf_pack<R>, grid, block, 0, s, F, n, (R)ta, (R)273.15, (R)sbc, t4a, landcover,
+                       twater - ta, rec, o + 17 * n);
+    hipLaunchKernelGGL(k_gvf_walk<R>, grid, block, 0, s, F, rec, rows, cols, offsets, stride, bounds, first,
+                       second, (R)ta, (R)273.15, (R)(sbc * ewall), (R)(sbc * ewall * t4a), (R)albedo_b, (R)sbc,
+                       (R)t4a, o);
+    return hip_check(hipGetLastError(), "gvf launch");
 }
 
 int launch_adjoint_hour(const sw_domain *dom, const int32_t *bs, int total, int hour, bool top,
@@ -1931,6 +2105,24 @@ SW_SUB_ENTRY(sw_lside, 3)
 SW_SUB_ENTRY(sw_surface_temperature, 4)
 SW_SUB_ENTRY(sw_assemble_tmrt, 5)
 #undef SW_SUB_ENTRY
+
+int sw_gvf(const void *const *fields, const long long *counts, int n_fields, int rows, int cols,
+           const int32_t *offsets, const double *bounds, int stride, int first, int second,
+           const double *params, int n_params, int real_bytes, void *work, void *out, void *stream) {
+    if (!fields || !counts || !offsets || !bounds || !params || !work || !out)
+        return fail(SW_ERR_ARGUMENT, "sw_gvf: NULL pointer");
+    if (n_fields != 10 || n_params != 6) return fail(SW_ERR_ARGUMENT, "sw_gvf: expects 10 fields, 6 parameters");
+    if (real_bytes != 4 && real_bytes != 8) return fail(SW_ERR_ARGUMENT, "sw_gvf: precision is 4 or 8 bytes");
+    if (rows <= 0 || cols <= 0 || first < 1 || second < 1 || stride < second)
+        return fail(SW_ERR_ARGUMENT, "sw_gvf: bad raster or step counts");
+    const long long n = (long long)rows * cols;
+    FieldSet F = {};
+    const int rc = make_fieldset(fields, counts, n_fields, n, F);
+    if (rc) return rc;
+    const hipStream_t s = (hipStream_t)stream;
+    return real_bytes == 8 ? launch_gvf<double>(F, rows, cols, offsets, stride, bounds, first, second, params, work, out, s)
+                           : launch_gvf<float>(F, rows, cols, offsets, stride, bounds, first, second, params, work, out, s);
+}
 
 int sw_utci_scalar(const float *tmrt, long long n, const double *coefficients, double ta32,
                    int sentinel, const uint8_t *ground, float *out, void *stream) {

@@ -43,12 +43,17 @@ def _shape_of(values, shape):
     return (1, 1)
 
 
-def _run(name, values, params, n_out, dtype, shape=None, fp64_fields=()):
+def working_dtype(dtype):
     dtype = np.dtype(dtype)
     if dtype not in (np.dtype(np.float32), np.dtype(np.float64)):
         raise ValueError('dtype must be float32 or float64')
-    shape = _shape_of(values, shape)
-    n = int(np.prod(shape))
+    return dtype
+
+
+def field_table(values, n, dtype, fp64_fields=()):
+    """Device pointers and value counts of the fields of a sub-model call: device arrays of the
+    expected dtype are passed as they are, anything else is uploaded (the uploads are returned
+    and must stay alive until the call has finished)."""
     ptrs, counts, keep = [], [], []
     for k, v in enumerate(values):
         want = np.dtype(np.float64) if k in fp64_fields else dtype
@@ -66,11 +71,17 @@ def _run(name, values, params, n_out, dtype, shape=None, fp64_fields=()):
         if c not in (1, n):
             raise ValueError(f'field {k} holds {c} values; expected 1 or {n}')
         counts.append(c)
+    return (ctypes.c_void_p * len(ptrs))(*ptrs), (ctypes.c_longlong * len(counts))(*counts), keep
+
+
+def _run(name, values, params, n_out, dtype, shape=None, fp64_fields=()):
+    dtype = working_dtype(dtype)
+    shape = _shape_of(values, shape)
+    n = int(np.prod(shape))
+    P, C, keep = field_table(values, n, dtype, fp64_fields)
     out = rt.DeviceArray((n_out, *shape), dtype)
-    P = (ctypes.c_void_p * len(ptrs))(*ptrs)
-    C = (ctypes.c_longlong * len(counts))(*counts)
     Q = (ctypes.c_double * len(params))(*[float(x) for x in params])
-    rt.check(_entry(name)(P, C, len(ptrs), n, Q, len(params), dtype.itemsize, out.ptr, None))
+    rt.check(_entry(name)(P, C, len(values), n, Q, len(params), dtype.itemsize, out.ptr, None))
     rt.check(rt.lib().sw_stream_sync(None))
     del keep
     return tuple(out.plane(k, shape) for k in range(n_out))

@@ -315,3 +315,4 @@ def cast_shadow(geometry, azimuth, altitude):
 # per-step sub-models with the reference's names (model.py:736-958)
 from .flux import (Kup_veg_2015a, Lside_veg_v2022a, Perez_v3, TsWaveDelay_2015a, assemble_tmrt,  # noqa: E402,F401
                    cylindric_wedge, surface_temperature)
+from .groundview import GvfGeometry, GvfResult, gvf, prepare_gvf_geometry  # noqa: E402,F401

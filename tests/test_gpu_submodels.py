@@ -110,3 +110,70 @@ def test_lside_matches_umep_golden(rt, case):
         out = np.stack([v.to_host() for v in flux.Lside_veg_v2022a(*args, ani)])
         expect = ref if ani == 0 else np.stack(umep.lside(*args, 1))
         np.testing.assert_allclose(out, expect, rtol=1e-12, atol=1e-9)
+
+
+def _gvf_fields(seed, rows, cols, landcover):
+    rng = np.random.default_rng(seed)
+    u = lambda: rng.random((rows, cols))
+    buildings = (u() < 0.8).astype(np.float64)
+    buildings[u() < 0.1] = 0.5                     # a ground mask of any value: the walk's running minimum
+    walls = np.where(u() < 0.3, rng.uniform(1.0, 12.0, (rows, cols)), 0.0)
+    wallsun = np.where(u() < 0.6, walls, walls * 0.3)
+    dirwalls = rng.uniform(0.0, 360.0, (rows, cols))
+    shadow = u()
+    shadow[u() < 0.3] = 1.0
+    shadow[u() < 0.2] = 0.0
+    f = dict(buildings=buildings, walls=walls, wallsun=wallsun, dirwalls=dirwalls, shadow=shadow,
+             Tg=rng.uniform(-2.0, 15.0, (rows, cols)), emis=rng.uniform(0.9, 0.98, (rows, cols)),
+             alb=rng.uniform(0.1, 0.3, (rows, cols)), Ta=23.4)
+    if landcover:
+        f.update(Tgwall=rng.uniform(0.0, 9.0, (rows, cols)), lc=rng.choice([1.0, 2.0, 3.0, 5.0, 6.0, 7.0], (rows, cols)),
+                 Twater=17.9)
+    else:
+        f.update(Tgwall=4.2, lc=np.zeros((rows, cols)), Twater=None)
+    return f
+
+
+@pytest.mark.parametrize('landcover', [0, 1])
+@pytest.mark.parametrize('scale,first,second', [(1.0, 1.0, 22.0), (2.0, 1.0, 6.0)])
+def test_gvf_matches_oracle(rt, scale, first, second, landcover):
+    """Standalone ground view factors (17 planes + the water-replaced Tg) on a 23 x 31 raster of
+    random fields, a fractional shadow and a ground mask with 0.5 values; land cover with water
+    pixels; a scale of 2 makes the near field two steps long."""
+    from solweig_amd import model
+    print(f'gvf scale {scale} first {first} second {second} landcover {landcover}', flush=True)
+    f = _gvf_fields(5 + landcover, 23, 31, landcover)
+    geo = model.prepare_gvf_geometry(f['walls'], f['buildings'], f['dirwalls'], scale, first, second)
+    tg_ref = f['Tg'].copy()
+    ref = umep.gvf(f['wallsun'], f['walls'], f['buildings'], scale, f['shadow'], first, second, f['dirwalls'],
+                   tg_ref, f['Tgwall'], f['Ta'], f['emis'], 0.9, f['alb'], 0.2, f['Twater'], f['lc'], landcover)
+    ref = np.stack(ref)
+    for dtype, rtol, atol in ((np.float64, 1e-11, 1e-9), (np.float32, 5e-5, 2e-4)):
+        res = model.gvf(geo, f['wallsun'], f['shadow'], f['Tg'], f['Tgwall'], f['Ta'], f['emis'], 0.9, f['alb'],
+                        umep.SBC, 0.2, f['Twater'], f['lc'], landcover, dtype=dtype)
+        got = np.stack([v.to_host() for v in res.outputs]).astype(np.float64)
+        for k in range(17):
+            np.testing.assert_allclose(got[k], ref[k], rtol=rtol, atol=atol, err_msg=f'{np.dtype(dtype)} plane {k}')
+        np.testing.assert_allclose(res.surface_temperature.to_host(), tg_ref, rtol=rtol, atol=1e-6)
+
+
+@pytest.mark.parametrize('prefix,hour', [('dayA_', 8), ('dayA_', 14), ('dayB_', 10), ('dayC_', 12)])
+def test_gvf_matches_umep_golden(rt, golden, prefix, hour):
+    """The ground view factors UMEP wrote for three domains (walls, land cover) against the device."""
+    from solweig_amd import model
+    from golden_inputs import static_from_golden
+    p = static_from_golden(golden, prefix)
+    met = golden[prefix + 'met']
+    sh = golden[prefix + f'shadow{hour}'].astype(np.float64)
+    shadow = golden[prefix + f'gvf{hour}_shadow'].astype(np.float64)
+    tg = golden[prefix + f'gvf{hour}_Tg'].astype(np.float64)
+    twater = float(np.mean(met[:, 11])) if p['landcover'] else None
+    geo = model.prepare_gvf_geometry(p['walls'], p['buildings'], p['dirwalls'], 1.0, 1.0, 22.0)
+    res = model.gvf(geo, sh[4], shadow, tg, 3.5, float(met[hour, 11]), p['emis_grid'], 0.9, p['alb_grid'],
+                    umep.SBC, 0.2, twater, p['lc_grid'], p['landcover'])
+    got = np.stack([v.to_host() for v in res.outputs])
+    np.testing.assert_allclose(got, golden[prefix + f'gvf{hour}'].astype(np.float64), rtol=2e-6, atol=1e-4)
+    ref = np.stack(umep.gvf(sh[4], p['walls'], p['buildings'], 1.0, shadow, 1.0, 22.0, p['dirwalls'], tg.copy(),
+                            3.5, float(met[hour, 11]), p['emis_grid'], 0.9, p['alb_grid'], 0.2, twater,
+                            p['lc_grid'], p['landcover']))
+    np.testing.assert_allclose(got, ref, rtol=1e-11, atol=1e-9)
