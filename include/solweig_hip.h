@@ -268,6 +268,39 @@ int sw_gvf(const void *const *fields, const long long *counts, int n_fields, int
            int second_steps, const double *params, int n_params, int real_bytes, void *work_dev,
            void *out_dev, void *stream);
 
+/* Visibility flags of a caller's patch geometry (replaces prepare_geometry's pack,
+ * model.py:273-306, geometry.cu:1-8): shmat, vegshmat, vbshvegshmat are [n][patches] device
+ * arrays of mask_bytes-sized values (1: uint8, 4: float, 8: double). flags_dev receives
+ * [4][ceil(patches / 32)][n] words; bit k % 32 of word [f][k / 32][p] is flag f of patch k at
+ * pixel p: 0 sky (sh == 1 and veg == 1), 1 vegetation (veg == 0 or vb == 0), 2 wall
+ * ((1 - sh) * vb == 1), 3 obstructed (any of the three == 0).                             */
+int sw_pack_visibility(const void *shmat_dev, const void *vegshmat_dev, const void *vbshvegshmat_dev,
+                       int mask_bytes, long long n, int patches, uint32_t *flags_dev, void *stream);
+/* Anisotropic-sky longwave of one step (replaces model.longwave, model.py:367-392, and
+ * longwave.cu; UMEP Lcyl_v2022a / define_patch_characteristics, umep_solweig.py:946-1359).
+ * table_dev: [patches][20] FP64 per-patch coefficients built by the host for the call (columns:
+ * sky Ldown, sky Lside, vegetation x cos, x sin, shaded wall x cos, x sin, steradian, cos alt,
+ * sin alt, direction cosines E, W, N, S, yi, sunlit-below and shaded-from thresholds of
+ * tan(asvf) + yi, sun-test eligibility). fields: Lup, Tgwall (working precision), tan(asvf)
+ * (FP64); params: Ta, ewall, SBC. out_dev: 6 planes Ldown, Lside, Least, Lwest, Lnorth, Lsouth. */
+int sw_longwave(const uint32_t *flags_dev, int patches, long long n, const double *table_dev,
+                const void *const *fields, const long long *counts, int n_fields, const double *params,
+                int n_params, int real_bytes, void *out_dev, void *stream);
+/* Shortwave on the body's sides (replaces model.Kside_veg_v2022a, model.py:443-506, and
+ * shortwave.cu; UMEP umep_solweig.py:666-911), every (cyl, anisotropic_diffuse) branch.
+ * diffsh_dev: [patches][n] working-precision diffuse transmissivities (prepare_shortwave's
+ * transpose, model.py:401-411); table_dev: [patches][20] FP64 (lumChi, cylinder incidence,
+ * steradian, box incidences E, S, W, N, sunlit-surface and shaded-surface terms, reflection
+ * cosines E, S, W, N, yi, the two thresholds, sector flags). flags, diffsh and table may be NULL
+ * when anisotropic_diffuse is 0. fields: shadow, F_sh, KupE, KupS, KupW, KupN, svfE, svfS, svfW,
+ * svfN, svfEveg, svfSveg, svfWveg, svfNveg (working precision), tan(asvf) (FP64). params (16):
+ * radI, radD, radG, cos(altitude), psi, albedo, sin of the sun's azimuth on faces E, S, W, N,
+ * those faces lit (0/1), cyl, anisotropic_diffuse. out_dev: 7 planes Keast, Ksouth, Kwest,
+ * Knorth, KsideI, KsideD, Kside.                                                          */
+int sw_kside(const uint32_t *flags_dev, const void *diffsh_dev, int patches, long long n,
+             const double *table_dev, const void *const *fields, const long long *counts, int n_fields,
+             const double *params, int n_params, int real_bytes, void *out_dev, void *stream);
+
 /* UTCI of a Tmrt plane for scalar weather (replaces utci.utci_calculator's scalar-weather
  * path, utci.py:22-33, utci.cu:243-523): ``coefficients`` are 7 host FP64 values c_k with
  * UTCI = sum_k c_k (Tmrt - ta32)^k; sentinel != 0 writes -999; ground_dev (nullable) NaNs

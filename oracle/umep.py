@@ -571,6 +571,106 @@ def kside_cylinder(radI, radD, shadow, azimuth, altitude, albedo, KupE, KupS, Ku
     return KupE * 0.5, KupS * 0.5, KupW * 0.5, KupN * 0.5, KsideI, KsideD, Kside
 
 
+def _kvikt(svf, svfveg, total=4.4897):
+    """Building and vegetation view weights of a vertical face (Kvikt_veg, umep_solweig.py:930-943)."""
+    poly = lambda x: 63.227 * x ** 6 - 161.51 * x ** 5 + 156.91 * x ** 4 - 70.424 * x ** 3 + 16.773 * x ** 2 - 0.4863 * x
+    wall = (total - poly(svf)) / total
+    veg = (total - poly(svfveg + svf - 1)) / total
+    return veg - wall, wall
+
+
+def kside(radI, radD, radG, shadow, svfS, svfW, svfN, svfE, svfEveg, svfSveg, svfWveg, svfNveg, azimuth,
+          altitude, psi, t, albedo, F_sh, KupE, KupS, KupW, KupN, cyl, lv, anisotropic_diffuse, diffsh, asvf,
+          shmat, vegshmat, vbshvegshmat):
+    """Kside_veg_v2022a, every branch (umep_solweig.py:666-911): cylinder or box body, anisotropic
+    (Perez patches) or weighted isotropic diffuse. Returns (Keast, Ksouth, Kwest, Knorth, KsideI,
+    KsideD, Kside); the box and isotropic branches leave KsideD and Kside zero as UMEP does."""
+    shape = np.shape(shadow)
+    cos_alt = np.cos(altitude * DEG)
+    # direct beam: on the cylinder, or per face lit by the sun (face f at azimuth rotation r)
+    faces = {'E': (0, azimuth > 360 - t or azimuth <= 180 - t), 'S': (90, 90 - t < azimuth <= 270 - t),
+             'W': (180, 180 - t < azimuth <= 360 - t), 'N': (270, azimuth <= 90 - t or azimuth > 270 - t)}
+    direct = {}
+    if cyl == 1:
+        KsideI = shadow * radI * cos_alt
+        direct = {f: 0 for f in faces}
+    else:
+        for f, (rot, lit) in faces.items():
+            direct[f] = radI * shadow * cos_alt * np.sin((azimuth - rot + t) * DEG) if lit else 0
+        KsideI = shadow * 0
+    zero = lambda: np.zeros(shape)
+    KsideD, Kside = zero(), zero()
+    svf = {'E': (svfE, svfEveg), 'S': (svfS, svfSveg), 'W': (svfW, svfWveg), 'N': (svfN, svfNveg)}
+    out = {}
+    if anisotropic_diffuse != 1:
+        for f, (s_, v_) in svf.items():
+            veg, wall = _kvikt(s_, v_)
+            w = wall + veg * (1 - psi)
+            out[f] = direct[f] + (radD * (1 - w) + albedo * (w * (radG * (1 - F_sh) + radD * F_sh))
+                                  + {'E': KupE, 'S': KupS, 'W': KupW, 'N': KupN}[f]) * 0.5
+        return out['E'], out['S'], out['W'], out['N'], KsideI, KsideD, Kside
+    alt, azi, lum = lv[:, 0], lv[:, 1], lv[:, 2]
+    ster = steradians(alt)
+    radTot = np.zeros(1)
+    for i in range(alt.shape[0]):
+        radTot += lum[i] * ster[i] * np.sin(alt[i] * DEG)
+    lumChi = (lum * radD) / radTot
+    sunlit_surface = ((albedo * (radI * cos_alt) + (radD * 0.5)) / np.pi)
+    shaded_surface = ((albedo * radD * 0.5) / np.pi)
+    Kref_veg, Kref_sun, Kref_sh = zero(), zero(), zero()
+    # face f: diffuse sector (azimuth bounds inclusive above), reflection sector (exclusive), face normal
+    diff_sector = {'E': lambda a: a > 360 or a <= 180, 'S': lambda a: 90 < a <= 270,
+                   'W': lambda a: 180 < a <= 360, 'N': lambda a: a > 270 or a <= 90}
+    refl_sector = {'E': lambda a: a > 360 or a < 180, 'S': lambda a: 90 < a < 270,
+                   'W': lambda a: 180 < a < 360, 'N': lambda a: a > 270 or a < 90}
+    normal = {'E': 90, 'S': 180, 'W': 270, 'N': 0}
+    dRad = {f: zero() for f in faces}
+    r_veg = {f: zero() for f in faces}
+    r_sun = {f: zero() for f in faces}
+    r_sh = {f: zero() for f in faces}
+    for k in range(alt.shape[0]):
+        ck = np.cos(alt[k] * DEG)
+        if cyl == 1:
+            KsideD += diffsh[:, :, k] * lumChi[k] * (ck * np.cos(0)) * ster[k]
+        else:
+            for f in faces:
+                if diff_sector[f](azi[k]):
+                    dRad[f] += diffsh[:, :, k] * lumChi[k] * (ck * np.cos((normal[f] - azi[k] + t) * DEG)) * ster[k]
+        vegsh = (vegshmat[:, :, k] == 0) | (vbshvegshmat[:, :, k] == 0)
+        Kref_veg += shaded_surface * vegsh * ster[k] * ck
+        wall = ((1 - shmat[:, :, k]) * vbshvegshmat[:, :, k]) == 1
+        if cyl == 1:
+            sun, shd = shaded_or_sunlit(altitude, azimuth, alt[k], azi[k], asvf)
+            Kref_sun += sunlit_surface * sun * wall * ster[k] * ck
+            Kref_sh += shaded_surface * shd * wall * ster[k] * ck
+            continue
+        for f in faces:
+            if refl_sector[f](azi[k]):
+                r_veg[f] += shaded_surface * ster[k] * ck * vegsh * np.cos((normal[f] - azi[k] + t) * DEG)
+        d = np.abs(azimuth - azi[k])
+        if 90 < d < 270:
+            sun, shd = shaded_or_sunlit(altitude, azimuth, alt[k], azi[k], asvf)
+            Kref_sun += sunlit_surface * sun * wall * ster[k] * ck
+            Kref_sh += shaded_surface * shd * wall * ster[k] * ck
+            for f in faces:
+                if refl_sector[f](azi[k]):
+                    c = np.cos((normal[f] - azi[k] + t) * DEG)
+                    r_sun[f] += sunlit_surface * sun * ster[k] * ck * wall * c
+                    r_sh[f] += shaded_surface * shd * ster[k] * ck * wall * c
+        else:
+            Kref_sh += shaded_surface * wall * ster[k] * ck
+            for f in faces:
+                if refl_sector[f](azi[k]):
+                    r_sh[f] += shaded_surface * ster[k] * ck * wall * np.cos((normal[f] - azi[k] + t) * DEG)
+    up = {'E': KupE, 'S': KupS, 'W': KupW, 'N': KupN}
+    if cyl == 1:
+        Kside = KsideI + KsideD + Kref_sun + Kref_sh + Kref_veg
+        return KupE * 0.5, KupS * 0.5, KupW * 0.5, KupN * 0.5, KsideI, KsideD, Kside
+    for f in faces:
+        out[f] = direct[f] + dRad[f] + r_sun[f] + r_sh[f] + r_veg[f] + up[f] * 0.5
+    return out['E'], out['S'], out['W'], out['N'], KsideI, KsideD, Kside
+
+
 def kup(radI, radD, radG, altitude, svfbuveg, albedo_b, F_sh, gvfalb, gvfalbE, gvfalbS, gvfalbW,
         gvfalbN, gvfalbnosh, gvfalbnoshE, gvfalbnoshS, gvfalbnoshW, gvfalbnoshN):
     """Reflected shortwave from the ground (Kup_veg_2015a, umep_solweig.py:915-927)."""

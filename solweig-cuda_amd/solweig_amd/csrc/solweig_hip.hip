@@ -2,10 +2,10 @@
 //
 // Layout: every raster is a row-major plane [rows][cols]; a domain is described by an
 // sw_domain record in device memory; per-(domain, hour) scalars live in an FP64 table of
-// SW_HOUR_STRIDE values built on the host. Work is tiled 32 columns x 8 rows per 256-thread
-// workgroup (a wave covers two 32-pixel row segments), tiles of all domains of a launch are
+// SW_HOUR_STRIDE values built on the host. Work is tiled W columns x 256 / W rows per 256-thread
+// workgroup (W in 16..128, chosen per domain at run time), tiles of all domains of a launch are
 // concatenated, and the workgroup index is remapped so that consecutive tiles stay on one XCD
-// (one L2) of the 8.
+// (one L2) of the 8. The standalone sub-model kernels at the end map one thread to one pixel.
 //
 // Exactness: the building/vegetation march (shadows and sky-view visibility) compares FP64
 // values formed exactly as the NumPy reference forms them (float32 rasters widened to FP64,
@@ -18,6 +18,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <type_traits>
 
 #include "../../../include/solweig_hip.h"
 
@@ -1851,6 +1852,259 @@ int launch_gvf(const FieldSet &F, int rows, int cols, const int32_t *offsets, in
     return hip_check(hipGetLastError(), "gvf launch");
 }
 
+// =====================================================================================
+// Anisotropic-sky sub-models for caller masks (model.prepare_geometry, longwave,
+// prepare_shortwave and Kside_veg_v2022a, model.py:273-506). The three [rows, cols, K] masks of a
+// patch geometry are packed once into four flag bit planes [4][W][P], W = ceil(K / 32): bit
+// k % 32 of word [f][k / 32][p] is flag f of patch k at pixel p, with f = 0 sky (sh == 1 and
+// veg == 1), 1 vegetation (veg == 0 or vb == 0), 2 wall ((1 - sh) vb == 1), 3 obstructed (sh == 0
+// or veg == 0 or vb == 0): the comparisons of umep_solweig.py:666-911 and 1205-1359. Per-patch
+// coefficients are host FP64 tables built for each call; the pixel kernels sum over the patches
+// in UMEP's order with contraction off, so the FP64 form rounds as the NumPy code does.
+// =====================================================================================
+template <typename M>
+__global__ void k_pack_flags(const M *sh, const M *veg, const M *vb, long long n, int patches, int words,
+                             uint32_t *flags) {
+#pragma clang fp contract(off)
+    using A = typename std::conditional<std::is_integral<M>::value, int, M>::type;
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n * words) return;
+    const int w = (int)(i / n);
+    const long long p = i - (long long)w * n;
+    uint32_t fs = 0u, fv = 0u, fw = 0u, fo = 0u;
+    const int k0 = w * 32, k1 = min(patches, k0 + 32);
+    for (int k = k0; k < k1; ++k) {
+        const long long q = p * patches + k;
+        const A s = (A)sh[q], v = (A)veg[q], b = (A)vb[q];
+        const uint32_t bit = 1u << (k - k0);
+        if (s == A(1) && v == A(1)) fs |= bit;
+        if (v == A(0) || b == A(0)) fv |= bit;
+        if ((A(1) - s) * b == A(1)) fw |= bit;
+        if (s == A(0) || v == A(0) || b == A(0)) fo |= bit;
+    }
+    flags[(0LL * words + w) * n + p] = fs;
+    flags[(1LL * words + w) * n + p] = fv;
+    flags[(2LL * words + w) * n + p] = fw;
+    flags[(3LL * words + w) * n + p] = fo;
+}
+
+// Longwave patch table columns (one FP64 row per patch): sky Ldown / Lside of the patch;
+// vegetation and shaded-wall terms (surface x steradian x cos or sin of the patch altitude);
+// steradian, cos, sin; the cardinal direction cosines E, W, N, S (0 outside the face's sector);
+// yi of the wall classification, its two thresholds, and the eligibility of the sun test.
+enum { LW_LDN, LW_LSD, LW_VEG_C, LW_VEG_S, LW_SH_C, LW_SH_S, LW_STER, LW_COS, LW_SIN, LW_DE, LW_DW, LW_DN, LW_DS,
+       LW_YI, LW_THR_A, LW_THR_B, LW_ELIG, LW_COLS = 20 };
+
+// fields: Lup, Tgwall (R), tan(asvf) (FP64). Output planes Ldown, Lside, Least, Lwest, Lnorth,
+// Lsouth (Lcyl_v2022a + define_patch_characteristics, umep_solweig.py:946-1029, 1205-1359).
+template <typename R>
+__global__ void __launch_bounds__(256) k_longwave(const uint32_t *flags, int words, long long n, int patches,
+                                                  const double *tab, FieldSet F, R ta, R t273, R ewall_sbc,
+                                                  R one_minus_ewall, R *out) {
+#pragma clang fp contract(off)
+    const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const R PI = R(3.141592653589793);
+    const R lup = fld<R>(F, 0, p);
+    const double tan_asvf = fld<double>(F, 2, p);
+    const R ss = (ewall_sbc * pow(ta + fld<R>(F, 1, p) + t273, R(4))) / PI;
+    R d_sky = 0, s_sky = 0, d_veg = 0, s_veg = 0, d_sun = 0, s_sun = 0, d_sh = 0, s_sh = 0;
+    R E = 0, W = 0, N = 0, S = 0;
+    for (int w = 0; w < words; ++w) {
+      const uint32_t f_sky = flags[(0LL * words + w) * n + p], f_veg = flags[(1LL * words + w) * n + p];
+      const uint32_t f_wall = flags[(2LL * words + w) * n + p];
+      const int k1 = min(patches, 32 * w + 32);
+      for (int k = 32 * w; k < k1; ++k) {
+        const int b = k - 32 * w;
+        const double *c = tab + (long long)k * LW_COLS;
+        const R dE = (R)c[LW_DE], dW = (R)c[LW_DW], dN = (R)c[LW_DN], dS = (R)c[LW_DS];
+        if ((f_sky >> b) & 1u) {
+            const R lsd = (R)c[LW_LSD];
+            d_sky = d_sky + (R)c[LW_LDN];
+            s_sky = s_sky + lsd;
+            E = E + lsd * dE; W = W + lsd * dW; N = N + lsd * dN; S = S + lsd * dS;
+        }
+        if ((f_veg >> b) & 1u) {
+            const R vc = (R)c[LW_VEG_C];
+            s_veg = s_veg + vc;
+            d_veg = d_veg + (R)c[LW_VEG_S];
+            E = E + vc * dE; W = W + vc * dW; N = N + vc * dN; S = S + vc * dS;
+        }
+        if ((f_wall >> b) & 1u) {
+            bool sun = false, shd = true;
+            if (c[LW_ELIG] != 0.0) {
+                const double x = tan_asvf + c[LW_YI];
+                sun = x < c[LW_THR_A];
+                shd = x >= c[LW_THR_B];
+            }
+            if (sun) {
+                const R st = ss * (R)c[LW_STER];
+                const R sc = st * (R)c[LW_COS];
+                s_sun = s_sun + sc;
+                d_sun = d_sun + st * (R)c[LW_SIN];
+                E = E + sc * dE; W = W + sc * dW; N = N + sc * dN; S = S + sc * dS;
+            }
+            if (shd) {
+                const R hc = (R)c[LW_SH_C];
+                s_sh = s_sh + hc;
+                d_sh = d_sh + (R)c[LW_SH_S];
+                E = E + hc * dE; W = W + hc * dW; N = N + hc * dN; S = S + hc * dS;
+            }
+        }
+      }
+    }
+    const R refl = (((d_sky + lup) * one_minus_ewall) * R(0.5)) / PI;
+    R d_ref = 0, s_ref = 0;
+    for (int k = 0; k < patches; ++k) {
+        if (!((flags[(3LL * words + (k >> 5)) * n + p] >> (k & 31)) & 1u)) continue;
+        const double *c = tab + (long long)k * LW_COLS;
+        const R st = refl * (R)c[LW_STER];
+        const R sc = st * (R)c[LW_COS];
+        s_ref = s_ref + sc;
+        d_ref = d_ref + st * (R)c[LW_SIN];
+        E = E + sc * (R)c[LW_DE]; W = W + sc * (R)c[LW_DW]; N = N + sc * (R)c[LW_DN]; S = S + sc * (R)c[LW_DS];
+    }
+    out[p] = d_sky + d_veg + d_sh + d_sun + d_ref;
+    out[n + p] = s_sky + s_veg + s_sh + s_sun + s_ref;
+    out[2 * n + p] = E;
+    out[3 * n + p] = W;
+    out[4 * n + p] = N;
+    out[5 * n + p] = S;
+}
+
+// Shortwave patch table columns: lumChi, the cylinder incidence cos(alt) cos(0), steradian, the
+// box incidences E, S, W, N; sunlit- and shaded-surface terms (x steradian x cos alt); the
+// reflection direction cosines E, S, W, N; yi and the two thresholds; sector flags (bits 0-3:
+// diffuse sectors E, S, W, N; 4-7 reflection sectors; 8: the box's sun test applies).
+enum { KS_LUMCHI, KS_ANGC, KS_STER, KS_ANG, KS_SUN_C = KS_ANG + 4, KS_SH_C, KS_DIRC, KS_YI = KS_DIRC + 4, KS_THR_A,
+       KS_THR_B, KS_FLAGS, KS_COLS = 20 };
+
+// fields: shadow, F_sh, KupE, KupS, KupW, KupN, svfE, svfS, svfW, svfN, svfEveg, svfSveg, svfWveg,
+// svfNveg (R), tan(asvf) (FP64). Output planes Keast, Ksouth, Kwest, Knorth, KsideI, KsideD,
+// Kside (Kside_veg_v2022a, umep_solweig.py:666-911).
+struct KsideParams {
+    double radI, radD, radG, cos_alt, one_minus_psi, albedo, sin_face[4];
+    int lit[4], cyl, ani;
+};
+
+template <typename R>
+__global__ void __launch_bounds__(256) k_kside(const uint32_t *flags, int words, const R *diffsh, long long n,
+                                               int patches, const double *tab, FieldSet F, KsideParams q,
+                                               R *out) {
+#pragma clang fp contract(off)
+    const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const R radI = (R)q.radI, radD = (R)q.radD, radG = (R)q.radG, cos_alt = (R)q.cos_alt;
+    const R albedo = (R)q.albedo;
+    const R shadow = fld<R>(F, 0, p), fsh = fld<R>(F, 1, p);
+    R direct[4] = {0, 0, 0, 0}, KsideI;
+    if (q.cyl == 1) {
+        KsideI = shadow * radI * cos_alt;
+    } else {
+        for (int f = 0; f < 4; ++f)
+            if (q.lit[f]) direct[f] = radI * shadow * cos_alt * (R)q.sin_face[f];
+        KsideI = shadow * R(0);
+    }
+    R K[4], KsideD = 0, Kside = 0;
+    if (q.ani != 1) {
+        const R total = R(4.4897);
+        for (int f = 0; f < 4; ++f) {
+            const R svf = fld<R>(F, 6 + f, p), veg = fld<R>(F, 10 + f, p);
+            const R wall = (total - view_poly(svf)) / total;
+            const R vegw = (total - view_poly(veg + svf - R(1))) / total - wall;
+            const R w = wall + vegw * (R)q.one_minus_psi;
+            K[f] = direct[f] + (radD * (R(1) - w) + albedo * (w * (radG * (R(1) - fsh) + radD * fsh)) +
+                                fld<R>(F, 2 + f, p)) * R(0.5);
+        }
+    } else {
+        const double tan_asvf = fld<double>(F, 14, p);
+        R dRad[4] = {0, 0, 0, 0}, rveg[4] = {0, 0, 0, 0}, rsun[4] = {0, 0, 0, 0}, rsh[4] = {0, 0, 0, 0};
+        R kveg = 0, ksun = 0, ksh = 0;
+        for (int k = 0; k < patches; ++k) {
+            const double *c = tab + (long long)k * KS_COLS;
+            const int fl = (int)c[KS_FLAGS];
+            const R ds = diffsh[(long long)k * n + p];
+            if (q.cyl == 1) {
+                KsideD = KsideD + ds * (R)c[KS_LUMCHI] * (R)c[KS_ANGC] * (R)c[KS_STER];
+            } else {
+                for (int f = 0; f < 4; ++f)
+                    if ((fl >> f) & 1) dRad[f] = dRad[f] + ds * (R)c[KS_LUMCHI] * (R)c[KS_ANG + f] * (R)c[KS_STER];
+            }
+            const R shc = (R)c[KS_SH_C], sunc = (R)c[KS_SUN_C];
+            const uint32_t bit = 1u << (k & 31);
+            if (flags[(1LL * words + (k >> 5)) * n + p] & bit) {
+                kveg = kveg + shc;
+                if (q.cyl != 1)
+                    for (int f = 0; f < 4; ++f)
+                        if ((fl >> (4 + f)) & 1) rveg[f] = rveg[f] + shc * (R)c[KS_DIRC + f];
+            }
+            if (!(flags[(2LL * words + (k >> 5)) * n + p] & bit)) continue;
+            const bool test = q.cyl == 1 || ((fl >> 8) & 1);
+            bool sun = false, shd = true;
+            if (test) {
+                const double x = tan_asvf + c[KS_YI];
+                sun = x < c[KS_THR_A];
+                shd = x >= c[KS_THR_B];
+            }
+            if (q.cyl == 1) {
+                if (sun) ksun = ksun + sunc;
+                if (shd) ksh = ksh + shc;
+                continue;
+            }
+            for (int f = 0; f < 4; ++f) {
+                if (!((fl >> (4 + f)) & 1)) continue;
+                const R d = (R)c[KS_DIRC + f];
+                if (sun) rsun[f] = rsun[f] + sunc * d;
+                if (shd) rsh[f] = rsh[f] + shc * d;
+            }
+        }
+        for (int f = 0; f < 4; ++f) {
+            const R half = fld<R>(F, 2 + f, p) * R(0.5);
+            K[f] = q.cyl == 1 ? half : direct[f] + dRad[f] + rsun[f] + rsh[f] + rveg[f] + half;
+        }
+        if (q.cyl == 1) Kside = KsideI + KsideD + ksun + ksh + kveg;
+    }
+    for (int f = 0; f < 4; ++f) out[f * n + p] = K[f];
+    out[4 * n + p] = KsideI;
+    out[5 * n + p] = KsideD;
+    out[6 * n + p] = Kside;
+}
+
+int launch_pack(const void *sh, const void *veg, const void *vb, int mask_bytes, long long n, int patches,
+                uint32_t *flags, hipStream_t s) {
+    const int words = (patches + 31) / 32;
+    const long long total = n * words;
+    const dim3 grid((unsigned)((total + 255) / 256)), block(256);
+    switch (mask_bytes) {
+    case 1: hipLaunchKernelGGL(k_pack_flags<uint8_t>, grid, block, 0, s, (const uint8_t *)sh, (const uint8_t *)veg,
+                               (const uint8_t *)vb, n, patches, words, flags); break;
+    case 4: hipLaunchKernelGGL(k_pack_flags<float>, grid, block, 0, s, (const float *)sh, (const float *)veg,
+                               (const float *)vb, n, patches, words, flags); break;
+    default: hipLaunchKernelGGL(k_pack_flags<double>, grid, block, 0, s, (const double *)sh, (const double *)veg,
+                                (const double *)vb, n, patches, words, flags); break;
+    }
+    return hip_check(hipGetLastError(), "k_pack_flags launch");
+}
+
+// params: Ta, ewall, SBC
+template <typename R>
+int launch_longwave(const uint32_t *flags, int patches, long long n, const double *tab, const FieldSet &F,
+                    const double *q, void *out, hipStream_t s) {
+    const dim3 grid((unsigned)((n + 255) / 256)), block(256);
+    hipLaunchKernelGGL(k_longwave<R>, grid, block, 0, s, flags, (patches + 31) / 32, n, patches, tab, F, (R)q[0],
+                       (R)273.15, (R)(q[1] * q[2]), (R)(1.0 - q[1]), (R *)out);
+    return hip_check(hipGetLastError(), "k_longwave launch");
+}
+
+template <typename R>
+int launch_kside(const uint32_t *flags, const void *diffsh, int patches, long long n, const double *tab,
+                 const FieldSet &F, const KsideParams &q, void *out, hipStream_t s) {
+    const dim3 grid((unsigned)((n + 255) / 256)), block(256);
+    hipLaunchKernelGGL(k_kside<R>, grid, block, 0, s, flags, (patches + 31) / 32, (const R *)diffsh, n, patches,
+                       tab, F, q, (R *)out);
+    return hip_check(hipGetLastError(), "k_kside launch");
+}
+
 int launch_adjoint_hour(const sw_domain *dom, const int32_t *bs, int total, int hour, bool top,
                         const sw_adjoint *adj, const int32_t *gofs, const double *gb, int first,
                         int second, double inv_n, hipStream_t s) {
@@ -2122,6 +2376,55 @@ int sw_gvf(const void *const *fields, const long long *counts, int n_fields, int
     const hipStream_t s = (hipStream_t)stream;
     return real_bytes == 8 ? launch_gvf<double>(F, rows, cols, offsets, stride, bounds, first, second, params, work, out, s)
                            : launch_gvf<float>(F, rows, cols, offsets, stride, bounds, first, second, params, work, out, s);
+}
+
+int sw_pack_visibility(const void *sh, const void *veg, const void *vb, int mask_bytes, long long n, int patches,
+                       uint32_t *flags, void *stream) {
+    if (!sh || !veg || !vb || !flags) return fail(SW_ERR_ARGUMENT, "sw_pack_visibility: NULL pointer");
+    if (mask_bytes != 1 && mask_bytes != 4 && mask_bytes != 8)
+        return fail(SW_ERR_ARGUMENT, "sw_pack_visibility: mask values are 1, 4 or 8 bytes");
+    if (n <= 0 || patches <= 0) return fail(SW_ERR_ARGUMENT, "sw_pack_visibility: empty raster or patch set");
+    return launch_pack(sh, veg, vb, mask_bytes, n, patches, flags, (hipStream_t)stream);
+}
+
+int sw_longwave(const uint32_t *flags, int patches, long long n, const double *table,
+                const void *const *fields, const long long *counts, int n_fields, const double *params,
+                int n_params, int real_bytes, void *out, void *stream) {
+    if (!flags || !table || !fields || !counts || !params || !out)
+        return fail(SW_ERR_ARGUMENT, "sw_longwave: NULL pointer");
+    if (n_fields != 3 || n_params != 3) return fail(SW_ERR_ARGUMENT, "sw_longwave: expects 3 fields, 3 parameters");
+    if (real_bytes != 4 && real_bytes != 8) return fail(SW_ERR_ARGUMENT, "sw_longwave: precision is 4 or 8 bytes");
+    if (patches <= 0) return fail(SW_ERR_ARGUMENT, "sw_longwave: no patches");
+    if (n <= 0) return SW_OK;
+    FieldSet F = {};
+    const int rc = make_fieldset(fields, counts, n_fields, n, F);
+    if (rc) return rc;
+    const hipStream_t s = (hipStream_t)stream;
+    return real_bytes == 8 ? launch_longwave<double>(flags, patches, n, table, F, params, out, s)
+                           : launch_longwave<float>(flags, patches, n, table, F, params, out, s);
+}
+
+int sw_kside(const uint32_t *flags, const void *diffsh, int patches, long long n, const double *table,
+             const void *const *fields, const long long *counts, int n_fields, const double *params, int n_params,
+             int real_bytes, void *out, void *stream) {
+    if (!fields || !counts || !params || !out) return fail(SW_ERR_ARGUMENT, "sw_kside: NULL pointer");
+    if (n_fields != 15 || n_params != 16) return fail(SW_ERR_ARGUMENT, "sw_kside: expects 15 fields, 16 parameters");
+    if (real_bytes != 4 && real_bytes != 8) return fail(SW_ERR_ARGUMENT, "sw_kside: precision is 4 or 8 bytes");
+    KsideParams q = {};
+    q.radI = params[0]; q.radD = params[1]; q.radG = params[2]; q.cos_alt = params[3];
+    q.one_minus_psi = 1.0 - params[4]; q.albedo = params[5];
+    for (int f = 0; f < 4; ++f) { q.sin_face[f] = params[6 + f]; q.lit[f] = params[10 + f] != 0.0; }
+    q.cyl = (int)params[14]; q.ani = (int)params[15];
+    if (q.ani == 1 && (!flags || !diffsh || !table || patches <= 0))
+        return fail(SW_ERR_ARGUMENT, "sw_kside: the anisotropic form needs flags, diffsh and a patch table");
+    if (q.ani != 1) patches = 0;
+    if (n <= 0) return SW_OK;
+    FieldSet F = {};
+    const int rc = make_fieldset(fields, counts, n_fields, n, F);
+    if (rc) return rc;
+    const hipStream_t s = (hipStream_t)stream;
+    return real_bytes == 8 ? launch_kside<double>(flags, diffsh, patches, n, table, F, q, out, s)
+                           : launch_kside<float>(flags, diffsh, patches, n, table, F, q, out, s);
 }
 
 int sw_utci_scalar(const float *tmrt, long long n, const double *coefficients, double ta32,

@@ -316,3 +316,5 @@ def cast_shadow(geometry, azimuth, altitude):
 from .flux import (Kup_veg_2015a, Lside_veg_v2022a, Perez_v3, TsWaveDelay_2015a, assemble_tmrt,  # noqa: E402,F401
                    cylindric_wedge, surface_temperature)
 from .groundview import GvfGeometry, GvfResult, gvf, prepare_gvf_geometry  # noqa: E402,F401
+from .anisotropic import (Kside_veg_v2022a, PreparedGeometry, PreparedShortwave, longwave,  # noqa: E402,F401
+                          prepare_geometry, prepare_shortwave)

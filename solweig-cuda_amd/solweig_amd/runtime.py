@@ -129,7 +129,8 @@ EXPORTED = ('sw_version', 'sw_last_error', 'sw_device_count', 'sw_set_device', '
             'sw_svf_reduce', 'sw_sky_sums', 'sw_hour_shadow', 'sw_hour_pixels', 'sw_hours_forward', 'sw_utci_scalar',
             'sw_copy', 'sw_graph_begin', 'sw_graph_end', 'sw_graph_launch', 'sw_graph_destroy',
             'sw_adjoint_hour', 'sw_day_adjoint', 'sw_cylindric_wedge', 'sw_wave_delay', 'sw_kup', 'sw_lside',
-            'sw_surface_temperature', 'sw_assemble_tmrt', 'sw_gvf')
+            'sw_surface_temperature', 'sw_assemble_tmrt', 'sw_gvf', 'sw_pack_visibility', 'sw_longwave',
+            'sw_kside')
 
 
 def check(rc):

@@ -177,3 +177,53 @@ def test_gvf_matches_umep_golden(rt, golden, prefix, hour):
                             3.5, float(met[hour, 11]), p['emis_grid'], 0.9, p['alb_grid'], 0.2, twater,
                             p['lc_grid'], p['landcover']))
     np.testing.assert_allclose(got, ref, rtol=1e-11, atol=1e-9)
+
+
+def _aniso():
+    import os
+    return np.load(os.path.join(os.path.dirname(__file__), 'golden', 'umep_aniso_golden.npz'))
+
+
+@pytest.mark.parametrize('case', range(3))
+def test_longwave_matches_umep_golden(rt, case):
+    """Anisotropic longwave over random masks (day patches from Perez, a night step) against the
+    vectors UMEP's Lcyl_v2022a wrote and the FP64 oracle; masks as uint8 and as float32."""
+    from solweig_amd import model
+    from test_oracle_submodels import lcyl_case
+    g = _aniso()
+    args, ref = lcyl_case(g, case)
+    esky, patches, Ta, Tgwall, ewall, Lup, sh, veg, vb, salt, sazi, asvf = args
+    masks = g[f'l{case}_masks']
+    for mk in (masks, masks.astype(np.float32)):
+        geo = model.prepare_geometry(patches, *mk)
+        for dtype, rtol, atol in ((np.float64, 1e-12, 1e-9), (np.float32, 2e-5, 2e-3)):
+            out = np.stack([v.to_host() for v in model.longwave(geo, esky, Ta, Tgwall, ewall, Lup, salt, sazi, asvf,
+                                                                dtype=dtype)])
+            np.testing.assert_allclose(out, ref, rtol=rtol, atol=atol, err_msg=str(np.dtype(dtype)))
+    tgw = np.random.default_rng(case).uniform(0.0, 9.0, Lup.shape)
+    oref = np.stack(umep.lcyl(esky, patches, Ta, tgw, ewall, Lup, sh, veg, vb, salt, sazi, asvf))
+    out = np.stack([v.to_host() for v in model.longwave(geo, esky, Ta, tgw, ewall, Lup, salt, sazi, asvf)])
+    np.testing.assert_allclose(out, oref, rtol=1e-12, atol=1e-9)
+
+
+@pytest.mark.parametrize('case', range(5))
+def test_kside_matches_umep_golden(rt, case):
+    """Kside_veg_v2022a in the cylinder / box x anisotropic / weighted branches against UMEP's
+    vectors (FP64 and FP32), with and without a prepared shortwave state."""
+    from solweig_amd import model
+    from test_oracle_submodels import kside_case
+    g = _aniso()
+    args, ref = kside_case(g, case)
+    (radI, radD, radG, shadow, svfS, svfW, svfN, svfE, svfEveg, svfSveg, svfWveg, svfNveg, az, alt, psi, t, albedo,
+     F_sh, KupE, KupS, KupW, KupN, cyl, lv, ani, diffsh, asvf, sh, veg, vb) = args
+    rows, cols = shadow.shape
+    print(f'kside case {case}: cyl {cyl} anisotropic {ani}', flush=True)
+    for dtype, rtol, atol in ((np.float64, 1e-12, 1e-9), (np.float32, 2e-5, 2e-3)):
+        common = (radI, radD, radG, shadow, svfS, svfW, svfN, svfE, svfEveg, svfSveg, svfWveg, svfNveg, az, alt, psi,
+                  t, albedo, F_sh, KupE, KupS, KupW, KupN, cyl, lv, ani, diffsh, rows, cols, asvf, sh, veg, vb)
+        out = np.stack([v.to_host() for v in model.Kside_veg_v2022a(*common, dtype=dtype)])
+        np.testing.assert_allclose(out, ref, rtol=rtol, atol=atol, err_msg=str(np.dtype(dtype)))
+        if ani == 1:
+            prep = model.prepare_shortwave(model.prepare_geometry(lv, sh, veg, vb), diffsh, dtype=dtype)
+            again = np.stack([v.to_host() for v in model.Kside_veg_v2022a(*common, dtype=dtype, prepared=prep)])
+            np.testing.assert_array_equal(again, out)

@@ -72,3 +72,55 @@ def test_perez_v3_drop_in_shape_and_normalisation():
     ref = sky.perez(35.0, 160.0, 180.0, 540.0, 172)
     np.testing.assert_allclose(lv[:, 2], ref[2], rtol=1e-12)      # no patch-sun cosine leaves [-1, 1] here
     assert clear > 1 and bright > 0
+
+
+ANISO = os.path.join(os.path.dirname(__file__), 'golden', 'umep_aniso_golden.npz')
+KSIDE_CASES = 5
+LCYL_CASES = 3
+
+
+def kside_case(g, c):
+    """Arguments of umep.kside (UMEP order, masks as float planes) and UMEP's 7 outputs."""
+    p = f'k{c}_'
+    radI, radD, radG, az, alt, psi, t, albedo, cyl, ani = g[p + 'scalars']
+    sh, veg, vb = (m.astype(np.float64) for m in g[p + 'masks'])
+    args = [radI, radD, radG, g[p + 'shadow'], g[p + 'svfS'], g[p + 'svfW'], g[p + 'svfN'], g[p + 'svfE'],
+            g[p + 'svfEveg'], g[p + 'svfSveg'], g[p + 'svfWveg'], g[p + 'svfNveg'], az, alt, psi, t, albedo,
+            g[p + 'F_sh'], g[p + 'KupE'], g[p + 'KupS'], g[p + 'KupW'], g[p + 'KupN'], int(cyl), g[p + 'lv'],
+            int(ani), g[p + 'diffsh'], g[p + 'asvf'], sh, veg, vb]
+    return args, g[p + 'out']
+
+
+def lcyl_case(g, c):
+    p = f'l{c}_'
+    esky, Ta, Tgwall, ewall, salt, sazi = g[p + 'scalars']
+    sh, veg, vb = (m.astype(np.float64) for m in g[p + 'masks'])
+    args = [esky, g[p + 'patches'], Ta, Tgwall, ewall, g[p + 'Lup'], sh, veg, vb, salt, sazi, g[p + 'asvf']]
+    return args, g[p + 'out']
+
+
+@pytest.mark.parametrize('case', range(KSIDE_CASES))
+def test_kside_all_branches_match_umep(case):
+    g = np.load(ANISO)
+    args, ref = kside_case(g, case)
+    out = np.stack([np.broadcast_to(o, ref.shape[1:]) for o in umep.kside(*args)])
+    np.testing.assert_allclose(out, ref, rtol=1e-13, atol=1e-10)
+
+
+def test_kside_cylinder_is_the_general_form():
+    g = np.load(ANISO)
+    args, _ = kside_case(g, 0)
+    (radI, radD, _, shadow, *_svf), rest = args[:12], args[12:]
+    az, alt, _, _, albedo, _, KupE, KupS, KupW, KupN, _, lv, _, diffsh, asvf, sh, veg, vb = rest
+    a = umep.kside(*args)
+    b = umep.kside_cylinder(radI, radD, shadow, az, alt, albedo, KupE, KupS, KupW, KupN, lv, diffsh, asvf, sh, veg, vb)
+    for x, y in zip(a, b):
+        np.testing.assert_array_equal(x, y)
+
+
+@pytest.mark.parametrize('case', range(LCYL_CASES))
+def test_lcyl_random_masks_match_umep(case):
+    g = np.load(ANISO)
+    args, ref = lcyl_case(g, case)
+    out = np.stack(umep.lcyl(*args))
+    np.testing.assert_allclose(out, ref, rtol=1e-13, atol=1e-10)
