@@ -117,6 +117,11 @@ int sw_host_free(void *ptr);
 int sw_memcpy_h2d(void *dst_dev, const void *src, size_t bytes, void *stream);
 int sw_memcpy_d2h(void *dst, const void *src_dev, size_t bytes, void *stream);
 int sw_memcpy_d2d(void *dst_dev, const void *src_dev, size_t bytes, void *stream);
+/* Output path (replaces the side copy stream and pinned transfers of runtime.py:79-163): an
+ * asynchronous device-to-host copy into page-locked memory from sw_host_alloc, and a stream
+ * wait on an event recorded on another stream.                                            */
+int sw_memcpy_d2h_async(void *dst_pinned, const void *src_dev, size_t bytes, void *stream);
+int sw_stream_wait_event(void *stream, void *event);
 int sw_memset(void *dst_dev, int value, size_t bytes, void *stream);
 int sw_stream_create(void **stream);
 int sw_stream_destroy(void *stream);
@@ -300,6 +305,20 @@ int sw_longwave(const uint32_t *flags_dev, int patches, long long n, const doubl
 int sw_kside(const uint32_t *flags_dev, const void *diffsh_dev, int patches, long long n,
              const double *table_dev, const void *const *fields, const long long *counts, int n_fields,
              const double *params, int n_params, int real_bytes, void *out_dev, void *stream);
+
+/* Wall heights and wall aspect from a DSM (replaces walls.calculate_dsm_walls, walls.py:24-142,
+ * and its wall_heights / wall_directions kernels, geometry.cu:15-63). dsm_dev: FP64
+ * [rows][cols]; threshold: minimum wall height; step: np.gradient spacing (1 / (1 / pixel
+ * size)). Templates (host-built, walls.py:33-56): n_angles (180, or 0 when the raster is not
+ * larger than 2 radius + 2 on both sides) rotated line templates, line_dev holding (row, col)
+ * offset pairs of angle k at [line_start[k], line_start[k+1]); side_dev the two-sided
+ * templates, angle k's halves at [side_start[3k], side_start[3k+1]) and
+ * [side_start[3k+1], side_start[3k+2]). Offsets stay within +-radius. occupancy_dev: rows*cols
+ * bytes of scratch. Outputs FP64 walls_dev (m) and aspect_dev (degrees clockwise from north).  */
+int sw_dsm_walls(const double *dsm_dev, int rows, int cols, double threshold, double step, int radius,
+                 int n_angles, const int32_t *line_dev, const int32_t *line_start_dev, const int32_t *side_dev,
+                 const int32_t *side_start_dev, uint8_t *occupancy_dev, double *walls_dev, double *aspect_dev,
+                 void *stream);
 
 /* UTCI of a Tmrt plane for scalar weather (replaces utci.utci_calculator's scalar-weather
  * path, utci.py:22-33, utci.cu:243-523): ``coefficients`` are 7 host FP64 values c_k with

@@ -2105,6 +2105,78 @@ int launch_kside(const uint32_t *flags, const void *diffsh, int patches, long lo
     return hip_check(hipGetLastError(), "k_kside launch");
 }
 
+// =====================================================================================
+// Wall heights and aspect from a DSM (calculate_dsm_walls, walls.py:24-142; geometry.cu:15-63).
+// Heights: the largest edge neighbour minus the pixel where that is >= threshold; on the first
+// row the reference's column test (geometry.cu:30-33) keeps -dsm for columns rows..cols-2.
+// Aspect: a wall pixel away from the border takes the first of 180 rotated line templates that
+// covers the most wall pixels, and faces the lower of the two halves of the rotated two-sided
+// template; pixels left at 0 take the DSM gradient direction (np.gradient, walls.py:125-131).
+// Templates are host tables of (row, col) offsets.
+// =====================================================================================
+__global__ void k_wall_heights(const double *dsm, int rows, int cols, double threshold, double *walls,
+                               uint8_t *occupancy) {
+    const long long n = (long long)rows * cols;
+    const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const int r = (int)(p / cols), c = (int)(p % cols);
+    double h = 0.0;
+    if (r > 0 && r < rows - 1 && c > 0 && c < cols - 1) {
+        const double nb = fmax(fmax(dsm[p - cols], dsm[p + cols]), fmax(dsm[p - 1], dsm[p + 1]));
+        const double d = nb - dsm[p];
+        if (d >= threshold) h = d;
+    } else if (r == 0 && rows > 1 && c >= rows && c < cols - 1) {
+        const double d = -dsm[p];
+        if (d >= threshold) h = d;
+    }
+    walls[p] = h;
+    occupancy[p] = h > 0.0 ? 1 : 0;
+}
+
+__global__ void k_wall_aspect(const double *dsm, const double *walls, const uint8_t *occupancy, int rows,
+                              int cols, int radius, int n_angles, const int32_t *line, const int32_t *line_start,
+                              const int32_t *side, const int32_t *side_start, double step, double *aspect) {
+#pragma clang fp contract(off)
+    const long long n = (long long)rows * cols;
+    const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    if (!(walls[p] > 0.0)) { aspect[p] = 0.0; return; }
+    const int r = (int)(p / cols), c = (int)(p % cols);
+    double a = 0.0;
+    if (n_angles > 0 && r >= radius && r < rows - radius - 2 && c >= radius && c < cols - radius - 2) {
+        int best = 0, angle = -1;
+        for (int k = 0; k < n_angles; ++k) {
+            int score = 0;
+            for (int j = line_start[k]; j < line_start[k + 1]; ++j)
+                score += occupancy[p + (long long)line[2 * j] * cols + line[2 * j + 1]];
+            if (score > best) { best = score; angle = k; }
+        }
+        if (angle >= 0) {
+            const int32_t *ss = side_start + 3 * angle;
+            double first = 0.0, second = 0.0;
+            for (int j = ss[0]; j < ss[1]; ++j) first += dsm[p + (long long)side[2 * j] * cols + side[2 * j + 1]];
+            for (int j = ss[1]; j < ss[2]; ++j) second += dsm[p + (long long)side[2 * j] * cols + side[2 * j + 1]];
+            a = 270.0 - angle;
+            if (first > second) a -= 180.0;
+            if (a < 0.0) a += 360.0;
+        }
+    }
+    if (a == 0.0) {
+        // np.gradient with unit-free spacing ``step``: central inside, one-sided on the border
+        double fy, fx;
+        if (r == 0) fy = (dsm[p + cols] - dsm[p]) / step;
+        else if (r == rows - 1) fy = (dsm[p] - dsm[p - cols]) / step;
+        else fy = (dsm[p + cols] - dsm[p - cols]) / (2.0 * step);
+        if (c == 0) fx = (dsm[p + 1] - dsm[p]) / step;
+        else if (c == cols - 1) fx = (dsm[p] - dsm[p - 1]) / step;
+        else fx = (dsm[p + 1] - dsm[p - 1]) / (2.0 * step);
+        double g = -atan2(fx, fy);
+        if (g < 0.0) g += 2.0 * 3.141592653589793;
+        a = a + g / (3.141592653589793 / 180.0);
+    }
+    aspect[p] = a;
+}
+
 int launch_adjoint_hour(const sw_domain *dom, const int32_t *bs, int total, int hour, bool top,
                         const sw_adjoint *adj, const int32_t *gofs, const double *gb, int first,
                         int second, double inv_n, hipStream_t s) {
@@ -2186,6 +2258,17 @@ int sw_memcpy_d2h(void *dst, const void *src, size_t bytes, void *stream) {
                        "hipMemcpyAsync D2H");
     if (rc) return rc;
     return hip_check(hipStreamSynchronize((hipStream_t)stream), "hipStreamSynchronize");
+}
+
+int sw_memcpy_d2h_async(void *dst, const void *src, size_t bytes, void *stream) {
+    if (!bytes) return SW_OK;
+    return hip_check(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, (hipStream_t)stream),
+                     "hipMemcpyAsync D2H");
+}
+
+int sw_stream_wait_event(void *stream, void *event) {
+    if (!event) return fail(SW_ERR_ARGUMENT, "sw_stream_wait_event: NULL event");
+    return hip_check(hipStreamWaitEvent((hipStream_t)stream, (hipEvent_t)event, 0), "hipStreamWaitEvent");
 }
 
 int sw_memcpy_d2d(void *dst, const void *src, size_t bytes, void *stream) {
@@ -2425,6 +2508,23 @@ int sw_kside(const uint32_t *flags, const void *diffsh, int patches, long long n
     const hipStream_t s = (hipStream_t)stream;
     return real_bytes == 8 ? launch_kside<double>(flags, diffsh, patches, n, table, F, q, out, s)
                            : launch_kside<float>(flags, diffsh, patches, n, table, F, q, out, s);
+}
+
+int sw_dsm_walls(const double *dsm, int rows, int cols, double threshold, double step, int radius, int n_angles,
+                 const int32_t *line, const int32_t *line_start, const int32_t *side, const int32_t *side_start,
+                 uint8_t *occupancy, double *walls, double *aspect, void *stream) {
+    if (!dsm || !occupancy || !walls || !aspect) return fail(SW_ERR_ARGUMENT, "sw_dsm_walls: NULL pointer");
+    if (rows < 2 || cols < 2 || !(step > 0.0)) return fail(SW_ERR_ARGUMENT, "sw_dsm_walls: raster below 2 x 2 or bad step");
+    if (n_angles != 0 && (n_angles > 180 || !line || !line_start || !side || !side_start || radius < 0 ||
+                          rows <= 2 * radius + 2 || cols <= 2 * radius + 2))
+        return fail(SW_ERR_ARGUMENT, "sw_dsm_walls: bad template tables for this raster");
+    const long long n = (long long)rows * cols;
+    const dim3 grid((unsigned)((n + 255) / 256)), block(256);
+    const hipStream_t s = (hipStream_t)stream;
+    hipLaunchKernelGGL(k_wall_heights, grid, block, 0, s, dsm, rows, cols, threshold, walls, occupancy);
+    hipLaunchKernelGGL(k_wall_aspect, grid, block, 0, s, dsm, walls, occupancy, rows, cols, radius, n_angles, line,
+                       line_start, side, side_start, step, aspect);
+    return hip_check(hipGetLastError(), "wall kernels launch");
 }
 
 int sw_utci_scalar(const float *tmrt, long long n, const double *coefficients, double ta32,

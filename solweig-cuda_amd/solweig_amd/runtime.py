@@ -95,6 +95,7 @@ def lib():
         'sw_event_sync': ([vp], i32),
         'sw_memcpy_h2d': ([vp, vp, sz, vp], i32), 'sw_memcpy_d2h': ([vp, vp, sz, vp], i32),
         'sw_memcpy_d2d': ([vp, vp, sz, vp], i32), 'sw_memset': ([vp, i32, sz, vp], i32),
+        'sw_memcpy_d2h_async': ([vp, vp, sz, vp], i32), 'sw_stream_wait_event': ([vp, vp], i32),
         'sw_stream_create': ([ctypes.POINTER(vp)], i32), 'sw_stream_destroy': ([vp], i32),
         'sw_stream_sync': ([vp], i32), 'sw_event_create': ([ctypes.POINTER(vp)], i32),
         'sw_event_destroy': ([vp], i32), 'sw_event_record': ([vp, vp], i32),
@@ -130,7 +131,7 @@ EXPORTED = ('sw_version', 'sw_last_error', 'sw_device_count', 'sw_set_device', '
             'sw_copy', 'sw_graph_begin', 'sw_graph_end', 'sw_graph_launch', 'sw_graph_destroy',
             'sw_adjoint_hour', 'sw_day_adjoint', 'sw_cylindric_wedge', 'sw_wave_delay', 'sw_kup', 'sw_lside',
             'sw_surface_temperature', 'sw_assemble_tmrt', 'sw_gvf', 'sw_pack_visibility', 'sw_longwave',
-            'sw_kside')
+            'sw_kside', 'sw_dsm_walls', 'sw_memcpy_d2h_async', 'sw_stream_wait_event')
 
 
 def check(rc):
@@ -267,6 +268,10 @@ class Stream:
     def sync(self):
         call('sw_stream_sync', self.handle)
 
+    def wait(self, event):
+        """Later work on this stream waits for ``event`` (recorded on any stream)."""
+        call('sw_stream_wait_event', self.handle, event.handle)
+
     def __del__(self):
         try:
             if self._h.value:
@@ -279,6 +284,10 @@ class Event:
     def __init__(self):
         self._h = ctypes.c_void_p()
         call('sw_event_create', ctypes.byref(self._h))
+
+    @property
+    def handle(self):
+        return self._h.value
 
     def record(self, stream=None):
         call('sw_event_record', self._h, _s(stream))

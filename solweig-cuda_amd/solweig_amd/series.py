@@ -2,10 +2,13 @@
 and run.load_case, run.py:70-86).
 
 Rasters are read from .npy files or passed as arrays (GeoTIFF needs GDAL, which this build does
-not use); walls and aspect come from the DSM (synthetic.wall_heights / wall_aspect). The day loop
-runs on the batched engine: one upload of the day's tables and 24 x 2 launches per day, the lag
-state reset at each day start, UTCI NaN outside the ground mask (series.py:466-469). With
-``plain=True`` every hour goes through the per-step API (model.Solweig_2022a_calc) instead.
+not use); walls and aspect are computed from the DSM on the device (walls.calculate_dsm_walls,
+series.py:212-236) and rounded to float32 as the reference stores them. The day loop runs on the
+batched engine: one upload of the day's tables and 24 x 2 launches per day, the lag state reset
+at each day start, UTCI NaN outside the ground mask (series.py:466-469). Outputs leave through
+output.DayOutputs: staged on the device, copied to page-locked memory on a side stream and written
+by a worker thread while the next days run. With ``plain=True`` every hour goes through the
+per-step API (model.Solweig_2022a_calc) instead.
 """
 from __future__ import annotations
 
@@ -18,6 +21,8 @@ from pathlib import Path
 import numpy as np
 
 from . import batched, synthetic
+from . import walls as dsm_walls
+from .output import DayOutputs
 from .sun import met_sun
 
 
@@ -87,8 +92,9 @@ def run_timeseries(building_dsm_path, tree_path, dem_path, met_file, output_path
     dsm = np.asarray(_raster(building_dsm_path), dtype=np.float32)
     trees = np.asarray(_raster(tree_path), dtype=np.float32)
     dem = np.asarray(_raster(dem_path), dtype=np.float32)
-    walls = synthetic.wall_heights(dsm, dsm_wall_threshold_m).astype(np.float32)
-    aspect = synthetic.wall_aspect(dsm, walls, pixel_size_m).astype(np.float32)
+    walls, aspect = dsm_walls.calculate_dsm_walls(dsm.astype(np.float64), float(pixel_size_m),
+                                                  float(dsm_wall_threshold_m))
+    walls, aspect = walls.astype(np.float32), aspect.astype(np.float32)
     utc = float(longitude) / 15.0 if utc_offset_hours is None else float(utc_offset_hours)
     lc = _raster(landcover_path)
     dom = synthetic.Domain(dsm, trees, dem, walls, aspect, forcing.values, float(latitude), float(longitude),
@@ -98,35 +104,53 @@ def run_timeseries(building_dsm_path, tree_path, dem_path, met_file, output_path
     variables = ['UTCI'] + (['TMRT'] if save_tmrt else [])
     crop = (slice(None), slice(None)) if output_core is None else \
         (slice(output_core[0], output_core[0] + output_core[2]), slice(output_core[1], output_core[1] + output_core[3]))
-    engine = None
-    for day in forcing.days:
-        n = day.values.shape[0]
-        if progress:
-            print(f'[{datetime.now():%H:%M:%S}] {day.date}: {n} steps', flush=True)
-        sun = met_sun(day.values, dom.location, utc)
-        if plain:
-            tmrt, utci = _plain_day(dom, day.values, sun)
-        else:
-            if engine is None or engine.n_hours != n:
-                engine = batched.Engine([dom], n_hours=n)
-            engine.load(engine.tables(met=[day.values], sun=[sun]))
-            engine.reset_state()
-            engine.forward_day()
-            tmrt, utci = engine.outputs(0)
-        if output_valid_mask is not None:
-            utci = np.where(np.asarray(output_valid_mask)[None], utci, np.nan)
+    valid = None if output_valid_mask is None else np.asarray(output_valid_mask, dtype=bool)
+
+    def write(date, tmrt, utci):
+        if valid is not None:
+            utci = np.where(valid[None], utci, np.nan)
+        folder = out / date
+        folder.mkdir(parents=True, exist_ok=True)
         day_files = {v: [] for v in variables}
-        if not no_write:
-            folder = out / day.date
-            folder.mkdir(parents=True, exist_ok=True)
-            for name, stack in (('UTCI', utci), ('TMRT', tmrt)):
-                if name in variables:
-                    p = folder / f'{name}.npy'
-                    np.save(p, stack[(slice(None),) + crop])
-                    day_files[name].append(str(p))
-        else:
-            day_files = {'UTCI': utci, 'TMRT': tmrt}
-        files['days'][day.date] = day_files
+        for name, stack in (('UTCI', utci), ('TMRT', tmrt)):
+            if name in variables:
+                p = folder / f'{name}.npy'
+                np.save(p, stack[(slice(None),) + crop])
+                day_files[name].append(str(p))
+        files['days'][date] = day_files
+
+    engine = writer = None
+    try:
+        for day in forcing.days:
+            n = day.values.shape[0]
+            if progress:
+                print(f'[{datetime.now():%H:%M:%S}] {day.date}: {n} steps', flush=True)
+            sun = met_sun(day.values, dom.location, utc)
+            if plain:
+                tmrt, utci = _plain_day(dom, day.values, sun)
+            else:
+                if engine is None or engine.n_hours != n:
+                    if writer is not None:
+                        writer.close()
+                    engine = batched.Engine([dom], n_hours=n)
+                    writer = None if no_write else DayOutputs(engine, slots=max(1, int(output_queue_size)))
+                engine.load(engine.tables(met=[day.values], sun=[sun]))
+                engine.reset_state()
+                engine.forward_day()
+                if writer is not None:
+                    writer.submit(day.date, write)
+                    continue
+                tmrt, utci = engine.outputs(0)
+            if no_write:
+                if valid is not None:
+                    utci = np.where(valid[None], utci, np.nan)
+                files['days'][day.date] = {'UTCI': utci, 'TMRT': tmrt}
+            else:
+                write(day.date, tmrt, utci)
+    finally:
+        if writer is not None:
+            writer.close()
+    files['days'] = dict(sorted(files['days'].items()))
     return files
 
 
