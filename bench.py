@@ -26,7 +26,7 @@ B_ALG = 157.5          # algorithmic bytes per pixel-hour of the forward (static
 HBM_PEAK = 8.0e12      # MI355X HBM3E bytes/s
 # HBM bytes of one hour (shadow + pixel launch, mean over a day) of the 2000x2000 workload, from
 # rocprofv3 FETCH_SIZE (x 1.9999 copy calibration) + WRITE_SIZE; profiles/k_hour_pair_c2_2000x2000.md
-TRAFFIC_PER_HOUR_2000 = 1.054e9
+TRAFFIC_PER_HOUR_2000 = 0.999e9
 
 
 def log(msg):
@@ -58,6 +58,36 @@ def cpu_baseline(hours=(10, 11, 12, 13), size=200):
     return dict(value=px_h / dt, unit='pixel-hours/s', cores=1, kind='port',
                 sample=f'oracle FP64 NumPy UMEP step on c0 {size}x{size}, hours {hrs[0]}-{hrs[-1]} '
                        f'({px_h} px-h in {dt:.1f} s; static SVF prep {t_static:.1f} s excluded)')
+
+
+def static_leg(dom):
+    """Static preparation on the device, timed with events: wall heights and aspect from the DSM
+    (the reference's calculate_dsm_walls; the domain's walls and aspect are replaced by these, as
+    run_timeseries does), the sky-view march and reduction, the static sky sums."""
+    from solweig_amd import runtime as rt, static as pstatic, svf as psvf, walls as pwalls
+    stream = rt.Stream()
+    ev = [rt.Event() for _ in range(4)]
+    dsm64 = rt.DeviceArray.from_host(dom.dsm.astype(np.float64))
+    pwalls.dsm_walls_device(dsm64, 1.0, 3.0, stream=stream)          # warm-up (tables, code load)
+    ev[0].record(stream)
+    w, a = pwalls.dsm_walls_device(dsm64, 1.0, 3.0, stream=stream)
+    ev[1].record(stream)
+    stream.sync()
+    out = dict(walls_ms=ev[0].elapsed_ms(ev[1]))
+    dom.walls, dom.aspect = w.to_host().astype(np.float32), a.to_host().astype(np.float32)
+    out['wall_pixels'] = int((dom.walls > 0).sum())
+    prep = pstatic.PreparedDomain(dom, stream=stream)                 # includes one sky view
+    d = prep.dev
+    sv = psvf.sky_view(d['dsm'], d['trees'], prep.dsm_raise, prep.shape, prep.amaxvalue, dom.scale, prep.trans,
+                       stream, events=ev[:3])
+    out['svf_visibility_ms'], out['svf_reduce_ms'] = ev[0].elapsed_ms(ev[1]), ev[1].elapsed_ms(ev[2])
+    ev[0].record(stream)
+    pstatic.sky_sums(sv.bits, prep.P, stream)
+    ev[1].record(stream)
+    stream.sync()
+    out['sky_sums_ms'] = ev[0].elapsed_ms(ev[1])
+    del prep, sv
+    return out
 
 
 def adjoint_leg(size=1000, repeats=3):
@@ -108,6 +138,8 @@ def main():
                     help='run a 256 MiB streaming copy (sw_copy) 3 times for HBM counter scaling')
     ap.add_argument('--no-adjoint', action='store_true', help='skip the forward + adjoint leg')
     ap.add_argument('--no-tune', action='store_true', help='keep the default 32x8 tile (no autotuning)')
+    ap.add_argument('--tile', type=int, default=0,
+                    help='fixed tile width (16, 32, 64, 128) instead of the autotuner (for profiling runs)')
     ap.add_argument('--plain-days', type=int, default=1,
                     help='also time the plain per-step forward for this many days (0 = skip)')
     args = ap.parse_args()
@@ -122,14 +154,22 @@ def main():
     dom = synthetic.make_c1(seed=2, size=args.size, days=days, start=__import__('datetime').date(2025, 6, 6),
                             name='c2')
     log(f'[bench] generated in {time.perf_counter() - t0:.1f} s; static preparation')
+    static = static_leg(dom)
     t0 = time.perf_counter()
     eng = batched.Engine([dom], n_hours=24)
     rt.synchronize()
     t_static = time.perf_counter() - t0
-    log(f'[bench] static prep {t_static:.2f} s')
+    static['engine_setup_s'] = t_static
+    log(f'[bench] static prep {t_static:.2f} s (host derivations, uploads, sky view, sky sums); '
+        f'device: walls {static["walls_ms"]:.2f} ms, SVF march {static["svf_visibility_ms"]:.1f} ms, '
+        f'SVF reduce {static["svf_reduce_ms"]:.2f} ms, sky sums {static["sky_sums_ms"]:.2f} ms')
     P = eng.domains[0].P
     tuned = None
-    if not args.no_tune:
+    if args.tile:
+        eng.set_tiles(args.tile)
+        tuned = dict(tile_cols=args.tile, fixed=True)
+        log(f'[bench] fixed tile width {args.tile}')
+    elif not args.no_tune:
         from solweig_amd import autotune
         tuned = autotune.tune(eng, eng.tables(day_index=0), use_cache=False)
         log(f'[bench] autotuned tile width {tuned["tile_cols"]} (ms over hours {tuned.get("hours")}: {tuned["ms"]})')
@@ -239,7 +279,7 @@ def main():
                config=dict(workload=f'c2 batched forward {args.size}x{args.size}, 24 h per step '
                                     f'(30-day forcing cycled), host day tables inside the timed region '
                                     f'(built for day k+1 while the device runs day k)',
-                           graph=bool(args.graph), static_prep_s=t_static, tile=tuned, plain_forward=plain,
+                           graph=bool(args.graph), static_prep=static, tile=tuned, plain_forward=plain,
                            forward_adjoint=adj),
                roofline=roof, cpu_baseline=cpu)
     print(json.dumps(out), flush=True)

@@ -100,8 +100,10 @@ def unpack_visibility(bits, shape, patches=153):
     return tuple(out)
 
 
-def sky_view(dsm_dev, trees_dev, dsm_raise, shape, amaxvalue, scale, transmissivity=0.03, stream=None):
-    """Run the static visibility march and band reduction for one domain."""
+def sky_view(dsm_dev, trees_dev, dsm_raise, shape, amaxvalue, scale, transmissivity=0.03, stream=None,
+             events=None):
+    """Run the static visibility march and band reduction for one domain. ``events``: three
+    runtime.Event recorded before the march, between the passes and after the reduction."""
     rows, cols = shape
     t = ray_tables(rows, cols, float(amaxvalue), float(scale))
     P = rows * cols
@@ -114,10 +116,16 @@ def sky_view(dsm_dev, trees_dev, dsm_raise, shape, amaxvalue, scale, transmissiv
     bits = rt.DeviceArray((3, rt.SW_VIS_WORDS, P), np.uint32)
     planes = rt.DeviceArray((16, P), np.float64)
     s = rt._s(stream)
+    if events:
+        events[0].record(stream)
     rt.call('sw_svf_visibility', dsm_dev.ptr, trees_dev.ptr, float(dsm_raise), rows, cols, off.ptr, hts.ptr,
             lens.ptr, int(t['capacity']), int(t['patches']), bits.ptr, s)
+    if events:
+        events[1].record(stream)
     rt.call('sw_svf_reduce', bits.ptr, dsm_dev.ptr, trees_dev.ptr, float(dsm_raise), rows, cols, sec.ptr,
             bo.ptr, w.ptr, int(t['bands']), int(t['patches']), float(transmissivity), planes.ptr, s)
+    if events:
+        events[2].record(stream)
     rt.call('sw_stream_sync', s)
     return SkyView(bits, planes, shape)
 
