@@ -65,7 +65,7 @@ enum {
     SWF_KSIDED, SWF_DRAD, SWF_KSIDE, SWF_SHADOW, SWF_WALLSUN, SWF_FSH, SW_FULL_PLANES
 };
 
-/* Device-side description of one domain (280 bytes). All fields are 8 bytes wide; pointers are device
+/* Device-side description of one domain (296 bytes). All fields are 8 bytes wide; pointers are device
  * pointers. Optional outputs may be NULL. */
 typedef struct sw_domain {
     int64_t rows, cols, pixels, landcover;
@@ -77,7 +77,9 @@ typedef struct sw_domain {
     const double *tan_asvf, *tan_svfalfa;                    /* [P]                           */
     const uint8_t *ground;                                   /* [P] 1 where dsm-dem < 2        */
     const uint32_t *vis;                                     /* [3][5][P] sh, veg, vb bits     */
-    void *rec;                                               /* [P] 16-byte hour records        */
+    void *rec;            /* [P] 8-byte hour records written by the shadow pass: the Lup field of
+                             sunonsurface_2018a, then albedo x shadow whose sign bit marks a
+                             sunlit wall (gvf.cu fields 3-5)                                  */
     float *lag;                                              /* [6][P] Tgmap1, E, S, W, N, TgOut1 */
     float *tmrt, *utci;                                      /* [n_hours][P] (may be NULL)     */
     const double *hours;                                     /* [n_hours][SW_HOUR_STRIDE]     */
@@ -85,13 +87,19 @@ typedef struct sw_domain {
     const double *ptab_d;                                    /* [rows][153][SW_PATCH_D]       */
     const int32_t *ray_off;                                  /* [sum L][2] (row, col) shifts  */
     const double *ray_dz;                                    /* [sum L] ray heights            */
-    uint8_t *masks;                                          /* [P] bits of the last shadow pass */
+    uint8_t *masks;       /* [P] bits of the last shadow pass (read by the pixel pass): 1 not
+                             building-shadowed, 2 no canopy shadow, 4 vbshvegsh, 8 sunlit wall,
+                             16 facesh                                                         */
     double *shadow_planes;                                   /* [8][P] cast_shadow planes      */
     float *full;                                             /* [SW_FULL_PLANES][P]           */
     const float *skysums;                                    /* [SW_SKY_SUMS][P] static sums   */
     const float *hmax;    /* [ceil(rows/16)][hmax_cols] upper bound of max(dsm, canopy top) per
                              16 x 16 cell (FP32 rounded up), lets the sun-ray march skip steps */
     int64_t hmax_cols;    /* ceil(cols/16)                                                    */
+    const uint8_t *gvf_len;  /* [18][P] ground-view steps before the first non-ground sample
+                                (static, sw_gvf_static)                                        */
+    const float *gvf_nosh;   /* [10][P] static albedo sums of the ground view: a (all, E, S, W,
+                                N) then b; gvfalbnosh sum = a + albedo_b * b                    */
 } sw_domain;
 
 /* Static per-pixel sums over the patches of each visibility class (day patch geometry), the
@@ -258,6 +266,15 @@ int sw_surface_temperature(const void *const *fields, const long long *counts, i
                            const double *params, int n_params, int real_bytes, void *out_dev, void *stream);
 int sw_assemble_tmrt(const void *const *fields, const long long *counts, int n_fields, long long n,
                      const double *params, int n_params, int real_bytes, void *out_dev, void *stream);
+
+/* Static part of the ground view of a domain (the walk of gvf_2018a depends on the hour only
+ * through Lup, albedo x shadow and the sunlit walls): per pixel and direction the number of
+ * steps before the first non-ground sample (len_dev [18][P] bytes), and the unshadowed-albedo
+ * sums of gvfalbnosh split as a + albedo_b * b over all 18 directions and the E, S, W, N halves
+ * (nosh_dev [10][P]). ground_dev: [P] 0/1; alb_dev: [P]; offsets as sw_hour_pixels takes them. */
+int sw_gvf_static(const uint8_t *ground_dev, const float *alb_dev, int rows, int cols,
+                  const int32_t *gvf_offsets_dev, int first_steps, int second_steps, uint8_t *len_dev,
+                  float *nosh_dev, void *stream);
 
 /* Ground view factors of one step for arbitrary caller fields (replaces model.gvf,
  * model.py:569-622, and its gvf_pack / gvf_compact / gvf kernels, gvf.cu:79-292; UMEP

@@ -22,7 +22,7 @@
 
 #include "../../../include/solweig_hip.h"
 
-static_assert(sizeof(sw_domain) == 280, "sw_domain must stay 35 eight-byte fields");
+static_assert(sizeof(sw_domain) == 296, "sw_domain must stay 37 eight-byte fields");
 
 namespace {
 
@@ -66,6 +66,7 @@ struct PixelAt {
     long long p;
     bool valid;
     int wr0, wr1, wc0, wc1;     // rows and columns spanned by this pixel's wave (uniform)
+    int tr0, tc0, tx, ty;       // the workgroup's tile: first row and column, width, height
 };
 
 // Address spaces: per-pixel planes are read and written through global (1) pointers; every
@@ -84,7 +85,7 @@ struct DomainG {
     const AS1 double *tan_asvf, *tan_svfalfa;
     const AS1 uint8_t *ground;
     const AS1 uint32_t *vis;
-    AS1 v4f *rec;
+    AS1 v2f *rec;
     AS1 float *lag, *tmrt, *utci;
     const AS4 double *hours;
     const AS4 float *ptab_f;
@@ -97,6 +98,8 @@ struct DomainG {
     const AS1 float *skysums;
     const AS4 float *hmax;
     int hmax_cols;
+    const AS1 uint8_t *gvf_len;
+    const AS1 float *gvf_nosh;
 };
 
 __device__ __forceinline__ DomainG domain_at(const sw_domain *D) {
@@ -113,7 +116,7 @@ __device__ __forceinline__ DomainG domain_at(const sw_domain *D) {
     g.tmaxlst = (const AS1 float *)d->tmaxlst; g.svfbuveg = (const AS1 float *)d->svfbuveg;
     g.tan_asvf = (const AS1 double *)d->tan_asvf; g.tan_svfalfa = (const AS1 double *)d->tan_svfalfa;
     g.ground = (const AS1 uint8_t *)d->ground; g.vis = (const AS1 uint32_t *)d->vis;
-    g.rec = (AS1 v4f *)d->rec; g.lag = (AS1 float *)d->lag;
+    g.rec = (AS1 v2f *)d->rec; g.lag = (AS1 float *)d->lag;
     g.tmrt = (AS1 float *)d->tmrt; g.utci = (AS1 float *)d->utci;
     g.hours = (const AS4 double *)d->hours;
     g.ptab_f = (const AS4 float *)d->ptab_f; g.ptab_d = (const AS4 double *)d->ptab_d;
@@ -121,6 +124,7 @@ __device__ __forceinline__ DomainG domain_at(const sw_domain *D) {
     g.masks = (AS1 uint8_t *)d->masks; g.shadow_planes = (AS1 double *)d->shadow_planes;
     g.full = (AS1 float *)d->full; g.skysums = (const AS1 float *)d->skysums;
     g.hmax = (const AS4 float *)d->hmax; g.hmax_cols = (int)d->hmax_cols;
+    g.gvf_len = (const AS1 uint8_t *)d->gvf_len; g.gvf_nosh = (const AS1 float *)d->gvf_nosh;
     return g;
 }
 
@@ -161,6 +165,7 @@ __device__ __forceinline__ PixelAt locate(const sw_domain *doms, const int32_t *
     at.wr1 = tr0 + ((w64 + 63) >> sh);
     at.wc0 = tx > 64 ? tc0 + (w64 & (tx - 1)) : tc0;
     at.wc1 = tx > 64 ? at.wc0 + 63 : tc0 + tx - 1;
+    at.tr0 = tr0; at.tc0 = tc0; at.tx = tx; at.ty = ty;
     at.valid = at.r < rows && at.c < cols;
     at.p = (long long)at.r * cols + at.c;
     return at;
@@ -305,6 +310,12 @@ __global__ __launch_bounds__(256) void k_svf_reduce(
 // =====================================================================================
 __device__ __forceinline__ float shadow_of_class(uint32_t cls, float shv_canopy) {
     return cls == 2u ? 1.0f : (cls == 1u ? shv_canopy : 0.0f);
+}
+
+// Shadow class from the shadow pass's mask byte (bit 0: not building-shadowed, bit 1: no
+// canopy shadow): 0 building shadow, 1 canopy shadow, 2 sunlit.
+__device__ __forceinline__ uint32_t class_of(uint8_t m) {
+    return (m & 1u) ? ((m & 2u) ? 2u : 1u) : 0u;
 }
 
 struct March {
@@ -488,12 +499,10 @@ __global__ __launch_bounds__(BLOCK) void k_hour_shadow(const sw_domain *__restri
     const double tk = tg * shadow + ta + 273.15, ta4 = (ta + 273.15);
     const double lup = SBC * emis * (tk * tk) * (tk * tk) - SBC * emis * (ta4 * ta4) * (ta4 * ta4);
     const uint32_t cls = blocked ? 0u : (veg ? 1u : 2u);
-    const float alb = G.alb[p];
-    v4f rec;
+    const float albsh = G.alb[p] * shadow_of_class(cls, (float)(1.0 - (1.0 - H[SWH_PSI])));
+    v2f rec;
     rec.x = (float)lup;
-    rec.y = alb;
-    rec.z = alb * shadow_of_class(cls, (float)(1.0 - (1.0 - H[SWH_PSI])));
-    rec.w = __uint_as_float((ground ? 1u : 0u) | (sunwall ? 0x100u : 0u) | (cls << 16));
+    rec.y = __uint_as_float(__float_as_uint(albsh) | (sunwall ? 0x80000000u : 0u));
     G.rec[p] = rec;
     if (G.masks)
         G.masks[p] = (uint8_t)((blocked ? 0 : 1) | (veg ? 0 : 2) | ((ever && !veg) ? 0 : 4) |
@@ -565,29 +574,57 @@ struct GvfOut {
     Triple all, e, s, w, n;
 };
 
-// Running state of one direction of the walk. ``af`` is 1 while the running minimum of the
-// ground mask is 1 (umep_solweig.py:1498) and 0 from the first non-ground sample on; every
+// Running state of one direction of the adjoint's walk. ``af`` is 1 while the running minimum of
+// the ground mask is 1 (umep_solweig.py:1498) and 0 from the first non-ground sample on; every
 // term is multiplied by it, so the walk needs no per-lane branches. ``L`` counts the ground
 // steps, ``wf`` is 1 once a sunlit wall was passed, ``cw`` counts the ground steps from there.
 struct Walk {
     float af, lup, alb, albsh, L, wf, cw;
 };
 
-// One pixel's walk over the 18 directions x ``second`` steps (sunonsurface_2018a). After the
-// walk, the steps from the first non-ground sample to the end are added in closed form: each of
-// them adds one "any wall" count and, if a sunlit wall was passed, one sunlit-wall count
-// (gvf.cu:209-239 sums the same terms step by step). The snapshot after ``first`` steps gives
-// the near-field terms. On the domain edge a step whose shift leaves the raster repeats the
-// previous sample, as the shifted-window assignment of the reference leaves it.
-template <bool CHECK>
-__device__ __forceinline__ GvfOut ground_view(const DomainG &G, const PixelAt &at, const v4f me,
-                                              const AS4 double *H, const AS4 int32_t *gofs,
-                                              const AS4 double *gbounds, int first, int second) {
+// One pixel's walk over the 18 directions x ``second`` steps (sunonsurface_2018a). The number
+// of steps before the first non-ground sample, L, depends only on the ground mask and is a static
+// plane (gvf_len); the walk sums the hour's Lup and albedo x shadow over those L steps and keeps
+// a bit per step where the sample is a sunlit wall (the record's sign bit). Past step L every
+// step adds one "any wall" count and, once a sunlit wall was passed, one sunlit-wall count
+// (gvf.cu:209-239 sums the same terms step by step): with k the first sunlit-wall step, the far
+// field counts second - k sunlit-wall steps and second - L wall steps, the near field (the first
+// ``first`` steps) first - k and first - min(L, first). The unshadowed-albedo sums do not depend
+// on the hour and come from the static planes (gvf_nosh). On the domain edge a step whose shift
+// leaves the raster repeats the previous sample, as the shifted-window assignment of the
+// reference leaves it.
+// The hour records of a workgroup's tile and its ground-view halo, staged in LDS (rows of
+// ``pitch`` records) when the whole halo lies inside the raster.
+extern __shared__ v2f gv_lds[];
+
+// Copy the tile's records and a halo of h pixels on every side into LDS (all threads of the
+// workgroup take part; the caller has checked that the region lies inside the raster).
+__device__ __forceinline__ void stage_tile(const DomainG &G, const PixelAt &at, int h, int pitch) {
+    const int cols = (int)G.cols, nrows = at.ty + 2 * h;
+    const int wave = (int)threadIdx.x >> 6, lane = (int)threadIdx.x & 63;
+    const AS1 v2f *rec = G.rec;
+    for (int rr = wave; rr < nrows; rr += BLOCK / 64) {
+        const long long src = (long long)(at.tr0 - h + rr) * cols + (at.tc0 - h);
+        for (int cc = lane; cc < pitch; cc += 64) gv_lds[rr * pitch + cc] = rec[src + cc];
+    }
+    __syncthreads();
+}
+
+// SRC: 0 global records, the pixel at least second - 1 from the raster edge; 1 global records
+// near the edge (off-raster steps repeat the last sample); 2 records staged in LDS by stage_tile.
+template <int SRC>
+__device__ __forceinline__ GvfOut ground_view(const DomainG &G, const PixelAt &at, const v2f me,
+                                              const float building, const AS4 double *H,
+                                              const AS4 int32_t *gofs, const AS4 double *gbounds,
+                                              int first, int second) {
+    constexpr bool CHECK = SRC == 1;
+    const int hh = second - 1, pitch = at.tx + 2 * hh;
+    const int lbase = SRC == 2 ? (at.r - at.tr0 + hh) * pitch + (at.c - at.tc0 + hh) : 0;
     const int rows = (int)G.rows, cols = (int)G.cols;
-    const AS1 v4f *rec = G.rec;
+    const long long P = G.pixels;
+    const AS1 v2f *rec = G.rec;
     const float lwall = (float)H[SWH_LWALL];
     const float albedo_b = (float)H[SWH_ALBEDO_B];
-    const float building = (float)(__float_as_uint(me.w) & 0xffu);
     const bool wallbol = G.walls[at.p] > 0.0f;
     const double aspect = ((double)G.aspect[at.p] * 3.141592653589793) / 180.0;
     const float firstf = (float)first, secondf = (float)second;
@@ -595,54 +632,68 @@ __device__ __forceinline__ GvfOut ground_view(const DomainG &G, const PixelAt &a
     const float inv_s = 1.0f / secondf, inv_s1 = 1.0f / (secondf + 1.0f);
     const int stride = (second + CHUNK - 1) / CHUNK * CHUNK;   // padded step axis of gofs
     const float local_lup = me.x * (1.0f - building);
-    const float local_albsh = me.z * (1.0f - building);
-    const float local_nosh = me.y * (1.0f - building);
+    const float local_albsh = fabsf(me.y) * (1.0f - building);
     GvfOut o = {};
     for (int dir = 0; dir < 18; ++dir) {
         const AS4 int32_t *off = gofs + dir * stride * 2;
-        Walk w = {building, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-        Walk f = w;
-        bool snapped = false;
+        const int L = (int)G.gvf_len[(long long)dir * P + at.p];
+        const float Lsteps = (float)L;
+        float lup = 0.f, albsh = 0.f, f_lup = 0.f, f_albsh = 0.f;
+        int kw = 255;                                 // first step on a sunlit wall (255: none)
+        // near field: the sums over the first ``first`` steps (step 0 is the receiver itself)
+        {
+            long long qn = at.p;
+            for (int st = 0; st < first; ++st) {
+                const int rr = at.r + off[2 * st], cc = at.c + off[2 * st + 1];
+                if (!CHECK || ((unsigned)rr < (unsigned)rows && (unsigned)cc < (unsigned)cols))
+                    qn = (long long)rr * cols + cc;
+                const v2f r = st == 0 ? me
+                                      : (SRC == 2 ? gv_lds[lbase + off[2 * st] * pitch + off[2 * st + 1]] : rec[qn]);
+                const float on = __builtin_amdgcn_fmed3f(Lsteps - (float)st, 0.0f, 1.0f);
+                f_lup = fmaf(on, r.x, f_lup);
+                f_albsh = fmaf(on, fabsf(r.y), f_albsh);
+            }
+        }
         long long qprev = at.p;
         for (int s0 = 0; s0 < second; s0 += CHUNK) {
-            if (__ballot(w.af != 0.0f) == 0) break;    // every lane of the wave has stopped
-            v4f buf[CHUNK];
+            if (__ballot(s0 < L) == 0) break;         // every lane's walk has ended
+            v2f buf[CHUNK];
 #pragma unroll
             for (int i = 0; i < CHUNK; ++i) {
                 const int dr = off[2 * (s0 + i)], dc = off[2 * (s0 + i) + 1];
-                long long q;
-                if (!CHECK) {
-                    q = at.p + (long long)(dr * cols + dc);
+                if (SRC == 2) {
+                    buf[i] = gv_lds[lbase + dr * pitch + dc];
                 } else {
-                    const int rr = at.r + dr, cc = at.c + dc;
-                    const bool in = (unsigned)rr < (unsigned)rows && (unsigned)cc < (unsigned)cols;
-                    q = in ? (long long)rr * cols + cc : qprev;
-                    qprev = q;
+                    long long q;
+                    if (!CHECK) {
+                        q = at.p + (long long)(dr * cols + dc);
+                    } else {
+                        const int rr = at.r + dr, cc = at.c + dc;
+                        const bool in = (unsigned)rr < (unsigned)rows && (unsigned)cc < (unsigned)cols;
+                        q = in ? (long long)rr * cols + cc : qprev;
+                        qprev = q;
+                    }
+                    buf[i] = rec[q];
                 }
-                buf[i] = rec[q];
             }
+            // step s is summed while s < L: on = clamp(L - s, 0, 1) (steps past ``second`` are
+            // padding with L <= second, so they are off); the sunlit-wall bits of the chunk are
+            // kept unmasked, a wall past L is discarded below by kw < L
+            const float rem = Lsteps - (float)s0;
+            uint32_t cm = 0u;
 #pragma unroll
             for (int i = 0; i < CHUNK; ++i) {
-                const int st = s0 + i;
-                if (st >= second) break;
-                const uint32_t fl = __float_as_uint(buf[i].w);
-                w.af *= (float)(fl & 0xffu);
-                w.lup = fmaf(w.af, buf[i].x, w.lup);
-                w.alb = fmaf(w.af, buf[i].y, w.alb);
-                w.albsh = fmaf(w.af, buf[i].z, w.albsh);
-                w.L += w.af;
-                // wf is 0 or 1: the larger bit pattern is the larger value
-                w.wf = __uint_as_float(max(__float_as_uint(w.wf),
-                                           __float_as_uint(w.af * (float)((fl >> 8) & 0xffu))));
-                w.cw = fmaf(w.wf, w.af, w.cw);
-                if (st == first - 1) { f = w; snapped = true; }
+                const float on = __builtin_amdgcn_fmed3f(rem - (float)i, 0.0f, 1.0f);
+                lup = fmaf(on, buf[i].x, lup);
+                albsh = fmaf(on, fabsf(buf[i].y), albsh);
+                cm |= (__float_as_uint(buf[i].y) >> 31) << i;
             }
+            if (kw == 255 && cm != 0u) kw = s0 + __builtin_ctz(cm);
         }
-        if (!snapped) f = w;                          // the wave stopped before step ``first``
-        const float cnt_wall = w.cw + w.wf * (secondf - w.L), cnt_any = secondf - w.L;
-        const float f_cnt_wall = f.cw + f.wf * (firstf - f.L), f_cnt_any = firstf - f.L;
+        const int Lf = L < first ? L : first;
+        const float cnt_wall = kw < L ? secondf - (float)kw : 0.f, cnt_any = secondf - (float)L;
+        const float f_cnt_wall = kw < Lf ? firstf - (float)kw : 0.f;
         const bool sunin2 = cnt_wall > 0.f, f_sunin = f_cnt_wall > 0.f;
-        const bool wallin2 = cnt_any > 0.f, f_wallin = f_cnt_any > 0.f;
         const AS4 double *bd = gbounds + dir * 3;
         int facesh;
         if (bd[2] == 0.0)
@@ -650,28 +701,94 @@ __device__ __forceinline__ GvfOut ground_view(const DomainG &G, const PixelAt &a
         else
             facesh = 1 - (int)(aspect > bd[0] || aspect <= bd[1]);
         const bool keep = (cnt_wall == secondf) && facesh == 0;
+        (void)cnt_any;
         const float s_lwall = keep ? 0.f : cnt_wall * lwall;
         const float s_albwall = keep ? 0.f : cnt_wall * albedo_b;
         const float f_lwall = f_cnt_wall * lwall, f_albwall = f_cnt_wall * albedo_b;
-        const float s_albwallnosh = cnt_any * albedo_b, f_albwallnosh = f_cnt_any * albedo_b;
-        const float lup_d = distance_weight(influence(f_lwall, f.lup, inv_f1, inv_f, f_sunin),
-                                            influence(s_lwall, w.lup, inv_s1, inv_s, sunin2)) +
+        const float lup_d = distance_weight(influence(f_lwall, f_lup, inv_f1, inv_f, f_sunin),
+                                            influence(s_lwall, lup, inv_s1, inv_s, sunin2)) +
                             local_lup;
-        const float alb_d = distance_weight(influence(f_albwall, f.albsh, inv_f1, inv_f, f_sunin),
-                                            influence(s_albwall, w.albsh, inv_s1, inv_s, sunin2)) +
+        const float alb_d = distance_weight(influence(f_albwall, f_albsh, inv_f1, inv_f, f_sunin),
+                                            influence(s_albwall, albsh, inv_s1, inv_s, sunin2)) +
                             local_albsh;
-        const float nosh_d = distance_weight(influence(f_albwallnosh, f.alb, inv_f1, inv_f, f_wallin),
-                                             influence(s_albwallnosh, w.alb, inv_s, inv_s, wallin2)) *
-                                 building +
-                             local_nosh;
-        o.all.lup += lup_d; o.all.alb += alb_d; o.all.nosh += nosh_d;
+        o.all.lup += lup_d; o.all.alb += alb_d;
         const int az = 5 + 20 * dir;
-        if (az < 180) { o.e.lup += lup_d; o.e.alb += alb_d; o.e.nosh += nosh_d; }
-        if (az >= 90 && az < 270) { o.s.lup += lup_d; o.s.alb += alb_d; o.s.nosh += nosh_d; }
-        if (az >= 180) { o.w.lup += lup_d; o.w.alb += alb_d; o.w.nosh += nosh_d; }
-        if (az >= 270 || az < 90) { o.n.lup += lup_d; o.n.alb += alb_d; o.n.nosh += nosh_d; }
+        if (az < 180) { o.e.lup += lup_d; o.e.alb += alb_d; }
+        if (az >= 90 && az < 270) { o.s.lup += lup_d; o.s.alb += alb_d; }
+        if (az >= 180) { o.w.lup += lup_d; o.w.alb += alb_d; }
+        if (az >= 270 || az < 90) { o.n.lup += lup_d; o.n.alb += alb_d; }
     }
+    const AS1 float *ns = G.gvf_nosh;
+    const long long p = at.p;
+    o.all.nosh = fmaf(albedo_b, ns[5 * P + p], ns[p]);
+    o.e.nosh = fmaf(albedo_b, ns[6 * P + p], ns[P + p]);
+    o.s.nosh = fmaf(albedo_b, ns[7 * P + p], ns[2 * P + p]);
+    o.w.nosh = fmaf(albedo_b, ns[8 * P + p], ns[3 * P + p]);
+    o.n.nosh = fmaf(albedo_b, ns[9 * P + p], ns[4 * P + p]);
     return o;
+}
+
+// The 16-byte view of a pixel's hour record used by the adjoint kernels: Lup field, albedo,
+// albedo x shadow, and the flag word (byte 0 ground, byte 1 sunlit wall, byte 2 shadow class).
+__device__ __forceinline__ v4f record16(const DomainG &G, long long q) {
+    const v2f r = G.rec[q];
+    v4f o;
+    o.x = r.x;
+    o.y = G.alb[q];
+    o.z = fabsf(r.y);
+    o.w = __uint_as_float((G.ground[q] ? 1u : 0u) | ((__float_as_uint(r.y) >> 31) << 8) |
+                          (class_of(G.masks[q]) << 16));
+    return o;
+}
+
+// Static ground-view planes of a domain: per direction the steps before the first non-ground
+// sample, and the unshadowed-albedo sums of the walk (gvfalbnosh) as a + albedo_b b.
+__global__ __launch_bounds__(256) void k_gvf_static(const uint8_t *__restrict__ ground,
+                                                    const float *__restrict__ alb, int rows, int cols,
+                                                    const int32_t *__restrict__ gofs, int first, int second,
+                                                    uint8_t *__restrict__ len, float *__restrict__ nosh) {
+    const long long P = (long long)rows * cols;
+    const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= P) return;
+    const int r = (int)(p / cols), c = (int)(p % cols);
+    const float building = ground[p] ? 1.0f : 0.0f;
+    const float local = alb[p] * (1.0f - building);
+    const float firstf = (float)first, secondf = (float)second;
+    const float inv_f = 1.0f / firstf, inv_f1 = 1.0f / (firstf + 1.0f), inv_s = 1.0f / secondf;
+    const int stride = (second + CHUNK - 1) / CHUNK * CHUNK;
+    float a[5] = {0.f, 0.f, 0.f, 0.f, 0.f}, b[5] = {0.f, 0.f, 0.f, 0.f, 0.f};
+    for (int dir = 0; dir < 18; ++dir) {
+        const int32_t *off = gofs + dir * stride * 2;
+        float af = building, wa = 0.f, fa = 0.f;
+        int L = 0, fL = 0;
+        bool snapped = false;
+        long long q = p;
+        for (int st = 0; st < second; ++st) {
+            const int rr = r + off[2 * st], cc = c + off[2 * st + 1];
+            if ((unsigned)rr < (unsigned)rows && (unsigned)cc < (unsigned)cols) q = (long long)rr * cols + cc;
+            af *= ground[q] ? 1.0f : 0.0f;
+            if (af != 0.f) { ++L; wa += alb[q]; }
+            if (st == first - 1) { fa = wa; fL = L; snapped = true; }
+        }
+        if (!snapped) { fa = wa; fL = L; }
+        const float cnt_any = secondf - (float)L, f_cnt_any = firstf - (float)fL;
+        const float a_near = f_cnt_any > 0.f ? fa * inv_f1 : fa * inv_f;
+        const float b_near = f_cnt_any > 0.f ? f_cnt_any * inv_f1 : 0.f;
+        const float a_far = wa * inv_s, b_far = cnt_any > 0.f ? cnt_any * inv_s : 0.f;
+        const float ad = distance_weight(a_near, a_far) * building + local;
+        const float bd = distance_weight(b_near, b_far) * building;
+        len[(long long)dir * P + p] = (uint8_t)L;
+        a[0] += ad; b[0] += bd;
+        const int az = 5 + 20 * dir;
+        if (az < 180) { a[1] += ad; b[1] += bd; }
+        if (az >= 90 && az < 270) { a[2] += ad; b[2] += bd; }
+        if (az >= 180) { a[3] += ad; b[3] += bd; }
+        if (az >= 270 || az < 90) { a[4] += ad; b[4] += bd; }
+    }
+    for (int g = 0; g < 5; ++g) {
+        nosh[g * P + p] = a[g];
+        nosh[(5 + g) * P + p] = b[g];
+    }
 }
 
 // Cylinder wedge: fraction of sunlit walls seen by a standing cylinder (umep_solweig.py:383-426).
@@ -767,7 +884,7 @@ __global__ __launch_bounds__(BLOCK) void k_hour_pixels(const sw_domain *__restri
                                                        int total, int hour,
                                                        const int32_t *__restrict__ gofs_,
                                                        const double *__restrict__ gbounds_,
-                                                       int first, int second) {
+                                                       int first, int second, int lds_cap) {
     const AS4 int32_t *gofs = (const AS4 int32_t *)gofs_;
     const AS4 double *gbounds = (const AS4 double *)gbounds_;
     DomainG G;
@@ -783,12 +900,23 @@ __global__ __launch_bounds__(BLOCK) void k_hour_pixels(const sw_domain *__restri
     float Kdown = 0.f, Kup = 0.f, KupE = 0.f, KupS = 0.f, KupW = 0.f, KupN = 0.f;
     float KsideI = 0.f, Lup, LupE, LupS, LupW, LupN, shadow = 0.f, Fsh = 0.f, tgout = 0.f, tg = 0.f;
     if (day) {
-        const v4f me = G.rec[p];
-        shadow = shadow_of_class((__float_as_uint(me.w) >> 16) & 3u, (float)(1.0 - (1.0 - H[SWH_PSI])));
+        const v2f me = G.rec[p];
+        const float building = G.ground[p] ? 1.0f : 0.0f;
+        shadow = shadow_of_class(class_of(G.masks[p]), (float)(1.0 - (1.0 - H[SWH_PSI])));
         const int m = second - 1;
-        const bool interior = at.r >= m && at.c >= m && at.r < (int)G.rows - m && at.c < (int)G.cols - m;
-        const GvfOut g = interior ? ground_view<false>(G, at, me, H, gofs, gbounds, first, second)
-                                  : ground_view<true>(G, at, me, H, gofs, gbounds, first, second);
+        // the tile and its halo inside the raster and within the LDS allocation: stage it
+        const bool lds_tile = lds_cap > 0 && at.tx <= 32 && at.tr0 >= m && at.tc0 >= m &&
+                              at.tr0 + at.ty + m <= (int)G.rows && at.tc0 + at.tx + m <= (int)G.cols &&
+                              (at.tx + 2 * m) * (at.ty + 2 * m) <= lds_cap;
+        GvfOut g;
+        if (lds_tile) {
+            stage_tile(G, at, m, at.tx + 2 * m);
+            g = ground_view<2>(G, at, me, building, H, gofs, gbounds, first, second);
+        } else {
+            const bool interior = at.r >= m && at.c >= m && at.r < (int)G.rows - m && at.c < (int)G.cols - m;
+            g = interior ? ground_view<0>(G, at, me, building, H, gofs, gbounds, first, second)
+                         : ground_view<1>(G, at, me, building, H, gofs, gbounds, first, second);
+        }
         const float amb = flux4 * emis;
         const double w = H[SWH_LAG_WEIGHT];
         const bool lf = H[SWH_LAG_FIRST] != 0.0, lu = H[SWH_LAG_UPDATE] != 0.0;
@@ -1012,15 +1140,27 @@ int launch_hour_shadow(const sw_domain *doms, const int32_t *bs, int nd, int tot
     return hip_check(hipGetLastError(), "k_hour_shadow launch");
 }
 
+// Records one workgroup stages for the ground view: the larger of a 16 x 16 and a 32 x 8 tile
+// with a halo of second - 1 pixels (wider tiles read the records from global memory), or 0 when
+// that exceeds 64 KiB.
+int lds_records(int second) {
+    const int h = second - 1;
+    const int a = (16 + 2 * h) * (16 + 2 * h), b = (32 + 2 * h) * (8 + 2 * h);
+    const int n = a > b ? a : b;
+    return n * (int)sizeof(float) * 2 <= 65536 ? n : 0;
+}
+
 int launch_hour_pixels(const sw_domain *doms, const int32_t *bs, int nd, int total, int hour,
                        const int32_t *gofs, const double *gb, int first, int second, bool full,
                        hipStream_t s) {
+    const int cap = lds_records(second);
+    const size_t bytes = (size_t)cap * sizeof(float) * 2;
     if (full)
-        hipLaunchKernelGGL(k_hour_pixels<true>, dim3(total), dim3(BLOCK), 0, s, doms, bs, nd, total,
-                           hour, gofs, gb, first, second);
+        hipLaunchKernelGGL(k_hour_pixels<true>, dim3(total), dim3(BLOCK), bytes, s, doms, bs, nd, total,
+                           hour, gofs, gb, first, second, cap);
     else
-        hipLaunchKernelGGL(k_hour_pixels<false>, dim3(total), dim3(BLOCK), 0, s, doms, bs, nd, total,
-                           hour, gofs, gb, first, second);
+        hipLaunchKernelGGL(k_hour_pixels<false>, dim3(total), dim3(BLOCK), bytes, s, doms, bs, nd, total,
+                           hour, gofs, gb, first, second, cap);
     return hip_check(hipGetLastError(), "k_hour_pixels launch");
 }
 
@@ -1099,7 +1239,6 @@ __device__ __forceinline__ void adj_walk(const DomainG &G, const AdjG &A, const 
                                          const float (&ln)[5], float &d_lwall, float &d_albedo_b) {
     const int rows = (int)G.rows, cols = (int)G.cols;
     const long long P = G.pixels, p = at.p;
-    const AS1 v4f *rec = G.rec;
     const float b = (float)(__float_as_uint(me.w) & 0xffu);
     const bool wallbol = G.walls[p] > 0.0f;
     const double aspect = ((double)G.aspect[p] * 3.141592653589793) / 180.0;
@@ -1137,7 +1276,7 @@ __device__ __forceinline__ void adj_walk(const DomainG &G, const AdjG &A, const 
                     q = in[i] ? (long long)rr * cols + cc : qprev;
                     qprev = q;
                 }
-                buf[i] = rec[q];
+                buf[i] = record16(G, q);
             }
 #pragma unroll
             for (int i = 0; i < CHUNK; ++i) {
@@ -1216,9 +1355,11 @@ __global__ __launch_bounds__(BLOCK) void k_adj_prep(const sw_domain *__restrict_
     if (at.valid && H[SWH_DAY] != 0.0) {
         const long long P = G.pixels, p = at.p;
         const float emis = G.emis[p];
-        const v4f me = G.rec[p];
+        const v4f me = record16(G, p);
         const uint32_t cls = (__float_as_uint(me.w) >> 16) & 3u;
         const float shadow = shadow_of_class(cls, (float)(1.0 - (1.0 - H[SWH_PSI])));
+        const v2f me2 = G.rec[p];
+        const float building = G.ground[p] ? 1.0f : 0.0f;
         const float flux4 = (float)H[SWH_FLUX4];
         const float amb = flux4 * emis;
         const double w = H[SWH_LAG_WEIGHT];
@@ -1229,8 +1370,8 @@ __global__ __launch_bounds__(BLOCK) void k_adj_prep(const sw_domain *__restrict_
         float la[5] = {0.f, 0.f, 0.f, 0.f, 0.f}, ln[5] = {0.f, 0.f, 0.f, 0.f, 0.f};
         if (TOP) {
             // ---- forward of the hour (as k_hour_pixels, day branch) ----------------------
-            const GvfOut g = interior ? ground_view<false>(G, at, me, H, gofs, gbounds, first, second)
-                                      : ground_view<true>(G, at, me, H, gofs, gbounds, first, second);
+            const GvfOut g = interior ? ground_view<0>(G, at, me2, building, H, gofs, gbounds, first, second)
+                                      : ground_view<1>(G, at, me2, building, H, gofs, gbounds, first, second);
             const AS1 float *lg = G.lag;
             float s0 = lg[p], s1 = lg[P + p], s2 = lg[2 * P + p], s3 = lg[3 * P + p], s4 = lg[4 * P + p];
             const float Lup = lag(g.all.lup * INV18 + amb, s0, w, lf, lu);
@@ -1455,8 +1596,7 @@ __global__ __launch_bounds__(BLOCK) void k_adj_gather(const sw_domain *__restric
             }
         }
         // the record of q: Lup = SBC emis ((Tg sh + Ta + 273.15)^4 - (Ta + 273.15)^4), alb, alb sh
-        const v4f r = G.rec[q];
-        const uint32_t cls = (__float_as_uint(r.w) >> 16) & 3u;
+        const uint32_t cls = class_of(G.masks[q]);
         const float sh = shadow_of_class(cls, (float)(1.0 - (1.0 - H[SWH_PSI])));
         const double snup = H[SWH_SNUP];
         const double amp = (double)G.tgk[q] * H[SWH_ALTMAX] + (double)G.tstart[q];
@@ -2525,6 +2665,17 @@ int sw_dsm_walls(const double *dsm, int rows, int cols, double threshold, double
     hipLaunchKernelGGL(k_wall_aspect, grid, block, 0, s, dsm, walls, occupancy, rows, cols, radius, n_angles, line,
                        line_start, side, side_start, step, aspect);
     return hip_check(hipGetLastError(), "wall kernels launch");
+}
+
+int sw_gvf_static(const uint8_t *ground, const float *alb, int rows, int cols, const int32_t *gofs, int first,
+                  int second, uint8_t *len, float *nosh, void *stream) {
+    if (!ground || !alb || !gofs || !len || !nosh) return fail(SW_ERR_ARGUMENT, "sw_gvf_static: NULL pointer");
+    if (rows <= 0 || cols <= 0 || first < 1 || second < 1 || second > MAX_SECOND)
+        return fail(SW_ERR_ARGUMENT, "sw_gvf_static: bad raster or step counts");
+    const long long n = (long long)rows * cols;
+    hipLaunchKernelGGL(k_gvf_static, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, (hipStream_t)stream, ground,
+                       alb, rows, cols, gofs, first, second, len, nosh);
+    return hip_check(hipGetLastError(), "k_gvf_static launch");
 }
 
 int sw_utci_scalar(const float *tmrt, long long n, const double *coefficients, double ta32,

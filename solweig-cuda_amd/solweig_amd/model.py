@@ -261,7 +261,7 @@ def prepare_shadow(a, walls, aspect, scale, *, vegdem=None, vegdem2=None, bush=N
     zero = up(np.float32(0))
     dev = dict(dsm=up(dsm32), trees=up(trees32), walls=up(np.asarray(_host(walls), np.float32)), aspect=up(deg32),
                ground=up(np.uint8(1), np.uint8), zero=zero, one=up(np.float32(1)),
-               tan64=rt.DeviceArray((P,), np.float64), rec=rt.DeviceArray((P, 4), np.float32),
+               tan64=rt.DeviceArray((P,), np.float64), rec=rt.DeviceArray((P, 2), np.float32),
                hmax=rt.DeviceArray.from_host(height_bound(a, trees32)))
     building_max = float(a.max())
     veg_max = float(amaxvalue) if (use and amaxvalue is not None) else float(max(a.max() - a.min(), trees32.max()))

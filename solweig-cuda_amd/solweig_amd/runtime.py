@@ -52,7 +52,8 @@ class SwDomain(ctypes.Structure):
                                         'tmaxlst', 'svfbuveg', 'tan_asvf', 'tan_svfalfa', 'ground', 'vis',
                                         'rec', 'lag', 'tmrt', 'utci', 'hours', 'ptab_f', 'ptab_d',
                                         'ray_off', 'ray_dz', 'masks', 'shadow_planes', 'full', 'skysums',
-                                        'hmax')] + [('hmax_cols', ctypes.c_int64)]
+                                        'hmax')] + [('hmax_cols', ctypes.c_int64)] + \
+        [(n, ctypes.c_uint64) for n in ('gvf_len', 'gvf_nosh')]
 HMAX_CELL = 16
 
 
@@ -113,6 +114,7 @@ def lib():
         'sw_day_adjoint': ([vp, vp, i32, i32, i32, vp, vp, vp, i32, i32, f64, vp], i32),
         'sw_graph_begin': ([vp], i32), 'sw_graph_end': ([vp, ctypes.POINTER(vp)], i32),
         'sw_graph_launch': ([vp, vp], i32), 'sw_graph_destroy': ([vp], i32),
+        'sw_gvf_static': ([vp, vp, i32, i32, vp, i32, i32, vp, vp, vp], i32),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)
@@ -131,7 +133,7 @@ EXPORTED = ('sw_version', 'sw_last_error', 'sw_device_count', 'sw_set_device', '
             'sw_copy', 'sw_graph_begin', 'sw_graph_end', 'sw_graph_launch', 'sw_graph_destroy',
             'sw_adjoint_hour', 'sw_day_adjoint', 'sw_cylindric_wedge', 'sw_wave_delay', 'sw_kup', 'sw_lside',
             'sw_surface_temperature', 'sw_assemble_tmrt', 'sw_gvf', 'sw_pack_visibility', 'sw_longwave',
-            'sw_kside', 'sw_dsm_walls', 'sw_memcpy_d2h_async', 'sw_stream_wait_event')
+            'sw_kside', 'sw_dsm_walls', 'sw_memcpy_d2h_async', 'sw_stream_wait_event', 'sw_gvf_static')
 
 
 def check(rc):

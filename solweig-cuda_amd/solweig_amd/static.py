@@ -139,10 +139,13 @@ class PreparedDomain:
         self.dev['svfveg'] = up(svfveg)
         self.dev['tan_asvf'] = up(np.tan(asvf), np.float64)
         self.dev['tan_svfalfa'] = up(np.tan(svfalfa), np.float64)
-        self.dev['rec'] = rt.DeviceArray((self.P, 4), np.float32)
+        self.dev['rec'] = rt.DeviceArray((self.P, 2), np.float32)
+        self.dev['masks'] = rt.DeviceArray((self.P,), np.uint8)
         self.dev['lag'] = rt.DeviceArray((6, self.P), np.float32)
         self.dev['skysums'] = sky_sums(self.dev['vis'], self.P, stream)
         self.dev['hmax'] = up(height_bound(a, trees))
+        self._check_albedo(maps['alb'])
+        ground_view_static(self, stream)
         if keep_host:
             self.host = dict(a=a, trees=trees, ground=ground, svf_planes=planes, svfbuveg=svfbuveg,
                              asvf=asvf, svfalfa=svfalfa, maps=maps)
@@ -160,14 +163,26 @@ class PreparedDomain:
         self.dev['svfbuveg'].upload(svfbuveg, stream)
 
     def set_maps(self, alb=None, emis=None, stream=None):
-        """Replace the ground albedo / emissivity maps (host arrays or device arrays)."""
+        """Replace the ground albedo / emissivity maps (host arrays or device arrays); the static
+        ground-view albedo sums follow the new albedo map."""
         for key, v in (('alb', alb), ('emis', emis)):
             if v is None:
                 continue
             if isinstance(v, (rt.DeviceArray, rt.DeviceView)):
                 rt.call('sw_memcpy_d2d', self.dev[key].ptr, v.ptr, self.dev[key].nbytes, rt._s(stream))
             else:
+                if key == 'alb':
+                    self._check_albedo(v)
                 self.dev[key].upload(np.asarray(v, dtype=np.float32).reshape(self.shape), stream)
+        if alb is not None:
+            ground_view_static(self, stream)
+
+    @staticmethod
+    def _check_albedo(alb):
+        """The hour record stores albedo x shadow with its sign bit free for the sunlit-wall flag."""
+        a = np.asarray(alb, dtype=np.float32)
+        if not np.all(a >= 0):
+            raise ValueError('albedo maps must be >= 0')
 
     def struct(self, **extra):
         """The sw_domain record for this domain (extra: hours, ptab_f, ... device pointers)."""
@@ -180,12 +195,29 @@ class PreparedDomain:
                            ('emis', 'emis'), ('alb', 'alb'), ('tgk', 'tgk'), ('tstart', 'tstart'),
                            ('tmaxlst', 'tmax'), ('svfbuveg', 'svfbuveg'), ('tan_asvf', 'tan_asvf'),
                            ('tan_svfalfa', 'tan_svfalfa'), ('ground', 'ground'), ('vis', 'vis'),
-                           ('rec', 'rec'), ('lag', 'lag'), ('skysums', 'skysums'), ('hmax', 'hmax')):
+                           ('rec', 'rec'), ('lag', 'lag'), ('skysums', 'skysums'), ('hmax', 'hmax'),
+                           ('masks', 'masks'), ('gvf_len', 'gvf_len'), ('gvf_nosh', 'gvf_nosh')):
             setattr(s, field, d[key].ptr)
         s.hmax_cols = -(-self.shape[1] // rt.HMAX_CELL)
         for k, v in extra.items():
             setattr(s, k, v or 0)
         return s
+
+
+def ground_view_static(prep, stream=None):
+    """(Re)compute the static ground-view planes of a prepared domain (sw_gvf_static): per
+    direction the steps before the first non-ground sample, and the unshadowed-albedo sums."""
+    from .sky import gvf_geometry
+    off, _, first, second = gvf_geometry(prep.domain.scale)
+    d = prep.dev
+    if 'gvf_len' not in d:
+        d['gvf_off'] = rt.DeviceArray.from_host(off, np.int32)
+        d['gvf_len'] = rt.DeviceArray((18, prep.P), np.uint8)
+        d['gvf_nosh'] = rt.DeviceArray((10, prep.P), np.float32)
+    rows, cols = prep.shape
+    rt.call('sw_gvf_static', d['ground'].ptr, d['alb'].ptr, rows, cols, d['gvf_off'].ptr, first, second,
+            d['gvf_len'].ptr, d['gvf_nosh'].ptr, rt._s(stream))
+    rt.call('sw_stream_sync', rt._s(stream))
 
 
 def pack_visibility(shmat, vegshmat, vbshvegshmat):
@@ -233,10 +265,13 @@ def prepared_from_arrays(*, dsm, vegdem, walls, dirwalls, buildings, svfbuveg, a
                     tan_asvf=up(np.tan(np.asarray(asvf, dtype=np.float64)), np.float64),
                     tan_svfalfa=up(np.tan(np.asarray(svfalfa, dtype=np.float64)), np.float64),
                     vis=up(pack_visibility(shmat, vegshmat, vbshvegshmat), np.uint32),
-                    rec=rt.DeviceArray((rows * cols, 4), np.float32),
+                    rec=rt.DeviceArray((rows * cols, 2), np.float32),
+                    masks=rt.DeviceArray((rows * cols,), np.uint8),
                     lag=rt.DeviceArray((6, rows * cols), np.float32))
     self.dev['skysums'] = sky_sums(self.dev['vis'], rows * cols, stream)
     self.dev['hmax'] = up(height_bound(a, trees32))
+    PreparedDomain._check_albedo(plane(alb_grid))
+    ground_view_static(self, stream)
     self.skyview = None
     self.tile_cols = rt.TILE_X
     return self

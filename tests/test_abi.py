@@ -50,12 +50,14 @@ def test_hour_record_layout_matches_header():
     assert vals['SW_HOUR_STRIDE'] == rt.SW_HOUR_STRIDE
 
 
-def test_domain_struct_is_280_bytes():
-    assert ctypes.sizeof(rt.SwDomain) == 280
-    fields = re.findall(r'\*(\w+)', header_text()[header_text().index('typedef struct sw_domain'):])
-    names = [f[0] for f in rt.SwDomain._fields_][6:-1]
-    assert fields[:len(names)] == names
-    assert rt.SwDomain._fields_[-1][0] == 'hmax_cols'
+def test_domain_struct_is_296_bytes():
+    assert ctypes.sizeof(rt.SwDomain) == 296
+    text = header_text()[header_text().index('typedef struct sw_domain'):header_text().index('} sw_domain;')]
+    fields = re.findall(r'\*(\w+)', text)
+    names = [f[0] for f in rt.SwDomain._fields_ if f[0] not in ('rows', 'cols', 'pixels', 'landcover',
+                                                                'dsm_raise', 'tile_cols', 'hmax_cols')]
+    assert fields == names
+    assert [f[0] for f in rt.SwDomain._fields_][-3:] == ['hmax_cols', 'gvf_len', 'gvf_nosh']
 
 
 def test_adjoint_slots_and_struct_match_header():
