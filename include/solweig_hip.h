@@ -269,11 +269,14 @@ int sw_assemble_tmrt(const void *const *fields, const long long *counts, int n_f
 
 /* Static part of the ground view of a domain (the walk of gvf_2018a depends on the hour only
  * through Lup, albedo x shadow and the sunlit walls): per pixel and direction the number of
- * steps before the first non-ground sample (len_dev [18][P] bytes), and the unshadowed-albedo
- * sums of gvfalbnosh split as a + albedo_b * b over all 18 directions and the E, S, W, N halves
- * (nosh_dev [10][P]). ground_dev: [P] 0/1; alb_dev: [P]; offsets as sw_hour_pixels takes them. */
-int sw_gvf_static(const uint8_t *ground_dev, const float *alb_dev, int rows, int cols,
-                  const int32_t *gvf_offsets_dev, int first_steps, int second_steps, uint8_t *len_dev,
+ * steps before the first non-ground sample, with bit 7 set where the pixel's wall faces away
+ * from the direction (facesh == 0, umep_solweig.py:1554-1563) (len_dev [18][P] bytes), and the
+ * unshadowed-albedo sums of gvfalbnosh split as a + albedo_b * b over all 18 directions and the
+ * E, S, W, N halves (nosh_dev [10][P]). ground_dev: [P] 0/1; alb_dev, walls_dev, aspect_dev
+ * (degrees): [P]; offsets and bounds as sw_hour_pixels takes them.                          */
+int sw_gvf_static(const uint8_t *ground_dev, const float *alb_dev, const float *walls_dev,
+                  const float *aspect_dev, int rows, int cols, const int32_t *gvf_offsets_dev,
+                  const double *gvf_bounds_dev, int first_steps, int second_steps, uint8_t *len_dev,
                   float *nosh_dev, void *stream);
 
 /* Ground view factors of one step for arbitrary caller fields (replaces model.gvf,

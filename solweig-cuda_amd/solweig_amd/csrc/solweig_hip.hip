@@ -625,8 +625,7 @@ __device__ __forceinline__ GvfOut ground_view(const DomainG &G, const PixelAt &a
     const AS1 v2f *rec = G.rec;
     const float lwall = (float)H[SWH_LWALL];
     const float albedo_b = (float)H[SWH_ALBEDO_B];
-    const bool wallbol = G.walls[at.p] > 0.0f;
-    const double aspect = ((double)G.aspect[at.p] * 3.141592653589793) / 180.0;
+    (void)gbounds;
     const float firstf = (float)first, secondf = (float)second;
     const float inv_f = 1.0f / firstf, inv_f1 = 1.0f / (firstf + 1.0f);
     const float inv_s = 1.0f / secondf, inv_s1 = 1.0f / (secondf + 1.0f);
@@ -636,7 +635,8 @@ __device__ __forceinline__ GvfOut ground_view(const DomainG &G, const PixelAt &a
     GvfOut o = {};
     for (int dir = 0; dir < 18; ++dir) {
         const AS4 int32_t *off = gofs + dir * stride * 2;
-        const int L = (int)G.gvf_len[(long long)dir * P + at.p];
+        const uint32_t lb = G.gvf_len[(long long)dir * P + at.p];
+        const int L = (int)(lb & 0x7fu);
         const float Lsteps = (float)L;
         float lup = 0.f, albsh = 0.f, f_lup = 0.f, f_albsh = 0.f;
         int kw = 255;                                 // first step on a sunlit wall (255: none)
@@ -694,13 +694,9 @@ __device__ __forceinline__ GvfOut ground_view(const DomainG &G, const PixelAt &a
         const float cnt_wall = kw < L ? secondf - (float)kw : 0.f, cnt_any = secondf - (float)L;
         const float f_cnt_wall = kw < Lf ? firstf - (float)kw : 0.f;
         const bool sunin2 = cnt_wall > 0.f, f_sunin = f_cnt_wall > 0.f;
-        const AS4 double *bd = gbounds + dir * 3;
-        int facesh;
-        if (bd[2] == 0.0)
-            facesh = (int)(aspect < bd[0] || aspect >= bd[1]) - (wallbol ? 1 : 0) + 1;
-        else
-            facesh = 1 - (int)(aspect > bd[0] || aspect <= bd[1]);
-        const bool keep = (cnt_wall == secondf) && facesh == 0;
+        // the wall is removed when the receiver itself is a sunlit wall facing the direction
+        // (facesh == 0, a static bit of gvf_len)
+        const bool keep = kw == 0 && (lb & 0x80u) != 0u;
         (void)cnt_any;
         const float s_lwall = keep ? 0.f : cnt_wall * lwall;
         const float s_albwall = keep ? 0.f : cnt_wall * albedo_b;
@@ -744,8 +740,11 @@ __device__ __forceinline__ v4f record16(const DomainG &G, long long q) {
 // Static ground-view planes of a domain: per direction the steps before the first non-ground
 // sample, and the unshadowed-albedo sums of the walk (gvfalbnosh) as a + albedo_b b.
 __global__ __launch_bounds__(256) void k_gvf_static(const uint8_t *__restrict__ ground,
-                                                    const float *__restrict__ alb, int rows, int cols,
-                                                    const int32_t *__restrict__ gofs, int first, int second,
+                                                    const float *__restrict__ alb,
+                                                    const float *__restrict__ walls,
+                                                    const float *__restrict__ aspect_deg, int rows, int cols,
+                                                    const int32_t *__restrict__ gofs,
+                                                    const double *__restrict__ gbounds, int first, int second,
                                                     uint8_t *__restrict__ len, float *__restrict__ nosh) {
     const long long P = (long long)rows * cols;
     const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -753,6 +752,8 @@ __global__ __launch_bounds__(256) void k_gvf_static(const uint8_t *__restrict__ 
     const int r = (int)(p / cols), c = (int)(p % cols);
     const float building = ground[p] ? 1.0f : 0.0f;
     const float local = alb[p] * (1.0f - building);
+    const bool wallbol = walls[p] > 0.0f;
+    const double aspect = ((double)aspect_deg[p] * 3.141592653589793) / 180.0;
     const float firstf = (float)first, secondf = (float)second;
     const float inv_f = 1.0f / firstf, inv_f1 = 1.0f / (firstf + 1.0f), inv_s = 1.0f / secondf;
     const int stride = (second + CHUNK - 1) / CHUNK * CHUNK;
@@ -777,7 +778,14 @@ __global__ __launch_bounds__(256) void k_gvf_static(const uint8_t *__restrict__ 
         const float a_far = wa * inv_s, b_far = cnt_any > 0.f ? cnt_any * inv_s : 0.f;
         const float ad = distance_weight(a_near, a_far) * building + local;
         const float bd = distance_weight(b_near, b_far) * building;
-        len[(long long)dir * P + p] = (uint8_t)L;
+        // facesh of the direction (umep_solweig.py:1554-1563): the wall faces away from it
+        const double *gbd = gbounds + dir * 3;
+        int facesh;
+        if (gbd[2] == 0.0)
+            facesh = (int)(aspect < gbd[0] || aspect >= gbd[1]) - (wallbol ? 1 : 0) + 1;
+        else
+            facesh = 1 - (int)(aspect > gbd[0] || aspect <= gbd[1]);
+        len[(long long)dir * P + p] = (uint8_t)(L | (facesh == 0 ? 0x80 : 0));
         a[0] += ad; b[0] += bd;
         const int az = 5 + 20 * dir;
         if (az < 180) { a[1] += ad; b[1] += bd; }
@@ -2667,14 +2675,16 @@ int sw_dsm_walls(const double *dsm, int rows, int cols, double threshold, double
     return hip_check(hipGetLastError(), "wall kernels launch");
 }
 
-int sw_gvf_static(const uint8_t *ground, const float *alb, int rows, int cols, const int32_t *gofs, int first,
-                  int second, uint8_t *len, float *nosh, void *stream) {
-    if (!ground || !alb || !gofs || !len || !nosh) return fail(SW_ERR_ARGUMENT, "sw_gvf_static: NULL pointer");
+int sw_gvf_static(const uint8_t *ground, const float *alb, const float *walls, const float *aspect, int rows,
+                  int cols, const int32_t *gofs, const double *gbounds, int first, int second, uint8_t *len,
+                  float *nosh, void *stream) {
+    if (!ground || !alb || !walls || !aspect || !gofs || !gbounds || !len || !nosh)
+        return fail(SW_ERR_ARGUMENT, "sw_gvf_static: NULL pointer");
     if (rows <= 0 || cols <= 0 || first < 1 || second < 1 || second > MAX_SECOND)
         return fail(SW_ERR_ARGUMENT, "sw_gvf_static: bad raster or step counts");
     const long long n = (long long)rows * cols;
     hipLaunchKernelGGL(k_gvf_static, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, (hipStream_t)stream, ground,
-                       alb, rows, cols, gofs, first, second, len, nosh);
+                       alb, walls, aspect, rows, cols, gofs, gbounds, first, second, len, nosh);
     return hip_check(hipGetLastError(), "k_gvf_static launch");
 }
 

@@ -114,7 +114,7 @@ def lib():
         'sw_day_adjoint': ([vp, vp, i32, i32, i32, vp, vp, vp, i32, i32, f64, vp], i32),
         'sw_graph_begin': ([vp], i32), 'sw_graph_end': ([vp, ctypes.POINTER(vp)], i32),
         'sw_graph_launch': ([vp, vp], i32), 'sw_graph_destroy': ([vp], i32),
-        'sw_gvf_static': ([vp, vp, i32, i32, vp, i32, i32, vp, vp, vp], i32),
+        'sw_gvf_static': ([vp, vp, vp, vp, i32, i32, vp, vp, i32, i32, vp, vp, vp], i32),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)

@@ -206,17 +206,20 @@ class PreparedDomain:
 
 def ground_view_static(prep, stream=None):
     """(Re)compute the static ground-view planes of a prepared domain (sw_gvf_static): per
-    direction the steps before the first non-ground sample, and the unshadowed-albedo sums."""
+    direction the steps before the first non-ground sample (bit 7: the wall faces away from the
+    direction), and the unshadowed-albedo sums."""
     from .sky import gvf_geometry
-    off, _, first, second = gvf_geometry(prep.domain.scale)
+    off, bounds, first, second = gvf_geometry(prep.domain.scale)
     d = prep.dev
     if 'gvf_len' not in d:
         d['gvf_off'] = rt.DeviceArray.from_host(off, np.int32)
+        d['gvf_bounds'] = rt.DeviceArray.from_host(bounds, np.float64)
         d['gvf_len'] = rt.DeviceArray((18, prep.P), np.uint8)
         d['gvf_nosh'] = rt.DeviceArray((10, prep.P), np.float32)
     rows, cols = prep.shape
-    rt.call('sw_gvf_static', d['ground'].ptr, d['alb'].ptr, rows, cols, d['gvf_off'].ptr, first, second,
-            d['gvf_len'].ptr, d['gvf_nosh'].ptr, rt._s(stream))
+    rt.call('sw_gvf_static', d['ground'].ptr, d['alb'].ptr, d['walls'].ptr, d['aspect'].ptr, rows, cols,
+            d['gvf_off'].ptr, d['gvf_bounds'].ptr, first, second, d['gvf_len'].ptr, d['gvf_nosh'].ptr,
+            rt._s(stream))
     rt.call('sw_stream_sync', rt._s(stream))
 
 

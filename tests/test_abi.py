@@ -73,3 +73,4 @@ def test_adjoint_slots_and_struct_match_header():
     block = text[text.index('typedef struct sw_adjoint'):text.index('} sw_adjoint;')]
     fields = re.findall(r'\*(\w+);', block)
     assert fields == [f[0] for f in rt.SwAdjoint._fields_][:-1]
+
