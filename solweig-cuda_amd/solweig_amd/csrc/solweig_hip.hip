@@ -678,7 +678,8 @@ __device__ __forceinline__ GvfOut ground_view(const DomainG &G, const PixelAt &a
             }
             // step s is summed while s < L: on = clamp(L - s, 0, 1) (steps past ``second`` are
             // padding with L <= second, so they are off); the sunlit-wall bits of the chunk are
-            // kept unmasked, a wall past L is discarded below by kw < L
+            // kept unmasked, a wall past L is discarded below by kw < L (kw is the first sunlit
+            // wall of the whole walk, so kw < L exactly when one lies among the alive steps)
             const float rem = Lsteps - (float)s0;
             uint32_t cm = 0u;
 #pragma unroll
@@ -686,9 +687,10 @@ __device__ __forceinline__ GvfOut ground_view(const DomainG &G, const PixelAt &a
                 const float on = __builtin_amdgcn_fmed3f(rem - (float)i, 0.0f, 1.0f);
                 lup = fmaf(on, buf[i].x, lup);
                 albsh = fmaf(on, fabsf(buf[i].y), albsh);
-                cm |= (__float_as_uint(buf[i].y) >> 31) << i;
+                // shift the record's sign bit in from the right: bit 7 - i is step s0 + i
+                cm = __builtin_amdgcn_alignbit(cm, __float_as_uint(buf[i].y), 31);
             }
-            if (kw == 255 && cm != 0u) kw = s0 + __builtin_ctz(cm);
+            if (kw == 255 && cm != 0u) kw = s0 + __builtin_clz(cm) - (32 - CHUNK);
         }
         const int Lf = L < first ? L : first;
         const float cnt_wall = kw < L ? secondf - (float)kw : 0.f, cnt_any = secondf - (float)L;
