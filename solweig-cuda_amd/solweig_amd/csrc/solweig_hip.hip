@@ -339,8 +339,8 @@ __device__ __forceinline__ double wave_min_uniform(double v) {
 // True when every sample of steps [k0, ke] of every lane of the wave is at most a_min above
 // the ray: max over the 16 x 16 cells covering the samples' bounding box (the shifts grow
 // monotonically, so the box spans the wave's pixels shifted by the first and the last step)
-// minus the height of the step before k0. Such steps change nothing: a - dz, top - dz,
-// top - dz_prev and bot - dz all stay <= the receiver's height.
+// minus the height of the step before k0. With a_min the smallest threshold of the wave's lanes,
+// such steps change nothing: a - dz, top - dz, top - dz_prev and bot - dz all stay <= it.
 __device__ __forceinline__ bool window_below(const DomainG &G, const PixelAt &at, const AS4 int32_t *off,
                                              int k0, int ke, int rows, int cols, double dz_before,
                                              double a_min) {
@@ -409,19 +409,24 @@ __global__ __launch_bounds__(BLOCK) void k_hour_shadow(const sw_domain *__restri
                                                        int total, int hour) {
 #pragma clang fp contract(off)
     DomainG G;
-    const PixelAt at = locate(doms, bs, nd, total, G);
+    PixelAt at = locate(doms, bs, nd, total, G);
     const AS4 double *H = G.hours + (long long)hour * SW_HOUR_STRIDE;
     if (H[SWH_DAY] == 0.0) return;
     const int rows = (int)G.rows, cols = (int)G.cols;
+    // Lanes past the raster (on the last tile row or column) march along with a copy of the
+    // nearest pixel, so that the wave-wide thresholds below see every lane; they neither limit a
+    // skip nor keep the wave marching, and write nothing.
+    const bool valid = at.valid;
+    if (!valid) {
+        at.r = min(at.r, rows - 1);
+        at.c = min(at.c, cols - 1);
+        at.p = (long long)at.r * cols + at.c;
+    }
     const long long P = G.pixels, p = at.p;
     const AS1 float *dsm = G.dsm, *trees = G.trees;
     const double raise = G.dsm_raise;
-    Heights me = {0.0, 0.0, 0.0};
-    if (at.valid) me = heights_at(dsm, trees, raise, p);
-    // lowest receiver of the wave: a window of steps whose samples all stay at or below it
-    // (bounded through the coarse height map) cannot change any lane's march
-    const double a_min = wave_min_uniform(at.valid ? me.a : __builtin_huge_val());
-    if (!at.valid) return;
+    const Heights me = heights_at(dsm, trees, raise, p);
+    const bool wall_px = valid && G.walls[p] > 0.0f;
     const int start = uniform_index(H[SWH_RAY_START]);
     const int L = uniform_index(H[SWH_RAY_LEN]);
     const AS4 int32_t *off = G.ray_off + 2LL * start;
@@ -442,8 +447,18 @@ __global__ __launch_bounds__(BLOCK) void k_hour_shadow(const sw_domain *__restri
     const uint64_t active = __ballot(all_lanes);
     for (int k0 = 0; k0 < L; k0 += CHUNK) {
         if (k0 > 0 && (k0 & (SKIP_WINDOW - 1)) == 0) {
+            // A lane's remaining steps matter only through samples above its threshold: while
+            // not building-shadowed, its own height (f, the canopy test and topmax change only
+            // through samples above it); once shadowed, veg is false for good (so ever and
+            // topmax no longer matter), and the running maximum f matters only on a wall pixel.
+            // A window whose samples stay at or below every lane's threshold changes nothing;
+            // when no lane has a threshold left, the march is over.
+            const double thr = !valid ? __builtin_huge_val()
+                                      : (m.f > me.a ? (wall_px ? m.f : __builtin_huge_val()) : me.a);
+            const double thr_min = wave_min_uniform(thr);
+            if (thr_min == __builtin_huge_val()) break;
             const int ke = (k0 + SKIP_WINDOW < L ? k0 + SKIP_WINDOW : L) - 1;
-            if (window_below(G, at, off, k0, ke, rows, cols, dzs[k0 - 1], a_min)) {
+            if (window_below(G, at, off, k0, ke, rows, cols, dzs[k0 - 1], thr_min)) {
                 m.prev = dzs[ke];
                 k0 = ke + 1 - CHUNK;
                 continue;
@@ -452,7 +467,7 @@ __global__ __launch_bounds__(BLOCK) void k_hour_shadow(const sw_domain *__restri
         const int kl = k0 + CHUNK - 1 < L ? k0 + CHUNK - 1 : L - 1;
         const bool in_last = (unsigned)(at.r + off[2 * kl]) < (unsigned)rows &&
                              (unsigned)(at.c + off[2 * kl + 1]) < (unsigned)cols;
-        const bool in_first = (unsigned)(at.r + off[2 * k0]) < (unsigned)rows &&
+        const bool in_first = valid && (unsigned)(at.r + off[2 * k0]) < (unsigned)rows &&
                               (unsigned)(at.c + off[2 * k0 + 1]) < (unsigned)cols;
         if (__ballot(in_first) == 0) break;          // every lane's ray has left the raster
         if (k0 + CHUNK <= L && __ballot(in_last) == active)
@@ -460,6 +475,7 @@ __global__ __launch_bounds__(BLOCK) void k_hour_shadow(const sw_domain *__restri
         else
             march_chunk<true>(m, dsm, trees, raise, me.a, at, rows, cols, off, dzs, k0, L);
     }
+    if (!valid) return;
     const double f = m.f, topmax = m.topmax;
     const bool veg = m.veg, ever = m.ever;
     const bool blocked = f > me.a;
