@@ -648,7 +648,11 @@ __device__ __forceinline__ GvfOut ground_view(const DomainG &G, const PixelAt &a
     const int stride = (second + CHUNK - 1) / CHUNK * CHUNK;   // padded step axis of gofs
     const float local_lup = me.x * (1.0f - building);
     const float local_albsh = fabsf(me.y) * (1.0f - building);
-    GvfOut o = {};
+    // per direction weights of the near (first steps) and far (all steps) fields, with and
+    // without a sunlit wall in the field (the divisors n + 1 and n of gvf.cu:209-239)
+    const float kF1 = FAR_W * inv_s1, kF0 = FAR_W * inv_s, kN1 = NEAR_W * inv_f1, kN0 = NEAR_W * inv_f;
+    // group sums (all, E, S, W, N) of the wall weight, the Lup and the albedo x shadow terms
+    float aB[5] = {0.f, 0.f, 0.f, 0.f, 0.f}, aL[5] = {0.f, 0.f, 0.f, 0.f, 0.f}, aA[5] = {0.f, 0.f, 0.f, 0.f, 0.f};
     for (int dir = 0; dir < 18; ++dir) {
         const AS4 int32_t *off = gofs + dir * stride * 2;
         const uint32_t lb = G.gvf_len[(long long)dir * P + at.p];
@@ -657,7 +661,11 @@ __device__ __forceinline__ GvfOut ground_view(const DomainG &G, const PixelAt &a
         float lup = 0.f, albsh = 0.f, f_lup = 0.f, f_albsh = 0.f;
         int kw = 255;                                 // first step on a sunlit wall (255: none)
         // near field: the sums over the first ``first`` steps (step 0 is the receiver itself)
-        {
+        if (first == 1) {
+            const float on = __builtin_amdgcn_fmed3f(Lsteps, 0.0f, 1.0f);
+            f_lup = on * me.x;
+            f_albsh = on * fabsf(me.y);
+        } else {
             long long qn = at.p;
             for (int st = 0; st < first; ++st) {
                 const int rr = at.r + off[2 * st], cc = at.c + off[2 * st + 1];
@@ -708,29 +716,35 @@ __device__ __forceinline__ GvfOut ground_view(const DomainG &G, const PixelAt &a
             }
             if (kw == 255 && cm != 0u) kw = s0 + __builtin_clz(cm) - (32 - CHUNK);
         }
+        // A sunlit wall in the far field (kw < L) or the near field (kw < min(L, first)) adds
+        // (n - kw) wall terms and switches the field's divisor from n to n + 1; ``keep`` removes
+        // the far-field wall terms of a receiver that is itself a sunlit wall facing the
+        // direction (facesh == 0, a static bit of gvf_len). Each direction's Lup and albedo terms
+        // are then linear in the wall flux (lwall) and wall albedo (albedo_b) with one weight B.
         const int Lf = L < first ? L : first;
-        const float cnt_wall = kw < L ? secondf - (float)kw : 0.f, cnt_any = secondf - (float)L;
-        const float f_cnt_wall = kw < Lf ? firstf - (float)kw : 0.f;
-        const bool sunin2 = cnt_wall > 0.f, f_sunin = f_cnt_wall > 0.f;
-        // the wall is removed when the receiver itself is a sunlit wall facing the direction
-        // (facesh == 0, a static bit of gvf_len)
+        const bool sunin2 = kw < L, f_sunin = kw < Lf;
         const bool keep = kw == 0 && (lb & 0x80u) != 0u;
-        (void)cnt_any;
-        const float s_lwall = keep ? 0.f : cnt_wall * lwall;
-        const float s_albwall = keep ? 0.f : cnt_wall * albedo_b;
-        const float f_lwall = f_cnt_wall * lwall, f_albwall = f_cnt_wall * albedo_b;
-        const float lup_d = distance_weight(influence(f_lwall, f_lup, inv_f1, inv_f, f_sunin),
-                                            influence(s_lwall, lup, inv_s1, inv_s, sunin2)) +
-                            local_lup;
-        const float alb_d = distance_weight(influence(f_albwall, f_albsh, inv_f1, inv_f, f_sunin),
-                                            influence(s_albwall, albsh, inv_s1, inv_s, sunin2)) +
-                            local_albsh;
-        o.all.lup += lup_d; o.all.alb += alb_d;
+        const float kF = sunin2 ? kF1 : kF0, kN = f_sunin ? kN1 : kN0;
+        const float wF = (sunin2 && !keep) ? secondf - (float)kw : 0.f;
+        const float wN = f_sunin ? firstf - (float)kw : 0.f;
+        const float B = fmaf(kN, wN, kF * wF);
+        const float tl = fmaf(kN, f_lup, kF * lup);
+        const float ta = fmaf(kN, f_albsh, kF * albsh);
+        aB[0] += B; aL[0] += tl; aA[0] += ta;
         const int az = 5 + 20 * dir;
-        if (az < 180) { o.e.lup += lup_d; o.e.alb += alb_d; }
-        if (az >= 90 && az < 270) { o.s.lup += lup_d; o.s.alb += alb_d; }
-        if (az >= 180) { o.w.lup += lup_d; o.w.alb += alb_d; }
-        if (az >= 270 || az < 90) { o.n.lup += lup_d; o.n.alb += alb_d; }
+#pragma unroll
+        for (int g = 1; g < 5; ++g) {
+            const bool in = g == 1 ? az < 180 : (g == 2 ? (az >= 90 && az < 270) : (g == 3 ? az >= 180 : (az >= 270 || az < 90)));
+            if (in) { aB[g] += B; aL[g] += tl; aA[g] += ta; }
+        }
+    }
+    GvfOut o;
+    Triple *grp[5] = {&o.all, &o.e, &o.s, &o.w, &o.n};
+#pragma unroll
+    for (int g = 0; g < 5; ++g) {
+        const float n = g == 0 ? 18.0f : 9.0f;
+        grp[g]->lup = fmaf(lwall, aB[g], aL[g]) + n * local_lup;
+        grp[g]->alb = fmaf(albedo_b, aB[g], aA[g]) + n * local_albsh;
     }
     const AS1 float *ns = G.gvf_nosh;
     const long long p = at.p;
