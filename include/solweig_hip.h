@@ -47,10 +47,14 @@ enum {
     SWH_TGWALL,
     SWH_SKY_A, SWH_SKY_B, /* sky patch flux = A*omega*trig - B*g*omega*trig, A = SBC*T^4/pi,
                              B = (1 - esky)*A, g = exp(0.308*(1.7 - 1/cos(zenith))) per band */
+    SWH_ELIG,             /* 5 values: the 32-patch words of the sun-test eligibility bits
+                             (patch k: bit k % 32 of word k / 32), as exact integers          */
     SW_HOUR_STRIDE = 64
 };
 
-/* patch tables per hour row: SW_PATCH_F floats and SW_PATCH_D doubles per patch */
+/* patch tables per hour row: SW_PATCH_F floats per patch ([153][SW_PATCH_F]) and SW_PATCH_D
+ * rows of 153 doubles ([SW_PATCH_D][153]: yi of every patch, then the two thresholds, equal
+ * within an altitude band)                                                                   */
 enum {
     SWP_LV = 0, SWP_KD, SWP_LDN, SWP_LSD, SWP_WCOS, SWP_WSIN, SWP_DE, SWP_DW, SWP_DN, SWP_DS,
     SWP_ELIGIBLE, SW_PATCH_F = 12
@@ -84,7 +88,7 @@ typedef struct sw_domain {
     float *tmrt, *utci;                                      /* [n_hours][P] (may be NULL)     */
     const double *hours;                                     /* [n_hours][SW_HOUR_STRIDE]     */
     const float *ptab_f;                                     /* [rows][153][SW_PATCH_F]       */
-    const double *ptab_d;                                    /* [rows][153][SW_PATCH_D]       */
+    const double *ptab_d;                                    /* [rows][SW_PATCH_D][153]       */
     const int32_t *ray_off;                                  /* [sum L][2] (row, col) shifts  */
     const double *ray_dz;                                    /* [sum L] ray heights            */
     uint8_t *masks;       /* [P] bits of the last shadow pass (read by the pixel pass): 1 not

@@ -32,7 +32,7 @@ class Engine:
             cap = self.n_hours * (max(d.shape) + 2 + STEP_BLOCK)
             b = dict(hours=rt.DeviceArray((self.n_hours, rt.SW_HOUR_STRIDE), np.float64),
                      ptab_f=rt.DeviceArray((self.n_hours, rt.SW_PATCHES, rt.SW_PATCH_F), np.float32),
-                     ptab_d=rt.DeviceArray((self.n_hours, rt.SW_PATCHES, rt.SW_PATCH_D), np.float64),
+                     ptab_d=rt.DeviceArray((self.n_hours, rt.SW_PATCH_D, rt.SW_PATCHES), np.float64),
                      ray_off=rt.DeviceArray((cap, 2), np.int32), ray_dz=rt.DeviceArray((cap,), np.float64),
                      cap=cap)
             if keep_outputs:

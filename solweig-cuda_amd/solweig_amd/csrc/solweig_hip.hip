@@ -868,7 +868,7 @@ constexpr int BAND_START[9] = {0, 31, 61, 89, 113, 132, 145, 152, 153};
 // since the trig weights are per band, those sums are band counts times the band weight.
 __device__ __forceinline__ void day_patch_sums(const AS1 uint32_t *vis, long long P, long long p,
                                                const AS4 float *pf, const AS4 double *pd,
-                                               double tan_asvf, float dveg, float &a_lv, float &a_kd,
+                                               const AS4 double *H, double tan_asvf, float dveg, float &a_lv, float &a_kd,
                                                float &a_swsun, float &a_swsh, float &a_lsunc,
                                                float &a_lsuns, float &a_lshc, float &a_lshs) {
 #pragma unroll
@@ -886,20 +886,22 @@ __device__ __forceinline__ void day_patch_sums(const AS1 uint32_t *vis, long lon
             a_kd = fmaf(ds, c[SWP_KD], a_kd);
         }
         if (__ballot(bld != 0u) == 0) continue;       // no building patch in this word
-        uint32_t sun = 0u, shd = 0u, elig = 0u;
-#pragma unroll
-        for (int j = 0; j < n; ++j) {
-            const AS4 double *cd = pd + (k0 + j) * SW_PATCH_D;
-            const double x = tan_asvf + cd[SWP_YI];
-            sun |= (x < cd[SWP_SUN_BELOW] ? 1u : 0u) << j;
-            shd |= (x >= cd[SWP_SHADE_FROM] ? 1u : 0u) << j;
-            elig |= (pf[(k0 + j) * SW_PATCH_F + SWP_ELIGIBLE] != 0.0f ? 1u : 0u) << j;
-        }
+        const uint32_t elig = (uint32_t)H[SWH_ELIG + w];
 #pragma unroll
         for (int b = 0; b < 8; ++b) {
             const int lo = BAND_START[b] > k0 ? BAND_START[b] : k0;
             const int hi = BAND_START[b + 1] < k0 + n ? BAND_START[b + 1] : k0 + n;
             if (lo >= hi) continue;
+            // the two thresholds depend on the patch altitude only: one pair per band
+            const double t_sun = pd[SWP_SUN_BELOW * SW_PATCHES + BAND_START[b]];
+            const double t_sh = pd[SWP_SHADE_FROM * SW_PATCHES + BAND_START[b]];
+            uint32_t sun = 0u, shd = 0u;
+#pragma unroll
+            for (int k = lo; k < hi; ++k) {
+                const double x = tan_asvf + pd[SWP_YI * SW_PATCHES + k];
+                sun |= (x < t_sun ? 1u : 0u) << (k - k0);
+                shd |= (x >= t_sh ? 1u : 0u) << (k - k0);
+            }
             const uint32_t span = (uint32_t)(hi - lo);
             const uint32_t mask = (span == 32u ? 0xffffffffu : ((1u << span) - 1u)) << (lo - k0);
             const uint32_t bm = bld & mask;
@@ -999,7 +1001,7 @@ __global__ __launch_bounds__(BLOCK) void k_hour_pixels(const sw_domain *__restri
     // ---- anisotropic patch sums over the 153 sky patches --------------------------------
     const int row = uniform_index(H[SWH_PATCH_TABLE]);
     const AS4 float *pf = G.ptab_f + (long long)row * SW_PATCHES * SW_PATCH_F;
-    const AS4 double *pd = G.ptab_d + (long long)row * SW_PATCHES * SW_PATCH_D;
+    const AS4 double *pd = G.ptab_d + (long long)row * SW_PATCH_D * SW_PATCHES;
     const double tan_asvf = G.tan_asvf[p];
     const float dveg = (float)H[SWH_DIFF_VEG];
     float a_lv = 0.f, a_kd = 0.f, a_ldn = 0.f, a_lsd = 0.f, a_vc = 0.f, a_vs = 0.f;
@@ -1025,7 +1027,7 @@ __global__ __launch_bounds__(BLOCK) void k_hour_pixels(const sw_domain *__restri
         }
     }
     if (!FULL && day) {
-        day_patch_sums(vis, P, p, pf, pd, tan_asvf, dveg, a_lv, a_kd, a_swsun, a_swsh, a_lsunc,
+        day_patch_sums(vis, P, p, pf, pd, H, tan_asvf, dveg, a_lv, a_kd, a_swsun, a_swsh, a_lsunc,
                        a_lsuns, a_lshc, a_lshs);
     }
     if (FULL) {
@@ -1040,7 +1042,6 @@ __global__ __launch_bounds__(BLOCK) void k_hour_pixels(const sw_domain *__restri
             for (int k = k0; k < k1; ++k) {
                 const int j = k - k0;
                 const AS4 float *c = pf + k * SW_PATCH_F;
-                const AS4 double *cd = pd + k * SW_PATCH_D;
                 const float wcos = c[SWP_WCOS], wsin = c[SWP_WSIN];
                 if ((sky >> j) & 1) { lv_sky += c[SWP_LV]; kd_sky += c[SWP_KD]; }
                 if ((vegonly >> j) & 1) { lv_veg += c[SWP_LV]; kd_veg += c[SWP_KD]; }
@@ -1067,9 +1068,9 @@ __global__ __launch_bounds__(BLOCK) void k_hour_pixels(const sw_domain *__restri
                 if ((bld >> j) & 1) {
                     bool sunlit = false, shaded = true;
                     if (day) {
-                        const double x = tan_asvf + cd[SWP_YI];
-                        sunlit = x < cd[SWP_SUN_BELOW];
-                        shaded = x >= cd[SWP_SHADE_FROM];
+                        const double x = tan_asvf + pd[SWP_YI * SW_PATCHES + k];
+                        sunlit = x < pd[SWP_SUN_BELOW * SW_PATCHES + k];
+                        shaded = x >= pd[SWP_SHADE_FROM * SW_PATCHES + k];
                     }
                     if (sunlit) a_swsun += wcos;
                     if (shaded) a_swsh += wcos;
@@ -1435,10 +1436,10 @@ __global__ __launch_bounds__(BLOCK) void k_adj_prep(const sw_domain *__restrict_
             const float KsideI = shadow * radI * cosalt;
             const int row = uniform_index(H[SWH_PATCH_TABLE]);
             const AS4 float *pf = G.ptab_f + (long long)row * SW_PATCHES * SW_PATCH_F;
-            const AS4 double *pd = G.ptab_d + (long long)row * SW_PATCHES * SW_PATCH_D;
+            const AS4 double *pd = G.ptab_d + (long long)row * SW_PATCH_D * SW_PATCHES;
             float a_lv = 0.f, a_kd = 0.f, a_swsun = 0.f, a_swsh = 0.f, a_lsunc = 0.f, a_lsuns = 0.f;
             float a_lshc = 0.f, a_lshs = 0.f;
-            day_patch_sums(G.vis, P, p, pf, pd, G.tan_asvf[p], (float)H[SWH_DIFF_VEG], a_lv, a_kd,
+            day_patch_sums(G.vis, P, p, pf, pd, H, G.tan_asvf[p], (float)H[SWH_DIFF_VEG], a_lv, a_kd,
                            a_swsun, a_swsh, a_lsunc, a_lsuns, a_lshc, a_lshs);
             const AS1 float *ss = G.skysums;
             const float SA = (float)H[SWH_SKY_A], SB = (float)H[SWH_SKY_B];

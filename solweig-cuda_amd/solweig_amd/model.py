@@ -95,7 +95,7 @@ def prepare_from_domain(dom, location, scale=1.0, first=1.0, second=22.0, stream
     cap = max(rows, cols) + 2 + sky.STEP_BLOCK
     bufs = dict(hours=rt.DeviceArray((1, rt.SW_HOUR_STRIDE), np.float64),
                 ptab_f=rt.DeviceArray((1, rt.SW_PATCHES, rt.SW_PATCH_F), np.float32),
-                ptab_d=rt.DeviceArray((1, rt.SW_PATCHES, rt.SW_PATCH_D), np.float64),
+                ptab_d=rt.DeviceArray((1, rt.SW_PATCH_D, rt.SW_PATCHES), np.float64),
                 ray_off=rt.DeviceArray((cap, 2), np.int32), ray_dz=rt.DeviceArray((cap,), np.float64),
                 full=rt.DeviceArray((len(rt.FULL_PLANES), P), np.float32),
                 shadow_planes=rt.DeviceArray((8, P), np.float64), tmrt=rt.DeviceArray((1, P), np.float32))
@@ -146,7 +146,7 @@ def run_step(prep, *, altitude, azimuth, zen, jday, dectime, altmax, Ta, RH, rad
     h[rt.H['PATCH_TABLE']] = 0
     prep.hours.upload(h[None], prep.stream)
     prep.ptab_f.upload(pf[None], prep.stream)
-    prep.ptab_d.upload(pd[None], prep.stream)
+    prep.ptab_d.upload(np.ascontiguousarray(pd.T)[None], prep.stream)
     if len(dz):
         off, dz = sky.pad_steps(off, dz)
         prep.ray_off.upload(off, prep.stream)

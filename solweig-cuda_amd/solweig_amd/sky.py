@@ -14,7 +14,7 @@ from functools import lru_cache
 
 import numpy as np
 
-from .runtime import H, PD, PF, SW_HOUR_STRIDE, SW_PATCH_D, SW_PATCH_F, SW_PATCHES
+from .runtime import H, PD, PF, SW_HOUR_STRIDE, SW_PATCH_D, SW_PATCH_F, SW_PATCHES, SW_VIS_WORDS
 from . import utci as _utci
 
 SBC = 5.67051e-8
@@ -488,6 +488,11 @@ def hour_tables(alt, azi, zen, jday, dectime, altmax, Ta, RH, radG, P, Ws, doy, 
     h[H['SKY_A']] = SBC * ((Ta + 273.15) ** 4) / np.pi
     h[H['SKY_B']] = (1 - esky) * h[H['SKY_A']]
     pf, pd = patch_rows(alt > 0, esky, Ta, alt, azi, lv, radD, f64=f64) if rows else (None, None)
+    if rows:
+        bits = np.flatnonzero(pf[:, PF['ELIGIBLE']] != 0)
+        for w in range(SW_VIS_WORDS):
+            word = bits[(bits >= 32 * w) & (bits < 32 * w + 32)] - 32 * w
+            h[H['ELIG'] + w] = float(sum(1 << int(j) for j in word))
     meta = dict(CI=CI, esky=esky, radI=radI, radD=radD, tgwall=tgwall, I0=I0, ea=ea, CI_Tg=citg,
                 CI_TgG=citgg, lv=lv, firstdaytime=firstdaytime, timeadd=timeadd)
     return h, pf, pd, off, dz, (CI, firstdaytime, timeadd, timestepdec), meta
@@ -502,19 +507,20 @@ def build_day(met, sun, latitude, shape, scale, amaxvalue, *, landcover=0, wall_
     st = (1., 1, 0., timestepdec) if state is None else tuple(state) + (timestepdec,)
     hours = np.zeros((n, SW_HOUR_STRIDE))
     pf = np.zeros((n, SW_PATCHES, SW_PATCH_F), dtype=np.float32)
-    pd = np.zeros((n, SW_PATCHES, SW_PATCH_D))
+    pd = np.zeros((n, SW_PATCH_D, SW_PATCHES))
     offs, dzs, meta = [], [], []
     start = 0
     for i in range(n):
         if dectime[i] - np.floor(dectime[i]) == 0:
             st = (1.,) + st[1:]
         st_in = st
-        h, pf[i], pd[i], off, dz, st, m = hour_tables(
+        h, pf[i], d_i, off, dz, st, m = hour_tables(
             float(sun['altitude'][i]), float(sun['azimuth'][i]), float(sun['zen'][i]), float(sun['jday'][i]),
             float(dectime[i]), float(sun['altmax'][i]), *(float(met[i, k]) for k in (11, 10, 14, 12, 9)),
             met[i, 1], latitude, shape, scale, amaxvalue, st, wall_params=wall_params, constants=constants,
             psi=psi)
         m['state_in'] = st_in
+        pd[i] = d_i.T
         h[H['RAY_START']] = start
         h[H['PATCH_TABLE']] = i
         hours[i] = h
