@@ -100,8 +100,9 @@ typedef struct sw_domain {
     const float *hmax;    /* [ceil(rows/16)][hmax_cols] upper bound of max(dsm, canopy top) per
                              16 x 16 cell (FP32 rounded up), lets the sun-ray march skip steps */
     int64_t hmax_cols;    /* ceil(cols/16)                                                    */
-    const uint8_t *gvf_len;  /* [18][P] ground-view steps before the first non-ground sample
-                                (static, sw_gvf_static)                                        */
+    const uint32_t *gvf_len; /* [5][P] ground-view steps before the first non-ground sample,
+                                one byte per direction (direction d: byte d % 4 of word d / 4;
+                                static, sw_gvf_static)                                         */
     const float *gvf_nosh;   /* [10][P] static albedo sums of the ground view: a (all, E, S, W,
                                 N) then b; gvfalbnosh sum = a + albedo_b * b                    */
 } sw_domain;
@@ -274,14 +275,22 @@ int sw_assemble_tmrt(const void *const *fields, const long long *counts, int n_f
 /* Static part of the ground view of a domain (the walk of gvf_2018a depends on the hour only
  * through Lup, albedo x shadow and the sunlit walls): per pixel and direction the number of
  * steps before the first non-ground sample, with bit 7 set where the pixel's wall faces away
- * from the direction (facesh == 0, umep_solweig.py:1554-1563) (len_dev [18][P] bytes), and the
+ * from the direction (facesh == 0, umep_solweig.py:1554-1563) (len_dev [5][P] 32-bit words,
+ * direction d in byte d % 4 of word d / 4, bytes 2 and 3 of word 4 zero), and the
  * unshadowed-albedo sums of gvfalbnosh split as a + albedo_b * b over all 18 directions and the
  * E, S, W, N halves (nosh_dev [10][P]). ground_dev: [P] 0/1; alb_dev, walls_dev, aspect_dev
  * (degrees): [P]; offsets and bounds as sw_hour_pixels takes them.                          */
 int sw_gvf_static(const uint8_t *ground_dev, const float *alb_dev, const float *walls_dev,
                   const float *aspect_dev, int rows, int cols, const int32_t *gvf_offsets_dev,
-                  const double *gvf_bounds_dev, int first_steps, int second_steps, uint8_t *len_dev,
+                  const double *gvf_bounds_dev, int first_steps, int second_steps, uint32_t *len_dev,
                   float *nosh_dev, void *stream);
+
+/* The ground-view step shifts compiled into the library: out receives [18][n][2] (row, column)
+ * for the first n <= 22 steps of the 18 directions (sunonsurface_2018a, umep_solweig.py:1477-1492).
+ * With first_steps 1 and second_steps 22 (scale 1) sw_hour_pixels walks these shifts instead of
+ * gvf_offsets_dev, which must then hold the same values (as prepare_gvf_geometry builds them).
+ * No counterpart in the reference (it passes the table only).                                */
+int sw_gvf_fixed_steps(int32_t *out, int n);
 
 /* Ground view factors of one step for arbitrary caller fields (replaces model.gvf,
  * model.py:569-622, and its gvf_pack / gvf_compact / gvf kernels, gvf.cu:79-292; UMEP

@@ -98,7 +98,7 @@ struct DomainG {
     const AS1 float *skysums;
     const AS4 float *hmax;
     int hmax_cols;
-    const AS1 uint8_t *gvf_len;
+    const AS1 uint32_t *gvf_len;
     const AS1 float *gvf_nosh;
 };
 
@@ -124,7 +124,7 @@ __device__ __forceinline__ DomainG domain_at(const sw_domain *D) {
     g.masks = (AS1 uint8_t *)d->masks; g.shadow_planes = (AS1 double *)d->shadow_planes;
     g.full = (AS1 float *)d->full; g.skysums = (const AS1 float *)d->skysums;
     g.hmax = (const AS4 float *)d->hmax; g.hmax_cols = (int)d->hmax_cols;
-    g.gvf_len = (const AS1 uint8_t *)d->gvf_len; g.gvf_nosh = (const AS1 float *)d->gvf_nosh;
+    g.gvf_len = (const AS1 uint32_t *)d->gvf_len; g.gvf_nosh = (const AS1 float *)d->gvf_nosh;
     return g;
 }
 
@@ -626,6 +626,76 @@ __device__ __forceinline__ void stage_tile(const DomainG &G, const PixelAt &at, 
     __syncthreads();
 }
 
+// The per-direction sums of one pixel's walk folded into the five groups (all, E, S, W, N).
+// A sunlit wall in the far field (kw < L) or the near field (kw < min(L, first)) adds (n - kw)
+// wall terms and switches the field's divisor from n to n + 1 (gvf.cu:209-239); ``keep``
+// removes the far-field wall terms of a receiver that is itself a sunlit wall facing the
+// direction (facesh == 0, a static bit of gvf_len). Each direction's Lup and albedo terms are
+// then linear in the wall flux (lwall) and wall albedo (albedo_b) with one weight B.
+struct GvSums {
+    float firstf, secondf, kF1, kF0, kN1, kN0;
+    float aB[5], aL[5], aA[5];     // wall weight, Lup and albedo x shadow sums per group
+
+    __device__ __forceinline__ GvSums(int first, int second) {
+        firstf = (float)first;
+        secondf = (float)second;
+        // per direction weights of the near (first steps) and far (all steps) fields, with and
+        // without a sunlit wall in the field (the divisors n + 1 and n)
+        kF1 = FAR_W * (1.0f / (secondf + 1.0f));
+        kF0 = FAR_W * (1.0f / secondf);
+        kN1 = NEAR_W * (1.0f / (firstf + 1.0f));
+        kN0 = NEAR_W * (1.0f / firstf);
+#pragma unroll
+        for (int g = 0; g < 5; ++g) aB[g] = aL[g] = aA[g] = 0.f;
+    }
+
+    __device__ __forceinline__ void add(int dir, int L, uint32_t lb, int kw, int first, float f_lup,
+                                        float f_albsh, float lup, float albsh) {
+        const int Lf = L < first ? L : first;
+        const bool sunin2 = kw < L, f_sunin = kw < Lf;
+        const bool keep = kw == 0 && (lb & 0x80u) != 0u;
+        const float kF = sunin2 ? kF1 : kF0, kN = f_sunin ? kN1 : kN0;
+        const float wF = (sunin2 && !keep) ? secondf - (float)kw : 0.f;
+        const float wN = f_sunin ? firstf - (float)kw : 0.f;
+        const float B = fmaf(kN, wN, kF * wF);
+        const float tl = fmaf(kN, f_lup, kF * lup);
+        const float ta = fmaf(kN, f_albsh, kF * albsh);
+        aB[0] += B; aL[0] += tl; aA[0] += ta;
+        const int az = 5 + 20 * dir;
+#pragma unroll
+        for (int g = 1; g < 5; ++g) {
+            const bool in = g == 1 ? az < 180 : (g == 2 ? (az >= 90 && az < 270) : (g == 3 ? az >= 180 : (az >= 270 || az < 90)));
+            if (in) { aB[g] += B; aL[g] += tl; aA[g] += ta; }
+        }
+    }
+
+    // the receiver's own terms (n x its Lup and albedo x shadow on non-building pixels) and the
+    // hour-independent unshadowed-albedo sums of the static planes (gvf_nosh)
+    __device__ __forceinline__ GvfOut finish(const DomainG &G, const PixelAt &at, const AS4 double *H,
+                                             const v2f me, float building) const {
+        const float lwall = (float)H[SWH_LWALL];
+        const float albedo_b = (float)H[SWH_ALBEDO_B];
+        const float local_lup = me.x * (1.0f - building);
+        const float local_albsh = fabsf(me.y) * (1.0f - building);
+        GvfOut o;
+        Triple *grp[5] = {&o.all, &o.e, &o.s, &o.w, &o.n};
+#pragma unroll
+        for (int g = 0; g < 5; ++g) {
+            const float n = g == 0 ? 18.0f : 9.0f;
+            grp[g]->lup = fmaf(lwall, aB[g], aL[g]) + n * local_lup;
+            grp[g]->alb = fmaf(albedo_b, aB[g], aA[g]) + n * local_albsh;
+        }
+        const AS1 float *ns = G.gvf_nosh;
+        const long long P = G.pixels, p = at.p;
+        o.all.nosh = fmaf(albedo_b, ns[5 * P + p], ns[p]);
+        o.e.nosh = fmaf(albedo_b, ns[6 * P + p], ns[P + p]);
+        o.s.nosh = fmaf(albedo_b, ns[7 * P + p], ns[2 * P + p]);
+        o.w.nosh = fmaf(albedo_b, ns[8 * P + p], ns[3 * P + p]);
+        o.n.nosh = fmaf(albedo_b, ns[9 * P + p], ns[4 * P + p]);
+        return o;
+    }
+};
+
 // SRC: 0 global records, the pixel at least second - 1 from the raster edge; 1 global records
 // near the edge (off-raster steps repeat the last sample); 2 records staged in LDS by stage_tile.
 template <int SRC>
@@ -639,23 +709,12 @@ __device__ __forceinline__ GvfOut ground_view(const DomainG &G, const PixelAt &a
     const int rows = (int)G.rows, cols = (int)G.cols;
     const long long P = G.pixels;
     const AS1 v2f *rec = G.rec;
-    const float lwall = (float)H[SWH_LWALL];
-    const float albedo_b = (float)H[SWH_ALBEDO_B];
     (void)gbounds;
-    const float firstf = (float)first, secondf = (float)second;
-    const float inv_f = 1.0f / firstf, inv_f1 = 1.0f / (firstf + 1.0f);
-    const float inv_s = 1.0f / secondf, inv_s1 = 1.0f / (secondf + 1.0f);
     const int stride = (second + CHUNK - 1) / CHUNK * CHUNK;   // padded step axis of gofs
-    const float local_lup = me.x * (1.0f - building);
-    const float local_albsh = fabsf(me.y) * (1.0f - building);
-    // per direction weights of the near (first steps) and far (all steps) fields, with and
-    // without a sunlit wall in the field (the divisors n + 1 and n of gvf.cu:209-239)
-    const float kF1 = FAR_W * inv_s1, kF0 = FAR_W * inv_s, kN1 = NEAR_W * inv_f1, kN0 = NEAR_W * inv_f;
-    // group sums (all, E, S, W, N) of the wall weight, the Lup and the albedo x shadow terms
-    float aB[5] = {0.f, 0.f, 0.f, 0.f, 0.f}, aL[5] = {0.f, 0.f, 0.f, 0.f, 0.f}, aA[5] = {0.f, 0.f, 0.f, 0.f, 0.f};
+    GvSums gv(first, second);
     for (int dir = 0; dir < 18; ++dir) {
         const AS4 int32_t *off = gofs + dir * stride * 2;
-        const uint32_t lb = G.gvf_len[(long long)dir * P + at.p];
+        const uint32_t lb = (G.gvf_len[(long long)(dir >> 2) * P + at.p] >> (8 * (dir & 3))) & 0xffu;
         const int L = (int)(lb & 0x7fu);
         const float Lsteps = (float)L;
         float lup = 0.f, albsh = 0.f, f_lup = 0.f, f_albsh = 0.f;
@@ -716,44 +775,74 @@ __device__ __forceinline__ GvfOut ground_view(const DomainG &G, const PixelAt &a
             }
             if (kw == 255 && cm != 0u) kw = s0 + __builtin_clz(cm) - (32 - CHUNK);
         }
-        // A sunlit wall in the far field (kw < L) or the near field (kw < min(L, first)) adds
-        // (n - kw) wall terms and switches the field's divisor from n to n + 1; ``keep`` removes
-        // the far-field wall terms of a receiver that is itself a sunlit wall facing the
-        // direction (facesh == 0, a static bit of gvf_len). Each direction's Lup and albedo terms
-        // are then linear in the wall flux (lwall) and wall albedo (albedo_b) with one weight B.
-        const int Lf = L < first ? L : first;
-        const bool sunin2 = kw < L, f_sunin = kw < Lf;
-        const bool keep = kw == 0 && (lb & 0x80u) != 0u;
-        const float kF = sunin2 ? kF1 : kF0, kN = f_sunin ? kN1 : kN0;
-        const float wF = (sunin2 && !keep) ? secondf - (float)kw : 0.f;
-        const float wN = f_sunin ? firstf - (float)kw : 0.f;
-        const float B = fmaf(kN, wN, kF * wF);
-        const float tl = fmaf(kN, f_lup, kF * lup);
-        const float ta = fmaf(kN, f_albsh, kF * albsh);
-        aB[0] += B; aL[0] += tl; aA[0] += ta;
-        const int az = 5 + 20 * dir;
+        gv.add(dir, L, lb, kw, first, f_lup, f_albsh, lup, albsh);
+    }
+    return gv.finish(G, at, H, me, building);
+}
+
+// The ground-view shifts of the 18 directions for the first GV_FIXED_STEPS steps
+// (sunonsurface_2018a, umep_solweig.py:1477-1492), the prefix of every scale's table.
+constexpr int GV_FIXED_STEPS = 22;
+constexpr int8_t GV_STEPS[18][GV_FIXED_STEPS][2] = {
+#include "gvf_steps.inc"
+};
+
+// ground_view for a tile of width TX staged in LDS, first = 1 and second = GV_FIXED_STEPS
+// (scale 1): the same sums as ground_view<2>, with every step's LDS address an immediate offset
+// from the pixel's position in the staged tile (no per-step address arithmetic or table loads).
+template <int TX>
+__device__ __forceinline__ GvfOut ground_view_fixed(const DomainG &G, const PixelAt &at, const v2f me,
+                                                    const float building, const AS4 double *H) {
+    constexpr int NS = GV_FIXED_STEPS, HH = NS - 1, PITCH = TX + 2 * HH;
+    const long long P = G.pixels;
+    // the pixel's record sits at (HH, HH) from this base; every step's index is base + constant
+    const int base = (at.r - at.tr0) * PITCH + (at.c - at.tc0);
+    uint32_t lw[5];                                   // the 18 direction bytes, loaded at once
 #pragma unroll
-        for (int g = 1; g < 5; ++g) {
-            const bool in = g == 1 ? az < 180 : (g == 2 ? (az >= 90 && az < 270) : (g == 3 ? az >= 180 : (az >= 270 || az < 90)));
-            if (in) { aB[g] += B; aL[g] += tl; aA[g] += ta; }
+    for (int w = 0; w < 5; ++w) lw[w] = G.gvf_len[w * P + at.p];
+    GvSums gv(1, NS);
+#pragma unroll
+    for (int dir = 0; dir < 18; ++dir) {
+        const uint32_t lb = (lw[dir >> 2] >> (8 * (dir & 3))) & 0xffu;
+        const int L = (int)(lb & 0x7fu);
+        // kept opaque: as an integer-valued float the compiler rewrites the clamps below into
+        // integer subtract, sign-extend and convert sequences
+        float Lsteps = (float)L;
+        asm volatile("" : "+v"(Lsteps));
+        // near field: step 0, the receiver itself
+        const float on0 = __builtin_amdgcn_fmed3f(Lsteps, 0.0f, 1.0f);
+        const float f_lup = on0 * me.x, f_albsh = on0 * fabsf(me.y);
+        float lup = 0.f, albsh = 0.f;
+        int kw = 255;
+#pragma unroll
+        for (int s0 = 0; s0 < NS; s0 += CHUNK) {
+            if (__ballot(s0 < L) == 0) break;
+            const int n = NS - s0 < CHUNK ? NS - s0 : CHUNK;
+            v2f buf[CHUNK];
+#pragma unroll
+            for (int i = 0; i < n; ++i) {
+                const int k = (GV_STEPS[dir][s0 + i][0] + HH) * PITCH + GV_STEPS[dir][s0 + i][1] + HH;
+                buf[i] = gv_lds[base + k];
+            }
+            const float rem = Lsteps - (float)s0;
+            uint32_t cm = 0u;
+#pragma unroll
+            for (int i = 0; i < n; ++i) {
+                const float on = __builtin_amdgcn_fmed3f(rem - (float)i, 0.0f, 1.0f);
+                lup = fmaf(on, buf[i].x, lup);
+                albsh = fmaf(on, fabsf(buf[i].y), albsh);
+                cm = __builtin_amdgcn_alignbit(cm, __float_as_uint(buf[i].y), 31);
+            }
+            if (kw == 255 && cm != 0u) kw = s0 + __builtin_clz(cm) - (32 - n);
         }
+        gv.add(dir, L, lb, kw, 1, f_lup, f_albsh, lup, albsh);
+        // the group sums are complete here: without this the compiler keeps every direction's
+        // partial terms live to the end of the walk (150 VGPRs instead of about 80)
+        asm volatile("" : "+v"(gv.aB[0]), "+v"(gv.aB[1]), "+v"(gv.aB[2]), "+v"(gv.aB[3]), "+v"(gv.aB[4]),
+                     "+v"(gv.aL[0]), "+v"(gv.aL[1]), "+v"(gv.aL[2]), "+v"(gv.aL[3]), "+v"(gv.aL[4]),
+                     "+v"(gv.aA[0]), "+v"(gv.aA[1]), "+v"(gv.aA[2]), "+v"(gv.aA[3]), "+v"(gv.aA[4]));
     }
-    GvfOut o;
-    Triple *grp[5] = {&o.all, &o.e, &o.s, &o.w, &o.n};
-#pragma unroll
-    for (int g = 0; g < 5; ++g) {
-        const float n = g == 0 ? 18.0f : 9.0f;
-        grp[g]->lup = fmaf(lwall, aB[g], aL[g]) + n * local_lup;
-        grp[g]->alb = fmaf(albedo_b, aB[g], aA[g]) + n * local_albsh;
-    }
-    const AS1 float *ns = G.gvf_nosh;
-    const long long p = at.p;
-    o.all.nosh = fmaf(albedo_b, ns[5 * P + p], ns[p]);
-    o.e.nosh = fmaf(albedo_b, ns[6 * P + p], ns[P + p]);
-    o.s.nosh = fmaf(albedo_b, ns[7 * P + p], ns[2 * P + p]);
-    o.w.nosh = fmaf(albedo_b, ns[8 * P + p], ns[3 * P + p]);
-    o.n.nosh = fmaf(albedo_b, ns[9 * P + p], ns[4 * P + p]);
-    return o;
+    return gv.finish(G, at, H, me, building);
 }
 
 // The 16-byte view of a pixel's hour record used by the adjoint kernels: Lup field, albedo,
@@ -777,7 +866,7 @@ __global__ __launch_bounds__(256) void k_gvf_static(const uint8_t *__restrict__ 
                                                     const float *__restrict__ aspect_deg, int rows, int cols,
                                                     const int32_t *__restrict__ gofs,
                                                     const double *__restrict__ gbounds, int first, int second,
-                                                    uint8_t *__restrict__ len, float *__restrict__ nosh) {
+                                                    uint32_t *__restrict__ len, float *__restrict__ nosh) {
     const long long P = (long long)rows * cols;
     const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= P) return;
@@ -790,6 +879,7 @@ __global__ __launch_bounds__(256) void k_gvf_static(const uint8_t *__restrict__ 
     const float inv_f = 1.0f / firstf, inv_f1 = 1.0f / (firstf + 1.0f), inv_s = 1.0f / secondf;
     const int stride = (second + CHUNK - 1) / CHUNK * CHUNK;
     float a[5] = {0.f, 0.f, 0.f, 0.f, 0.f}, b[5] = {0.f, 0.f, 0.f, 0.f, 0.f};
+    uint32_t words[5] = {0u, 0u, 0u, 0u, 0u};
     for (int dir = 0; dir < 18; ++dir) {
         const int32_t *off = gofs + dir * stride * 2;
         float af = building, wa = 0.f, fa = 0.f;
@@ -817,7 +907,7 @@ __global__ __launch_bounds__(256) void k_gvf_static(const uint8_t *__restrict__ 
             facesh = (int)(aspect < gbd[0] || aspect >= gbd[1]) - (wallbol ? 1 : 0) + 1;
         else
             facesh = 1 - (int)(aspect > gbd[0] || aspect <= gbd[1]);
-        len[(long long)dir * P + p] = (uint8_t)(L | (facesh == 0 ? 0x80 : 0));
+        words[dir >> 2] |= (uint32_t)(L | (facesh == 0 ? 0x80 : 0)) << (8 * (dir & 3));
         a[0] += ad; b[0] += bd;
         const int az = 5 + 20 * dir;
         if (az < 180) { a[1] += ad; b[1] += bd; }
@@ -828,6 +918,7 @@ __global__ __launch_bounds__(256) void k_gvf_static(const uint8_t *__restrict__ 
     for (int g = 0; g < 5; ++g) {
         nosh[g * P + p] = a[g];
         nosh[(5 + g) * P + p] = b[g];
+        len[g * P + p] = words[g];
     }
 }
 
@@ -953,7 +1044,10 @@ __global__ __launch_bounds__(BLOCK) void k_hour_pixels(const sw_domain *__restri
         GvfOut g;
         if (lds_tile) {
             stage_tile(G, at, m, at.tx + 2 * m);
-            g = ground_view<2>(G, at, me, building, H, gofs, gbounds, first, second);
+            const bool fixed = first == 1 && second == GV_FIXED_STEPS;
+            if (fixed && at.tx == 16) g = ground_view_fixed<16>(G, at, me, building, H);
+            else if (fixed && at.tx == 32) g = ground_view_fixed<32>(G, at, me, building, H);
+            else g = ground_view<2>(G, at, me, building, H, gofs, gbounds, first, second);
         } else {
             const bool interior = at.r >= m && at.c >= m && at.r < (int)G.rows - m && at.c < (int)G.cols - m;
             g = interior ? ground_view<0>(G, at, me, building, H, gofs, gbounds, first, second)
@@ -2709,7 +2803,7 @@ int sw_dsm_walls(const double *dsm, int rows, int cols, double threshold, double
 }
 
 int sw_gvf_static(const uint8_t *ground, const float *alb, const float *walls, const float *aspect, int rows,
-                  int cols, const int32_t *gofs, const double *gbounds, int first, int second, uint8_t *len,
+                  int cols, const int32_t *gofs, const double *gbounds, int first, int second, uint32_t *len,
                   float *nosh, void *stream) {
     if (!ground || !alb || !walls || !aspect || !gofs || !gbounds || !len || !nosh)
         return fail(SW_ERR_ARGUMENT, "sw_gvf_static: NULL pointer");
@@ -2719,6 +2813,15 @@ int sw_gvf_static(const uint8_t *ground, const float *alb, const float *walls, c
     hipLaunchKernelGGL(k_gvf_static, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, (hipStream_t)stream, ground,
                        alb, walls, aspect, rows, cols, gofs, gbounds, first, second, len, nosh);
     return hip_check(hipGetLastError(), "k_gvf_static launch");
+}
+
+int sw_gvf_fixed_steps(int32_t *out, int n) {
+    if (!out) return fail(SW_ERR_ARGUMENT, "sw_gvf_fixed_steps: NULL pointer");
+    if (n < 0 || n > GV_FIXED_STEPS) return fail(SW_ERR_ARGUMENT, "sw_gvf_fixed_steps: n must be 0..22");
+    for (int d = 0; d < 18; ++d)
+        for (int k = 0; k < n; ++k)
+            for (int j = 0; j < 2; ++j) out[(d * n + k) * 2 + j] = GV_STEPS[d][k][j];
+    return SW_OK;
 }
 
 int sw_utci_scalar(const float *tmrt, long long n, const double *coefficients, double ta32,

@@ -115,6 +115,7 @@ def lib():
         'sw_graph_begin': ([vp], i32), 'sw_graph_end': ([vp, ctypes.POINTER(vp)], i32),
         'sw_graph_launch': ([vp, vp], i32), 'sw_graph_destroy': ([vp], i32),
         'sw_gvf_static': ([vp, vp, vp, vp, i32, i32, vp, vp, i32, i32, vp, vp, vp], i32),
+        'sw_gvf_fixed_steps': ([vp, i32], i32),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)
@@ -133,7 +134,8 @@ EXPORTED = ('sw_version', 'sw_last_error', 'sw_device_count', 'sw_set_device', '
             'sw_copy', 'sw_graph_begin', 'sw_graph_end', 'sw_graph_launch', 'sw_graph_destroy',
             'sw_adjoint_hour', 'sw_day_adjoint', 'sw_cylindric_wedge', 'sw_wave_delay', 'sw_kup', 'sw_lside',
             'sw_surface_temperature', 'sw_assemble_tmrt', 'sw_gvf', 'sw_pack_visibility', 'sw_longwave',
-            'sw_kside', 'sw_dsm_walls', 'sw_memcpy_d2h_async', 'sw_stream_wait_event', 'sw_gvf_static')
+            'sw_kside', 'sw_dsm_walls', 'sw_memcpy_d2h_async', 'sw_stream_wait_event', 'sw_gvf_static',
+            'sw_gvf_fixed_steps')
 
 
 def check(rc):

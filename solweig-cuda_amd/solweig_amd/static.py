@@ -214,7 +214,7 @@ def ground_view_static(prep, stream=None):
     if 'gvf_len' not in d:
         d['gvf_off'] = rt.DeviceArray.from_host(off, np.int32)
         d['gvf_bounds'] = rt.DeviceArray.from_host(bounds, np.float64)
-        d['gvf_len'] = rt.DeviceArray((18, prep.P), np.uint8)
+        d['gvf_len'] = rt.DeviceArray((5, prep.P), np.uint32)
         d['gvf_nosh'] = rt.DeviceArray((10, prep.P), np.float32)
     rows, cols = prep.shape
     rt.call('sw_gvf_static', d['ground'].ptr, d['alb'].ptr, d['walls'].ptr, d['aspect'].ptr, rows, cols,

@@ -74,3 +74,20 @@ def test_adjoint_slots_and_struct_match_header():
     fields = re.findall(r'\*(\w+);', block)
     assert fields == [f[0] for f in rt.SwAdjoint._fields_][:-1]
 
+
+
+@pytest.mark.parametrize('scale', [1.0, 0.5, 2.0, 1.4])
+def test_compiled_ground_view_steps_match_geometry(scale):
+    """The step shifts compiled into the library (walked instead of the offsets table at
+    scale 1) equal the first steps of the ground-view geometry at every scale."""
+    if not os.path.exists(rt.LIBRARY):
+        pytest.skip('library not built')
+    import numpy as np
+    from solweig_amd.sky import gvf_geometry
+    off, _, _, second = gvf_geometry(scale)
+    n = min(second, 22)
+    out = np.zeros((18, n, 2), np.int32)
+    rt.call('sw_gvf_fixed_steps', out.ctypes.data, n)
+    np.testing.assert_array_equal(out, off[:, :n])
+    with pytest.raises(RuntimeError):
+        rt.call('sw_gvf_fixed_steps', out.ctypes.data, 23)

@@ -97,6 +97,45 @@ def test_tile_widths_give_identical_days(rt):
             assert np.array_equal(out[0], ref[0], equal_nan=True) and np.array_equal(out[1], ref[1], equal_nan=True), w
 
 
+def _staged_tiles(rows, cols, width, second):
+    """Tiles whose ground-view halo lies inside the raster (the pixel pass stages them in LDS)."""
+    m, height = second - 1, 256 // width
+    if width > 32:
+        return 0
+    return sum(1 for r0 in range(0, rows, height) for c0 in range(0, cols, width)
+               if r0 >= m and c0 >= m and r0 + height + m <= rows and c0 + width + m <= cols)
+
+
+@pytest.mark.parametrize('scale', [1.0, 0.5])
+def test_tile_widths_identical_with_staged_tiles(rt, scale):
+    """96 x 104 raster: at widths 16 and 32 interior tiles are staged in LDS (at scale 1 with the
+    compiled step shifts, at scale 0.5 with the offsets table), at 64 and 128 none are; every
+    width gives the same bits."""
+    from solweig_amd import batched, synthetic
+    from solweig_amd.sky import gvf_geometry
+    d = synthetic.small_domain(seed=62, size=104, maps=True)
+    d.dsm, d.trees, d.dem = d.dsm[:96], d.trees[:96], d.dem[:96]
+    d.walls, d.aspect = d.walls[:96], d.aspect[:96]
+    d.alb_grid, d.emis_grid = d.alb_grid[:96], d.emis_grid[:96]
+    d.scale = scale
+    second = gvf_geometry(scale)[3]
+    assert _staged_tiles(96, 104, 16, second) > 0 and _staged_tiles(96, 104, 32, second) > 0
+    eng = batched.Engine([d], n_hours=24)
+    tabs = eng.tables()
+    ref = None
+    for w in rt.TILE_CHOICES:
+        print(f'scale {scale} tile width {w}: {_staged_tiles(96, 104, w, second)} staged tiles', flush=True)
+        eng.set_tiles(w)
+        eng.load(tabs)
+        eng.reset_state()
+        eng.forward_day()
+        out = eng.outputs(0)
+        if ref is None:
+            ref = out
+        else:
+            assert np.array_equal(out[0], ref[0], equal_nan=True) and np.array_equal(out[1], ref[1], equal_nan=True), w
+
+
 def test_autotuner_picks_a_measured_width(rt, tmp_path, monkeypatch):
     from solweig_amd import autotune, batched, synthetic
     monkeypatch.setattr(autotune, 'CACHE', str(tmp_path / 'tiles.json'))

@@ -96,6 +96,27 @@ def test_day_matches_oracle(rt, kw):
     assert et < TOL_K and eu < TOL_K and nan_ok
 
 
+@pytest.mark.parametrize('width', [16, 32])
+def test_day_matches_oracle_with_staged_tiles(rt, width):
+    """96 x 96: the interior tiles walk the ground view from LDS with the compiled step shifts."""
+    from solweig_amd import batched, sun as psun
+    d = _domain(96, 12, maps=True)
+    print(f'day vs oracle 96x96, tile width {width}', flush=True)
+    s = psun.met_sun(d.met, d.location, d.utc)
+    eng = batched.Engine([d], n_hours=d.met.shape[0])
+    eng.set_tiles(width)
+    eng.load(eng.tables(sun=[s]))
+    eng.reset_state()
+    eng.forward_day()
+    tmrt, utci = eng.outputs(0)
+    p = oday.prepare(d.dsm, d.trees, d.dem, d.walls, d.aspect, d.latitude, landcover=d.landcover,
+                     alb_grid=d.alb_grid, emis_grid=d.emis_grid)
+    ref = oday.run_day(p, d.met, s, keep=('Tmrt',))
+    et, eu, nan_ok = _day_errors(tmrt, utci, ref)
+    print(f'  max |dTmrt| {et:.3e} K, max |dUTCI| {eu:.3e} K', flush=True)
+    assert et < TOL_K and eu < TOL_K and nan_ok
+
+
 @pytest.mark.parametrize('prefix', ['dayA_', 'dayB_', 'dayC_'])
 def test_day_matches_umep_golden(rt, golden, prefix):
     from solweig_amd import batched, synthetic
