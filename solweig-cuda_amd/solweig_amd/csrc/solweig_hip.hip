@@ -626,15 +626,18 @@ __device__ __forceinline__ void stage_tile(const DomainG &G, const PixelAt &at, 
     __syncthreads();
 }
 
-// The per-direction sums of one pixel's walk folded into the five groups (all, E, S, W, N).
-// A sunlit wall in the far field (kw < L) or the near field (kw < min(L, first)) adds (n - kw)
-// wall terms and switches the field's divisor from n to n + 1 (gvf.cu:209-239); ``keep``
-// removes the far-field wall terms of a receiver that is itself a sunlit wall facing the
-// direction (facesh == 0, a static bit of gvf_len). Each direction's Lup and albedo terms are
-// then linear in the wall flux (lwall) and wall albedo (albedo_b) with one weight B.
+// The per-direction sums of one pixel's walk folded into the four halves E (azimuth < 180),
+// S (90 <= azimuth < 270), W and N; the sum over all 18 directions is E + W. A sunlit wall in
+// the far field (kw < L) or the near field (kw < min(L, first)) adds (n - kw) wall terms and
+// switches the field's divisor from n to n + 1 (gvf.cu:209-239); ``keep`` removes the far-field
+// wall terms of a receiver that is itself a sunlit wall facing the direction (facesh == 0, a
+// static bit of gvf_len). Each direction's Lup and albedo terms are then linear in the wall flux
+// (lwall) and wall albedo (albedo_b) with one weight B. With first == 1 the near field is the
+// receiver alone (L >= 1 exactly on ground pixels, kw == 0 exactly on a sunlit-wall receiver),
+// the same for every direction: it is added once per half by finish().
 struct GvSums {
     float firstf, secondf, kF1, kF0, kN1, kN0;
-    float aB[5], aL[5], aA[5];     // wall weight, Lup and albedo x shadow sums per group
+    float hB[4], hL[4], hA[4];     // wall weight, Lup and albedo x shadow sums of E, S, W, N
 
     __device__ __forceinline__ GvSums(int first, int second) {
         firstf = (float)first;
@@ -646,45 +649,63 @@ struct GvSums {
         kN1 = NEAR_W * (1.0f / (firstf + 1.0f));
         kN0 = NEAR_W * (1.0f / firstf);
 #pragma unroll
-        for (int g = 0; g < 5; ++g) aB[g] = aL[g] = aA[g] = 0.f;
+        for (int g = 0; g < 4; ++g) hB[g] = hL[g] = hA[g] = 0.f;
     }
 
-    __device__ __forceinline__ void add(int dir, int L, uint32_t lb, int kw, int first, float f_lup,
-                                        float f_albsh, float lup, float albsh) {
-        const int Lf = L < first ? L : first;
-        const bool sunin2 = kw < L, f_sunin = kw < Lf;
-        const bool keep = kw == 0 && (lb & 0x80u) != 0u;
-        const float kF = sunin2 ? kF1 : kF0, kN = f_sunin ? kN1 : kN0;
-        const float wF = (sunin2 && !keep) ? secondf - (float)kw : 0.f;
-        const float wN = f_sunin ? firstf - (float)kw : 0.f;
-        const float B = fmaf(kN, wN, kF * wF);
-        const float tl = fmaf(kN, f_lup, kF * lup);
-        const float ta = fmaf(kN, f_albsh, kF * albsh);
-        aB[0] += B; aL[0] += tl; aA[0] += ta;
+    __device__ __forceinline__ void add_to(int dir, float B, float tl, float ta) {
         const int az = 5 + 20 * dir;
-#pragma unroll
-        for (int g = 1; g < 5; ++g) {
-            const bool in = g == 1 ? az < 180 : (g == 2 ? (az >= 90 && az < 270) : (g == 3 ? az >= 180 : (az >= 270 || az < 90)));
-            if (in) { aB[g] += B; aL[g] += tl; aA[g] += ta; }
-        }
+        const int ew = az < 180 ? 0 : 2, sn = (az >= 90 && az < 270) ? 1 : 3;
+        hB[ew] += B; hL[ew] += tl; hA[ew] += ta;
+        hB[sn] += B; hL[sn] += tl; hA[sn] += ta;
     }
 
-    // the receiver's own terms (n x its Lup and albedo x shadow on non-building pixels) and the
-    // hour-independent unshadowed-albedo sums of the static planes (gvf_nosh)
+    // far field of one direction: lup and albsh are the sums over its L alive steps, kw the
+    // first sunlit-wall step of the walk (any value >= L when none lies among the alive steps)
+    __device__ __forceinline__ void add_far(int dir, int L, uint32_t lb, uint32_t kw, float lup, float albsh) {
+        const bool sunin2 = kw < (uint32_t)L;
+        const bool keep = kw == 0u && (lb & 0x80u) != 0u;
+        const float kF = sunin2 ? kF1 : kF0;
+        const float wF = (sunin2 && !keep) ? secondf - (float)kw : 0.f;
+        add_to(dir, kF * wF, kF * lup, kF * albsh);
+    }
+
+    // near field of one direction when first > 1: sums over its first min(L, first) steps
+    __device__ __forceinline__ void add_near(int dir, int L, uint32_t kw, int first, float f_lup, float f_albsh) {
+        const uint32_t Lf = (uint32_t)(L < first ? L : first);
+        const bool f_sunin = kw < Lf;
+        const float kN = f_sunin ? kN1 : kN0;
+        const float wN = f_sunin ? firstf - (float)kw : 0.f;
+        add_to(dir, kN * wN, kN * f_lup, kN * f_albsh);
+    }
+
+    // the receiver's own terms (n x its Lup and albedo x shadow on non-building pixels), the
+    // uniform near field when first == 1, and the hour-independent unshadowed-albedo sums of the
+    // static planes (gvf_nosh)
     __device__ __forceinline__ GvfOut finish(const DomainG &G, const PixelAt &at, const AS4 double *H,
-                                             const v2f me, float building) const {
+                                             const v2f me, float building, bool near_uniform) {
         const float lwall = (float)H[SWH_LWALL];
         const float albedo_b = (float)H[SWH_ALBEDO_B];
         const float local_lup = me.x * (1.0f - building);
         const float local_albsh = fabsf(me.y) * (1.0f - building);
-        GvfOut o;
-        Triple *grp[5] = {&o.all, &o.e, &o.s, &o.w, &o.n};
+        if (near_uniform) {
+            // receiver on the ground (L >= 1) and a sunlit wall (its record's sign bit): kw == 0
+            const bool f_sunin = building != 0.0f && (__float_as_uint(me.y) >> 31) != 0u;
+            const float kN = f_sunin ? kN1 : kN0;
+            const float nB = 9.0f * (f_sunin ? kN : 0.0f);
+            const float nL = 9.0f * (kN * (building * me.x));
+            const float nA = 9.0f * (kN * (building * fabsf(me.y)));
 #pragma unroll
-        for (int g = 0; g < 5; ++g) {
-            const float n = g == 0 ? 18.0f : 9.0f;
-            grp[g]->lup = fmaf(lwall, aB[g], aL[g]) + n * local_lup;
-            grp[g]->alb = fmaf(albedo_b, aB[g], aA[g]) + n * local_albsh;
+            for (int g = 0; g < 4; ++g) { hB[g] += nB; hL[g] += nL; hA[g] += nA; }
         }
+        GvfOut o;
+        Triple *grp[4] = {&o.e, &o.s, &o.w, &o.n};
+#pragma unroll
+        for (int g = 0; g < 4; ++g) {
+            grp[g]->lup = fmaf(lwall, hB[g], hL[g]) + 9.0f * local_lup;
+            grp[g]->alb = fmaf(albedo_b, hB[g], hA[g]) + 9.0f * local_albsh;
+        }
+        o.all.lup = fmaf(lwall, hB[0] + hB[2], hL[0] + hL[2]) + 18.0f * local_lup;
+        o.all.alb = fmaf(albedo_b, hB[0] + hB[2], hA[0] + hA[2]) + 18.0f * local_albsh;
         const AS1 float *ns = G.gvf_nosh;
         const long long P = G.pixels, p = at.p;
         o.all.nosh = fmaf(albedo_b, ns[5 * P + p], ns[p]);
@@ -718,13 +739,10 @@ __device__ __forceinline__ GvfOut ground_view(const DomainG &G, const PixelAt &a
         const int L = (int)(lb & 0x7fu);
         const float Lsteps = (float)L;
         float lup = 0.f, albsh = 0.f, f_lup = 0.f, f_albsh = 0.f;
-        int kw = 255;                                 // first step on a sunlit wall (255: none)
-        // near field: the sums over the first ``first`` steps (step 0 is the receiver itself)
-        if (first == 1) {
-            const float on = __builtin_amdgcn_fmed3f(Lsteps, 0.0f, 1.0f);
-            f_lup = on * me.x;
-            f_albsh = on * fabsf(me.y);
-        } else {
+        uint32_t kw = 255u;                           // first step on a sunlit wall (255: none)
+        // near field when first > 1: the sums over the first ``first`` steps (step 0 is the
+        // receiver itself); with first == 1 GvSums::finish adds it
+        if (first > 1) {
             long long qn = at.p;
             for (int st = 0; st < first; ++st) {
                 const int rr = at.r + off[2 * st], cc = at.c + off[2 * st + 1];
@@ -773,11 +791,12 @@ __device__ __forceinline__ GvfOut ground_view(const DomainG &G, const PixelAt &a
                 // shift the record's sign bit in from the right: bit 7 - i is step s0 + i
                 cm = __builtin_amdgcn_alignbit(cm, __float_as_uint(buf[i].y), 31);
             }
-            if (kw == 255 && cm != 0u) kw = s0 + __builtin_clz(cm) - (32 - CHUNK);
+            if (kw == 255u && cm != 0u) kw = (uint32_t)(s0 + __builtin_clz(cm) - (32 - CHUNK));
         }
-        gv.add(dir, L, lb, kw, first, f_lup, f_albsh, lup, albsh);
+        if (first > 1) gv.add_near(dir, L, kw, first, f_lup, f_albsh);
+        gv.add_far(dir, L, lb, kw, lup, albsh);
     }
-    return gv.finish(G, at, H, me, building);
+    return gv.finish(G, at, H, me, building, first == 1);
 }
 
 // The ground-view shifts of the 18 directions for the first GV_FIXED_STEPS steps
@@ -809,11 +828,10 @@ __device__ __forceinline__ GvfOut ground_view_fixed(const DomainG &G, const Pixe
         // integer subtract, sign-extend and convert sequences
         float Lsteps = (float)L;
         asm volatile("" : "+v"(Lsteps));
-        // near field: step 0, the receiver itself
-        const float on0 = __builtin_amdgcn_fmed3f(Lsteps, 0.0f, 1.0f);
-        const float f_lup = on0 * me.x, f_albsh = on0 * fabsf(me.y);
         float lup = 0.f, albsh = 0.f;
-        int kw = 255;
+        // the sign bits (sunlit walls) of every executed step, step 0 highest; nexec steps
+        uint32_t cm = 0u;
+        int nexec = 0;
 #pragma unroll
         for (int s0 = 0; s0 < NS; s0 += CHUNK) {
             if (__ballot(s0 < L) == 0) break;
@@ -825,7 +843,6 @@ __device__ __forceinline__ GvfOut ground_view_fixed(const DomainG &G, const Pixe
                 buf[i] = gv_lds[base + k];
             }
             const float rem = Lsteps - (float)s0;
-            uint32_t cm = 0u;
 #pragma unroll
             for (int i = 0; i < n; ++i) {
                 const float on = __builtin_amdgcn_fmed3f(rem - (float)i, 0.0f, 1.0f);
@@ -833,16 +850,19 @@ __device__ __forceinline__ GvfOut ground_view_fixed(const DomainG &G, const Pixe
                 albsh = fmaf(on, fabsf(buf[i].y), albsh);
                 cm = __builtin_amdgcn_alignbit(cm, __float_as_uint(buf[i].y), 31);
             }
-            if (kw == 255 && cm != 0u) kw = s0 + __builtin_clz(cm) - (32 - n);
+            nexec = s0 + n;
         }
-        gv.add(dir, L, lb, kw, 1, f_lup, f_albsh, lup, albsh);
-        // the group sums are complete here: without this the compiler keeps every direction's
+        // first sunlit-wall step; with none the unsigned difference wraps far above L
+        const uint32_t lead = cm == 0u ? 0xffffffffu : (uint32_t)__builtin_clz(cm);
+        const uint32_t kw = lead - (uint32_t)(32 - nexec);
+        gv.add_far(dir, L, lb, kw, lup, albsh);
+        // the half sums are complete here: without this the compiler keeps every direction's
         // partial terms live to the end of the walk (150 VGPRs instead of about 80)
-        asm volatile("" : "+v"(gv.aB[0]), "+v"(gv.aB[1]), "+v"(gv.aB[2]), "+v"(gv.aB[3]), "+v"(gv.aB[4]),
-                     "+v"(gv.aL[0]), "+v"(gv.aL[1]), "+v"(gv.aL[2]), "+v"(gv.aL[3]), "+v"(gv.aL[4]),
-                     "+v"(gv.aA[0]), "+v"(gv.aA[1]), "+v"(gv.aA[2]), "+v"(gv.aA[3]), "+v"(gv.aA[4]));
+        asm volatile("" : "+v"(gv.hB[0]), "+v"(gv.hB[1]), "+v"(gv.hB[2]), "+v"(gv.hB[3]),
+                     "+v"(gv.hL[0]), "+v"(gv.hL[1]), "+v"(gv.hL[2]), "+v"(gv.hL[3]),
+                     "+v"(gv.hA[0]), "+v"(gv.hA[1]), "+v"(gv.hA[2]), "+v"(gv.hA[3]));
     }
-    return gv.finish(G, at, H, me, building);
+    return gv.finish(G, at, H, me, building, true);
 }
 
 // The 16-byte view of a pixel's hour record used by the adjoint kernels: Lup field, albedo,
