@@ -31,8 +31,8 @@ CONSTANTS = dict(albedo_b=0.2, albedo_g=0.15, ewall=0.9, eground=0.95, absK=0.7,
 
 # ----------------------------------------------------------------------------- sky vault
 
-def patches():
-    """153 patch centres (altitude, azimuth) in degrees (create_patches option 2)."""
+@lru_cache(maxsize=1)
+def _patch_centres():
     alt, azi = [], []
     for n, a, s in zip(BAND_COUNTS, BAND_ALTITUDES, BAND_STARTS):
         step = 360 / n
@@ -40,6 +40,12 @@ def patches():
             alt.append(float(a))
             azi.append(k * step + s)
     return np.array(alt), np.array(azi)
+
+
+def patches():
+    """153 patch centres (altitude, azimuth) in degrees (create_patches option 2)."""
+    alt, azi = _patch_centres()
+    return alt.copy(), azi.copy()
 
 
 def steradians(alt):
@@ -349,20 +355,22 @@ def _geometry(round_trip):
         np.where((azi > 270) | (azi < 90), np.cos((0 - azi) * DEG), 0.0),
         np.where((azi > 90) & (azi < 270), np.cos((180 - azi) * DEG), 0.0)])
     thr = np.array([sunlit_thresholds(a) for a in alt])
-    return alt, azi, ster, wcos, wsin, dirs, thr
+    # per patch sin(altitude) evaluated one value at a time, as the luminance normalisation of
+    # Kside_veg_v2022a forms it (umep_solweig.py:689-691), and the band of every patch
+    sin_alt = [float(np.sin(a * DEG)) for a in alt]
+    levels = np.unique(alt)
+    band = np.searchsorted(levels, alt)
+    return alt, azi, ster, wcos, wsin, dirs, thr, sin_alt, levels, band
 
 
 def patch_rows(day, esky, Ta, altitude, azimuth, lv=None, radD=0.0, f64=False):
     """One hour's patch table: float [153, SW_PATCH_F] and double [153, SW_PATCH_D]
     (``f64``: the float table left in FP64, for derivatives)."""
-    alt, azi, ster, wcos, wsin, dirs, thr = _geometry(bool(day))
+    alt, azi, ster, wcos, wsin, dirs, thr, sin_alt, levels, band = _geometry(bool(day))
     f = np.zeros((SW_PATCHES, SW_PATCH_F))
     d = np.zeros((SW_PATCHES, SW_PATCH_D))
-    levels = np.unique(alt)
     band_emis = 1 - (1 - esky) * np.exp(0.308 * (1.7 - (1 / np.cos((90 - levels) * DEG))))
-    e = np.empty(SW_PATCHES)
-    for lvl, be in zip(levels, band_emis):
-        e[alt == lvl] = be
+    e = band_emis[band]
     flux = (e * SBC * ((Ta + 273.15) ** 4)) / np.pi
     f[:, PF['LDN']] = flux * ster * np.sin(alt * DEG)
     f[:, PF['LSD']] = flux * ster * np.cos(alt * DEG)
@@ -371,10 +379,11 @@ def patch_rows(day, esky, Ta, altitude, azimuth, lv=None, radD=0.0, f64=False):
     f[:, PF['DE']], f[:, PF['DW']], f[:, PF['DN']], f[:, PF['DS']] = dirs.T
     d[:, PD['SUN_BELOW']], d[:, PD['SHADE_FROM']] = thr[:, 0], thr[:, 1]
     if day:
-        rad_tot = np.zeros(1)
-        for k in range(SW_PATCHES):
-            rad_tot += lv[k] * ster[k] * np.sin(alt[k] * DEG)
-        lum_chi = (lv * radD) / rad_tot
+        # sequential FP64 sum in patch order (Python floats round as NumPy float64 scalars)
+        rad_tot = 0.0
+        for lvk, sk, snk in zip(lv.tolist(), ster.tolist(), sin_alt):
+            rad_tot += lvk * sk * snk
+        lum_chi = (lv * radD) / np.array([rad_tot])
         f[:, PF['LV']] = lv
         f[:, PF['KD']] = lum_chi * (np.cos(alt * DEG) * np.cos(0)) * ster
         diff = np.abs(azimuth - azi)

@@ -33,7 +33,7 @@ def normalise_landcover(values):
 def sky_sum_table():
     """Per patch (w sin, w cos, g w sin, g w cos) of the day patch geometry, w the steradian and
     g = exp(0.308 (1.7 - 1/cos(zenith))) the Martin-Berdahl band factor (umep_solweig.py:526-527)."""
-    alt, _, ster, wcos, wsin, _, _ = _geometry(True)
+    alt, _, ster, wcos, wsin = _geometry(True)[:5]
     g = np.exp(0.308 * (1.7 - (1 / np.cos((90 - alt) * DEG))))
     return np.ascontiguousarray(np.column_stack((wsin, wcos, g * wsin, g * wcos)), dtype=np.float32)
 
