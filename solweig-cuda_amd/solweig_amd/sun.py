@@ -139,48 +139,151 @@ def spa(year, month, day, hour, minute, second, utc, latitude, longitude, altitu
     return float(zenith), float(_fold(azimuth, 360))
 
 
+def _fold_v(value, upper):
+    value = value - upper * np.floor(value / upper)
+    return np.where(value < 0, value + upper, value)
+
+
+def _series_v(groups, jme):
+    """_series for a vector of instants: per instant the same element-wise terms and the same
+    contiguous (pairwise) sum of each group."""
+    parts = []
+    for k, (a, b, c) in enumerate(groups):
+        term = a[None, :] * np.cos(b[None, :] + c[None, :] * jme[:, None])
+        parts.append(np.sum(term, axis=1) if k < len(groups) - 1 or len(a) > 1 else term[:, 0])
+    total = parts[0]
+    for k in range(1, len(parts)):
+        total = total + parts[k] * (jme if k == 1 else np.power(jme, k))
+    return total / 1e8
+
+
+def spa_batch(stamps, utc, latitude, longitude, altitude_m):
+    """``spa`` for a list of datetime instants at one site: (zenith, azimuth) arrays in degrees.
+    The arithmetic is spa's, element by element (the nutation arguments per instant)."""
+    n = len(stamps)
+    jd = np.empty(n)
+    for i, st in enumerate(stamps):
+        jd[i] = _julian(st.year, st.month, st.day, st.hour, st.minute, 0, utc)[0]
+    jc = (jd - 2451545) / 36525
+    jce, jme = jc, jc / 10
+    lon = _fold_v(_series_v(_L, jme) * 180 / np.pi, 360)
+    a0, b0, c0 = _B[0]
+    a1, b1, c1 = _B[1]
+    lat = (np.sum(a0[None, :] * np.cos(b0[None, :] + c0[None, :] * jme[:, None]), axis=1)
+           + np.sum(a1[None, :] * np.cos(b1[None, :] + c1[None, :] * jme[:, None]), axis=1) * jme) / 1e8
+    lat = _fold_v(lat * 180 / np.pi, 360)
+    radius = _series_v(_R, jme)
+    geo_lon = _fold_v(lon + 180, 360)
+    geo_lat = _fold_v(-lat, 360)
+    x0 = (1 / 189474) * np.power(jce, 3) + -0.0019142 * np.power(jce, 2) + 445267.11148 * jce + 297.85036
+    x1 = -(1 / 300000) * np.power(jce, 3) + -0.0001603 * np.power(jce, 2) + 35999.05034 * jce + 357.52772
+    x2 = (1 / 56250) * np.power(jce, 3) + 0.0086972 * np.power(jce, 2) + 477198.867398 * jce + 134.96298
+    x3 = (1 / 327270) * np.power(jce, 3) + -0.0036825 * np.power(jce, 2) + 483202.017538 * jce + 93.27191
+    x4 = (1 / 450000) * np.power(jce, 3) + 0.0020708 * np.power(jce, 2) + -1934.136261 * jce + 125.04452
+    X = np.array([x0, x1, x2, x3, x4])
+    dpsi = np.empty(n)
+    deps = np.empty(n)
+    for i in range(n):
+        arg = _NUT_ARGS.dot(np.ascontiguousarray(X[:, i])) * (np.pi / 180)
+        dpsi[i] = np.sum((_NUT[:, 0] + _NUT[:, 1] * jce[i]) * np.sin(arg)) / 36000000
+        deps[i] = np.sum((_NUT[:, 2] + _NUT[:, 3] * jce[i]) * np.cos(arg)) / 36000000
+    u = jme / 10
+    poly = (2.45, 5.79, 27.87, 7.12, -39.05, -249.67, -51.38, 1999.25, -1.55, -4680.93, 84381.448)
+    eps0 = np.zeros(n)
+    for k, c in enumerate(poly[:-2]):
+        eps0 = eps0 + c * np.power(u, 10 - k)
+    eps0 = eps0 + poly[-2] * u + poly[-1]
+    eps = eps0 / 3600 + deps
+    aberration = -20.4898 / (3600 * radius)
+    lam = geo_lon + dpsi + aberration
+    nu0 = 280.46061837 + (360.98564736629 * (jd - 2451545)) + (0.000387933 * np.power(jc, 2)) \
+        - (np.power(jc, 3) / 38710000)
+    nu = _fold_v(nu0, 360) + dpsi * np.cos(eps * np.pi / 180)
+    alpha = np.arctan2(np.sin(lam * np.pi / 180) * np.cos(eps * np.pi / 180)
+                       - np.tan(geo_lat * np.pi / 180) * np.sin(eps * np.pi / 180),
+                       np.cos(lam * np.pi / 180)) * 180 / np.pi
+    alpha = _fold_v(alpha, 360)
+    delta = np.arcsin(np.sin(geo_lat * np.pi / 180) * np.cos(eps * np.pi / 180)
+                      + np.cos(geo_lat * np.pi / 180) * np.sin(eps * np.pi / 180)
+                      * np.sin(lam * np.pi / 180)) * 180 / np.pi
+    h = _fold_v(nu + longitude - alpha, 360)
+    xi = 8.794 / (3600 * radius)
+    uu = np.arctan(0.99664719 * np.tan(latitude * np.pi / 180))
+    x = np.cos(uu) + (altitude_m / 6378140) * np.cos(latitude * np.pi / 180)
+    y = 0.99664719 * np.sin(uu) + (altitude_m / 6378140) * np.sin(latitude * np.pi / 180)
+    dalpha = np.arctan2(-x * np.sin(xi * np.pi / 180) * np.sin(h * np.pi / 180),
+                        np.cos(delta * np.pi / 180) - x * np.sin(xi * np.pi / 180) * np.cos(h * np.pi / 180))
+    delta_t = np.arctan2((np.sin(delta * np.pi / 180) - y * np.sin(xi * np.pi / 180)) * np.cos(dalpha),
+                         np.cos(delta * np.pi / 180) - y * np.sin(xi * np.pi / 180)
+                         * np.cos(h * np.pi / 180)) * 180 / np.pi
+    h_t = h - dalpha * 180 / np.pi
+    e0 = np.arcsin(np.sin(latitude * np.pi / 180) * np.sin(delta_t * np.pi / 180)
+                   + np.cos(latitude * np.pi / 180) * np.cos(delta_t * np.pi / 180)
+                   * np.cos(h_t * np.pi / 180)) * 180 / np.pi
+    refraction = 1.02 / (60 * np.tan((e0 + 10.3 / (e0 + 5.11)) * np.pi / 180))
+    zenith = 90 - (e0 + refraction)
+    azimuth = np.arctan2(np.sin(h_t * np.pi / 180),
+                         np.cos(h_t * np.pi / 180) * np.sin(latitude * np.pi / 180)
+                         - np.tan(delta_t * np.pi / 180) * np.cos(latitude * np.pi / 180)) * 180 / np.pi + 180
+    return zenith, _fold_v(azimuth, 360)
+
+
 def _instant(stamp, utc, location):
     return spa(stamp.year, stamp.month, stamp.day, stamp.hour, stamp.minute, 0, utc,
                float(location['latitude']), float(location['longitude']), float(location['altitude']))
 
 
+def _altmax(ymd, utc, location):
+    """The day's maximum altitude by the 15-minute search from 10:00 local: the altitude of the
+    last sample before it stops increasing (umep_solweig.py:2454-2469), evaluated in blocks."""
+    lat, lon, alt = float(location['latitude']), float(location['longitude']), float(location['altitude'])
+    fifteen, sunmaximum, zenith = 0., -90., 90.
+    while True:
+        stamps = []
+        f = fifteen
+        for _ in range(48):
+            f = f + 15. / 1440.
+            stamps.append(ymd + datetime.timedelta(days=(60 * 10) / 1440.0 + f))
+        zens, _ = spa_batch(stamps, utc, lat, lon, alt)
+        for z in zens:
+            if not sunmaximum <= 90. - zenith:
+                return sunmaximum
+            sunmaximum = 90. - zenith
+            fifteen = fifteen + 15. / 1440.
+            zenith = float(z)
+        if not sunmaximum <= 90. - zenith:
+            return sunmaximum
+
+
 def met_sun(met, location, utc):
     """Per-row (altitude deg, azimuth deg, zen rad, jday, dectime, altmax) for a forcing table.
 
-    ``met`` is the [n, >=4] forcing table (year, doy, hour, minute, ...).
+    ``met`` is the [n, >=4] forcing table (year, doy, hour, minute, ...). The rows' instants are
+    evaluated together (``spa_batch``, the same values as ``spa`` one at a time).
     """
     met = np.asarray(met, dtype=np.float64)
     n = met.shape[0]
     dectime = met[:, 1] + met[:, 2] / 24 + met[:, 3] / (60 * 24.)
     half = 0 if n == 1 else (dectime[1] - dectime[0]) / 2.
-    altitude = np.empty(n)
-    azimuth = np.empty(n)
-    zen = np.empty(n)
-    jday = np.empty(n)
     altmax = np.empty(n)
+    jday = np.empty(n)
+    stamps = []
     sunmaximum = 0.
     for i in range(n):
         ymd = datetime.datetime(int(met[i, 0]), 1, 1) + datetime.timedelta(int(met[i, 1]) - 1)
         if i == 0 or np.mod(dectime[i], np.floor(dectime[i])) == 0:
-            fifteen = 0.
-            sunmaximum = -90.
-            zenith = 90.
-            while sunmaximum <= 90. - zenith:
-                sunmaximum = 90. - zenith
-                fifteen = fifteen + 15. / 1440.
-                stamp = ymd + datetime.timedelta(days=(60 * 10) / 1440.0 + fifteen)
-                zenith, _ = _instant(stamp, utc, location)
+            sunmaximum = _altmax(ymd, utc, location)
         altmax[i] = sunmaximum
-        stamp = (ymd + datetime.timedelta(hours=met[i, 2]) + datetime.timedelta(minutes=met[i, 3])
-                 - datetime.timedelta(days=half))
-        zenith, azi = _instant(stamp, utc, location)
-        if 89.0 < zenith <= 90.0:
-            zenith = 89.0
-        altitude[i] = 90. - zenith
-        azimuth[i] = azi
-        zen[i] = zenith * (np.pi / 180.)
+        stamps.append(ymd + datetime.timedelta(hours=met[i, 2]) + datetime.timedelta(minutes=met[i, 3])
+                      - datetime.timedelta(days=half))
         jday[i] = ymd.timetuple().tm_yday
-    return dict(altitude=altitude, azimuth=azimuth, zen=zen, jday=jday, dectime=dectime, altmax=altmax)
+    zenith, azimuth = spa_batch(stamps, utc, float(location['latitude']), float(location['longitude']),
+                                float(location['altitude']))
+    zenith = np.where((zenith > 89.0) & (zenith <= 90.0), 89.0, zenith)
+    altitude = 90. - zenith
+    zen = zenith * (np.pi / 180.)
+    return dict(altitude=altitude, azimuth=np.asarray(azimuth, dtype=np.float64), zen=zen, jday=jday,
+                dectime=dectime, altmax=altmax)
 
 
 def is_leap(year):
