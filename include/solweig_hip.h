@@ -105,6 +105,11 @@ typedef struct sw_domain {
                                 static, sw_gvf_static)                                         */
     const float *gvf_nosh;   /* [10][P] static albedo sums of the ground view: a (all, E, S, W,
                                 N) then b; gvfalbnosh sum = a + albedo_b * b                    */
+    const float *march_trees; /* [P] or NULL: the canopy heights the sun-ray march reads, trees
+                                 where (double)trees + a != a (a = dsm + dsm_raise) and -inf where
+                                 the sum equals a (no canopy top). Only valid when no pixel has
+                                 0.25 * trees + a == a != trees + a; NULL: the march forms the
+                                 canopy tops from trees (series.py:252-287)                   */
 } sw_domain;
 
 /* Static per-pixel sums over the patches of each visibility class (day patch geometry), the

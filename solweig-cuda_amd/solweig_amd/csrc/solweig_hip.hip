@@ -22,7 +22,7 @@
 
 #include "../../../include/solweig_hip.h"
 
-static_assert(sizeof(sw_domain) == 296, "sw_domain must stay 37 eight-byte fields");
+static_assert(sizeof(sw_domain) == 304, "sw_domain must stay 37 eight-byte fields");
 
 namespace {
 
@@ -100,6 +100,7 @@ struct DomainG {
     int hmax_cols;
     const AS1 uint32_t *gvf_len;
     const AS1 float *gvf_nosh;
+    const AS1 float *march_trees;
 };
 
 __device__ __forceinline__ DomainG domain_at(const sw_domain *D) {
@@ -125,6 +126,7 @@ __device__ __forceinline__ DomainG domain_at(const sw_domain *D) {
     g.full = (AS1 float *)d->full; g.skysums = (const AS1 float *)d->skysums;
     g.hmax = (const AS4 float *)d->hmax; g.hmax_cols = (int)d->hmax_cols;
     g.gvf_len = (const AS1 uint32_t *)d->gvf_len; g.gvf_nosh = (const AS1 float *)d->gvf_nosh;
+    g.march_trees = (const AS1 float *)d->march_trees;
     return g;
 }
 
@@ -323,17 +325,18 @@ struct March {
     bool veg, ever;
 };
 
-constexpr int SKIP_WINDOW = 32;   // steps per test of the coarse height bound (multiple of CHUNK)
+constexpr int SKIP_WINDOW = 32;   // steps one test of the coarse height bound may skip
+constexpr int SKIP_TEST = 16;     // steps between tests (multiple of CHUNK)
 constexpr int HCELL_SHIFT = 4;    // 16 x 16 cells of the coarse height bound
 
-// Minimum over the wave's lanes (every lane calls), returned as a uniform value.
-__device__ __forceinline__ double wave_min_uniform(double v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o, 64));
-    const uint64_t b = (uint64_t)__double_as_longlong(v);
-    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)b);
-    const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(b >> 32));
-    return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+extern "C" __device__ uint32_t __ockl_wfred_min_u32(uint32_t);
+
+// A lower bound of the minimum over the wave's lanes of v >= 0 (every lane calls), as a uniform
+// value: each v rounded down to FP32, whose bit patterns order as the values do, reduced with the
+// DPP wave minimum. +inf stays +inf.
+__device__ __forceinline__ double wave_min_below(double v) {
+    const uint32_t b = __ockl_wfred_min_u32(__float_as_uint(__double2float_rd(v)));
+    return (double)__uint_as_float(__builtin_amdgcn_readfirstlane(b));
 }
 
 // True when every sample of steps [k0, ke] of every lane of the wave is at most a_min above
@@ -358,8 +361,11 @@ __device__ __forceinline__ bool window_below(const DomainG &G, const PixelAt &at
 
 // CHUNK steps [k0, k0 + CHUNK) of the sun-ray march of one pixel. MASKED: per-lane bounds
 // tests (a step outside the raster is made a no-op by an infinite dz) and only steps < L count;
-// otherwise all CHUNK steps are inside the raster for every lane and below L.
-template <bool MASKED>
+// otherwise all CHUNK steps are inside the raster for every lane and below L. TOPS: ``trees`` is
+// the march plane, -inf where a sample has no canopy top: top = t + a and bot = 0.25 t + a are
+// then -inf there, which leaves topmax (>= 0), the pergola test and the receiver threshold
+// (>= 0, against dz >= 0) exactly as the zeroed tops of the reference do.
+template <bool MASKED, bool TOPS>
 __device__ __forceinline__ void march_chunk(March &m, const AS1 float *dsm, const AS1 float *trees,
                                             double raise, double me_a, const PixelAt &at, int rows,
                                             int cols, const AS4 int32_t *off, const AS4 double *dzs,
@@ -391,9 +397,9 @@ __device__ __forceinline__ void march_chunk(March &m, const AS1 float *dsm, cons
         const double a = (double)av[i] + raise;
         const double t = (double)tv[i];
         const double v1 = t + a;
-        const double top = (v1 == a) ? 0.0 : v1;
         const double v2 = __builtin_fma(t, 0.25, a);   // t * 0.25 is exact: one rounding, as NumPy
-        const double bot = (v2 == a) ? 0.0 : v2;
+        const double top = TOPS ? v1 : ((v1 == a) ? 0.0 : v1);
+        const double bot = TOPS ? v2 : ((v2 == a) ? 0.0 : v2);
         m.f = fmax(m.f, a - dz);
         m.topmax = fmax(m.topmax, top - dz);
         const bool vstep = (top - dzp > me_a) && !(bot - dz > me_a);
@@ -426,6 +432,8 @@ __global__ __launch_bounds__(BLOCK) void k_hour_shadow(const sw_domain *__restri
     const AS1 float *dsm = G.dsm, *trees = G.trees;
     const double raise = G.dsm_raise;
     const Heights me = heights_at(dsm, trees, raise, p);
+    const bool tops = G.march_trees != nullptr;       // uniform: one domain per workgroup
+    const AS1 float *mtrees = tops ? G.march_trees : trees;
     const bool wall_px = valid && G.walls[p] > 0.0f;
     const int start = uniform_index(H[SWH_RAY_START]);
     const int L = uniform_index(H[SWH_RAY_LEN]);
@@ -446,7 +454,7 @@ __global__ __launch_bounds__(BLOCK) void k_hour_shadow(const sw_domain *__restri
     const bool all_lanes = true;
     const uint64_t active = __ballot(all_lanes);
     for (int k0 = 0; k0 < L; k0 += CHUNK) {
-        if (k0 > 0 && (k0 & (SKIP_WINDOW - 1)) == 0) {
+        if (k0 > 0 && (k0 & (SKIP_TEST - 1)) == 0) {
             // A lane's remaining steps matter only through samples above its threshold: while
             // not building-shadowed, its own height (f, the canopy test and topmax change only
             // through samples above it); once shadowed, veg is false for good (so ever and
@@ -455,7 +463,7 @@ __global__ __launch_bounds__(BLOCK) void k_hour_shadow(const sw_domain *__restri
             // when no lane has a threshold left, the march is over.
             const double thr = !valid ? __builtin_huge_val()
                                       : (m.f > me.a ? (wall_px ? m.f : __builtin_huge_val()) : me.a);
-            const double thr_min = wave_min_uniform(thr);
+            const double thr_min = wave_min_below(thr);
             if (thr_min == __builtin_huge_val()) break;
             const int ke = (k0 + SKIP_WINDOW < L ? k0 + SKIP_WINDOW : L) - 1;
             if (window_below(G, at, off, k0, ke, rows, cols, dzs[k0 - 1], thr_min)) {
@@ -470,10 +478,15 @@ __global__ __launch_bounds__(BLOCK) void k_hour_shadow(const sw_domain *__restri
         const bool in_first = valid && (unsigned)(at.r + off[2 * k0]) < (unsigned)rows &&
                               (unsigned)(at.c + off[2 * k0 + 1]) < (unsigned)cols;
         if (__ballot(in_first) == 0) break;          // every lane's ray has left the raster
-        if (k0 + CHUNK <= L && __ballot(in_last) == active)
-            march_chunk<false>(m, dsm, trees, raise, me.a, at, rows, cols, off, dzs, k0, L);
+        const bool full = k0 + CHUNK <= L && __ballot(in_last) == active;
+        if (full && tops)
+            march_chunk<false, true>(m, dsm, mtrees, raise, me.a, at, rows, cols, off, dzs, k0, L);
+        else if (full)
+            march_chunk<false, false>(m, dsm, mtrees, raise, me.a, at, rows, cols, off, dzs, k0, L);
+        else if (tops)
+            march_chunk<true, true>(m, dsm, mtrees, raise, me.a, at, rows, cols, off, dzs, k0, L);
         else
-            march_chunk<true>(m, dsm, trees, raise, me.a, at, rows, cols, off, dzs, k0, L);
+            march_chunk<true, false>(m, dsm, mtrees, raise, me.a, at, rows, cols, off, dzs, k0, L);
     }
     if (!valid) return;
     const double f = m.f, topmax = m.topmax;

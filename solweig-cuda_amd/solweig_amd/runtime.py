@@ -53,7 +53,7 @@ class SwDomain(ctypes.Structure):
                                         'rec', 'lag', 'tmrt', 'utci', 'hours', 'ptab_f', 'ptab_d',
                                         'ray_off', 'ray_dz', 'masks', 'shadow_planes', 'full', 'skysums',
                                         'hmax')] + [('hmax_cols', ctypes.c_int64)] + \
-        [(n, ctypes.c_uint64) for n in ('gvf_len', 'gvf_nosh')]
+        [(n, ctypes.c_uint64) for n in ('gvf_len', 'gvf_nosh', 'march_trees')]
 HMAX_CELL = 16
 
 

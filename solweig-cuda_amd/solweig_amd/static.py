@@ -47,6 +47,21 @@ def sky_sums(vis, P, stream=None):
     return out
 
 
+def march_trees(a, trees):
+    """The canopy plane of the sun-ray march (sw_domain.march_trees): trees where the FP64
+    canopy top trees + a differs from a, -inf where it does not (series.py:282-287 zeroes those
+    tops), or None when some pixel has 0.25 * trees + a == a but trees + a != a (then the march
+    forms the tops itself). ``a`` is the FP64 DSM raised to >= 0."""
+    a = np.asarray(a, dtype=np.float64)
+    t32 = np.asarray(trees, dtype=np.float32)
+    t = t32.astype(np.float64)
+    no_top = (t + a) == a
+    no_bot = (t * 0.25 + a) == a
+    if np.any(no_bot & ~no_top):
+        return None
+    return np.where(no_top, np.float32(-np.inf), t32).astype(np.float32)
+
+
 def height_bound(a, trees):
     """Per 16 x 16 cell, an FP32 upper bound of max(a, canopy top) with a the FP64 DSM (raised
     to >= 0) and top = trees + a (series.py:282-287): no sample of the sun-ray march in the cell
@@ -144,6 +159,8 @@ class PreparedDomain:
         self.dev['lag'] = rt.DeviceArray((6, self.P), np.float32)
         self.dev['skysums'] = sky_sums(self.dev['vis'], self.P, stream)
         self.dev['hmax'] = up(height_bound(a, trees))
+        mt = march_trees(a, trees)
+        self.dev['march_trees'] = None if mt is None else up(mt)
         self._check_albedo(maps['alb'])
         ground_view_static(self, stream)
         if keep_host:
@@ -198,6 +215,7 @@ class PreparedDomain:
                            ('rec', 'rec'), ('lag', 'lag'), ('skysums', 'skysums'), ('hmax', 'hmax'),
                            ('masks', 'masks'), ('gvf_len', 'gvf_len'), ('gvf_nosh', 'gvf_nosh')):
             setattr(s, field, d[key].ptr)
+        s.march_trees = d['march_trees'].ptr if d.get('march_trees') is not None else 0
         s.hmax_cols = -(-self.shape[1] // rt.HMAX_CELL)
         for k, v in extra.items():
             setattr(s, k, v or 0)
@@ -273,6 +291,8 @@ def prepared_from_arrays(*, dsm, vegdem, walls, dirwalls, buildings, svfbuveg, a
                     lag=rt.DeviceArray((6, rows * cols), np.float32))
     self.dev['skysums'] = sky_sums(self.dev['vis'], rows * cols, stream)
     self.dev['hmax'] = up(height_bound(a, trees32))
+    mt = march_trees(a, trees32)
+    self.dev['march_trees'] = None if mt is None else up(mt)
     PreparedDomain._check_albedo(plane(alb_grid))
     ground_view_static(self, stream)
     self.skyview = None

@@ -73,6 +73,45 @@ def test_shadow_masks_bit_exact_every_hour(rt):
     assert checked >= 10
 
 
+@pytest.mark.parametrize('plant', [False, True])
+def test_shadow_masks_bit_exact_with_and_without_march_plane(rt, plant):
+    """The march reads the canopy plane with -inf for absent tops; a planted tree of 2e-15 m on
+    a 16 m roof (its quarter vanishes against the roof, itself does not) makes the static prep
+    fall back to forming the tops in the march. Both give the oracle's masks bit for bit."""
+    from solweig_amd import batched, sun as psun
+    d = _domain(40, 23)
+    if plant:
+        d.dsm = d.dsm.copy()
+        d.trees = d.trees.copy()
+        d.dsm[5, 5] = np.float32(16.0)
+        d.trees[5, 5] = np.float32(2e-15)
+    s = psun.met_sun(d.met, d.location, d.utc)
+    eng = batched.Engine([d], n_hours=24)
+    assert (eng.domains[0].dev['march_trees'] is None) == plant
+    masks = rt.DeviceArray((eng.domains[0].P,), np.uint8)
+    eng.structs[0].masks = masks.ptr
+    rt.call('sw_memcpy_h2d', eng.dom_dev.ptr, __import__('ctypes').addressof(eng.structs),
+            __import__('ctypes').sizeof(eng.structs), eng.stream.handle)
+    eng.load(eng.tables(sun=[s]))
+    eng.reset_state()
+    p = oday.prepare(d.dsm, d.trees, d.dem, d.walls, d.aspect, d.latitude)
+    checked = 0
+    for h in range(24):
+        if s['altitude'][h] <= 0 or h % 3:
+            continue
+        print(f'march plane {not plant}: hour {h} alt {s["altitude"][h]:.2f}', flush=True)
+        eng.hour(h)
+        m = masks.to_host(eng.stream).reshape(d.shape)
+        vegsh, sh, vb, *_ = umep.shadows(
+            p['dsm'], p['vegdem'], p['vegdem2'], s['azimuth'][h], s['altitude'][h], 1.0, p['amaxvalue'],
+            p['bush'], p['walls'], p['dirwalls'] * np.pi / 180.)
+        np.testing.assert_array_equal((m & 1) != 0, sh == 1)
+        np.testing.assert_array_equal((m & 2) != 0, vegsh == 1)
+        np.testing.assert_array_equal((m & 4) != 0, vb == 1)
+        checked += 1
+    assert checked >= 3
+
+
 def _day_errors(tmrt, utci, ref):
     et = max(float(np.max(np.abs(tmrt[h] - ref[h]['Tmrt']))) for h in range(len(ref)))
     g = [np.isfinite(ref[h]['UTCI']) for h in range(len(ref))]

@@ -75,3 +75,16 @@ def test_day_tables_follow_the_oracle_scalars():
         if s['altitude'][i] > 0:
             assert m['radI'] == out['radI'] and m['radD'] == out['radD']
             assert m['tgwall'] == out['Tgwall']
+
+
+def test_march_canopy_plane():
+    """-inf exactly where the FP64 canopy top equals the DSM; None when a quarter canopy vanishes
+    against the DSM while the canopy itself does not."""
+    from solweig_amd.static import march_trees
+    a = np.array([[0.0, 10.0, 16.0, 3.5]])
+    t = np.array([[0.0, 4.0, 0.0, 1e-30]], dtype=np.float32)
+    m = march_trees(a, t)
+    assert m.dtype == np.float32
+    assert np.isneginf(m[0, 0]) and m[0, 1] == np.float32(4.0) and np.isneginf(m[0, 2]) and np.isneginf(m[0, 3])
+    t2 = np.array([[0.0, 4.0, 2e-15, 0.0]], dtype=np.float32)
+    assert march_trees(a, t2) is None
