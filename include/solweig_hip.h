@@ -62,6 +62,15 @@ enum {
 enum { SWP_YI = 0, SWP_SUN_BELOW, SWP_SHADE_FROM, SW_PATCH_D = 4 };
 enum { SW_PATCHES = 153, SW_VIS_WORDS = 5 };
 
+/* Per hour row, the 32-bit words of the patch search tables (sw_domain.ptab_s):
+ *   SWS_NIB   float [39][16][2]: over the 4 patches 4i..4i+3, the sums of (luminance,
+ *             lumChi cos omega) of the patches whose bit is set in the 4-bit index;
+ *   SWS_YS    double [8][32]: per altitude band its patches' yi in ascending order, +inf after;
+ *   SWS_MASK  uint32 [8][33]: per band, the band-local bits (bit i: patch BAND_START + i) of the
+ *             j smallest-yi patches, j = 0..32;
+ *   SWS_ELIG  uint32 [8]: per band, the band-local sun-test eligibility bits.              */
+enum { SWS_NIB = 0, SWS_YS = 1248, SWS_MASK = 1760, SWS_ELIG = 2024, SW_SEARCH_WORDS = 2048 };
+
 /* FULL-mode output planes of sw_hour_pixels (the per-step 39-tuple's per-pixel members) */
 enum {
     SWF_KDOWN = 0, SWF_KUP, SWF_LDOWN, SWF_LUP, SWF_TG, SWF_KEAST, SWF_KSOUTH, SWF_KWEST,
@@ -110,6 +119,8 @@ typedef struct sw_domain {
                                  the sum equals a (no canopy top). Only valid when no pixel has
                                  0.25 * trees + a == a != trees + a; NULL: the march forms the
                                  canopy tops from trees (series.py:252-287)                   */
+    const uint32_t *ptab_s;   /* [rows][SW_SEARCH_WORDS] patch search tables, or NULL (the
+                                 pixel pass then tests every patch)                            */
 } sw_domain;
 
 /* Static per-pixel sums over the patches of each visibility class (day patch geometry), the
@@ -124,6 +135,10 @@ enum {
 
 /* ----- runtime ------------------------------------------------------------------------ */
 int sw_version(void);
+/* sizeof(sw_domain) (kind 0) or sizeof(sw_adjoint) (kind 1) as this library was built, so that a
+ * binding can refuse a library whose records do not match its own; -1 for another kind.
+ * No counterpart in the reference (its records are CuPy arrays).                         */
+int sw_record_bytes(int kind);
 const char *sw_last_error(void);
 int sw_device_count(int *count);
 int sw_set_device(int device);

@@ -33,6 +33,7 @@ class Engine:
             b = dict(hours=rt.DeviceArray((self.n_hours, rt.SW_HOUR_STRIDE), np.float64),
                      ptab_f=rt.DeviceArray((self.n_hours, rt.SW_PATCHES, rt.SW_PATCH_F), np.float32),
                      ptab_d=rt.DeviceArray((self.n_hours, rt.SW_PATCH_D, rt.SW_PATCHES), np.float64),
+                     ptab_s=rt.DeviceArray((self.n_hours, rt.SW_SEARCH_WORDS), np.uint32),
                      ray_off=rt.DeviceArray((cap, 2), np.int32), ray_dz=rt.DeviceArray((cap,), np.float64),
                      cap=cap)
             if keep_outputs:
@@ -48,6 +49,7 @@ class Engine:
         self._stage = [[dict(hours=rt.PinnedArray(b['hours'].shape, np.float64),
                              ptab_f=rt.PinnedArray(b['ptab_f'].shape, np.float32),
                              ptab_d=rt.PinnedArray(b['ptab_d'].shape, np.float64),
+                             ptab_s=rt.PinnedArray(b['ptab_s'].shape, np.uint32),
                              ray_off=rt.PinnedArray((b['cap'], 2), np.int32),
                              ray_dz=rt.PinnedArray((b['cap'],), np.float64)) for b in self.buf]
                        for _ in range(2)]
@@ -64,7 +66,7 @@ class Engine:
         self.block_start.upload(starts, self.stream)
         for i, (d, b) in enumerate(zip(self.domains, self.buf)):
             self.structs[i] = d.struct(hours=b['hours'].ptr, ptab_f=b['ptab_f'].ptr, ptab_d=b['ptab_d'].ptr,
-                                       ray_off=b['ray_off'].ptr, ray_dz=b['ray_dz'].ptr,
+                                       ptab_s=b['ptab_s'].ptr, ray_off=b['ray_off'].ptr, ray_dz=b['ray_dz'].ptr,
                                        tmrt=b['tmrt'].ptr if 'tmrt' in b else 0,
                                        utci=b['utci'].ptr if 'utci' in b else 0)
         rt.call('sw_memcpy_h2d', self.dom_dev.ptr, ctypes.addressof(self.structs),
@@ -115,9 +117,10 @@ class Engine:
             st['hours'].array[...] = t.hours
             st['ptab_f'].array[...] = t.ptab_f
             st['ptab_d'].array[...] = t.ptab_d
+            st['ptab_s'].array[...] = t.ptab_s
             st['ray_off'].array[:n] = t.ray_off
             st['ray_dz'].array[:n] = t.ray_dz
-            for key in ('hours', 'ptab_f', 'ptab_d'):
+            for key in ('hours', 'ptab_f', 'ptab_d', 'ptab_s'):
                 rt.call('sw_memcpy_h2d', b[key].ptr, st[key].ptr, st[key].nbytes, h)
             rt.call('sw_memcpy_h2d', b['ray_off'].ptr, st['ray_off'].ptr, n * 8, h)
             rt.call('sw_memcpy_h2d', b['ray_dz'].ptr, st['ray_dz'].ptr, n * 8, h)

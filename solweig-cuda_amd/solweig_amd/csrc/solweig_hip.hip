@@ -22,7 +22,7 @@
 
 #include "../../../include/solweig_hip.h"
 
-static_assert(sizeof(sw_domain) == 304, "sw_domain must stay 37 eight-byte fields");
+static_assert(sizeof(sw_domain) == 312, "sw_domain must stay 37 eight-byte fields");
 
 namespace {
 
@@ -101,6 +101,7 @@ struct DomainG {
     const AS1 uint32_t *gvf_len;
     const AS1 float *gvf_nosh;
     const AS1 float *march_trees;
+    const AS1 uint32_t *ptab_s;
 };
 
 __device__ __forceinline__ DomainG domain_at(const sw_domain *D) {
@@ -127,6 +128,7 @@ __device__ __forceinline__ DomainG domain_at(const sw_domain *D) {
     g.hmax = (const AS4 float *)d->hmax; g.hmax_cols = (int)d->hmax_cols;
     g.gvf_len = (const AS1 uint32_t *)d->gvf_len; g.gvf_nosh = (const AS1 float *)d->gvf_nosh;
     g.march_trees = (const AS1 float *)d->march_trees;
+    g.ptab_s = (const AS1 uint32_t *)d->ptab_s;
     return g;
 }
 
@@ -1044,6 +1046,92 @@ __device__ __forceinline__ void day_patch_sums(const AS1 uint32_t *vis, long lon
     }
 }
 
+// Bisection steps over the patches of each altitude band (ceil(log2(count + 1))).
+constexpr int BAND_STEPS[8] = {5, 5, 5, 5, 5, 4, 3, 1};
+
+#define AS3 __attribute__((address_space(3)))
+
+// Copy one hour row of the patch search tables into LDS (every thread of the workgroup calls;
+// the caller has synchronised the workgroup since the LDS was last read).
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ void stage_search(const AS1 uint32_t *src) {
+    AS3 u32x4 *dst = (AS3 u32x4 *)(AS3 void *)gv_lds;
+    const AS1 u32x4 *s4 = (const AS1 u32x4 *)src;
+    for (int i = (int)threadIdx.x; i < SW_SEARCH_WORDS / 4; i += BLOCK) dst[i] = s4[i];
+    __syncthreads();
+}
+
+// day_patch_sums with the hour's search tables in LDS (stage_search): the diffuse sums are 8
+// nibble-table reads per visibility word and mask, and per altitude band the sunlit and the
+// shaded wall patches are the patches below a bisected position in the band's yi order (the
+// exact FP64 tests tan(asvf) + yi < threshold, evaluated at the bisection probes only).
+__device__ __forceinline__ void day_patch_sums_tab(const AS1 uint32_t *vis, long long P, long long p,
+                                                   const AS4 float *pf, const AS4 double *pd,
+                                                   double tan_asvf, float dveg, float &a_lv, float &a_kd,
+                                                   float &a_swsun, float &a_swsh, float &a_lsunc,
+                                                   float &a_lsuns, float &a_lshc, float &a_lshs) {
+    const AS3 uint32_t *T = (const AS3 uint32_t *)(AS3 void *)gv_lds;
+    const AS3 v2f *nib = (const AS3 v2f *)(T + SWS_NIB);
+    const AS3 double *ys = (const AS3 double *)(T + SWS_YS);
+    const AS3 uint32_t *mk = T + SWS_MASK;
+    const AS3 uint32_t *el = T + SWS_ELIG;
+    float sl = 0.f, sk = 0.f, vl = 0.f, vk = 0.f;
+    uint32_t bld[SW_VIS_WORDS];
+    // ``tb`` (always 0) orders each word's table reads after the previous word's sums: left free,
+    // the compiler issues all 78 reads at once (over 200 VGPRs)
+    int tb = 0;
+#pragma unroll
+    for (int w = 0; w < SW_VIS_WORDS; ++w) {
+        const uint32_t vsh = vis[(0 * SW_VIS_WORDS + w) * P + p];
+        const uint32_t vveg = vis[(1 * SW_VIS_WORDS + w) * P + p];
+        const uint32_t vvb = vis[(2 * SW_VIS_WORDS + w) * P + p];
+        const uint32_t sky = vsh & vveg, vegonly = vsh & ~vveg;
+        bld[w] = ~vsh & vvb;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const int i = 8 * w + q;
+            if (i >= 39) break;
+            const v2f t1 = nib[tb + i * 16 + (int)((sky >> (4 * q)) & 15u)];
+            const v2f t2 = nib[tb + i * 16 + (int)((vegonly >> (4 * q)) & 15u)];
+            sl += t1.x; sk += t1.y; vl += t2.x; vk += t2.y;
+        }
+        asm volatile("" : "+v"(tb), "+v"(sl), "+v"(sk), "+v"(vl), "+v"(vk));
+    }
+    a_lv = fmaf(dveg, vl, sl);
+    a_kd = fmaf(dveg, vk, sk);
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+        const int lo = BAND_START[b], n = BAND_START[b + 1] - BAND_START[b];
+        const int wi = lo >> 5, sh = lo & 31;
+        const uint32_t full = n == 32 ? 0xffffffffu : ((1u << n) - 1u);
+        const uint32_t bm = (sh + n <= 32 ? (bld[wi] >> sh) : __builtin_amdgcn_alignbit(bld[wi + 1], bld[wi], sh)) & full;
+        if (__ballot(bm != 0u) == 0) continue;       // no building patch of the band in this wave
+        const double t_sun = pd[SWP_SUN_BELOW * SW_PATCHES + lo];
+        const double t_sh = pd[SWP_SHADE_FROM * SW_PATCHES + lo];
+        int js = tb, jh = tb;
+#pragma unroll
+        for (int s = 1 << (BAND_STEPS[b] - 1); s > 0; s >>= 1) {
+            const double ya = ys[b * 32 + js + s - 1];
+            const double yb = ys[b * 32 + jh + s - 1];
+            js += (tan_asvf + ya < t_sun) ? s : 0;
+            jh += (tan_asvf + yb < t_sh) ? s : 0;
+        }
+        const uint32_t sun = mk[b * 33 + js], shd = full & ~mk[b * 33 + jh], elig = el[b];
+        const float wc = pf[lo * SW_PATCH_F + SWP_WCOS];
+        const float wsn = pf[lo * SW_PATCH_F + SWP_WSIN];
+        const float c_sun = (float)__popc(bm & sun), c_sh = (float)__popc(bm & shd);
+        const float c_lsun = (float)__popc(bm & elig & sun);
+        const float c_lsh = (float)__popc(bm & (~elig | shd));
+        a_swsun = fmaf(wc, c_sun, a_swsun);
+        a_swsh = fmaf(wc, c_sh, a_swsh);
+        a_lsunc = fmaf(wc, c_lsun, a_lsunc);
+        a_lsuns = fmaf(wsn, c_lsun, a_lsuns);
+        a_lshc = fmaf(wc, c_lsh, a_lshc);
+        a_lshs = fmaf(wsn, c_lsh, a_lshs);
+        asm volatile("" : "+v"(tb), "+v"(a_swsun), "+v"(a_lshs));
+    }
+}
+
 template <bool FULL>
 __global__ __launch_bounds__(BLOCK) void k_hour_pixels(const sw_domain *__restrict__ doms,
                                                        const int32_t *__restrict__ bs, int nd,
@@ -1054,8 +1142,15 @@ __global__ __launch_bounds__(BLOCK) void k_hour_pixels(const sw_domain *__restri
     const AS4 int32_t *gofs = (const AS4 int32_t *)gofs_;
     const AS4 double *gbounds = (const AS4 double *)gbounds_;
     DomainG G;
-    const PixelAt at = locate(doms, bs, nd, total, G);
-    if (!at.valid) return;
+    PixelAt at = locate(doms, bs, nd, total, G);
+    // Lanes past the raster (partial tiles) compute a copy of the nearest pixel, so that every
+    // wave takes part in the workgroup's barriers, and write nothing.
+    const bool valid = at.valid;
+    if (!valid) {
+        at.r = min(at.r, (int)G.rows - 1);
+        at.c = min(at.c, (int)G.cols - 1);
+        at.p = (long long)at.r * G.cols + at.c;
+    }
     const AS4 double *H = G.hours + (long long)hour * SW_HOUR_STRIDE;
     const long long P = G.pixels, p = at.p;
     const bool day = H[SWH_DAY] != 0.0;
@@ -1096,7 +1191,7 @@ __global__ __launch_bounds__(BLOCK) void k_hour_pixels(const sw_domain *__restri
         LupS = lag(g.s.lup * INV9 + amb, s2, w, lf, lu);
         LupW = lag(g.w.lup * INV9 + amb, s3, w, lf, lu);
         LupN = lag(g.n.lup * INV9 + amb, s4, w, lf, lu);
-        lg[p] = s0; lg[P + p] = s1; lg[2 * P + p] = s2; lg[3 * P + p] = s3; lg[4 * P + p] = s4;
+        if (valid) { lg[p] = s0; lg[P + p] = s1; lg[2 * P + p] = s2; lg[3 * P + p] = s3; lg[4 * P + p] = s4; }
         if (FULL) {
             const double snup = H[SWH_SNUP];
             const double amp = (double)G.tgk[p] * H[SWH_ALTMAX] + (double)G.tstart[p];
@@ -1106,7 +1201,7 @@ __global__ __launch_bounds__(BLOCK) void k_hour_pixels(const sw_domain *__restri
             tg = (float)t;
             float s5 = lg[5 * P + p];
             tgout = lag((float)(t * (double)shadow + H[SWH_TA]), s5, w, lf, lu);
-            lg[5 * P + p] = s5;
+            if (valid) lg[5 * P + p] = s5;
         }
         Fsh = wedge(G.tan_svfalfa[p], H[SWH_TANZEN]);
         const float svfbuveg = G.svfbuveg[p];
@@ -1153,7 +1248,12 @@ __global__ __launch_bounds__(BLOCK) void k_hour_pixels(const sw_domain *__restri
             a_lshs = ss[SWS_BLD_SIN * P + p];
         }
     }
-    if (!FULL && day) {
+    if (!FULL && day && G.ptab_s) {
+        __syncthreads();                              // every wave's walk is done with the LDS
+        stage_search(G.ptab_s + (long long)row * SW_SEARCH_WORDS);
+        day_patch_sums_tab(vis, P, p, pf, pd, tan_asvf, dveg, a_lv, a_kd, a_swsun, a_swsh, a_lsunc,
+                           a_lsuns, a_lshc, a_lshs);
+    } else if (!FULL && day) {
         day_patch_sums(vis, P, p, pf, pd, H, tan_asvf, dveg, a_lv, a_kd, a_swsun, a_swsh, a_lsunc,
                        a_lsuns, a_lshc, a_lshs);
     }
@@ -1242,6 +1342,7 @@ __global__ __launch_bounds__(BLOCK) void k_hour_pixels(const sw_domain *__restri
         for (int k = 5; k >= 0; --k) acc = acc * dT + H[SWH_UTCI0 + k];
         u = (float)acc;
     }
+    if (!valid) return;
     if (G.tmrt) G.tmrt[(long long)hour * P + p] = Tmrt;
     if (G.utci) G.utci[(long long)hour * P + p] = u;
     if (FULL && G.full) {
@@ -1313,6 +1414,7 @@ int launch_hour_shadow(const sw_domain *doms, const int32_t *bs, int nd, int tot
 // that exceeds 64 KiB.
 int lds_records(int second) {
     const int h = second - 1;
+    static_assert(SW_SEARCH_WORDS * 4 <= 65536, "search tables exceed the LDS limit");
     const int a = (16 + 2 * h) * (16 + 2 * h), b = (32 + 2 * h) * (8 + 2 * h);
     const int n = a > b ? a : b;
     return n * (int)sizeof(float) * 2 <= 65536 ? n : 0;
@@ -1322,7 +1424,9 @@ int launch_hour_pixels(const sw_domain *doms, const int32_t *bs, int nd, int tot
                        const int32_t *gofs, const double *gb, int first, int second, bool full,
                        hipStream_t s) {
     const int cap = lds_records(second);
-    const size_t bytes = (size_t)cap * sizeof(float) * 2;
+    // the staged records during the walk, then the hour's patch search tables
+    size_t bytes = (size_t)cap * sizeof(float) * 2;
+    if (bytes < (size_t)SW_SEARCH_WORDS * 4) bytes = (size_t)SW_SEARCH_WORDS * 4;
     if (full)
         hipLaunchKernelGGL(k_hour_pixels<true>, dim3(total), dim3(BLOCK), bytes, s, doms, bs, nd, total,
                            hour, gofs, gb, first, second, cap);
@@ -2521,6 +2625,10 @@ int launch_adjoint_hour(const sw_domain *dom, const int32_t *bs, int total, int 
 extern "C" {
 
 int sw_version(void) { return 100; }
+
+int sw_record_bytes(int kind) {
+    return kind == 0 ? (int)sizeof(sw_domain) : (kind == 1 ? (int)sizeof(sw_adjoint) : -1);
+}
 
 const char *sw_last_error(void) { return g_error.c_str(); }
 

@@ -19,6 +19,7 @@ SW_PATCH_F = 12
 SW_PATCH_D = 4
 SW_PATCHES = 153
 SW_VIS_WORDS = 5
+SWS_NIB, SWS_YS, SWS_MASK, SWS_ELIG, SW_SEARCH_WORDS = 0, 1248, 1760, 2024, 2048
 TILE_X, TILE_Y = 32, 8          # default workgroup tile (256 threads)
 TILE_CHOICES = (16, 32, 64, 128)  # tile widths the hour kernels accept (height = 256 / width)
 
@@ -53,7 +54,7 @@ class SwDomain(ctypes.Structure):
                                         'rec', 'lag', 'tmrt', 'utci', 'hours', 'ptab_f', 'ptab_d',
                                         'ray_off', 'ray_dz', 'masks', 'shadow_planes', 'full', 'skysums',
                                         'hmax')] + [('hmax_cols', ctypes.c_int64)] + \
-        [(n, ctypes.c_uint64) for n in ('gvf_len', 'gvf_nosh', 'march_trees')]
+        [(n, ctypes.c_uint64) for n in ('gvf_len', 'gvf_nosh', 'march_trees', 'ptab_s')]
 HMAX_CELL = 16
 
 
@@ -89,7 +90,7 @@ def lib():
     L = ctypes.CDLL(LIBRARY)
     vp, i32, i64, f64, sz = ctypes.c_void_p, ctypes.c_int, ctypes.c_longlong, ctypes.c_double, ctypes.c_size_t
     sig = {
-        'sw_version': ([], i32), 'sw_last_error': ([], ctypes.c_char_p),
+        'sw_version': ([], i32), 'sw_last_error': ([], ctypes.c_char_p), 'sw_record_bytes': ([i32], i32),
         'sw_device_count': ([ctypes.POINTER(i32)], i32), 'sw_set_device': ([i32], i32),
         'sw_malloc': ([ctypes.POINTER(vp), sz], i32), 'sw_free': ([vp], i32),
         'sw_host_alloc': ([ctypes.POINTER(vp), sz], i32), 'sw_host_free': ([vp], i32),
@@ -121,11 +122,15 @@ def lib():
         fn = getattr(L, name)
         fn.argtypes = args
         fn.restype = res
+    for kind, rec in ((0, SwDomain), (1, SwAdjoint)):
+        if L.sw_record_bytes(kind) != ctypes.sizeof(rec):
+            raise SolweigHipError(f'{LIBRARY} was built for {L.sw_record_bytes(kind)}-byte {rec.__name__} records, '
+                                  f'these bindings use {ctypes.sizeof(rec)}; rebuild it')
     _lib = L
     return L
 
 
-EXPORTED = ('sw_version', 'sw_last_error', 'sw_device_count', 'sw_set_device', 'sw_malloc', 'sw_free',
+EXPORTED = ('sw_version', 'sw_record_bytes', 'sw_last_error', 'sw_device_count', 'sw_set_device', 'sw_malloc', 'sw_free',
             'sw_host_alloc', 'sw_host_free', 'sw_event_sync',
             'sw_memcpy_h2d', 'sw_memcpy_d2h', 'sw_memcpy_d2d', 'sw_memset', 'sw_stream_create',
             'sw_stream_destroy', 'sw_stream_sync', 'sw_event_create', 'sw_event_destroy',

@@ -14,7 +14,8 @@ from functools import lru_cache
 
 import numpy as np
 
-from .runtime import H, PD, PF, SW_HOUR_STRIDE, SW_PATCH_D, SW_PATCH_F, SW_PATCHES, SW_VIS_WORDS
+from .runtime import (H, PD, PF, SW_HOUR_STRIDE, SW_PATCH_D, SW_PATCH_F, SW_PATCHES, SW_VIS_WORDS,
+                      SWS_ELIG, SWS_MASK, SWS_NIB, SWS_YS, SW_SEARCH_WORDS)
 from . import utci as _utci
 
 SBC = 5.67051e-8
@@ -393,14 +394,57 @@ def patch_rows(day, esky, Ta, altitude, azimuth, lv=None, radD=0.0, f64=False):
     return (f if f64 else f.astype(np.float32)), d
 
 
+_NIB_BITS = ((np.arange(16)[:, None] >> np.arange(4)[None, :]) & 1).astype(np.float64)   # [16][4]
+_BAND_BITS = np.left_shift(np.uint32(1), np.arange(32, dtype=np.uint32))
+
+
+def search_row(f, d):
+    """The patch search tables of one hour (SWS_* words of sw_domain.ptab_s) from its float
+    patch rows ``f`` [153][SW_PATCH_F] and double rows ``d`` [153][SW_PATCH_D]:
+
+    * nibble sums: for the 4 patches 4i..4i+3 and every 4-bit subset, the FP64 sums (rounded
+      once to FP32) of luminance and lumChi cos omega, so that a visibility word's diffuse sums
+      are 8 table reads;
+    * per altitude band the patches' yi in ascending order and the band-local bit masks of the
+      j smallest: tan(asvf) + yi is non-decreasing in yi (rounding is monotonic), so the patches
+      of a band below each exact threshold are the first j of that order, j found by bisection;
+    * per band the eligibility bits of the sunlit-wall test."""
+    out = np.zeros(SW_SEARCH_WORDS, np.uint32)
+    w = np.zeros((156, 2))
+    w[:SW_PATCHES, 0] = f[:, PF['LV']]
+    w[:SW_PATCHES, 1] = f[:, PF['KD']]
+    nib = np.einsum('ej,ijc->iec', _NIB_BITS, w.reshape(39, 4, 2)).astype(np.float32)
+    out[SWS_NIB:SWS_NIB + 39 * 16 * 2] = nib.reshape(-1).view(np.uint32)
+    ys = np.full((8, 32), np.inf)
+    masks = np.zeros((8, 33), np.uint32)
+    elig = np.zeros(8, np.uint32)
+    yi_all = np.asarray(d[:, PD['YI']], dtype=np.float64)
+    el_all = np.asarray(f[:, PF['ELIGIBLE']]) != 0
+    lo = 0
+    for b, n in enumerate(BAND_COUNTS):
+        yi = yi_all[lo:lo + n]
+        order = np.argsort(yi, kind='stable')
+        ys[b, :n] = yi[order]
+        bits = np.left_shift(np.uint32(1), order.astype(np.uint32))
+        masks[b, 1:n + 1] = np.bitwise_or.accumulate(bits)
+        masks[b, n + 1:] = masks[b, n]
+        elig[b] = np.bitwise_or.reduce(np.where(el_all[lo:lo + n], _BAND_BITS[:n], np.uint32(0)))
+        lo += n
+    out[SWS_YS:SWS_YS + 512] = ys.reshape(-1).view(np.uint32)
+    out[SWS_MASK:SWS_MASK + 264] = masks.reshape(-1)
+    out[SWS_ELIG:SWS_ELIG + 8] = elig
+    return out
+
+
 # ----------------------------------------------------------------------------- a day
 
 class DayTables:
     """Host tables of every hour of one day for one domain (the inputs of sw_hours_forward)."""
 
-    def __init__(self, hours, ptab_f, ptab_d, ray_off, ray_dz, meta):
+    def __init__(self, hours, ptab_f, ptab_d, ray_off, ray_dz, meta, ptab_s=None):
         self.hours, self.ptab_f, self.ptab_d = hours, ptab_f, ptab_d
         self.ray_off, self.ray_dz, self.meta = ray_off, ray_dz, meta
+        self.ptab_s = ptab_s
 
     @property
     def n_hours(self):
@@ -517,6 +561,7 @@ def build_day(met, sun, latitude, shape, scale, amaxvalue, *, landcover=0, wall_
     hours = np.zeros((n, SW_HOUR_STRIDE))
     pf = np.zeros((n, SW_PATCHES, SW_PATCH_F), dtype=np.float32)
     pd = np.zeros((n, SW_PATCH_D, SW_PATCHES))
+    ps = np.zeros((n, SW_SEARCH_WORDS), np.uint32)
     offs, dzs, meta = [], [], []
     start = 0
     for i in range(n):
@@ -530,6 +575,8 @@ def build_day(met, sun, latitude, shape, scale, amaxvalue, *, landcover=0, wall_
             psi=psi)
         m['state_in'] = st_in
         pd[i] = d_i.T
+        if h[H['DAY']]:
+            ps[i] = search_row(pf[i], d_i)
         h[H['RAY_START']] = start
         h[H['PATCH_TABLE']] = i
         hours[i] = h
@@ -541,6 +588,6 @@ def build_day(met, sun, latitude, shape, scale, amaxvalue, *, landcover=0, wall_
         meta.append(m)
     ray_off = np.concatenate(offs) if offs else np.zeros((1, 2), np.int32)
     ray_dz = np.concatenate(dzs) if dzs else np.zeros(1)
-    t = DayTables(hours, pf, pd, ray_off, ray_dz, meta)
+    t = DayTables(hours, pf, pd, ray_off, ray_dz, meta, ps)
     t.end_state = st[:3]
     return t
