@@ -106,8 +106,10 @@ typedef struct sw_domain {
     double *shadow_planes;                                   /* [8][P] cast_shadow planes      */
     float *full;                                             /* [SW_FULL_PLANES][P]           */
     const float *skysums;                                    /* [SW_SKY_SUMS][P] static sums   */
-    const float *hmax;    /* [ceil(rows/16)][hmax_cols] upper bound of max(dsm, canopy top) per
-                             16 x 16 cell (FP32 rounded up), lets the sun-ray march skip steps */
+    const float *hmax;    /* [3][3][ceil(rows/16)][hmax_cols]: per 16 x 16 cell an upper bound
+                             of max(dsm, canopy top) (FP32 rounded up); level (i, j) holds at
+                             cell (r, c) its maximum over cells [r, r + 2^i) x [c, c + 2^j).
+                             Lets the sun-ray march skip steps                              */
     int64_t hmax_cols;    /* ceil(cols/16)                                                    */
     const uint32_t *gvf_len; /* [5][P] ground-view steps before the first non-ground sample,
                                 one byte per direction (direction d: byte d % 4 of word d / 4;

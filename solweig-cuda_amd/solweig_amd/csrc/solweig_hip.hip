@@ -354,10 +354,22 @@ __device__ __forceinline__ bool window_below(const DomainG &G, const PixelAt &at
     int c0 = at.wc0 + min(dc0, dc1), c1 = at.wc1 + max(dc0, dc1);
     r0 = max(r0, 0); c0 = max(c0, 0); r1 = min(r1, rows - 1); c1 = min(c1, cols - 1);
     if (r0 > r1 || c0 > c1) return true;             // every sample is off the raster
-    float hm = -__builtin_huge_valf();
-    for (int cr = r0 >> HCELL_SHIFT; cr <= (r1 >> HCELL_SHIFT); ++cr)
-        for (int cc = c0 >> HCELL_SHIFT; cc <= (c1 >> HCELL_SHIFT); ++cc)
-            hm = fmaxf(hm, G.hmax[cr * G.hmax_cols + cc]);
+    const int r0c = r0 >> HCELL_SHIFT, r1c = r1 >> HCELL_SHIFT, c0c = c0 >> HCELL_SHIFT, c1c = c1 >> HCELL_SHIFT;
+    const int h = r1c - r0c + 1, w = c1c - c0c + 1, hc = G.hmax_cols;
+    const long long level = (long long)((rows + (1 << HCELL_SHIFT) - 1) >> HCELL_SHIFT) * hc;
+    float hm;
+    if (h <= 8 && w <= 8) {
+        // four blocks of 2^i x 2^j cells (i, j the floor log2 of the box's sides, at most 2)
+        // cover the box exactly: four independent scalar loads
+        const int i = h >= 4 ? 2 : (h >= 2 ? 1 : 0), j = w >= 4 ? 2 : (w >= 2 ? 1 : 0);
+        const AS4 float *M = G.hmax + (i * 3 + j) * level;
+        const int ra = r0c * hc, rb = (r1c - (1 << i) + 1) * hc, cb = c1c - (1 << j) + 1;
+        hm = fmaxf(fmaxf(M[ra + c0c], M[ra + cb]), fmaxf(M[rb + c0c], M[rb + cb]));
+    } else {
+        hm = -__builtin_huge_valf();
+        for (int cr = r0c; cr <= r1c; ++cr)
+            for (int cc = c0c; cc <= c1c; ++cc) hm = fmaxf(hm, G.hmax[cr * hc + cc]);
+    }
     return (double)hm - dz_before <= a_min;
 }
 

@@ -65,7 +65,9 @@ def march_trees(a, trees):
 def height_bound(a, trees):
     """Per 16 x 16 cell, an FP32 upper bound of max(a, canopy top) with a the FP64 DSM (raised
     to >= 0) and top = trees + a (series.py:282-287): no sample of the sun-ray march in the cell
-    can exceed it."""
+    can exceed it. Returned as the [3][3][R][C] table of its block maxima: level (i, j) holds at
+    cell (r, c) the maximum over the cells [r, r + 2^i) x [c, c + 2^j) of the grid, so the
+    maximum over any box of at most 8 x 8 cells is that of four blocks of one level."""
     a = np.asarray(a, dtype=np.float64)
     t = np.asarray(trees, dtype=np.float64)
     h = np.maximum(a, t + a)
@@ -78,7 +80,19 @@ def height_bound(a, trees):
     m32 = m.astype(np.float32)
     low = m32.astype(np.float64) < m
     m32[low] = np.nextafter(m32[low], np.float32(np.inf))
-    return np.ascontiguousarray(m32)
+    out = np.empty((3, 3, R, C), np.float32)
+    for i in range(3):
+        if i == 0:
+            out[0, 0] = m32
+        else:
+            prev, k = out[i - 1, 0], 1 << (i - 1)
+            out[i, 0] = prev
+            out[i, 0, :R - k] = np.maximum(prev[:R - k], prev[k:])
+        for j in range(1, 3):
+            prev, k = out[i, j - 1], 1 << (j - 1)
+            out[i, j] = prev
+            out[i, j, :, :C - k] = np.maximum(prev[:, :C - k], prev[:, k:])
+    return np.ascontiguousarray(out)
 
 
 class PreparedDomain:

@@ -88,3 +88,30 @@ def test_march_canopy_plane():
     assert np.isneginf(m[0, 0]) and m[0, 1] == np.float32(4.0) and np.isneginf(m[0, 2]) and np.isneginf(m[0, 3])
     t2 = np.array([[0.0, 4.0, 2e-15, 0.0]], dtype=np.float32)
     assert march_trees(a, t2) is None
+
+
+def test_height_bound_block_maxima():
+    """Level (0, 0) bounds every sample of its 16 x 16 cell from above; the four-block query of
+    the march's window test equals the maximum over any box of at most 8 x 8 cells."""
+    from solweig_amd.static import height_bound
+    rng = np.random.default_rng(5)
+    rows, cols = 150, 117
+    a = rng.uniform(0, 40, (rows, cols))
+    t = np.where(rng.random((rows, cols)) < 0.2, rng.uniform(0, 12, (rows, cols)), 0).astype(np.float32)
+    hb = height_bound(a, t)
+    R, C = -(-rows // 16), -(-cols // 16)
+    assert hb.shape == (3, 3, R, C) and hb.dtype == np.float32
+    top = np.maximum(a, t.astype(np.float64) + a)
+    for r in range(R):
+        for c in range(C):
+            assert float(hb[0, 0, r, c]) >= top[16 * r:16 * r + 16, 16 * c:16 * c + 16].max()
+    for _ in range(300):
+        r0, c0 = rng.integers(0, R), rng.integers(0, C)
+        r1, c1 = min(R - 1, r0 + rng.integers(0, 8)), min(C - 1, c0 + rng.integers(0, 8))
+        h, w = r1 - r0 + 1, c1 - c0 + 1
+        i = 2 if h >= 4 else (1 if h >= 2 else 0)
+        j = 2 if w >= 4 else (1 if w >= 2 else 0)
+        M = hb[i, j]
+        rb, cb = r1 - (1 << i) + 1, c1 - (1 << j) + 1
+        q = max(M[r0, c0], M[r0, cb], M[rb, c0], M[rb, cb])
+        assert q == hb[0, 0, r0:r1 + 1, c0:c1 + 1].max()
