@@ -33,7 +33,9 @@ enum {
     SWH_ALTMAX, SWH_FRAC, SWH_SNUP, SWH_CITGG, /* surface temperature wave (model.py:141-142)  */
     SWH_LAG_WEIGHT, SWH_LAG_FIRST, SWH_LAG_UPDATE, /* TsWaveDelay of the Lup maps (742-756)  */
     SWH_FACE_LOW, SWH_FACE_HIGH, SWH_FACE_MIDDLE,  /* self-shadowed wall bounds (678-684)    */
-    SWH_RAY_START, SWH_RAY_LEN,              /* sun ray slice in the ray tables               */
+    SWH_RAY_START, SWH_RAY_LEN,              /* sun ray slice in the ray tables: the start is a
+                                                multiple of 8 steps and the slice is padded to a
+                                                multiple of 8 (the last step repeated)          */
     SWH_LWALL,            /* SBC*ewall*((Tgwall+Ta+273.15)^4-(Ta+273.15)^4), gvf.cu field 7 */
     SWH_FLUX4,            /* SBC*(Ta+273.15)^4                                             */
     SWH_LW_AMBIENT, SWH_LW_SUNLIT, SWH_LW_REFL,  /* ewall*SBC*T^4/pi, (1-ewall)*0.5/pi     */

@@ -373,6 +373,9 @@ __device__ __forceinline__ bool window_below(const DomainG &G, const PixelAt &at
     return (double)hm - dz_before <= a_min;
 }
 
+typedef int32_t i32x16 __attribute__((ext_vector_type(16)));
+typedef double f64x8 __attribute__((ext_vector_type(8)));
+
 // CHUNK steps [k0, k0 + CHUNK) of the sun-ray march of one pixel. MASKED: per-lane bounds
 // tests (a step outside the raster is made a no-op by an infinite dz) and only steps < L count;
 // otherwise all CHUNK steps are inside the raster for every lane and below L. TOPS: ``trees`` is
@@ -385,19 +388,27 @@ __device__ __forceinline__ void march_chunk(March &m, const AS1 float *dsm, cons
                                             int cols, const AS4 int32_t *off, const AS4 double *dzs,
                                             int k0, int L) {
 #pragma clang fp contract(off)
+    static_assert(CHUNK == 8, "one 16-dword load of shifts and one of heights per chunk");
     float av[CHUNK], tv[CHUNK];
     bool in[CHUNK];
+    // the chunk's 8 shift pairs and 8 heights as two scalar loads (ray slices start at multiples
+    // of 8 steps)
+    const i32x16 ov = *(const AS4 i32x16 *)(off + 2 * k0);
+    const f64x8 zv = *(const AS4 f64x8 *)(dzs + k0);
 #pragma unroll
     for (int i = 0; i < CHUNK; ++i) {
-        const int dr = off[2 * (k0 + i)], dc = off[2 * (k0 + i) + 1];
+        const int dr = ov[2 * i], dc = ov[2 * i + 1];
         long long q;
         if (!MASKED) {
             in[i] = true;
             q = at.p + (long long)(dr * cols + dc);
         } else {
+            // a lane outside the raster reads its own pixel: selects, not branches, so that the
+            // chunk stays one block and its 8 offset pairs load together
             const int rr = at.r + dr, cc = at.c + dc;
             in[i] = (unsigned)rr < (unsigned)rows && (unsigned)cc < (unsigned)cols;
-            q = in[i] ? (long long)rr * cols + cc : at.p;
+            const int rq = in[i] ? rr : at.r, cq = in[i] ? cc : at.c;
+            q = (long long)rq * cols + cq;
         }
         av[i] = dsm[q];
         tv[i] = trees[q];
@@ -405,7 +416,7 @@ __device__ __forceinline__ void march_chunk(March &m, const AS1 float *dsm, cons
 #pragma unroll
     for (int i = 0; i < CHUNK; ++i) {
         if (MASKED && k0 + i >= L) break;
-        const double z = dzs[k0 + i];
+        const double z = zv[i];
         const double dz = (!MASKED || in[i]) ? z : __builtin_huge_val();
         const double dzp = (!MASKED || in[i]) ? m.prev : __builtin_huge_val();
         const double a = (double)av[i] + raise;
