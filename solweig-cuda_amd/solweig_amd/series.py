@@ -79,6 +79,21 @@ def _raster(src):
     raise ValueError(f'{p}: only .npy rasters are read (GeoTIFF input needs GDAL)')
 
 
+def make_domain(dsm, trees, dem, forcing, latitude, longitude, utc, site_altitude_m=3., pixel_size_m=1.0,
+                dsm_wall_threshold_m=3., landcover=None):
+    """The model domain of one raster set (series.py:212-365): float32 rasters, walls and aspect
+    from the DSM on the device (rounded to float32 as series.py:235-236 stores them)."""
+    dsm = np.asarray(dsm, dtype=np.float32)
+    trees = np.asarray(trees, dtype=np.float32)
+    dem = np.asarray(dem, dtype=np.float32)
+    walls, aspect = dsm_walls.calculate_dsm_walls(dsm.astype(np.float64), float(pixel_size_m),
+                                                  float(dsm_wall_threshold_m))
+    walls, aspect = walls.astype(np.float32), aspect.astype(np.float32)
+    return synthetic.Domain(dsm, trees, dem, walls, aspect, forcing.values, float(latitude), float(longitude),
+                            float(utc), float(site_altitude_m), scale=1.0 / float(pixel_size_m),
+                            landcover=None if landcover is None else np.asarray(landcover))
+
+
 def run_timeseries(building_dsm_path, tree_path, dem_path, met_file, output_path, longitude, latitude, *,
                    landcover_path=None, utc_offset_hours=None, site_altitude_m=3., wall_method='dsm',
                    dsm_wall_threshold_m=3., save_tmrt=True, dtype=np.float32, no_write=False,
@@ -89,16 +104,10 @@ def run_timeseries(building_dsm_path, tree_path, dem_path, met_file, output_path
     if wall_method != 'dsm':
         raise ValueError("only wall_method='dsm' is supported")
     forcing = load_meteorology(met_file)
-    dsm = np.asarray(_raster(building_dsm_path), dtype=np.float32)
-    trees = np.asarray(_raster(tree_path), dtype=np.float32)
-    dem = np.asarray(_raster(dem_path), dtype=np.float32)
-    walls, aspect = dsm_walls.calculate_dsm_walls(dsm.astype(np.float64), float(pixel_size_m),
-                                                  float(dsm_wall_threshold_m))
-    walls, aspect = walls.astype(np.float32), aspect.astype(np.float32)
     utc = float(longitude) / 15.0 if utc_offset_hours is None else float(utc_offset_hours)
-    lc = _raster(landcover_path)
-    dom = synthetic.Domain(dsm, trees, dem, walls, aspect, forcing.values, float(latitude), float(longitude),
-                           utc, float(site_altitude_m), scale=1.0 / float(pixel_size_m), landcover=lc)
+    dom = make_domain(_raster(building_dsm_path), _raster(tree_path), _raster(dem_path), forcing, latitude,
+                      longitude, utc, site_altitude_m, pixel_size_m, dsm_wall_threshold_m,
+                      landcover=_raster(landcover_path))
     out = Path(output_path).resolve()
     files = {'static': {}, 'days': {}}
     variables = ['UTCI'] + (['TMRT'] if save_tmrt else [])
@@ -144,7 +153,7 @@ def run_timeseries(building_dsm_path, tree_path, dem_path, met_file, output_path
             if no_write:
                 if valid is not None:
                     utci = np.where(valid[None], utci, np.nan)
-                files['days'][day.date] = {'UTCI': utci, 'TMRT': tmrt}
+                files['days'][day.date] = {'UTCI': utci[(slice(None),) + crop], 'TMRT': tmrt[(slice(None),) + crop]}
             else:
                 write(day.date, tmrt, utci)
     finally:
@@ -215,3 +224,35 @@ def load_case(path):
     case.setdefault('output_queue_size', 2)
     case.setdefault('output_chunk_steps', 24)
     return case, settings
+
+
+def run_case(case_path, *, output=None, no_write=False, progress=False, batch_tiles=True, **overrides):
+    """run.py main for a local case (run.py:20-67): the case.json fields (``overrides`` replace
+    them), and with ``core_area_m`` / ``buffer_area_m`` the spatial tiles of run.py:114-129, each
+    written under ``output/<tile_id>``. ``batch_tiles`` runs the tiles as batches of one launch
+    (tiling.run_tiles) instead of one run per tile. Returns the files dict of run_timeseries, or
+    {tile_id: files} for a tiled case."""
+    from . import tiling
+    case, settings = load_case(case_path)
+    case.update(overrides)
+    if settings.get('case_type', 'local') != 'local':
+        raise ValueError('only local cases are run here (open-data acquisition needs the network)')
+    default = Path(case_path).resolve().parent / 'output'
+    output = Path(output or case.pop('output_path', default)).resolve()
+    case.pop('output_path', None)
+    forcing = load_meteorology(case['met_file'])
+    tiles_dir = Path(case_path).resolve().parent / 'tiles'
+    if 'core_area_m' in settings and batch_tiles:
+        res = float(case.get('pixel_size_m', settings.get('resolution_m', 1)))
+        core, halo = (int(round(settings[k] / res)) for k in ('core_area_m', 'buffer_area_m'))
+        tiles = tiling.create_tiles(Path(case['building_dsm_path']).parent, tiles_dir, core=core, halo=halo)
+        keep = {k: case[k] for k in ('utc_offset_hours', 'site_altitude_m', 'dsm_wall_threshold_m', 'save_tmrt',
+                                     'output_queue_size') if k in case}
+        return tiling.run_tiles(tiles, forcing, output, case['longitude'], case['latitude'], pixel_size_m=res,
+                                no_write=no_write, progress=progress, **keep)
+    results = {}
+    for label, options in tiling.tile_jobs(case, settings, tiles_dir):
+        destination = output / label if 'core_area_m' in settings else output
+        results[label] = run_timeseries(**{**options, 'met_file': forcing}, output_path=destination,
+                                        no_write=no_write, progress=progress)
+    return results if 'core_area_m' in settings else results[next(iter(results))]
