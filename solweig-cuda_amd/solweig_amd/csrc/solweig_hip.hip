@@ -755,6 +755,35 @@ struct GvSums {
     }
 };
 
+// stage_tile for a TX-wide tile and a halo of HH pixels: every load of a thread is issued before
+// the first LDS store (the loop form waits for each row's load in turn). Wave w copies rows
+// w, w + 4, ... of the region, its lanes the row's records; lanes past the region's last row or
+// column copy that row or column again (the same value to the same place), so that no load or
+// store is conditional.
+template <int TX, int HH>
+__device__ __forceinline__ void stage_tile_fixed(const DomainG &G, const PixelAt &at) {
+    constexpr int PITCH = TX + 2 * HH, NROWS = BLOCK / TX + 2 * HH;
+    constexpr int WAVES = BLOCK / 64, ROWS = (NROWS + WAVES - 1) / WAVES, PASSES = (PITCH + 63) / 64;
+    const int cols = (int)G.cols, wave = (int)threadIdx.x >> 6, lane = (int)threadIdx.x & 63;
+    const AS1 v2f *rec = G.rec + ((long long)(at.tr0 - HH) * cols + (at.tc0 - HH));
+    v2f v[ROWS][PASSES];
+#pragma unroll
+    for (int k = 0; k < ROWS; ++k)
+#pragma unroll
+        for (int j = 0; j < PASSES; ++j) {
+            const int rr = min(wave + WAVES * k, NROWS - 1), cc = min(lane + 64 * j, PITCH - 1);
+            v[k][j] = rec[(long long)rr * cols + cc];
+        }
+#pragma unroll
+    for (int k = 0; k < ROWS; ++k)
+#pragma unroll
+        for (int j = 0; j < PASSES; ++j) {
+            const int rr = min(wave + WAVES * k, NROWS - 1), cc = min(lane + 64 * j, PITCH - 1);
+            gv_lds[rr * PITCH + cc] = v[k][j];
+        }
+    __syncthreads();
+}
+
 // SRC: 0 global records, the pixel at least second - 1 from the raster edge; 1 global records
 // near the edge (off-raster steps repeat the last sample); 2 records staged in LDS by stage_tile.
 template <int SRC>
@@ -1194,11 +1223,17 @@ __global__ __launch_bounds__(BLOCK) void k_hour_pixels(const sw_domain *__restri
                               (at.tx + 2 * m) * (at.ty + 2 * m) <= lds_cap;
         GvfOut g;
         if (lds_tile) {
-            stage_tile(G, at, m, at.tx + 2 * m);
             const bool fixed = first == 1 && second == GV_FIXED_STEPS;
-            if (fixed && at.tx == 16) g = ground_view_fixed<16>(G, at, me, building, H);
-            else if (fixed && at.tx == 32) g = ground_view_fixed<32>(G, at, me, building, H);
-            else g = ground_view<2>(G, at, me, building, H, gofs, gbounds, first, second);
+            if (fixed && at.tx == 16) {
+                stage_tile_fixed<16, GV_FIXED_STEPS - 1>(G, at);
+                g = ground_view_fixed<16>(G, at, me, building, H);
+            } else if (fixed && at.tx == 32) {
+                stage_tile_fixed<32, GV_FIXED_STEPS - 1>(G, at);
+                g = ground_view_fixed<32>(G, at, me, building, H);
+            } else {
+                stage_tile(G, at, m, at.tx + 2 * m);
+                g = ground_view<2>(G, at, me, building, H, gofs, gbounds, first, second);
+            }
         } else {
             const bool interior = at.r >= m && at.c >= m && at.r < (int)G.rows - m && at.c < (int)G.cols - m;
             g = interior ? ground_view<0>(G, at, me, building, H, gofs, gbounds, first, second)
