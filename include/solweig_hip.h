@@ -125,6 +125,9 @@ typedef struct sw_domain {
                                  canopy tops from trees (series.py:252-287)                   */
     const uint32_t *ptab_s;   /* [rows][SW_SEARCH_WORDS] patch search tables, or NULL (the
                                  pixel pass then tests every patch)                            */
+    const uint32_t *gvf_edge; /* [5][P] or NULL: per direction (byte d % 4 of word d / 4) the
+                                 number of ground-view steps whose shift stays inside the raster
+                                 (static, sw_gvf_static); later steps repeat the last of them */
 } sw_domain;
 
 /* Static per-pixel sums over the patches of each visibility class (day patch geometry), the
@@ -300,14 +303,15 @@ int sw_assemble_tmrt(const void *const *fields, const long long *counts, int n_f
  * through Lup, albedo x shadow and the sunlit walls): per pixel and direction the number of
  * steps before the first non-ground sample, with bit 7 set where the pixel's wall faces away
  * from the direction (facesh == 0, umep_solweig.py:1554-1563) (len_dev [5][P] 32-bit words,
- * direction d in byte d % 4 of word d / 4, bytes 2 and 3 of word 4 zero), and the
+ * direction d in byte d % 4 of word d / 4, bytes 2 and 3 of word 4 zero), in the same layout the
+ * number of steps whose shift stays inside the raster (edge_dev, may be NULL), and the
  * unshadowed-albedo sums of gvfalbnosh split as a + albedo_b * b over all 18 directions and the
  * E, S, W, N halves (nosh_dev [10][P]). ground_dev: [P] 0/1; alb_dev, walls_dev, aspect_dev
  * (degrees): [P]; offsets and bounds as sw_hour_pixels takes them.                          */
 int sw_gvf_static(const uint8_t *ground_dev, const float *alb_dev, const float *walls_dev,
                   const float *aspect_dev, int rows, int cols, const int32_t *gvf_offsets_dev,
                   const double *gvf_bounds_dev, int first_steps, int second_steps, uint32_t *len_dev,
-                  float *nosh_dev, void *stream);
+                  float *nosh_dev, uint32_t *edge_dev, void *stream);
 
 /* The ground-view step shifts compiled into the library: out receives [18][n][2] (row, column)
  * for the first n <= 22 steps of the 18 directions (sunonsurface_2018a, umep_solweig.py:1477-1492).

@@ -22,7 +22,7 @@
 
 #include "../../../include/solweig_hip.h"
 
-static_assert(sizeof(sw_domain) == 312, "sw_domain must stay 37 eight-byte fields");
+static_assert(sizeof(sw_domain) == 320, "sw_domain must stay 37 eight-byte fields");
 
 namespace {
 
@@ -102,6 +102,7 @@ struct DomainG {
     const AS1 float *gvf_nosh;
     const AS1 float *march_trees;
     const AS1 uint32_t *ptab_s;
+    const AS1 uint32_t *gvf_edge;
 };
 
 __device__ __forceinline__ DomainG domain_at(const sw_domain *D) {
@@ -129,6 +130,7 @@ __device__ __forceinline__ DomainG domain_at(const sw_domain *D) {
     g.gvf_len = (const AS1 uint32_t *)d->gvf_len; g.gvf_nosh = (const AS1 float *)d->gvf_nosh;
     g.march_trees = (const AS1 float *)d->march_trees;
     g.ptab_s = (const AS1 uint32_t *)d->ptab_s;
+    g.gvf_edge = (const AS1 uint32_t *)d->gvf_edge;
     return g;
 }
 
@@ -760,19 +762,27 @@ struct GvSums {
 // w, w + 4, ... of the region, its lanes the row's records; lanes past the region's last row or
 // column copy that row or column again (the same value to the same place), so that no load or
 // store is conditional.
-template <int TX, int HH>
+// CLAMP (a tile whose halo leaves the raster): positions outside the raster take the nearest
+// raster record; the walk never uses them (ground_view_fixed<TX, true> repeats the last in-raster
+// sample instead).
+template <int TX, int HH, bool CLAMP>
 __device__ __forceinline__ void stage_tile_fixed(const DomainG &G, const PixelAt &at) {
     constexpr int PITCH = TX + 2 * HH, NROWS = BLOCK / TX + 2 * HH;
     constexpr int WAVES = BLOCK / 64, ROWS = (NROWS + WAVES - 1) / WAVES, PASSES = (PITCH + 63) / 64;
-    const int cols = (int)G.cols, wave = (int)threadIdx.x >> 6, lane = (int)threadIdx.x & 63;
-    const AS1 v2f *rec = G.rec + ((long long)(at.tr0 - HH) * cols + (at.tc0 - HH));
+    const int rows = (int)G.rows, cols = (int)G.cols, wave = (int)threadIdx.x >> 6, lane = (int)threadIdx.x & 63;
+    const AS1 v2f *rec = G.rec + (CLAMP ? 0LL : ((long long)(at.tr0 - HH) * cols + (at.tc0 - HH)));
     v2f v[ROWS][PASSES];
 #pragma unroll
     for (int k = 0; k < ROWS; ++k)
 #pragma unroll
         for (int j = 0; j < PASSES; ++j) {
             const int rr = min(wave + WAVES * k, NROWS - 1), cc = min(lane + 64 * j, PITCH - 1);
-            v[k][j] = rec[(long long)rr * cols + cc];
+            if (CLAMP) {
+                const int sr = min(max(at.tr0 - HH + rr, 0), rows - 1), sc = min(max(at.tc0 - HH + cc, 0), cols - 1);
+                v[k][j] = rec[(long long)sr * cols + sc];
+            } else {
+                v[k][j] = rec[(long long)rr * cols + cc];
+            }
         }
 #pragma unroll
     for (int k = 0; k < ROWS; ++k)
@@ -876,26 +886,34 @@ constexpr int8_t GV_STEPS[18][GV_FIXED_STEPS][2] = {
 // ground_view for a tile of width TX staged in LDS, first = 1 and second = GV_FIXED_STEPS
 // (scale 1): the same sums as ground_view<2>, with every step's LDS address an immediate offset
 // from the pixel's position in the staged tile (no per-step address arithmetic or table loads).
-template <int TX>
+// EDGE (a tile whose halo leaves the raster): the steps past the pixel's last in-raster step (the
+// static gvf_edge plane, E) repeat that step's sample, as ground_view<1> and the reference's
+// shifted windows do, in the same order of operations.
+template <int TX, bool EDGE>
 __device__ __forceinline__ GvfOut ground_view_fixed(const DomainG &G, const PixelAt &at, const v2f me,
                                                     const float building, const AS4 double *H) {
     constexpr int NS = GV_FIXED_STEPS, HH = NS - 1, PITCH = TX + 2 * HH;
     const long long P = G.pixels;
     // the pixel's record sits at (HH, HH) from this base; every step's index is base + constant
     const int base = (at.r - at.tr0) * PITCH + (at.c - at.tc0);
-    uint32_t lw[5];                                   // the 18 direction bytes, loaded at once
+    uint32_t lw[5], ew[5];                            // the 18 direction bytes, loaded at once
 #pragma unroll
-    for (int w = 0; w < 5; ++w) lw[w] = G.gvf_len[w * P + at.p];
+    for (int w = 0; w < 5; ++w) {
+        lw[w] = G.gvf_len[w * P + at.p];
+        ew[w] = EDGE ? G.gvf_edge[w * P + at.p] : 0u;
+    }
     GvSums gv(1, NS);
 #pragma unroll
     for (int dir = 0; dir < 18; ++dir) {
         const uint32_t lb = (lw[dir >> 2] >> (8 * (dir & 3))) & 0xffu;
         const int L = (int)(lb & 0x7fu);
+        const int E = EDGE ? (int)((ew[dir >> 2] >> (8 * (dir & 3))) & 0xffu) : NS;
         // kept opaque: as an integer-valued float the compiler rewrites the clamps below into
         // integer subtract, sign-extend and convert sequences
         float Lsteps = (float)L;
         asm volatile("" : "+v"(Lsteps));
         float lup = 0.f, albsh = 0.f;
+        v2f last = me;                                // EDGE: the last in-raster sample so far
         // the sign bits (sunlit walls) of every executed step, step 0 highest; nexec steps
         uint32_t cm = 0u;
         int nexec = 0;
@@ -912,6 +930,10 @@ __device__ __forceinline__ GvfOut ground_view_fixed(const DomainG &G, const Pixe
             const float rem = Lsteps - (float)s0;
 #pragma unroll
             for (int i = 0; i < n; ++i) {
+                if (EDGE) {
+                    if (s0 + i < E) last = buf[i];
+                    buf[i] = last;
+                }
                 const float on = __builtin_amdgcn_fmed3f(rem - (float)i, 0.0f, 1.0f);
                 lup = fmaf(on, buf[i].x, lup);
                 albsh = fmaf(on, fabsf(buf[i].y), albsh);
@@ -953,7 +975,8 @@ __global__ __launch_bounds__(256) void k_gvf_static(const uint8_t *__restrict__ 
                                                     const float *__restrict__ aspect_deg, int rows, int cols,
                                                     const int32_t *__restrict__ gofs,
                                                     const double *__restrict__ gbounds, int first, int second,
-                                                    uint32_t *__restrict__ len, float *__restrict__ nosh) {
+                                                    uint32_t *__restrict__ len, float *__restrict__ nosh,
+                                                    uint32_t *__restrict__ edge) {
     const long long P = (long long)rows * cols;
     const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= P) return;
@@ -966,16 +989,17 @@ __global__ __launch_bounds__(256) void k_gvf_static(const uint8_t *__restrict__ 
     const float inv_f = 1.0f / firstf, inv_f1 = 1.0f / (firstf + 1.0f), inv_s = 1.0f / secondf;
     const int stride = (second + CHUNK - 1) / CHUNK * CHUNK;
     float a[5] = {0.f, 0.f, 0.f, 0.f, 0.f}, b[5] = {0.f, 0.f, 0.f, 0.f, 0.f};
-    uint32_t words[5] = {0u, 0u, 0u, 0u, 0u};
+    uint32_t words[5] = {0u, 0u, 0u, 0u, 0u}, ewords[5] = {0u, 0u, 0u, 0u, 0u};
     for (int dir = 0; dir < 18; ++dir) {
         const int32_t *off = gofs + dir * stride * 2;
         float af = building, wa = 0.f, fa = 0.f;
-        int L = 0, fL = 0;
+        int L = 0, fL = 0, E = second;
         bool snapped = false;
         long long q = p;
         for (int st = 0; st < second; ++st) {
             const int rr = r + off[2 * st], cc = c + off[2 * st + 1];
             if ((unsigned)rr < (unsigned)rows && (unsigned)cc < (unsigned)cols) q = (long long)rr * cols + cc;
+            else if (E == second) E = st;
             af *= ground[q] ? 1.0f : 0.0f;
             if (af != 0.f) { ++L; wa += alb[q]; }
             if (st == first - 1) { fa = wa; fL = L; snapped = true; }
@@ -995,6 +1019,7 @@ __global__ __launch_bounds__(256) void k_gvf_static(const uint8_t *__restrict__ 
         else
             facesh = 1 - (int)(aspect > gbd[0] || aspect <= gbd[1]);
         words[dir >> 2] |= (uint32_t)(L | (facesh == 0 ? 0x80 : 0)) << (8 * (dir & 3));
+        ewords[dir >> 2] |= (uint32_t)E << (8 * (dir & 3));
         a[0] += ad; b[0] += bd;
         const int az = 5 + 20 * dir;
         if (az < 180) { a[1] += ad; b[1] += bd; }
@@ -1006,6 +1031,7 @@ __global__ __launch_bounds__(256) void k_gvf_static(const uint8_t *__restrict__ 
         nosh[g * P + p] = a[g];
         nosh[(5 + g) * P + p] = b[g];
         len[g * P + p] = words[g];
+        if (edge) edge[g * P + p] = ewords[g];
     }
 }
 
@@ -1217,23 +1243,32 @@ __global__ __launch_bounds__(BLOCK) void k_hour_pixels(const sw_domain *__restri
         const float building = G.ground[p] ? 1.0f : 0.0f;
         shadow = shadow_of_class(class_of(G.masks[p]), (float)(1.0 - (1.0 - H[SWH_PSI])));
         const int m = second - 1;
-        // the tile and its halo inside the raster and within the LDS allocation: stage it
-        const bool lds_tile = lds_cap > 0 && at.tx <= 32 && at.tr0 >= m && at.tc0 >= m &&
-                              at.tr0 + at.ty + m <= (int)G.rows && at.tc0 + at.tx + m <= (int)G.cols &&
-                              (at.tx + 2 * m) * (at.ty + 2 * m) <= lds_cap;
+        // tiles of width 16 or 32 stage their records and halo in LDS: with the halo inside the
+        // raster always, with the scale-1 geometry also at the raster edge (clamped, gvf_edge)
+        const bool fits = lds_cap > 0 && at.tx <= 32 && (at.tx + 2 * m) * (at.ty + 2 * m) <= lds_cap;
+        const bool inner = at.tr0 >= m && at.tc0 >= m && at.tr0 + at.ty + m <= (int)G.rows &&
+                           at.tc0 + at.tx + m <= (int)G.cols;
+        const bool fixed = first == 1 && second == GV_FIXED_STEPS && (at.tx == 16 || at.tx == 32) &&
+                           (inner || G.gvf_edge != nullptr);
         GvfOut g;
-        if (lds_tile) {
-            const bool fixed = first == 1 && second == GV_FIXED_STEPS;
-            if (fixed && at.tx == 16) {
-                stage_tile_fixed<16, GV_FIXED_STEPS - 1>(G, at);
-                g = ground_view_fixed<16>(G, at, me, building, H);
-            } else if (fixed && at.tx == 32) {
-                stage_tile_fixed<32, GV_FIXED_STEPS - 1>(G, at);
-                g = ground_view_fixed<32>(G, at, me, building, H);
+        if (fits && fixed) {
+            constexpr int HH = GV_FIXED_STEPS - 1;
+            if (at.tx == 16 && inner) {
+                stage_tile_fixed<16, HH, false>(G, at);
+                g = ground_view_fixed<16, false>(G, at, me, building, H);
+            } else if (at.tx == 16) {
+                stage_tile_fixed<16, HH, true>(G, at);
+                g = ground_view_fixed<16, true>(G, at, me, building, H);
+            } else if (inner) {
+                stage_tile_fixed<32, HH, false>(G, at);
+                g = ground_view_fixed<32, false>(G, at, me, building, H);
             } else {
-                stage_tile(G, at, m, at.tx + 2 * m);
-                g = ground_view<2>(G, at, me, building, H, gofs, gbounds, first, second);
+                stage_tile_fixed<32, HH, true>(G, at);
+                g = ground_view_fixed<32, true>(G, at, me, building, H);
             }
+        } else if (fits && inner) {
+            stage_tile(G, at, m, at.tx + 2 * m);
+            g = ground_view<2>(G, at, me, building, H, gofs, gbounds, first, second);
         } else {
             const bool interior = at.r >= m && at.c >= m && at.r < (int)G.rows - m && at.c < (int)G.cols - m;
             g = interior ? ground_view<0>(G, at, me, building, H, gofs, gbounds, first, second)
@@ -3003,14 +3038,14 @@ int sw_dsm_walls(const double *dsm, int rows, int cols, double threshold, double
 
 int sw_gvf_static(const uint8_t *ground, const float *alb, const float *walls, const float *aspect, int rows,
                   int cols, const int32_t *gofs, const double *gbounds, int first, int second, uint32_t *len,
-                  float *nosh, void *stream) {
+                  float *nosh, uint32_t *edge, void *stream) {
     if (!ground || !alb || !walls || !aspect || !gofs || !gbounds || !len || !nosh)
         return fail(SW_ERR_ARGUMENT, "sw_gvf_static: NULL pointer");
     if (rows <= 0 || cols <= 0 || first < 1 || second < 1 || second > MAX_SECOND)
         return fail(SW_ERR_ARGUMENT, "sw_gvf_static: bad raster or step counts");
     const long long n = (long long)rows * cols;
     hipLaunchKernelGGL(k_gvf_static, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, (hipStream_t)stream, ground,
-                       alb, walls, aspect, rows, cols, gofs, gbounds, first, second, len, nosh);
+                       alb, walls, aspect, rows, cols, gofs, gbounds, first, second, len, nosh, edge);
     return hip_check(hipGetLastError(), "k_gvf_static launch");
 }
 

@@ -54,7 +54,7 @@ class SwDomain(ctypes.Structure):
                                         'rec', 'lag', 'tmrt', 'utci', 'hours', 'ptab_f', 'ptab_d',
                                         'ray_off', 'ray_dz', 'masks', 'shadow_planes', 'full', 'skysums',
                                         'hmax')] + [('hmax_cols', ctypes.c_int64)] + \
-        [(n, ctypes.c_uint64) for n in ('gvf_len', 'gvf_nosh', 'march_trees', 'ptab_s')]
+        [(n, ctypes.c_uint64) for n in ('gvf_len', 'gvf_nosh', 'march_trees', 'ptab_s', 'gvf_edge')]
 HMAX_CELL = 16
 
 
@@ -115,7 +115,7 @@ def lib():
         'sw_day_adjoint': ([vp, vp, i32, i32, i32, vp, vp, vp, i32, i32, f64, vp], i32),
         'sw_graph_begin': ([vp], i32), 'sw_graph_end': ([vp, ctypes.POINTER(vp)], i32),
         'sw_graph_launch': ([vp, vp], i32), 'sw_graph_destroy': ([vp], i32),
-        'sw_gvf_static': ([vp, vp, vp, vp, i32, i32, vp, vp, i32, i32, vp, vp, vp], i32),
+        'sw_gvf_static': ([vp, vp, vp, vp, i32, i32, vp, vp, i32, i32, vp, vp, vp, vp], i32),
         'sw_gvf_fixed_steps': ([vp, i32], i32),
     }
     for name, (args, res) in sig.items():

@@ -227,7 +227,8 @@ class PreparedDomain:
                            ('tmaxlst', 'tmax'), ('svfbuveg', 'svfbuveg'), ('tan_asvf', 'tan_asvf'),
                            ('tan_svfalfa', 'tan_svfalfa'), ('ground', 'ground'), ('vis', 'vis'),
                            ('rec', 'rec'), ('lag', 'lag'), ('skysums', 'skysums'), ('hmax', 'hmax'),
-                           ('masks', 'masks'), ('gvf_len', 'gvf_len'), ('gvf_nosh', 'gvf_nosh')):
+                           ('masks', 'masks'), ('gvf_len', 'gvf_len'), ('gvf_nosh', 'gvf_nosh'),
+                           ('gvf_edge', 'gvf_edge')):
             setattr(s, field, d[key].ptr)
         s.march_trees = d['march_trees'].ptr if d.get('march_trees') is not None else 0
         s.hmax_cols = -(-self.shape[1] // rt.HMAX_CELL)
@@ -248,10 +249,11 @@ def ground_view_static(prep, stream=None):
         d['gvf_bounds'] = rt.DeviceArray.from_host(bounds, np.float64)
         d['gvf_len'] = rt.DeviceArray((5, prep.P), np.uint32)
         d['gvf_nosh'] = rt.DeviceArray((10, prep.P), np.float32)
+        d['gvf_edge'] = rt.DeviceArray((5, prep.P), np.uint32)
     rows, cols = prep.shape
     rt.call('sw_gvf_static', d['ground'].ptr, d['alb'].ptr, d['walls'].ptr, d['aspect'].ptr, rows, cols,
             d['gvf_off'].ptr, d['gvf_bounds'].ptr, first, second, d['gvf_len'].ptr, d['gvf_nosh'].ptr,
-            rt._s(stream))
+            d['gvf_edge'].ptr, rt._s(stream))
     rt.call('sw_stream_sync', rt._s(stream))
 
 
