@@ -1878,30 +1878,43 @@ __global__ __launch_bounds__(BLOCK) void k_adj_prep(const sw_domain *__restrict_
 
 // Per-patch sums of lambda_Kdown and lambda_Kside over the pixels that see the patch as sky or
 // through vegetation only (objective hour).
+// One visibility word w (blockIdx.y) per launch row: each thread sums its pixels' lambda_Kdown and
+// lambda_Kside over the word's 32 patches in registers (grid-stride over the pixels), then each
+// sum is reduced over the wave and added once per wave.
+constexpr int ADJ_PATCH_BLOCKS = 512;
 __global__ __launch_bounds__(256) void k_adj_patch(const uint32_t *__restrict__ bits,
                                                    const float *__restrict__ lam, long long P,
                                                    double *__restrict__ out) {
-    const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    const bool valid = p < P;
-    const float lK = valid ? lam[p] : 0.f, lS = valid ? lam[P + p] : 0.f;
-    for (int w = 0; w < SW_VIS_WORDS; ++w) {
-        const uint32_t vsh = valid ? bits[(0 * SW_VIS_WORDS + w) * P + p] : 0u;
-        const uint32_t vveg = valid ? bits[(1 * SW_VIS_WORDS + w) * P + p] : 0u;
+    const int w = (int)blockIdx.y, k0 = w * 32;
+    float sK[32], vK[32], sS[32], vS[32];
+#pragma unroll
+    for (int j = 0; j < 32; ++j) sK[j] = vK[j] = sS[j] = vS[j] = 0.f;
+    for (long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x; p < P;
+         p += (long long)gridDim.x * blockDim.x) {
+        const float lK = lam[p], lS = lam[P + p];
+        const uint32_t vsh = bits[(0 * SW_VIS_WORDS + w) * P + p];
+        const uint32_t vveg = bits[(1 * SW_VIS_WORDS + w) * P + p];
         const uint32_t sky = vsh & vveg, vegonly = vsh & ~vveg;
-        const int k0 = w * 32, n = SW_PATCHES - k0 < 32 ? SW_PATCHES - k0 : 32;
-        for (int j = 0; j < n; ++j) {
+#pragma unroll
+        for (int j = 0; j < 32; ++j) {
             const bool a = (sky >> j) & 1u, v = (vegonly >> j) & 1u;
-            const uint64_t any = __ballot(a || v);
-            if (any == 0) continue;
-            const float t0 = wave_sum(a ? lK : 0.f), t1 = wave_sum(v ? lK : 0.f);
-            const float t2 = wave_sum(a ? lS : 0.f), t3 = wave_sum(v ? lS : 0.f);
-            if ((threadIdx.x & 63) == 0) {
-                const int k = k0 + j;
-                if (t0 != 0.f) atomicAdd(out + SWA_PATCH_SKY_KDOWN * SW_PATCHES + k, (double)t0);
-                if (t1 != 0.f) atomicAdd(out + SWA_PATCH_VEG_KDOWN * SW_PATCHES + k, (double)t1);
-                if (t2 != 0.f) atomicAdd(out + SWA_PATCH_SKY_KSIDE * SW_PATCHES + k, (double)t2);
-                if (t3 != 0.f) atomicAdd(out + SWA_PATCH_VEG_KSIDE * SW_PATCHES + k, (double)t3);
-            }
+            sK[j] += a ? lK : 0.f;
+            vK[j] += v ? lK : 0.f;
+            sS[j] += a ? lS : 0.f;
+            vS[j] += v ? lS : 0.f;
+        }
+    }
+    const int n = SW_PATCHES - k0 < 32 ? SW_PATCHES - k0 : 32;
+#pragma unroll
+    for (int j = 0; j < 32; ++j) {
+        if (j >= n) break;
+        const float t0 = wave_sum(sK[j]), t1 = wave_sum(vK[j]), t2 = wave_sum(sS[j]), t3 = wave_sum(vS[j]);
+        if ((threadIdx.x & 63) == 0) {
+            const int k = k0 + j;
+            if (t0 != 0.f) atomicAdd(out + SWA_PATCH_SKY_KDOWN * SW_PATCHES + k, (double)t0);
+            if (t1 != 0.f) atomicAdd(out + SWA_PATCH_VEG_KDOWN * SW_PATCHES + k, (double)t1);
+            if (t2 != 0.f) atomicAdd(out + SWA_PATCH_SKY_KSIDE * SW_PATCHES + k, (double)t2);
+            if (t3 != 0.f) atomicAdd(out + SWA_PATCH_VEG_KSIDE * SW_PATCHES + k, (double)t3);
         }
     }
 }
@@ -2704,8 +2717,10 @@ int launch_adjoint_hour(const sw_domain *dom, const int32_t *bs, int total, int 
         rc = hip_check(hipStreamSynchronize(s), "adjoint record sync");
         if (rc) return rc;
         const long long P = d.pixels;
-        hipLaunchKernelGGL(k_adj_patch, dim3((unsigned)((P + 255) / 256)), dim3(256), 0, s, d.vis,
-                           a.lam_patch, P, a.patch);
+        const long long blocks = (P + 255) / 256;
+        hipLaunchKernelGGL(k_adj_patch, dim3((unsigned)(blocks < ADJ_PATCH_BLOCKS ? blocks : ADJ_PATCH_BLOCKS),
+                                             (unsigned)SW_VIS_WORDS), dim3(256), 0, s, d.vis, a.lam_patch, P,
+                           a.patch);
         rc = hip_check(hipGetLastError(), "k_adj_patch launch");
         if (rc) return rc;
     }
