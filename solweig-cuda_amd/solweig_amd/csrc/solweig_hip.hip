@@ -417,10 +417,12 @@ __device__ __forceinline__ void march_chunk(March &m, const AS1 float *dsm, cons
     }
 #pragma unroll
     for (int i = 0; i < CHUNK; ++i) {
-        if (MASKED && k0 + i >= L) break;
+        // a step outside the raster or past the ray's end (the padding) changes nothing: its dz
+        // is +inf, so the chunk has no branches and its loads issue together
+        const bool live = !MASKED || (in[i] && k0 + i < L);
         const double z = zv[i];
-        const double dz = (!MASKED || in[i]) ? z : __builtin_huge_val();
-        const double dzp = (!MASKED || in[i]) ? m.prev : __builtin_huge_val();
+        const double dz = live ? z : __builtin_huge_val();
+        const double dzp = live ? m.prev : __builtin_huge_val();
         const double a = (double)av[i] + raise;
         const double t = (double)tv[i];
         const double v1 = t + a;
