@@ -225,7 +225,8 @@ enum {
     SWA_FLUX4, SWA_SW_SUNLIT, SWA_SW_SHADED, SWA_SKY_A, SWA_SKY_B, SWA_LW_AMBIENT, SWA_LW_SUNLIT,
     SWA_TA32, SWA_UTCI0,                      /* 7 UTCI coefficients                                */
     SWA_TRANS_STATIC = SWA_UTCI0 + 7,         /* dJ/d transmissivity through svfbuveg           */
-    SWA_SLOTS = 24
+    SWA_SLOTS = 24,
+    SWA_PARTIAL_SLOTS = SWA_SLOTS + 2         /* the hour's slots, then the objective sum and count */
 };
 /* per-patch sums at the objective hour: sum over pixels of lambda_Kdown (resp. lambda_Kside)
  * over the pixels that see patch k as sky (resp. through vegetation only)                   */
@@ -244,7 +245,8 @@ typedef struct sw_adjoint {
     double *scalars;       /* [n_hours][SWA_SLOTS]                                             */
     double *patch;         /* [SWA_PATCH_SUMS][153]                                            */
     double *objective;     /* [2] sum of UTCI over ground pixels at the objective hour, count */
-    int64_t reserved[6];
+    double *partial;       /* [SWA_PARTIAL_SLOTS][total_blocks] per-workgroup sums (work area)  */
+    int64_t reserved[5];
 } sw_adjoint;
 
 /* One hour of the backward sweep for a single domain: the shadow pass of ``hour`` must have

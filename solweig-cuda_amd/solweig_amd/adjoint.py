@@ -53,6 +53,7 @@ class UtciGradient:
         if not 0 <= self.h_obj < self.n_hours:
             raise ValueError('objective hour outside the forcing rows')
         self.albedo_b, self.trans = float(albedo_b), float(trans)
+        self._cached = None
         self.stream = stream or rt.Stream()
         self.pd = PreparedDomain(domain, stream=self.stream, keep_host=True, trans=self.trans)
         self.eng = Engine([self.pd], n_hours=self.n_hours, stream=self.stream)
@@ -61,12 +62,16 @@ class UtciGradient:
         if self.count == 0:
             raise ValueError('no ground pixels')
         P = self.pd.P
+        rows, cols = self.pd.shape
+        # workgroup partial sums: one column per tile of the widest-tile count any width gives
+        self.partial_cols = max(-(-rows // (256 // w)) * -(-cols // w) for w in rt.TILE_CHOICES)
         self.buf = dict(lam_state=rt.DeviceArray((5, P), np.float32), lam_rec=rt.DeviceArray((3, P), np.float32),
                         coef=rt.DeviceArray((18, P, 4), np.float32), grad_alb=rt.DeviceArray((P,), np.float32),
                         grad_emis=rt.DeviceArray((P,), np.float32), lam_patch=rt.DeviceArray((2, P), np.float32),
                         scalars=rt.DeviceArray((self.n_hours, rt.ADJ_SLOTS), np.float64),
                         patch=rt.DeviceArray((4, rt.SW_PATCHES), np.float64),
-                        objective=rt.DeviceArray((2,), np.float64))
+                        objective=rt.DeviceArray((2,), np.float64),
+                        partial=rt.DeviceArray((rt.ADJ_PARTIAL_SLOTS, self.partial_cols), np.float64))
         a = rt.SwAdjoint()
         for k, v in self.buf.items():
             setattr(a, k, v.ptr)
@@ -95,9 +100,15 @@ class UtciGradient:
         return c
 
     def _tables(self):
-        sun = met_sun(self.met, self.domain.location, self.domain.utc)
-        tabs = self.eng.tables(met=[self.met], sun=[sun], constants=self._constants(), psi=self.trans)
-        return sun, tabs
+        """Sun positions and hour tables of the current forcing and scalar parameters, built once
+        per parameter set: the gradient after an objective of the same parameters reuses the
+        objective's tables (they are host data of a few hundred KB, not per-pixel state)."""
+        key = (self.albedo_b, self.trans, self.met.tobytes())
+        if self._cached is None or self._cached[0] != key:
+            sun = met_sun(self.met, self.domain.location, self.domain.utc)
+            tabs = self.eng.tables(met=[self.met], sun=[sun], constants=self._constants(), psi=self.trans)
+            self._cached = (key, sun, tabs)
+        return self._cached[1], self._cached[2]
 
     # ------------------------------------------------------------------ forward
     def objective(self):
@@ -123,6 +134,8 @@ class UtciGradient:
         while first > 0 and t.hours[first - 1, rt.H['DAY']] and not t.hours[first, rt.H['LAG_FIRST']]:
             first -= 1
         self._last = (sun, t, first)
+        if self.eng.total_blocks > self.partial_cols:
+            raise ValueError('more tiles than the partial-sum buffer holds')
         self.eng.load(tabs)
         self.eng.reset_state()
         for v in self.buf.values():
@@ -197,7 +210,7 @@ class UtciGradient:
             float(sun['dectime'][h]), float(sun['altmax'][h]), Ta, RH, radG, float(row[12]), Ws, row[1],
             self.domain.latitude, self.pd.shape, self.domain.scale, -1.0, m['state_in'],
             wall_params=self.pd.wall_params, constants=self._constants(albedo_b, trans), psi=trans, f64=True,
-            rows=with_patches)
+            rows=with_patches, ray=False)
         v = [hh[rt.H[k]] for k in VALUES] + list(hh[rt.H['UTCI0']:rt.H['UTCI0'] + 7])
         if with_patches:
             v += list(pf[:, rt.PF['LV']]) + list(pf[:, rt.PF['KD']]) + [hh[rt.H['DIFF_VEG']]]

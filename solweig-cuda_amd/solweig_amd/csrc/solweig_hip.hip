@@ -1553,7 +1553,7 @@ struct AdjG {
     AS1 float *lam_state, *lam_rec, *grad_alb, *grad_emis, *lam_patch;
     AS1 v4f *coef;
     const AS1 float *svfveg;
-    AS1 double *scalars, *patch, *objective;
+    AS1 double *scalars, *patch, *objective, *partial;
 };
 
 __device__ __forceinline__ AdjG adjoint_at(const sw_adjoint *a_) {
@@ -1564,6 +1564,7 @@ __device__ __forceinline__ AdjG adjoint_at(const sw_adjoint *a_) {
     g.grad_emis = (AS1 float *)a->grad_emis; g.lam_patch = (AS1 float *)a->lam_patch;
     g.svfveg = (const AS1 float *)a->svfveg; g.scalars = (AS1 double *)a->scalars;
     g.patch = (AS1 double *)a->patch; g.objective = (AS1 double *)a->objective;
+    g.partial = (AS1 double *)a->partial;
     return g;
 }
 
@@ -1573,13 +1574,24 @@ __device__ __forceinline__ float wave_sum(float v) {
     return v;
 }
 
-// Adds the per-lane values v[0..n) of every lane of the wave to dst[0..n) (every lane must call).
+// Sums the per-thread values v[0..N) over the workgroup and stores them as the workgroup's
+// partial sums of slots slot0 .. slot0 + N in part ([slot][total], one column per workgroup);
+// k_adj_reduce adds the columns up. Every thread must call. (Device-scope atomics from every wave
+// to the same few addresses serialise: 15.6k waves adding 3 values took 0.56 ms.)
 template <int N>
-__device__ __forceinline__ void add_wave(AS1 double *dst, const float (&v)[N]) {
+__device__ __forceinline__ void block_partials(AS1 double *part, int total, int slot0, const float (&v)[N]) {
+    __shared__ float red[BLOCK / 64][N];
 #pragma unroll
     for (int i = 0; i < N; ++i) {
         const float t = wave_sum(v[i]);
-        if ((threadIdx.x & 63) == 0 && t != 0.0f) atomicAdd((double *)(dst + i), (double)t);
+        if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6][i] = t;
+    }
+    __syncthreads();
+    for (int i = (int)threadIdx.x; i < N; i += BLOCK) {
+        double s = 0.0;
+#pragma unroll
+        for (int w = 0; w < BLOCK / 64; ++w) s += (double)red[w][i];
+        part[(long long)(slot0 + i) * total + blockIdx.x] = s;
     }
 }
 
@@ -1702,191 +1714,362 @@ __device__ __forceinline__ void adj_walk(const DomainG &G, const AdjG &A, const 
     atomicAdd((float *)(A.lam_rec + 2 * P + p), self_alb);
 }
 
+// adj_walk for a tile staged in LDS by stage_tile_fixed<TX, GV_FIXED_STEPS - 1, EDGE> (first == 1,
+// second == GV_FIXED_STEPS). A direction's walk state is fixed by three numbers: L, the steps
+// before the first non-ground sample (static, gvf_len), kw, the first step on a sunlit wall (the
+// records' sign bits, read from LDS as ground_view_fixed reads them), and for EDGE tiles E, the
+// in-raster steps (static, gvf_edge; later steps repeat step E - 1's sample). adj_walk's running
+// state gives, after the walk: sunlit-wall count second - kw when kw < L (else 0), wall count
+// second - L, and for the near field (step 0, the receiver) L >= 1 and kw == 0; the receiver's
+// facing bit of gvf_len decides ``keep`` as in GvSums::add_far. Only the walked steps that stay
+// alive past E (L - E of them) land on the last in-raster sample.
+template <int TX, bool EDGE>
+__device__ __forceinline__ void adj_walk_fixed(const DomainG &G, const AdjG &A, const PixelAt &at, const bool valid,
+                                               const v4f me, const uint32_t me_y, const AS4 int32_t *gofs,
+                                               const float (&lg)[5], const float (&la)[5], const float (&ln)[5],
+                                               float &d_lwall, float &d_albedo_b) {
+    constexpr int NS = GV_FIXED_STEPS, HH = NS - 1, PITCH = TX + 2 * HH;
+    constexpr int stride = (NS + CHUNK - 1) / CHUNK * CHUNK;   // padded step axis of gofs
+    const long long P = G.pixels, p = at.p;
+    const float b = (float)(__float_as_uint(me.w) & 0xffu);
+    const int base = (at.r - at.tr0) * PITCH + (at.c - at.tc0);
+    uint32_t lw[5], ew[5];
+#pragma unroll
+    for (int w = 0; w < 5; ++w) {
+        lw[w] = G.gvf_len[w * P + p];
+        ew[w] = EDGE ? G.gvf_edge[w * P + p] : 0u;
+    }
+    const float secondf = (float)NS;
+    const float k4 = FAR_W, k5 = NEAR_W;
+    const float inv_f = 1.0f, inv_f1 = 1.0f / 2.0f;
+    const float inv_s = 1.0f / secondf, inv_s1 = 1.0f / (secondf + 1.0f);
+    float self_lup = 0.f, self_albsh = 0.f, self_alb = 0.f;
+#pragma unroll
+    for (int dir = 0; dir < 18; ++dir) {
+        float mu = lg[0], nu = la[0], ka = ln[0];
+#pragma unroll
+        for (int c = 1; c < 5; ++c)
+            if (dir_in(dir, c)) { mu += lg[c]; nu += la[c]; ka += ln[c]; }
+        const uint32_t lb = (lw[dir >> 2] >> (8 * (dir & 3))) & 0xffu;
+        const int L = (int)(lb & 0x7fu);
+        const int E = EDGE ? (int)((ew[dir >> 2] >> (8 * (dir & 3))) & 0xffu) : NS;
+        uint32_t cm = 0u, last = me_y;
+        int nexec = 0;
+#pragma unroll
+        for (int s0 = 0; s0 < NS; s0 += CHUNK) {
+            if (__ballot(s0 < L) == 0) break;
+            const int n = NS - s0 < CHUNK ? NS - s0 : CHUNK;
+            uint32_t y[CHUNK];
+#pragma unroll
+            for (int i = 0; i < n; ++i) {
+                const int k = (GV_STEPS[dir][s0 + i][0] + HH) * PITCH + GV_STEPS[dir][s0 + i][1] + HH;
+                y[i] = __float_as_uint(gv_lds[base + k].y);
+            }
+#pragma unroll
+            for (int i = 0; i < n; ++i) {
+                if (EDGE) {
+                    if (s0 + i < E) last = y[i];
+                    y[i] = last;
+                }
+                cm = __builtin_amdgcn_alignbit(cm, y[i], 31);
+            }
+            nexec = s0 + n;
+        }
+        const uint32_t lead = cm == 0u ? 0xffffffffu : (uint32_t)__builtin_clz(cm);
+        const uint32_t kw = lead - (uint32_t)(32 - nexec);
+        const float Lf = (float)L;
+        const bool sunin2 = kw < (uint32_t)L;
+        const bool f_alive = L >= 1;                   // the near field's one step (the receiver)
+        const bool f_sunin = f_alive && kw == 0u;
+        const bool keep = sunin2 && kw == 0u && (lb & 0x80u) != 0u;
+        const float cnt_wall = sunin2 ? secondf - (float)kw : 0.f, cnt_any = secondf - Lf;
+        const bool wallin2 = cnt_any > 0.f;
+        const float f_cnt_any = f_alive ? 0.f : 1.f;
+        const float cs = k4 * (sunin2 ? inv_s1 : inv_s);
+        const float cf = k5 * (f_sunin ? inv_f1 : inv_f);
+        const float a_lup = mu * cs, a_albsh = nu * cs, a_nosh = ka * b * (k4 * inv_s);
+        if (valid) {
+            v4f cv;
+            cv.x = a_lup; cv.y = a_albsh; cv.z = a_nosh; cv.w = Lf;
+            A.coef[(long long)dir * P + p] = cv;
+        }
+        if (f_alive) {                                 // near field: step 0 is the receiver itself
+            self_lup += mu * cf;
+            self_albsh += nu * cf;
+            self_alb += ka * b * (k5 * inv_f);
+        }
+        self_lup += mu * (1.0f - b);                  // local terms of a non-ground receiver
+        self_albsh += nu * (1.0f - b);
+        self_alb += ka * (1.0f - b);
+        const float wall_s = (sunin2 && !keep) ? cnt_wall * inv_s1 : 0.f;
+        const float wall_f = f_sunin ? inv_f1 : 0.f;
+        d_lwall += mu * (k4 * wall_s + k5 * wall_f);
+        d_albedo_b += nu * (k4 * wall_s + k5 * wall_f) +
+                      ka * b * (k4 * (wallin2 ? cnt_any * inv_s : 0.f) + k5 * (f_alive ? 0.f : f_cnt_any * inv_f1));
+        if (EDGE && valid && L > E) {                  // alive steps past the raster repeat step E - 1
+            const AS4 int32_t *off = gofs + dir * stride * 2;
+            const long long q = (long long)(at.r + off[2 * (E - 1)]) * G.cols + (at.c + off[2 * (E - 1) + 1]);
+            const float rep = (float)(L - E);
+            atomicAdd((float *)(A.lam_rec + q), a_lup * rep);
+            atomicAdd((float *)(A.lam_rec + P + q), a_albsh * rep);
+            atomicAdd((float *)(A.lam_rec + 2 * P + q), a_nosh * rep);
+        }
+    }
+    if (valid) {
+        atomicAdd((float *)(A.lam_rec + p), self_lup);
+        atomicAdd((float *)(A.lam_rec + P + p), self_albsh);
+        atomicAdd((float *)(A.lam_rec + 2 * P + p), self_alb);
+    }
+}
+
+// One day hour of k_adj_prep for one pixel. MODE 0-3: the tile (width 16 for 0-1, 32 for 2-3;
+// odd: the halo leaves the raster) and its ground-view halo are staged in LDS and the walks read
+// them there (first == 1, second == GV_FIXED_STEPS); every thread of the workgroup calls, lanes
+// past the raster (valid false) compute a copy of the nearest pixel and write nothing. MODE 4:
+// global records, called for valid pixels only.
+template <bool TOP, int MODE>
+__device__ __forceinline__ void adj_prep_day(const DomainG &G, const AdjG &A, const PixelAt &at, const bool valid,
+                                             const AS4 double *H, const AS4 int32_t *gofs,
+                                             const AS4 double *gbounds, int first, int second, double inv_n,
+                                             float (&sl)[SWA_SLOTS], float (&obj)[2]) {
+    constexpr bool STAGED = MODE < 4, EDGE = (MODE & 1) != 0;
+    constexpr int TX = MODE < 2 ? 16 : 32;
+    if (STAGED) stage_tile_fixed<TX, GV_FIXED_STEPS - 1, EDGE>(G, at);
+    const long long P = G.pixels, p = at.p;
+    const float emis = G.emis[p];
+    const v4f me = record16(G, p);
+    const uint32_t cls = (__float_as_uint(me.w) >> 16) & 3u;
+    const float shadow = shadow_of_class(cls, (float)(1.0 - (1.0 - H[SWH_PSI])));
+    const v2f me2 = G.rec[p];
+    const float building = G.ground[p] ? 1.0f : 0.0f;
+    const float flux4 = (float)H[SWH_FLUX4];
+    const float amb = flux4 * emis;
+    const double w = H[SWH_LAG_WEIGHT];
+    const bool lf = H[SWH_LAG_FIRST] != 0.0, lu = H[SWH_LAG_UPDATE] != 0.0;
+    const int m = second - 1;
+    const bool interior = at.r >= m && at.c >= m && at.r < (int)G.rows - m && at.c < (int)G.cols - m;
+    float ld[5] = {0.f, 0.f, 0.f, 0.f, 0.f};      // adjoint of the delayed Lup (all, E, S, W, N)
+    float la[5] = {0.f, 0.f, 0.f, 0.f, 0.f}, ln[5] = {0.f, 0.f, 0.f, 0.f, 0.f};
+    if (TOP) {
+        // ---- forward of the hour (as k_hour_pixels, day branch) ----------------------
+        GvfOut g;
+        if (STAGED) g = ground_view_fixed<TX, EDGE>(G, at, me2, building, H);
+        else if (interior) g = ground_view<0>(G, at, me2, building, H, gofs, gbounds, first, second);
+        else g = ground_view<1>(G, at, me2, building, H, gofs, gbounds, first, second);
+        const AS1 float *lg = G.lag;
+        float s0 = lg[p], s1 = lg[P + p], s2 = lg[2 * P + p], s3 = lg[3 * P + p], s4 = lg[4 * P + p];
+        const float Lup = lag(g.all.lup * INV18 + amb, s0, w, lf, lu);
+        const float LupE = lag(g.e.lup * INV9 + amb, s1, w, lf, lu);
+        const float LupS = lag(g.s.lup * INV9 + amb, s2, w, lf, lu);
+        const float LupW = lag(g.w.lup * INV9 + amb, s3, w, lf, lu);
+        const float LupN = lag(g.n.lup * INV9 + amb, s4, w, lf, lu);
+        const float radI = (float)H[SWH_RADI], radD = (float)H[SWH_RADD], radG = (float)H[SWH_RADG];
+        const float albedo_b = (float)H[SWH_ALBEDO_B];
+        const float Fsh = wedge(G.tan_svfalfa[p], H[SWH_TANZEN]);
+        const float svfbuveg = G.svfbuveg[p];
+        const float sinalt = (float)H[SWH_SINALT], cosalt = (float)H[SWH_COSALT];
+        const float X = radG * (1.0f - Fsh) + radD * Fsh;
+        const float around = albedo_b * (1.0f - svfbuveg) * X;
+        const float refl = radD * svfbuveg + around;
+        const float Kup = (g.all.alb * INV18) * radI * sinalt + refl * (g.all.nosh * INV18);
+        const float KupE = (g.e.alb * INV9) * radI * sinalt + refl * (g.e.nosh * INV9);
+        const float KupS = (g.s.alb * INV9) * radI * sinalt + refl * (g.s.nosh * INV9);
+        const float KupW = (g.w.alb * INV9) * radI * sinalt + refl * (g.w.nosh * INV9);
+        const float KupN = (g.n.alb * INV9) * radI * sinalt + refl * (g.n.nosh * INV9);
+        const float KsideI = shadow * radI * cosalt;
+        const int row = uniform_index(H[SWH_PATCH_TABLE]);
+        const AS4 float *pf = G.ptab_f + (long long)row * SW_PATCHES * SW_PATCH_F;
+        const AS4 double *pd = G.ptab_d + (long long)row * SW_PATCH_D * SW_PATCHES;
+        float a_lv = 0.f, a_kd = 0.f, a_swsun = 0.f, a_swsh = 0.f, a_lsunc = 0.f, a_lsuns = 0.f;
+        float a_lshc = 0.f, a_lshs = 0.f;
+        day_patch_sums(G.vis, P, p, pf, pd, H, G.tan_asvf[p], (float)H[SWH_DIFF_VEG], a_lv, a_kd,
+                       a_swsun, a_swsh, a_lsunc, a_lsuns, a_lshc, a_lshs);
+        const AS1 float *ss = G.skysums;
+        const float SA = (float)H[SWH_SKY_A], SB = (float)H[SWH_SKY_B];
+        const float ss0 = ss[SWS_SKY_SIN * P + p], ss1 = ss[SWS_SKY_GSIN * P + p];
+        const float ss2 = ss[SWS_SKY_COS * P + p], ss3 = ss[SWS_SKY_GCOS * P + p];
+        const float a_ldn = SA * ss0 - SB * ss1, a_lsd = SA * ss2 - SB * ss3;
+        const float a_vc = ss[SWS_VEG_COS * P + p], a_vs = ss[SWS_VEG_SIN * P + p];
+        const float a_rc = ss[SWS_OBS_COS * P + p], a_rs = ss[SWS_OBS_SIN * P + p];
+        const float dRad = a_lv * radD;
+        const float Kdown = radI * shadow * sinalt + around + dRad;
+        const float sw_sun = (float)H[SWH_SW_SUNLIT], sw_sh = (float)H[SWH_SW_SHADED];
+        const float Kside = KsideI + a_kd + sw_sun * a_swsun + sw_sh * a_swsh + sw_sh * a_vc;
+        const float lw_amb = (float)H[SWH_LW_AMBIENT], lw_sun = (float)H[SWH_LW_SUNLIT];
+        const float lw_refl = (float)H[SWH_LW_REFL];
+        const float reflsurf = (a_ldn + Lup) * lw_refl;
+        const float Ldown = a_ldn + lw_amb * a_vs + lw_amb * a_lshs + lw_sun * a_lsuns + reflsurf * a_rs;
+        const float Lside = a_lsd + lw_amb * a_vc + lw_amb * a_lshc + lw_sun * a_lsunc + reflsurf * a_rc;
+        const float absK = (float)H[SWH_ABSK], absL = (float)H[SWH_ABSL];
+        const float Fup = (float)H[SWH_FUP], Fside = (float)H[SWH_FSIDE], Fcyl = (float)H[SWH_FCYL];
+        const float Ksum = (KupN + KupE + KupS + KupW) * 0.5f;
+        const float Lsum = (LupN + LupE + LupS + LupW) * 0.5f;
+        const float Sstr = absK * (Kside * Fcyl + (Kdown + Kup) * Fup + Ksum * Fside) +
+                           absL * ((Ldown + Lup) * Fup + Lside * Fcyl + Lsum * Fside);
+        const float Tmrt = sqrtf(sqrtf(Sstr / (absL * (float)SBC))) - 273.2f;
+        const bool ground = G.ground[p] != 0;
+        // ---- reverse ---------------------------------------------------------------------
+        float lU = 0.f, lT = 0.f;
+        if (ground && H[SWH_UTCI0 + 6] != -999.0 && Tmrt > -999.0f) {
+            const double dT = (double)(Tmrt - (float)H[SWH_TA32]);
+            double acc = H[SWH_UTCI0 + 6], der = 6.0 * H[SWH_UTCI0 + 6];
+            for (int k = 5; k >= 0; --k) {
+                acc = acc * dT + H[SWH_UTCI0 + k];
+                if (k >= 1) der = der * dT + (double)k * H[SWH_UTCI0 + k];
+            }
+            obj[0] = (float)acc;
+            obj[1] = 1.0f;
+            lU = (float)inv_n;
+            lT = (float)(inv_n * der);
+            double pw = 1.0;
+            for (int k = 0; k < 7; ++k) { sl[SWA_UTCI0 + k] = (float)(inv_n * pw); pw *= dT; }
+            sl[SWA_TA32] = -lT;
+        }
+        (void)lU;
+        const float lS = lT * (Tmrt + 273.2f) / (4.0f * Sstr);
+        const float lKside = lS * absK * Fcyl, lKdown = lS * absK * Fup, lKup = lS * absK * Fup;
+        const float lKupC = lS * absK * Fside * 0.5f;
+        const float lLdown = lS * absL * Fup, lLside = lS * absL * Fcyl;
+        const float lLupC = lS * absL * Fside * 0.5f;
+        const float lrefls = lLdown * a_rs + lLside * a_rc;
+        ld[0] = lS * absL * Fup + lrefls * lw_refl;
+        ld[1] = ld[2] = ld[3] = ld[4] = lLupC;
+        const float l_ldn = lLdown + lrefls * lw_refl;
+        sl[SWA_SKY_A] = l_ldn * ss0 + lLside * ss2;
+        sl[SWA_SKY_B] = -(l_ldn * ss1 + lLside * ss3);
+        sl[SWA_LW_AMBIENT] = lLdown * (a_vs + a_lshs) + lLside * (a_vc + a_lshc);
+        sl[SWA_LW_SUNLIT] = lLdown * a_lsuns + lLside * a_lsunc;
+        sl[SWA_SW_SUNLIT] = lKside * a_swsun;
+        sl[SWA_SW_SHADED] = lKside * (a_swsh + a_vc);
+        if (valid) {
+            A.lam_patch[p] = lKdown;
+            A.lam_patch[P + p] = lKside;
+        }
+        // Kup and the ground-view albedo sums
+        const float gsum_alb = g.all.alb * INV18, gsum_nosh = g.all.nosh * INV18;
+        float lrefl = lKup * gsum_nosh;
+        float d_radI = lKup * sinalt * gsum_alb;
+        la[0] = lKup * radI * sinalt * INV18;
+        ln[0] = lKup * refl * INV18;
+        const Triple gc[4] = {g.e, g.s, g.w, g.n};
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            la[c + 1] = lKupC * radI * sinalt * INV9;
+            ln[c + 1] = lKupC * refl * INV9;
+            lrefl += lKupC * (gc[c].nosh * INV9);
+            d_radI += lKupC * sinalt * (gc[c].alb * INV9);
+        }
+        const float laround = lKdown + lrefl;
+        d_radI += lKdown * shadow * sinalt + lKside * shadow * cosalt;
+        sl[SWA_RADI] = d_radI;
+        sl[SWA_RADD] = lKdown * a_lv + lrefl * svfbuveg + laround * albedo_b * (1.0f - svfbuveg) * Fsh;
+        sl[SWA_RADG] = laround * albedo_b * (1.0f - svfbuveg) * (1.0f - Fsh);
+        sl[SWA_ALBEDO_B] = laround * (1.0f - svfbuveg) * X;
+        const float lsvfbuveg = lrefl * radD - laround * albedo_b * X;
+        sl[SWA_TRANS_STATIC] = lsvfbuveg * (1.0f - A.svfveg[p]);
+        const float lshadow = lKdown * radI * sinalt + lKside * radI * cosalt;
+        sl[SWA_PSI] = cls == 1u ? lshadow : 0.f;
+    }
+    // ---- lag recursion, reversed (the delayed value is used this hour, the state by the next)
+    AS1 float *ls = A.lam_state;
+    float lcur[5];
+    const float wf = (float)w;
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+        const float lso = ls[k * P + p];
+        const float ldt = ld[k] + (lu ? lso : 0.f);
+        const float lprev = ldt * wf + (lu ? 0.f : lso);
+        lcur[k] = ldt * (1.0f - wf) + (lf ? lprev : 0.f);
+        if (valid) ls[k * P + p] = lf ? 0.f : lprev;
+    }
+    // current Lup = ground-view sum / 18 (or / 9 per half) + flux4 * emis
+    const float lg5[5] = {lcur[0] * INV18, lcur[1] * INV9, lcur[2] * INV9, lcur[3] * INV9, lcur[4] * INV9};
+    const float lamb = lcur[0] + lcur[1] + lcur[2] + lcur[3] + lcur[4];
+    if (valid) A.grad_emis[p] += lamb * flux4;
+    sl[SWA_FLUX4] = lamb * emis;
+    float d_lwall = 0.f, d_albedo_b = 0.f;
+    if (STAGED)
+        adj_walk_fixed<TX, EDGE>(G, A, at, valid, me, __float_as_uint(me2.y), gofs, lg5, la, ln, d_lwall,
+                                 d_albedo_b);
+    else if (interior)
+        adj_walk<false>(G, A, at, me, H, gofs, gbounds, first, second, lg5, la, ln, d_lwall, d_albedo_b);
+    else
+        adj_walk<true>(G, A, at, me, H, gofs, gbounds, first, second, lg5, la, ln, d_lwall, d_albedo_b);
+    sl[SWA_LWALL] = d_lwall;
+    sl[SWA_ALBEDO_B] += d_albedo_b;
+}
+
 template <bool TOP>
 __global__ __launch_bounds__(BLOCK) void k_adj_prep(const sw_domain *__restrict__ doms,
                                                     const int32_t *__restrict__ bs, int total, int hour,
                                                     const sw_adjoint *__restrict__ adj_,
                                                     const int32_t *__restrict__ gofs_,
                                                     const double *__restrict__ gbounds_, int first,
-                                                    int second, double inv_n) {
+                                                    int second, double inv_n, int lds_cap) {
     const AS4 int32_t *gofs = (const AS4 int32_t *)gofs_;
     const AS4 double *gbounds = (const AS4 double *)gbounds_;
     DomainG G;
-    const PixelAt at = locate(doms, bs, 1, total, G);
+    PixelAt at = locate(doms, bs, 1, total, G);
+    const bool valid = at.valid;
+    if (!valid) {
+        at.r = min(at.r, (int)G.rows - 1);
+        at.c = min(at.c, (int)G.cols - 1);
+        at.p = (long long)at.r * G.cols + at.c;
+    }
     const AdjG A = adjoint_at(adj_);
     const AS4 double *H = G.hours + (long long)hour * SW_HOUR_STRIDE;
     float sl[SWA_SLOTS];
 #pragma unroll
     for (int i = 0; i < SWA_SLOTS; ++i) sl[i] = 0.f;
     float obj[2] = {0.f, 0.f};
-    if (at.valid && H[SWH_DAY] != 0.0) {
-        const long long P = G.pixels, p = at.p;
-        const float emis = G.emis[p];
-        const v4f me = record16(G, p);
-        const uint32_t cls = (__float_as_uint(me.w) >> 16) & 3u;
-        const float shadow = shadow_of_class(cls, (float)(1.0 - (1.0 - H[SWH_PSI])));
-        const v2f me2 = G.rec[p];
-        const float building = G.ground[p] ? 1.0f : 0.0f;
-        const float flux4 = (float)H[SWH_FLUX4];
-        const float amb = flux4 * emis;
-        const double w = H[SWH_LAG_WEIGHT];
-        const bool lf = H[SWH_LAG_FIRST] != 0.0, lu = H[SWH_LAG_UPDATE] != 0.0;
+    if (H[SWH_DAY] != 0.0) {
         const int m = second - 1;
-        const bool interior = at.r >= m && at.c >= m && at.r < (int)G.rows - m && at.c < (int)G.cols - m;
-        float ld[5] = {0.f, 0.f, 0.f, 0.f, 0.f};      // adjoint of the delayed Lup (all, E, S, W, N)
-        float la[5] = {0.f, 0.f, 0.f, 0.f, 0.f}, ln[5] = {0.f, 0.f, 0.f, 0.f, 0.f};
-        if (TOP) {
-            // ---- forward of the hour (as k_hour_pixels, day branch) ----------------------
-            const GvfOut g = interior ? ground_view<0>(G, at, me2, building, H, gofs, gbounds, first, second)
-                                      : ground_view<1>(G, at, me2, building, H, gofs, gbounds, first, second);
-            const AS1 float *lg = G.lag;
-            float s0 = lg[p], s1 = lg[P + p], s2 = lg[2 * P + p], s3 = lg[3 * P + p], s4 = lg[4 * P + p];
-            const float Lup = lag(g.all.lup * INV18 + amb, s0, w, lf, lu);
-            const float LupE = lag(g.e.lup * INV9 + amb, s1, w, lf, lu);
-            const float LupS = lag(g.s.lup * INV9 + amb, s2, w, lf, lu);
-            const float LupW = lag(g.w.lup * INV9 + amb, s3, w, lf, lu);
-            const float LupN = lag(g.n.lup * INV9 + amb, s4, w, lf, lu);
-            const float radI = (float)H[SWH_RADI], radD = (float)H[SWH_RADD], radG = (float)H[SWH_RADG];
-            const float albedo_b = (float)H[SWH_ALBEDO_B];
-            const float Fsh = wedge(G.tan_svfalfa[p], H[SWH_TANZEN]);
-            const float svfbuveg = G.svfbuveg[p];
-            const float sinalt = (float)H[SWH_SINALT], cosalt = (float)H[SWH_COSALT];
-            const float X = radG * (1.0f - Fsh) + radD * Fsh;
-            const float around = albedo_b * (1.0f - svfbuveg) * X;
-            const float refl = radD * svfbuveg + around;
-            const float Kup = (g.all.alb * INV18) * radI * sinalt + refl * (g.all.nosh * INV18);
-            const float KupE = (g.e.alb * INV9) * radI * sinalt + refl * (g.e.nosh * INV9);
-            const float KupS = (g.s.alb * INV9) * radI * sinalt + refl * (g.s.nosh * INV9);
-            const float KupW = (g.w.alb * INV9) * radI * sinalt + refl * (g.w.nosh * INV9);
-            const float KupN = (g.n.alb * INV9) * radI * sinalt + refl * (g.n.nosh * INV9);
-            const float KsideI = shadow * radI * cosalt;
-            const int row = uniform_index(H[SWH_PATCH_TABLE]);
-            const AS4 float *pf = G.ptab_f + (long long)row * SW_PATCHES * SW_PATCH_F;
-            const AS4 double *pd = G.ptab_d + (long long)row * SW_PATCH_D * SW_PATCHES;
-            float a_lv = 0.f, a_kd = 0.f, a_swsun = 0.f, a_swsh = 0.f, a_lsunc = 0.f, a_lsuns = 0.f;
-            float a_lshc = 0.f, a_lshs = 0.f;
-            day_patch_sums(G.vis, P, p, pf, pd, H, G.tan_asvf[p], (float)H[SWH_DIFF_VEG], a_lv, a_kd,
-                           a_swsun, a_swsh, a_lsunc, a_lsuns, a_lshc, a_lshs);
-            const AS1 float *ss = G.skysums;
-            const float SA = (float)H[SWH_SKY_A], SB = (float)H[SWH_SKY_B];
-            const float ss0 = ss[SWS_SKY_SIN * P + p], ss1 = ss[SWS_SKY_GSIN * P + p];
-            const float ss2 = ss[SWS_SKY_COS * P + p], ss3 = ss[SWS_SKY_GCOS * P + p];
-            const float a_ldn = SA * ss0 - SB * ss1, a_lsd = SA * ss2 - SB * ss3;
-            const float a_vc = ss[SWS_VEG_COS * P + p], a_vs = ss[SWS_VEG_SIN * P + p];
-            const float a_rc = ss[SWS_OBS_COS * P + p], a_rs = ss[SWS_OBS_SIN * P + p];
-            const float dRad = a_lv * radD;
-            const float Kdown = radI * shadow * sinalt + around + dRad;
-            const float sw_sun = (float)H[SWH_SW_SUNLIT], sw_sh = (float)H[SWH_SW_SHADED];
-            const float Kside = KsideI + a_kd + sw_sun * a_swsun + sw_sh * a_swsh + sw_sh * a_vc;
-            const float lw_amb = (float)H[SWH_LW_AMBIENT], lw_sun = (float)H[SWH_LW_SUNLIT];
-            const float lw_refl = (float)H[SWH_LW_REFL];
-            const float reflsurf = (a_ldn + Lup) * lw_refl;
-            const float Ldown = a_ldn + lw_amb * a_vs + lw_amb * a_lshs + lw_sun * a_lsuns + reflsurf * a_rs;
-            const float Lside = a_lsd + lw_amb * a_vc + lw_amb * a_lshc + lw_sun * a_lsunc + reflsurf * a_rc;
-            const float absK = (float)H[SWH_ABSK], absL = (float)H[SWH_ABSL];
-            const float Fup = (float)H[SWH_FUP], Fside = (float)H[SWH_FSIDE], Fcyl = (float)H[SWH_FCYL];
-            const float Ksum = (KupN + KupE + KupS + KupW) * 0.5f;
-            const float Lsum = (LupN + LupE + LupS + LupW) * 0.5f;
-            const float Sstr = absK * (Kside * Fcyl + (Kdown + Kup) * Fup + Ksum * Fside) +
-                               absL * ((Ldown + Lup) * Fup + Lside * Fcyl + Lsum * Fside);
-            const float Tmrt = sqrtf(sqrtf(Sstr / (absL * (float)SBC))) - 273.2f;
-            const bool ground = G.ground[p] != 0;
-            // ---- reverse ---------------------------------------------------------------------
-            float lU = 0.f, lT = 0.f;
-            if (ground && H[SWH_UTCI0 + 6] != -999.0 && Tmrt > -999.0f) {
-                const double dT = (double)(Tmrt - (float)H[SWH_TA32]);
-                double acc = H[SWH_UTCI0 + 6], der = 6.0 * H[SWH_UTCI0 + 6];
-                for (int k = 5; k >= 0; --k) {
-                    acc = acc * dT + H[SWH_UTCI0 + k];
-                    if (k >= 1) der = der * dT + (double)k * H[SWH_UTCI0 + k];
-                }
-                obj[0] = (float)acc;
-                obj[1] = 1.0f;
-                lU = (float)inv_n;
-                lT = (float)(inv_n * der);
-                double pw = 1.0;
-                for (int k = 0; k < 7; ++k) { sl[SWA_UTCI0 + k] = (float)(inv_n * pw); pw *= dT; }
-                sl[SWA_TA32] = -lT;
-            }
-            (void)lU;
-            const float lS = lT * (Tmrt + 273.2f) / (4.0f * Sstr);
-            const float lKside = lS * absK * Fcyl, lKdown = lS * absK * Fup, lKup = lS * absK * Fup;
-            const float lKupC = lS * absK * Fside * 0.5f;
-            const float lLdown = lS * absL * Fup, lLside = lS * absL * Fcyl;
-            const float lLupC = lS * absL * Fside * 0.5f;
-            const float lrefls = lLdown * a_rs + lLside * a_rc;
-            ld[0] = lS * absL * Fup + lrefls * lw_refl;
-            ld[1] = ld[2] = ld[3] = ld[4] = lLupC;
-            const float l_ldn = lLdown + lrefls * lw_refl;
-            sl[SWA_SKY_A] = l_ldn * ss0 + lLside * ss2;
-            sl[SWA_SKY_B] = -(l_ldn * ss1 + lLside * ss3);
-            sl[SWA_LW_AMBIENT] = lLdown * (a_vs + a_lshs) + lLside * (a_vc + a_lshc);
-            sl[SWA_LW_SUNLIT] = lLdown * a_lsuns + lLside * a_lsunc;
-            sl[SWA_SW_SUNLIT] = lKside * a_swsun;
-            sl[SWA_SW_SHADED] = lKside * (a_swsh + a_vc);
-            A.lam_patch[p] = lKdown;
-            A.lam_patch[P + p] = lKside;
-            // Kup and the ground-view albedo sums
-            const float gsum_alb = g.all.alb * INV18, gsum_nosh = g.all.nosh * INV18;
-            float lrefl = lKup * gsum_nosh;
-            float d_radI = lKup * sinalt * gsum_alb;
-            la[0] = lKup * radI * sinalt * INV18;
-            ln[0] = lKup * refl * INV18;
-            const Triple gc[4] = {g.e, g.s, g.w, g.n};
-#pragma unroll
-            for (int c = 0; c < 4; ++c) {
-                la[c + 1] = lKupC * radI * sinalt * INV9;
-                ln[c + 1] = lKupC * refl * INV9;
-                lrefl += lKupC * (gc[c].nosh * INV9);
-                d_radI += lKupC * sinalt * (gc[c].alb * INV9);
-            }
-            const float laround = lKdown + lrefl;
-            d_radI += lKdown * shadow * sinalt + lKside * shadow * cosalt;
-            sl[SWA_RADI] = d_radI;
-            sl[SWA_RADD] = lKdown * a_lv + lrefl * svfbuveg + laround * albedo_b * (1.0f - svfbuveg) * Fsh;
-            sl[SWA_RADG] = laround * albedo_b * (1.0f - svfbuveg) * (1.0f - Fsh);
-            sl[SWA_ALBEDO_B] = laround * (1.0f - svfbuveg) * X;
-            const float lsvfbuveg = lrefl * radD - laround * albedo_b * X;
-            sl[SWA_TRANS_STATIC] = lsvfbuveg * (1.0f - A.svfveg[p]);
-            const float lshadow = lKdown * radI * sinalt + lKside * radI * cosalt;
-            sl[SWA_PSI] = cls == 1u ? lshadow : 0.f;
+        const bool fits = lds_cap > 0 && at.tx <= 32 && (at.tx + 2 * m) * (at.ty + 2 * m) <= lds_cap;
+        const bool inner = at.tr0 >= m && at.tc0 >= m && at.tr0 + at.ty + m <= (int)G.rows &&
+                           at.tc0 + at.tx + m <= (int)G.cols;
+        const bool fixed = first == 1 && second == GV_FIXED_STEPS && (at.tx == 16 || at.tx == 32) &&
+                           (inner || G.gvf_edge != nullptr);
+        if (fits && fixed) {
+            if (at.tx == 16 && inner)
+                adj_prep_day<TOP, 0>(G, A, at, valid, H, gofs, gbounds, first, second, inv_n, sl, obj);
+            else if (at.tx == 16)
+                adj_prep_day<TOP, 1>(G, A, at, valid, H, gofs, gbounds, first, second, inv_n, sl, obj);
+            else if (inner)
+                adj_prep_day<TOP, 2>(G, A, at, valid, H, gofs, gbounds, first, second, inv_n, sl, obj);
+            else
+                adj_prep_day<TOP, 3>(G, A, at, valid, H, gofs, gbounds, first, second, inv_n, sl, obj);
+        } else if (valid) {
+            adj_prep_day<TOP, 4>(G, A, at, true, H, gofs, gbounds, first, second, inv_n, sl, obj);
         }
-        // ---- lag recursion, reversed (the delayed value is used this hour, the state by the next)
-        AS1 float *ls = A.lam_state;
-        float lcur[5];
-        const float wf = (float)w;
+        if (!valid) {
 #pragma unroll
-        for (int k = 0; k < 5; ++k) {
-            const float lso = ls[k * P + p];
-            const float ldt = ld[k] + (lu ? lso : 0.f);
-            const float lprev = ldt * wf + (lu ? 0.f : lso);
-            lcur[k] = ldt * (1.0f - wf) + (lf ? lprev : 0.f);
-            ls[k * P + p] = lf ? 0.f : lprev;
+            for (int i = 0; i < SWA_SLOTS; ++i) sl[i] = 0.f;
+            obj[0] = obj[1] = 0.f;
         }
-        // current Lup = ground-view sum / 18 (or / 9 per half) + flux4 * emis
-        const float lg5[5] = {lcur[0] * INV18, lcur[1] * INV9, lcur[2] * INV9, lcur[3] * INV9, lcur[4] * INV9};
-        const float lamb = lcur[0] + lcur[1] + lcur[2] + lcur[3] + lcur[4];
-        A.grad_emis[p] += lamb * flux4;
-        sl[SWA_FLUX4] = lamb * emis;
-        float d_lwall = 0.f, d_albedo_b = 0.f;
-        if (interior) adj_walk<false>(G, A, at, me, H, gofs, gbounds, first, second, lg5, la, ln, d_lwall, d_albedo_b);
-        else adj_walk<true>(G, A, at, me, H, gofs, gbounds, first, second, lg5, la, ln, d_lwall, d_albedo_b);
-        sl[SWA_LWALL] = d_lwall;
-        sl[SWA_ALBEDO_B] += d_albedo_b;
     }
-    add_wave(A.scalars + (long long)hour * SWA_SLOTS, sl);
-    if (TOP) add_wave(A.objective, obj);
+    float all[SWA_PARTIAL_SLOTS];
+#pragma unroll
+    for (int i = 0; i < SWA_SLOTS; ++i) all[i] = sl[i];
+    all[SWA_SLOTS] = obj[0];
+    all[SWA_SLOTS + 1] = obj[1];
+    block_partials(A.partial, total, 0, all);
 }
 
 // Per-patch sums of lambda_Kdown and lambda_Kside over the pixels that see the patch as sky or
 // through vegetation only (objective hour).
 // One visibility word w (blockIdx.y) per launch row: each thread sums its pixels' lambda_Kdown and
 // lambda_Kside over the word's 32 patches in registers (grid-stride over the pixels), then each
-// sum is reduced over the wave and added once per wave.
-constexpr int ADJ_PATCH_BLOCKS = 512;
-__global__ __launch_bounds__(256) void k_adj_patch(const uint32_t *__restrict__ bits,
-                                                   const float *__restrict__ lam, long long P,
-                                                   double *__restrict__ out) {
+// sum is reduced over the workgroup and added once per workgroup. The pixel count and buffers are
+// read from the device records, so the launch needs nothing from the host.
+constexpr int ADJ_PATCH_BLOCKS = 128;
+__global__ __launch_bounds__(256) void k_adj_patch(const sw_domain *__restrict__ dom,
+                                                   const sw_adjoint *__restrict__ adj_) {
+    const DomainG G = domain_at(dom);
+    const AdjG A = adjoint_at(adj_);
+    const AS1 uint32_t *bits = G.vis;
+    const AS1 float *lam = A.lam_patch;
+    AS1 double *out = A.patch;
+    const long long P = G.pixels;
     const int w = (int)blockIdx.y, k0 = w * 32;
     float sK[32], vK[32], sS[32], vS[32];
 #pragma unroll
@@ -1907,99 +2090,221 @@ __global__ __launch_bounds__(256) void k_adj_patch(const uint32_t *__restrict__ 
         }
     }
     const int n = SW_PATCHES - k0 < 32 ? SW_PATCHES - k0 : 32;
+    __shared__ float red[4][4][32];                   // [wave][sum][patch of the word]
+    const int wave = (int)threadIdx.x >> 6;
 #pragma unroll
     for (int j = 0; j < 32; ++j) {
         if (j >= n) break;
         const float t0 = wave_sum(sK[j]), t1 = wave_sum(vK[j]), t2 = wave_sum(sS[j]), t3 = wave_sum(vS[j]);
         if ((threadIdx.x & 63) == 0) {
-            const int k = k0 + j;
-            if (t0 != 0.f) atomicAdd(out + SWA_PATCH_SKY_KDOWN * SW_PATCHES + k, (double)t0);
-            if (t1 != 0.f) atomicAdd(out + SWA_PATCH_VEG_KDOWN * SW_PATCHES + k, (double)t1);
-            if (t2 != 0.f) atomicAdd(out + SWA_PATCH_SKY_KSIDE * SW_PATCHES + k, (double)t2);
-            if (t3 != 0.f) atomicAdd(out + SWA_PATCH_VEG_KSIDE * SW_PATCHES + k, (double)t3);
+            red[wave][0][j] = t0;
+            red[wave][1][j] = t1;
+            red[wave][2][j] = t2;
+            red[wave][3][j] = t3;
         }
     }
+    __syncthreads();
+    if (threadIdx.x < 128) {
+        const int kind = (int)threadIdx.x >> 5, j = (int)threadIdx.x & 31;
+        static_assert(SWA_PATCH_SKY_KDOWN == 0 && SWA_PATCH_VEG_KDOWN == 1 && SWA_PATCH_SKY_KSIDE == 2 &&
+                      SWA_PATCH_VEG_KSIDE == 3, "patch sum order");
+        if (j < n) {
+            const float t = (red[0][kind][j] + red[1][kind][j]) + (red[2][kind][j] + red[3][kind][j]);
+            if (t != 0.f) atomicAdd((double *)(out + kind * SW_PATCHES + k0 + j), (double)t);
+        }
+    }
+}
+
+// Receivers whose walk in direction DIR reaches a TX-wide tile of sources: the sample of step s
+// of receiver p lands on q = p + shift(DIR, s), so the receivers of the tile's sources form the
+// tile grown by the step shifts' extent on the far side (a box of nr x nc pixels whose corner
+// is (r0, c0) from the tile's corner).
+struct GatherBox {
+    int r0, c0, nr, nc;
+};
+constexpr GatherBox gather_box(int dir, int tx) {
+    int rmin = 0, rmax = 0, cmin = 0, cmax = 0;
+    for (int s = 0; s < GV_FIXED_STEPS; ++s) {
+        const int dr = GV_STEPS[dir][s][0], dc = GV_STEPS[dir][s][1];
+        rmin = dr < rmin ? dr : rmin; rmax = dr > rmax ? dr : rmax;
+        cmin = dc < cmin ? dc : cmin; cmax = dc > cmax ? dc : cmax;
+    }
+    return GatherBox{-rmax, -cmax, BLOCK / tx + rmax - rmin, tx + cmax - cmin};
+}
+constexpr int gather_box_max(int tx) {
+    int n = 0;
+    for (int d = 0; d < 18; ++d) {
+        const GatherBox b = gather_box(d, tx);
+        n = b.nr * b.nc > n ? b.nr * b.nc : n;
+    }
+    return n;
+}
+// LDS of k_adj_gather's staged form: two coefficient boxes (one direction is read while the next
+// is staged), for the larger of the 16- and 32-wide tiles.
+constexpr int GATHER_BOX_RECORDS = gather_box_max(16) > gather_box_max(32) ? gather_box_max(16) : gather_box_max(32);
+constexpr int GATHER_LDS_BYTES = 2 * GATHER_BOX_RECORDS * (int)sizeof(v4f);
+static_assert(GATHER_LDS_BYTES <= 65536, "gather boxes exceed the LDS limit");
+
+// One direction of the transposed walk for a tile staged box by box: the receivers' coefficients
+// of direction DIR (zero for receivers outside the raster) are copied into LDS buffer DIR & 1,
+// then each source sums the coefficients of the receivers whose step s lands on it while alive
+// (s < L), in the order of the global form (steps ascending).
+template <int TX, int DIR>
+__device__ __forceinline__ void gather_dir(const DomainG &G, const AdjG &A, const PixelAt &at, float &l_lup,
+                                          float &l_albsh, float &l_alb) {
+    constexpr GatherBox bx = gather_box(DIR, TX);
+    constexpr int N = bx.nr * bx.nc, IT = (N + BLOCK - 1) / BLOCK;
+    v4f *buf = reinterpret_cast<v4f *>(gv_lds) + (DIR & 1) * GATHER_BOX_RECORDS;
+    const int rows = (int)G.rows, cols = (int)G.cols;
+    const AS1 v4f *cd = A.coef + (long long)DIR * G.pixels;
+    v4f v[IT];
+#pragma unroll
+    for (int it = 0; it < IT; ++it) {
+        const int idx = min((int)threadIdx.x + BLOCK * it, N - 1);
+        const int gr = at.tr0 + bx.r0 + idx / bx.nc, gc = at.tc0 + bx.c0 + idx % bx.nc;
+        const bool in = (unsigned)gr < (unsigned)rows && (unsigned)gc < (unsigned)cols;
+        v[it] = cd[in ? (long long)gr * cols + gc : 0];
+        if (!in) v[it] = v4f{0.f, 0.f, 0.f, 0.f};
+    }
+#pragma unroll
+    for (int it = 0; it < IT; ++it) buf[min((int)threadIdx.x + BLOCK * it, N - 1)] = v[it];
+    __syncthreads();
+    const int base = (at.r - at.tr0 - bx.r0) * bx.nc + (at.c - at.tc0 - bx.c0);
+#pragma unroll
+    for (int s = 0; s < GV_FIXED_STEPS; ++s) {
+        const v4f x = buf[base - GV_STEPS[DIR][s][0] * bx.nc - GV_STEPS[DIR][s][1]];
+        const float on = (float)s < x.w ? 1.0f : 0.0f;
+        l_lup = fmaf(on, x.x, l_lup);
+        l_albsh = fmaf(on, x.y, l_albsh);
+        l_alb = fmaf(on, x.z, l_alb);
+    }
+}
+
+template <int TX, int DIR = 0>
+__device__ __forceinline__ void gather_fixed(const DomainG &G, const AdjG &A, const PixelAt &at, float &l_lup,
+                                            float &l_albsh, float &l_alb) {
+    gather_dir<TX, DIR>(G, A, at, l_lup, l_albsh, l_alb);
+    if constexpr (DIR + 1 < 18) gather_fixed<TX, DIR + 1>(G, A, at, l_lup, l_albsh, l_alb);
 }
 
 // Transposed walk for each source pixel q: the sample of receiver p = q - shift(d, s) at step s
 // reaches q when p's walk is alive there (s < L_d(p)). Then the neighbour record of q is
 // reversed to the albedo and emissivity maps and the hour's Ta, CI_TgG and canopy shadow.
+// Tiles 16 or 32 wide at second == GV_FIXED_STEPS stage the receivers' coefficients in LDS
+// (gather_fixed; every thread of the workgroup takes part, lanes past the raster compute a copy
+// of the nearest pixel and write nothing); other tiles gather them from global memory.
 __global__ __launch_bounds__(BLOCK) void k_adj_gather(const sw_domain *__restrict__ doms,
                                                       const int32_t *__restrict__ bs, int total,
                                                       int hour, const sw_adjoint *__restrict__ adj_,
-                                                      const int32_t *__restrict__ gofs_, int second) {
+                                                      const int32_t *__restrict__ gofs_, int second,
+                                                      int lds_bytes) {
 #pragma clang fp contract(off)
     const AS4 int32_t *gofs = (const AS4 int32_t *)gofs_;
     DomainG G;
-    const PixelAt at = locate(doms, bs, 1, total, G);
+    PixelAt at = locate(doms, bs, 1, total, G);
+    const bool valid = at.valid;
+    if (!valid) {
+        at.r = min(at.r, (int)G.rows - 1);
+        at.c = min(at.c, (int)G.cols - 1);
+        at.p = (long long)at.r * G.cols + at.c;
+    }
     const AdjG A = adjoint_at(adj_);
     const AS4 double *H = G.hours + (long long)hour * SW_HOUR_STRIDE;
-    float sl[3] = {0.f, 0.f, 0.f};                    // Ta, CI_TgG, psi
-    if (at.valid && H[SWH_DAY] != 0.0) {
+    static_assert(SWA_TA == 0 && SWA_PSI == 1 && SWA_CITGG == 2, "gather slots");
+    float sl[3] = {0.f, 0.f, 0.f};                    // Ta, psi, CI_TgG
+    if (H[SWH_DAY] != 0.0) {
         const int rows = (int)G.rows, cols = (int)G.cols;
         const long long P = G.pixels, q = at.p;
-        const int stride = (second + CHUNK - 1) / CHUNK * CHUNK;
-        const int m = second - 1;
-        const bool interior = at.r >= m && at.c >= m && at.r < rows - m && at.c < cols - m;
         AS1 float *lr = A.lam_rec;
         float l_lup = lr[q], l_albsh = lr[P + q], l_alb = lr[2 * P + q];
-        lr[q] = 0.f; lr[P + q] = 0.f; lr[2 * P + q] = 0.f;
-        for (int dir = 0; dir < 18; ++dir) {
-            const AS4 int32_t *off = gofs + dir * stride * 2;
-            const AS1 v4f *cd = A.coef + (long long)dir * P;
-            for (int s0 = 0; s0 < second; s0 += CHUNK) {
-                v4f buf[CHUNK];
-                bool in[CHUNK];
+        const bool staged = second == GV_FIXED_STEPS && (at.tx == 16 || at.tx == 32) &&
+                            lds_bytes >= GATHER_LDS_BYTES;
+        if (staged) {
+            if (at.tx == 16) gather_fixed<16>(G, A, at, l_lup, l_albsh, l_alb);
+            else gather_fixed<32>(G, A, at, l_lup, l_albsh, l_alb);
+        } else if (valid) {
+            const int stride = (second + CHUNK - 1) / CHUNK * CHUNK;
+            const int m = second - 1;
+            const bool interior = at.r >= m && at.c >= m && at.r < rows - m && at.c < cols - m;
+            for (int dir = 0; dir < 18; ++dir) {
+                const AS4 int32_t *off = gofs + dir * stride * 2;
+                const AS1 v4f *cd = A.coef + (long long)dir * P;
+                for (int s0 = 0; s0 < second; s0 += CHUNK) {
+                    v4f buf[CHUNK];
+                    bool in[CHUNK];
 #pragma unroll
-                for (int i = 0; i < CHUNK; ++i) {
-                    const int dr = off[2 * (s0 + i)], dc = off[2 * (s0 + i) + 1];
-                    long long pr;
-                    if (interior) {
-                        in[i] = true;
-                        pr = q - (long long)(dr * cols + dc);
-                    } else {
-                        const int rr = at.r - dr, cc = at.c - dc;
-                        in[i] = (unsigned)rr < (unsigned)rows && (unsigned)cc < (unsigned)cols;
-                        pr = in[i] ? (long long)rr * cols + cc : q;
+                    for (int i = 0; i < CHUNK; ++i) {
+                        const int dr = off[2 * (s0 + i)], dc = off[2 * (s0 + i) + 1];
+                        long long pr;
+                        if (interior) {
+                            in[i] = true;
+                            pr = q - (long long)(dr * cols + dc);
+                        } else {
+                            const int rr = at.r - dr, cc = at.c - dc;
+                            in[i] = (unsigned)rr < (unsigned)rows && (unsigned)cc < (unsigned)cols;
+                            pr = in[i] ? (long long)rr * cols + cc : q;
+                        }
+                        buf[i] = cd[pr];
                     }
-                    buf[i] = cd[pr];
-                }
 #pragma unroll
-                for (int i = 0; i < CHUNK; ++i) {
-                    const int st = s0 + i;
-                    if (st >= second) break;
-                    const float on = (in[i] && (float)st < buf[i].w) ? 1.0f : 0.0f;
-                    l_lup = fmaf(on, buf[i].x, l_lup);
-                    l_albsh = fmaf(on, buf[i].y, l_albsh);
-                    l_alb = fmaf(on, buf[i].z, l_alb);
+                    for (int i = 0; i < CHUNK; ++i) {
+                        const int st = s0 + i;
+                        if (st >= second) break;
+                        const float on = (in[i] && (float)st < buf[i].w) ? 1.0f : 0.0f;
+                        l_lup = fmaf(on, buf[i].x, l_lup);
+                        l_albsh = fmaf(on, buf[i].y, l_albsh);
+                        l_alb = fmaf(on, buf[i].z, l_alb);
+                    }
                 }
             }
         }
-        // the record of q: Lup = SBC emis ((Tg sh + Ta + 273.15)^4 - (Ta + 273.15)^4), alb, alb sh
-        const uint32_t cls = class_of(G.masks[q]);
-        const float sh = shadow_of_class(cls, (float)(1.0 - (1.0 - H[SWH_PSI])));
-        const double snup = H[SWH_SNUP];
-        const double amp = (double)G.tgk[q] * H[SWH_ALTMAX] + (double)G.tstart[q];
-        const double base = amp * sin(((H[SWH_FRAC] - snup / 24.0) / ((double)G.tmaxlst[q] / 24.0 - snup / 24.0)) *
-                                      3.141592653589793 / 2.0);
-        double tg = base * H[SWH_CITGG];
-        const bool clamped = G.landcover == 1 && tg < 0.0;
-        if (clamped) tg = 0.0;
-        const double ta = H[SWH_TA], emis = (double)G.emis[q];
-        const double tk = tg * (double)sh + ta + 273.15, ta4 = ta + 273.15;
-        const double ll = (double)l_lup;
-        A.grad_emis[q] += (float)(ll * SBC * ((tk * tk) * (tk * tk) - (ta4 * ta4) * (ta4 * ta4)));
-        A.grad_alb[q] += l_alb + l_albsh * sh;
-        const double dtk = ll * SBC * emis * 4.0 * (tk * tk * tk);
-        sl[0] = (float)(dtk - ll * SBC * emis * 4.0 * (ta4 * ta4 * ta4));
-        sl[1] = clamped ? 0.f : (float)(dtk * (double)sh * base);
-        sl[2] = cls == 1u ? (float)(dtk * tg) + l_albsh * G.alb[q] : 0.f;
+        if (valid) {
+            lr[q] = 0.f; lr[P + q] = 0.f; lr[2 * P + q] = 0.f;
+            // the record of q: Lup = SBC emis ((Tg sh + Ta + 273.15)^4 - (Ta + 273.15)^4), alb, alb sh
+            const uint32_t cls = class_of(G.masks[q]);
+            const float sh = shadow_of_class(cls, (float)(1.0 - (1.0 - H[SWH_PSI])));
+            const double snup = H[SWH_SNUP];
+            const double amp = (double)G.tgk[q] * H[SWH_ALTMAX] + (double)G.tstart[q];
+            const double base = amp * sin(((H[SWH_FRAC] - snup / 24.0) / ((double)G.tmaxlst[q] / 24.0 - snup / 24.0)) *
+                                          3.141592653589793 / 2.0);
+            double tg = base * H[SWH_CITGG];
+            const bool clamped = G.landcover == 1 && tg < 0.0;
+            if (clamped) tg = 0.0;
+            const double ta = H[SWH_TA], emis = (double)G.emis[q];
+            const double tk = tg * (double)sh + ta + 273.15, ta4 = ta + 273.15;
+            const double ll = (double)l_lup;
+            A.grad_emis[q] += (float)(ll * SBC * ((tk * tk) * (tk * tk) - (ta4 * ta4) * (ta4 * ta4)));
+            A.grad_alb[q] += l_alb + l_albsh * sh;
+            const double dtk = ll * SBC * emis * 4.0 * (tk * tk * tk);
+            sl[0] = (float)(dtk - ll * SBC * emis * 4.0 * (ta4 * ta4 * ta4));
+            sl[2] = clamped ? 0.f : (float)(dtk * (double)sh * base);
+            sl[1] = cls == 1u ? (float)(dtk * tg) + l_albsh * G.alb[q] : 0.f;
+        }
     }
-    AS1 double *d = A.scalars + (long long)hour * SWA_SLOTS;
-    const float v0[1] = {sl[0]}, v1[1] = {sl[1]}, v2[1] = {sl[2]};
-    add_wave(d + SWA_TA, v0);
-    add_wave(d + SWA_CITGG, v1);
-    add_wave(d + SWA_PSI, v2);
+    block_partials(A.partial, total, SWA_TA, sl);
+}
+
+// Adds the workgroup partial sums of the slots in mask (one workgroup per slot) to the hour's
+// record (slots below SWA_SLOTS) or to the objective sum and count.
+__global__ __launch_bounds__(256) void k_adj_reduce(const sw_adjoint *__restrict__ adj_, int total, int hour,
+                                                    uint32_t mask) {
+    const int slot = (int)blockIdx.x;
+    if (!((mask >> slot) & 1u)) return;
+    const AdjG A = adjoint_at(adj_);
+    const AS1 double *src = A.partial + (long long)slot * total;
+    double s = 0.0;
+    for (int i = (int)threadIdx.x; i < total; i += 256) s += src[i];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+    __shared__ double red[4];
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const double t = (red[0] + red[1]) + (red[2] + red[3]);
+        AS1 double *dst = slot < SWA_SLOTS ? A.scalars + (long long)hour * SWA_SLOTS + slot
+                                           : A.objective + (slot - SWA_SLOTS);
+        *dst += t;
+    }
 }
 
 // =====================================================================================
@@ -2700,34 +3005,33 @@ __global__ void k_wall_aspect(const double *dsm, const double *walls, const uint
 int launch_adjoint_hour(const sw_domain *dom, const int32_t *bs, int total, int hour, bool top,
                         const sw_adjoint *adj, const int32_t *gofs, const double *gb, int first,
                         int second, double inv_n, hipStream_t s) {
+    // the prep stages tiles as the pixel pass does; the gather stages coefficient boxes
+    const int cap = lds_records(second);
+    const size_t bytes = (size_t)cap * sizeof(float) * 2;
     if (top)
-        hipLaunchKernelGGL(k_adj_prep<true>, dim3(total), dim3(BLOCK), 0, s, dom, bs, total, hour, adj,
-                           gofs, gb, first, second, inv_n);
+        hipLaunchKernelGGL(k_adj_prep<true>, dim3(total), dim3(BLOCK), bytes, s, dom, bs, total, hour, adj,
+                           gofs, gb, first, second, inv_n, cap);
     else
-        hipLaunchKernelGGL(k_adj_prep<false>, dim3(total), dim3(BLOCK), 0, s, dom, bs, total, hour, adj,
-                           gofs, gb, first, second, inv_n);
+        hipLaunchKernelGGL(k_adj_prep<false>, dim3(total), dim3(BLOCK), bytes, s, dom, bs, total, hour, adj,
+                           gofs, gb, first, second, inv_n, cap);
     int rc = hip_check(hipGetLastError(), "k_adj_prep launch");
     if (rc) return rc;
+    hipLaunchKernelGGL(k_adj_reduce, dim3(SWA_PARTIAL_SLOTS), dim3(256), 0, s, adj, total, hour,
+                       (uint32_t)((1ull << SWA_PARTIAL_SLOTS) - 1));
+    rc = hip_check(hipGetLastError(), "k_adj_reduce launch");
+    if (rc) return rc;
     if (top) {
-        // the per-patch sums need the pixel count and the visibility bits: read the records
-        sw_domain d;
-        sw_adjoint a;
-        rc = hip_check(hipMemcpyAsync(&d, dom, sizeof d, hipMemcpyDeviceToHost, s), "domain record");
-        if (rc) return rc;
-        rc = hip_check(hipMemcpyAsync(&a, adj, sizeof a, hipMemcpyDeviceToHost, s), "adjoint record");
-        if (rc) return rc;
-        rc = hip_check(hipStreamSynchronize(s), "adjoint record sync");
-        if (rc) return rc;
-        const long long P = d.pixels;
-        const long long blocks = (P + 255) / 256;
-        hipLaunchKernelGGL(k_adj_patch, dim3((unsigned)(blocks < ADJ_PATCH_BLOCKS ? blocks : ADJ_PATCH_BLOCKS),
-                                             (unsigned)SW_VIS_WORDS), dim3(256), 0, s, d.vis, a.lam_patch, P,
-                           a.patch);
+        hipLaunchKernelGGL(k_adj_patch, dim3(ADJ_PATCH_BLOCKS, SW_VIS_WORDS), dim3(256), 0, s, dom, adj);
         rc = hip_check(hipGetLastError(), "k_adj_patch launch");
         if (rc) return rc;
     }
-    hipLaunchKernelGGL(k_adj_gather, dim3(total), dim3(BLOCK), 0, s, dom, bs, total, hour, adj, gofs, second);
-    return hip_check(hipGetLastError(), "k_adj_gather launch");
+    hipLaunchKernelGGL(k_adj_gather, dim3(total), dim3(BLOCK), GATHER_LDS_BYTES, s, dom, bs, total, hour, adj, gofs,
+                       second, GATHER_LDS_BYTES);
+    rc = hip_check(hipGetLastError(), "k_adj_gather launch");
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_adj_reduce, dim3(SWA_PARTIAL_SLOTS), dim3(256), 0, s, adj, total, hour,
+                       (uint32_t)((1u << SWA_TA) | (1u << SWA_PSI) | (1u << SWA_CITGG)));
+    return hip_check(hipGetLastError(), "k_adj_reduce launch");
 }
 
 }  // namespace

@@ -61,8 +61,9 @@ HMAX_CELL = 16
 class SwAdjoint(ctypes.Structure):
     """Mirror of sw_adjoint: device pointers of the adjoint buffers."""
     _fields_ = [(n, ctypes.c_uint64) for n in ('lam_state', 'lam_rec', 'coef', 'grad_alb', 'grad_emis',
-                                               'lam_patch', 'svfveg', 'scalars', 'patch', 'objective')] + \
-        [('reserved', ctypes.c_int64 * 6)]
+                                               'lam_patch', 'svfveg', 'scalars', 'patch', 'objective',
+                                               'partial')] + \
+        [('reserved', ctypes.c_int64 * 5)]
 
 
 # slots of the per-hour adjoint record (enum SWA_* of solweig_hip.h)
@@ -70,6 +71,7 @@ _ADJ_FIELDS = 'TA PSI CITGG LWALL ALBEDO_B RADI RADD RADG FLUX4 SW_SUNLIT SW_SHA
 ADJ = {name: i for i, name in enumerate(_ADJ_FIELDS)}
 ADJ['TRANS_STATIC'] = ADJ['UTCI0'] + 7
 ADJ_SLOTS = 24
+ADJ_PARTIAL_SLOTS = ADJ_SLOTS + 2
 ADJ_PATCH = {'SKY_KDOWN': 0, 'VEG_KDOWN': 1, 'SKY_KSIDE': 2, 'VEG_KSIDE': 3}
 
 

@@ -453,13 +453,14 @@ class DayTables:
 
 def hour_tables(alt, azi, zen, jday, dectime, altmax, Ta, RH, radG, P, Ws, doy, latitude, shape, scale,
                 amaxvalue, state, *, wall_params=(0.37, -3.41, 15.0), constants=CONSTANTS, psi=None,
-                f64=False, rows=True):
+                f64=False, rows=True, ray=True):
     """One step's hour record, patch rows and sun ray (Solweig_2022a_calc's host part,
     model.py:73-218). ``state`` = (CI, firstdaytime, timeadd, timestepdec) before the step; the
     returned state is the one after it (the CI reset at midnight rows is the caller's).
     ``f64``: every value as a smooth FP64 function of the forcing (no float32 rounding of the
     UTCI inputs or the patch table), for the derivatives of the adjoint; ``rows=False`` skips the
-    Perez luminance and the patch rows (returned as None)."""
+    Perez luminance and the patch rows (returned as None), ``ray=False`` the sun ray (its record
+    entries stay 0)."""
     CI, firstdaytime, timeadd, timestepdec = state
     c = constants
     tgk_w, tstart_w, tmax_w = wall_params
@@ -506,12 +507,13 @@ def hour_tables(alt, azi, zen, jday, dectime, altmax, Ta, RH, radG, P, Ws, doy, 
             timeadd = timeadd + timestepdec
             weight = math.exp(-33.27 * timeadd)
         firstdaytime = 0
-        off, dz, low, high, middle = shadow_ray(azi, alt, scale, amaxvalue, shape)
         h[H['DAY']] = 1
         h[H['CITGG']], h[H['FRAC']] = citgg, frac
         h[H['LAG_WEIGHT']], h[H['LAG_FIRST']], h[H['LAG_UPDATE']] = weight, first, update
-        h[H['FACE_LOW']], h[H['FACE_HIGH']], h[H['FACE_MIDDLE']] = low, high, float(middle)
-        h[H['RAY_LEN']] = len(dz)
+        if ray:
+            off, dz, low, high, middle = shadow_ray(azi, alt, scale, amaxvalue, shape)
+            h[H['FACE_LOW']], h[H['FACE_HIGH']], h[H['FACE_MIDDLE']] = low, high, float(middle)
+            h[H['RAY_LEN']] = len(dz)
     else:
         timeadd = 0.
         firstdaytime = 1

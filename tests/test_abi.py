@@ -66,6 +66,7 @@ def test_adjoint_slots_and_struct_match_header():
     for name, idx in rt.ADJ.items():
         assert vals['SWA_' + name] == idx, name
     assert vals['SWA_SLOTS'] == rt.ADJ_SLOTS
+    assert vals['SWA_PARTIAL_SLOTS'] == rt.ADJ_PARTIAL_SLOTS
     pv = _enum_values('SWA_PATCH_SKY_KDOWN = 0')
     for name, idx in rt.ADJ_PATCH.items():
         assert pv['SWA_PATCH_' + name] == idx

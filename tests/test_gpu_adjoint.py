@@ -111,3 +111,28 @@ def test_torch_autograd_wrapper(case):
     np.testing.assert_allclose(alb.grad.cpu().numpy(), g['alb'], rtol=1e-5, atol=1e-9)
     np.testing.assert_allclose(emis.grad.cpu().numpy(), g['emis'], rtol=1e-5, atol=1e-9)
     np.testing.assert_allclose(Ta.grad.numpy(), g['Ta'], rtol=1e-6, atol=1e-12)
+
+
+def test_gradient_independent_of_tile_width(rt):
+    """Tiles 16 and 32 wide stage the walks in LDS (records for the reverse walk, coefficient boxes
+    for the transposed walk, inner and edge tiles); 64-wide tiles read global memory. All three
+    must give the same gradient up to FP32 summation order."""
+    from solweig_amd import adjoint, synthetic
+    d = synthetic.small_domain(seed=22, size=96, maps=True)
+    print(f'tile-width case: {d.shape}', flush=True)
+    prob = adjoint.UtciGradient(d, objective_hour=H_OBJ)
+    res = {}
+    for w in (64, 16, 32):
+        prob.eng.set_tiles(w)
+        print(f'  tile width {w}', flush=True)
+        res[w] = prob.gradient()
+    ref = res[64]
+    for w in (16, 32):
+        g = res[w]
+        assert abs(g['J'] - ref['J']) <= 1e-6 * abs(ref['J'])
+        for key in ('alb', 'emis'):
+            np.testing.assert_allclose(g[key], ref[key], rtol=1e-4, atol=1e-5 * float(np.abs(ref[key]).max()))
+        for name in ('Ta', 'RH', 'Ws', 'radG'):
+            np.testing.assert_allclose(g[name], ref[name], rtol=1e-4, atol=1e-5 * float(np.abs(ref[name]).max()))
+        for name in ('albedo_b', 'trans'):
+            assert abs(g[name] - ref[name]) <= 1e-4 * abs(ref[name]) + 1e-9, name
