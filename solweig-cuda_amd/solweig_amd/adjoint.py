@@ -7,7 +7,10 @@ radiation of the forcing (columns 11, 10, 9, 14). Shadow and sky-view masks are 
 
 The per-pixel chain runs on the device (``sw_day_adjoint``): the forward of the hours before
 the objective hour, then a backward sweep from the objective hour to the day's first daytime
-hour that recomputes each hour's shadow pass and ground-view walk instead of storing them. It
+hour that recomputes each hour's shadow pass and ground-view walk instead of storing them. The
+backward needs one forward state, the lag planes entering the objective hour (the lag recursion
+is linear, so earlier hours need none): ``objective()`` keeps a copy of them, and a gradient of
+the same parameters starts from that copy instead of running the forward again. It
 returns dJ/d(albedo map), dJ/d(emissivity map), per hour dJ/d(value) for the hour-record values
 the kernels read (Ta, psi, CI_TgG, wall flux, wall albedo, radI, radD, radG, sigma T^4, the wall
 and sky flux factors, the UTCI coefficients) and, at the objective hour, per-patch sums for the
@@ -54,6 +57,8 @@ class UtciGradient:
             raise ValueError('objective hour outside the forcing rows')
         self.albedo_b, self.trans = float(albedo_b), float(trans)
         self._cached = None
+        self._maps_version = 0
+        self._ckpt_key = None
         self.stream = stream or rt.Stream()
         self.pd = PreparedDomain(domain, stream=self.stream, keep_host=True, trans=self.trans)
         self.eng = Engine([self.pd], n_hours=self.n_hours, stream=self.stream)
@@ -72,6 +77,8 @@ class UtciGradient:
                         patch=rt.DeviceArray((4, rt.SW_PATCHES), np.float64),
                         objective=rt.DeviceArray((2,), np.float64),
                         partial=rt.DeviceArray((rt.ADJ_PARTIAL_SLOTS, self.partial_cols), np.float64))
+        # lag planes entering the objective hour, kept by objective() for the gradient
+        self._ckpt = rt.DeviceArray(self.pd.dev['lag'].shape, np.float32)
         a = rt.SwAdjoint()
         for k, v in self.buf.items():
             setattr(a, k, v.ptr)
@@ -84,6 +91,8 @@ class UtciGradient:
     # ------------------------------------------------------------------ parameters
     def set_parameters(self, alb=None, emis=None, albedo_b=None, trans=None, met=None):
         self.pd.set_maps(alb=alb, emis=emis, stream=self.stream)
+        if alb is not None or emis is not None:
+            self._maps_version += 1
         if albedo_b is not None:
             self.albedo_b = float(albedo_b)
         if trans is not None and float(trans) != self.pd.trans:
@@ -116,9 +125,15 @@ class UtciGradient:
         sun, tabs = self._tables()
         self.eng.load(tabs)
         self.eng.reset_state()
-        rt.call('sw_hours_forward', self.eng.dom_dev.ptr, self.eng.block_start.ptr, 1, self.eng.total_blocks,
-                0, self.h_obj + 1, self.eng.gvf_off.ptr, self.eng.gvf_bounds.ptr, self.eng.first,
-                self.eng.second, self.stream.handle)
+        lag = self.pd.dev['lag']
+        for h0, h1 in ((0, self.h_obj), (self.h_obj, self.h_obj + 1)):
+            if h0 == self.h_obj:
+                rt.call('sw_memcpy_d2d', self._ckpt.ptr, lag.ptr, lag.nbytes, self.stream.handle)
+            if h1 > h0:
+                rt.call('sw_hours_forward', self.eng.dom_dev.ptr, self.eng.block_start.ptr, 1,
+                        self.eng.total_blocks, h0, h1, self.eng.gvf_off.ptr, self.eng.gvf_bounds.ptr,
+                        self.eng.first, self.eng.second, self.stream.handle)
+        self._ckpt_key = (self._cached[0], self._maps_version)
         u = self.eng.buf[0]['utci'].plane(self.h_obj).to_host(self.stream)
         self.stream.sync()
         return float(np.nanmean(u))
@@ -140,9 +155,21 @@ class UtciGradient:
         self.eng.reset_state()
         for v in self.buf.values():
             v.zero(self.stream)
-        rt.call('sw_day_adjoint', self.eng.dom_dev.ptr, self.eng.block_start.ptr, self.eng.total_blocks,
-                self.h_obj, first, self.adj_dev.ptr, self.eng.gvf_off.ptr, self.eng.gvf_bounds.ptr,
-                self.eng.first, self.eng.second, 1.0 / self.count, self.stream.handle)
+        e = self.eng
+        if self._ckpt_key == (self._cached[0], self._maps_version):
+            # the objective of these parameters ran last: start the backward from its lag planes
+            lag = self.pd.dev['lag']
+            rt.call('sw_memcpy_d2d', lag.ptr, self._ckpt.ptr, lag.nbytes, self.stream.handle)
+            for h in range(self.h_obj, first - 1, -1):
+                rt.call('sw_hour_shadow', e.dom_dev.ptr, e.block_start.ptr, 1, e.total_blocks, h, 0,
+                        self.stream.handle)
+                rt.call('sw_adjoint_hour', e.dom_dev.ptr, e.block_start.ptr, e.total_blocks, h, self.h_obj,
+                        self.adj_dev.ptr, e.gvf_off.ptr, e.gvf_bounds.ptr, e.first, e.second, 1.0 / self.count,
+                        self.stream.handle)
+        else:
+            rt.call('sw_day_adjoint', e.dom_dev.ptr, e.block_start.ptr, e.total_blocks, self.h_obj, first,
+                    self.adj_dev.ptr, e.gvf_off.ptr, e.gvf_bounds.ptr, e.first, e.second, 1.0 / self.count,
+                    self.stream.handle)
 
     def gradient(self):
         """J and dJ/d every parameter (host arrays / floats). The derivatives of the hour tables

@@ -136,3 +136,12 @@ def test_gradient_independent_of_tile_width(rt):
             np.testing.assert_allclose(g[name], ref[name], rtol=1e-4, atol=1e-5 * float(np.abs(ref[name]).max()))
         for name in ('albedo_b', 'trans'):
             assert abs(g[name] - ref[name]) <= 1e-4 * abs(ref[name]) + 1e-9, name
+    # a gradient after the objective of the same parameters starts from the objective's lag planes
+    prob.objective()
+    g2 = prob.gradient()
+    g = res[32]
+    assert abs(g2['J'] - g['J']) <= 1e-9 * abs(g['J'])
+    for key in ('alb', 'emis'):
+        np.testing.assert_allclose(g2[key], g[key], rtol=1e-5, atol=1e-6 * float(np.abs(g[key]).max()))
+    for name in ('Ta', 'RH', 'Ws', 'radG'):
+        np.testing.assert_allclose(g2[name], g[name], rtol=1e-5, atol=1e-6 * float(np.abs(g[name]).max()))
