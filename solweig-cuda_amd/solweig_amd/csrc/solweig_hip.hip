@@ -2157,17 +2157,21 @@ __device__ __forceinline__ void gather_dir(const DomainG &G, const AdjG &A, cons
     v4f *buf = reinterpret_cast<v4f *>(gv_lds) + (DIR & 1) * GATHER_BOX_RECORDS;
     const int rows = (int)G.rows, cols = (int)G.cols;
     const AS1 v4f *cd = A.coef + (long long)DIR * G.pixels;
+    // opaque per direction: otherwise the box positions of all 18 directions are computed up
+    // front and held (about 180 VGPRs)
+    int t = (int)threadIdx.x;
+    asm volatile("" : "+v"(t));
     v4f v[IT];
 #pragma unroll
     for (int it = 0; it < IT; ++it) {
-        const int idx = min((int)threadIdx.x + BLOCK * it, N - 1);
+        const int idx = min(t + BLOCK * it, N - 1);
         const int gr = at.tr0 + bx.r0 + idx / bx.nc, gc = at.tc0 + bx.c0 + idx % bx.nc;
         const bool in = (unsigned)gr < (unsigned)rows && (unsigned)gc < (unsigned)cols;
         v[it] = cd[in ? (long long)gr * cols + gc : 0];
         if (!in) v[it] = v4f{0.f, 0.f, 0.f, 0.f};
     }
 #pragma unroll
-    for (int it = 0; it < IT; ++it) buf[min((int)threadIdx.x + BLOCK * it, N - 1)] = v[it];
+    for (int it = 0; it < IT; ++it) buf[min(t + BLOCK * it, N - 1)] = v[it];
     __syncthreads();
     const int base = (at.r - at.tr0 - bx.r0) * bx.nc + (at.c - at.tc0 - bx.c0);
 #pragma unroll
@@ -2177,6 +2181,8 @@ __device__ __forceinline__ void gather_dir(const DomainG &G, const AdjG &A, cons
         l_lup = fmaf(on, x.x, l_lup);
         l_albsh = fmaf(on, x.y, l_albsh);
         l_alb = fmaf(on, x.z, l_alb);
+        // at most CHUNK reads in flight, and no direction's loads moved into another's
+        if (s % CHUNK == CHUNK - 1 || s == GV_FIXED_STEPS - 1) __builtin_amdgcn_sched_barrier(0);
     }
 }
 
