@@ -146,6 +146,66 @@ def adjoint_leg(size=1000, repeats=3):
                 J=g['J'], hours=list(g['hours']))
 
 
+def c4_main(args):
+    """The uneven-size batch: 64 domains (sides U_int[256, 1536], latitudes U(0, 60) N, each with
+    its own morphology and forcing), all in one launch per pass and hour. Timed: the device day
+    with the day tables resident (one day of forcing, built before the timed region and reported
+    separately). Two domains are also run alone and must match their part of the batch bit for bit."""
+    from solweig_amd import runtime as rt, synthetic, batched
+    t0 = time.perf_counter()
+    doms = synthetic.make_c4(count=args.c4_count)
+    sides = [d.shape[0] for d in doms]
+    P = sum(d.dsm.size for d in doms)
+    log(f'[bench] c4: {len(doms)} domains, sides {min(sides)}..{max(sides)}, {P / 1e6:.1f} Mpx '
+        f'(generated in {time.perf_counter() - t0:.1f} s)')
+    t0 = time.perf_counter()
+    eng = batched.Engine(doms, n_hours=24)
+    rt.synchronize()
+    t_prep = time.perf_counter() - t0
+    eng.set_tiles(args.tile or 16)
+    log(f'[bench] c4 static preparation {t_prep:.1f} s, {eng.total_blocks} tiles')
+    t0 = time.perf_counter()
+    tabs = eng.tables()
+    t_tab = time.perf_counter() - t0
+    eng.load(tabs)
+    for _ in range(args.warmup):
+        eng.reset_state()
+        eng.forward_day()
+        eng.stream.sync()
+    rt.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        eng.reset_state()
+        eng.forward_day()
+    eng.stream.sync()
+    rt.synchronize()
+    dt = time.perf_counter() - t0
+    value = P * 24 * args.steps / dt
+    log(f'[bench] c4: {dt / args.steps * 1e3:.1f} ms/day, {value:.3e} px-h/s; host tables {t_tab * 1e3:.0f} ms/day')
+    checks = []
+    for k in (int(np.argmin(sides)), int(np.argmax(sides))):
+        tm, ut = eng.outputs(k)
+        alone = batched.Engine([doms[k]], n_hours=24)
+        alone.set_tiles(args.tile or 16)
+        alone.load(alone.tables())
+        alone.reset_state()
+        alone.forward_day()
+        tm1, ut1 = alone.outputs(0)
+        same = bool(np.array_equal(tm, tm1, equal_nan=True) and np.array_equal(ut, ut1, equal_nan=True))
+        checks.append(dict(domain=k, side=sides[k], identical_to_single_domain_run=same))
+        log(f'[bench] c4 domain {k} ({sides[k]}^2) alone vs in the batch: identical {same}')
+        del alone
+    out = dict(metric='Tmrt+UTCI pixel-hours/s, uneven batch of 64 domains, 1 GPU',
+               value=value, unit='pixel-hours/s', n_gpus=1, steps=args.steps, warmup=args.warmup,
+               ms_per_step=dt / args.steps * 1e3, higher_is_better=True, scaling='weak', vs_baseline=None,
+               dtype='f32 (f64 masks/lag/wedge/UTCI)', data='synthetic',
+               config=dict(workload=f'c4 uneven batch: {len(doms)} domains, sides {min(sides)}..{max(sides)}, '
+                                    f'{P} pixels, 24 h per step, one launch per pass and hour, day tables resident',
+                           tile=args.tile or 16, static_prep_s=t_prep, host_tables_ms=t_tab * 1e3,
+                           checks=checks))
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
@@ -162,11 +222,17 @@ def main():
                     help='fixed tile width (16, 32, 64, 128) instead of the autotuner (for profiling runs)')
     ap.add_argument('--plain-days', type=int, default=1,
                     help='also time the plain per-step forward for this many days (0 = skip)')
+    ap.add_argument('--workload', choices=('c2', 'c4'), default='c2',
+                    help='c2: the 2000x2000 day (default); c4: the 64-domain uneven batch')
+    ap.add_argument('--c4-count', type=int, default=64, help='domains of the c4 batch')
     args = ap.parse_args()
     if args.gpus != 1:
         print(f'bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU sharding is a '
               f'design note, not part of this build', file=sys.stderr)
         sys.exit(2)
+    if args.workload == 'c4':
+        c4_main(args)
+        return
     from solweig_amd import runtime as rt, synthetic, batched
     days = 30 if args.size == 2000 else 3
     log(f'[bench] generating c2 morphology {args.size}x{args.size}')
