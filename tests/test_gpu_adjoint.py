@@ -85,10 +85,11 @@ def test_map_gradients_match_fp64_differences(case):
     d, prob, g, s, p = case
     rng = np.random.default_rng(33)
     ground = np.argwhere(p['buildings'] == 1)
-    picks = ground[rng.choice(len(ground), size=6, replace=False)]
-    for key, grid in (('alb', 'alb_grid'), ('emis', 'emis_grid')):
+    # 20 seeded ground pixels: the albedo map at the first 10, the emissivity map at the others
+    picks = ground[rng.choice(len(ground), size=20, replace=False)]
+    for (key, grid), sel in ((('alb', 'alb_grid'), picks[:10]), (('emis', 'emis_grid'), picks[10:])):
         scale = float(np.abs(g[key]).max())
-        for r, c in picks:
+        for r, c in sel:
             step = 1e-3
             up, dn = p[grid].copy(), p[grid].copy()
             up[r, c] += step
