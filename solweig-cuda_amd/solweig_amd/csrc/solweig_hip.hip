@@ -933,7 +933,11 @@ __device__ __forceinline__ GvfOut ground_view_fixed(const DomainG &G, const Pixe
 #pragma unroll
             for (int i = 0; i < n; ++i) {
                 if (EDGE) {
-                    if (s0 + i < E) last = buf[i];
+                    // last = s0 + i < E ? buf[i] : last, as a bit select (a plain conditional
+                    // becomes an exec-mask branch per step)
+                    const uint32_t keep = s0 + i < E ? 0xffffffffu : 0u;
+                    last.x = __uint_as_float((__float_as_uint(buf[i].x) & keep) | (__float_as_uint(last.x) & ~keep));
+                    last.y = __uint_as_float((__float_as_uint(buf[i].y) & keep) | (__float_as_uint(last.y) & ~keep));
                     buf[i] = last;
                 }
                 const float on = __builtin_amdgcn_fmed3f(rem - (float)i, 0.0f, 1.0f);
@@ -1768,7 +1772,8 @@ __device__ __forceinline__ void adj_walk_fixed(const DomainG &G, const AdjG &A, 
 #pragma unroll
             for (int i = 0; i < n; ++i) {
                 if (EDGE) {
-                    if (s0 + i < E) last = y[i];
+                    const uint32_t keep = s0 + i < E ? 0xffffffffu : 0u;   // bit select, no branch
+                    last = (y[i] & keep) | (last & ~keep);
                     y[i] = last;
                 }
                 cm = __builtin_amdgcn_alignbit(cm, y[i], 31);
