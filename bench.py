@@ -33,7 +33,7 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(hours=(10, 11, 12, 13), size=200):
+def cpu_baseline(hours=tuple(range(5, 21)), size=200):
     """The FP64 NumPy oracle (a port of the UMEP step) on the 200x200 c0 workload, a bounded
     sample of daytime hours after its static preparation, on one core."""
     os.environ.setdefault('OMP_NUM_THREADS', '1')
