@@ -245,7 +245,8 @@ typedef struct sw_adjoint {
     double *scalars;       /* [n_hours][SWA_SLOTS]                                             */
     double *patch;         /* [SWA_PATCH_SUMS][153]                                            */
     double *objective;     /* [2] sum of UTCI over ground pixels at the objective hour, count */
-    double *partial;       /* [SWA_PARTIAL_SLOTS][total_blocks] per-workgroup sums (work area)  */
+    double *partial;       /* [SWA_PARTIAL_SLOTS][total_blocks] per-workgroup sums (work area;
+                              NULL: one FP64 atomic per workgroup and slot instead, slower)      */
     int64_t reserved[5];
 } sw_adjoint;
 

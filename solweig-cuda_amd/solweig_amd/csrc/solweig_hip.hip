@@ -1579,11 +1579,14 @@ __device__ __forceinline__ float wave_sum(float v) {
 }
 
 // Sums the per-thread values v[0..N) over the workgroup and stores them as the workgroup's
-// partial sums of slots slot0 .. slot0 + N in part ([slot][total], one column per workgroup);
-// k_adj_reduce adds the columns up. Every thread must call. (Device-scope atomics from every wave
-// to the same few addresses serialise: 15.6k waves adding 3 values took 0.56 ms.)
+// partial sums of slots slot0 .. slot0 + N in A.partial ([slot][total], one column per
+// workgroup); k_adj_reduce adds the columns up. Every thread must call. (Device-scope atomics
+// from every wave to the same few addresses serialise: 15.6k waves adding 3 values took 0.56 ms.)
+// A record without the work area (partial NULL) gets one atomic add per workgroup and slot into
+// the hour's slots or the objective instead.
 template <int N>
-__device__ __forceinline__ void block_partials(AS1 double *part, int total, int slot0, const float (&v)[N]) {
+__device__ __forceinline__ void block_partials(const AdjG &A, int total, int hour, int slot0, const float (&v)[N]) {
+    AS1 double *part = A.partial;
     __shared__ float red[BLOCK / 64][N];
 #pragma unroll
     for (int i = 0; i < N; ++i) {
@@ -1595,7 +1598,14 @@ __device__ __forceinline__ void block_partials(AS1 double *part, int total, int 
         double s = 0.0;
 #pragma unroll
         for (int w = 0; w < BLOCK / 64; ++w) s += (double)red[w][i];
-        part[(long long)(slot0 + i) * total + blockIdx.x] = s;
+        const int slot = slot0 + i;
+        if (part) {
+            part[(long long)slot * total + blockIdx.x] = s;
+        } else if (s != 0.0) {
+            AS1 double *dst = slot < SWA_SLOTS ? A.scalars + (long long)hour * SWA_SLOTS + slot
+                                               : A.objective + (slot - SWA_SLOTS);
+            atomicAdd((double *)dst, s);
+        }
     }
 }
 
@@ -2057,7 +2067,7 @@ __global__ __launch_bounds__(BLOCK) void k_adj_prep(const sw_domain *__restrict_
     for (int i = 0; i < SWA_SLOTS; ++i) all[i] = sl[i];
     all[SWA_SLOTS] = obj[0];
     all[SWA_SLOTS + 1] = obj[1];
-    block_partials(A.partial, total, 0, all);
+    block_partials(A, total, hour, 0, all);
 }
 
 // Per-patch sums of lambda_Kdown and lambda_Kside over the pixels that see the patch as sky or
@@ -2292,7 +2302,7 @@ __global__ __launch_bounds__(BLOCK) void k_adj_gather(const sw_domain *__restric
             sl[1] = cls == 1u ? (float)(dtk * tg) + l_albsh * G.alb[q] : 0.f;
         }
     }
-    block_partials(A.partial, total, SWA_TA, sl);
+    block_partials(A, total, hour, SWA_TA, sl);
 }
 
 // Adds the workgroup partial sums of the slots in mask (one workgroup per slot) to the hour's
@@ -2302,6 +2312,7 @@ __global__ __launch_bounds__(256) void k_adj_reduce(const sw_adjoint *__restrict
     const int slot = (int)blockIdx.x;
     if (!((mask >> slot) & 1u)) return;
     const AdjG A = adjoint_at(adj_);
+    if (!A.partial) return;                           // the workgroups added their sums themselves
     const AS1 double *src = A.partial + (long long)slot * total;
     double s = 0.0;
     for (int i = (int)threadIdx.x; i < total; i += 256) s += src[i];

@@ -146,3 +146,21 @@ def test_gradient_independent_of_tile_width(rt):
         np.testing.assert_allclose(g2[key], g[key], rtol=1e-5, atol=1e-6 * float(np.abs(g[key]).max()))
     for name in ('Ta', 'RH', 'Ws', 'radG'):
         np.testing.assert_allclose(g2[name], g[name], rtol=1e-5, atol=1e-6 * float(np.abs(g[name]).max()))
+    # a record without the partial-sum work area: the workgroups add their sums with atomics
+    import ctypes
+    from solweig_amd import runtime
+    a = prob._adj_host
+    keep = a.partial
+    a.partial = 0
+    runtime.call('sw_memcpy_h2d', prob.adj_dev.ptr, ctypes.addressof(a), ctypes.sizeof(a), prob.stream.handle)
+    try:
+        g3 = prob.gradient()
+    finally:
+        a.partial = keep
+        runtime.call('sw_memcpy_h2d', prob.adj_dev.ptr, ctypes.addressof(a), ctypes.sizeof(a), prob.stream.handle)
+        prob.stream.sync()
+    assert abs(g3['J'] - g['J']) <= 1e-9 * abs(g['J'])
+    for name in ('Ta', 'RH', 'Ws', 'radG'):
+        np.testing.assert_allclose(g3[name], g[name], rtol=1e-5, atol=1e-6 * float(np.abs(g[name]).max()))
+    for name in ('albedo_b', 'trans'):
+        assert abs(g3[name] - g[name]) <= 1e-5 * abs(g[name]) + 1e-12, name
