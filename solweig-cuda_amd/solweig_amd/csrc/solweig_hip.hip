@@ -330,7 +330,10 @@ struct March {
 };
 
 constexpr int SKIP_WINDOW = 32;   // steps one test of the coarse height bound may skip
+constexpr int SKIP_WINDOW_MAX = 64;   // after a skip the next test tries a window twice as long
 constexpr int SKIP_TEST = 16;     // steps between tests (multiple of CHUNK)
+static_assert(SKIP_WINDOW % SKIP_TEST == 0 && SKIP_WINDOW_MAX % SKIP_TEST == 0 && SKIP_TEST % CHUNK == 0,
+              "tests stay on multiples of SKIP_TEST");
 constexpr int HCELL_SHIFT = 4;    // 16 x 16 cells of the coarse height bound
 
 extern "C" __device__ uint32_t __ockl_wfred_min_u32(uint32_t);
@@ -482,6 +485,7 @@ __global__ __launch_bounds__(BLOCK) void k_hour_shadow(const sw_domain *__restri
     m.ever = false;
     const bool all_lanes = true;
     const uint64_t active = __ballot(all_lanes);
+    int win = SKIP_WINDOW;                            // uniform: doubled after a skip, reset after a miss
     for (int k0 = 0; k0 < L; k0 += CHUNK) {
         if (k0 > 0 && (k0 & (SKIP_TEST - 1)) == 0) {
             // A lane's remaining steps matter only through samples above its threshold: while
@@ -494,12 +498,14 @@ __global__ __launch_bounds__(BLOCK) void k_hour_shadow(const sw_domain *__restri
                                       : (m.f > me.a ? (wall_px ? m.f : __builtin_huge_val()) : me.a);
             const double thr_min = wave_min_below(thr);
             if (thr_min == __builtin_huge_val()) break;
-            const int ke = (k0 + SKIP_WINDOW < L ? k0 + SKIP_WINDOW : L) - 1;
+            const int ke = (k0 + win < L ? k0 + win : L) - 1;
             if (window_below(G, at, off, k0, ke, rows, cols, dzs[k0 - 1], thr_min)) {
                 m.prev = dzs[ke];
                 k0 = ke + 1 - CHUNK;
+                win = win < SKIP_WINDOW_MAX ? 2 * win : SKIP_WINDOW_MAX;
                 continue;
             }
+            win = SKIP_WINDOW;
         }
         const int kl = k0 + CHUNK - 1 < L ? k0 + CHUNK - 1 : L - 1;
         const bool in_last = (unsigned)(at.r + off[2 * kl]) < (unsigned)rows &&
