@@ -467,6 +467,7 @@ __global__ __launch_bounds__(BLOCK) void k_hour_shadow(const sw_domain *__restri
     const bool tops = G.march_trees != nullptr;       // uniform: one domain per workgroup
     const AS1 float *mtrees = tops ? G.march_trees : trees;
     const bool wall_px = valid && G.walls[p] > 0.0f;
+    const double wall_h = (double)G.walls[p];
     const int start = uniform_index(H[SWH_RAY_START]);
     const int L = uniform_index(H[SWH_RAY_LEN]);
     const AS4 int32_t *off = G.ray_off + 2LL * start;
@@ -491,11 +492,14 @@ __global__ __launch_bounds__(BLOCK) void k_hour_shadow(const sw_domain *__restri
             // A lane's remaining steps matter only through samples above its threshold: while
             // not building-shadowed, its own height (f, the canopy test and topmax change only
             // through samples above it); once shadowed, veg is false for good (so ever and
-            // topmax no longer matter), and the running maximum f matters only on a wall pixel.
+            // topmax no longer matter), and the running maximum f matters only on a wall pixel:
+            // for the FP64 wall planes (FULL), else only while wall - (f - a) still rounds to
+            // the wall height (a sunlit wall needs wallsun == wall, and f only grows).
             // A window whose samples stay at or below every lane's threshold changes nothing;
             // when no lane has a threshold left, the march is over.
+            const bool wall_open = wall_px && (FULL || wall_h - (m.f - me.a) == wall_h);
             const double thr = !valid ? __builtin_huge_val()
-                                      : (m.f > me.a ? (wall_px ? m.f : __builtin_huge_val()) : me.a);
+                                      : (m.f > me.a ? (wall_open ? m.f : __builtin_huge_val()) : me.a);
             const double thr_min = wave_min_below(thr);
             if (thr_min == __builtin_huge_val()) break;
             const int ke = (k0 + win < L ? k0 + win : L) - 1;
