@@ -418,6 +418,27 @@ __device__ __forceinline__ void march_chunk(March &m, const AS1 float *dsm, cons
         av[i] = dsm[q];
         tv[i] = trees[q];
     }
+    if (TOPS) {
+        // no canopy top among the chunk's samples of any lane: top and bot are -inf, so topmax
+        // keeps its value and the pergola test fails at every step; only f, veg and ever move
+        bool canopy = false;
+#pragma unroll
+        for (int i = 0; i < CHUNK; ++i) canopy = canopy || tv[i] != -__builtin_huge_valf();
+        if (__ballot(canopy) == 0) {
+#pragma unroll
+            for (int i = 0; i < CHUNK; ++i) {
+                const bool live = !MASKED || (in[i] && k0 + i < L);
+                const double z = zv[i];
+                const double dz = live ? z : __builtin_huge_val();
+                const double a = (double)av[i] + raise;
+                m.f = fmax(m.f, a - dz);
+                m.prev = z;
+                m.veg = m.veg && !(m.f > me_a);
+                m.ever = m.ever || m.veg;
+            }
+            return;
+        }
+    }
 #pragma unroll
     for (int i = 0; i < CHUNK; ++i) {
         // a step outside the raster or past the ray's end (the padding) changes nothing: its dz
