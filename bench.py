@@ -101,7 +101,7 @@ def adjoint_leg(size=1000, repeats=3):
     # every timed call starts from a new parameter set (the hour tables are rebuilt); the gradient
     # after an objective of the same parameters reuses that objective's tables
     def tables():
-        prob._cached = None
+        prob.clear_cache()
         return prob._tables()
 
     def forward_day():
@@ -112,7 +112,7 @@ def adjoint_leg(size=1000, repeats=3):
         eng.stream.sync()
 
     def forward_adjoint():
-        prob._cached = None
+        prob.clear_cache()
         t0 = time.perf_counter()
         prob.objective()
         t1 = time.perf_counter()

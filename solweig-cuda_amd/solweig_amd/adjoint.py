@@ -108,6 +108,12 @@ class UtciGradient:
         c['trans'] = self.trans if trans is None else trans
         return c
 
+    def clear_cache(self):
+        """Forget the hour tables and the objective's lag planes (the next objective or gradient
+        rebuilds them, as after a change of parameters)."""
+        self._cached = None
+        self._ckpt_key = None
+
     def _tables(self):
         """Sun positions and hour tables of the current forcing and scalar parameters, built once
         per parameter set: the gradient after an objective of the same parameters reuses the
