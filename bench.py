@@ -26,7 +26,7 @@ B_ALG = 157.5          # algorithmic bytes per pixel-hour of the forward (static
 HBM_PEAK = 8.0e12      # MI355X HBM3E bytes/s
 # HBM bytes of one hour (shadow + pixel launch, mean over a day) of the 2000x2000 workload, from
 # rocprofv3 FETCH_SIZE (x 1.9999 copy calibration) + WRITE_SIZE; profiles/k_hour_pair_c2_2000x2000.md
-TRAFFIC_PER_HOUR_2000 = 0.976e9
+TRAFFIC_PER_HOUR_2000 = 0.977e9
 
 
 def log(msg):
