@@ -135,6 +135,27 @@ def test_day_matches_oracle(rt, kw):
     assert et < TOL_K and eu < TOL_K and nan_ok
 
 
+# a seeded sweep: sizes, latitudes from the equator to 60 N, with and without land cover, maps
+# and canopy
+SWEEP = [(30 + i, 26 + 3 * i, 7.5 * i, dict(landcover=i % 3 == 1, maps=i % 2 == 0, trees=i % 4 != 3))
+         for i in range(9)]
+
+
+@pytest.mark.parametrize('seed,size,lat,kw', SWEEP)
+def test_day_matches_oracle_sweep(rt, seed, size, lat, kw):
+    from solweig_amd import batched, sun as psun
+    d = _domain(size, seed, latitude=lat, **kw)
+    print(f'sweep: seed {seed}, {size}x{size}, latitude {lat}, {kw}', flush=True)
+    s = psun.met_sun(d.met, d.location, d.utc)
+    tmrt, utci = batched.run_day(d, sun=s)
+    p = oday.prepare(d.dsm, d.trees, d.dem, d.walls, d.aspect, d.latitude, landcover=d.landcover,
+                     alb_grid=d.alb_grid, emis_grid=d.emis_grid)
+    ref = oday.run_day(p, d.met, s, keep=('Tmrt',))
+    et, eu, nan_ok = _day_errors(tmrt, utci, ref)
+    print(f'  max |dTmrt| {et:.3e} K, max |dUTCI| {eu:.3e} K', flush=True)
+    assert et < TOL_K and eu < TOL_K and nan_ok
+
+
 @pytest.mark.parametrize('width', [16, 32])
 def test_day_matches_oracle_with_staged_tiles(rt, width):
     """96 x 96: the interior tiles walk the ground view from LDS with the compiled step shifts."""
