@@ -21,6 +21,12 @@ from .sun import met_sun
 
 class Engine:
     def __init__(self, domains, n_hours=24, stream=None, keep_outputs=True):
+        scales = {float(d.domain.scale if isinstance(d, PreparedDomain) else d.scale) for d in domains}
+        if len(scales) != 1:
+            # one launch shares one ground-view step table (the step shifts and counts follow the
+            # pixel size, umep_solweig.py:1466-1476)
+            raise ValueError(f'the domains of one engine must share one pixel size, got {sorted(scales)}; '
+                             f'run each pixel size in an engine of its own')
         self.domains = [d if isinstance(d, PreparedDomain) else PreparedDomain(d) for d in domains]
         self.n_hours = int(n_hours)
         self.stream = stream if stream is not None else rt.Stream()

@@ -22,7 +22,7 @@
 
 #include "../../../include/solweig_hip.h"
 
-static_assert(sizeof(sw_domain) == 320, "sw_domain must stay 37 eight-byte fields");
+static_assert(sizeof(sw_domain) == 320, "sw_domain must stay 40 eight-byte fields");
 
 namespace {
 

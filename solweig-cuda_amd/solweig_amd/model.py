@@ -236,6 +236,11 @@ def prepare_shadow(a, walls, aspect, scale, *, vegdem=None, vegdem2=None, bush=N
     dsm32 = a.astype(np.float32)
     if not np.array_equal(dsm32.astype(np.float64), a):
         raise ValueError('prepare_shadow: the DSM must be FP32-exact')
+    if not (a.min() >= 0.0):
+        # the march skips windows against a wave-wide lower bound of non-negative thresholds and
+        # treats off-raster steps as no-ops, both exact only for heights >= 0
+        raise ValueError('prepare_shadow: the DSM has heights below 0; raise it to a minimum of 0 first, '
+                         'as the series preparation does (series.py:252-257)')
     use = vegdem is not None
     if use:
         top = np.asarray(_host(vegdem), dtype=np.float64)

@@ -155,6 +155,8 @@ class PreparedDomain:
         # The derived planes are formed on the host with NumPy so that tan(asvf) - which decides
         # the sunlit/shaded wall split - is the same number the NumPy model uses.
         planes = sv.planes.to_host().reshape(16, rows, cols)
+        sv.planes.free()                     # 128 B per pixel; the host copy is all that is read
+        sv.planes = None
         svf = planes[0].astype(np.float32).astype(np.float64)
         svfveg = planes[5].astype(np.float32).astype(np.float64)
         self._svf, self._svfveg = svf, svfveg

@@ -1,73 +1,84 @@
 """Sky view factors on the device (drop-in for svf.svf_calculator / compute_diffsh, svf.py:81-131).
 
-Ray tables are built on the host exactly as svf.py:19-78 builds them (round-half-even shifts,
-FP64 heights, the first failing step included); the device marches every (pixel, patch) pair
-and keeps the three visibility masks as bits (3 x 153 bits per pixel instead of three FP32
-[rows, cols, 153] arrays), then reduces them to the 15 SVF planes.
+A sky patch is seen from a pixel exactly as the sun is: the visibility march of a patch is the
+shadow march of a sun standing at the patch centre, with the step index starting at 0 and the
+first failing step included (svf.py:19-78 builds its tables with the same step rule as the
+shadow loop of umep_solweig.py:2286-2358). The per-patch ray tables are therefore the hourly sun
+rays of ``sky.shadow_ray`` evaluated at the 153 patch centres. The device marches every
+(pixel, patch) pair and keeps the three visibility masks as bits (3 x 153 bits per pixel instead
+of three FP32 [rows, cols, 153] arrays), then reduces them to the 15 SVF planes.
 """
 from __future__ import annotations
 
 import numpy as np
 
 from . import runtime as rt
-from .sky import BAND_ALTITUDES, BAND_COUNTS, BAND_STARTS
+from .sky import BAND_COUNTS, patches as patch_centres, shadow_ray
 
-_EDGES = (0, 12, 24, 36, 48, 60, 72, 84, 90)
 # kernel plane order -> UMEP tuple order (svf.py:113-114)
 UMEP_ORDER = (0, 10, 1, 11, 6, 4, 14, 9, 2, 12, 7, 5, 3, 13, 8)
 UMEP_NAMES = ('svf', 'svfaveg', 'svfE', 'svfEaveg', 'svfEveg', 'svfN', 'svfNaveg', 'svfNveg',
               'svfS', 'svfSaveg', 'svfSveg', 'svfveg', 'svfW', 'svfWaveg', 'svfWveg')
+ANNULI_PER_BAND = 12          # 1-degree annuli summed into each band's weight; the zenith band has 6
+
+
+def annulus_weight(k):
+    """Cosine-weighted solid-angle share of the 1-degree annulus k (1..90 from the zenith), FP64
+    in the operation order of svf.py:40-46: sin(1 deg) sin((2 (91 - k) - 1) deg) / (2 pi)."""
+    ring = 91. - float(k)
+    return (1. / (2. * np.pi)) * np.sin(np.pi / 180.) * np.sin((np.pi * (2. * ring - 1.)) / 180.)
+
+
+def band_weights(counts=BAND_COUNTS):
+    """[bands, 2] FP64: per band the weight of one patch over the full circle (column 0) and over
+    a half circle (column 1, the E/S/W/N sector planes), each the running sum over the band's
+    annuli in zenith-to-horizon order. Bands span 12 annuli; the last one the remaining 6."""
+    out = np.zeros((len(counts), 2))
+    first = 1
+    for b, n in enumerate(counts):
+        last = min(90, first + ANNULI_PER_BAND - 1)
+        full = (360. / n) * (np.pi / 180.)
+        half = (360. / np.ceil(n / 2.)) * (np.pi / 180.)
+        for k in range(first, last + 1):
+            w = annulus_weight(k)
+            out[b, 0] += full * w
+            out[b, 1] += half * w
+        first = last + 1
+    return out
+
+
+def sector_bits(azimuth):
+    """uint8 per patch: bit d (0 E, 1 S, 2 W, 3 N) is set when the azimuth lies in the half circle
+    centred on that compass direction, i.e. in quadrant d or d + 1 of [0, 360) (quadrant q covers
+    [90 q, 90 (q + 1)) degrees; svf.py:36-39)."""
+    az = np.asarray(azimuth, dtype=np.float64)
+    quadrant = (az >= 90.).astype(np.int32) + (az >= 180.) + (az >= 270.)
+    bits = np.zeros(az.shape, np.uint8)
+    for d in range(4):
+        bits |= (((quadrant == d) | (quadrant == (d + 1) % 4)).astype(np.uint8) << d)
+    return bits
 
 
 def ray_tables(rows, cols, amaxvalue, scale):
-    """Per-patch ray offsets/heights/lengths, sector bits and band weights (svf.py:19-78)."""
-    counts = np.asarray(BAND_COUNTS, dtype=np.int32)
-    band = np.repeat(np.arange(len(counts), dtype=np.int32), counts)
-    band_offsets = np.r_[0, np.cumsum(counts)].astype(np.int32)
-    azimuth = np.concatenate([np.arange(n) * (360. / n) + s for n, s in zip(counts, BAND_STARTS)])
-    azimuth[azimuth > 360.] -= 360.
-    altitude = np.asarray(BAND_ALTITUDES)[band].astype(np.float64)
-    sectors = ((azimuth >= 0) & (azimuth < 180)).astype(np.uint8)
-    sectors |= (((azimuth >= 90) & (azimuth < 270)).astype(np.uint8) << 1)
-    sectors |= (((azimuth >= 180) & (azimuth < 360)).astype(np.uint8) << 2)
-    sectors |= (((azimuth >= 270) | (azimuth < 90)).astype(np.uint8) << 3)
-    weights = np.zeros((len(counts), 2), dtype=np.float64)
-    for b, n in enumerate(counts):
-        for k in np.arange(_EDGES[b] + 1, _EDGES[b + 1] + 1):
-            annulus = 91. - k
-            w = (1. / (2. * np.pi)) * np.sin(np.pi / 180.) * np.sin((np.pi * (2. * annulus - 1.)) / 180.)
-            weights[b, 0] += ((360. / n) * (np.pi / 180.)) * w
-            weights[b, 1] += ((360. / np.ceil(n / 2.)) * (np.pi / 180.)) * w
-    patches = len(azimuth)
-    capacity = max(rows, cols) + 1
-    idx = np.arange(capacity, dtype=np.float64)
-    offsets = np.zeros((patches, capacity, 2), dtype=np.int32)
-    heights = np.zeros((patches, capacity), dtype=np.float64)
-    lengths = np.zeros(patches, dtype=np.int32)
-    for p, (azi, alt) in enumerate(zip(azimuth, altitude)):
-        if amaxvalue < 0:
-            continue
-        az, el = azi * (np.pi / 180.), alt * (np.pi / 180.)
-        sn, cs, tn = np.sin(az), np.cos(az), np.tan(az)
-        if np.pi / 4 <= az < 3 * np.pi / 4 or 5 * np.pi / 4 <= az < 7 * np.pi / 4:
-            dy = np.sign(sn) * idx
-            dx = -np.sign(cs) * np.abs(np.round(idx / tn))
-            ds = np.abs(1. / sn)
-        else:
-            dy = np.sign(sn) * np.abs(np.round(idx * tn))
-            dx = -np.sign(cs) * idx
-            ds = np.abs(1. / cs)
-        dz = (ds * idx) * (np.tan(el) / scale)
-        bad = np.flatnonzero((dz > amaxvalue) | (np.abs(dx) >= rows) | (np.abs(dy) >= cols))
-        n = int(bad[0]) + 1 if len(bad) else capacity
-        lengths[p] = n
-        offsets[p, :n, 0], offsets[p, :n, 1] = dx[:n], dy[:n]
-        heights[p, :n] = dz[:n]
+    """Per-patch ray offsets/heights/lengths, sector bits and band weights of the 153-patch sky
+    (the tables of svf.py:19-78): patch p's ray is the sun ray of sky.shadow_ray at the patch
+    centre (altitude, azimuth), indices from 0, cut after the first step whose height exceeds
+    ``amaxvalue`` or whose shift leaves the raster. Rays are left-aligned in a [153, capacity]
+    table, capacity = the longest ray."""
+    altitude, azimuth = patch_centres()
+    azimuth = np.where(azimuth > 360., azimuth - 360., azimuth)
+    rays = [shadow_ray(z, a, scale, amaxvalue, (rows, cols), first=0)[:2] for a, z in zip(altitude, azimuth)]
+    lengths = np.array([len(dz) for _, dz in rays], dtype=np.int32)
     capacity = max(1, int(lengths.max()))
-    return dict(band_offsets=band_offsets, sectors=sectors, weights=weights,
-                offsets=np.ascontiguousarray(offsets[:, :capacity]),
-                heights=np.ascontiguousarray(heights[:, :capacity]), lengths=lengths,
-                patches=patches, bands=len(counts), capacity=capacity)
+    offsets = np.zeros((len(rays), capacity, 2), dtype=np.int32)
+    heights = np.zeros((len(rays), capacity), dtype=np.float64)
+    for p, (off, dz) in enumerate(rays):
+        offsets[p, :len(dz)] = off
+        heights[p, :len(dz)] = dz
+    counts = np.asarray(BAND_COUNTS, dtype=np.int32)
+    return dict(band_offsets=np.concatenate([[0], np.cumsum(counts)]).astype(np.int32),
+                sectors=sector_bits(azimuth), weights=band_weights(counts), offsets=offsets,
+                heights=heights, lengths=lengths, patches=len(rays), bands=len(counts), capacity=capacity)
 
 
 class SkyView:
@@ -80,6 +91,9 @@ class SkyView:
 
     def umep_tuple(self):
         """Host planes in the order svf_calculator returns them (svf.py:113-114)."""
+        if self.planes is None:
+            raise ValueError('the FP64 sky-view planes were released after the static preparation '
+                             '(PreparedDomain(keep_host=True) keeps a host copy)')
         v = self.planes.to_host().reshape(16, *self.shape)
         return tuple(v[i] for i in UMEP_ORDER)
 

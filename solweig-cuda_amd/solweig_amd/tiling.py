@@ -105,6 +105,28 @@ def tile_jobs(case, settings, tiles_dir):
             for t in tiles]
 
 
+# Device bytes that one pixel of a prepared domain keeps for the whole run (static.PreparedDomain):
+# rasters and maps 37 (dsm, trees, walls, aspect, albedo, emissivity, TgK, Tstart, TmaxLST f32,
+# ground u8), visibility bits 60, sky-view derived planes 24 (svfbuveg, svfveg f32, tan(asvf),
+# tan(svfalfa) f64), neighbour record, mask byte and lag state 33, static sky sums 40, march canopy
+# plane 4, ground-view planes 80 (step counts, unshadowed albedo sums, in-raster step counts).
+STATIC_BYTES_PER_PIXEL = 278
+# Default pixel budget of one batch: at 24 h and 2 output slots about 85 GB of device memory and
+# 38 GB of page-locked host memory (batch_bytes).
+DEFAULT_BATCH_PIXELS = 100_000_000
+
+
+def batch_bytes(pixels, n_hours=24, slots=2):
+    """(device bytes, page-locked host bytes) that a batch of ``pixels`` input pixels holds while
+    it runs: the static state, the day's Tmrt and UTCI (4 B each per hour) and, per output slot,
+    one device staging copy and one page-locked host copy of them (output.DayOutputs). The sky-view
+    pass adds 128 B per pixel of the domain being prepared while it runs."""
+    per_day = 8 * int(n_hours)
+    device = int(pixels) * (STATIC_BYTES_PER_PIXEL + per_day * (1 + int(slots)))
+    host = int(pixels) * per_day * int(slots)
+    return device, host
+
+
 def _groups(tiles, max_pixels):
     group, size = [], 0
     for t in tiles:
@@ -120,9 +142,10 @@ def _groups(tiles, max_pixels):
 
 def run_tiles(tiles, met_file, output_path, longitude, latitude, *, utc_offset_hours=None, site_altitude_m=3.,
               dsm_wall_threshold_m=3., pixel_size_m=1.0, save_tmrt=True, no_write=False, output_queue_size=2,
-              max_batch_pixels=250_000_000, progress=False):
+              max_batch_pixels=DEFAULT_BATCH_PIXELS, progress=False):
     """Simulate every tile of ``tiles`` (``create_tiles`` output) over the forcing, the tiles of a
-    group (up to ``max_batch_pixels`` input pixels) as one batch of the batched engine. Outputs per
+    group (up to ``max_batch_pixels`` input pixels; ``batch_bytes`` gives the device and
+    page-locked host memory a group holds) as one batch of the batched engine. Outputs per
     tile and day, cropped to the tile's core: ``output_path/<tile_id>/<date>/UTCI.npy`` (and
     TMRT.npy), as run_timeseries writes them with ``output_core``. Returns {tile_id: {'days':
     {date: {var: [path]}}}} (with ``no_write``, the cropped host arrays instead of paths)."""

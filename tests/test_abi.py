@@ -93,3 +93,18 @@ def test_compiled_ground_view_steps_match_geometry(scale):
     np.testing.assert_array_equal(out, off[:, :n])
     with pytest.raises(RuntimeError):
         rt.call('sw_gvf_fixed_steps', out.ctypes.data, 23)
+
+
+def test_bindings_refuse_a_library_of_another_record_size(monkeypatch):
+    """A library built for another sw_domain layout (a stale .so beside newer bindings) is refused
+    at load time instead of reading domain records at the wrong stride."""
+    if not os.path.exists(rt.LIBRARY):
+        pytest.skip('library not built')
+
+    class Wider(ctypes.Structure):
+        _fields_ = rt.SwDomain._fields_ + [('extra', ctypes.c_int64)]
+
+    monkeypatch.setattr(rt, '_lib', None)
+    monkeypatch.setattr(rt, 'SwDomain', Wider)
+    with pytest.raises(rt.SolweigHipError, match='320-byte .* records, these bindings use 328'):
+        rt.lib()
