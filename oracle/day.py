@@ -47,20 +47,40 @@ def tg_maps(lc):
     return out, (wall[3], wall[4], wall[5])
 
 
-def prepare(dsm, trees, dem, walls, aspect, latitude, *, scale=1.0, landcover=None, alb_grid=None,
-            emis_grid=None, svf=None):
-    """Static inputs of the step (series.py:252-365). ``svf`` may carry precomputed
-    (15-tuple + vegmat, vbmat, shmat, total) from ``oracle.svf.svf_calculator``."""
+def shadow_inputs(dsm, trees, dem, walls, aspect):
+    """The static inputs of the sun-ray march only (series.py:252-292): raised DSM and DEM,
+    vegetation DSMs, bush, amaxvalue, ground mask, walls and aspect (degrees)."""
     a = np.asarray(dsm, dtype=np.float64)
     t = np.asarray(trees, dtype=np.float64).copy()
     g = np.asarray(dem, dtype=np.float64)
-    rows, cols = a.shape
     dsm_raise = max(0.0, -float(a.min()))
     dem_raise = max(0.0, -float(g.min()))
     if dsm_raise:
         a = a + dsm_raise
     if dem_raise:
         g = g + dem_raise
+    t[t < 0.] = 0.
+    vegdsm = t + a
+    vegdsm[vegdsm == a] = 0
+    vegdsm2 = t * 0.25 + a
+    vegdsm2[vegdsm2 == a] = 0
+    bush = np.logical_not(vegdsm2 * vegdsm) * vegdsm
+    amaxvalue = max(a.max() - a.min(), t.max())
+    buildings = a - g
+    buildings[buildings < 2.] = 1.
+    buildings[buildings >= 2.] = 0.
+    return dict(dsm=a, dem=g, trees=t, vegdem=vegdsm, vegdem2=vegdsm2, bush=bush, amaxvalue=float(amaxvalue),
+                buildings=buildings, walls=np.asarray(walls, dtype=np.float64),
+                dirwalls=np.asarray(aspect, dtype=np.float64))
+
+
+def prepare(dsm, trees, dem, walls, aspect, latitude, *, scale=1.0, landcover=None, alb_grid=None,
+            emis_grid=None, svf=None):
+    """Static inputs of the step (series.py:252-365). ``svf`` may carry precomputed
+    (15-tuple + vegmat, vbmat, shmat, total) from ``oracle.svf.svf_calculator``."""
+    q = shadow_inputs(dsm, trees, dem, walls, aspect)
+    a, t, g = q['dsm'], q['trees'], q['dem']
+    rows, cols = a.shape
     p = dict(CONSTANTS)
     z = np.zeros((rows, cols), dtype=np.float32)
     if landcover is not None:
@@ -83,16 +103,8 @@ def prepare(dsm, trees, dem, walls, aspect, latitude, *, scale=1.0, landcover=No
         p['alb_grid'] = np.asarray(alb_grid, dtype=np.float32).astype(np.float64)
     if emis_grid is not None:
         p['emis_grid'] = np.asarray(emis_grid, dtype=np.float32).astype(np.float64)
-    t[t < 0.] = 0.
-    vegdsm = t + a
-    vegdsm[vegdsm == a] = 0
-    vegdsm2 = t * 0.25 + a
-    vegdsm2[vegdsm2 == a] = 0
-    bush = np.logical_not(vegdsm2 * vegdsm) * vegdsm
-    amaxvalue = max(a.max() - a.min(), t.max())
-    buildings = a - g
-    buildings[buildings < 2.] = 1.
-    buildings[buildings >= 2.] = 0.
+    vegdsm, vegdsm2, bush, amaxvalue, buildings = (q[k] for k in ('vegdem', 'vegdem2', 'bush', 'amaxvalue',
+                                                                   'buildings'))
     if svf is None:
         svf = osvf.svf_calculator(amaxvalue, a, vegdsm, vegdsm2, bush, scale, p['trans'])
     planes = tuple(np.asarray(v).astype(np.float32).astype(np.float64) for v in svf[:15])
@@ -105,8 +117,8 @@ def prepare(dsm, trees, dem, walls, aspect, latitude, *, scale=1.0, landcover=No
     with np.errstate(divide='ignore'):
         svfalfa = np.arcsin(np.exp(np.log(1.0 - tmp) / 2.0))
     p.update(dsm=a, dem=g, vegdem=vegdsm, vegdem2=vegdsm2, bush=bush, amaxvalue=float(amaxvalue),
-             buildings=buildings, walls=np.asarray(walls, dtype=np.float64),
-             dirwalls=np.asarray(aspect, dtype=np.float64), scale=float(scale), latitude=float(latitude),
+             buildings=buildings, walls=q['walls'], dirwalls=q['dirwalls'], scale=float(scale),
+             latitude=float(latitude),
              svf=s, svfaveg=saveg, svfE=sE, svfEaveg=sEaveg, svfEveg=sEveg, svfN=sN, svfNaveg=sNaveg,
              svfNveg=sNveg, svfS=sS, svfSaveg=sSaveg, svfSveg=sSveg, svfveg=sveg, svfW=sW,
              svfWaveg=sWaveg, svfWveg=sWveg, svfbuveg=svfbuveg, asvf=asvf, svfalfa=svfalfa,

@@ -115,3 +115,72 @@ def test_height_bound_block_maxima():
         rb, cb = r1 - (1 << i) + 1, c1 - (1 << j) + 1
         q = max(M[r0, c0], M[r0, cb], M[rb, c0], M[rb, cb])
         assert q == hb[0, 0, r0:r1 + 1, c0:c1 + 1].max()
+
+
+def _window_box(wave, off, k0, ke, rows, cols, hb):
+    """CPU restatement of the march's window test geometry (window_below): the raster box spanned
+    by the wave's pixels shifted by the window's first and last step, its 16 x 16 cells, and the
+    height bound over them (four blocks of one level for boxes of at most 8 x 8 cells, else a
+    loop over the cells). None when the box leaves the raster entirely."""
+    wr0, wr1, wc0, wc1 = wave
+    dr0, dr1, dc0, dc1 = off[k0, 0], off[ke, 0], off[k0, 1], off[ke, 1]
+    r0, r1 = max(wr0 + min(dr0, dr1), 0), min(wr1 + max(dr0, dr1), rows - 1)
+    c0, c1 = max(wc0 + min(dc0, dc1), 0), min(wc1 + max(dc0, dc1), cols - 1)
+    if r0 > r1 or c0 > c1:
+        return None
+    r0c, r1c, c0c, c1c = r0 >> 4, r1 >> 4, c0 >> 4, c1 >> 4
+    h, w = r1c - r0c + 1, c1c - c0c + 1
+    if h <= 8 and w <= 8:
+        i = 2 if h >= 4 else (1 if h >= 2 else 0)
+        j = 2 if w >= 4 else (1 if w >= 2 else 0)
+        M = hb[i, j]
+        rb, cb = r1c - (1 << i) + 1, c1c - (1 << j) + 1
+        hm = max(M[r0c, c0c], M[r0c, cb], M[rb, c0c], M[rb, cb])
+    else:
+        hm = hb[0, 0, r0c:r1c + 1, c0c:c1c + 1].max()
+    return (r0, r1, c0, c1), float(hm)
+
+
+def test_window_box_bounds_every_sample_of_the_window():
+    """For waves of every tile width (a wave is 64 lanes: 64 / W rows of W columns, or 64 columns
+    of one row) and windows of 32 and 64 steps along sun rays at many azimuths and low altitudes,
+    every in-raster sample of every lane lies inside the window's box, and the box's height bound
+    is at least max(a, canopy top) of each of them: a window the test skips holds no sample above
+    the wave's lowest threshold."""
+    from solweig_amd.static import height_bound
+    rng = np.random.default_rng(11)
+    rows, cols = 300, 233
+    a = np.where(rng.random((rows, cols)) < 0.3, rng.uniform(0, 50, (rows, cols)), 0.0)
+    t = np.where(rng.random((rows, cols)) < 0.2, rng.uniform(0, 12, (rows, cols)), 0).astype(np.float32)
+    top = np.maximum(a, t.astype(np.float64) + a)
+    hb = height_bound(a, t)
+    windows = 0
+    for az in np.arange(3.0, 360.0, 29.0):
+        for alt in (2.0, 6.0, 25.0):
+            off, dz, *_ = sky.shadow_ray(az, alt, 1.0, 60.0, (rows, cols))
+            L = len(dz)
+            for tx in (16, 32, 64, 128):
+                ty = 256 // tx
+                tr0 = int(rng.integers(0, -(-rows // ty))) * ty
+                tc0 = int(rng.integers(0, -(-cols // tx))) * tx
+                w64 = int(rng.integers(0, 4)) * 64
+                sh = tx.bit_length() - 1
+                wr0, wr1 = tr0 + (w64 >> sh), tr0 + ((w64 + 63) >> sh)
+                wc0 = tc0 + (w64 & (tx - 1)) if tx > 64 else tc0
+                wc1 = wc0 + 63 if tx > 64 else tc0 + tx - 1
+                lanes = [(min(r, rows - 1), min(c, cols - 1)) for r in range(wr0, wr1 + 1) for c in range(wc0, wc1 + 1)]
+                for k0 in range(16, L, 16):
+                    for win in (32, 64):
+                        ke = min(k0 + win, L) - 1
+                        got = _window_box((wr0, wr1, wc0, wc1), off, k0, ke, rows, cols, hb)
+                        windows += 1
+                        for r, c in lanes:
+                            for k in range(k0, ke + 1):
+                                rr, cc = r + off[k, 0], c + off[k, 1]
+                                if not (0 <= rr < rows and 0 <= cc < cols):
+                                    continue
+                                assert got is not None, (az, alt, tx, k0)
+                                (r0, r1, c0, c1), hm = got
+                                assert r0 <= rr <= r1 and c0 <= cc <= c1, (az, alt, tx, k0, k)
+                                assert hm >= top[rr, cc]
+    assert windows > 1000
