@@ -337,6 +337,21 @@ int sw_gvf(const void *const *fields, const long long *counts, int n_fields, int
            int second_steps, const double *params, int n_params, int real_bytes, void *work_dev,
            void *out_dev, void *stream);
 
+/* sw_gvf without the work buffer: no counterpart in the reference. For callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc). The per-pixel samples go to a library-owned static device array of
+ * SW_GVF_SCRATCH_BYTES (64 MiB, held by every GPU the library runs on until the process ends):
+ * rows * cols * 8 working-precision values must fit, i.e. at most 2097152 pixels in FP32 and
+ * 1048576 in FP64; larger rasters return SW_ERR_LIMIT before any device work. Arguments and
+ * results are those of sw_gvf. Every call on one GPU shares that GPU's array: calls that
+ * overlap on one GPU (other streams, other host threads) race, so serialise them. The device
+ * must be current (sw_set_device) on the calling thread.                                  */
+#define SW_GVF_SCRATCH_BYTES (64ull << 20)
+int sw_gvf_noalloc(const void *const *fields, const long long *counts, int n_fields, int rows, int cols,
+                   const int32_t *offsets_dev, const double *bounds_dev, int step_stride, int first_steps,
+                   int second_steps, const double *params, int n_params, int real_bytes, void *out_dev,
+                   void *stream);
+
 /* Visibility flags of a caller's patch geometry (replaces prepare_geometry's pack,
  * model.py:273-306, geometry.cu:1-8): shmat, vegshmat, vbshvegshmat are [n][patches] device
  * arrays of mask_bytes-sized values (1: uint8, 4: float, 8: double). flags_dev receives

@@ -17,6 +17,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <type_traits>
 
@@ -3089,6 +3090,39 @@ int launch_adjoint_hour(const sw_domain *dom, const int32_t *bs, int total, int 
 
 }  // namespace
 
+// Scratch of sw_gvf_noalloc: the per-pixel neighbour samples of the ground-view walk, in a
+// library-owned device array instead of a caller's work buffer. Host code never passes the
+// array's name to a kernel (on the host it is a stand-in, not a device address): it looks up
+// the array's address on each device once (hipGetSymbolAddress, with that device current) and
+// passes that pointer.
+static __device__ __align__(256) unsigned char scratch[SW_GVF_SCRATCH_BYTES];
+
+namespace {
+
+constexpr int SCRATCH_DEVICES = 64;
+std::mutex g_scratch_lock;
+void *g_scratch_ptr[SCRATCH_DEVICES] = {};
+
+// The device address of ``scratch`` on the current device (looked up on the first call there).
+int scratch_on_current_device(void **out) {
+    int dev = -1;
+    int rc = hip_check(hipGetDevice(&dev), "hipGetDevice");
+    if (rc) return rc;
+    if (dev < 0 || dev >= SCRATCH_DEVICES) return fail(SW_ERR_LIMIT, "sw_gvf_noalloc: device index beyond 63");
+    std::lock_guard<std::mutex> lock(g_scratch_lock);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        rc = hip_check(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+        if (rc) return rc;
+        if (!p) return fail(SW_ERR_HIP, "hipGetSymbolAddress(scratch) returned NULL");
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
+    return SW_OK;
+}
+
+}  // namespace
+
 extern "C" {
 
 int sw_version(void) { return 100; }
@@ -3342,6 +3376,28 @@ int sw_gvf(const void *const *fields, const long long *counts, int n_fields, int
     const hipStream_t s = (hipStream_t)stream;
     return real_bytes == 8 ? launch_gvf<double>(F, rows, cols, offsets, stride, bounds, first, second, params, work, out, s)
                            : launch_gvf<float>(F, rows, cols, offsets, stride, bounds, first, second, params, work, out, s);
+}
+
+int sw_gvf_noalloc(const void *const *fields, const long long *counts, int n_fields, int rows, int cols,
+                   const int32_t *offsets, const double *bounds, int stride, int first, int second,
+                   const double *params, int n_params, int real_bytes, void *out, void *stream) {
+    if (!fields || !counts || !offsets || !bounds || !params || !out)
+        return fail(SW_ERR_ARGUMENT, "sw_gvf_noalloc: NULL pointer");
+    if (real_bytes != 4 && real_bytes != 8) return fail(SW_ERR_ARGUMENT, "sw_gvf_noalloc: precision is 4 or 8 bytes");
+    if (rows <= 0 || cols <= 0) return fail(SW_ERR_ARGUMENT, "sw_gvf_noalloc: bad raster");
+    // the samples need rows * cols * 8 values of the working precision (GvfSample)
+    const unsigned long long need = (unsigned long long)rows * (unsigned long long)cols * 8ull * (unsigned)real_bytes;
+    if (need > SW_GVF_SCRATCH_BYTES)
+        return fail(SW_ERR_LIMIT, "sw_gvf_noalloc: " + std::to_string(rows) + " x " + std::to_string(cols) +
+                                      " pixels need " + std::to_string(need) + " bytes of samples; the library's " +
+                                      "scratch holds " + std::to_string(SW_GVF_SCRATCH_BYTES) +
+                                      " (use sw_gvf with a work buffer)");
+    static_assert(sizeof(GvfSample<float>) == 8 * 4 && sizeof(GvfSample<double>) == 8 * 8, "GvfSample is 8 values");
+    void *work = nullptr;
+    const int rc = scratch_on_current_device(&work);
+    if (rc) return rc;
+    return sw_gvf(fields, counts, n_fields, rows, cols, offsets, bounds, stride, first, second, params, n_params,
+                  real_bytes, work, out, stream);
 }
 
 int sw_pack_visibility(const void *sh, const void *veg, const void *vb, int mask_bytes, long long n, int patches,

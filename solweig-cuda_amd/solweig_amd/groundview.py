@@ -19,16 +19,17 @@ from . import runtime as rt
 from . import sky
 from .flux import field_table, working_dtype
 
-_SIG = []
+_SIG = set()
 
 
-def _entry():
-    f = rt.lib().sw_gvf
-    if not _SIG:
+def _entry(name='sw_gvf'):
+    f = getattr(rt.lib(), name)
+    if name not in _SIG:
         vp, ci = ctypes.c_void_p, ctypes.c_int
-        f.argtypes = [vp, vp, ci, ci, ci, vp, vp, ci, ci, ci, vp, ci, ci, vp, vp, vp]
+        work = [vp] if name == 'sw_gvf' else []
+        f.argtypes = [vp, vp, ci, ci, ci, vp, vp, ci, ci, ci, vp, ci, ci] + work + [vp, vp]
         f.restype = ci
-        _SIG.append(True)
+        _SIG.add(name)
     return f
 
 
@@ -74,8 +75,10 @@ def prepare_gvf_geometry(walls, buildings, dirwalls, scale, first, second):
 
 
 def gvf(geometry, wallsun, shadow, Tg, Tgwall, Ta, emis_grid, ewall, alb_grid, SBC, albedo_b, Twater=None,
-        lc_grid=None, landcover=0, *, dtype=np.float64):
-    """Ground view factors of one step (model.py:569-622) -> GvfResult."""
+        lc_grid=None, landcover=0, *, dtype=np.float64, no_alloc=False):
+    """Ground view factors of one step (model.py:569-622) -> GvfResult. ``no_alloc``: the walk's
+    samples go to the library's static scratch (sw_gvf_noalloc, rasters up to 2097152 pixels in
+    FP32, 1048576 in FP64) instead of a work buffer allocated for the call."""
     g = geometry
     dtype = working_dtype(dtype)
     landcover = int(float(landcover))
@@ -89,12 +92,17 @@ def gvf(geometry, wallsun, shadow, Tg, Tgwall, Ta, emis_grid, ewall, alb_grid, S
     n = rows * cols
     values = (g.buildings, g.walls, g.aspect, wallsun, shadow, Tg, Tgwall, emis_grid, alb_grid, lc_grid)
     P, C, keep = field_table(values, n, dtype, fp64_fields=(0, 1, 2, 3, 9))
-    work = rt.DeviceArray((n, 8), dtype)
     out = rt.DeviceArray((18, rows, cols), dtype)
     params = (float(Ta), float(ewall), float(SBC), float(albedo_b), float(landcover), twater)
     Q = (ctypes.c_double * 6)(*params)
-    rt.check(_entry()(P, C, len(values), rows, cols, g.offsets.ptr, g.shade_bounds.ptr, g.step_stride,
-                      g.first_steps, g.second_steps, Q, 6, dtype.itemsize, work.ptr, out.ptr, None))
+    head = (P, C, len(values), rows, cols, g.offsets.ptr, g.shade_bounds.ptr, g.step_stride, g.first_steps,
+            g.second_steps, Q, 6, dtype.itemsize)
+    if no_alloc:
+        work = None
+        rt.check(_entry('sw_gvf_noalloc')(*head, out.ptr, None))
+    else:
+        work = rt.DeviceArray((n, 8), dtype)
+        rt.check(_entry()(*head, work.ptr, out.ptr, None))
     rt.check(rt.lib().sw_stream_sync(None))
     del keep, work
     return GvfResult(tuple(out.plane(k, g.shape) for k in range(17)), out.plane(17, g.shape))

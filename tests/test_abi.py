@@ -108,3 +108,24 @@ def test_bindings_refuse_a_library_of_another_record_size(monkeypatch):
     monkeypatch.setattr(rt, 'SwDomain', Wider)
     with pytest.raises(rt.SolweigHipError, match='320-byte .* records, these bindings use 328'):
         rt.lib()
+
+
+@pytest.mark.parametrize('rows,cols,real_bytes', [(2048, 1025, 4), (1025, 1024, 8), (4000, 4000, 4)])
+def test_noalloc_gvf_refuses_rasters_beyond_its_scratch(rows, cols, real_bytes):
+    """The size check runs on the host before any device call (no GPU needed): 64 MiB of samples
+    hold 2048 x 1024 pixels in FP32 and 1024 x 1024 in FP64."""
+    if not os.path.exists(rt.LIBRARY):
+        pytest.skip('library not built')
+    L = rt.lib()
+    f = L.sw_gvf_noalloc
+    vp, ci = ctypes.c_void_p, ctypes.c_int
+    f.argtypes = [vp, vp, ci, ci, ci, vp, vp, ci, ci, ci, vp, ci, ci, vp, vp]
+    f.restype = ci
+    dummy = ctypes.create_string_buffer(64)
+    ptrs = (ctypes.c_void_p * 10)(*([ctypes.addressof(dummy)] * 10))
+    counts = (ctypes.c_longlong * 10)(*([1] * 10))
+    params = (ctypes.c_double * 6)()
+    rc = f(ptrs, counts, 10, rows, cols, ctypes.addressof(dummy), ctypes.addressof(dummy), 22, 1, 22, params, 6,
+           real_bytes, ctypes.addressof(dummy), None)
+    assert rc == -3                                            # SW_ERR_LIMIT
+    assert b'scratch holds 67108864' in L.sw_last_error()
