@@ -2,17 +2,32 @@
 
 A step is one day (24 hourly steps) of the batched forward over the 2000x2000 c2 workload
 (seeded synthetic morphology, its 30-day forcing cycled): per step the host builds the day's
-tables, uploads them and the device runs 24 x (shadow pass + pixel pass); Tmrt and UTCI stay in
-device memory. Static preparation (sky view) runs before the timed region and is reported apart.
+tables (solar position, clearness, Perez, patch rows, rays) in CPU worker processes, the engine
+uploads them, and the device replays the day's 24 x (shadow pass + pixel pass) as a captured
+hipGraph; Tmrt and UTCI stay in device memory. Static preparation (walls, sky view) runs before
+the timed region and is reported apart.
 
-Prints one JSON line (see the fields below). ``--gpus N`` with N > 1 is refused: this build runs
-on one GPU.
+Beside the headline the JSON line carries:
+* ``config.eager``: the same days launched eagerly with HIP events around every pass, which give
+  the per-kernel durations of ``roofline`` (dominant kernel: the pixel pass);
+* ``config.plain_forward``: the per-step drop-in API (host tables, uploads, two full-output
+  launches, the UTCI kernel and a stream sync every hour);
+* ``config.forward_adjoint``: forward vs forward + adjoint on the 1000x1000 differentiation workload;
+* ``config.c4``: the 64-domain uneven batch, one launch per pass and hour, tables built inside its
+  timed region;
+* ``cpu_baseline``: the FP64 NumPy oracle (a port of the UMEP step) on the 200x200 c0 workload,
+  its full 24 h under both land-cover readings, on one host core.
+
+``--gpus N`` with N > 1 is refused: this build runs on one GPU.
 """
 from __future__ import annotations
 
 import argparse
+from collections import deque
+import datetime
 import json
 import os
+import platform
 import sys
 import time
 
@@ -22,42 +37,72 @@ sys.path.insert(0, os.path.join(ROOT, 'solweig-cuda_amd'))
 
 import numpy as np  # noqa: E402
 
-B_ALG = 157.5          # algorithmic bytes per pixel-hour of the forward (static inputs + outputs)
+# Algorithmic bytes per pixel-hour of the forward: the static inputs at their minimal exact size,
+# read once, plus the outputs. Shadow pass: DSM, canopy, walls, aspect (16 B). Pixel pass: ground
+# bit (0.125), 15 SVF planes FP32 (60), tan(asvf) and tan(svfalfa) FP64 (16), 3 x 153 visibility
+# bits (57.375), Tmrt + UTCI FP32 (8): 141.5 B.
+B_SHADOW = 16.0
+B_PIXELS = 141.5
+B_ALG = B_SHADOW + B_PIXELS
 HBM_PEAK = 8.0e12      # MI355X HBM3E bytes/s
-# HBM bytes of one hour (shadow + pixel launch, mean over a day) of the 2000x2000 workload, from
-# rocprofv3 FETCH_SIZE (x 1.9999 copy calibration) + WRITE_SIZE; profiles/k_hour_pair_c2_2000x2000.md
-TRAFFIC_PER_HOUR_2000 = 0.977e9
+# HBM bytes per k_hour_pixels launch at 2000x2000 (mean over a day): rocprofv3 FETCH_SIZE scaled by
+# the k_copy calibration + WRITE_SIZE, from the profile named in TRAFFIC_SOURCE (a profile value, not
+# measured by this run).
+TRAFFIC_PIXELS_2000 = 0.825e9
+TRAFFIC_SOURCE = 'profiles/k_hour_pair_c2_2000x2000.md'
 
 
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(hours=tuple(range(5, 21)), size=200):
-    """The FP64 NumPy oracle (a port of the UMEP step) on the 200x200 c0 workload, a bounded
-    sample of daytime hours after its static preparation, on one core."""
-    os.environ.setdefault('OMP_NUM_THREADS', '1')
+def host_info():
+    model = platform.processor() or ''
     try:
-        os.sched_setaffinity(0, {sorted(os.sched_getaffinity(0))[0]})
+        with open('/proc/cpuinfo') as f:
+            for line in f:
+                if line.startswith('model name'):
+                    model = line.split(':', 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    aff = sorted(os.sched_getaffinity(0)) if hasattr(os, 'sched_getaffinity') else []
+    return model, os.cpu_count(), aff
+
+
+def cpu_baseline():
+    """The FP64 NumPy oracle on the 200x200 c0 workload: its full 24 h under both readings of the
+    land cover (all paved, and the default maps), after each reading's static sky-view preparation
+    (timed apart), pinned to one host core."""
+    model, count, aff = host_info()
+    core = aff[0] if aff else 0
+    try:
+        os.sched_setaffinity(0, {core})
     except (AttributeError, OSError):
         pass
     from solweig_amd import synthetic, sun as psun
-    from oracle import day as oday, umep
-    d = synthetic.make_c0(size=size)
-    s = psun.met_sun(d.met, d.location, d.utc)
-    t0 = time.perf_counter()
-    p = oday.prepare(d.dsm, d.trees, d.dem, d.walls, d.aspect, d.latitude)
-    t_static = time.perf_counter() - t0
-    hrs = list(hours)
-    met = d.met
-    sub = {k: np.asarray(v)[hrs[0]:hrs[-1] + 1] for k, v in s.items()}
-    t0 = time.perf_counter()
-    oday.run_day(p, met[hrs[0]:hrs[-1] + 1], sub)
-    dt = time.perf_counter() - t0
-    px_h = size * size * len(hrs)
-    return dict(value=px_h / dt, unit='pixel-hours/s', cores=1, kind='port',
-                sample=f'oracle FP64 NumPy UMEP step on c0 {size}x{size}, hours {hrs[0]}-{hrs[-1]} '
-                       f'({px_h} px-h in {dt:.1f} s; static SVF prep {t_static:.1f} s excluded)')
+    from oracle import day as oday
+    runs, total_px_h, total_s = [], 0, 0.0
+    for paved in (True, False):
+        d = synthetic.make_c0(landcover=paved)
+        s = psun.met_sun(d.met, d.location, d.utc)
+        t0 = time.perf_counter()
+        p = oday.prepare(d.dsm, d.trees, d.dem, d.walls, d.aspect, d.latitude, landcover=d.landcover)
+        t_static = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        oday.run_day(p, d.met, s)
+        dt = time.perf_counter() - t0
+        px_h = d.dsm.size * d.met.shape[0]
+        total_px_h += px_h
+        total_s += dt
+        runs.append(dict(land_cover='all paved (code 1)' if paved else 'default maps', pixel_hours=px_h,
+                         seconds=dt, static_svf_seconds=t_static))
+        log(f'[bench] cpu baseline c0 {"paved" if paved else "default"}: {px_h} px-h in {dt:.1f} s '
+            f'(static sky view {t_static:.1f} s)')
+    return dict(value=total_px_h / total_s, unit='pixel-hours/s', cores=1, kind='port',
+                sample=f'oracle FP64 NumPy UMEP step on c0 200x200, all 24 h, both land-cover readings '
+                       f'({total_px_h} px-h in {total_s:.1f} s; static sky-view preparation excluded)',
+                cpu_model=model, os_cpu_count=count, affinity=[core], affinity_before=aff, runs=runs)
 
 
 def static_leg(dom):
@@ -146,48 +191,97 @@ def adjoint_leg(size=1000, repeats=3):
                 J=g['J'], hours=list(g['hours']))
 
 
-def c4_main(args):
+class DayPipeline:
+    """Day tables from the table workers, kept ``depth`` days ahead of the device over one
+    continuous sequence of days (warm-up, timed and event-timed days alike), so that the timed
+    days see the steady state: while the device runs a day, the workers build the following ones."""
+
+    def __init__(self, eng, day_indices):
+        self.eng, self.days, self.nxt = eng, list(day_indices), 0
+        self.depth = max(2, getattr(eng, 'workers', 0))
+        self.pending = deque()
+        self._fill()
+
+    def _fill(self):
+        while self.nxt < len(self.days) and len(self.pending) < self.depth:
+            self.pending.append(self.eng.tables_async(day_index=self.days[self.nxt]))
+            self.nxt += 1
+
+    def next(self):
+        tabs = self.pending.popleft().result()
+        self._fill()
+        return tabs
+
+
+def run_days(eng, pipe, n, use_graph, events=None):
+    """``n`` days from the pipeline: per day the upload (stream-ordered behind the previous day),
+    the lag reset and the 24 hours. With ``events`` (one list of 3 x 24 + 1 per day) every pass is
+    bracketed by HIP events."""
+    from solweig_amd import runtime as rt
+    used = []
+    for k in range(n):
+        tabs = pipe.next()
+        eng.load(tabs)
+        eng.reset_state()
+        if events is None:
+            eng.forward_day(use_graph=use_graph)
+        else:
+            ev = events[k]
+            for h in range(eng.n_hours):
+                ev[3 * h].record(eng.stream)
+                rt.call('sw_hour_shadow', eng.dom_dev.ptr, eng.block_start.ptr, len(eng.domains), eng.total_blocks,
+                        h, 0, eng.stream.handle)
+                ev[3 * h + 1].record(eng.stream)
+                rt.call('sw_hour_pixels', eng.dom_dev.ptr, eng.block_start.ptr, len(eng.domains),
+                        eng.total_blocks, h, eng.gvf_off.ptr, eng.gvf_bounds.ptr, eng.first, eng.second, 0,
+                        eng.stream.handle)
+                ev[3 * h + 2].record(eng.stream)
+        used.append(tabs)
+    eng.stream.sync()
+    return used
+
+
+def c4_leg(args, workers):
     """The uneven-size batch: 64 domains (sides U_int[256, 1536], latitudes U(0, 60) N, each with
-    its own morphology and forcing), all in one launch per pass and hour. Timed: the device day
-    with the day tables resident (one day of forcing, built before the timed region and reported
-    separately). Two domains are also run alone and must match their part of the batch bit for bit."""
+    its own morphology and forcing) in one engine, one launch per pass and hour over all of them.
+    A step is one day; each day's tables (64 domains) are built by the table workers inside the
+    timed region. The smallest and the largest domain are run again alone and must match their part
+    of the batch bit for bit."""
     from solweig_amd import runtime as rt, synthetic, batched
     t0 = time.perf_counter()
     doms = synthetic.make_c4(count=args.c4_count)
     sides = [d.shape[0] for d in doms]
     P = sum(d.dsm.size for d in doms)
+    t_gen = time.perf_counter() - t0
     log(f'[bench] c4: {len(doms)} domains, sides {min(sides)}..{max(sides)}, {P / 1e6:.1f} Mpx '
-        f'(generated in {time.perf_counter() - t0:.1f} s)')
+        f'(generated in {t_gen:.1f} s)')
     t0 = time.perf_counter()
     eng = batched.Engine(doms, n_hours=24)
     rt.synchronize()
     t_prep = time.perf_counter() - t0
     eng.set_tiles(args.tile or 16)
-    log(f'[bench] c4 static preparation {t_prep:.1f} s, {eng.total_blocks} tiles')
+    eng.workers = 0 if args.no_workers else eng.start_table_workers(workers)
+    log(f'[bench] c4 static preparation {t_prep:.1f} s, {eng.total_blocks} tiles, {eng.workers} table workers')
     t0 = time.perf_counter()
     tabs = eng.tables()
-    t_tab = time.perf_counter() - t0
-    eng.load(tabs)
-    for _ in range(args.warmup):
-        eng.reset_state()
-        eng.forward_day()
-        eng.stream.sync()
+    t_tab_serial = time.perf_counter() - t0
+    warm = max(1, args.warmup)
+    pipe = DayPipeline(eng, [0] * (warm + args.c4_steps))
+    run_days(eng, pipe, warm, use_graph=False)
     rt.synchronize()
     t0 = time.perf_counter()
-    for _ in range(args.steps):
-        eng.reset_state()
-        eng.forward_day()
-    eng.stream.sync()
+    run_days(eng, pipe, args.c4_steps, use_graph=False)
     rt.synchronize()
     dt = time.perf_counter() - t0
-    value = P * 24 * args.steps / dt
-    log(f'[bench] c4: {dt / args.steps * 1e3:.1f} ms/day, {value:.3e} px-h/s; host tables {t_tab * 1e3:.0f} ms/day')
+    value = P * 24 * args.c4_steps / dt
+    log(f'[bench] c4: {dt / args.c4_steps * 1e3:.1f} ms/day, {value:.3e} px-h/s (tables built in the timed region; '
+        f'one core would take {t_tab_serial * 1e3:.0f} ms/day)')
     checks = []
     for k in (int(np.argmin(sides)), int(np.argmax(sides))):
         tm, ut = eng.outputs(k)
         alone = batched.Engine([doms[k]], n_hours=24)
         alone.set_tiles(args.tile or 16)
-        alone.load(alone.tables())
+        alone.load([tabs[k]])
         alone.reset_state()
         alone.forward_day()
         tm1, ut1 = alone.outputs(0)
@@ -195,15 +289,12 @@ def c4_main(args):
         checks.append(dict(domain=k, side=sides[k], identical_to_single_domain_run=same))
         log(f'[bench] c4 domain {k} ({sides[k]}^2) alone vs in the batch: identical {same}')
         del alone
-    out = dict(metric='Tmrt+UTCI pixel-hours/s, uneven batch of 64 domains, 1 GPU',
-               value=value, unit='pixel-hours/s', n_gpus=1, steps=args.steps, warmup=args.warmup,
-               ms_per_step=dt / args.steps * 1e3, higher_is_better=True, scaling='weak', vs_baseline=None,
-               dtype='f32 (f64 masks/lag/wedge/UTCI)', data='synthetic',
-               config=dict(workload=f'c4 uneven batch: {len(doms)} domains, sides {min(sides)}..{max(sides)}, '
-                                    f'{P} pixels, 24 h per step, one launch per pass and hour, day tables resident',
-                           tile=args.tile or 16, static_prep_s=t_prep, host_tables_ms=t_tab * 1e3,
-                           checks=checks))
-    print(json.dumps(out), flush=True)
+    eng.close_table_workers()
+    return dict(value=value, unit='pixel-hours/s', ms_per_day=dt / args.c4_steps * 1e3, steps=args.c4_steps,
+                workload=f'c4 uneven batch: {len(doms)} domains, sides {min(sides)}..{max(sides)}, {P} pixels, '
+                         f'24 h per step, one launch per pass and hour, day tables built in the timed region',
+                tile=args.tile or 16, generation_s=t_gen, static_prep_s=t_prep,
+                host_tables_one_core_ms=t_tab_serial * 1e3, table_workers=eng.workers, checks=checks)
 
 
 def main():
@@ -212,7 +303,11 @@ def main():
     ap.add_argument('--steps', type=int, default=10)
     ap.add_argument('--warmup', type=int, default=1)
     ap.add_argument('--size', type=int, default=2000)
-    ap.add_argument('--graph', action='store_true', help='replay the day as a captured hipGraph')
+    ap.add_argument('--eager', action='store_true', help='headline with eager launches instead of the hipGraph')
+    ap.add_argument('--eager-days', type=int, default=3, help='days of the eager, event-timed leg (0 = skip)')
+    ap.add_argument('--workers', type=int, default=0, help='table worker processes (0 = host cores - 1, at most 8)')
+    ap.add_argument('--no-workers', action='store_true',
+                    help='build the day tables in this process (for profiler runs, which must not spawn)')
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--copy-calibration', action='store_true',
                     help='run a 256 MiB streaming copy (sw_copy) 3 times for HBM counter scaling')
@@ -222,31 +317,27 @@ def main():
                     help='fixed tile width (16, 32, 64, 128) instead of the autotuner (for profiling runs)')
     ap.add_argument('--plain-days', type=int, default=1,
                     help='also time the plain per-step forward for this many days (0 = skip)')
-    ap.add_argument('--workload', choices=('c2', 'c4'), default='c2',
-                    help='c2: the 2000x2000 day (default); c4: the 64-domain uneven batch')
+    ap.add_argument('--no-c4', action='store_true', help='skip the 64-domain uneven batch leg')
+    ap.add_argument('--c4-steps', type=int, default=3)
     ap.add_argument('--c4-count', type=int, default=64, help='domains of the c4 batch')
     args = ap.parse_args()
     if args.gpus != 1:
         print(f'bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU sharding is a '
               f'design note, not part of this build', file=sys.stderr)
         sys.exit(2)
-    if args.workload == 'c4':
-        c4_main(args)
-        return
     from solweig_amd import runtime as rt, synthetic, batched
+    workers = args.workers or None
     days = 30 if args.size == 2000 else 3
     log(f'[bench] generating c2 morphology {args.size}x{args.size}')
     t0 = time.perf_counter()
-    dom = synthetic.make_c1(seed=2, size=args.size, days=days, start=__import__('datetime').date(2025, 6, 6),
-                            name='c2')
+    dom = synthetic.make_c1(seed=2, size=args.size, days=days, start=datetime.date(2025, 6, 6), name='c2')
     log(f'[bench] generated in {time.perf_counter() - t0:.1f} s; static preparation')
     static = static_leg(dom)
     t0 = time.perf_counter()
     eng = batched.Engine([dom], n_hours=24)
     rt.synchronize()
-    t_static = time.perf_counter() - t0
-    static['engine_setup_s'] = t_static
-    log(f'[bench] static prep {t_static:.2f} s (host derivations, uploads, sky view, sky sums); '
+    static['engine_setup_s'] = time.perf_counter() - t0
+    log(f'[bench] static prep {static["engine_setup_s"]:.2f} s (host derivations, uploads, sky view, sky sums); '
         f'device: walls {static["walls_ms"]:.2f} ms, SVF march {static["svf_visibility_ms"]:.1f} ms, '
         f'SVF reduce {static["svf_reduce_ms"]:.2f} ms, sky sums {static["sky_sums_ms"]:.2f} ms')
     P = eng.domains[0].P
@@ -259,76 +350,66 @@ def main():
         from solweig_amd import autotune
         tuned = autotune.tune(eng, eng.tables(day_index=0), use_cache=False)
         log(f'[bench] autotuned tile width {tuned["tile_cols"]} (ms over hours {tuned.get("hours")}: {tuned["ms"]})')
-
-    def launch_day(evs):
-        """Queue the 24 hours of the loaded day; with events, one per pass boundary."""
-        if args.graph:
-            eng.forward_day(use_graph=True)
-            return
-        for h in range(24):
-            evs[3 * h].record(eng.stream)
-            rt.call('sw_hour_shadow', eng.dom_dev.ptr, eng.block_start.ptr, 1, eng.total_blocks, h, 0,
-                    eng.stream.handle)
-            evs[3 * h + 1].record(eng.stream)
-            rt.call('sw_hour_pixels', eng.dom_dev.ptr, eng.block_start.ptr, 1, eng.total_blocks, h,
-                    eng.gvf_off.ptr, eng.gvf_bounds.ptr, eng.first, eng.second, 0, eng.stream.handle)
-            evs[3 * h + 2].record(eng.stream)
-
-    warm_ev = [rt.Event() for _ in range(3 * 24 + 1)]
-    for k in range(args.warmup):
-        eng.load(eng.tables(day_index=k % days))
-        eng.reset_state()
-        launch_day(warm_ev)
-        eng.stream.sync()
-        log(f'[bench] warmup day {k} done')
+    eng.workers = n_workers = 0 if args.no_workers else eng.start_table_workers(workers)
+    log(f'[bench] {n_workers} table worker processes')
+    use_graph = not args.eager
+    n_total = args.warmup + args.steps + max(0, args.eager_days)
+    pipe = DayPipeline(eng, [k % days for k in range(n_total)])
+    run_days(eng, pipe, args.warmup, use_graph)
     rt.synchronize()
-    # Timed: the host builds day k+1's tables (solar position, Perez, rays, patch tables) while the
-    # device runs day k; the uploads are queued in stream order behind day k's launches.
-    evs = [[rt.Event() for _ in range(3 * 24 + 1)] for _ in range(args.steps)]
-    all_tabs = []
+    log(f'[bench] {args.warmup} warm-up days done')
+    # Timed: the table workers build the next days while the device replays the current one.
     t0 = time.perf_counter()
-    tabs = eng.tables(day_index=args.warmup % days)
-    for k in range(args.steps):
-        eng.load(tabs)
-        eng.reset_state()
-        launch_day(evs[k])
-        all_tabs.append(tabs)
-        if k + 1 < args.steps:
-            tabs = eng.tables(day_index=(args.warmup + k + 1) % days)
-    eng.stream.sync()
+    run_days(eng, pipe, args.steps, use_graph)
     rt.synchronize()
     dt = time.perf_counter() - t0
-    t_shadow = t_pixel = 0.0
-    n_day_hours = 0
-    if not args.graph:
-        for k in range(args.steps):
+    value = P * 24 * args.steps / dt
+    log(f'[bench] c2 {"graph" if use_graph else "eager"}: {dt / args.steps * 1e3:.2f} ms/day, {value:.3e} px-h/s')
+    tmrt, utci = eng.outputs(0)
+    log(f'[bench] Tmrt range {np.nanmin(tmrt):.2f}..{np.nanmax(tmrt):.2f}, UTCI {np.nanmin(utci):.2f}..{np.nanmax(utci):.2f}')
+
+    # eager leg: per-pass HIP events on the engine's stream (the kernels' launch durations)
+    eager = roof = None
+    if args.eager_days > 0:
+        evs = [[rt.Event() for _ in range(3 * 24 + 1)] for _ in range(args.eager_days)]
+        t0 = time.perf_counter()
+        etabs = run_days(eng, pipe, args.eager_days, False, events=evs)
+        rt.synchronize()
+        edt = time.perf_counter() - t0
+        t_sh = t_px = 0.0
+        n_day_hours = 0
+        for k in range(args.eager_days):
             per_hour = []
             for h in range(24):
-                if all_tabs[k][0].hours[h, 0]:
+                if etabs[k][0].hours[h, 0]:
                     n_day_hours += 1
                 a = evs[k][3 * h].elapsed_ms(evs[k][3 * h + 1])
                 b = evs[k][3 * h + 1].elapsed_ms(evs[k][3 * h + 2])
-                t_shadow += a
-                t_pixel += b
-                per_hour.append(f'{h}:{a:.2f}/{b:.2f}(alt {all_tabs[k][0].hours[h, 1]:.0f})')
-            log(f'[bench] day {k} per hour shadow/pixel ms: ' + ' '.join(per_hour))
-    px_h = P * 24 * args.steps
-    value = px_h / dt
-    launches = 24 * args.steps
-    pair_s = (t_shadow + t_pixel) / 1e3 / launches if launches and t_pixel else None
-    tmrt, utci = eng.outputs(0)
-    log(f'[bench] Tmrt range {np.nanmin(tmrt):.2f}..{np.nanmax(tmrt):.2f}, UTCI {np.nanmin(utci):.2f}..{np.nanmax(utci):.2f}')
-    roof = None
-    if pair_s:
-        achieved = B_ALG * P / pair_s
-        roof = dict(bound='hbm', achieved=achieved / 1e9, peak=HBM_PEAK / 1e9, unit='GB/s',
-                    frac=achieved / HBM_PEAK,
-                    traffic=TRAFFIC_PER_HOUR_2000 if args.size == 2000 else None,
-                    algorithmic_bytes_per_hour=B_ALG * P,
-                    kernel='k_hour_shadow + k_hour_pixels (one hour of the forward)',
-                    avg_ms=dict(k_hour_shadow=t_shadow / launches, k_hour_pixels=t_pixel / launches,
-                                daytime_hours=n_day_hours / max(args.steps, 1),
-                                device_ms_per_day=(t_shadow + t_pixel) / args.steps))
+                t_sh += a
+                t_px += b
+                per_hour.append(f'{h}:{a:.3f}/{b:.3f}(alt {etabs[k][0].hours[h, 1]:.0f})')
+            log(f'[bench] eager day {k} per hour shadow/pixel ms: ' + ' '.join(per_hour))
+        launches = 24 * args.eager_days
+        sh_ms, px_ms = t_sh / launches, t_px / launches
+        eager = dict(value=P * 24 * args.eager_days / edt, ms_per_day=edt / args.eager_days * 1e3,
+                     days=args.eager_days, device_ms_per_day=(t_sh + t_px) / args.eager_days,
+                     avg_ms=dict(k_hour_shadow=sh_ms, k_hour_pixels=px_ms),
+                     daytime_hours_per_day=n_day_hours / args.eager_days)
+        dom_ms, dom_name, dom_b = (px_ms, 'k_hour_pixels', B_PIXELS) if px_ms >= sh_ms else \
+            (sh_ms, 'k_hour_shadow', B_SHADOW)
+        achieved = dom_b * P / (dom_ms * 1e-3)
+        pair = B_ALG * P / ((sh_ms + px_ms) * 1e-3)
+        roof = dict(bound='hbm', achieved=achieved / 1e9, peak=HBM_PEAK / 1e9, unit='GB/s', frac=achieved / HBM_PEAK,
+                    traffic=TRAFFIC_PIXELS_2000 if (args.size == 2000 and dom_name == 'k_hour_pixels') else None,
+                    traffic_source=f'{TRAFFIC_SOURCE} (rocprofv3 FETCH_SIZE x copy calibration + WRITE_SIZE; '
+                                   f'a profile value, not measured in this run)',
+                    kernel=dom_name, algorithmic_bytes_per_launch=dom_b * P, avg_ms=dom_ms,
+                    pair=dict(kernels='k_hour_shadow + k_hour_pixels (one hour)', achieved=pair / 1e9,
+                              frac=pair / HBM_PEAK, algorithmic_bytes=B_ALG * P, avg_ms=sh_ms + px_ms))
+        log(f'[bench] eager: {eager["ms_per_day"]:.2f} ms/day; avg shadow {sh_ms:.4f} ms, pixels {px_ms:.4f} ms; '
+            f'{dom_name} {achieved / 1e9:.0f} GB/s = {100 * achieved / HBM_PEAK:.1f} % of HBM')
+    eng.close_table_workers()
+
     plain = None
     if args.plain_days > 0:
         from solweig_amd import model, series
@@ -349,6 +430,7 @@ def main():
                      note='per-step API: host tables + H2D + full-output shadow and pixel passes + UTCI '
                           'kernel + stream sync every hour, outputs kept on the device')
         log(f'[bench] plain per-step forward {plain["value"]:.3e} px-h/s ({plain["ms_per_day"]:.1f} ms/day)')
+        del prep
     if args.copy_calibration:
         src = rt.DeviceArray((64 << 20,), np.float32)
         dst = rt.DeviceArray((64 << 20,), np.float32)
@@ -357,16 +439,20 @@ def main():
         eng.stream.sync()
         log('[bench] copy calibration: 3 x 256 MiB read + 256 MiB written')
     adj = None if args.no_adjoint else adjoint_leg()
+    del eng
+    c4 = None if args.no_c4 else c4_leg(args, workers)
     cpu = None if args.no_cpu else cpu_baseline()
     out = dict(metric='Tmrt+UTCI pixel-hours/s at 2000^2 x24h, 1 GPU; batched forward',
                value=value, unit='pixel-hours/s', n_gpus=1, steps=args.steps, warmup=args.warmup,
                ms_per_step=dt / args.steps * 1e3, higher_is_better=True, scaling='weak', vs_baseline=None,
                dtype='f32 (f64 masks/lag/wedge/UTCI)', data='synthetic',
-               config=dict(workload=f'c2 batched forward {args.size}x{args.size}, 24 h per step '
-                                    f'(30-day forcing cycled), host day tables inside the timed region '
-                                    f'(built for day k+1 while the device runs day k)',
-                           graph=bool(args.graph), static_prep=static, tile=tuned, plain_forward=plain,
-                           forward_adjoint=adj),
+               config=dict(workload=f'c2 batched forward {args.size}x{args.size}, 24 h per step (30-day forcing '
+                                    f'cycled), day tables built '
+                                    f'{f"by {n_workers} CPU worker processes" if n_workers else "in the main process"}'
+                                    f' inside the timed region, '
+                                    f'{"hipGraph day replay" if use_graph else "eager launches"}',
+                           graph=use_graph, static_prep=static, tile=tuned, eager=eager, plain_forward=plain,
+                           forward_adjoint=adj, c4=c4),
                roofline=roof, cpu_baseline=cpu)
     print(json.dumps(out), flush=True)
 

@@ -51,6 +51,10 @@ enum {
                              B = (1 - esky)*A, g = exp(0.308*(1.7 - 1/cos(zenith))) per band */
     SWH_ELIG,             /* 5 values: the 32-patch words of the sun-test eligibility bits
                              (patch k: bit k % 32 of word k / 32), as exact integers          */
+    SWH_TG_SINE = SWH_ELIG + 5, /* sin(((frac - SNUP/24) / (TmaxLST/24 - SNUP/24)) pi/2) of the
+                             surface temperature wave (umep_solweig.py:144-170), host FP64, for
+                             the TmaxLST of SWH_TG_TMAX; other TmaxLST values take it in FP32 */
+    SWH_TG_TMAX,
     SW_HOUR_STRIDE = 64
 };
 

@@ -9,7 +9,10 @@ for every further day after the tables are refreshed.
 """
 from __future__ import annotations
 
+import concurrent.futures as cf
 import ctypes
+import multiprocessing as mp
+import os
 
 import numpy as np
 
@@ -90,19 +93,47 @@ class Engine:
         self._write_domains()
 
     # ------------------------------------------------------------------ tables
+    def _jobs(self, day_index, met, sun, constants, psi):
+        for k, d in enumerate(self.domains):
+            dm = d.domain
+            rows = met[k] if met is not None else dm.met[day_index * self.n_hours:(day_index + 1) * self.n_hours]
+            yield (np.asarray(rows), None if sun is None else sun[k], dm.location, dm.utc, dm.latitude, d.shape,
+                   dm.scale, d.amaxvalue, d.landcover, d.wall_params, constants or CONSTANTS, psi)
+
     def tables(self, day_index=0, met=None, sun=None, constants=None, psi=None):
         """Host tables of one day for every domain. ``met`` / ``sun`` override per domain;
         ``constants`` (sky.CONSTANTS-like) and ``psi`` (canopy transmissivity of the hour's
         shadow, default by season) override the model constants."""
-        out = []
-        for k, d in enumerate(self.domains):
-            dm = d.domain
-            rows = met[k] if met is not None else dm.met[day_index * self.n_hours:(day_index + 1) * self.n_hours]
-            s = sun[k] if sun is not None else met_sun(rows, dm.location, dm.utc)
-            out.append(build_day(rows, s, dm.latitude, d.shape, dm.scale, d.amaxvalue,
-                                 landcover=d.landcover, wall_params=d.wall_params,
-                                 constants=constants or CONSTANTS, psi=psi))
-        return out
+        return [_day_tables(*job) for job in self._jobs(day_index, met, sun, constants, psi)]
+
+    def start_table_workers(self, workers=None):
+        """Build day tables in ``workers`` CPU processes from now on (``tables_async``). The
+        tables are per-hour host FP64 work (solar position, clearness, Perez, patch rows, search
+        tables, rays); one day of one domain is one job, so days ahead and domains spread over
+        the host's cores while the device runs. Workers are spawned and touch no GPU."""
+        if workers is None:
+            workers = max(1, min(8, (len(os.sched_getaffinity(0)) if hasattr(os, 'sched_getaffinity')
+                                     else os.cpu_count() or 2) - 1))
+        self.close_table_workers()
+        self._pool = cf.ProcessPoolExecutor(int(workers), mp_context=mp.get_context('spawn'))
+        # import the table code in every worker now, not inside the first timed day
+        list(self._pool.map(_warm, range(int(workers))))
+        return int(workers)
+
+    def close_table_workers(self):
+        pool, self._pool = getattr(self, '_pool', None), None
+        if pool is not None:
+            pool.shutdown(wait=True, cancel_futures=True)
+
+    def tables_async(self, day_index=0, met=None, sun=None, constants=None, psi=None):
+        """``tables`` on the table workers: returns an object whose ``result()`` is the list of
+        DayTables. Without workers (start_table_workers not called) the tables are built in this
+        process when ``result()`` is called."""
+        if getattr(self, '_pool', None) is None:
+            jobs = list(self._jobs(day_index, met, sun, constants, psi))
+            return _Lazy(lambda: [_day_tables(*job) for job in jobs])
+        return _Pending([self._pool.submit(_day_tables, *job)
+                         for job in self._jobs(day_index, met, sun, constants, psi)])
 
     def load(self, tables):
         """Queue the upload of one day's tables on the engine's stream (fixed device addresses, so
@@ -166,6 +197,34 @@ class Engine:
         b, d = self.buf[k], self.domains[k]
         return (b['tmrt'].to_host().reshape(self.n_hours, *d.shape),
                 b['utci'].to_host().reshape(self.n_hours, *d.shape))
+
+
+class _Pending:
+    def __init__(self, futures):
+        self.futures = futures
+
+    def result(self):
+        return [f.result() for f in self.futures]
+
+
+class _Lazy:
+    def __init__(self, build):
+        self.build = build
+
+    def result(self):
+        return self.build()
+
+
+def _warm(_):
+    return os.getpid()
+
+
+def _day_tables(rows, sun, location, utc, latitude, shape, scale, amaxvalue, landcover, wall_params, constants,
+                psi):
+    """One domain's day tables (worker-process entry point as well)."""
+    s = sun if sun is not None else met_sun(rows, location, utc)
+    return build_day(rows, s, latitude, shape, scale, amaxvalue, landcover=landcover, wall_params=wall_params,
+                     constants=constants, psi=psi)
 
 
 def run_day(domain, met=None, sun=None, use_graph=False):

@@ -308,6 +308,16 @@ __global__ __launch_bounds__(256) void k_svf_reduce(
     out[15 * P + p] = v[0] - (1.0 - v[5]) * (1.0 - trans);
 }
 
+// The sine of the ground-temperature wave, sin(((frac - SNUP/24) / (TmaxLST/24 - SNUP/24)) pi/2)
+// (umep_solweig.py:144-170): the hour record carries it in FP64 for the TmaxLST of SWH_TG_TMAX
+// (the default maps' value); a pixel with another TmaxLST evaluates it in FP32.
+__device__ __forceinline__ double tg_sine(const AS4 double *H, float tmaxlst) {
+    if (tmaxlst == (float)H[SWH_TG_TMAX]) return H[SWH_TG_SINE];
+    const float snup = (float)H[SWH_SNUP] * (1.0f / 24.0f);
+    const float arg = ((float)H[SWH_FRAC] - snup) / (tmaxlst * (1.0f / 24.0f) - snup);
+    return (double)sinf(arg * 1.57079632679489662f);
+}
+
 // =====================================================================================
 // Per-hour sun-ray shadow march (shadowingfunction_wallheight_23, umep_solweig.py:2201-2406;
 // geometry.cu:185-253) and the 16-byte record the ground-view walk gathers:
@@ -579,10 +589,8 @@ __global__ __launch_bounds__(BLOCK) void k_hour_shadow(const sw_domain *__restri
     const bool ground = G.ground[p] != 0;
     const bool sunwall = (wallsun / wall * (ground ? 1.0 : 0.0)) == 1.0;
     // Surface temperature of the hour (umep_solweig.py:144-170) and the sunonsurface Lup field.
-    const double snup = H[SWH_SNUP];
     const double amp = (double)G.tgk[p] * H[SWH_ALTMAX] + (double)G.tstart[p];
-    double tg = amp * sin(((H[SWH_FRAC] - snup / 24.0) / ((double)G.tmaxlst[p] / 24.0 - snup / 24.0)) *
-                          3.141592653589793 / 2.0);
+    double tg = amp * tg_sine(H, G.tmaxlst[p]);
     tg = tg * H[SWH_CITGG];
     if (G.landcover == 1 && tg < 0.0) tg = 0.0;
     const double ta = H[SWH_TA], emis = (double)G.emis[p];
@@ -1073,21 +1081,27 @@ __global__ __launch_bounds__(256) void k_gvf_static(const uint8_t *__restrict__ 
     }
 }
 
-// Cylinder wedge: fraction of sunlit walls seen by a standing cylinder (umep_solweig.py:383-426).
+// Cylinder wedge: fraction of sunlit walls seen by a standing cylinder (umep_solweig.py:383-426),
+// in FP32. It depends on y = tan(svfalfa) tan(zen) only: with v = 2 / y the reference's terms
+// reduce to F = 1 - (v + (1 - v) area) / pi, the wall term present only for v > 1 (y < 2), where
+// area = (sin phi - phi cos phi) / (1 - cos phi), phi = atan(t), t = sqrt(v^2 - 1/4). With
+// R = sqrt(v^2 + 3/4) = 1 / cos phi, R - t = 1 / (R + t) and t - v = -(1/4) / (t + v), so
+// F = 1 - [v / (R + t) - (1/4) / (t + v) + phi (v - 1)] / (pi (R - 1)),
+// free of the cancellation the direct form has for large v; beyond v = 2^20 the asymptote
+// 1/2 + 1 / (2 pi v) holds to 1e-12. y = 0 (open sky, svfalfa = 0) is the reference's NaN, which
+// model.py:167 replaces by 0.5; the limit of F for y -> 0 is 0.5 as well.
 __device__ __forceinline__ float wedge(double ta, double tb) {
-    const double PI = 3.141592653589793;
-    const double xa = 1.0 - 2.0 / (ta * tb), ha = 2.0 / (ta * tb), ba = 1.0 / ta;
-    const double hkil = 2.0 * ba * ha;
-    double ukil = 0.0;
-    if (xa < 0.0) {
-        const double qa = tb / 2.0;
-        const double za = sqrt(ba * ba - (qa * qa) / 4.0);
-        const double phi = atan(za / qa);
-        const double area = (sin(phi) - phi * cos(phi)) / (1.0 - cos(phi));
-        ukil = 2.0 * ba * xa * area;
-    }
-    const double F = (2.0 * PI * ba - (hkil + ukil)) / (2.0 * PI * ba);
-    return isnan(F) ? 0.5f : (float)F;
+    const float y = (float)(ta * tb);
+    if (!(y > 0.0f)) return 0.5f;
+    constexpr float INV_PI = 0.318309886183790672f;
+    const float v = 2.0f * __builtin_amdgcn_rcpf(y);
+    if (v <= 1.0f) return 1.0f - v * INV_PI;
+    if (v > 1048576.0f) return 0.5f + (0.5f * INV_PI) * __builtin_amdgcn_rcpf(v);
+    const float v2 = v * v;
+    const float R = __builtin_sqrtf(v2 + 0.75f), t = __builtin_sqrtf(v2 - 0.25f);
+    const float phi = atanf(t);
+    const float num = v * __builtin_amdgcn_rcpf(R + t) - 0.25f * __builtin_amdgcn_rcpf(t + v) + phi * (v - 1.0f);
+    return 1.0f - num * INV_PI * __builtin_amdgcn_rcpf(R - 1.0f);
 }
 
 __device__ __forceinline__ float lag(float current, float &state, double w, bool first,
@@ -1324,10 +1338,8 @@ __global__ __launch_bounds__(BLOCK) void k_hour_pixels(const sw_domain *__restri
         LupN = lag(g.n.lup * INV9 + amb, s4, w, lf, lu);
         if (valid) { lg[p] = s0; lg[P + p] = s1; lg[2 * P + p] = s2; lg[3 * P + p] = s3; lg[4 * P + p] = s4; }
         if (FULL) {
-            const double snup = H[SWH_SNUP];
             const double amp = (double)G.tgk[p] * H[SWH_ALTMAX] + (double)G.tstart[p];
-            double t = amp * sin(((H[SWH_FRAC] - snup / 24.0) / ((double)G.tmaxlst[p] / 24.0 - snup / 24.0)) *
-                                 3.141592653589793 / 2.0) * H[SWH_CITGG];
+            double t = amp * tg_sine(H, G.tmaxlst[p]) * H[SWH_CITGG];
             if (G.landcover == 1 && t < 0.0) t = 0.0;
             tg = (float)t;
             float s5 = lg[5 * P + p];
@@ -2316,10 +2328,8 @@ __global__ __launch_bounds__(BLOCK) void k_adj_gather(const sw_domain *__restric
             // the record of q: Lup = SBC emis ((Tg sh + Ta + 273.15)^4 - (Ta + 273.15)^4), alb, alb sh
             const uint32_t cls = class_of(G.masks[q]);
             const float sh = shadow_of_class(cls, (float)(1.0 - (1.0 - H[SWH_PSI])));
-            const double snup = H[SWH_SNUP];
             const double amp = (double)G.tgk[q] * H[SWH_ALTMAX] + (double)G.tstart[q];
-            const double base = amp * sin(((H[SWH_FRAC] - snup / 24.0) / ((double)G.tmaxlst[q] / 24.0 - snup / 24.0)) *
-                                          3.141592653589793 / 2.0);
+            const double base = amp * tg_sine(H, G.tmaxlst[q]);
             double tg = base * H[SWH_CITGG];
             const bool clamped = G.landcover == 1 && tg < 0.0;
             if (clamped) tg = 0.0;

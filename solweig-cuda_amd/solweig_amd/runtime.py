@@ -38,6 +38,7 @@ H['PATCH_TABLE'] = H['UTCI0'] + 7
 for _k, _name in enumerate('TGOUT_WEIGHT TGOUT_UPDATE ALBEDO_B ABSK ABSL FUP FSIDE FCYL DIFF_VEG TGWALL '
                            'SKY_A SKY_B ELIG'.split()):
     H[_name] = H['PATCH_TABLE'] + 1 + _k
+H['TG_SINE'], H['TG_TMAX'] = H['ELIG'] + 5, H['ELIG'] + 6
 PF = {n: i for i, n in enumerate('LV KD LDN LSD WCOS WSIN DE DW DN DS ELIGIBLE'.split())}
 PD = {n: i for i, n in enumerate('YI SUN_BELOW SHADE_FROM'.split())}
 SKY_SUMS = 10

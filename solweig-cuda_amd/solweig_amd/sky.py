@@ -24,6 +24,7 @@ BAND_COUNTS = (31, 30, 28, 24, 19, 13, 7, 1)
 BAND_ALTITUDES = (6, 18, 30, 42, 54, 66, 78, 90)
 BAND_STARTS = (0, 4, 2, 5, 8, 0, 10, 0)
 FIRSTDAYLEAF, LASTDAYLEAF = 97, 300
+TG_TMAX = 15.0      # TmaxLST whose surface-temperature sine the hour record carries
 
 # Standing person, wall and human constants of series.py:35-51.
 CONSTANTS = dict(albedo_b=0.2, albedo_g=0.15, ewall=0.9, eground=0.95, absK=0.7, absL=0.95,
@@ -493,6 +494,10 @@ def hour_tables(alt, azi, zen, jday, dectime, altmax, Ta, RH, radG, P, Ws, doy, 
         if (citgg > 1) or (citgg == np.inf):
             citgg = 1
         frac = dectime - np.floor(dectime)
+        # the ground-temperature wave's sine for TmaxLST = 15 (the default maps and most land-cover
+        # classes); the device evaluates other TmaxLST values itself
+        h[H['TG_TMAX']] = TG_TMAX
+        h[H['TG_SINE']] = np.sin(((frac - snup / 24) / (TG_TMAX / 24 - snup / 24)) * np.pi / 2)
         tgwall = (tgk_w * altmax + tstart_w) * np.sin(((frac - snup / 24) / (tmax_w / 24 - snup / 24)) * np.pi / 2)
         if tgwall < 0:
             tgwall = 0

@@ -67,3 +67,36 @@ def test_step_api_matches_umep(golden, prefix):
     for k, v in worst.items():
         if k != 'Tmrt':
             assert v < 2e-3, (k, v)
+
+
+def test_fused_wedge_against_the_fp64_oracle(golden):
+    """The pixel pass evaluates the cylinder wedge F_sh in FP32 from y = tan(svfalfa) tan(zen)
+    (umep_solweig.py:383-426); against the FP64 oracle over svfalfa from 0 to pi/2 and zeniths
+    from 2 to 88 degrees the difference stays below 1e-6 (y = 0 gives the reference's NaN -> 0.5)."""
+    from solweig_amd import model, runtime
+    from oracle import umep
+    if runtime.device_count() < 1:
+        pytest.skip('no GPU')
+    p = static_from_golden(golden, 'dayA_')
+    rows, cols = p['dsm'].shape
+    sweep = np.linspace(0.0, np.pi / 2, rows * cols).reshape(rows, cols)
+    sweep.flat[1:40] = np.geomspace(1e-6, 1e-2, 39)
+    args = dict(p)
+    args.update(svfalfa=sweep, rows=rows, cols=cols, location={'latitude': 53.35}, usevegdem=1,
+                anisotropic_sky=1, patch_option=2, cyl=1)
+    prep = model.prepare_model(**args)
+    k = runtime.FULL_PLANES.index('F_sh')
+    worst = 0.0
+    for zen_deg in (2.0, 15.0, 35.0, 60.0, 80.0, 88.0):
+        print(f'fused wedge: zenith {zen_deg}', flush=True)
+        model.run_step(prep, altitude=90.0 - zen_deg, azimuth=180.0, zen=np.radians(zen_deg), jday=172.0,
+                       dectime=172.5, altmax=60.0, Ta=20.0, RH=60.0, radG=600.0, P=101.3, psi=0.03, CI=1.0,
+                       firstdaytime=1, timeadd=0.0, timestepdec=1 / 24, lags=[0.0] * 6)
+        prep.stream.sync()
+        got = prep.full.plane(k, (rows, cols)).to_host().astype(np.float64)
+        with np.errstate(all='ignore'):
+            ref = umep.cylindric_wedge(np.radians(zen_deg), sweep)
+        ref = np.where(np.isnan(ref), 0.5, ref)
+        worst = max(worst, float(np.max(np.abs(got - ref))))
+    print(f'  max |F_sh - FP64 oracle| = {worst:.2e}', flush=True)
+    assert worst < 1e-6
