@@ -124,7 +124,7 @@ def static_leg(dom):
     prep = pstatic.PreparedDomain(dom, stream=stream)                 # includes one sky view
     d = prep.dev
     sv = psvf.sky_view(d['dsm'], d['trees'], prep.dsm_raise, prep.shape, prep.amaxvalue, dom.scale, prep.trans,
-                       stream, events=ev[:3])
+                       stream, events=ev[:3], hmax=d['hmax'], march_trees=d['march_trees'])
     out['svf_visibility_ms'], out['svf_reduce_ms'] = ev[0].elapsed_ms(ev[1]), ev[1].elapsed_ms(ev[2])
     ev[0].record(stream)
     pstatic.sky_sums(sv.bits, prep.P, stream)

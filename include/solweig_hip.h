@@ -179,11 +179,19 @@ int sw_device_sync(void);
 
 /* ----- static sky view (replaces svf.svf_calculator, svf.py:81-114; geometry.cu:77-159) -- */
 /* Visibility bits of 153 patches for every pixel: bit k of word w of mask m is patch 32w+k;
- * m = 0 sky not blocked by buildings (shmat), 1 not vegetation-blocked (vegmat), 2 vbmat.   */
-int sw_svf_visibility(const float *dsm_dev, const float *trees_dev, double dsm_raise,
-                      int rows, int cols, const int32_t *offsets_dev, const double *heights_dev,
-                      const int32_t *lengths_dev, int capacity, int patches,
-                      uint32_t *bits_dev, void *stream);
+ * m = 0 sky not blocked by buildings (shmat), 1 not vegetation-blocked (vegmat), 2 vbmat.
+ * Patch k's ray (svf.py:19-78, from step 0): offsets_dev [patches][capacity][2] (row, col),
+ * heights_dev [patches][capacity] FP64, lengths_dev [patches] steps. With hmax_dev (the
+ * [3][3][R][C] block maxima of the 16 x 16-cell height bound of max(dsm + raise, canopy top),
+ * R = ceil(rows / 16), C = hmax_cols = ceil(cols / 16)) each patch is marched as the hour's sun
+ * ray is, with windows below every live lane's height skipped; this needs dsm + raise >= 0
+ * everywhere and capacity % 8 == 0. march_trees_dev (may be NULL) is the march's canopy plane
+ * (trees, -inf where the canopy top equals the DSM). hmax_dev NULL: one thread per (pixel,
+ * 32 patches), any heights.                                                               */
+int sw_svf_visibility(const float *dsm_dev, const float *trees_dev, const float *march_trees_dev,
+                      const float *hmax_dev, int hmax_cols, double dsm_raise, int rows, int cols,
+                      const int32_t *offsets_dev, const double *heights_dev, const int32_t *lengths_dev,
+                      int capacity, int patches, uint32_t *bits_dev, void *stream);
 /* Band-weighted reduction to 15 SVF planes + svftotal, FP64 [16][P] in the kernel order
  * (total, E, S, W, N for building, vegetation, vegetation-blocking-building).              */
 int sw_svf_reduce(const uint32_t *bits_dev, const float *dsm_dev, const float *trees_dev,

@@ -620,6 +620,112 @@ __global__ __launch_bounds__(BLOCK) void k_hour_shadow(const sw_domain *__restri
 }
 
 // Static per-pixel patch sums of the hour-independent longwave terms (from the visibility bits).
+// Static sky-view visibility as the sun-ray march of the hour kernel (geometry.cu:77-119): a patch is
+// seen from a pixel exactly when a sun at the patch centre would light it, so every patch's ray
+// (svf.py:19-78, step 0 included) is marched with march_chunk: 8-step chunks without bounds tests
+// while the whole wave stays inside the raster, a window of 32 (then 64) steps skipped when no
+// 16 x 16 height-bound cell it crosses rises above the wave's lowest live threshold (a pixel's own
+// height until a building blocks it: the reference stops there, its canopy flag cleared), and the
+// patch ends when every lane is blocked or has left the raster. An off-raster step changes nothing
+// for heights >= 0 (the reference's sh = a < 0 exit), which the caller guarantees (dsm + raise).
+// One wave holds a 16 x 4 block of a 16 x 16 tile and marches its pixels through the 32 patches of
+// blockIdx.y's visibility word.
+__global__ __launch_bounds__(BLOCK) void k_svf_march(const float *__restrict__ dsm_, const float *__restrict__ trees_,
+                                                     const float *__restrict__ mtrees_, const float *__restrict__ hmax_,
+                                                     int hmax_cols, double raise, int rows, int cols,
+                                                     const int32_t *__restrict__ offsets_,
+                                                     const double *__restrict__ heights_,
+                                                     const int32_t *__restrict__ lengths_, int capacity, int patches,
+                                                     uint32_t *__restrict__ bits, int tiles) {
+#pragma clang fp contract(off)
+    constexpr int TX = 16, SH = 4;
+    DomainG G = {};
+    G.rows = rows; G.cols = cols; G.pixels = (long long)rows * cols; G.dsm_raise = raise; G.tile_shift = SH;
+    G.hmax = (const AS4 float *)hmax_; G.hmax_cols = hmax_cols;
+    PixelAt at;
+    const int tpr = (cols + TX - 1) >> SH;
+    const int b = xcd_remap((int)blockIdx.x, tiles);
+    at.tr0 = (b / tpr) * (BLOCK / TX);
+    at.tc0 = (b % tpr) * TX;
+    at.tx = TX; at.ty = BLOCK / TX;
+    at.r = at.tr0 + ((int)threadIdx.x >> SH);
+    at.c = at.tc0 + ((int)threadIdx.x & (TX - 1));
+    const int w64 = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6) * 64;
+    at.wr0 = at.tr0 + (w64 >> SH);
+    at.wr1 = at.tr0 + ((w64 + 63) >> SH);
+    at.wc0 = at.tc0;
+    at.wc1 = at.tc0 + TX - 1;
+    // lanes past the raster march with a copy of the nearest pixel (so the wave-wide thresholds see
+    // every lane) and write nothing
+    const bool valid = at.r < rows && at.c < cols;
+    at.valid = valid;
+    if (!valid) {
+        at.r = min(at.r, rows - 1);
+        at.c = min(at.c, cols - 1);
+    }
+    at.p = (long long)at.r * cols + at.c;
+    const long long P = G.pixels, p = at.p;
+    const AS1 float *dsm = (const AS1 float *)dsm_, *trees = (const AS1 float *)trees_;
+    const bool tops = mtrees_ != nullptr;
+    const AS1 float *mtrees = tops ? (const AS1 float *)mtrees_ : trees;
+    const Heights me = heights_at(dsm, trees, raise, p);
+    const bool bushplant = (me.bot == 0.0) && me.top > 1.0;
+    const AS4 int32_t *lens = (const AS4 int32_t *)lengths_;
+    const int w = (int)blockIdx.y;
+    const int kb = w * 32, ke_patch = min(patches, kb + 32);
+    const uint64_t active = __ballot(true);
+    uint32_t wsh = 0u, wveg = 0u, wvb = 0u;
+    for (int k = kb; k < ke_patch; ++k) {
+        const AS4 int32_t *off = (const AS4 int32_t *)offsets_ + (long long)k * capacity * 2;
+        const AS4 double *dzs = (const AS4 double *)heights_ + (long long)k * capacity;
+        const int L = lens[k];
+        March m;
+        m.f = me.a; m.topmax = me.top; m.prev = 0.0;
+        m.veg = bushplant;
+        m.ever = false;
+        int win = SKIP_WINDOW;
+        for (int k0 = 0; k0 < L; k0 += CHUNK) {
+            if (k0 > 0 && (k0 & (SKIP_TEST - 1)) == 0) {
+                const double thr = (!valid || m.f > me.a) ? __builtin_huge_val() : me.a;
+                const double thr_min = wave_min_below(thr);
+                if (thr_min == __builtin_huge_val()) break;       // every lane blocked
+                const int kl = (k0 + win < L ? k0 + win : L) - 1;
+                if (window_below(G, at, off, k0, kl, rows, cols, dzs[k0 - 1], thr_min)) {
+                    m.prev = dzs[kl];
+                    k0 = kl + 1 - CHUNK;
+                    win = win < SKIP_WINDOW_MAX ? 2 * win : SKIP_WINDOW_MAX;
+                    continue;
+                }
+                win = SKIP_WINDOW;
+            }
+            const int kl = k0 + CHUNK - 1 < L ? k0 + CHUNK - 1 : L - 1;
+            const bool in_last = (unsigned)(at.r + off[2 * kl]) < (unsigned)rows &&
+                                 (unsigned)(at.c + off[2 * kl + 1]) < (unsigned)cols;
+            const bool in_first = valid && (unsigned)(at.r + off[2 * k0]) < (unsigned)rows &&
+                                  (unsigned)(at.c + off[2 * k0 + 1]) < (unsigned)cols;
+            if (__ballot(in_first) == 0) break;              // every lane's ray has left the raster
+            const bool full = k0 + CHUNK <= L && __ballot(in_last) == active;
+            if (full && tops)
+                march_chunk<false, true>(m, dsm, mtrees, raise, me.a, at, rows, cols, off, dzs, k0, L);
+            else if (full)
+                march_chunk<false, false>(m, dsm, mtrees, raise, me.a, at, rows, cols, off, dzs, k0, L);
+            else if (tops)
+                march_chunk<true, true>(m, dsm, mtrees, raise, me.a, at, rows, cols, off, dzs, k0, L);
+            else
+                march_chunk<true, false>(m, dsm, mtrees, raise, me.a, at, rows, cols, off, dzs, k0, L);
+        }
+        const uint32_t bit = 1u << (k - kb);
+        const bool sh = m.f > me.a;
+        if (!sh) wsh |= bit;                                 // shmat = 1 - sh
+        if (!m.veg) wveg |= bit;                             // vegmat = 1 - veg
+        if (!(m.ever && !m.veg)) wvb |= bit;                 // vbmat = 1 - (ever - veg)
+    }
+    if (!valid) return;
+    bits[(0 * SW_VIS_WORDS + w) * P + p] = wsh;
+    bits[(1 * SW_VIS_WORDS + w) * P + p] = wveg;
+    bits[(2 * SW_VIS_WORDS + w) * P + p] = wvb;
+}
+
 __global__ __launch_bounds__(256) void k_sky_sums(const uint32_t *__restrict__ bits, long long P,
                                                   const float *__restrict__ tab, float *__restrict__ out) {
     const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -3244,15 +3350,26 @@ int sw_event_elapsed_ms(void *start, void *stop, float *ms) {
 
 int sw_device_sync(void) { return hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize"); }
 
-int sw_svf_visibility(const float *dsm, const float *trees, double raise, int rows, int cols,
-                      const int32_t *offsets, const double *heights, const int32_t *lengths,
-                      int capacity, int patches, uint32_t *bits, void *stream) {
+int sw_svf_visibility(const float *dsm, const float *trees, const float *march_trees, const float *hmax,
+                      int hmax_cols, double raise, int rows, int cols, const int32_t *offsets,
+                      const double *heights, const int32_t *lengths, int capacity, int patches, uint32_t *bits,
+                      void *stream) {
     if (!dsm || !trees || !offsets || !heights || !lengths || !bits)
         return fail(SW_ERR_ARGUMENT, "sw_svf_visibility: NULL pointer");
     if (rows <= 0 || cols <= 0 || capacity <= 0 || patches <= 0 || patches > 32 * SW_VIS_WORDS)
         return fail(SW_ERR_ARGUMENT, "sw_svf_visibility: bad sizes");
-    const int tiles = ((cols + TILE_X - 1) / TILE_X) * ((rows + TILE_Y - 1) / TILE_Y);
     const int words = (patches + 31) / 32;
+    if (hmax) {
+        if (capacity % CHUNK != 0 || hmax_cols != (cols + (1 << HCELL_SHIFT) - 1) >> HCELL_SHIFT)
+            return fail(SW_ERR_ARGUMENT, "sw_svf_visibility: the march needs capacity % 8 == 0 and "
+                                         "hmax_cols == ceil(cols / 16)");
+        const int tiles = ((cols + 15) / 16) * ((rows + 15) / 16);
+        hipLaunchKernelGGL(k_svf_march, dim3(tiles, words), dim3(BLOCK), 0, (hipStream_t)stream, dsm, trees,
+                           march_trees, hmax, hmax_cols, raise, rows, cols, offsets, heights, lengths, capacity,
+                           patches, bits, tiles);
+        return hip_check(hipGetLastError(), "k_svf_march launch");
+    }
+    const int tiles = ((cols + TILE_X - 1) / TILE_X) * ((rows + TILE_Y - 1) / TILE_Y);
     hipLaunchKernelGGL(k_svf_visibility, dim3(tiles, words), dim3(BLOCK), 0, (hipStream_t)stream,
                        dsm, trees, raise, rows, cols, offsets, heights, lengths, capacity, patches,
                        bits, tiles);

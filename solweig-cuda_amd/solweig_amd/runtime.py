@@ -106,7 +106,7 @@ def lib():
         'sw_event_destroy': ([vp], i32), 'sw_event_record': ([vp, vp], i32),
         'sw_event_elapsed_ms': ([vp, vp, ctypes.POINTER(ctypes.c_float)], i32),
         'sw_device_sync': ([], i32),
-        'sw_svf_visibility': ([vp, vp, f64, i32, i32, vp, vp, vp, i32, i32, vp, vp], i32),
+        'sw_svf_visibility': ([vp, vp, vp, vp, i32, f64, i32, i32, vp, vp, vp, i32, i32, vp, vp], i32),
         'sw_svf_reduce': ([vp, vp, vp, f64, i32, i32, vp, vp, vp, i32, i32, f64, vp, vp], i32),
         'sw_hour_shadow': ([vp, vp, i32, i32, i32, i32, vp], i32),
         'sw_hour_pixels': ([vp, vp, i32, i32, i32, vp, vp, i32, i32, i32, vp], i32),

@@ -148,8 +148,11 @@ class PreparedDomain:
         self.dev['ground'] = up(ground, np.uint8)
         for key in ('alb', 'emis', 'tgk', 'tstart', 'tmax'):
             self.dev[key] = up(maps[key])
+        self.dev['hmax'] = up(height_bound(a, trees))
+        mt = march_trees(a, trees)
+        self.dev['march_trees'] = None if mt is None else up(mt)
         sv = sky_view(self.dev['dsm'], self.dev['trees'], raise_, self.shape, self.amaxvalue,
-                      domain.scale, self.trans, stream)
+                      domain.scale, self.trans, stream, hmax=self.dev['hmax'], march_trees=self.dev['march_trees'])
         self.skyview = sv
         self.dev['vis'] = sv.bits
         # The derived planes are formed on the host with NumPy so that tan(asvf) - which decides
@@ -174,9 +177,6 @@ class PreparedDomain:
         self.dev['masks'] = rt.DeviceArray((self.P,), np.uint8)
         self.dev['lag'] = rt.DeviceArray((6, self.P), np.float32)
         self.dev['skysums'] = sky_sums(self.dev['vis'], self.P, stream)
-        self.dev['hmax'] = up(height_bound(a, trees))
-        mt = march_trees(a, trees)
-        self.dev['march_trees'] = None if mt is None else up(mt)
         self._check_albedo(maps['alb'])
         ground_view_static(self, stream)
         if keep_host:

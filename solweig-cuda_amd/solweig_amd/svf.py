@@ -115,14 +115,24 @@ def unpack_visibility(bits, shape, patches=153):
 
 
 def sky_view(dsm_dev, trees_dev, dsm_raise, shape, amaxvalue, scale, transmissivity=0.03, stream=None,
-             events=None):
+             events=None, hmax=None, march_trees=None):
     """Run the static visibility march and band reduction for one domain. ``events``: three
-    runtime.Event recorded before the march, between the passes and after the reduction."""
+    runtime.Event recorded before the march, between the passes and after the reduction.
+    ``hmax`` (device array of static.height_bound, heights dsm + raise >= 0) selects the windowed
+    sun-ray march; ``march_trees`` is its optional canopy plane (static.march_trees)."""
     rows, cols = shape
     t = ray_tables(rows, cols, float(amaxvalue), float(scale))
     P = rows * cols
-    off = rt.DeviceArray.from_host(t['offsets'], np.int32)
-    hts = rt.DeviceArray.from_host(t['heights'], np.float64)
+    offsets, heights, capacity = t['offsets'], t['heights'], int(t['capacity'])
+    if hmax is not None:
+        # the march loads 8 steps at a time: rays start at multiples of 8 steps
+        capacity = -(-capacity // 8) * 8
+        offsets = np.zeros((t['patches'], capacity, 2), np.int32)
+        heights = np.zeros((t['patches'], capacity), np.float64)
+        offsets[:, :t['capacity']] = t['offsets']
+        heights[:, :t['capacity']] = t['heights']
+    off = rt.DeviceArray.from_host(offsets, np.int32)
+    hts = rt.DeviceArray.from_host(heights, np.float64)
     lens = rt.DeviceArray.from_host(t['lengths'], np.int32)
     sec = rt.DeviceArray.from_host(t['sectors'], np.uint8)
     bo = rt.DeviceArray.from_host(t['band_offsets'], np.int32)
@@ -132,8 +142,9 @@ def sky_view(dsm_dev, trees_dev, dsm_raise, shape, amaxvalue, scale, transmissiv
     s = rt._s(stream)
     if events:
         events[0].record(stream)
-    rt.call('sw_svf_visibility', dsm_dev.ptr, trees_dev.ptr, float(dsm_raise), rows, cols, off.ptr, hts.ptr,
-            lens.ptr, int(t['capacity']), int(t['patches']), bits.ptr, s)
+    rt.call('sw_svf_visibility', dsm_dev.ptr, trees_dev.ptr, 0 if march_trees is None else march_trees.ptr,
+            0 if hmax is None else hmax.ptr, -(-cols // rt.HMAX_CELL), float(dsm_raise), rows, cols, off.ptr,
+            hts.ptr, lens.ptr, capacity, int(t['patches']), bits.ptr, s)
     if events:
         events[1].record(stream)
     rt.call('sw_svf_reduce', bits.ptr, dsm_dev.ptr, trees_dev.ptr, float(dsm_raise), rows, cols, sec.ptr,
@@ -162,7 +173,15 @@ def svf_calculator(patch_option, amaxvalue, a, vegdem, vegdem2, bush, scale,
         raise ValueError('svf_calculator: DSM and canopy heights must be FP32-exact rasters')
     d = rt.DeviceArray.from_host(dsm32)
     tr = rt.DeviceArray.from_host(trees32)
-    sv = sky_view(d, tr, 0.0, a.shape, amaxvalue, scale, transmissivity)
+    hmax = mt = None
+    if a.min() >= 0.0:
+        # heights >= 0: the windowed sun-ray march (a DSM below 0 keeps the plain per-pixel march,
+        # whose off-raster rule depends on the sign of the height)
+        from .static import height_bound, march_trees
+        hmax = rt.DeviceArray.from_host(height_bound(a, trees32))
+        m = march_trees(a, trees32)
+        mt = None if m is None else rt.DeviceArray.from_host(m)
+    sv = sky_view(d, tr, 0.0, a.shape, amaxvalue, scale, transmissivity, hmax=hmax, march_trees=mt)
     planes = sv.planes.to_host().reshape(16, *a.shape)
     shm, vgm, vbm = sv.masks()
     return tuple(planes[i] for i in UMEP_ORDER) + (vgm, vbm, shm, planes[15])

@@ -192,3 +192,29 @@ def test_day_matches_umep_golden(rt, golden, prefix):
     err = float(np.max(np.abs(tmrt - golden[prefix + 'step_Tmrt'])))
     print(f'  max |dTmrt| vs UMEP {err:.3e} K', flush=True)
     assert err < TOL_K
+
+
+@pytest.mark.parametrize('shift', [0.0, -3.0])
+def test_svf_calculator_drop_in_matches_oracle(rt, shift):
+    """svf.svf_calculator on caller rasters: heights >= 0 run the windowed sun-ray march, a DSM
+    partly below 0 (ground at -3 m) the per-pixel march with the reference's off-raster rule
+    (sh = a < 0). Masks and the 15 planes + svftotal equal the oracle's."""
+    from solweig_amd import svf as psvf
+    d = _domain(34, 19)
+    a32 = (d.dsm + np.float32(shift)).astype(np.float32)
+    a = a32.astype(np.float64)
+    t = d.trees.astype(np.float64)
+    top = t + a
+    top[top == a] = 0
+    bot = t * 0.25 + a
+    bot[bot == a] = 0
+    bush = np.logical_not(bot * top) * top
+    amax = float(max(a.max() - a.min(), t.max()))
+    print(f'svf_calculator drop-in, DSM shifted by {shift} (min {a.min():.1f})', flush=True)
+    got = psvf.svf_calculator(2, amax, a, top, bot, bush, 1.0)
+    ref = osvf.svf_calculator(amax, a, top, bot, bush, 1.0)
+    for k in range(15):
+        np.testing.assert_array_equal(got[k], ref[k], err_msg=f'plane {k}')
+    for k in (15, 16, 17):
+        np.testing.assert_array_equal(got[k], ref[k], err_msg=f'mask {k}')
+    np.testing.assert_array_equal(got[18], ref[18])
