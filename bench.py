@@ -297,6 +297,50 @@ def c4_leg(args, workers):
                 host_tables_one_core_ms=t_tab_serial * 1e3, table_workers=eng.workers, checks=checks)
 
 
+def e2e_leg(repeats=3):
+    """The complete workflow of series.run_timeseries on the 2000x2000 c2 rasters (.npy inputs):
+    walls and aspect on the device, static preparation (sky view, sums, ground-view planes), the
+    day loop and the outputs (UTCI and TMRT stacks written as .npy with their tag files under a
+    temporary directory; the reference writes DEFLATE GeoTIFF). 1 day written and 30 days in the
+    reference's no-write mode, each after one warm-up run as the median of ``repeats`` runs (the
+    protocol of the reference's README.md:26-32); 30 days written once (23 GB of .npy). Context: the reference publishes 2.7 s (1 day) and 21 s
+    (30 days) for this raster size on an RTX 4090 (README.md:17-24)."""
+    import shutil
+    import tempfile
+    from pathlib import Path
+    from solweig_amd import series, synthetic
+    d = synthetic.make_c1(seed=2, size=2000, days=30, start=datetime.date(2025, 6, 6), name='c2')
+    tmp = Path(tempfile.mkdtemp(prefix='solweig_e2e_', dir='/tmp' if os.path.isdir('/tmp') else None))
+    try:
+        for name in ('dsm', 'trees', 'dem'):
+            np.save(tmp / f'{name}.npy', getattr(d, name))
+        paths = [tmp / f'{name}.npy' for name in ('dsm', 'trees', 'dem')]
+
+        def run(days, write, k):
+            out = tmp / f'out{k}'
+            t0 = time.perf_counter()
+            series.run_timeseries(*paths, d.met[:24 * days], out, d.longitude, d.latitude, utc_offset_hours=0.0,
+                                  no_write=not write)
+            dt = time.perf_counter() - t0
+            shutil.rmtree(out, ignore_errors=True)
+            return dt
+
+        res = {}
+        for days, write, reps in ((1, True, repeats), (30, False, repeats), (30, True, 1)):
+            if reps > 1:
+                run(days, write, 0)
+            ts = [run(days, write, k + 1) for k in range(reps)]
+            key = f'{days}_day{"s" if days > 1 else ""}_{"written" if write else "no_write"}_s'
+            res[key] = float(np.median(ts))
+            log(f'[bench] e2e run_timeseries 2000x2000, {days} day(s), {"written" if write else "no-write"}: '
+                f'{res[key]:.2f} s (runs {", ".join(f"{t:.2f}" for t in ts)})')
+        res['reference_published_s'] = {'1_day': 2.7, '30_days': 21.0,
+                                        'note': 'RTX 4090, complete workflow incl. GeoTIFF writes (README.md:17-24)'}
+        return res
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
@@ -320,6 +364,7 @@ def main():
     ap.add_argument('--no-c4', action='store_true', help='skip the 64-domain uneven batch leg')
     ap.add_argument('--c4-steps', type=int, default=3)
     ap.add_argument('--c4-count', type=int, default=64, help='domains of the c4 batch')
+    ap.add_argument('--no-e2e', action='store_true', help='skip the end-to-end run_timeseries leg')
     args = ap.parse_args()
     if args.gpus != 1:
         print(f'bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU sharding is a '
@@ -441,6 +486,7 @@ def main():
     adj = None if args.no_adjoint else adjoint_leg()
     del eng
     c4 = None if args.no_c4 else c4_leg(args, workers)
+    e2e = None if args.no_e2e else e2e_leg()
     cpu = None if args.no_cpu else cpu_baseline()
     out = dict(metric='Tmrt+UTCI pixel-hours/s at 2000^2 x24h, 1 GPU; batched forward',
                value=value, unit='pixel-hours/s', n_gpus=1, steps=args.steps, warmup=args.warmup,
@@ -452,7 +498,7 @@ def main():
                                     f' inside the timed region, '
                                     f'{"hipGraph day replay" if use_graph else "eager launches"}',
                            graph=use_graph, static_prep=static, tile=tuned, eager=eager, plain_forward=plain,
-                           forward_adjoint=adj, c4=c4),
+                           forward_adjoint=adj, c4=c4, end_to_end=e2e),
                roofline=roof, cpu_baseline=cpu)
     print(json.dumps(out), flush=True)
 

@@ -84,8 +84,13 @@ enum {
     SWF_KSIDED, SWF_DRAD, SWF_KSIDE, SWF_SHADOW, SWF_WALLSUN, SWF_FSH, SW_FULL_PLANES
 };
 
-/* Device-side description of one domain (296 bytes). All fields are 8 bytes wide; pointers are device
- * pointers. Optional outputs may be NULL. */
+/* Per-hour extra outputs of the batched (non-FULL) pixel pass: with sw_domain.full set, hour h
+ * of a day writes plane k at full + (h * SW_EXTRA_PLANES + k) * P (the series outputs Kdown,
+ * Kup, Ldown, Lup and Shadow of series.py:458-460).                                          */
+enum { SWE_KDOWN = 0, SWE_KUP, SWE_LDOWN, SWE_LUP, SWE_SHADOW, SW_EXTRA_PLANES };
+
+/* Device-side description of one domain (320 bytes: 40 fields, all 8 bytes wide); pointers are
+ * device pointers. Optional outputs may be NULL. */
 typedef struct sw_domain {
     int64_t rows, cols, pixels, landcover;
     double dsm_raise;
@@ -110,7 +115,8 @@ typedef struct sw_domain {
                              building-shadowed, 2 no canopy shadow, 4 vbshvegsh, 8 sunlit wall,
                              16 facesh                                                         */
     double *shadow_planes;                                   /* [8][P] cast_shadow planes      */
-    float *full;                                             /* [SW_FULL_PLANES][P]           */
+    float *full;          /* FULL mode: [SW_FULL_PLANES][P]; batched mode (optional):
+                             [hours][SW_EXTRA_PLANES][P] per-hour extra outputs            */
     const float *skysums;                                    /* [SW_SKY_SUMS][P] static sums   */
     const float *hmax;    /* [3][3][ceil(rows/16)][hmax_cols]: per 16 x 16 cell an upper bound
                              of max(dsm, canopy top) (FP32 rounded up); level (i, j) holds at

@@ -23,7 +23,9 @@ from .sun import met_sun
 
 
 class Engine:
-    def __init__(self, domains, n_hours=24, stream=None, keep_outputs=True):
+    def __init__(self, domains, n_hours=24, stream=None, keep_outputs=True, extra_outputs=False):
+        """``extra_outputs``: also keep each hour's Kdown, Kup, Ldown, Lup and Shadow planes
+        ([n_hours][5][P] per domain, rt.EXTRA_NAMES order), the optional series outputs."""
         scales = {float(d.domain.scale if isinstance(d, PreparedDomain) else d.scale) for d in domains}
         if len(scales) != 1:
             # one launch shares one ground-view step table (the step shifts and counts follow the
@@ -48,6 +50,8 @@ class Engine:
             if keep_outputs:
                 b['tmrt'] = rt.DeviceArray((self.n_hours, d.P), np.float32)
                 b['utci'] = rt.DeviceArray((self.n_hours, d.P), np.float32)
+            if extra_outputs:
+                b['extra'] = rt.DeviceArray((self.n_hours, rt.SW_EXTRA_PLANES, d.P), np.float32)
             self.buf.append(b)
         self.block_start = rt.DeviceArray((len(self.domains) + 1,), np.int32)
         self.structs = (rt.SwDomain * len(self.domains))()
@@ -77,7 +81,8 @@ class Engine:
             self.structs[i] = d.struct(hours=b['hours'].ptr, ptab_f=b['ptab_f'].ptr, ptab_d=b['ptab_d'].ptr,
                                        ptab_s=b['ptab_s'].ptr, ray_off=b['ray_off'].ptr, ray_dz=b['ray_dz'].ptr,
                                        tmrt=b['tmrt'].ptr if 'tmrt' in b else 0,
-                                       utci=b['utci'].ptr if 'utci' in b else 0)
+                                       utci=b['utci'].ptr if 'utci' in b else 0,
+                                       full=b['extra'].ptr if 'extra' in b else 0)
         rt.call('sw_memcpy_h2d', self.dom_dev.ptr, ctypes.addressof(self.structs),
                 ctypes.sizeof(self.structs), self.stream.handle)
         self.stream.sync()
@@ -197,6 +202,13 @@ class Engine:
         b, d = self.buf[k], self.domains[k]
         return (b['tmrt'].to_host().reshape(self.n_hours, *d.shape),
                 b['utci'].to_host().reshape(self.n_hours, *d.shape))
+
+    def extra_outputs(self, k=0):
+        """Host {name: [n_hours, rows, cols]} of the extra planes (engine built with extra_outputs)."""
+        self.stream.sync()
+        b, d = self.buf[k], self.domains[k]
+        e = b['extra'].to_host().reshape(self.n_hours, rt.SW_EXTRA_PLANES, *d.shape)
+        return {name: e[:, i] for i, name in enumerate(rt.EXTRA_NAMES)}
 
 
 class _Pending:

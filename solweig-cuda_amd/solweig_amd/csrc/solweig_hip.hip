@@ -1594,6 +1594,14 @@ __global__ __launch_bounds__(BLOCK) void k_hour_pixels(const sw_domain *__restri
     if (!valid) return;
     if (G.tmrt) G.tmrt[(long long)hour * P + p] = Tmrt;
     if (G.utci) G.utci[(long long)hour * P + p] = u;
+    if (!FULL && G.full) {
+        AS1 float *o = G.full + (long long)hour * SW_EXTRA_PLANES * P;
+        o[SWE_KDOWN * P + p] = Kdown;
+        o[SWE_KUP * P + p] = Kup;
+        o[SWE_LDOWN * P + p] = Ldown;
+        o[SWE_LUP * P + p] = Lup;
+        o[SWE_SHADOW * P + p] = shadow;
+    }
     if (FULL && G.full) {
         AS1 float *o = G.full;
         const float refl = reflsurf;

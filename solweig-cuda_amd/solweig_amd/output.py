@@ -19,14 +19,21 @@ from . import runtime as rt
 
 
 class DayOutputs:
+    """``write(tag, tmrt, utci)``, or ``write(tag, tmrt, utci, extras)`` with extras a dict of the
+    engine's extra planes ({name: [n_hours, rows, cols]}) when the engine keeps them."""
+
     def __init__(self, engine, k=0, slots=2):
         self.engine, self.k = engine, int(k)
         d = engine.domains[self.k]
+        self.extra = 'extra' in engine.buf[self.k]
         self.shape = (2, engine.n_hours, *d.shape)
+        self.extra_shape = (engine.n_hours, rt.SW_EXTRA_PLANES, *d.shape)
         n = engine.n_hours * d.P
         self._bytes = n * 4
-        self._dev = [rt.DeviceArray((2, n), np.float32) for _ in range(slots)]
-        self._host = [rt.PinnedArray((2, n), np.float32) for _ in range(slots)]
+        self._extra_bytes = rt.SW_EXTRA_PLANES * n * 4 if self.extra else 0
+        words = 2 * n + (rt.SW_EXTRA_PLANES * n if self.extra else 0)
+        self._dev = [rt.DeviceArray((words,), np.float32) for _ in range(slots)]
+        self._host = [rt.PinnedArray((words,), np.float32) for _ in range(slots)]
         self._staged = [rt.Event() for _ in range(slots)]
         self._landed = [rt.Event() for _ in range(slots)]
         self._copy = rt.Stream()
@@ -48,9 +55,12 @@ class DayOutputs:
         st, dev = self.engine.stream, self._dev[slot]
         rt.call('sw_memcpy_d2d', dev.ptr, b['tmrt'].ptr, self._bytes, st.handle)
         rt.call('sw_memcpy_d2d', dev.ptr + self._bytes, b['utci'].ptr, self._bytes, st.handle)
+        if self.extra:
+            rt.call('sw_memcpy_d2d', dev.ptr + 2 * self._bytes, b['extra'].ptr, self._extra_bytes, st.handle)
         self._staged[slot].record(st)
         self._copy.wait(self._staged[slot])
-        rt.call('sw_memcpy_d2h_async', self._host[slot].ptr, dev.ptr, 2 * self._bytes, self._copy.handle)
+        rt.call('sw_memcpy_d2h_async', self._host[slot].ptr, dev.ptr, 2 * self._bytes + self._extra_bytes,
+                self._copy.handle)
         self._landed[slot].record(self._copy)
         self._jobs.put((slot, tag, write))
 
@@ -62,9 +72,14 @@ class DayOutputs:
             slot, tag, write = job
             try:
                 self._landed[slot].sync()
-                a = self._host[slot].array.reshape(self.shape)
+                flat = self._host[slot].array
+                a = flat[:2 * (self._bytes // 4)].reshape(self.shape)
                 if self._error is None:
-                    write(tag, a[0], a[1])
+                    if self.extra:
+                        e = flat[2 * (self._bytes // 4):].reshape(self.extra_shape)
+                        write(tag, a[0], a[1], {name: e[:, i] for i, name in enumerate(rt.EXTRA_NAMES)})
+                    else:
+                        write(tag, a[0], a[1])
             except BaseException as e:          # reported by the next submit or close
                 self._error = e
             finally:

@@ -42,6 +42,8 @@ H['TG_SINE'], H['TG_TMAX'] = H['ELIG'] + 5, H['ELIG'] + 6
 PF = {n: i for i, n in enumerate('LV KD LDN LSD WCOS WSIN DE DW DN DS ELIGIBLE'.split())}
 PD = {n: i for i, n in enumerate('YI SUN_BELOW SHADE_FROM'.split())}
 SKY_SUMS = 10
+SW_EXTRA_PLANES = 5
+EXTRA_NAMES = ('Kdown', 'Kup', 'Ldown', 'Lup', 'Shadow')   # enum SWE_* of solweig_hip.h
 FULL_PLANES = ('Kdown Kup Ldown Lup Tg Keast Ksouth Kwest Knorth Least Lsouth Lwest Lnorth KsideI '
                'TgOut Lside KsideD dRad Kside shadow wallsun F_sh').split()
 

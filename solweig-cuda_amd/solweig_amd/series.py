@@ -94,13 +94,79 @@ def make_domain(dsm, trees, dem, forcing, latitude, longitude, utc, site_altitud
                             landcover=None if landcover is None else np.asarray(landcover))
 
 
+UNITS = {'UTCI': 'degree_Celsius', 'TMRT': 'degree_Celsius', 'Shadow': '1', 'Kup': 'W m-2', 'Kdown': 'W m-2',
+         'Lup': 'W m-2', 'Ldown': 'W m-2'}
+MODEL_TAG = 'SOLWEIG 2022a (solweig_amd, MI355X)'
+
+
+def band_tags(variable, date, hours, minutes, utc):
+    """The dataset and per-band tags of one output stack, as the reference writes them into its
+    GeoTIFF (output.py:132-157): band description ``<variable>_HH:MM_local``, ISO time of the
+    forcing row, time basis, UTC offset, solar-sampling rule and units."""
+    units = UNITS[variable]
+    base = datetime.fromisoformat(date)
+    bands = []
+    for h, m in zip(hours, minutes):
+        h, m = int(h), int(m)
+        bands.append({'description': f'{variable}_{h:02d}:{m:02d}_local',
+                      'Time': base.replace(hour=h, minute=m).isoformat(), 'time_basis': 'forcing_local_time',
+                      'utc_offset_hours': str(utc), 'solar_sampling': 'forcing_time_minus_half_timestep',
+                      'units': units})
+    return {'model': MODEL_TAG, 'variable': variable, 'units': units, 'time_basis': 'forcing_local_time',
+            'bands': bands}
+
+
+def write_day(folder, date, stacks, met_rows, chunk_steps, utc, crop=None, valid=None):
+    """Write each variable's [hours, rows, cols] stack of one day under ``folder``: files of at most
+    ``chunk_steps`` bands, suffixed _0001, _0002, ... when the day has more rows than that
+    (series.py:472-489), each ``<name>.npy`` beside a ``<name>.json`` of its tags (band_tags).
+    ``crop`` (row slice, col slice) is the output window; ``valid`` a mask of the window's shape,
+    NaN written where it is False in every variable (output.py:205-213). Returns {var: [paths]}."""
+    folder = Path(folder)
+    folder.mkdir(parents=True, exist_ok=True)
+    n = met_rows.shape[0]
+    c = max(1, int(chunk_steps))
+    crop = (slice(None), slice(None)) if crop is None else tuple(crop)
+    files = {}
+    for var, stack in stacks.items():
+        paths = []
+        for k, lo in enumerate(range(0, n, c)):
+            hi = min(n, lo + c)
+            values = np.array(stack[lo:hi][(slice(None),) + crop], dtype=np.float32)
+            if valid is not None:
+                values[:, ~valid] = np.nan
+            name = f'{var}_{k + 1:04d}' if n > c else var
+            np.save(folder / f'{name}.npy', values)
+            tags = band_tags(var, date, met_rows[lo:hi, 2], met_rows[lo:hi, 3], utc)
+            (folder / f'{name}.json').write_text(json.dumps(tags, indent=1), encoding='utf-8')
+            paths.append(str(folder / f'{name}.npy'))
+        files[var] = paths
+    return files
+
+
+def _write_static(folder, name, values, variable, units, extra=None):
+    folder.mkdir(parents=True, exist_ok=True)
+    np.save(folder / f'{name}.npy', np.asarray(values, dtype=np.float32))
+    tags = {'model': MODEL_TAG, 'variable': variable, 'units': units, **(extra or {})}
+    (folder / f'{name}.json').write_text(json.dumps(tags, indent=1), encoding='utf-8')
+    return str(folder / f'{name}.npy')
+
+
 def run_timeseries(building_dsm_path, tree_path, dem_path, met_file, output_path, longitude, latitude, *,
                    landcover_path=None, utc_offset_hours=None, site_altitude_m=3., wall_method='dsm',
-                   dsm_wall_threshold_m=3., save_tmrt=True, dtype=np.float32, no_write=False,
+                   dsm_wall_threshold_m=3., save_tmrt=True, save_svf=False, save_kup=False, save_kdown=False,
+                   save_lup=False, save_ldown=False, save_shadow=False, dtype=np.float32, no_write=False,
                    output_core=None, output_chunk_steps=24, async_output=True, output_queue_size=2,
-                   output_valid_mask=None, progress=False, pixel_size_m=1.0, plain=False, **unused):
+                   output_valid_mask=None, progress=False, pixel_size_m=1.0, plain=False, return_arrays=False,
+                   table_workers=None, **unused):
     """Run every day of the forcing; returns {'static': {...}, 'days': {date: {var: [paths]}}}
-    (series.py:91-156). Outputs per day: UTCI (and TMRT) stacks [hours, rows, cols] as .npy."""
+    (series.py:91-156). Outputs per day: UTCI, and TMRT, Kup, Kdown, Lup, Ldown, Shadow as
+    enabled, as [hours, rows, cols] .npy stacks split into ``output_chunk_steps`` bands with a .json
+    of their tags; static: walls and aspect, and SVF (svfbuveg, series.py:343-378) with
+    ``save_svf``. ``no_write`` writes nothing and leaves the days empty, as the reference does;
+    ``return_arrays`` (with ``no_write``) returns the host stacks instead. Forcing of more than two
+    days builds the day tables ahead on ``table_workers`` CPU processes (default: host cores - 1,
+    at most 8; 0 builds them in this process)."""
     if wall_method != 'dsm':
         raise ValueError("only wall_method='dsm' is supported")
     forcing = load_meteorology(met_file)
@@ -110,78 +176,114 @@ def run_timeseries(building_dsm_path, tree_path, dem_path, met_file, output_path
                       landcover=_raster(landcover_path))
     out = Path(output_path).resolve()
     files = {'static': {}, 'days': {}}
-    variables = ['UTCI'] + (['TMRT'] if save_tmrt else [])
+    extras = [name for name, on in (('Kup', save_kup), ('Kdown', save_kdown), ('Lup', save_lup),
+                                    ('Ldown', save_ldown), ('Shadow', save_shadow)) if on]
+    variables = ['UTCI'] + (['TMRT'] if save_tmrt else []) + extras
     crop = (slice(None), slice(None)) if output_core is None else \
         (slice(output_core[0], output_core[0] + output_core[2]), slice(output_core[1], output_core[1] + output_core[3]))
     valid = None if output_valid_mask is None else np.asarray(output_valid_mask, dtype=bool)
+    if not no_write:
+        static = out / 'static'
+        files['static']['walls'] = _write_static(static, 'walls', dom.walls, 'wall_height', 'm',
+                                                 {'wall_method': 'dsm', 'wall_backend': 'HIP'})
+        files['static']['aspect'] = _write_static(static, 'aspect', dom.aspect, 'wall_aspect',
+                                                  'degree_clockwise_from_north', {'wall_method': 'dsm'})
+    rows_of = {d.date: d.values for d in forcing.days}
 
-    def write(date, tmrt, utci):
-        if valid is not None:
-            utci = np.where(valid[None], utci, np.nan)
-        folder = out / date
-        folder.mkdir(parents=True, exist_ok=True)
-        day_files = {v: [] for v in variables}
-        for name, stack in (('UTCI', utci), ('TMRT', tmrt)):
-            if name in variables:
-                p = folder / f'{name}.npy'
-                np.save(p, stack[(slice(None),) + crop])
-                day_files[name].append(str(p))
-        files['days'][date] = day_files
+    def stacks_of(tmrt, utci, extra=None):
+        got = {'UTCI': utci, 'TMRT': tmrt, **(extra or {})}
+        return {v: got[v] for v in variables}
 
+    def write(date, tmrt, utci, extra=None):
+        files['days'][date] = write_day(out / date, date, stacks_of(tmrt, utci, extra), rows_of[date],
+                                        output_chunk_steps, utc, crop, valid)
+
+    days = forcing.days
+    prefetch = len(days) > 2 and not plain and table_workers != 0
     engine = writer = None
+    pending = {}
     try:
-        for day in forcing.days:
+        for i, day in enumerate(days):
             n = day.values.shape[0]
             if progress:
                 print(f'[{datetime.now():%H:%M:%S}] {day.date}: {n} steps', flush=True)
-            sun = met_sun(day.values, dom.location, utc)
+            extra = None
             if plain:
-                tmrt, utci = _plain_day(dom, day.values, sun)
+                sun = met_sun(day.values, dom.location, utc)
+                tmrt, utci, extra = _plain_day(dom, day.values, sun, extras)
             else:
                 if engine is None or engine.n_hours != n:
                     if writer is not None:
                         writer.close()
-                    engine = batched.Engine([dom], n_hours=n)
+                    if engine is not None:
+                        engine.close_table_workers()
+                    engine = batched.Engine([dom], n_hours=n, extra_outputs=bool(extras),
+                                            keep_outputs=not no_write or return_arrays)
+                    pending = {}
+                    workers = engine.start_table_workers(table_workers) if prefetch else 0
+                    if save_svf and not no_write and 'SVF' not in files['static']:
+                        pd = engine.domains[0]
+                        svfbuveg = pd._svf - (1.0 - pd._svfveg) * (1.0 - pd.trans)
+                        files['static']['SVF'] = _write_static(out / 'static', 'SVF', svfbuveg, 'SVF', '1')
                     writer = None if no_write else DayOutputs(engine, slots=max(1, int(output_queue_size)))
-                engine.load(engine.tables(met=[day.values], sun=[sun]))
+                # the tables of the next days are built on the workers while this one runs
+                for j in range(i, min(len(days), i + 1 + (workers if prefetch else 0))):
+                    if j not in pending and days[j].values.shape[0] == n:
+                        pending[j] = engine.tables_async(met=[days[j].values])
+                engine.load(pending.pop(i).result())
                 engine.reset_state()
                 engine.forward_day()
                 if writer is not None:
                     writer.submit(day.date, write)
                     continue
+                if not return_arrays:
+                    files['days'][day.date] = {}
+                    continue
                 tmrt, utci = engine.outputs(0)
-            if no_write:
-                if valid is not None:
-                    utci = np.where(valid[None], utci, np.nan)
-                files['days'][day.date] = {'UTCI': utci[(slice(None),) + crop], 'TMRT': tmrt[(slice(None),) + crop]}
+                extra = engine.extra_outputs(0) if extras else None
+            if no_write and not return_arrays:
+                files['days'][day.date] = {}
+            elif no_write:
+                day_stacks = {}
+                for v, st in stacks_of(tmrt, utci, extra).items():
+                    a = np.array(st[(slice(None),) + crop], dtype=np.float32)
+                    if valid is not None:
+                        a[:, ~valid] = np.nan
+                    day_stacks[v] = a
+                files['days'][day.date] = day_stacks
             else:
-                write(day.date, tmrt, utci)
+                write(day.date, tmrt, utci, extra)
     finally:
         if writer is not None:
             writer.close()
+        if engine is not None:
+            engine.stream.sync()
+            engine.close_table_workers()
     files['days'] = dict(sorted(files['days'].items()))
     return files
 
 
-def _plain_day(dom, met, sun):
-    """One day through the per-step path (the plain form, series.py:445-469)."""
+def _plain_day(dom, met, sun, extras=()):
+    """One day through the per-step path (the plain form, series.py:445-469): (Tmrt, UTCI,
+    {extra variable: stack} or None)."""
     from . import model
     from .static import PreparedDomain
     pd = PreparedDomain(dom)
     prep = model.prepare_from_domain(pd, dom.location, dom.scale)
-    return plain_day(prep, met, sun, ground=pd.dev['ground'])
+    return plain_day(prep, met, sun, ground=pd.dev['ground'], extras=extras)
 
 
-def plain_day(prep, met, sun, ground=None, keep=True):
+def plain_day(prep, met, sun, ground=None, keep=True, extras=None):
     """Hour loop of series._run_day over model.Solweig_2022a_calc-style steps; returns host
-    (Tmrt, UTCI) stacks (``keep``) or leaves the last hour on the device."""
+    (Tmrt, UTCI) stacks (``keep``) or leaves the last hour on the device. With ``extras`` (a list
+    of rt.EXTRA_NAMES) the third value holds their host stacks from the full-mode planes."""
     from . import model, utci as u
     n = met.shape[0]
     dectime = sun['dectime']
     timestepdec = (dectime[1] - dectime[0]) if n > 1 else 1 / 24.
     CI, fdt, timeadd = 1., 1, 0.
     lags = [0.0] * 6
-    tm, ut = [], []
+    tm, ut, ex = [], [], {}
     for i in range(n):
         if dectime[i] - np.floor(dectime[i]) == 0:
             CI = 1.
@@ -197,8 +299,14 @@ def plain_day(prep, met, sun, ground=None, keep=True):
         if keep:
             tm.append(tmrt.to_host())
             ut.append(utci.to_host())
+            for name in extras or ():
+                ex.setdefault(name, []).append(prep.full.plane(FULL_OF[name], prep.shape).to_host())
+    if extras is not None:
+        return np.stack(tm), np.stack(ut), ({k: np.stack(v) for k, v in ex.items()} if extras else None)
     return (np.stack(tm), np.stack(ut)) if keep else (tmrt, utci)
 
+
+FULL_OF = {'Kdown': 0, 'Kup': 1, 'Ldown': 2, 'Lup': 3, 'Shadow': 19}   # rt.FULL_PLANES positions
 
 PATH_KEYS = {'building_dsm_path', 'tree_path', 'dem_path', 'met_file', 'building_geometry_path',
              'building_height_path', 'landcover_path', 'output_path'}
@@ -226,7 +334,8 @@ def load_case(path):
     return case, settings
 
 
-def run_case(case_path, *, output=None, no_write=False, progress=False, batch_tiles=True, **overrides):
+def run_case(case_path, *, output=None, no_write=False, progress=False, batch_tiles=True, return_arrays=False,
+             **overrides):
     """run.py main for a local case (run.py:20-67): the case.json fields (``overrides`` replace
     them), and with ``core_area_m`` / ``buffer_area_m`` the spatial tiles of run.py:114-129, each
     written under ``output/<tile_id>``. ``batch_tiles`` runs the tiles as batches of one launch
@@ -254,5 +363,5 @@ def run_case(case_path, *, output=None, no_write=False, progress=False, batch_ti
     for label, options in tiling.tile_jobs(case, settings, tiles_dir):
         destination = output / label if 'core_area_m' in settings else output
         results[label] = run_timeseries(**{**options, 'met_file': forcing}, output_path=destination,
-                                        no_write=no_write, progress=progress)
+                                        no_write=no_write, progress=progress, return_arrays=return_arrays)
     return results if 'core_area_m' in settings else results[next(iter(results))]

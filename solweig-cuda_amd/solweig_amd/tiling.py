@@ -142,7 +142,7 @@ def _groups(tiles, max_pixels):
 
 def run_tiles(tiles, met_file, output_path, longitude, latitude, *, utc_offset_hours=None, site_altitude_m=3.,
               dsm_wall_threshold_m=3., pixel_size_m=1.0, save_tmrt=True, no_write=False, output_queue_size=2,
-              max_batch_pixels=DEFAULT_BATCH_PIXELS, progress=False):
+              max_batch_pixels=DEFAULT_BATCH_PIXELS, output_chunk_steps=24, progress=False):
     """Simulate every tile of ``tiles`` (``create_tiles`` output) over the forcing, the tiles of a
     group (up to ``max_batch_pixels`` input pixels; ``batch_bytes`` gives the device and
     page-locked host memory a group holds) as one batch of the batched engine. Outputs per
@@ -152,7 +152,7 @@ def run_tiles(tiles, met_file, output_path, longitude, latitude, *, utc_offset_h
     from datetime import datetime
     from . import batched
     from .output import DayOutputs
-    from .series import _raster, load_meteorology, make_domain
+    from .series import _raster, load_meteorology, make_domain, write_day
     from .sun import met_sun
     forcing = load_meteorology(met_file)
     utc = float(longitude) / 15.0 if utc_offset_hours is None else float(utc_offset_hours)
@@ -160,23 +160,19 @@ def run_tiles(tiles, met_file, output_path, longitude, latitude, *, utc_offset_h
     variables = ['UTCI'] + (['TMRT'] if save_tmrt else [])
     results = {t['tile_id']: {'days': {}} for t in tiles}
 
+    rows_of = {d.date: d.values for d in forcing.days}
+
     def writer_for(tile):
         r0, c0, h, w = tile['output_core']
-        crop = (slice(None), slice(r0, r0 + h), slice(c0, c0 + w))
+        window = (slice(r0, r0 + h), slice(c0, c0 + w))
 
         def write(date, tmrt, utci):
-            day = {}
-            for name, stack in (('UTCI', utci), ('TMRT', tmrt)):
-                if name not in variables:
-                    continue
-                if no_write:
-                    day[name] = np.array(stack[crop])
-                    continue
-                folder = out / tile['tile_id'] / date
-                folder.mkdir(parents=True, exist_ok=True)
-                p = folder / f'{name}.npy'
-                np.save(p, stack[crop])
-                day[name] = [str(p)]
+            stacks = {name: stack for name, stack in (('UTCI', utci), ('TMRT', tmrt)) if name in variables}
+            if no_write:
+                day = {name: np.array(stack[(slice(None),) + window]) for name, stack in stacks.items()}
+            else:
+                day = write_day(out / tile['tile_id'] / date, date, stacks, rows_of[date], output_chunk_steps, utc,
+                                crop=window)
             results[tile['tile_id']]['days'][date] = day
         return write
 

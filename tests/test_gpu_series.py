@@ -36,6 +36,7 @@ def test_run_timeseries_writes_every_day(rt, tmp_path, slots):
         utci = np.load(files['UTCI'][0])
         t1, u1 = batched.run_day(dom, met=d.met[24 * k:24 * (k + 1)])
         u1 = np.where(mask[None], u1, np.nan)
+        t1 = np.where(mask[None], t1, np.nan)          # the valid mask applies to every variable
         assert np.array_equal(tmrt, t1, equal_nan=True), date
         assert np.array_equal(utci, u1, equal_nan=True), date
 
@@ -44,6 +45,41 @@ def test_run_timeseries_no_write_returns_arrays(rt, tmp_path):
     from solweig_amd import series, synthetic
     d = synthetic.make_c1(seed=22, size=40, days=1)
     res = series.run_timeseries(d.dsm, d.trees, d.dem, d.met, tmp_path, d.longitude, d.latitude,
-                                utc_offset_hours=0.0, no_write=True)
+                                utc_offset_hours=0.0, no_write=True, return_arrays=True)
     (day,) = res['days'].values()
     assert day['UTCI'].shape == (24, 40, 40) and np.isfinite(day['TMRT']).all()
+    # the reference's no-write mode keeps nothing: every day maps to an empty dict
+    res = series.run_timeseries(d.dsm, d.trees, d.dem, d.met, tmp_path, d.longitude, d.latitude,
+                                utc_offset_hours=0.0, no_write=True)
+    assert list(res['days'].values()) == [{}] and not (tmp_path / 'static').exists()
+
+
+def test_run_timeseries_extra_variables_chunks_and_static(rt, tmp_path):
+    """save_kup/kdown/lup/ldown/shadow through the batched pass's extra planes, split into
+    10-band files with their tags; they agree with the per-step full-mode planes (plain=True),
+    and the static walls, aspect and SVF files are written."""
+    import json
+    from solweig_amd import series, synthetic
+    d = synthetic.make_c1(seed=23, size=48, days=1)
+    kw = dict(utc_offset_hours=0.0, save_kup=True, save_kdown=True, save_lup=True, save_ldown=True,
+              save_shadow=True, save_svf=True, output_chunk_steps=10)
+    print('run_timeseries: extra variables, 10-band chunks, batched and plain', flush=True)
+    res = series.run_timeseries(d.dsm, d.trees, d.dem, d.met, tmp_path / 'b', d.longitude, d.latitude, **kw)
+    ref = series.run_timeseries(d.dsm, d.trees, d.dem, d.met, tmp_path / 'p', d.longitude, d.latitude, plain=True,
+                                **kw)
+    assert set(res['static']) == {'walls', 'aspect', 'SVF'}
+    svf = np.load(res['static']['SVF'])
+    assert svf.shape == (48, 48) and 0 < svf.min() and svf.max() <= 1
+    (date, files), = res['days'].items()
+    assert set(files) == {'UTCI', 'TMRT', 'Kup', 'Kdown', 'Lup', 'Ldown', 'Shadow'}
+    for var, paths in files.items():
+        assert [p.rsplit('/', 1)[1] for p in paths] == [f'{var}_000{k}.npy' for k in (1, 2, 3)]
+        got = np.concatenate([np.load(p) for p in paths])
+        exp = np.concatenate([np.load(p) for p in ref['days'][date][var]])
+        assert got.shape == (24, 48, 48)
+        tags = json.loads(open(paths[2].replace('.npy', '.json')).read())
+        assert tags['variable'] == var and len(tags['bands']) == 4
+        if var == 'Shadow':
+            np.testing.assert_array_equal(got, exp)
+        elif var != 'UTCI':
+            np.testing.assert_allclose(got, exp, rtol=2e-4, atol=2e-2, err_msg=var)

@@ -35,7 +35,7 @@ def test_full_halo_tiles_match_the_untiled_run(rt, tmp_path):
     res = tiling.run_tiles(tiles, d.met, tmp_path / 'out', d.longitude, d.latitude, utc_offset_hours=0.0,
                            no_write=True)
     ref = series.run_timeseries(d.dsm, d.trees, d.dem, d.met, tmp_path / 'ref', d.longitude, d.latitude,
-                                utc_offset_hours=0.0, no_write=True)
+                                utc_offset_hours=0.0, no_write=True, return_arrays=True)
     (date, want), = ref['days'].items()
     for t in tiles:
         col, row, w, h = t['core_window']
@@ -58,7 +58,8 @@ def test_tile_batch_equals_tiles_one_by_one(rt, tmp_path):
     print('tiled case: batched tiles', flush=True)
     batch = series.run_case(tmp_path / 'case.json', output=tmp_path / 'a', no_write=True)
     print('tiled case: one run per tile', flush=True)
-    single = series.run_case(tmp_path / 'case.json', output=tmp_path / 'b', no_write=True, batch_tiles=False)
+    single = series.run_case(tmp_path / 'case.json', output=tmp_path / 'b', no_write=True, batch_tiles=False,
+                             return_arrays=True)
     assert sorted(batch) == sorted(single) == ['r000_c000', 'r000_c001', 'r001_c000', 'r001_c001']
     for tile_id in batch:
         (date, a), = batch[tile_id]['days'].items()
