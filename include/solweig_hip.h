@@ -319,6 +319,23 @@ int sw_surface_temperature(const void *const *fields, const long long *counts, i
                            const double *params, int n_params, int real_bytes, void *out_dev, void *stream);
 int sw_assemble_tmrt(const void *const *fields, const long long *counts, int n_fields, long long n,
                      const double *params, int n_params, int real_bytes, void *out_dev, void *stream);
+/* The elementwise terms the reference's step evaluates as CuPy expressions (replaces
+ * model.py:120-126, 165-171, 183-195), one plane each, same conventions as above:
+ *   op 0  shadow = sh - (1 - vegsh)(1 - psi)   fields sh, vegsh (FP64); params psi
+ *   op 1  Kdown                                fields shadow, dRad, svfbuveg, F_sh;
+ *                                              params radI, sin(altitude), albedo_b, radG, radD
+ *   op 2  TgTemp = Tg shadow + Ta              fields Tg, shadow; params Ta
+ *   op 3  night Lup                            fields emis, lc_grid (FP64);
+ *                                              params Ta, SBC, landcover, water Lup
+ *   op 4  NaN replaced by a value              fields x; params value
+ *   op 5  a + b                                fields a, b; no params                      */
+int sw_step_terms(int op, const void *const *fields, const long long *counts, int n_fields, long long n,
+                  const double *params, int n_params, int real_bytes, void *out_dev, void *stream);
+/* dRad = radD sum_k diffsh[k][p] lum[k] (replaces model._diffuse, model.py:1019-1027, and the
+ * diffuse kernel of shortwave.cu:98-107): diffsh_dev [patches][n] of the working precision,
+ * lum_dev [patches] FP64.                                                                   */
+int sw_diffuse(const void *diffsh_dev, const double *lum_dev, long long n, int patches, double radD,
+               int real_bytes, void *out_dev, void *stream);
 
 /* Static part of the ground view of a domain (the walk of gvf_2018a depends on the hour only
  * through Lup, albedo x shadow and the sunlit walls): per pixel and direction the number of

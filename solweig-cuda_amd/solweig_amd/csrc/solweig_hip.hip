@@ -2640,6 +2640,65 @@ __global__ void k_sub_assemble(FieldSet F, long long n, R absK, R absL, R Fside,
 }
 
 // Host side: one launcher per sub-model, dispatching on the working precision.
+// The elementwise terms of one step that the reference evaluates as CuPy expressions
+// (model.py:120-126, 165-171, 183-195), in the working precision R with contraction off and the
+// reference's operation order:
+//   op 0  shadow = sh - (1 - vegsh)(1 - psi)        fields sh, vegsh (FP64)     params psi
+//   op 1  Kdown = radI shadow sin(alt) + dRad + albedo_b (1 - svfbuveg)(radG (1 - F_sh) + radD F_sh)
+//                                                   fields shadow, dRad, svfbuveg, F_sh
+//                                                   params radI, sin(alt), albedo_b, radG, radD
+//   op 2  TgTemp = Tg shadow + Ta                   fields Tg, shadow           params Ta
+//   op 3  night Lup = SBC emis (Ta + 273.15)^4, SBC 0.98 (Twater + 273.15)^4 on water with land cover
+//                                                   fields emis, lc_grid (FP64) params Ta, SBC, landcover, water Lup
+//   op 4  x, with NaN replaced by v                 fields x                    params v
+//   op 5  a + b (the cardinal longwave sums, model.py:244-251)  fields a, b     no params
+constexpr int TERMS_NFIELDS[6] = {2, 4, 2, 2, 1, 2};
+constexpr int TERMS_NPARAMS[6] = {1, 5, 1, 4, 1, 0};
+
+template <typename R>
+__global__ void k_step_terms(int op, FieldSet F, long long n, Coef7 q, R *out) {
+#pragma clang fp contract(off)
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    R v;
+    switch (op) {
+    case 0: v = (R)(fld<double>(F, 0, i) - (1.0 - fld<double>(F, 1, i)) * (1.0 - q.c[0])); break;
+    case 1: {
+        const R shadow = fld<R>(F, 0, i), dRad = fld<R>(F, 1, i), svfbuveg = fld<R>(F, 2, i), fsh = fld<R>(F, 3, i);
+        const R direct = (R)q.c[0] * shadow * (R)q.c[1] + dRad;
+        v = direct + (R)q.c[2] * (R(1) - svfbuveg) * ((R)q.c[3] * (R(1) - fsh) + (R)q.c[4] * fsh);
+        break;
+    }
+    case 2: v = fld<R>(F, 0, i) * fld<R>(F, 1, i) + (R)q.c[0]; break;
+    case 3: {
+        const R t = (R)q.c[0] + R(273.15);
+        v = (R)q.c[1] * fld<R>(F, 0, i) * pow(t, R(4));
+        if ((int)q.c[2] == 1 && fld<double>(F, 1, i) == 3.0) v = (R)q.c[3];
+        break;
+    }
+    case 4: {
+        const R x = fld<R>(F, 0, i);
+        v = isnan(x) ? (R)q.c[0] : x;
+        break;
+    }
+    default: v = fld<R>(F, 0, i) + fld<R>(F, 1, i); break;
+    }
+    out[i] = v;
+}
+
+// dRad = radD x sum_k diffsh[k][p] lum[k] (the reference's diffuse kernel, shortwave.cu:98-107): the
+// products in FP64, the running sum and the result rounded to R.
+template <typename R>
+__global__ void k_diffuse(const R *__restrict__ vis, const double *__restrict__ lum, long long n, int patches,
+                          double radD, R *__restrict__ out) {
+#pragma clang fp contract(off)
+    const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    R sum = R(0);
+    for (int k = 0; k < patches; ++k) sum = sum + (R)((double)vis[(long long)k * n + p] * lum[k]);
+    out[p] = (R)((double)sum * radD);
+}
+
 template <typename R>
 int launch_sub(int which, const FieldSet &F, long long n, const double *q, void *out, hipStream_t s) {
     const dim3 grid((unsigned)((n + 255) / 256)), block(256);
@@ -3494,6 +3553,43 @@ SW_SUB_ENTRY(sw_lside, 3)
 SW_SUB_ENTRY(sw_surface_temperature, 4)
 SW_SUB_ENTRY(sw_assemble_tmrt, 5)
 #undef SW_SUB_ENTRY
+
+int sw_step_terms(int op, const void *const *fields, const long long *counts, int n_fields, long long n,
+                  const double *params, int n_params, int real_bytes, void *out, void *stream) {
+    if (!fields || !counts || !params || !out) return fail(SW_ERR_ARGUMENT, "sw_step_terms: NULL pointer");
+    if (op < 0 || op > 5) return fail(SW_ERR_ARGUMENT, "sw_step_terms: op is 0..5");
+    if (n_fields != TERMS_NFIELDS[op] || n_params != TERMS_NPARAMS[op])
+        return fail(SW_ERR_ARGUMENT, "sw_step_terms: wrong number of fields or parameters");
+    if (real_bytes != 4 && real_bytes != 8) return fail(SW_ERR_ARGUMENT, "sw_step_terms: precision is 4 or 8 bytes");
+    if (n <= 0) return SW_OK;
+    FieldSet F = {};
+    const int rc = make_fieldset(fields, counts, n_fields, n, F);
+    if (rc) return rc;
+    Coef7 q = {};
+    for (int k = 0; k < n_params; ++k) q.c[k] = params[k];
+    const dim3 grid((unsigned)((n + 255) / 256)), block(256);
+    if (real_bytes == 8)
+        hipLaunchKernelGGL(k_step_terms<double>, grid, block, 0, (hipStream_t)stream, op, F, n, q, (double *)out);
+    else
+        hipLaunchKernelGGL(k_step_terms<float>, grid, block, 0, (hipStream_t)stream, op, F, n, q, (float *)out);
+    return hip_check(hipGetLastError(), "k_step_terms launch");
+}
+
+int sw_diffuse(const void *diffsh, const double *lum, long long n, int patches, double radD, int real_bytes, void *out,
+               void *stream) {
+    if (!diffsh || !lum || !out) return fail(SW_ERR_ARGUMENT, "sw_diffuse: NULL pointer");
+    if (real_bytes != 4 && real_bytes != 8) return fail(SW_ERR_ARGUMENT, "sw_diffuse: precision is 4 or 8 bytes");
+    if (patches <= 0) return fail(SW_ERR_ARGUMENT, "sw_diffuse: no patches");
+    if (n <= 0) return SW_OK;
+    const dim3 grid((unsigned)((n + 255) / 256)), block(256);
+    if (real_bytes == 8)
+        hipLaunchKernelGGL(k_diffuse<double>, grid, block, 0, (hipStream_t)stream, (const double *)diffsh, lum, n,
+                           patches, radD, (double *)out);
+    else
+        hipLaunchKernelGGL(k_diffuse<float>, grid, block, 0, (hipStream_t)stream, (const float *)diffsh, lum, n,
+                           patches, radD, (float *)out);
+    return hip_check(hipGetLastError(), "k_diffuse launch");
+}
 
 int sw_gvf(const void *const *fields, const long long *counts, int n_fields, int rows, int cols,
            const int32_t *offsets, const double *bounds, int stride, int first, int second,

@@ -62,11 +62,17 @@ def _host(v):
     return np.asarray(v.get() if hasattr(v, 'get') else v)
 
 
-def prepare_model(*, dtype=np.float32, **inputs):
-    """Upload the static fields of series._prepare_spatial's dict (model.py:987-1016)."""
+def prepare_model(*, dtype=np.float32, stepwise=False, **inputs):
+    """Upload the static fields of series._prepare_spatial's dict (model.py:987-1016). FP32 (the
+    default) prepares the fused hour passes; ``dtype=np.float64``, or ``stepwise=True`` in FP32,
+    prepares the reference's decomposition of the step into its sub-model calls
+    (stepwise.PreparedSteps), which Solweig_2022a_calc then runs."""
     dtype = np.dtype(dtype)
+    if dtype == np.float64 or stepwise:
+        from .stepwise import prepare_stepwise
+        return prepare_stepwise(dtype=dtype, **inputs)
     if dtype != np.float32:
-        raise ValueError('the device path computes radiation in float32')
+        raise ValueError('dtype must be float32 or float64')
     if not int(inputs.get('anisotropic_sky', 1)) or int(inputs.get('patch_option', 2)) != 2 \
             or not int(inputs.get('usevegdem', 1)) or not int(inputs.get('cyl', 1)):
         raise ValueError('supported configuration: cyl=1, anisotropic_sky=1, usevegdem=1, patch_option=2')
@@ -169,13 +175,32 @@ def Solweig_2022a_calc(i, dsm, scale, rows, cols, svf, svfN, svfW, svfE, svfS, s
                        svfbuveg, firstdaytime, timeadd, timestepdec, Tgmap1, Tgmap1E, Tgmap1S, Tgmap1W,
                        Tgmap1N, CI, TgOut1, diffsh, shmat, vegshmat, vbshvegshmat, anisotropic_sky, asvf,
                        patch_option, *, dtype=np.float32, prepared=None):
-    """One SOLWEIG 2022a step (model.py:15-256): returns the 39-tuple of OUTPUT_NAMES."""
+    """One SOLWEIG 2022a step (model.py:15-256): returns the 39-tuple of OUTPUT_NAMES. With a
+    stepwise preparation (dtype float64, or prepare_model(stepwise=True)) the step runs as the
+    reference's sequence of sub-model calls in that precision; otherwise as the two fused FP32
+    passes."""
+    from .stepwise import PreparedSteps, step as stepwise_step
+    if prepared is None and np.dtype(dtype) == np.float64:
+        args = {k: v for k, v in locals().items() if k not in ('prepared', 'PreparedSteps', 'stepwise_step')}
+        prepared = prepare_model(**args)
+    if isinstance(prepared, PreparedSteps):
+        out = stepwise_step(prepared, altitude=altitude, azimuth=azimuth, zen=zen, jday=jday, dectime=dectime,
+                            altmax=altmax, Ta=Ta, RH=RH, radG=radG, radD=radD, radI=radI, P=P, psi=psi, CI=CI,
+                            firstdaytime=firstdaytime, timeadd=timeadd, timestepdec=timestepdec, Tgmap1=Tgmap1,
+                            Tgmap1E=Tgmap1E, Tgmap1S=Tgmap1S, Tgmap1W=Tgmap1W, Tgmap1N=Tgmap1N, TgOut1=TgOut1,
+                            location=location, landcover=int(float(landcover)), Twater=Twater,
+                            albedo_b=float(albedo_b), absK=float(absK), absL=float(absL), ewall=float(ewall),
+                            Fside=float(Fside), Fup=float(Fup), Fcyl=float(Fcyl), cyl=int(cyl), elvis=float(elvis),
+                            onlyglobal=int(onlyglobal), TgK_wall=float(TgK_wall), Tstart_wall=float(Tstart_wall),
+                            TmaxLST_wall=float(TmaxLST_wall))
+        return tuple(out[n] for n in OUTPUT_NAMES)
     consts = (float(albedo_b), float(ewall), float(absK), float(absL), float(Fside), float(Fup), float(Fcyl))
     if consts != (0.2, 0.9, 0.7, 0.95, 0.22, 0.06, 0.28) or float(elvis) != 0 or int(onlyglobal) != 1:
         raise ValueError('the device path is built for the series.py constants (albedo_b 0.2, ewall 0.9, '
                          'absK 0.7, absL 0.95, Fside 0.22, Fup 0.06, Fcyl 0.28, elvis 0, onlyglobal 1)')
     if prepared is None:
-        args = {k: v for k, v in locals().items() if k not in ('prepared', 'consts')}
+        args = {k: v for k, v in locals().items() if k not in ('prepared', 'consts', 'PreparedSteps',
+                                                               'stepwise_step')}
         prepared = prepare_model(**args)
     prep = prepared
     prep.location = location

@@ -145,7 +145,7 @@ EXPORTED = ('sw_version', 'sw_record_bytes', 'sw_last_error', 'sw_device_count',
             'sw_adjoint_hour', 'sw_day_adjoint', 'sw_cylindric_wedge', 'sw_wave_delay', 'sw_kup', 'sw_lside',
             'sw_surface_temperature', 'sw_assemble_tmrt', 'sw_gvf', 'sw_pack_visibility', 'sw_longwave',
             'sw_kside', 'sw_dsm_walls', 'sw_memcpy_d2h_async', 'sw_stream_wait_event', 'sw_gvf_static',
-            'sw_gvf_fixed_steps', 'sw_gvf_noalloc')
+            'sw_gvf_fixed_steps', 'sw_gvf_noalloc', 'sw_step_terms', 'sw_diffuse')
 
 
 def check(rc):

@@ -316,3 +316,45 @@ def prepared_from_arrays(*, dsm, vegdem, walls, dirwalls, buildings, svfbuveg, a
     self.skyview = None
     self.tile_cols = rt.TILE_X
     return self
+
+
+def model_inputs(prep, location=None):
+    """The static dict of series._prepare_spatial (series.py:343-365) for a domain prepared with
+    ``PreparedDomain(domain, keep_host=True)``: the keyword inputs of model.prepare_model and the
+    static arguments of Solweig_2022a_calc. SVF planes are rounded to float32 (series.py:337) in
+    the UMEP tuple order, masks are uint8 [rows, cols, 153] unpacked from the device bits, diffsh
+    is compute_diffsh's FP32 3-value plane (svf.py:117-131)."""
+    from .svf import UMEP_NAMES, UMEP_ORDER, compute_diffsh, unpack_visibility
+    h = prep.host
+    a = h['a']
+    t = np.asarray(h['trees'], dtype=np.float64)
+    rows, cols = prep.shape
+    vegdem = t + a
+    vegdem[vegdem == a] = 0
+    vegdem2 = t * 0.25 + a
+    vegdem2[vegdem2 == a] = 0
+    bush = np.logical_not(vegdem2 * vegdem) * vegdem
+    planes = np.asarray(h['svf_planes'])
+    svf = {name: planes[k].astype(np.float32).astype(np.float64) for name, k in zip(UMEP_NAMES, UMEP_ORDER)}
+    shm, vgm, vbm = unpack_visibility(prep.dev['vis'].to_host(), prep.shape)
+    d = prep.domain
+    maps = h['maps']
+    f64 = lambda x: np.asarray(x, dtype=np.float32).astype(np.float64)
+    if prep.landcover:
+        lc = normalise_landcover(d.landcover).astype(np.float64)
+    else:
+        lc = np.zeros((rows, cols))
+    loc = location or {'latitude': float(d.latitude), 'longitude': float(d.longitude),
+                       'altitude': float(getattr(d, 'site_altitude', 3.0))}
+    out = dict(dsm=a, scale=float(d.scale), rows=rows, cols=cols, vegdem=vegdem, vegdem2=vegdem2,
+               albedo_b=CONSTANTS['albedo_b'], absK=CONSTANTS['absK'], absL=CONSTANTS['absL'],
+               ewall=CONSTANTS['ewall'], Fside=CONSTANTS['Fside'], Fup=CONSTANTS['Fup'], Fcyl=CONSTANTS['Fcyl'],
+               usevegdem=1, onlyglobal=1, buildings=np.where(h['ground'], 1.0, 0.0), location=loc,
+               landcover=int(prep.landcover), lc_grid=lc, dirwalls=f64(d.aspect), walls=f64(d.walls), cyl=1,
+               elvis=0.0, amaxvalue=float(prep.amaxvalue), bush=bush, TgK=f64(maps['tgk']), Tstart=f64(maps['tstart']),
+               alb_grid=f64(maps['alb']), emis_grid=f64(maps['emis']), TgK_wall=prep.wall_params[0],
+               Tstart_wall=prep.wall_params[1], TmaxLST=f64(maps['tmax']), TmaxLST_wall=prep.wall_params[2],
+               first=1.0, second=22.0, svfalfa=h['svfalfa'], svfbuveg=h['svfbuveg'],
+               diffsh=compute_diffsh(shm, vgm, prep.trans), shmat=shm.astype(np.uint8), vegshmat=vgm.astype(np.uint8),
+               vbshvegshmat=vbm.astype(np.uint8), anisotropic_sky=1, asvf=h['asvf'], patch_option=2, **svf)
+    return out
