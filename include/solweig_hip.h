@@ -328,7 +328,9 @@ int sw_assemble_tmrt(const void *const *fields, const long long *counts, int n_f
  *   op 3  night Lup                            fields emis, lc_grid (FP64);
  *                                              params Ta, SBC, landcover, water Lup
  *   op 4  NaN replaced by a value              fields x; params value
- *   op 5  a + b                                fields a, b; no params                      */
+ *   op 5  a + b                                fields a, b; no params
+ *   op 6  x rounded to the working precision   fields x (FP64); no params (the float32
+ *                                              Tmrt that utci_calculator takes, utci.py:8-13)  */
 int sw_step_terms(int op, const void *const *fields, const long long *counts, int n_fields, long long n,
                   const double *params, int n_params, int real_bytes, void *out_dev, void *stream);
 /* dRad = radD sum_k diffsh[k][p] lum[k] (replaces model._diffuse, model.py:1019-1027, and the

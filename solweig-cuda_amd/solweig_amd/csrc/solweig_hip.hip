@@ -2652,8 +2652,10 @@ __global__ void k_sub_assemble(FieldSet F, long long n, R absK, R absL, R Fside,
 //                                                   fields emis, lc_grid (FP64) params Ta, SBC, landcover, water Lup
 //   op 4  x, with NaN replaced by v                 fields x                    params v
 //   op 5  a + b (the cardinal longwave sums, model.py:244-251)  fields a, b     no params
-constexpr int TERMS_NFIELDS[6] = {2, 4, 2, 2, 1, 2};
-constexpr int TERMS_NPARAMS[6] = {1, 5, 1, 4, 1, 0};
+//   op 6  x rounded to R (utci_calculator's float32 Tmrt, utci.py:8-13)  fields x (FP64)  no params
+constexpr int TERMS_OPS = 7;
+constexpr int TERMS_NFIELDS[TERMS_OPS] = {2, 4, 2, 2, 1, 2, 1};
+constexpr int TERMS_NPARAMS[TERMS_OPS] = {1, 5, 1, 4, 1, 0, 0};
 
 template <typename R>
 __global__ void k_step_terms(int op, FieldSet F, long long n, Coef7 q, R *out) {
@@ -2681,7 +2683,8 @@ __global__ void k_step_terms(int op, FieldSet F, long long n, Coef7 q, R *out) {
         v = isnan(x) ? (R)q.c[0] : x;
         break;
     }
-    default: v = fld<R>(F, 0, i) + fld<R>(F, 1, i); break;
+    case 5: v = fld<R>(F, 0, i) + fld<R>(F, 1, i); break;
+    default: v = (R)fld<double>(F, 0, i); break;
     }
     out[i] = v;
 }
@@ -3557,7 +3560,7 @@ SW_SUB_ENTRY(sw_assemble_tmrt, 5)
 int sw_step_terms(int op, const void *const *fields, const long long *counts, int n_fields, long long n,
                   const double *params, int n_params, int real_bytes, void *out, void *stream) {
     if (!fields || !counts || !params || !out) return fail(SW_ERR_ARGUMENT, "sw_step_terms: NULL pointer");
-    if (op < 0 || op > 5) return fail(SW_ERR_ARGUMENT, "sw_step_terms: op is 0..5");
+    if (op < 0 || op >= TERMS_OPS) return fail(SW_ERR_ARGUMENT, "sw_step_terms: op is 0..6");
     if (n_fields != TERMS_NFIELDS[op] || n_params != TERMS_NPARAMS[op])
         return fail(SW_ERR_ARGUMENT, "sw_step_terms: wrong number of fields or parameters");
     if (real_bytes != 4 && real_bytes != 8) return fail(SW_ERR_ARGUMENT, "sw_step_terms: precision is 4 or 8 bytes");

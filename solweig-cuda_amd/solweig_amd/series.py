@@ -166,9 +166,15 @@ def run_timeseries(building_dsm_path, tree_path, dem_path, met_file, output_path
     ``save_svf``. ``no_write`` writes nothing and leaves the days empty, as the reference does;
     ``return_arrays`` (with ``no_write``) returns the host stacks instead. Forcing of more than two
     days builds the day tables ahead on ``table_workers`` CPU processes (default: host cores - 1,
-    at most 8; 0 builds them in this process)."""
+    at most 8; 0 builds them in this process). ``dtype=np.float64`` (run.py:28) runs every step
+    through the reference's decomposition in FP64 (stepwise_day); outputs are float32 files in
+    either precision (output.py:119)."""
     if wall_method != 'dsm':
         raise ValueError("only wall_method='dsm' is supported")
+    dtype = np.dtype(dtype)
+    if dtype not in (np.float32, np.float64):
+        raise ValueError('dtype must be float32 or float64')
+    fp64 = dtype == np.float64
     forcing = load_meteorology(met_file)
     utc = float(longitude) / 15.0 if utc_offset_hours is None else float(utc_offset_hours)
     dom = make_domain(_raster(building_dsm_path), _raster(tree_path), _raster(dem_path), forcing, latitude,
@@ -199,8 +205,8 @@ def run_timeseries(building_dsm_path, tree_path, dem_path, met_file, output_path
                                         output_chunk_steps, utc, crop, valid)
 
     days = forcing.days
-    prefetch = len(days) > 2 and not plain and table_workers != 0
-    engine = writer = None
+    prefetch = len(days) > 2 and not plain and not fp64 and table_workers != 0
+    engine = writer = steps = None
     pending = {}
     try:
         for i, day in enumerate(days):
@@ -208,7 +214,11 @@ def run_timeseries(building_dsm_path, tree_path, dem_path, met_file, output_path
             if progress:
                 print(f'[{datetime.now():%H:%M:%S}] {day.date}: {n} steps', flush=True)
             extra = None
-            if plain:
+            if fp64:
+                steps = steps or stepwise_context(dom)
+                sun = met_sun(day.values, dom.location, utc)
+                tmrt, utci, extra = stepwise_day(*steps, day.values, sun, extras)
+            elif plain:
                 sun = met_sun(day.values, dom.location, utc)
                 tmrt, utci, extra = _plain_day(dom, day.values, sun, extras)
             else:
@@ -304,6 +314,56 @@ def plain_day(prep, met, sun, ground=None, keep=True, extras=None):
     if extras is not None:
         return np.stack(tm), np.stack(ut), ({k: np.stack(v) for k, v in ex.items()} if extras else None)
     return (np.stack(tm), np.stack(ut)) if keep else (tmrt, utci)
+
+
+STEP_OF = {'Kdown': 'Kdown', 'Kup': 'Kup', 'Ldown': 'Ldown', 'Lup': 'Lup', 'Shadow': 'shadow'}
+STEP_CONSTANTS = ('albedo_b', 'absK', 'absL', 'ewall', 'Fside', 'Fup', 'Fcyl', 'cyl', 'elvis', 'onlyglobal',
+                  'TgK_wall', 'Tstart_wall', 'TmaxLST_wall')
+
+
+def stepwise_context(dom, dtype=np.float64):
+    """(PreparedDomain, static inputs, stepwise.PreparedSteps) of a domain for stepwise_day: the
+    static preparation of series._prepare_spatial (series.py:243-369) with the model prepared in
+    ``dtype``."""
+    from . import model
+    from .static import PreparedDomain, model_inputs
+    pd = PreparedDomain(dom, keep_host=True)
+    static = model_inputs(pd)
+    return pd, static, model.prepare_model(dtype=dtype, stepwise=True, **static)
+
+
+def stepwise_day(pd, static, prep, met, sun, extras=()):
+    """One day through the reference's step decomposition (stepwise.step, the dtype of ``prep``),
+    the hour loop of series._run_day (series.py:396-469): host float32 stacks (Tmrt, UTCI,
+    {extra variable: stack} or None). UTCI takes the float32-rounded Tmrt and is NaN off the ground
+    mask, as utci_calculator and series.py:468-469 make it."""
+    from . import stepwise, utci as u
+    n = met.shape[0]
+    dectime = sun['dectime']
+    timestepdec = (dectime[1] - dectime[0]) if n > 1 else 1 / 24.
+    CI, fdt, timeadd = 1., 1., 0.
+    state = dict.fromkeys(('Tgmap1', 'Tgmap1E', 'Tgmap1S', 'Tgmap1W', 'Tgmap1N', 'TgOut1'), 0.0)
+    twater = float(np.mean(met[:, 11])) if static['landcover'] else None
+    consts = {k: static[k] for k in STEP_CONSTANTS}
+    tm, ut, ex = [], [], {name: [] for name in extras}
+    for i in range(n):
+        if dectime[i] - np.floor(dectime[i]) == 0:
+            CI = 1.
+        o = stepwise.step(prep, altitude=sun['altitude'][i], azimuth=sun['azimuth'][i], zen=sun['zen'][i],
+                          jday=sun['jday'][i], dectime=dectime[i], altmax=sun['altmax'][i], Ta=met[i, 11],
+                          RH=met[i, 10], radG=met[i, 14], radD=met[i, 21], radI=met[i, 22], P=met[i, 12],
+                          psi=0.03 if 97 < met[i, 1] < 300 else 0.5, CI=CI, firstdaytime=fdt, timeadd=timeadd,
+                          timestepdec=timestepdec, location=static['location'], landcover=static['landcover'],
+                          Twater=twater, **state, **consts)
+        CI, fdt, timeadd = o['CI'], o['firstdaytime'], o['timeadd']
+        state = {k: o[k] for k in state}
+        t32 = stepwise.as_float32(o['Tmrt'], prep)
+        utci = u.utci_calculator(met[i, 11], met[i, 10], t32, met[i, 9], ground=pd.dev['ground'])
+        tm.append(t32.to_host())
+        ut.append(utci.to_host())
+        for name in extras:
+            ex[name].append(stepwise.as_float32(o[STEP_OF[name]], prep).to_host())
+    return np.stack(tm), np.stack(ut), ({k: np.stack(v) for k, v in ex.items()} if extras else None)
 
 
 FULL_OF = {'Kdown': 0, 'Kup': 1, 'Ldown': 2, 'Lup': 3, 'Shadow': 19}   # rt.FULL_PLANES positions

@@ -84,16 +84,17 @@ def prepare_stepwise(*, dtype=np.float32, **inputs):
     return PreparedSteps(fields, shadow, ground, lw, night, sw, sky_, dtype, shape, veg)
 
 
-def _terms(op, values, params, prep, fp64_fields=()):
+def _terms(op, values, params, prep, fp64_fields=(), out_dtype=None):
     n = prep.shape[0] * prep.shape[1]
-    P, C, keep = field_table(values, n, prep.dtype, fp64_fields)
-    out = rt.DeviceArray(prep.shape, prep.dtype)
+    D = prep.dtype if out_dtype is None else np.dtype(out_dtype)
+    P, C, keep = field_table(values, n, D, fp64_fields)
+    out = rt.DeviceArray(prep.shape, D)
     Q = (ctypes.c_double * max(1, len(params)))(*[float(x) for x in params])
     f = rt.lib().sw_step_terms
     vp, ci = ctypes.c_void_p, ctypes.c_int
     f.argtypes = [ci, vp, vp, ci, ctypes.c_longlong, vp, ci, ci, vp, vp]
     f.restype = ci
-    rt.check(f(int(op), P, C, len(values), n, Q, len(params), prep.dtype.itemsize, out.ptr, None))
+    rt.check(f(int(op), P, C, len(values), n, Q, len(params), D.itemsize, out.ptr, None))
     rt.check(rt.lib().sw_stream_sync(None))
     del keep
     return out
@@ -229,6 +230,13 @@ def step(prep, *, altitude, azimuth, zen, jday, dectime, altmax, Ta, RH, radG, r
             Tgmap1E, Tgmap1S, Tgmap1W, Tgmap1N, Keast, Ksouth, Kwest, Knorth, Least, Lsouth, Lwest, Lnorth, KsideI,
             TgOut1, TgOut, radI, radD, Lside, L_patches, CI_Tg, CI_TgG, KsideD, dRad, Kside)
     return dict(zip(model.OUTPUT_NAMES, vals))
+
+
+def as_float32(plane, prep):
+    """A plane of the step rounded to float32 on the device (an FP32 plane is returned as it is)."""
+    if plane.dtype == np.float32:
+        return plane
+    return _terms(6, (plane,), (), prep, fp64_fields=(0,), out_dtype=np.float32)
 
 
 def _add(a, b, prep):

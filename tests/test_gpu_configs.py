@@ -143,3 +143,25 @@ def test_long_ray_masks_bit_exact(rt, jobs, tile):
         np.testing.assert_array_equal(masks[h], ref[h], err_msg=f'hour {h} (sun {oj.LOW_SUNS[h - 5]})')
     shaded = [float(1 - (masks[h] & 1).mean()) for h in range(5, 9)]
     print(f'  masks identical; building-shadow fractions {np.round(shaded, 3).tolist()}', flush=True)
+
+
+@pytest.mark.timeout(300)
+@pytest.mark.parametrize('name', ['leafoff_jan', 'c0_paved'])
+def test_fp64_series_day_matches_oracle(rt, jobs, name):
+    """The dtype=float64 series day (the reference's step decomposition in FP64 on the device,
+    outputs rounded to float32 as the reference's output path writes them) against the FP64 oracle
+    day: Tmrt within half a float32 ulp of the oracle plus 1e-6 K, UTCI (from the float32 Tmrt)
+    within 1e-3 K, the same NaN pattern."""
+    from solweig_amd import series
+    d = oj.domain(name)
+    s = oj.sun(name, d)
+    print(f'fp64 series day {name}: {d.shape[0]}x{d.shape[1]}', flush=True)
+    ctx = series.stepwise_context(d)
+    tmrt, utci, _ = series.stepwise_day(*ctx, d.met, s)
+    ref = jobs[name].result(timeout=280)
+    excess = np.abs(tmrt.astype(np.float64) - ref['tmrt']) - 0.5 * np.spacing(np.abs(tmrt)).astype(np.float64)
+    g = np.isfinite(ref['utci'])
+    eu = float(np.max(np.abs(utci[g] - ref['utci'][g])))
+    print(f'  Tmrt beyond float32 rounding {float(excess.max()):.2e} K; max |dUTCI| {eu:.2e} K', flush=True)
+    assert float(excess.max()) < 1e-6
+    assert eu < 1e-3 and np.array_equal(np.isnan(utci), ~g)

@@ -83,3 +83,26 @@ def test_run_timeseries_extra_variables_chunks_and_static(rt, tmp_path):
             np.testing.assert_array_equal(got, exp)
         elif var != 'UTCI':
             np.testing.assert_allclose(got, exp, rtol=2e-4, atol=2e-2, err_msg=var)
+
+
+def test_run_timeseries_fp64(rt, tmp_path):
+    """dtype=np.float64 (run.py:28) runs the day through the FP64 step decomposition; it stays
+    within the FP32 bar of the batched FP32 series and keeps its NaN pattern and shadow."""
+    from solweig_amd import series, synthetic
+    d = synthetic.make_c1(seed=24, size=40, days=1)
+    kw = dict(utc_offset_hours=0.0, no_write=True, return_arrays=True, save_shadow=True, save_kdown=True)
+    print('run_timeseries: float64 vs float32', flush=True)
+    r64 = series.run_timeseries(d.dsm, d.trees, d.dem, d.met, tmp_path, d.longitude, d.latitude,
+                                dtype=np.float64, **kw)
+    r32 = series.run_timeseries(d.dsm, d.trees, d.dem, d.met, tmp_path, d.longitude, d.latitude, **kw)
+    (a,), (b,) = r64['days'].values(), r32['days'].values()
+    assert set(a) == set(b) == {'UTCI', 'TMRT', 'Shadow', 'Kdown'}
+    for v in a:
+        assert a[v].dtype == np.float32 and np.array_equal(np.isnan(a[v]), np.isnan(b[v])), v
+    assert float(np.nanmax(np.abs(a['TMRT'] - b['TMRT']))) < 0.05
+    assert float(np.nanmax(np.abs(a['UTCI'] - b['UTCI']))) < 0.05
+    np.testing.assert_allclose(a['Shadow'], b['Shadow'], atol=1e-6)
+    np.testing.assert_allclose(a['Kdown'], b['Kdown'], rtol=2e-4, atol=2e-2)
+    with pytest.raises(ValueError):
+        series.run_timeseries(d.dsm, d.trees, d.dem, d.met, tmp_path, d.longitude, d.latitude, dtype=np.float16,
+                              **kw)
