@@ -3,15 +3,22 @@
 A step is one day (24 hourly steps) of the batched forward over the 2000x2000 c2 workload
 (seeded synthetic morphology, its 30-day forcing cycled): per step the host builds the day's
 tables (solar position, clearness, Perez, patch rows, rays) in CPU worker processes, the engine
-uploads them, and the device replays the day's 24 x (shadow pass + pixel pass) as a captured
-hipGraph; Tmrt and UTCI stay in device memory. Static preparation (walls, sky view) runs before
-the timed region and is reported apart.
+uploads them, and the device replays the day as a captured hipGraph, in the launch form that a
+timed comparison before the warm-up picks: the resident day (two launches: the shadow pass of all
+24 hours, then the pixel pass in which every tile runs through the 24 hours with its lag state in
+registers) or two launches per hour. Tmrt and UTCI stay in device memory. Static preparation
+(walls, sky view) runs before the timed region and is reported apart.
 
 Beside the headline the JSON line carries:
-* ``config.eager``: the same days launched eagerly with HIP events around every pass, which give
+* ``config.eager``: the same days launched eagerly with HIP events around both launches, which give
   the per-kernel durations of ``roofline`` (dominant kernel: the pixel pass);
+* ``config.launch_form`` / ``other_launch_form``: the timed choice between the resident day and
+  two launches per hour (lag state through its device planes), and the same days in the form not
+  chosen;
 * ``config.plain_forward``: the per-step drop-in API (host tables, uploads, two full-output
-  launches, the UTCI kernel and a stream sync every hour);
+  launches, the UTCI kernel and a stream sync every hour), and under ``stepwise`` the per-step
+  forward in the reference's own decomposition (about 20 standalone sub-model launches, each
+  followed by a sync, per daytime hour) with the batched / stepwise ratio;
 * ``config.forward_adjoint``: forward vs forward + adjoint on the 1000x1000 differentiation workload;
 * ``config.c4``: the 64-domain uneven batch, one launch per pass and hour, tables built inside its
   timed region;
@@ -215,8 +222,9 @@ class DayPipeline:
 
 def run_days(eng, pipe, n, use_graph, events=None):
     """``n`` days from the pipeline: per day the upload (stream-ordered behind the previous day),
-    the lag reset and the 24 hours. With ``events`` (one list of 3 x 24 + 1 per day) every pass is
-    bracketed by HIP events."""
+    the lag reset and the 24 hours. With ``events`` (one list of 3 x 24 + 1 per day) every launch
+    is bracketed by HIP events: the two launches of a resident engine's day (events 0-2), or the
+    shadow and pixel launch of every hour (events 3h .. 3h + 2)."""
     from solweig_amd import runtime as rt
     used = []
     for k in range(n):
@@ -225,6 +233,15 @@ def run_days(eng, pipe, n, use_graph, events=None):
         eng.reset_state()
         if events is None:
             eng.forward_day(use_graph=use_graph)
+        elif eng.resident:
+            ev = events[k]
+            args = (eng.dom_dev.ptr, eng.block_start.ptr, len(eng.domains), eng.total_blocks, 0, eng.n_hours)
+            ev[0].record(eng.stream)
+            rt.call('sw_day_shadow', *args, eng.stream.handle)
+            ev[1].record(eng.stream)
+            rt.call('sw_day_pixels', *args, eng.gvf_off.ptr, eng.gvf_bounds.ptr, eng.first, eng.second,
+                    eng.stream.handle)
+            ev[2].record(eng.stream)
         else:
             ev = events[k]
             for h in range(eng.n_hours):
@@ -290,11 +307,45 @@ def c4_leg(args, workers):
         log(f'[bench] c4 domain {k} ({sides[k]}^2) alone vs in the batch: identical {same}')
         del alone
     eng.close_table_workers()
+    audit = c4_audit(eng, doms, sides, args.c4_audit)
     return dict(value=value, unit='pixel-hours/s', ms_per_day=dt / args.c4_steps * 1e3, steps=args.c4_steps,
                 workload=f'c4 uneven batch: {len(doms)} domains, sides {min(sides)}..{max(sides)}, {P} pixels, '
                          f'24 h per step, one launch per pass and hour, day tables built in the timed region',
                 tile=args.tile or 16, generation_s=t_gen, static_prep_s=t_prep,
-                host_tables_one_core_ms=t_tab_serial * 1e3, table_workers=eng.workers, checks=checks)
+                host_tables_one_core_ms=t_tab_serial * 1e3, table_workers=eng.workers, checks=checks,
+                fp64_audit=audit)
+
+
+def c4_audit(eng, doms, sides, count):
+    """FP32 batched outputs of c4 domains (smallest, median, largest, ...) against the FP64 step
+    decomposition of the same day on the device (series.stepwise_day in FP64, which equals the FP64
+    oracle to float32 rounding): max and RMS |dTmrt|, |dUTCI| in K over the pixel-hours."""
+    from solweig_amd import series
+    from solweig_amd.sun import met_sun
+    order = list(np.argsort(sides, kind='stable'))
+    picks = [order[0], order[len(order) // 2], order[-1]] + order[1:-1:max(1, len(order) // max(1, count - 3))]
+    out = []
+    for k in [int(i) for i in dict.fromkeys(picks)][:max(0, count)]:
+        d = doms[k]
+        tm, ut = eng.outputs(k)
+        t0 = time.perf_counter()
+        ctx = series.stepwise_context(d, dtype=np.float64)
+        m = d.met[:24]
+        t64, u64, _ = series.stepwise_day(*ctx, m, met_sun(m, d.location, d.utc))
+        del ctx
+        dt_ = np.abs(tm.astype(np.float64) - t64)
+        g = np.isfinite(u64) & np.isfinite(ut)
+        du = np.abs(ut[g].astype(np.float64) - u64[g])
+        same_nan = bool(np.array_equal(np.isnan(ut), np.isnan(u64)))
+        row = dict(domain=k, side=sides[k], pixel_hours=int(tm.size), tmrt_max=float(dt_.max()),
+                   tmrt_rms=float(np.sqrt(np.mean(dt_ ** 2))), utci_max=float(du.max()),
+                   utci_rms=float(np.sqrt(np.mean(du ** 2))), utci_nan_pattern_equal=same_nan,
+                   seconds=time.perf_counter() - t0)
+        out.append(row)
+        log(f'[bench] c4 FP64 audit domain {k} ({sides[k]}^2): Tmrt max {row["tmrt_max"]:.2e} rms '
+            f'{row["tmrt_rms"]:.2e} K, UTCI max {row["utci_max"]:.2e} rms {row["utci_rms"]:.2e} K, '
+            f'NaN pattern equal {same_nan} ({row["seconds"]:.1f} s)')
+    return out
 
 
 def e2e_leg(repeats=3):
@@ -349,6 +400,10 @@ def main():
     ap.add_argument('--size', type=int, default=2000)
     ap.add_argument('--eager', action='store_true', help='headline with eager launches instead of the hipGraph')
     ap.add_argument('--eager-days', type=int, default=3, help='days of the eager, event-timed leg (0 = skip)')
+    ap.add_argument('--form', choices=('auto', 'resident', 'hourly'), default='auto',
+                    help='day launch form: timed choice, resident day (2 launches), or 2 launches per hour')
+    ap.add_argument('--other-days', type=int, default=3,
+                    help='days timed in the launch form not chosen for the headline (0 = skip)')
     ap.add_argument('--workers', type=int, default=0, help='table worker processes (0 = host cores - 1, at most 8)')
     ap.add_argument('--no-workers', action='store_true',
                     help='build the day tables in this process (for profiler runs, which must not spawn)')
@@ -361,9 +416,13 @@ def main():
                     help='fixed tile width (16, 32, 64, 128) instead of the autotuner (for profiling runs)')
     ap.add_argument('--plain-days', type=int, default=1,
                     help='also time the plain per-step forward for this many days (0 = skip)')
+    ap.add_argument('--stepwise-days', type=int, default=1,
+                    help='also time the reference-decomposition per-step forward for this many days (0 = skip)')
     ap.add_argument('--no-c4', action='store_true', help='skip the 64-domain uneven batch leg')
     ap.add_argument('--c4-steps', type=int, default=3)
     ap.add_argument('--c4-count', type=int, default=64, help='domains of the c4 batch')
+    ap.add_argument('--c4-audit', type=int, default=3,
+                    help='c4 domains audited against the FP64 step decomposition (smallest, median, largest, ...)')
     ap.add_argument('--no-e2e', action='store_true', help='skip the end-to-end run_timeseries leg')
     args = ap.parse_args()
     if args.gpus != 1:
@@ -398,8 +457,22 @@ def main():
     eng.workers = n_workers = 0 if args.no_workers else eng.start_table_workers(workers)
     log(f'[bench] {n_workers} table worker processes')
     use_graph = not args.eager
-    n_total = args.warmup + args.steps + max(0, args.eager_days)
+    n_total = args.warmup + args.steps + max(0, args.eager_days) + (1 + args.other_days if args.other_days > 0 else 0)
     pipe = DayPipeline(eng, [k % days for k in range(n_total)])
+    # Launch form: the first three days, cycled with their uploads, in both forms (resident day and
+    # two launches per hour), alternated; the engine keeps the faster (autotune.tune_launch_form).
+    # The device also reaches its sustained clocks here after idling while the workers started.
+    # Outputs of these days are overwritten by the days that follow.
+    form = None
+    if args.form == 'auto':
+        from solweig_amd import autotune
+        t0 = time.perf_counter()
+        form = autotune.tune_launch_form(eng, [eng.tables(day_index=k) for k in range(3)], use_graph=use_graph)
+        log(f'[bench] launch form: {"resident day" if form["resident"] else "two launches per hour"} '
+            f'(ms per replayed day: {form["ms"]}; {time.perf_counter() - t0:.2f} s)')
+    else:
+        eng.resident, eng.graph = args.form == 'resident', None
+    resident = eng.resident
     run_days(eng, pipe, args.warmup, use_graph)
     rt.synchronize()
     log(f'[bench] {args.warmup} warm-up days done')
@@ -421,38 +494,59 @@ def main():
         etabs = run_days(eng, pipe, args.eager_days, False, events=evs)
         rt.synchronize()
         edt = time.perf_counter() - t0
-        t_sh = t_px = 0.0
-        n_day_hours = 0
-        for k in range(args.eager_days):
-            per_hour = []
-            for h in range(24):
-                if etabs[k][0].hours[h, 0]:
-                    n_day_hours += 1
-                a = evs[k][3 * h].elapsed_ms(evs[k][3 * h + 1])
-                b = evs[k][3 * h + 1].elapsed_ms(evs[k][3 * h + 2])
-                t_sh += a
-                t_px += b
-                per_hour.append(f'{h}:{a:.3f}/{b:.3f}(alt {etabs[k][0].hours[h, 1]:.0f})')
-            log(f'[bench] eager day {k} per hour shadow/pixel ms: ' + ' '.join(per_hour))
-        launches = 24 * args.eager_days
+        n_day_hours = sum(int(etabs[k][0].hours[h, 0] != 0) for k in range(args.eager_days) for h in range(24))
+        if eng.resident:
+            # two launches per day: the shadow pass of all hours, the pixel pass of all hours
+            t_sh = sum(evs[k][0].elapsed_ms(evs[k][1]) for k in range(args.eager_days))
+            t_px = sum(evs[k][1].elapsed_ms(evs[k][2]) for k in range(args.eager_days))
+            launches, hours_per_launch = args.eager_days, 24
+            names = ('k_hour_shadow (all hours)', 'k_day_pixels')
+        else:
+            t_sh = t_px = 0.0
+            for k in range(args.eager_days):
+                for h in range(24):
+                    t_sh += evs[k][3 * h].elapsed_ms(evs[k][3 * h + 1])
+                    t_px += evs[k][3 * h + 1].elapsed_ms(evs[k][3 * h + 2])
+            launches, hours_per_launch = 24 * args.eager_days, 1
+            names = ('k_hour_shadow', 'k_hour_pixels')
         sh_ms, px_ms = t_sh / launches, t_px / launches
         eager = dict(value=P * 24 * args.eager_days / edt, ms_per_day=edt / args.eager_days * 1e3,
                      days=args.eager_days, device_ms_per_day=(t_sh + t_px) / args.eager_days,
-                     avg_ms=dict(k_hour_shadow=sh_ms, k_hour_pixels=px_ms),
+                     avg_ms={names[0]: sh_ms, names[1]: px_ms}, hours_per_launch=hours_per_launch,
                      daytime_hours_per_day=n_day_hours / args.eager_days)
-        dom_ms, dom_name, dom_b = (px_ms, 'k_hour_pixels', B_PIXELS) if px_ms >= sh_ms else \
-            (sh_ms, 'k_hour_shadow', B_SHADOW)
-        achieved = dom_b * P / (dom_ms * 1e-3)
-        pair = B_ALG * P / ((sh_ms + px_ms) * 1e-3)
+        dom_ms, dom_name, dom_b = (px_ms, names[1], B_PIXELS) if px_ms >= sh_ms else (sh_ms, names[0], B_SHADOW)
+        dom_bytes = dom_b * P * hours_per_launch
+        achieved = dom_bytes / (dom_ms * 1e-3)
+        pair = B_ALG * P * hours_per_launch / ((sh_ms + px_ms) * 1e-3)
         roof = dict(bound='hbm', achieved=achieved / 1e9, peak=HBM_PEAK / 1e9, unit='GB/s', frac=achieved / HBM_PEAK,
-                    traffic=TRAFFIC_PIXELS_2000 if (args.size == 2000 and dom_name == 'k_hour_pixels') else None,
+                    traffic=TRAFFIC_PIXELS_2000 * hours_per_launch if (args.size == 2000 and dom_b == B_PIXELS)
+                    else None,
                     traffic_source=f'{TRAFFIC_SOURCE} (rocprofv3 FETCH_SIZE x copy calibration + WRITE_SIZE; '
                                    f'a profile value, not measured in this run)',
-                    kernel=dom_name, algorithmic_bytes_per_launch=dom_b * P, avg_ms=dom_ms,
-                    pair=dict(kernels='k_hour_shadow + k_hour_pixels (one hour)', achieved=pair / 1e9,
-                              frac=pair / HBM_PEAK, algorithmic_bytes=B_ALG * P, avg_ms=sh_ms + px_ms))
-        log(f'[bench] eager: {eager["ms_per_day"]:.2f} ms/day; avg shadow {sh_ms:.4f} ms, pixels {px_ms:.4f} ms; '
-            f'{dom_name} {achieved / 1e9:.0f} GB/s = {100 * achieved / HBM_PEAK:.1f} % of HBM')
+                    kernel=dom_name, algorithmic_bytes_per_launch=dom_bytes, avg_ms=dom_ms,
+                    algorithmic_bytes_per_pixel_hour=dom_b, hours_per_launch=hours_per_launch,
+                    pair=dict(kernels=f'{names[0]} + {names[1]}', achieved=pair / 1e9, frac=pair / HBM_PEAK,
+                              algorithmic_bytes=B_ALG * P * hours_per_launch, avg_ms=sh_ms + px_ms))
+        log(f'[bench] eager: {eager["ms_per_day"]:.2f} ms/day; per launch shadow {sh_ms:.4f} ms, pixels {px_ms:.4f} ms '
+            f'({hours_per_launch} h per launch); {dom_name} {achieved / 1e9:.0f} GB/s = '
+            f'{100 * achieved / HBM_PEAK:.1f} % of HBM')
+    other = None
+    if args.other_days > 0:
+        # the same days in the other launch form (resident day <-> two launches per hour)
+        chosen = eng.resident
+        eng.resident, eng.graph = not chosen, None
+        run_days(eng, pipe, 1, use_graph)
+        rt.synchronize()
+        t0 = time.perf_counter()
+        run_days(eng, pipe, args.other_days, use_graph)
+        rt.synchronize()
+        odt = time.perf_counter() - t0
+        eng.resident, eng.graph = chosen, None
+        other = dict(form='two launches per hour (sw_hours_forward)' if chosen else 'resident day (sw_day_forward)',
+                     value=P * 24 * args.other_days / odt, ms_per_day=odt / args.other_days * 1e3,
+                     days=args.other_days, headline_over_other=value / (P * 24 * args.other_days / odt))
+        log(f'[bench] other launch form ({other["form"]}): {other["ms_per_day"]:.2f} ms/day '
+            f'({other["value"]:.3e} px-h/s); headline / other = {other["headline_over_other"]:.3f}')
     eng.close_table_workers()
 
     plain = None
@@ -476,6 +570,34 @@ def main():
                           'kernel + stream sync every hour, outputs kept on the device')
         log(f'[bench] plain per-step forward {plain["value"]:.3e} px-h/s ({plain["ms_per_day"]:.1f} ms/day)')
         del prep
+    if args.stepwise_days > 0:
+        from solweig_amd import series
+        from solweig_amd.sun import met_sun
+        t0 = time.perf_counter()
+        ctx = series.stepwise_context(dom, dtype=np.float32)
+        rt.synchronize()
+        setup = time.perf_counter() - t0
+        mets = [dom.met[k * 24:(k + 1) * 24] for k in range(days)]
+        series.stepwise_day(*ctx, mets[0], met_sun(mets[0], dom.location, dom.utc))
+        rt.synchronize()
+        t0 = time.perf_counter()
+        for k in range(args.stepwise_days):
+            m = mets[(k + 1) % days]
+            series.stepwise_day(*ctx, m, met_sun(m, dom.location, dom.utc))
+        rt.synchronize()
+        sdt = time.perf_counter() - t0
+        sv = P * 24 * args.stepwise_days / sdt
+        stepwise = dict(value=sv, ms_per_day=sdt / args.stepwise_days * 1e3, batched_over_stepwise=value / sv,
+                        setup_s=setup,
+                        note='the reference decomposition of the step (model.py:15-256): standalone sub-model '
+                             'kernels in FP32 in Solweig_2022a_calc order, a stream sync after each, host '
+                             'scalars per hour, UTCI kernel, Tmrt and UTCI copied to the host every hour')
+        if plain is None:
+            plain = {}
+        plain['stepwise'] = stepwise
+        log(f'[bench] stepwise (reference decomposition) forward {sv:.3e} px-h/s ({stepwise["ms_per_day"]:.0f} '
+            f'ms/day; batched / stepwise = {value / sv:.1f}x; setup {setup:.1f} s)')
+        del ctx
     if args.copy_calibration:
         src = rt.DeviceArray((64 << 20,), np.float32)
         dst = rt.DeviceArray((64 << 20,), np.float32)
@@ -496,8 +618,11 @@ def main():
                                     f'cycled), day tables built '
                                     f'{f"by {n_workers} CPU worker processes" if n_workers else "in the main process"}'
                                     f' inside the timed region, '
-                                    f'{"hipGraph day replay" if use_graph else "eager launches"}',
-                           graph=use_graph, static_prep=static, tile=tuned, eager=eager, plain_forward=plain,
+                                    f'{"hipGraph replay" if use_graph else "eager launches"} of the '
+                                    f'{"resident day (sw_day_forward)" if resident else "two launches per hour (sw_hours_forward)"}',
+                           graph=use_graph, launch_form=form, resident=resident, static_prep=static, tile=tuned,
+                           eager=eager, other_launch_form=other,
+                           plain_forward=plain,
                            forward_adjoint=adj, c4=c4, end_to_end=e2e),
                roofline=roof, cpu_baseline=cpu)
     print(json.dumps(out), flush=True)

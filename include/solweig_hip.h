@@ -101,9 +101,10 @@ typedef struct sw_domain {
     const double *tan_asvf, *tan_svfalfa;                    /* [P]                           */
     const uint8_t *ground;                                   /* [P] 1 where dsm-dem < 2        */
     const uint32_t *vis;                                     /* [3][5][P] sh, veg, vb bits     */
-    void *rec;            /* [P] 8-byte hour records written by the shadow pass: the Lup field of
-                             sunonsurface_2018a, then albedo x shadow whose sign bit marks a
-                             sunlit wall (gvf.cu fields 3-5)                                  */
+    void *rec;            /* [P] 8-byte hour records written by the shadow pass ([hours][P] for
+                             sw_day_*: one slot per hour): the Lup field of sunonsurface_2018a,
+                             then albedo x shadow whose sign bit marks a sunlit wall (gvf.cu
+                             fields 3-5)                                                      */
     float *lag;                                              /* [6][P] Tgmap1, E, S, W, N, TgOut1 */
     float *tmrt, *utci;                                      /* [n_hours][P] (may be NULL)     */
     const double *hours;                                     /* [n_hours][SW_HOUR_STRIDE]     */
@@ -111,7 +112,8 @@ typedef struct sw_domain {
     const double *ptab_d;                                    /* [rows][SW_PATCH_D][153]       */
     const int32_t *ray_off;                                  /* [sum L][2] (row, col) shifts  */
     const double *ray_dz;                                    /* [sum L] ray heights            */
-    uint8_t *masks;       /* [P] bits of the last shadow pass (read by the pixel pass): 1 not
+    uint8_t *masks;       /* [P] ([hours][P] for sw_day_*) bits of the shadow pass (read by the
+                             pixel pass): 1 not
                              building-shadowed, 2 no canopy shadow, 4 vbshvegsh, 8 sunlit wall,
                              16 facesh                                                         */
     double *shadow_planes;                                   /* [8][P] cast_shadow planes      */
@@ -232,6 +234,21 @@ int sw_hour_pixels(const sw_domain *domains_dev, const int32_t *block_start_dev,
 int sw_hours_forward(const sw_domain *domains_dev, const int32_t *block_start_dev, int n_domains,
                      int total_blocks, int hour_begin, int hour_end, const int32_t *gvf_offsets_dev,
                      const double *gvf_bounds_dev, int first_steps, int second_steps, void *stream);
+/* The same hours in two launches (replaces the same day loop, series.py:445-469): the shadow
+ * pass of every hour, then one pixel pass in which each workgroup runs its tile through the hours
+ * in order with the lag state in registers (read from and written back to the lag planes once).
+ * Each domain's rec and masks planes must hold hour_end slots of P entries ([hours][P] x 8 B and
+ * [hours][P] x 1 B); hour h uses slot h.                                                    */
+int sw_day_forward(const sw_domain *domains_dev, const int32_t *block_start_dev, int n_domains,
+                   int total_blocks, int hour_begin, int hour_end, const int32_t *gvf_offsets_dev,
+                   const double *gvf_bounds_dev, int first_steps, int second_steps, void *stream);
+/* The two launches of sw_day_forward on their own: the shadow pass of every hour of the range
+ * (one launch, a workgroup row per hour), and the pixel pass of the range.                   */
+int sw_day_shadow(const sw_domain *domains_dev, const int32_t *block_start_dev, int n_domains,
+                  int total_blocks, int hour_begin, int hour_end, void *stream);
+int sw_day_pixels(const sw_domain *domains_dev, const int32_t *block_start_dev, int n_domains,
+                  int total_blocks, int hour_begin, int hour_end, const int32_t *gvf_offsets_dev,
+                  const double *gvf_bounds_dev, int first_steps, int second_steps, void *stream);
 
 /* ----- adjoint of J = mean over ground pixels of UTCI at one objective hour -------------------
  * (no counterpart in the reference: the differentiable form of the per-step chain

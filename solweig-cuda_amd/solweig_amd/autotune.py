@@ -91,3 +91,36 @@ def tune(engine, tables=None, candidates=rt.TILE_CHOICES, repeats=3, use_cache=T
     if use_cache:
         _store_cache(key, result)
     return dict(result, cached=False)
+
+
+def tune_launch_form(engine, tables=None, days=10, rounds=3, use_graph=True):
+    """Pick how the engine launches a day: resident (sw_day_forward: the shadow pass of every hour
+    in one launch, then one pixel pass with the lag state in registers) or two launches per hour
+    (sw_hours_forward). ``tables``: a list of day tables (one day's list per entry) cycled with an
+    upload before every timed day, as a run of consecutive days does (default: the loaded day,
+    replayed). Times ``days`` days in each form, alternating the forms for ``rounds`` rounds (so
+    that clock and cache drift falls on both), keeps the faster by the best round and leaves it set
+    on the engine; returns {'resident', 'ms'} (ms per day)."""
+    cycle = list(tables) if tables is not None else [None]
+    ev = [rt.Event() for _ in range(2)]
+    best = {True: float('inf'), False: float('inf')}
+
+    def day(k):
+        if cycle[k % len(cycle)] is not None:
+            engine.load(cycle[k % len(cycle)])
+        engine.reset_state()
+        engine.forward_day(use_graph=use_graph)
+
+    for _ in range(rounds):
+        for resident in (True, False):
+            engine.resident, engine.graph = resident, None
+            day(0)                                          # capture / warm this form
+            ev[0].record(engine.stream)
+            for k in range(days):
+                day(k + 1)
+            ev[1].record(engine.stream)
+            engine.stream.sync()
+            best[resident] = min(best[resident], ev[0].elapsed_ms(ev[1]) / days)
+    engine.resident, engine.graph = best[True] <= best[False], None
+    engine.reset_state()
+    return dict(resident=engine.resident, ms={'resident': best[True], 'hourly': best[False]})

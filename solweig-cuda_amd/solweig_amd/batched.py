@@ -23,9 +23,13 @@ from .sun import met_sun
 
 
 class Engine:
-    def __init__(self, domains, n_hours=24, stream=None, keep_outputs=True, extra_outputs=False):
+    def __init__(self, domains, n_hours=24, stream=None, keep_outputs=True, extra_outputs=False, resident=True):
         """``extra_outputs``: also keep each hour's Kdown, Kup, Ldown, Lup and Shadow planes
-        ([n_hours][5][P] per domain, rt.EXTRA_NAMES order), the optional series outputs."""
+        ([n_hours][5][P] per domain, rt.EXTRA_NAMES order), the optional series outputs.
+        ``resident``: run a day as two launches (sw_day_forward: the shadow pass of every hour into
+        per-hour record and mask planes, 9 B per pixel-hour, then one pixel pass in which each tile
+        runs through the hours with its lag state in registers) instead of two launches per hour
+        (sw_hours_forward)."""
         scales = {float(d.domain.scale if isinstance(d, PreparedDomain) else d.scale) for d in domains}
         if len(scales) != 1:
             # one launch shares one ground-view step table (the step shifts and counts follow the
@@ -52,7 +56,12 @@ class Engine:
                 b['utci'] = rt.DeviceArray((self.n_hours, d.P), np.float32)
             if extra_outputs:
                 b['extra'] = rt.DeviceArray((self.n_hours, rt.SW_EXTRA_PLANES, d.P), np.float32)
+            # the shadow pass's record (Lup, albedo x shadow | sunlit wall) and mask byte of every
+            # hour; the single-hour launches use the first slot
+            b['rec'] = rt.DeviceArray((self.n_hours if resident else 1, d.P, 2), np.float32)
+            b['masks'] = rt.DeviceArray((self.n_hours if resident else 1, d.P), np.uint8)
             self.buf.append(b)
+        self.resident = bool(resident)
         self.block_start = rt.DeviceArray((len(self.domains) + 1,), np.int32)
         self.structs = (rt.SwDomain * len(self.domains))()
         self.dom_dev = rt.DeviceArray((ctypes.sizeof(self.structs),), np.uint8)
@@ -82,7 +91,8 @@ class Engine:
                                        ptab_s=b['ptab_s'].ptr, ray_off=b['ray_off'].ptr, ray_dz=b['ray_dz'].ptr,
                                        tmrt=b['tmrt'].ptr if 'tmrt' in b else 0,
                                        utci=b['utci'].ptr if 'utci' in b else 0,
-                                       full=b['extra'].ptr if 'extra' in b else 0)
+                                       full=b['extra'].ptr if 'extra' in b else 0,
+                                       rec=b['rec'].ptr, masks=b['masks'].ptr)
         rt.call('sw_memcpy_h2d', self.dom_dev.ptr, ctypes.addressof(self.structs),
                 ctypes.sizeof(self.structs), self.stream.handle)
         self.stream.sync()
@@ -175,9 +185,15 @@ class Engine:
 
     # ------------------------------------------------------------------ launches
     def _launch_hours(self, h0, h1):
-        rt.call('sw_hours_forward', self.dom_dev.ptr, self.block_start.ptr, len(self.domains),
-                self.total_blocks, h0, h1, self.gvf_off.ptr, self.gvf_bounds.ptr, self.first,
+        rt.call('sw_day_forward' if self.resident else 'sw_hours_forward', self.dom_dev.ptr, self.block_start.ptr,
+                len(self.domains), self.total_blocks, h0, h1, self.gvf_off.ptr, self.gvf_bounds.ptr, self.first,
                 self.second, self.stream.handle)
+
+    def run_hours(self, h0, h1):
+        """Enqueue hours [h0, h1) of the loaded day (the lag state carries over from hour h0 - 1)."""
+        if not 0 <= h0 <= h1 <= self.n_hours:
+            raise ValueError(f'hours [{h0}, {h1}) outside the day of {self.n_hours}')
+        self._launch_hours(int(h0), int(h1))
 
     def forward_day(self, use_graph=False):
         """Enqueue all hours of the loaded day (state must be reset by the caller per day)."""
@@ -195,6 +211,14 @@ class Engine:
         rt.call('sw_hour_pixels', self.dom_dev.ptr, self.block_start.ptr, len(self.domains),
                 self.total_blocks, h, self.gvf_off.ptr, self.gvf_bounds.ptr, self.first, self.second,
                 int(full), self.stream.handle)
+
+    def hour_masks(self, k=0):
+        """Host mask bytes [rows, cols] of domain k from the last single-hour launch (``hour``):
+        bit 0 not building-shadowed, 1 no canopy shadow, 2 vbshvegsh, 3 sunlit wall, 4 wall facing
+        away from the sun."""
+        self.stream.sync()
+        d = self.domains[k]
+        return self.buf[k]['masks'].to_host()[0].reshape(d.shape)
 
     def outputs(self, k=0):
         """Host (Tmrt, UTCI) [n_hours, rows, cols] of domain k for the last day run."""

@@ -15,7 +15,11 @@ FLAGS = ['--offload-arch=gfx950', '-O3', '-std=c++17', '-fPIC', '-shared',
          # packed FP32 VALU forms stay disabled (they have returned wrong values on these parts)
          '-Xclang', '-target-feature', '-Xclang', '-packed-fp32-ops',
          # hardware float atomics (the adjoint's accumulations into device-memory buffers)
-         '-munsafe-fp-atomics']
+         '-munsafe-fp-atomics',
+         # no machine-level loop-invariant code motion: in the day pixel pass (one loop over the
+         # hours around the whole per-hour body) it hoists immediates into vector registers held
+         # across the loop (158 instead of 105 VGPRs); the other kernels are unaffected in time
+         '-mllvm', '-disable-machine-licm']
 
 
 def build(verbose=False):

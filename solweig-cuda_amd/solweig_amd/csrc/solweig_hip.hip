@@ -66,6 +66,7 @@ struct PixelAt {
     int d, r, c;
     long long p;
     bool valid;
+    int tid;                    // the lane's index in the workgroup (threadIdx.x)
     int wr0, wr1, wc0, wc1;     // rows and columns spanned by this pixel's wave (uniform)
     int tr0, tc0, tx, ty;       // the workgroup's tile: first row and column, width, height
 };
@@ -173,6 +174,7 @@ __device__ __forceinline__ PixelAt locate(const sw_domain *doms, const int32_t *
     at.wc0 = tx > 64 ? tc0 + (w64 & (tx - 1)) : tc0;
     at.wc1 = tx > 64 ? at.wc0 + 63 : tc0 + tx - 1;
     at.tr0 = tr0; at.tc0 = tc0; at.tx = tx; at.ty = ty;
+    at.tid = (int)threadIdx.x;
     at.valid = at.r < rows && at.c < cols;
     at.p = (long long)at.r * cols + at.c;
     return at;
@@ -473,15 +475,22 @@ __device__ __forceinline__ void march_chunk(March &m, const AS1 float *dsm, cons
     }
 }
 
+// Hour hour0 + blockIdx.y. With per_hour the hour's record and mask byte go to slot ``hour`` of
+// the domain's rec / masks planes ([hours][P]), else to their single plane.
 template <bool FULL>
 __global__ __launch_bounds__(BLOCK) void k_hour_shadow(const sw_domain *__restrict__ doms,
                                                        const int32_t *__restrict__ bs, int nd,
-                                                       int total, int hour) {
+                                                       int total, int hour0, int per_hour) {
 #pragma clang fp contract(off)
     DomainG G;
     PixelAt at = locate(doms, bs, nd, total, G);
+    const int hour = hour0 + (int)blockIdx.y;
     const AS4 double *H = G.hours + (long long)hour * SW_HOUR_STRIDE;
     if (H[SWH_DAY] == 0.0) return;
+    if (per_hour) {
+        G.rec += (long long)hour * G.pixels;
+        if (G.masks) G.masks += (long long)hour * G.pixels;
+    }
     const int rows = (int)G.rows, cols = (int)G.cols;
     // Lanes past the raster (on the last tile row or column) march along with a copy of the
     // nearest pixel, so that the wave-wide thresholds below see every lane; they neither limit a
@@ -805,7 +814,7 @@ extern __shared__ v2f gv_lds[];
 // workgroup take part; the caller has checked that the region lies inside the raster).
 __device__ __forceinline__ void stage_tile(const DomainG &G, const PixelAt &at, int h, int pitch) {
     const int cols = (int)G.cols, nrows = at.ty + 2 * h;
-    const int wave = (int)threadIdx.x >> 6, lane = (int)threadIdx.x & 63;
+    const int wave = at.tid >> 6, lane = at.tid & 63;
     const AS1 v2f *rec = G.rec;
     for (int rr = wave; rr < nrows; rr += BLOCK / 64) {
         const long long src = (long long)(at.tr0 - h + rr) * cols + (at.tc0 - h);
@@ -917,7 +926,7 @@ template <int TX, int HH, bool CLAMP>
 __device__ __forceinline__ void stage_tile_fixed(const DomainG &G, const PixelAt &at) {
     constexpr int PITCH = TX + 2 * HH, NROWS = BLOCK / TX + 2 * HH;
     constexpr int WAVES = BLOCK / 64, ROWS = (NROWS + WAVES - 1) / WAVES, PASSES = (PITCH + 63) / 64;
-    const int rows = (int)G.rows, cols = (int)G.cols, wave = (int)threadIdx.x >> 6, lane = (int)threadIdx.x & 63;
+    const int rows = (int)G.rows, cols = (int)G.cols, wave = at.tid >> 6, lane = at.tid & 63;
     const AS1 v2f *rec = G.rec + (CLAMP ? 0LL : ((long long)(at.tr0 - HH) * cols + (at.tc0 - HH)));
     v2f v[ROWS][PASSES];
 #pragma unroll
@@ -1290,10 +1299,10 @@ constexpr int BAND_STEPS[8] = {5, 5, 5, 5, 5, 4, 3, 1};
 // Copy one hour row of the patch search tables into LDS (every thread of the workgroup calls;
 // the caller has synchronised the workgroup since the LDS was last read).
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
-__device__ __forceinline__ void stage_search(const AS1 uint32_t *src) {
+__device__ __forceinline__ void stage_search(const AS1 uint32_t *src, int tid) {
     AS3 u32x4 *dst = (AS3 u32x4 *)(AS3 void *)gv_lds;
     const AS1 u32x4 *s4 = (const AS1 u32x4 *)src;
-    for (int i = (int)threadIdx.x; i < SW_SEARCH_WORDS / 4; i += BLOCK) dst[i] = s4[i];
+    for (int i = tid; i < SW_SEARCH_WORDS / 4; i += BLOCK) dst[i] = s4[i];
     __syncthreads();
 }
 
@@ -1368,25 +1377,14 @@ __device__ __forceinline__ void day_patch_sums_tab(const AS1 uint32_t *vis, long
     }
 }
 
+// One hour of the pixel pass for one pixel of a workgroup's tile (every lane of the workgroup
+// calls it: it holds workgroup barriers). The pixel's lag state (the five Lup delays, and in FULL
+// mode TgOut's) is st; day hours update it in place, the caller loads and stores it. G.rec and
+// G.masks address the hour's shadow-pass record and mask planes.
 template <bool FULL>
-__global__ __launch_bounds__(BLOCK) void k_hour_pixels(const sw_domain *__restrict__ doms,
-                                                       const int32_t *__restrict__ bs, int nd,
-                                                       int total, int hour,
-                                                       const int32_t *__restrict__ gofs_,
-                                                       const double *__restrict__ gbounds_,
-                                                       int first, int second, int lds_cap) {
-    const AS4 int32_t *gofs = (const AS4 int32_t *)gofs_;
-    const AS4 double *gbounds = (const AS4 double *)gbounds_;
-    DomainG G;
-    PixelAt at = locate(doms, bs, nd, total, G);
-    // Lanes past the raster (partial tiles) compute a copy of the nearest pixel, so that every
-    // wave takes part in the workgroup's barriers, and write nothing.
-    const bool valid = at.valid;
-    if (!valid) {
-        at.r = min(at.r, (int)G.rows - 1);
-        at.c = min(at.c, (int)G.cols - 1);
-        at.p = (long long)at.r * G.cols + at.c;
-    }
+__device__ __forceinline__ void pixel_hour(const DomainG &G, const PixelAt &at, const bool valid, const int hour,
+                                           const AS4 int32_t *gofs, const AS4 double *gbounds, int first,
+                                           int second, int lds_cap, float (&st)[6]) {
     const AS4 double *H = G.hours + (long long)hour * SW_HOUR_STRIDE;
     const long long P = G.pixels, p = at.p;
     const bool day = H[SWH_DAY] != 0.0;
@@ -1435,22 +1433,17 @@ __global__ __launch_bounds__(BLOCK) void k_hour_pixels(const sw_domain *__restri
         const float amb = flux4 * emis;
         const double w = H[SWH_LAG_WEIGHT];
         const bool lf = H[SWH_LAG_FIRST] != 0.0, lu = H[SWH_LAG_UPDATE] != 0.0;
-        AS1 float *lg = G.lag;
-        float s0 = lg[p], s1 = lg[P + p], s2 = lg[2 * P + p], s3 = lg[3 * P + p], s4 = lg[4 * P + p];
-        Lup = lag(g.all.lup * INV18 + amb, s0, w, lf, lu);
-        LupE = lag(g.e.lup * INV9 + amb, s1, w, lf, lu);
-        LupS = lag(g.s.lup * INV9 + amb, s2, w, lf, lu);
-        LupW = lag(g.w.lup * INV9 + amb, s3, w, lf, lu);
-        LupN = lag(g.n.lup * INV9 + amb, s4, w, lf, lu);
-        if (valid) { lg[p] = s0; lg[P + p] = s1; lg[2 * P + p] = s2; lg[3 * P + p] = s3; lg[4 * P + p] = s4; }
+        Lup = lag(g.all.lup * INV18 + amb, st[0], w, lf, lu);
+        LupE = lag(g.e.lup * INV9 + amb, st[1], w, lf, lu);
+        LupS = lag(g.s.lup * INV9 + amb, st[2], w, lf, lu);
+        LupW = lag(g.w.lup * INV9 + amb, st[3], w, lf, lu);
+        LupN = lag(g.n.lup * INV9 + amb, st[4], w, lf, lu);
         if (FULL) {
             const double amp = (double)G.tgk[p] * H[SWH_ALTMAX] + (double)G.tstart[p];
             double t = amp * tg_sine(H, G.tmaxlst[p]) * H[SWH_CITGG];
             if (G.landcover == 1 && t < 0.0) t = 0.0;
             tg = (float)t;
-            float s5 = lg[5 * P + p];
-            tgout = lag((float)(t * (double)shadow + H[SWH_TA]), s5, w, lf, lu);
-            if (valid) lg[5 * P + p] = s5;
+            tgout = lag((float)(t * (double)shadow + H[SWH_TA]), st[5], w, lf, lu);
         }
         Fsh = wedge(G.tan_svfalfa[p], H[SWH_TANZEN]);
         const float svfbuveg = G.svfbuveg[p];
@@ -1499,7 +1492,7 @@ __global__ __launch_bounds__(BLOCK) void k_hour_pixels(const sw_domain *__restri
     }
     if (!FULL && day && G.ptab_s) {
         __syncthreads();                              // every wave's walk is done with the LDS
-        stage_search(G.ptab_s + (long long)row * SW_SEARCH_WORDS);
+        stage_search(G.ptab_s + (long long)row * SW_SEARCH_WORDS, at.tid);
         day_patch_sums_tab(vis, P, p, pf, pd, tan_asvf, dveg, a_lv, a_kd, a_swsun, a_swsh, a_lsunc,
                            a_lsuns, a_lshc, a_lshs);
     } else if (!FULL && day) {
@@ -1630,6 +1623,81 @@ __global__ __launch_bounds__(BLOCK) void k_hour_pixels(const sw_domain *__restri
     }
 }
 
+// Lanes past the raster (partial tiles) compute a copy of the nearest pixel, so that every wave
+// takes part in the workgroup's barriers, and write nothing.
+__device__ __forceinline__ bool clamp_to_raster(const DomainG &G, PixelAt &at) {
+    if (at.valid) return true;
+    at.r = min(at.r, (int)G.rows - 1);
+    at.c = min(at.c, (int)G.cols - 1);
+    at.p = (long long)at.r * G.cols + at.c;
+    return false;
+}
+
+// The pixel pass of one hour: the lag state round-trips the domain's lag planes (day hours only).
+template <bool FULL>
+__global__ __launch_bounds__(BLOCK) void k_hour_pixels(const sw_domain *__restrict__ doms,
+                                                       const int32_t *__restrict__ bs, int nd,
+                                                       int total, int hour,
+                                                       const int32_t *__restrict__ gofs_,
+                                                       const double *__restrict__ gbounds_,
+                                                       int first, int second, int lds_cap) {
+    DomainG G;
+    PixelAt at = locate(doms, bs, nd, total, G);
+    const bool valid = clamp_to_raster(G, at);
+    const long long P = G.pixels, p = at.p;
+    const bool day = G.hours[(long long)hour * SW_HOUR_STRIDE + SWH_DAY] != 0.0;
+    constexpr int NS = FULL ? 6 : 5;
+    AS1 float *lg = G.lag;
+    float st[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    if (day)
+        for (int k = 0; k < NS; ++k) st[k] = lg[k * P + p];
+    pixel_hour<FULL>(G, at, valid, hour, (const AS4 int32_t *)gofs_, (const AS4 double *)gbounds_, first, second,
+                     lds_cap, st);
+    if (day && valid)
+        for (int k = 0; k < NS; ++k) lg[k * P + p] = st[k];
+}
+
+// The pixel pass of hours h0 .. h1-1 in one launch: each workgroup runs its tile through the
+// hours in order with the lag state in registers (loaded from and stored back to the lag planes
+// once), reading each hour's records and mask bytes from slot ``hour`` of the per-hour planes that
+// the shadow pass wrote with per_hour set. Tiles are independent, so no grid-wide barrier is needed.
+__global__ __launch_bounds__(BLOCK) void k_day_pixels(const sw_domain *__restrict__ doms,
+                                                      const int32_t *__restrict__ bs, int nd, int total,
+                                                      int h0, int h1, const int32_t *__restrict__ gofs_,
+                                                      const double *__restrict__ gbounds_, int first,
+                                                      int second, int lds_cap) {
+    DomainG G0;
+    PixelAt at = locate(doms, bs, nd, total, G0);
+    const bool valid = clamp_to_raster(G0, at);
+    const long long P = G0.pixels, p = at.p;
+    float st[6] = {G0.lag[p], G0.lag[P + p], G0.lag[2 * P + p], G0.lag[3 * P + p], G0.lag[4 * P + p], 0.f};
+    const sw_domain *const dom = doms + at.d;
+#pragma nounroll
+    for (int hour = h0; hour < h1; ++hour) {
+        if (hour > h0) __syncthreads();               // the previous hour's LDS reads are done
+        // Each hour starts from the domain record, the lane's indices and the launch arguments
+        // made opaque to the loop optimiser, so that what is derived from them (plane and staging
+        // addresses, table pointers) is recomputed per hour instead of being held in registers
+        // across the hours (the record alone would fill the scalar registers; the occupancy would
+        // drop).
+        const sw_domain *dh = dom;
+        const int32_t *go = gofs_;
+        const double *gb = gbounds_;
+        PixelAt ah = at;
+        int fs = first, ss = second, cap = lds_cap;
+        asm volatile("" : "+s"(dh), "+s"(go), "+s"(gb));
+        asm volatile("" : "+v"(ah.p), "+v"(ah.r), "+v"(ah.c), "+v"(ah.tid));
+        asm volatile("" : "+s"(ah.tr0), "+s"(ah.tc0), "+s"(ah.tx), "+s"(ah.ty), "+s"(fs), "+s"(ss), "+s"(cap));
+        asm volatile("" : "+s"(ah.wr0), "+s"(ah.wr1), "+s"(ah.wc0), "+s"(ah.wc1));
+        DomainG G = domain_at(dh);
+        G.rec += (long long)hour * P;
+        G.masks += (long long)hour * P;
+        pixel_hour<false>(G, ah, valid, hour, (const AS4 int32_t *)go, (const AS4 double *)gb, fs, ss, cap, st);
+    }
+    if (valid)
+        for (int k = 0; k < 5; ++k) G0.lag[k * P + p] = st[k];
+}
+
 struct Coef7 {
     double c[7];
 };
@@ -1658,11 +1726,12 @@ __global__ __launch_bounds__(256) void k_copy(const float4 *__restrict__ src, fl
 }
 
 int launch_hour_shadow(const sw_domain *doms, const int32_t *bs, int nd, int total, int hour,
-                       bool full, hipStream_t s) {
+                       bool full, hipStream_t s, int n_hours = 1, bool per_hour = false) {
+    const dim3 grid(total, n_hours);
     if (full)
-        hipLaunchKernelGGL(k_hour_shadow<true>, dim3(total), dim3(BLOCK), 0, s, doms, bs, nd, total, hour);
+        hipLaunchKernelGGL(k_hour_shadow<true>, grid, dim3(BLOCK), 0, s, doms, bs, nd, total, hour, (int)per_hour);
     else
-        hipLaunchKernelGGL(k_hour_shadow<false>, dim3(total), dim3(BLOCK), 0, s, doms, bs, nd, total, hour);
+        hipLaunchKernelGGL(k_hour_shadow<false>, grid, dim3(BLOCK), 0, s, doms, bs, nd, total, hour, (int)per_hour);
     return hip_check(hipGetLastError(), "k_hour_shadow launch");
 }
 
@@ -1677,13 +1746,26 @@ int lds_records(int second) {
     return n * (int)sizeof(float) * 2 <= 65536 ? n : 0;
 }
 
+// LDS of a pixel-pass workgroup: the staged records during the walk, then the hour's patch search
+// tables.
+size_t pixel_lds_bytes(int cap) {
+    size_t bytes = (size_t)cap * sizeof(float) * 2;
+    return bytes < (size_t)SW_SEARCH_WORDS * 4 ? (size_t)SW_SEARCH_WORDS * 4 : bytes;
+}
+
+int launch_day_pixels(const sw_domain *doms, const int32_t *bs, int nd, int total, int h0, int h1,
+                      const int32_t *gofs, const double *gb, int first, int second, hipStream_t s) {
+    const int cap = lds_records(second);
+    hipLaunchKernelGGL(k_day_pixels, dim3(total), dim3(BLOCK), pixel_lds_bytes(cap), s, doms, bs, nd, total, h0, h1,
+                       gofs, gb, first, second, cap);
+    return hip_check(hipGetLastError(), "k_day_pixels launch");
+}
+
 int launch_hour_pixels(const sw_domain *doms, const int32_t *bs, int nd, int total, int hour,
                        const int32_t *gofs, const double *gb, int first, int second, bool full,
                        hipStream_t s) {
     const int cap = lds_records(second);
-    // the staged records during the walk, then the hour's patch search tables
-    size_t bytes = (size_t)cap * sizeof(float) * 2;
-    if (bytes < (size_t)SW_SEARCH_WORDS * 4) bytes = (size_t)SW_SEARCH_WORDS * 4;
+    const size_t bytes = pixel_lds_bytes(cap);
     if (full)
         hipLaunchKernelGGL(k_hour_pixels<true>, dim3(total), dim3(BLOCK), bytes, s, doms, bs, nd, total,
                            hour, gofs, gb, first, second, cap);
@@ -3504,6 +3586,30 @@ int sw_hours_forward(const sw_domain *doms, const int32_t *bs, int nd, int total
         if (rc) return rc;
     }
     return SW_OK;
+}
+
+int sw_day_shadow(const sw_domain *doms, const int32_t *bs, int nd, int total, int h0, int h1, void *stream) {
+    int rc = check_launch(doms, bs, nd, total);
+    if (rc) return rc;
+    if (h0 < 0 || h1 < h0 || h1 > 65535) return fail(SW_ERR_ARGUMENT, "sw_day_shadow: bad hour range");
+    if (h1 == h0) return SW_OK;
+    return launch_hour_shadow(doms, bs, nd, total, h0, false, (hipStream_t)stream, h1 - h0, true);
+}
+
+int sw_day_pixels(const sw_domain *doms, const int32_t *bs, int nd, int total, int h0, int h1,
+                  const int32_t *gofs, const double *gb, int first, int second, void *stream) {
+    int rc = check_launch(doms, bs, nd, total);
+    if (rc) return rc;
+    if (!gofs || !gb || first < 1 || second < 1 || second > MAX_SECOND || first > second || h0 < 0 || h1 < h0)
+        return fail(SW_ERR_ARGUMENT, "sw_day_pixels: bad arguments");
+    if (h1 == h0) return SW_OK;
+    return launch_day_pixels(doms, bs, nd, total, h0, h1, gofs, gb, first, second, (hipStream_t)stream);
+}
+
+int sw_day_forward(const sw_domain *doms, const int32_t *bs, int nd, int total, int h0, int h1,
+                   const int32_t *gofs, const double *gb, int first, int second, void *stream) {
+    const int rc = sw_day_shadow(doms, bs, nd, total, h0, h1, stream);
+    return rc ? rc : sw_day_pixels(doms, bs, nd, total, h0, h1, gofs, gb, first, second, stream);
 }
 
 int sw_adjoint_hour(const sw_domain *dom, const int32_t *bs, int total, int hour, int objective_hour,

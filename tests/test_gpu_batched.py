@@ -147,3 +147,39 @@ def test_autotuner_picks_a_measured_width(rt, tmp_path, monkeypatch):
     assert set(res['ms']) == set(rt.TILE_CHOICES)
     again = autotune.tune(eng)
     assert again['cached'] and again['tile_cols'] == res['tile_cols']
+
+
+@pytest.mark.parametrize('tile', [16, 32])
+def test_resident_day_equals_hourly_launches(rt, tile):
+    """sw_day_forward (all shadow passes, then one pixel pass with each tile running through the
+    hours with its lag state in registers) gives the bits of the two-launches-per-hour loop, over
+    uneven domains, for a day split at hour 10 (lag state stored and reloaded between the two
+    launches) and for graph replays."""
+    from solweig_amd import batched, synthetic, sun as psun
+    doms = [synthetic.small_domain(seed=60 + i, size=n, latitude=lat, maps=(i % 2 == 0))
+            for i, (n, lat) in enumerate([(70, 8.0), (37, 47.0), (52, 33.0)])]
+    suns = [psun.met_sun(d.met, d.location, d.utc) for d in doms]
+    runs = {}
+    for resident in (False, True):
+        eng = batched.Engine(doms, n_hours=24, resident=resident)
+        eng.set_tiles(tile)
+        eng.load(eng.tables(sun=suns))
+        print(f'tile {tile}, resident {resident}: whole day', flush=True)
+        eng.reset_state()
+        eng.forward_day()
+        runs[resident, 'day'] = [eng.outputs(k) for k in range(len(doms))]
+        print(f'tile {tile}, resident {resident}: hours 0-10 and 10-24', flush=True)
+        eng.reset_state()
+        eng.run_hours(0, 10)
+        eng.run_hours(10, 24)
+        runs[resident, 'split'] = [eng.outputs(k) for k in range(len(doms))]
+        if resident:
+            for rep in range(2):
+                print(f'tile {tile}, resident: graph replay {rep}', flush=True)
+                eng.reset_state()
+                eng.forward_day(use_graph=True)
+                runs[resident, f'graph{rep}'] = [eng.outputs(k) for k in range(len(doms))]
+    ref = runs[False, 'day']
+    for key, got in runs.items():
+        for k, ((t, u), (t0, u0)) in enumerate(zip(got, ref)):
+            assert np.array_equal(t, t0) and np.array_equal(u, u0, equal_nan=True), (key, k)

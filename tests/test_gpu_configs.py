@@ -59,12 +59,11 @@ def _engine(d, s, tile=None):
 def _step_hours(eng, s, hours, keep):
     """Run ``hours`` one by one (both passes) and return the shadow pass's mask byte of the
     daytime hours in ``keep``."""
-    dev = eng.domains[0].dev['masks']
     out = {}
     for h in hours:
         eng.hour(h)
         if h in keep and s['altitude'][h] > 0:
-            out[h] = dev.to_host(eng.stream).reshape(eng.domains[0].shape)
+            out[h] = eng.hour_masks(0)
     return out
 
 
