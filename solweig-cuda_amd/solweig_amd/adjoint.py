@@ -31,7 +31,7 @@ import numpy as np
 
 from . import runtime as rt
 from .batched import Engine
-from .sky import CONSTANTS, hour_tables
+from .sky import CONSTANTS, hour_branches, hour_tables
 from .static import PreparedDomain
 from .sun import met_sun
 
@@ -212,15 +212,38 @@ class UtciGradient:
         base = dict(Ta=float(row[11]), RH=float(row[10]), Ws=float(row[9]), radG=float(row[14]),
                     albedo_b=self.albedo_b, trans=self.trans)
         names = ['Ta', 'RH', 'radG'] + (['Ws'] if top else [])
+        ci_in = t.meta[h]['state_in'][0]
+
+        def vec(p):
+            return self._vector(sun, t, h, p['Ta'], p['RH'], p['Ws'], p['radG'], p['albedo_b'], p['trans'], top)
+
+        def branches(p):
+            return hour_branches(float(sun['altitude'][h]), float(sun['azimuth'][h]), float(sun['zen'][h]),
+                                 float(sun['jday'][h]), p['Ta'], p['RH'], p['radG'], float(row[12]),
+                                 self.domain.latitude, ci_in)
+
+        b0, v0 = branches(base), None
         cols = {}
         for name in names:
+            # central differences inside the hour's branch (Reindl and Perez bins, CI clamps);
+            # one-sided on the side that stays in it when the forcing sits within a step of an edge
             step = STEPS[name]
-            hi, lo = dict(base), dict(base)
-            hi[name] += step
-            lo[name] -= step
-            vh = self._vector(sun, t, h, hi['Ta'], hi['RH'], hi['Ws'], hi['radG'], hi['albedo_b'], hi['trans'], top)
-            vl = self._vector(sun, t, h, lo['Ta'], lo['RH'], lo['Ws'], lo['radG'], lo['albedo_b'], lo['trans'], top)
-            cols[name] = (vh - vl) / (2.0 * step)
+            for _ in range(3):
+                hi, lo = dict(base), dict(base)
+                hi[name] += step
+                lo[name] -= step
+                same_hi, same_lo = branches(hi) == b0, branches(lo) == b0
+                if same_hi and same_lo:
+                    cols[name] = (vec(hi) - vec(lo)) / (2.0 * step)
+                elif same_hi or same_lo:
+                    v0 = vec(base) if v0 is None else v0
+                    cols[name] = (vec(hi) - v0) / step if same_hi else (v0 - vec(lo)) / step
+                else:
+                    step *= 0.01
+                    continue
+                break
+            else:
+                raise RuntimeError(f'hour {h}: the forcing sits on a branch edge of {name}')
         n = len(VALUES) + 7 + (2 * rt.SW_PATCHES + 1 if top else 0)
         hh = t.hours[h]
         ab, tr = np.zeros(n), np.zeros(n)

@@ -151,20 +151,26 @@ def clearness(zen, jday, Ta, RH, radG, lat, P):
     return I0, CI, Kt
 
 
-def diffuse_fraction(radG, altitude, Kt, Ta, RH):
-    """Reindl et al. (1990) split (diffusefraction, umep_solweig.py:1789-1832); RH in %."""
+def _reindl(radG, altitude, Kt, Ta, RH):
+    """Reindl et al. (1990) split before its limits: (radI, radD, clearness-index bin)."""
     s = np.sin(altitude * (np.pi / 180))
+    k = 0 if Kt <= 0.3 else (1 if Kt < 0.78 else 2)
     if Ta <= -999.00 or RH <= -999.00 or np.isnan(Ta) or np.isnan(RH):
-        radD = radG * ((1.020 - 0.248 * Kt) if Kt <= 0.3 else (1.45 - 1.67 * Kt) if Kt < 0.78 else 0.147)
+        radD = radG * ((1.020 - 0.248 * Kt) if k == 0 else (1.45 - 1.67 * Kt) if k == 1 else 0.147)
     else:
         rh = RH / 100
-        if Kt <= 0.3:
+        if k == 0:
             radD = radG * (1 - 0.232 * Kt + 0.0239 * s - 0.000682 * Ta + 0.0195 * rh)
-        elif Kt < 0.78:
+        elif k == 1:
             radD = radG * (1.329 - 1.716 * Kt + 0.267 * s - 0.00357 * Ta + 0.106 * rh)
         else:
             radD = radG * (0.426 * Kt - 0.256 * s + 0.00349 * Ta + 0.0734 * rh)
-    radI = (radG - radD) / s
+    return (radG - radD) / s, radD, k
+
+
+def diffuse_fraction(radG, altitude, Kt, Ta, RH):
+    """Reindl et al. (1990) split (diffusefraction, umep_solweig.py:1789-1832); RH in %."""
+    radI, radD, _ = _reindl(radG, altitude, Kt, Ta, RH)
     if radI < 0:
         radI = 0
     if altitude < 1 and radI > radG:
@@ -172,6 +178,29 @@ def diffuse_fraction(radG, altitude, Kt, Ta, RH):
     if radD > radG:
         radD = radG
     return radI, radD
+
+
+def hour_branches(alt, azi, zen, jday, Ta, RH, radG, P, latitude, CI=1.0):
+    """The branches that one hour's host chain takes for its forcing: at night (0, CI < 0.95);
+    by day (1, CI clamped, Reindl bin and its three limits, Perez clearness bin, CI_Tg and
+    CI_TgG clamped, CI < 0.95). The chain is smooth in the forcing while these stay the same, so
+    a central difference is valid only between points with equal branches."""
+    if alt <= 0:
+        return (0, CI < 0.95)
+    I0, CI, Kt = clearness(zen, jday, Ta, RH / 100., radG, latitude, P)
+    ci_clamp = bool(CI > 1 or CI == np.inf)
+    CI = 1 if ci_clamp else CI
+    rI, rD, kbin = _reindl(radG, alt, Kt, Ta, RH)
+    limits = (rI < 0, alt < 1 and max(rI, 0) > radG, rD > radG)
+    radI, radD = diffuse_fraction(radG, alt, Kt, Ta, RH)
+    *_, clear, _ = perez(zen * (180 / np.pi), azi, radD, radI, jday, full=True)
+    pbin = int(np.searchsorted(_CLEAR_EDGES, clear))
+    radI0, radD0 = diffuse_fraction(I0, alt, 1., Ta, RH)
+    corr = 0.1473 * np.log(90 - (zen / np.pi * 180)) + 0.3454
+    citg = (radG / radI0) + (1 - corr)
+    citgg = (radG / (radI0 * np.sin(alt * DEG) + radD0)) + (1 - corr)
+    return (1, ci_clamp, kbin, limits, pbin, bool(citg > 1 or citg == np.inf), bool(citgg > 1 or citgg == np.inf),
+            CI < 0.95)
 
 
 _PEREZ = np.array([  # rows a1..a4, b1..b4, c1..c4, d1..d4, e1..e4; columns = 8 clearness bins

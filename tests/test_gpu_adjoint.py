@@ -14,7 +14,11 @@ from oracle import day as oday
 pytestmark = pytest.mark.gpu
 
 H_OBJ = 14
-RTOL = 2e-2      # relative agreement of the gradient with the FP64 central differences
+RTOL = 3e-4      # relative agreement of the gradient with the FP64 central differences
+ATOL = 1e-5      # absolute floor, as a fraction of the largest gradient of the group
+# components above this fraction of the group's largest gradient must also fail the tolerance
+# when the FP64 reference is moved by 0.1 %: the check resolves a 0.1 % error there
+RESOLVE = 0.05
 
 
 @pytest.fixture(scope='module')
@@ -48,8 +52,27 @@ def _oracle_J(p, met, s, trans=None):
 
 def _close(dev, ref, scale, what):
     err = abs(dev - ref)
-    print(f'  {what}: device {dev:+.6e}  FP64 FD {ref:+.6e}  |diff| {err:.2e}', flush=True)
-    assert err <= RTOL * abs(ref) + 1e-3 * scale, what
+    tol = RTOL * abs(ref) + ATOL * scale
+    print(f'  {what}: device {dev:+.6e}  FP64 FD {ref:+.6e}  |diff| {err:.2e} (rel {err / max(abs(ref), 1e-300):.1e})',
+          flush=True)
+    assert err <= tol, what
+    if abs(ref) >= RESOLVE * scale:
+        # the oracle's value with a planted 0.1 % error would fail this comparison
+        assert abs(dev - ref * 1.001) > tol and abs(dev - ref * 0.999) > tol, what
+
+
+def _same_branches(d, s, h, col, step):
+    """The hour's host chain takes the same branches (Reindl and Perez bins, CI clamps) at both
+    difference points, so the central difference is taken inside one smooth piece."""
+    from solweig_amd.sky import hour_branches
+    out = []
+    for sign in (1, -1):
+        m = d.met[h].copy()
+        m[col] += sign * step
+        out.append(hour_branches(float(s['altitude'][h]), float(s['azimuth'][h]), float(s['zen'][h]),
+                                 float(s['jday'][h]), float(m[11]), float(m[10]), float(m[14]), float(m[12]),
+                                 d.latitude))
+    return out[0] == out[1]
 
 
 def test_objective_matches_oracle(case):
@@ -73,6 +96,7 @@ def test_scalar_gradients_match_fp64_differences(case):
         _close(g[name], (Jp - Jm) / (2 * step), scale, name)
     for name, col, step in (('Ta', 11, 1e-2), ('RH', 10, 1e-2), ('Ws', 9, 1e-3), ('radG', 14, 1e-2)):
         for h in (H_OBJ, H_OBJ - 2, H_OBJ - 5):
+            assert _same_branches(d, s, h, col, step), f'{name}[{h}] difference points straddle a branch edge'
             mp, mm = d.met.copy(), d.met.copy()
             mp[h, col] += step
             mm[h, col] -= step
