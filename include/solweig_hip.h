@@ -453,6 +453,17 @@ int sw_dsm_walls(const double *dsm_dev, int rows, int cols, double threshold, do
                  const int32_t *side_start_dev, uint8_t *occupancy_dev, double *walls_dev, double *aspect_dev,
                  void *stream);
 
+/* Solar position of forcing rows in FP64 (replaces Solweig_2015a_metdata_noload and
+ * sun_position, umep_solweig.py:2412-2503, 2529-3521: the NREL SPA with delta_t = 0 and fixed
+ * refraction). met_dev [n_rows][met_cols] holds the forcing rows (year, day of year, hour,
+ * minute, ...; met_cols >= 4). The rows are taken in independent chunks of chunk_rows (0: one
+ * chunk), each as one call of the reference: a row's sun is sampled half the chunk's first time
+ * step before its stamp, a zenith in (89, 90] becomes 89, and the day's maximum altitude is
+ * searched at the chunk's first row and at every midnight row. out_dev [6][n_rows]: altitude
+ * (deg), azimuth (deg), zenith (rad), day of year, dectime, altmax (deg).                   */
+int sw_sun_position(const double *met_dev, int n_rows, int met_cols, int chunk_rows, double latitude,
+                    double longitude, double altitude_m, double utc, double *out_dev, void *stream);
+
 /* UTCI of a Tmrt plane for scalar weather (replaces utci.utci_calculator's scalar-weather
  * path, utci.py:22-33, utci.cu:243-523): ``coefficients`` are 7 host FP64 values c_k with
  * UTCI = sum_k c_k (Tmrt - ta32)^k; sentinel != 0 writes -999; ground_dev (nullable) NaNs

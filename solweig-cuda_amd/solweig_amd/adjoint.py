@@ -33,7 +33,7 @@ from . import runtime as rt
 from .batched import Engine
 from .sky import CONSTANTS, hour_branches, hour_tables
 from .static import PreparedDomain
-from .sun import met_sun
+from .sun import met_sun_device
 
 # hour-record values whose derivative the device accumulates (solweig_hip.h SWA_* order)
 VALUES = ('TA', 'PSI', 'CITGG', 'LWALL', 'ALBEDO_B', 'RADI', 'RADD', 'RADG', 'FLUX4', 'SW_SUNLIT',
@@ -120,7 +120,7 @@ class UtciGradient:
         objective's tables (they are host data of a few hundred KB, not per-pixel state)."""
         key = (self.albedo_b, self.trans, self.met.tobytes())
         if self._cached is None or self._cached[0] != key:
-            sun = met_sun(self.met, self.domain.location, self.domain.utc)
+            sun = met_sun_device(self.met, self.domain.location, self.domain.utc)
             tabs = self.eng.tables(met=[self.met], sun=[sun], constants=self._constants(), psi=self.trans)
             self._cached = (key, sun, tabs)
         return self._cached[1], self._cached[2]

@@ -1,11 +1,14 @@
 """Batched day forward: every hour of a day, for one or many uneven domains, on one stream.
 
-Per day the host builds each domain's hour tables (sky.build_day), uploads them into buffers of
-fixed capacity, and the device runs, per hour, one shadow launch and one pixel launch that cover
-the tiles of all domains together (sw_hours_forward). Lag state stays on the device across the
-hours of a day and is cleared at each day start (series.py:425-436). Because buffer addresses
-never change, the 2 x n_hours launches of a day can be captured once as a hipGraph and replayed
-for every further day after the tables are refreshed.
+The solar position of every forcing row comes from the device (FP64 SPA, sw_sun_position, once
+per domain for its whole forcing). Per day the host builds each domain's hour tables
+(sky.build_day) and uploads them into buffers of fixed capacity; the device then runs the day
+either resident (sw_day_forward: the shadow pass of every hour in one launch, then one pixel pass
+in which each tile runs through the hours with its lag state in registers) or as one shadow and
+one pixel launch per hour (sw_hours_forward), each launch covering the tiles of all domains
+together. Lag state is cleared at each day start (series.py:425-436). Because buffer addresses
+never change, a day's launches can be captured once as a hipGraph and replayed for every further
+day after the tables are refreshed.
 """
 from __future__ import annotations
 
@@ -19,7 +22,7 @@ import numpy as np
 from . import runtime as rt
 from .sky import CONSTANTS, STEP_BLOCK, build_day, gvf_geometry
 from .static import PreparedDomain
-from .sun import met_sun
+from .sun import met_sun, met_sun_device
 
 
 class Engine:
@@ -62,6 +65,7 @@ class Engine:
             b['masks'] = rt.DeviceArray((self.n_hours if resident else 1, d.P), np.uint8)
             self.buf.append(b)
         self.resident = bool(resident)
+        self._sun_days = [None] * len(self.domains)
         self.block_start = rt.DeviceArray((len(self.domains) + 1,), np.int32)
         self.structs = (rt.SwDomain * len(self.domains))()
         self.dom_dev = rt.DeviceArray((ctypes.sizeof(self.structs),), np.uint8)
@@ -108,11 +112,25 @@ class Engine:
         self._write_domains()
 
     # ------------------------------------------------------------------ tables
+    def _sun(self, k, day_index, rows):
+        """Solar position of one day's rows of domain k, on the device (met_sun_device): the
+        domain's whole forcing is evaluated once, in chunks of n_hours rows (one chunk per day, as
+        a per-day met_sun call takes it), and sliced per day; other rows are evaluated on call."""
+        dm = self.domains[k].domain
+        if rows is not None:
+            return met_sun_device(rows, dm.location, dm.utc)
+        if self._sun_days[k] is None:
+            self._sun_days[k] = met_sun_device(dm.met, dm.location, dm.utc, chunk_rows=self.n_hours,
+                                               stream=self.stream)
+        lo, hi = day_index * self.n_hours, (day_index + 1) * self.n_hours
+        return {key: v[lo:hi] for key, v in self._sun_days[k].items()}
+
     def _jobs(self, day_index, met, sun, constants, psi):
         for k, d in enumerate(self.domains):
             dm = d.domain
             rows = met[k] if met is not None else dm.met[day_index * self.n_hours:(day_index + 1) * self.n_hours]
-            yield (np.asarray(rows), None if sun is None else sun[k], dm.location, dm.utc, dm.latitude, d.shape,
+            s = sun[k] if sun is not None else self._sun(k, day_index, None if met is None else np.asarray(rows))
+            yield (np.asarray(rows), s, dm.location, dm.utc, dm.latitude, d.shape,
                    dm.scale, d.amaxvalue, d.landcover, d.wall_params, constants or CONSTANTS, psi)
 
     def tables(self, day_index=0, met=None, sun=None, constants=None, psi=None):

@@ -3356,6 +3356,203 @@ int launch_adjoint_hour(const sw_domain *dom, const int32_t *bs, int total, int 
     return hip_check(hipGetLastError(), "k_adj_reduce launch");
 }
 
+// =====================================================================================
+// Solar position (replaces Solweig_2015a_metdata_noload + sun_position, umep_solweig.py:2412-2503,
+// 2529-3521): the NREL SPA of Reda & Andreas (2003) with delta_t = 0 and the fixed 283 K /
+// 1010 mbar refraction, in FP64, with the host restatement's arithmetic order. A forcing row is
+// sampled half a timestep before its stamp (seconds dropped), a zenith in (89, 90] becomes 89,
+// and altmax is the altitude of the last of the 15-minute samples from 10:15 local before the
+// altitude stops increasing (0 when the first sample is below the horizon), found at the first
+// row of a chunk and at every midnight row.
+#include "spa_terms.inc"
+
+__constant__ int SPA_MONTH_START[13] = {0, 31, 59, 90, 120, 151, 181, 212, 243, 273, 304, 334, 365};
+constexpr long long SPA_DAY_US = 86400000000LL;
+
+struct SunSite {
+    double latitude, longitude, altitude_m, utc;
+};
+
+__device__ __forceinline__ bool spa_leap(int y) { return (y % 4 == 0 && y % 100 != 0) || y % 400 == 0; }
+
+__device__ __forceinline__ double spa_fold(double v, double upper) {
+    v = v - upper * floor(v / upper);
+    return v < 0.0 ? v + upper : v;
+}
+
+// sum_k (sum over group g0 + k of A cos(B + C jme)) jme^k / 1e8
+__device__ double spa_series(int g0, int ng, double jme) {
+    double total = 0.0;
+    for (int k = 0; k < ng; ++k) {
+        const int s = SPA_GROUP[g0 + k][0], n = SPA_GROUP[g0 + k][1];
+        double part = 0.0;
+        for (int i = s; i < s + n; ++i) part += SPA_TERMS[i][0] * cos(SPA_TERMS[i][1] + SPA_TERMS[i][2] * jme);
+        total = k == 0 ? part : total + part * (k == 1 ? jme : pow(jme, (double)k));
+    }
+    return total / 1e8;
+}
+
+// Julian day of a local calendar instant ``us`` microseconds after 1 January 00:00 of ``year``
+// (either side of that year), seconds dropped.
+__device__ double spa_julian(int year, long long us, double utc) {
+    while (us < 0) {
+        year -= 1;
+        us += (spa_leap(year) ? 366 : 365) * SPA_DAY_US;
+    }
+    while (us >= (spa_leap(year) ? 366 : 365) * SPA_DAY_US) {
+        us -= (spa_leap(year) ? 366 : 365) * SPA_DAY_US;
+        year += 1;
+    }
+    const int d0 = (int)(us / SPA_DAY_US);
+    const long long rem = us % SPA_DAY_US;
+    const int hour = (int)(rem / 3600000000LL), minute = (int)((rem / 60000000LL) % 60);
+    const int leap = spa_leap(year) ? 1 : 0;
+    int month = 1;
+    while (month < 12 && d0 >= SPA_MONTH_START[month] + (month >= 2 ? leap : 0)) ++month;
+    const int day = d0 - (SPA_MONTH_START[month - 1] + (month - 1 >= 2 ? leap : 0)) + 1;
+    const double y = month <= 2 ? year - 1 : year, m = month <= 2 ? month + 12 : month;
+    const double ut = ((hour - utc) / 24.0) + (minute / 1440.0) + (0.0 / 86400.0);
+    const double d = day + ut;
+    double b = 0.0;
+    if (!(year < 1582 || (year == 1582 && (month < 10 || (month == 10 && day <= 4))))) {
+        const double a = floor(y / 100.0);
+        b = 2.0 - a + floor(a / 4.0);
+    }
+    return d + b + floor(365.25 * (y + 4716.0)) + floor(30.6001 * (m + 1.0)) - 1524.5;
+}
+
+// Topocentric zenith and azimuth (degrees) at Julian day jd.
+__device__ void spa_point(double jd, const SunSite &site, double &zenith, double &azimuth) {
+    const double PI = 3.141592653589793, D = PI / 180.0;      // D: the nutation argument's factor
+    const double jc = (jd - 2451545.0) / 36525.0, jce = jc, jme = jc / 10.0;
+    const double lon = spa_fold(spa_series(0, 6, jme) * 180.0 / PI, 360.0);
+    double b0 = 0.0, b1 = 0.0;
+    for (int i = SPA_GROUP[6][0]; i < SPA_GROUP[6][0] + SPA_GROUP[6][1]; ++i)
+        b0 += SPA_TERMS[i][0] * cos(SPA_TERMS[i][1] + SPA_TERMS[i][2] * jme);
+    for (int i = SPA_GROUP[7][0]; i < SPA_GROUP[7][0] + SPA_GROUP[7][1]; ++i)
+        b1 += SPA_TERMS[i][0] * cos(SPA_TERMS[i][1] + SPA_TERMS[i][2] * jme);
+    const double lat = spa_fold(((b0 + b1 * jme) / 1e8) * 180.0 / PI, 360.0);
+    const double radius = spa_series(8, 5, jme);
+    const double geo_lon = spa_fold(lon + 180.0, 360.0), geo_lat = spa_fold(-lat, 360.0);
+    const double j2 = pow(jce, 2.0), j3 = pow(jce, 3.0);
+    const double x[5] = {
+        (1.0 / 189474.0) * j3 + -0.0019142 * j2 + 445267.11148 * jce + 297.85036,
+        -(1.0 / 300000.0) * j3 + -0.0001603 * j2 + 35999.05034 * jce + 357.52772,
+        (1.0 / 56250.0) * j3 + 0.0086972 * j2 + 477198.867398 * jce + 134.96298,
+        (1.0 / 327270.0) * j3 + -0.0036825 * j2 + 483202.017538 * jce + 93.27191,
+        (1.0 / 450000.0) * j3 + 0.0020708 * j2 + -1934.136261 * jce + 125.04452};
+    double dpsi = 0.0, deps = 0.0;
+    for (int i = 0; i < SPA_NUT_ROWS; ++i) {
+        double arg = 0.0;
+        for (int j = 0; j < 5; ++j) arg += SPA_NUT_ARGS[i][j] * x[j];
+        arg *= D;
+        dpsi += (SPA_NUT_COEFFS[i][0] + SPA_NUT_COEFFS[i][1] * jce) * sin(arg);
+        deps += (SPA_NUT_COEFFS[i][2] + SPA_NUT_COEFFS[i][3] * jce) * cos(arg);
+    }
+    dpsi /= 36000000.0;
+    deps /= 36000000.0;
+    const double u = jme / 10.0;
+    const double poly[11] = {2.45, 5.79, 27.87, 7.12, -39.05, -249.67, -51.38, 1999.25, -1.55, -4680.93, 84381.448};
+    double eps0 = 0.0;
+    for (int k = 0; k < 9; ++k) eps0 = eps0 + poly[k] * pow(u, (double)(10 - k));
+    eps0 = eps0 + poly[9] * u + poly[10];
+    const double eps = eps0 / 3600.0 + deps;
+    const double aberration = -20.4898 / (3600.0 * radius);
+    const double lam = geo_lon + dpsi + aberration;
+    const double nu0 = 280.46061837 + (360.98564736629 * (jd - 2451545.0)) + (0.000387933 * pow(jc, 2.0)) -
+                       (pow(jc, 3.0) / 38710000.0);
+    const double nu = spa_fold(nu0, 360.0) + dpsi * cos(eps * PI / 180.0);
+    const double alpha = spa_fold(atan2(sin(lam * PI / 180.0) * cos(eps * PI / 180.0) - tan(geo_lat * PI / 180.0) * sin(eps * PI / 180.0), cos(lam * PI / 180.0)) *
+                                      180.0 / PI, 360.0);
+    const double delta = asin(sin(geo_lat * PI / 180.0) * cos(eps * PI / 180.0) + cos(geo_lat * PI / 180.0) * sin(eps * PI / 180.0) * sin(lam * PI / 180.0)) *
+                         180.0 / PI;
+    const double h = spa_fold(nu + site.longitude - alpha, 360.0);
+    const double xi = 8.794 / (3600.0 * radius);
+    const double uu = atan(0.99664719 * tan(site.latitude * PI / 180.0));
+    const double xx = cos(uu) + (site.altitude_m / 6378140.0) * cos(site.latitude * PI / 180.0);
+    const double yy = 0.99664719 * sin(uu) + (site.altitude_m / 6378140.0) * sin(site.latitude * PI / 180.0);
+    const double dalpha = atan2(-xx * sin(xi * PI / 180.0) * sin(h * PI / 180.0), cos(delta * PI / 180.0) - xx * sin(xi * PI / 180.0) * cos(h * PI / 180.0));
+    const double delta_t = atan2((sin(delta * PI / 180.0) - yy * sin(xi * PI / 180.0)) * cos(dalpha),
+                                 cos(delta * PI / 180.0) - yy * sin(xi * PI / 180.0) * cos(h * PI / 180.0)) * 180.0 / PI;
+    const double h_t = h - dalpha * 180.0 / PI;
+    const double e0 = asin(sin(site.latitude * PI / 180.0) * sin(delta_t * PI / 180.0) +
+                           cos(site.latitude * PI / 180.0) * cos(delta_t * PI / 180.0) * cos(h_t * PI / 180.0)) * 180.0 / PI;
+    const double refraction = 1.02 / (60.0 * tan((e0 + 10.3 / (e0 + 5.11)) * PI / 180.0));
+    zenith = 90.0 - (e0 + refraction);
+    azimuth = spa_fold(atan2(sin(h_t * PI / 180.0), cos(h_t * PI / 180.0) * sin(site.latitude * PI / 180.0) -
+                                               tan(delta_t * PI / 180.0) * cos(site.latitude * PI / 180.0)) * 180.0 / PI +
+                           180.0, 360.0);
+}
+
+__device__ __forceinline__ double met_dectime(const double *met, int cols, int i) {
+    return met[(long long)i * cols + 1] + met[(long long)i * cols + 2] / 24.0 + met[(long long)i * cols + 3] / (60.0 * 24.0);
+}
+
+// The row that starts row i's day: the first row of its chunk or the last midnight row before it.
+__device__ __forceinline__ bool sun_day_start(const double *met, int cols, int i, int chunk) {
+    if (i % chunk == 0) return true;
+    const double dt = met_dectime(met, cols, i);
+    return dt == floor(dt);
+}
+
+constexpr int ALTMAX_SAMPLES = 128;
+
+// One workgroup per row; rows that start a day search the day's maximum altitude: 15-minute
+// samples from 10:15 local, evaluated in parallel, then scanned in order (sequentially past the
+// first 128 samples, which only a sun still climbing after 42 hours would need).
+__global__ __launch_bounds__(ALTMAX_SAMPLES) void k_sun_altmax(const double *__restrict__ met, int n, int cols,
+                                                               int chunk, SunSite site, double *__restrict__ out) {
+    const int i = (int)blockIdx.x;
+    if (i >= n || !sun_day_start(met, cols, i, chunk)) return;
+    __shared__ double zen[ALTMAX_SAMPLES];
+    const int year = (int)met[(long long)i * cols], doy = (int)met[(long long)i * cols + 1];
+    const long long base = (long long)(doy - 1) * SPA_DAY_US + 600LL * 60000000LL;
+    const int k = (int)threadIdx.x + 1;
+    double z, a;
+    spa_point(spa_julian(year, base + (long long)k * 15LL * 60000000LL, site.utc), site, z, a);
+    zen[threadIdx.x] = z;
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    double sunmax = -90.0, zenith = 90.0;
+    for (int s = 1;; ++s) {
+        if (!(sunmax <= 90.0 - zenith)) break;
+        sunmax = 90.0 - zenith;
+        if (s <= ALTMAX_SAMPLES) {
+            zenith = zen[s - 1];
+        } else {
+            spa_point(spa_julian(year, base + (long long)s * 15LL * 60000000LL, site.utc), site, zenith, a);
+            if (s > 4 * 96) break;                    // a day has a maximum: never reached in practice
+        }
+    }
+    out[5LL * n + i] = sunmax;
+}
+
+// One thread per row: the sun half a timestep before the row's stamp, and the row's day maximum.
+__global__ void k_sun_rows(const double *__restrict__ met, int n, int cols, int chunk, SunSite site,
+                           double *__restrict__ out) {
+    const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i >= n) return;
+    const int c0 = i - i % chunk, len = min(chunk, n - c0);
+    const double dectime = met_dectime(met, cols, i);
+    const double half = len == 1 ? 0.0 : (met_dectime(met, cols, c0 + 1) - met_dectime(met, cols, c0)) / 2.0;
+    const int year = (int)met[(long long)i * cols], doy = (int)met[(long long)i * cols + 1];
+    const long long us = (long long)(doy - 1) * SPA_DAY_US + llrint(met[(long long)i * cols + 2] * 3600e6) +
+                         llrint(met[(long long)i * cols + 3] * 60e6) - llrint(half * 86400e6);
+    double zenith, azimuth;
+    spa_point(spa_julian(year, us, site.utc), site, zenith, azimuth);
+    if (zenith > 89.0 && zenith <= 90.0) zenith = 89.0;
+    out[i] = 90.0 - zenith;
+    out[(long long)n + i] = azimuth;
+    out[2LL * n + i] = zenith * (3.141592653589793 / 180.0);
+    out[3LL * n + i] = (double)doy;
+    out[4LL * n + i] = dectime;
+    if (!sun_day_start(met, cols, i, chunk)) {
+        int j = i;
+        while (!sun_day_start(met, cols, j, chunk)) --j;
+        out[5LL * n + i] = out[5LL * n + j];
+    }
+}
+
 }  // namespace
 
 // Scratch of sw_gvf_noalloc: the per-pixel neighbour samples of the ground-view walk, in a
@@ -3838,6 +4035,21 @@ int sw_utci_scalar(const float *tmrt, long long n, const double *coefficients, d
     hipLaunchKernelGGL(k_utci_scalar, dim3((unsigned)(blocks < 65536 ? blocks : 65536)), dim3(256), 0,
                        (hipStream_t)stream, tmrt, n, cf, (float)ta32, sentinel, ground, out);
     return hip_check(hipGetLastError(), "k_utci_scalar launch");
+}
+
+int sw_sun_position(const double *met, int n_rows, int met_cols, int chunk_rows, double latitude, double longitude,
+                    double altitude_m, double utc, double *out, void *stream) {
+    if (!met || !out || n_rows < 0 || met_cols < 4 || chunk_rows < 0)
+        return fail(SW_ERR_ARGUMENT, "sw_sun_position: bad arguments");
+    if (n_rows == 0) return SW_OK;
+    const int chunk = chunk_rows == 0 ? n_rows : chunk_rows;
+    const SunSite site = {latitude, longitude, altitude_m, utc};
+    const hipStream_t s = (hipStream_t)stream;
+    hipLaunchKernelGGL(k_sun_altmax, dim3(n_rows), dim3(ALTMAX_SAMPLES), 0, s, met, n_rows, met_cols, chunk, site, out);
+    int rc = hip_check(hipGetLastError(), "k_sun_altmax launch");
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_sun_rows, dim3((n_rows + 63) / 64), dim3(64), 0, s, met, n_rows, met_cols, chunk, site, out);
+    return hip_check(hipGetLastError(), "k_sun_rows launch");
 }
 
 int sw_copy(void *dst, const void *src, size_t bytes, void *stream) {

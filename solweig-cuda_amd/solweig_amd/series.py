@@ -23,7 +23,7 @@ import numpy as np
 from . import batched, synthetic
 from . import walls as dsm_walls
 from .output import DayOutputs
-from .sun import met_sun
+from .sun import met_sun_device
 
 
 @dataclass(frozen=True)
@@ -216,10 +216,10 @@ def run_timeseries(building_dsm_path, tree_path, dem_path, met_file, output_path
             extra = None
             if fp64:
                 steps = steps or stepwise_context(dom)
-                sun = met_sun(day.values, dom.location, utc)
+                sun = met_sun_device(day.values, dom.location, utc)
                 tmrt, utci, extra = stepwise_day(*steps, day.values, sun, extras)
             elif plain:
-                sun = met_sun(day.values, dom.location, utc)
+                sun = met_sun_device(day.values, dom.location, utc)
                 tmrt, utci, extra = _plain_day(dom, day.values, sun, extras)
             else:
                 if engine is None or engine.n_hours != n:

@@ -12,7 +12,8 @@ fixed 283 K / 1010 mbar refraction term, evaluated the way SOLWEIG 2022a evaluat
                       until the altitude stops increasing, redone at each midnight row.
 
 The per-hour arrays it returns drive the ray tables, the Perez sky and the surface-temperature
-wave; the device path receives them as plain FP64 tables.
+wave. ``met_sun_device`` evaluates the same per-row algorithm on the device in FP64 (the day
+loops' source of solar positions); ``met_sun`` is its host form.
 """
 from __future__ import annotations
 
@@ -286,8 +287,28 @@ def met_sun(met, location, utc):
                 dectime=dectime, altmax=altmax)
 
 
+SUN_KEYS = ('altitude', 'azimuth', 'zen', 'jday', 'dectime', 'altmax')
+
+
+def met_sun_device(met, location, utc, chunk_rows=0, stream=None):
+    """``met_sun`` evaluated on the device in FP64 (sw_sun_position), for every row of ``met`` at
+    once: the rows are taken in chunks of ``chunk_rows`` (0: one chunk), each chunk as one
+    met_sun call. Returns the same dict of host arrays."""
+    from . import runtime as rt
+    met = np.ascontiguousarray(np.asarray(met, dtype=np.float64))
+    n = met.shape[0]
+    if n == 0:
+        return {k: np.zeros(0) for k in SUN_KEYS}
+    dev_met = rt.DeviceArray.from_host(met)
+    out = rt.DeviceArray((6, n), np.float64)
+    rt.call('sw_sun_position', dev_met.ptr, n, met.shape[1], int(chunk_rows), float(location['latitude']),
+            float(location['longitude']), float(location['altitude']), float(utc), out.ptr, rt._s(stream))
+    host = out.to_host(stream)
+    return {k: host[i].copy() for i, k in enumerate(SUN_KEYS)}
+
+
 def is_leap(year):
     return calendar.isleap(int(year))
 
 
-__all__ = ["spa", "met_sun", "is_leap", "math"]
+__all__ = ["spa", "met_sun", "met_sun_device", "is_leap", "math"]

@@ -154,7 +154,7 @@ def run_tiles(tiles, met_file, output_path, longitude, latitude, *, utc_offset_h
     from . import batched
     from .output import DayOutputs
     from .series import _raster, load_meteorology, make_domain, write_day
-    from .sun import met_sun
+    from .sun import met_sun_device
     forcing = load_meteorology(met_file)
     utc = float(longitude) / 15.0 if utc_offset_hours is None else float(utc_offset_hours)
     out = Path(output_path).resolve()
@@ -194,7 +194,7 @@ def run_tiles(tiles, met_file, output_path, longitude, latitude, *, utc_offset_h
                         o.close()
                     engine = batched.Engine(doms, n_hours=n)
                     outs = [DayOutputs(engine, k, slots=max(1, int(output_queue_size))) for k in range(len(group))]
-                sun = met_sun(day.values, doms[0].location, utc)
+                sun = met_sun_device(day.values, doms[0].location, utc)
                 engine.load(engine.tables(met=[day.values] * len(group), sun=[sun] * len(group)))
                 engine.reset_state()
                 engine.forward_day()
