@@ -52,11 +52,16 @@ B_SHADOW = 16.0
 B_PIXELS = 141.5
 B_ALG = B_SHADOW + B_PIXELS
 HBM_PEAK = 8.0e12      # MI355X HBM3E bytes/s
-# HBM bytes per k_hour_pixels launch at 2000x2000 (mean over a day): rocprofv3 FETCH_SIZE scaled by
-# the k_copy calibration + WRITE_SIZE, from the profile named in TRAFFIC_SOURCE (a profile value, not
-# measured by this run).
-TRAFFIC_PIXELS_2000 = 0.825e9
-TRAFFIC_SOURCE = 'profiles/k_hour_pair_c2_2000x2000.md'
+# HBM bytes per pixel-pass launch at 2000x2000: rocprofv3 FETCH_SIZE scaled by the k_copy
+# calibration + WRITE_SIZE, from the profile named in TRAFFIC_SOURCE (profile values, not measured by
+# this run): k_day_pixels (one launch for 24 hours), k_hour_pixels (mean over a day's 24 launches).
+TRAFFIC_2000 = {'k_day_pixels': 21.98e9, 'k_hour_pixels': 0.819e9}
+TRAFFIC_SOURCE = 'profiles/k_day_pixels_c2_2000x2000.md'
+# VALU instructions (wave64) per pixel-pass launch at 2000x2000, SQ_INSTS_VALU x SQ_WAVES from the same
+# profile, and the single-issue VALU rate of MI355X (256 CUs x 2.4 GHz, one wave64 instruction per CU
+# per cycle without the packed FP32 forms): the ceiling the pixel pass actually runs against.
+VALU_2000 = {'k_day_pixels': 57106 * 62500, 'k_hour_pixels': 2323 * 62500}
+VALU_PEAK = 256 * 2.4e9
 
 
 def log(msg):
@@ -519,8 +524,12 @@ def main():
         achieved = dom_bytes / (dom_ms * 1e-3)
         pair = B_ALG * P * hours_per_launch / ((sh_ms + px_ms) * 1e-3)
         roof = dict(bound='hbm', achieved=achieved / 1e9, peak=HBM_PEAK / 1e9, unit='GB/s', frac=achieved / HBM_PEAK,
-                    traffic=TRAFFIC_PIXELS_2000 * hours_per_launch if (args.size == 2000 and dom_b == B_PIXELS)
-                    else None,
+                    traffic=TRAFFIC_2000.get(dom_name) if args.size == 2000 else None,
+                    valu_issue=(dict(wave_instructions_per_launch=VALU_2000[dom_name],
+                                     achieved=VALU_2000[dom_name] / (dom_ms * 1e-3), peak=VALU_PEAK,
+                                     frac=VALU_2000[dom_name] / (dom_ms * 1e-3) / VALU_PEAK, unit='wave-instr/s',
+                                     source=f'{TRAFFIC_SOURCE} (SQ_INSTS_VALU, a profile value)')
+                                if args.size == 2000 and dom_name in VALU_2000 else None),
                     traffic_source=f'{TRAFFIC_SOURCE} (rocprofv3 FETCH_SIZE x copy calibration + WRITE_SIZE; '
                                    f'a profile value, not measured in this run)',
                     kernel=dom_name, algorithmic_bytes_per_launch=dom_bytes, avg_ms=dom_ms,

@@ -193,7 +193,8 @@ def hour_branches(alt, azi, zen, jday, Ta, RH, radG, P, latitude, CI=1.0):
     rI, rD, kbin = _reindl(radG, alt, Kt, Ta, RH)
     limits = (rI < 0, alt < 1 and max(rI, 0) > radG, rD > radG)
     radI, radD = diffuse_fraction(radG, alt, Kt, Ta, RH)
-    *_, clear, _ = perez(zen * (180 / np.pi), azi, radD, radI, jday, full=True)
+    z = zen * (180 / np.pi) * DEG                     # perez's zenith, as it forms it
+    clear = ((radD + radI) / (radD + 1.041 * np.power(z, 3))) / (1 + 1.041 * np.power(z, 3))
     pbin = int(np.searchsorted(_CLEAR_EDGES, clear))
     radI0, radD0 = diffuse_fraction(I0, alt, 1., Ta, RH)
     corr = 0.1473 * np.log(90 - (zen / np.pi * 180)) + 0.3454
