@@ -76,6 +76,7 @@ struct PixelAt {
 // ground-view tables) is read through constant (4) pointers so it lands in scalar registers.
 #define AS1 __attribute__((address_space(1)))
 #define AS4 __attribute__((address_space(4)))
+#define AS3 __attribute__((address_space(3)))
 typedef float v2f __attribute__((ext_vector_type(2)));
 typedef float v4f __attribute__((ext_vector_type(4)));
 
@@ -149,8 +150,8 @@ __device__ __forceinline__ int uniform_index(double d) {
     return (int)(m >> (52 - e));
 }
 
-__device__ __forceinline__ PixelAt locate(const sw_domain *doms, const int32_t *bs_, int nd,
-                                          int total, DomainG &G) {
+__device__ __forceinline__ PixelAt locate_lane(const sw_domain *doms, const int32_t *bs_, int nd,
+                                               int total, DomainG &G, int lane) {
     const AS4 int32_t *bs = (const AS4 int32_t *)bs_;
     PixelAt at;
     const int b = xcd_remap((int)blockIdx.x, total);
@@ -166,18 +167,22 @@ __device__ __forceinline__ PixelAt locate(const sw_domain *doms, const int32_t *
     const int sh = G.tile_shift, tx = 1 << sh, ty = BLOCK >> sh;
     const int tpr = (cols + tx - 1) >> sh;
     const int tr0 = (t / tpr) * ty, tc0 = (t % tpr) * tx;
-    at.r = tr0 + ((int)threadIdx.x >> sh);
-    at.c = tc0 + ((int)threadIdx.x & (tx - 1));
-    const int w64 = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6) * 64;
+    at.r = tr0 + (lane >> sh);
+    at.c = tc0 + (lane & (tx - 1));
+    const int w64 = __builtin_amdgcn_readfirstlane(lane >> 6) * 64;
     at.wr0 = tr0 + (w64 >> sh);
     at.wr1 = tr0 + ((w64 + 63) >> sh);
     at.wc0 = tx > 64 ? tc0 + (w64 & (tx - 1)) : tc0;
     at.wc1 = tx > 64 ? at.wc0 + 63 : tc0 + tx - 1;
     at.tr0 = tr0; at.tc0 = tc0; at.tx = tx; at.ty = ty;
-    at.tid = (int)threadIdx.x;
+    at.tid = lane;
     at.valid = at.r < rows && at.c < cols;
     at.p = (long long)at.r * cols + at.c;
     return at;
+}
+
+__device__ __forceinline__ PixelAt locate(const sw_domain *doms, const int32_t *bs, int nd, int total, DomainG &G) {
+    return locate_lane(doms, bs, nd, total, G, (int)threadIdx.x);
 }
 
 // FP64 heights of a pixel as series.py:252-287 forms them: a = dsm (+ raise), vegetation
@@ -1079,10 +1084,14 @@ __device__ __forceinline__ GvfOut ground_view_fixed(const DomainG &G, const Pixe
             if (__ballot(s0 < L) == 0) break;
             const int n = NS - s0 < CHUNK ? NS - s0 : CHUNK;
             v2f buf[CHUNK];
+            // volatile: one ds_read_b64 per step with the step's offset as its immediate from the
+            // pixel's base (paired reads would need an address add per pair: their 8-bit offsets
+            // do not reach rows 21 pixels away)
+            const AS3 volatile v2f *lds = (const AS3 volatile v2f *)(AS3 void *)gv_lds;
 #pragma unroll
             for (int i = 0; i < n; ++i) {
                 const int k = (GV_STEPS[dir][s0 + i][0] + HH) * PITCH + GV_STEPS[dir][s0 + i][1] + HH;
-                buf[i] = gv_lds[base + k];
+                buf[i] = lds[base + k];
             }
             const float rem = Lsteps - (float)s0;
 #pragma unroll
@@ -1249,7 +1258,9 @@ __device__ __forceinline__ void day_patch_sums(const AS1 uint32_t *vis, long lon
         const uint32_t vvb = vis[(2 * SW_VIS_WORDS + w) * P + p];
         const uint32_t sky = vsh & vveg, vegonly = vsh & ~vveg, bld = ~vsh & vvb;
         const int k0 = w * 32, n = SW_PATCHES - k0 < 32 ? SW_PATCHES - k0 : 32;
-#pragma unroll
+        // rolled loops: this path (no search tables) is the ABI's fallback; unrolled, its per-bit
+        // lane masks fill the scalar registers of every kernel that inlines it
+#pragma nounroll
         for (int j = 0; j < n; ++j) {
             const AS4 float *c = pf + (k0 + j) * SW_PATCH_F;
             const float ds = ((sky >> j) & 1u) ? 1.0f : (((vegonly >> j) & 1u) ? dveg : 0.0f);
@@ -1267,7 +1278,7 @@ __device__ __forceinline__ void day_patch_sums(const AS1 uint32_t *vis, long lon
             const double t_sun = pd[SWP_SUN_BELOW * SW_PATCHES + BAND_START[b]];
             const double t_sh = pd[SWP_SHADE_FROM * SW_PATCHES + BAND_START[b]];
             uint32_t sun = 0u, shd = 0u;
-#pragma unroll
+#pragma nounroll
             for (int k = lo; k < hi; ++k) {
                 const double x = tan_asvf + pd[SWP_YI * SW_PATCHES + k];
                 sun |= (x < t_sun ? 1u : 0u) << (k - k0);
@@ -1294,7 +1305,6 @@ __device__ __forceinline__ void day_patch_sums(const AS1 uint32_t *vis, long lon
 // Bisection steps over the patches of each altitude band (ceil(log2(count + 1))).
 constexpr int BAND_STEPS[8] = {5, 5, 5, 5, 5, 4, 3, 1};
 
-#define AS3 __attribute__((address_space(3)))
 
 // Copy one hour row of the patch search tables into LDS (every thread of the workgroup calls;
 // the caller has synchronised the workgroup since the LDS was last read).
@@ -1666,36 +1676,37 @@ __global__ __launch_bounds__(BLOCK) void k_day_pixels(const sw_domain *__restric
                                                       int h0, int h1, const int32_t *__restrict__ gofs_,
                                                       const double *__restrict__ gbounds_, int first,
                                                       int second, int lds_cap) {
-    DomainG G0;
-    PixelAt at = locate(doms, bs, nd, total, G0);
-    const bool valid = clamp_to_raster(G0, at);
-    const long long P = G0.pixels, p = at.p;
-    float st[6] = {G0.lag[p], G0.lag[P + p], G0.lag[2 * P + p], G0.lag[3 * P + p], G0.lag[4 * P + p], 0.f};
-    const sw_domain *const dom = doms + at.d;
+    float st[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    {
+        DomainG G;
+        PixelAt at = locate(doms, bs, nd, total, G);
+        clamp_to_raster(G, at);
+        for (int k = 0; k < 5; ++k) st[k] = G.lag[k * G.pixels + at.p];
+    }
 #pragma nounroll
     for (int hour = h0; hour < h1; ++hour) {
         if (hour > h0) __syncthreads();               // the previous hour's LDS reads are done
-        // Each hour starts from the domain record, the lane's indices and the launch arguments
-        // made opaque to the loop optimiser, so that what is derived from them (plane and staging
-        // addresses, table pointers) is recomputed per hour instead of being held in registers
-        // across the hours (the record alone would fill the scalar registers; the occupancy would
-        // drop).
-        const sw_domain *dh = dom;
-        const int32_t *go = gofs_;
+        // Each hour locates its tile and reads the domain record again, from the launch arguments
+        // and the lane index made opaque to the loop optimiser, as if the launch began there:
+        // nothing derived from them (the record, plane and staging addresses, table pointers) is
+        // held in registers across the hours, where it would cost occupancy.
+        const sw_domain *dh = doms;
+        const int32_t *bh = bs, *go = gofs_;
         const double *gb = gbounds_;
-        PixelAt ah = at;
-        int fs = first, ss = second, cap = lds_cap;
-        asm volatile("" : "+s"(dh), "+s"(go), "+s"(gb));
-        asm volatile("" : "+v"(ah.p), "+v"(ah.r), "+v"(ah.c), "+v"(ah.tid));
-        asm volatile("" : "+s"(ah.tr0), "+s"(ah.tc0), "+s"(ah.tx), "+s"(ah.ty), "+s"(fs), "+s"(ss), "+s"(cap));
-        asm volatile("" : "+s"(ah.wr0), "+s"(ah.wr1), "+s"(ah.wc0), "+s"(ah.wc1));
-        DomainG G = domain_at(dh);
-        G.rec += (long long)hour * P;
-        G.masks += (long long)hour * P;
-        pixel_hour<false>(G, ah, valid, hour, (const AS4 int32_t *)go, (const AS4 double *)gb, fs, ss, cap, st);
+        int fs = first, ss = second, cap = lds_cap, lane = (int)threadIdx.x;
+        asm volatile("" : "+s"(dh), "+s"(bh), "+s"(go), "+s"(gb), "+s"(fs), "+s"(ss), "+s"(cap));
+        asm volatile("" : "+v"(lane));
+        DomainG G;
+        PixelAt at = locate_lane(dh, bh, nd, total, G, lane);
+        const bool valid = clamp_to_raster(G, at);
+        G.rec += (long long)hour * G.pixels;
+        G.masks += (long long)hour * G.pixels;
+        pixel_hour<false>(G, at, valid, hour, (const AS4 int32_t *)go, (const AS4 double *)gb, fs, ss, cap, st);
     }
-    if (valid)
-        for (int k = 0; k < 5; ++k) G0.lag[k * P + p] = st[k];
+    DomainG G;
+    PixelAt at = locate(doms, bs, nd, total, G);
+    if (clamp_to_raster(G, at))
+        for (int k = 0; k < 5; ++k) G.lag[k * G.pixels + at.p] = st[k];
 }
 
 struct Coef7 {
