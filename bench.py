@@ -2,9 +2,9 @@
 
 A step is one day (24 hourly steps) of the batched forward over the 2000x2000 c2 workload
 (seeded synthetic morphology, its 30-day forcing cycled): per step the host builds the day's
-tables (solar position, clearness, Perez, patch rows, rays) in CPU worker processes, the engine
-uploads them, and the device replays the day as a captured hipGraph, in the launch form that a
-timed comparison before the warm-up picks: the resident day (two launches: the shadow pass of all
+tables (clearness, Perez, patch rows, rays; the solar position comes from the device) in CPU
+worker processes, the engine uploads them, and the device replays the day as a captured hipGraph,
+in the launch form that a timed comparison before the warm-up picks: the resident day (two launches: the shadow pass of all
 24 hours, then the pixel pass in which every tile runs through the 24 hours with its lag state in
 registers) or two launches per hour. Tmrt and UTCI stay in device memory. Static preparation
 (walls, sky view) runs before the timed region and is reported apart.
@@ -25,7 +25,7 @@ Beside the headline the JSON line carries:
 * ``cpu_baseline``: the FP64 NumPy oracle (a port of the UMEP step) on the 200x200 c0 workload,
   its full 24 h under both land-cover readings, on one host core.
 
-``--gpus N`` with N > 1 is refused: this build runs on one GPU.
+``--gpus N`` with N > 1 is refused: the benchmark runs on one GPU.
 """
 from __future__ import annotations
 
@@ -431,8 +431,8 @@ def main():
     ap.add_argument('--no-e2e', action='store_true', help='skip the end-to-end run_timeseries leg')
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f'bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU sharding is a '
-              f'design note, not part of this build', file=sys.stderr)
+        print(f'bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU sharding is '
+              f'described, not implemented', file=sys.stderr)
         sys.exit(2)
     from solweig_amd import runtime as rt, synthetic, batched
     workers = args.workers or None

@@ -141,8 +141,8 @@ class Engine:
 
     def start_table_workers(self, workers=None):
         """Build day tables in ``workers`` CPU processes from now on (``tables_async``). The
-        tables are per-hour host FP64 work (solar position, clearness, Perez, patch rows, search
-        tables, rays); one day of one domain is one job, so days ahead and domains spread over
+        tables are per-hour host FP64 work (clearness, Perez, patch rows, search tables, rays);
+        one day of one domain is one job, so days ahead and domains spread over
         the host's cores while the device runs. Workers are spawned and touch no GPU."""
         if workers is None:
             workers = max(1, min(8, (len(os.sched_getaffinity(0)) if hasattr(os, 'sched_getaffinity')

@@ -1,10 +1,10 @@
 """Time-series driver (drop-in for series.run_timeseries / load_meteorology, series.py:91-194,
 and run.load_case, run.py:70-86).
 
-Rasters are read from .npy files or passed as arrays (GeoTIFF needs GDAL, which this build does
-not use); walls and aspect are computed from the DSM on the device (walls.calculate_dsm_walls,
+Rasters are read from .npy files or passed as arrays (GeoTIFF needs GDAL, which is not used
+here); walls and aspect are computed from the DSM on the device (walls.calculate_dsm_walls,
 series.py:212-236) and rounded to float32 as the reference stores them. The day loop runs on the
-batched engine: one upload of the day's tables and 24 x 2 launches per day, the lag state reset
+batched engine: one upload of the day's tables and the day's launches, the lag state reset
 at each day start, UTCI NaN outside the ground mask (series.py:466-469). Outputs leave through
 output.DayOutputs: staged on the device, copied to page-locked memory on a side stream and written
 by a worker thread while the next days run. With ``plain=True`` every hour goes through the
