@@ -15,7 +15,8 @@ from solweig_amd import spa_tables as T  # noqa: E402
 GROUPS = ('L0', 'L1', 'L2', 'L3', 'L4', 'L5', 'B0', 'B1', 'R0', 'R1', 'R2', 'R3', 'R4')
 
 
-def main():
+def render():
+    """The text of spa_terms.inc."""
     rows, spans = [], []
     for name in GROUPS:
         g = getattr(T, name)
@@ -33,8 +34,12 @@ def main():
     lines += ['};', '__constant__ double SPA_NUT_COEFFS[SPA_NUT_ROWS][4] = {']
     lines += ['    {%r, %r, %r, %r},' % tuple(float(v) for v in r) for r in T.NUTATION_COEFFS]
     lines += ['};']
+    return '\n'.join(lines) + '\n'
+
+
+def main():
     with open(os.path.join(HERE, 'spa_terms.inc'), 'w') as fh:
-        fh.write('\n'.join(lines) + '\n')
+        fh.write(render())
 
 
 if __name__ == '__main__':

@@ -129,3 +129,15 @@ def test_noalloc_gvf_refuses_rasters_beyond_its_scratch(rows, cols, real_bytes):
            real_bytes, ctypes.addressof(dummy), None)
     assert rc == -3                                            # SW_ERR_LIMIT
     assert b'scratch holds 67108864' in L.sw_last_error()
+
+
+def test_compiled_spa_terms_match_the_tables():
+    """The SPA term tables compiled into the library (csrc/spa_terms.inc) are the ones of
+    spa_tables.py, the tables the host restatement uses."""
+    import importlib.util
+    csrc = os.path.join(os.path.dirname(rt.__file__), 'csrc')
+    spec = importlib.util.spec_from_file_location('gen_spa_terms', os.path.join(csrc, 'gen_spa_terms.py'))
+    gen = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(gen)
+    with open(os.path.join(csrc, 'spa_terms.inc')) as fh:
+        assert fh.read() == gen.render()

@@ -184,3 +184,22 @@ def test_window_box_bounds_every_sample_of_the_window():
                                 assert r0 <= rr <= r1 and c0 <= cc <= c1, (az, alt, tx, k0, k)
                                 assert hm >= top[rr, cc]
     assert windows > 1000
+
+
+def test_hour_branches_follow_the_piecewise_edges():
+    """sky.hour_branches changes exactly where the hour chain's pieces change: the Reindl bins at
+    clearness index 0.3 and 0.78 (Kt = radG / (1370 cos(zen) D)), and the CI clamp."""
+    import numpy as np
+    from solweig_amd import sky
+    zen, alt, azi, jday, lat, P = np.radians(40.0), 50.0, 180.0, 172.0, 50.0, 101.3
+    _, _, kt1 = sky.clearness(zen, jday, 20.0, 0.5, 1.0, lat, P)           # Kt per W/m2 of radG
+    for edge, below, above in ((0.3, 0, 1), (0.78, 1, 2)):
+        g = edge / kt1
+        lo = sky.hour_branches(alt, azi, zen, jday, 20.0, 50.0, g * (1 - 1e-9), P, lat)
+        hi = sky.hour_branches(alt, azi, zen, jday, 20.0, 50.0, g * (1 + 1e-9), P, lat)
+        assert (lo[2], hi[2]) == (below, above), edge
+    night = sky.hour_branches(-5.0, azi, np.radians(95.0), jday, 20.0, 50.0, 0.0, P, lat, CI=0.9)
+    assert night == (0, True)
+    low = sky.hour_branches(alt, azi, zen, jday, 20.0, 50.0, 100.0, P, lat)
+    high = sky.hour_branches(alt, azi, zen, jday, 20.0, 50.0, 1200.0, P, lat)
+    assert low[1] is False and high[1] is True                         # CI clamped at high radG
