@@ -24,6 +24,10 @@ enum {
     SW_ERR_LIMIT = -3,
 };
 
+/* Largest raster (pixels) of one domain: the ray marches address the height planes with 32-bit
+ * byte offsets.                                                                               */
+#define SW_MAX_PIXELS ((1LL << 30) - 1)
+
 /* ----- per-hour parameter record: SW_HOUR_STRIDE doubles per (domain, hour) ------------- */
 enum {
     SWH_DAY = 0,          /* 1 when the sun altitude > 0 (model.py:96)                        */

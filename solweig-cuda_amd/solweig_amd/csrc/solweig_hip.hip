@@ -405,8 +405,16 @@ typedef double f64x8 __attribute__((ext_vector_type(8)));
 // the march plane, -inf where a sample has no canopy top: top = t + a and bot = 0.25 t + a are
 // then -inf there, which leaves topmax (>= 0), the pergola test and the receiver threshold
 // (>= 0, against dz >= 0) exactly as the zeroed tops of the reference do.
+// A buffer resource over one FP32 plane of n values (the march's gathers: a 32-bit byte offset per
+// lane instead of a 64-bit address, shared by the DSM and canopy planes of one layout; loads past
+// the plane return 0 instead of faulting).
+typedef __amdgpu_buffer_rsrc_t PlaneRsrc;
+__device__ __forceinline__ PlaneRsrc plane_rsrc(const AS1 float *plane, long long n) {
+    return __builtin_amdgcn_make_buffer_rsrc((void *)(uintptr_t)plane, (short)0, (int)(n * 4), 0x00020000);
+}
+
 template <bool MASKED, bool TOPS>
-__device__ __forceinline__ void march_chunk(March &m, const AS1 float *dsm, const AS1 float *trees,
+__device__ __forceinline__ void march_chunk(March &m, PlaneRsrc dsm, PlaneRsrc trees,
                                             double raise, double me_a, const PixelAt &at, int rows,
                                             int cols, const AS4 int32_t *off, const AS4 double *dzs,
                                             int k0, int L) {
@@ -418,23 +426,24 @@ __device__ __forceinline__ void march_chunk(March &m, const AS1 float *dsm, cons
     // of 8 steps)
     const i32x16 ov = *(const AS4 i32x16 *)(off + 2 * k0);
     const f64x8 zv = *(const AS4 f64x8 *)(dzs + k0);
+    const uint32_t p4 = (uint32_t)at.p * 4u;
 #pragma unroll
     for (int i = 0; i < CHUNK; ++i) {
         const int dr = ov[2 * i], dc = ov[2 * i + 1];
-        long long q;
+        uint32_t o;
         if (!MASKED) {
             in[i] = true;
-            q = at.p + (long long)(dr * cols + dc);
+            o = p4 + (uint32_t)(dr * cols + dc) * 4u;
         } else {
             // a lane outside the raster reads its own pixel: selects, not branches, so that the
             // chunk stays one block and its 8 offset pairs load together
             const int rr = at.r + dr, cc = at.c + dc;
             in[i] = (unsigned)rr < (unsigned)rows && (unsigned)cc < (unsigned)cols;
             const int rq = in[i] ? rr : at.r, cq = in[i] ? cc : at.c;
-            q = (long long)rq * cols + cq;
+            o = (uint32_t)(rq * cols + cq) * 4u;
         }
-        av[i] = dsm[q];
-        tv[i] = trees[q];
+        av[i] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(dsm, (int)o, 0, 0));
+        tv[i] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(trees, (int)o, 0, 0));
     }
     if (TOPS) {
         // no canopy top among the chunk's samples of any lane: top and bot are -inf, so topmax
@@ -511,7 +520,8 @@ __global__ __launch_bounds__(BLOCK) void k_hour_shadow(const sw_domain *__restri
     const double raise = G.dsm_raise;
     const Heights me = heights_at(dsm, trees, raise, p);
     const bool tops = G.march_trees != nullptr;       // uniform: one domain per workgroup
-    const AS1 float *mtrees = tops ? G.march_trees : trees;
+    const AS1 float *mtrees_ = tops ? G.march_trees : trees;
+    const PlaneRsrc rdsm = plane_rsrc(dsm, P), mtrees = plane_rsrc(mtrees_, P);
     const bool wall_px = valid && G.walls[p] > 0.0f;
     const double wall_h = (double)G.walls[p];
     const int start = uniform_index(H[SWH_RAY_START]);
@@ -565,13 +575,13 @@ __global__ __launch_bounds__(BLOCK) void k_hour_shadow(const sw_domain *__restri
         if (__ballot(in_first) == 0) break;          // every lane's ray has left the raster
         const bool full = k0 + CHUNK <= L && __ballot(in_last) == active;
         if (full && tops)
-            march_chunk<false, true>(m, dsm, mtrees, raise, me.a, at, rows, cols, off, dzs, k0, L);
+            march_chunk<false, true>(m, rdsm, mtrees, raise, me.a, at, rows, cols, off, dzs, k0, L);
         else if (full)
-            march_chunk<false, false>(m, dsm, mtrees, raise, me.a, at, rows, cols, off, dzs, k0, L);
+            march_chunk<false, false>(m, rdsm, mtrees, raise, me.a, at, rows, cols, off, dzs, k0, L);
         else if (tops)
-            march_chunk<true, true>(m, dsm, mtrees, raise, me.a, at, rows, cols, off, dzs, k0, L);
+            march_chunk<true, true>(m, rdsm, mtrees, raise, me.a, at, rows, cols, off, dzs, k0, L);
         else
-            march_chunk<true, false>(m, dsm, mtrees, raise, me.a, at, rows, cols, off, dzs, k0, L);
+            march_chunk<true, false>(m, rdsm, mtrees, raise, me.a, at, rows, cols, off, dzs, k0, L);
     }
     if (!valid) return;
     const double f = m.f, topmax = m.topmax;
@@ -681,7 +691,8 @@ __global__ __launch_bounds__(BLOCK) void k_svf_march(const float *__restrict__ d
     const long long P = G.pixels, p = at.p;
     const AS1 float *dsm = (const AS1 float *)dsm_, *trees = (const AS1 float *)trees_;
     const bool tops = mtrees_ != nullptr;
-    const AS1 float *mtrees = tops ? (const AS1 float *)mtrees_ : trees;
+    const PlaneRsrc rdsm = plane_rsrc(dsm, G.pixels),
+                    mtrees = plane_rsrc(tops ? (const AS1 float *)mtrees_ : trees, G.pixels);
     const Heights me = heights_at(dsm, trees, raise, p);
     const bool bushplant = (me.bot == 0.0) && me.top > 1.0;
     const AS4 int32_t *lens = (const AS4 int32_t *)lengths_;
@@ -720,13 +731,13 @@ __global__ __launch_bounds__(BLOCK) void k_svf_march(const float *__restrict__ d
             if (__ballot(in_first) == 0) break;              // every lane's ray has left the raster
             const bool full = k0 + CHUNK <= L && __ballot(in_last) == active;
             if (full && tops)
-                march_chunk<false, true>(m, dsm, mtrees, raise, me.a, at, rows, cols, off, dzs, k0, L);
+                march_chunk<false, true>(m, rdsm, mtrees, raise, me.a, at, rows, cols, off, dzs, k0, L);
             else if (full)
-                march_chunk<false, false>(m, dsm, mtrees, raise, me.a, at, rows, cols, off, dzs, k0, L);
+                march_chunk<false, false>(m, rdsm, mtrees, raise, me.a, at, rows, cols, off, dzs, k0, L);
             else if (tops)
-                march_chunk<true, true>(m, dsm, mtrees, raise, me.a, at, rows, cols, off, dzs, k0, L);
+                march_chunk<true, true>(m, rdsm, mtrees, raise, me.a, at, rows, cols, off, dzs, k0, L);
             else
-                march_chunk<true, false>(m, dsm, mtrees, raise, me.a, at, rows, cols, off, dzs, k0, L);
+                march_chunk<true, false>(m, rdsm, mtrees, raise, me.a, at, rows, cols, off, dzs, k0, L);
         }
         const uint32_t bit = 1u << (k - kb);
         const bool sh = m.f > me.a;
@@ -3718,6 +3729,8 @@ int sw_svf_visibility(const float *dsm, const float *trees, const float *march_t
         return fail(SW_ERR_ARGUMENT, "sw_svf_visibility: NULL pointer");
     if (rows <= 0 || cols <= 0 || capacity <= 0 || patches <= 0 || patches > 32 * SW_VIS_WORDS)
         return fail(SW_ERR_ARGUMENT, "sw_svf_visibility: bad sizes");
+    if ((long long)rows * cols > SW_MAX_PIXELS)
+        return fail(SW_ERR_LIMIT, "sw_svf_visibility: raster above SW_MAX_PIXELS");
     const int words = (patches + 31) / 32;
     if (hmax) {
         if (capacity % CHUNK != 0 || hmax_cols != (cols + (1 << HCELL_SHIFT) - 1) >> HCELL_SHIFT)

@@ -258,6 +258,8 @@ def prepare_shadow(a, walls, aspect, scale, *, vegdem=None, vegdem2=None, bush=N
     a = np.asarray(_host(a), dtype=np.float64)
     rows, cols = a.shape
     P = rows * cols
+    if P > rt.SW_MAX_PIXELS:
+        raise ValueError(f'prepare_shadow: a {rows}x{cols} raster exceeds the {rt.SW_MAX_PIXELS}-pixel limit')
     dsm32 = a.astype(np.float32)
     if not np.array_equal(dsm32.astype(np.float64), a):
         raise ValueError('prepare_shadow: the DSM must be FP32-exact')

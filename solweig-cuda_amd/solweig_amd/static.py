@@ -99,6 +99,10 @@ class PreparedDomain:
     """Device-resident static state of one raster domain."""
 
     def __init__(self, domain, *, stream=None, keep_host=False, trans=None):
+        rows, cols = np.shape(domain.dsm)
+        if rows * cols > rt.SW_MAX_PIXELS:
+            raise ValueError(f'a {rows}x{cols} raster exceeds the {rt.SW_MAX_PIXELS}-pixel limit of one domain; '
+                             f'split it into tiles (tiling.create_tiles)')
         self.domain = domain
         self.trans = float(CONSTANTS['trans'] if trans is None else trans)
         dsm = np.asarray(domain.dsm, dtype=np.float32)

@@ -1,4 +1,6 @@
 """Argument checks of the drop-in entry points that run before any device call (CPU)."""
+import os
+
 import numpy as np
 import pytest
 
@@ -27,3 +29,21 @@ def test_batch_bytes_model():
     assert host == 1_000_000 * 384
     dev, host = tiling.batch_bytes(tiling.DEFAULT_BATCH_PIXELS)
     assert dev < 120e9 and host < 50e9          # the default batch fits one 288 GB device with room
+
+
+def test_rasters_above_the_pixel_limit_are_refused():
+    """The marches address the height planes with 32-bit byte offsets: a domain of more than
+    SW_MAX_PIXELS pixels is refused before anything is allocated (the header's limit and the
+    bindings' agree)."""
+    import re
+    from types import SimpleNamespace
+    from solweig_amd import model, runtime as rt
+    from solweig_amd.static import PreparedDomain
+    text = open(os.path.join(os.path.dirname(__file__), '..', 'include', 'solweig_hip.h')).read()
+    m = re.search(r'#define SW_MAX_PIXELS \(\(1LL << (\d+)\) - 1\)', text)
+    assert m and (1 << int(m.group(1))) - 1 == rt.SW_MAX_PIXELS
+    big = np.broadcast_to(np.float32(1.0), (40000, 27000))          # 1.08e9 pixels, no memory
+    with pytest.raises(ValueError, match='pixel limit'):
+        PreparedDomain(SimpleNamespace(dsm=big, trees=big, dem=big))
+    with pytest.raises(ValueError, match='pixel limit'):
+        model.prepare_shadow(big, big, big, 1.0)
