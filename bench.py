@@ -3,25 +3,29 @@
 A step is one day (24 hourly steps) of the batched forward over the 2000x2000 c2 workload
 (seeded synthetic morphology, its 30-day forcing cycled): per step the host builds the day's
 tables (clearness, Perez, patch rows, rays; the solar position comes from the device) in CPU
-worker processes, the engine uploads them, and the device replays the day as a captured hipGraph,
-in the launch form that a timed comparison before the warm-up picks: the resident day (two launches: the shadow pass of all
-24 hours, then the pixel pass in which every tile runs through the 24 hours with its lag state in
-registers) or two launches per hour. Tmrt and UTCI stay in device memory. Static preparation
-(walls, sky view) runs before the timed region and is reported apart.
+worker processes, the engine uploads them, and the device replays the day's launches as captured
+hipGraphs, in the launch form that a timed comparison before the warm-up picks:
+* pipelined: resident days whose all-hours shadow pass runs on a side stream beside the previous
+  day's pixel pass (batched.Engine.run_days; every timed run starts with a lone shadow pass and
+  ends with a lone pixel pass, so all of its days' work lies inside the timed region);
+* resident: the shadow pass of all 24 hours, then the pixel pass in which every tile runs through
+  the 24 hours with its lag state in registers, one day after the other;
+* hourly: two launches per hour.
+Tmrt and UTCI stay in device memory. Static preparation (walls, sky view) runs before the timed
+region and is reported apart.
 
 Beside the headline the JSON line carries:
 * ``config.eager``: the same days launched eagerly with HIP events around both launches, which give
   the per-kernel durations of ``roofline`` (dominant kernel: the pixel pass);
-* ``config.launch_form`` / ``other_launch_form``: the timed choice between the resident day and
-  two launches per hour (lag state through its device planes), and the same days in the form not
-  chosen;
+* ``config.launch_form`` / ``other_launch_form``: the timed choice between the three forms, and the
+  same days in the forms not chosen;
 * ``config.plain_forward``: the per-step drop-in API (host tables, uploads, two full-output
   launches, the UTCI kernel and a stream sync every hour), and under ``stepwise`` the per-step
   forward in the reference's own decomposition (about 20 standalone sub-model launches, each
   followed by a sync, per daytime hour) with the batched / stepwise ratio;
 * ``config.forward_adjoint``: forward vs forward + adjoint on the 1000x1000 differentiation workload;
-* ``config.c4``: the 64-domain uneven batch, one launch per pass and hour, tables built inside its
-  timed region;
+* ``config.c4``: the 64-domain uneven batch, pipelined resident days (one launch per pass and day
+  over all domains), tables built inside its timed region;
 * ``cpu_baseline``: the FP64 NumPy oracle (a port of the UMEP step) on the 200x200 c0 workload,
   its full 24 h under both land-cover readings, on one host core.
 
@@ -62,6 +66,13 @@ TRAFFIC_SOURCE = 'profiles/k_day_pixels_c2_2000x2000.md'
 # per cycle without the packed FP32 forms): the ceiling the pixel pass actually runs against.
 VALU_2000 = {'k_day_pixels': 57106 * 62500, 'k_hour_pixels': 2323 * 62500}
 VALU_PEAK = 256 * 2.4e9
+FORM_TEXT = {
+    'pipelined': 'resident days pipelined over two streams (each day\'s all-hours shadow pass beside the '
+                 'previous day\'s pixel pass, batched.Engine.run_days; the timed run starts with a lone shadow '
+                 'pass and ends with a lone pixel pass)',
+    'resident': 'the resident day (sw_day_forward)',
+    'hourly': 'two launches per hour (sw_hours_forward)',
+}
 
 
 def log(msg):
@@ -232,6 +243,16 @@ def run_days(eng, pipe, n, use_graph, events=None):
     shadow and pixel launch of every hour (events 3h .. 3h + 2)."""
     from solweig_amd import runtime as rt
     used = []
+    if events is None and eng.pipelined:
+        # consecutive days with each day's shadow pass beside the previous day's pixel pass; the
+        # run starts with the first day's shadow pass alone and ends with the last pixel pass alone
+        def days():
+            for _ in range(n):
+                used.append(pipe.next())
+                yield used[-1]
+        eng.run_days(days(), use_graph=use_graph)
+        eng.stream.sync()
+        return used
     for k in range(n):
         tabs = pipe.next()
         eng.load(tabs)
@@ -265,8 +286,9 @@ def run_days(eng, pipe, n, use_graph, events=None):
 
 def c4_leg(args, workers):
     """The uneven-size batch: 64 domains (sides U_int[256, 1536], latitudes U(0, 60) N, each with
-    its own morphology and forcing) in one engine, one launch per pass and hour over all of them.
-    A step is one day; each day's tables (64 domains) are built by the table workers inside the
+    its own morphology and forcing) in one engine: pipelined resident days, one launch per pass and
+    day over all of them (each day's shadow pass beside the previous day's pixel pass). A step is
+    one day; each day's tables (64 domains) are built by the table workers inside the
     timed region. The smallest and the largest domain are run again alone and must match their part
     of the batch bit for bit."""
     from solweig_amd import runtime as rt, synthetic, batched
@@ -278,7 +300,7 @@ def c4_leg(args, workers):
     log(f'[bench] c4: {len(doms)} domains, sides {min(sides)}..{max(sides)}, {P / 1e6:.1f} Mpx '
         f'(generated in {t_gen:.1f} s)')
     t0 = time.perf_counter()
-    eng = batched.Engine(doms, n_hours=24)
+    eng = batched.Engine(doms, n_hours=24, pipelined=True)
     rt.synchronize()
     t_prep = time.perf_counter() - t0
     eng.set_tiles(args.tile or 16)
@@ -315,7 +337,8 @@ def c4_leg(args, workers):
     audit = c4_audit(eng, doms, sides, args.c4_audit)
     return dict(value=value, unit='pixel-hours/s', ms_per_day=dt / args.c4_steps * 1e3, steps=args.c4_steps,
                 workload=f'c4 uneven batch: {len(doms)} domains, sides {min(sides)}..{max(sides)}, {P} pixels, '
-                         f'24 h per step, one launch per pass and hour, day tables built in the timed region',
+                         f'24 h per step, pipelined resident days (one launch per pass and day over all '
+                         f'domains), day tables built in the timed region',
                 tile=args.tile or 16, generation_s=t_gen, static_prep_s=t_prep,
                 host_tables_one_core_ms=t_tab_serial * 1e3, table_workers=eng.workers, checks=checks,
                 fp64_audit=audit)
@@ -405,7 +428,7 @@ def main():
     ap.add_argument('--size', type=int, default=2000)
     ap.add_argument('--eager', action='store_true', help='headline with eager launches instead of the hipGraph')
     ap.add_argument('--eager-days', type=int, default=3, help='days of the eager, event-timed leg (0 = skip)')
-    ap.add_argument('--form', choices=('auto', 'resident', 'hourly'), default='auto',
+    ap.add_argument('--form', choices=('auto', 'pipelined', 'resident', 'hourly'), default='auto',
                     help='day launch form: timed choice, resident day (2 launches), or 2 launches per hour')
     ap.add_argument('--other-days', type=int, default=3,
                     help='days timed in the launch form not chosen for the headline (0 = skip)')
@@ -462,7 +485,7 @@ def main():
     eng.workers = n_workers = 0 if args.no_workers else eng.start_table_workers(workers)
     log(f'[bench] {n_workers} table worker processes')
     use_graph = not args.eager
-    n_total = args.warmup + args.steps + max(0, args.eager_days) + (1 + args.other_days if args.other_days > 0 else 0)
+    n_total = args.warmup + args.steps + max(0, args.eager_days) + (2 * (1 + args.other_days) if args.other_days > 0 else 0)
     pipe = DayPipeline(eng, [k % days for k in range(n_total)])
     # Launch form: the first three days, cycled with their uploads, in both forms (resident day and
     # two launches per hour), alternated; the engine keeps the faster (autotune.tune_launch_form).
@@ -473,11 +496,12 @@ def main():
         from solweig_amd import autotune
         t0 = time.perf_counter()
         form = autotune.tune_launch_form(eng, [eng.tables(day_index=k) for k in range(3)], use_graph=use_graph)
-        log(f'[bench] launch form: {"resident day" if form["resident"] else "two launches per hour"} '
-            f'(ms per replayed day: {form["ms"]}; {time.perf_counter() - t0:.2f} s)')
+        log(f'[bench] launch form: {form["form"]} (ms per day: {form["ms"]}; {time.perf_counter() - t0:.2f} s)')
     else:
-        eng.resident, eng.graph = args.form == 'resident', None
+        from solweig_amd import autotune
+        autotune.set_launch_form(eng, args.form)
     resident = eng.resident
+    form_name = 'pipelined' if eng.pipelined else ('resident' if eng.resident else 'hourly')
     run_days(eng, pipe, args.warmup, use_graph)
     rt.synchronize()
     log(f'[bench] {args.warmup} warm-up days done')
@@ -541,21 +565,26 @@ def main():
             f'{100 * achieved / HBM_PEAK:.1f} % of HBM')
     other = None
     if args.other_days > 0:
-        # the same days in the other launch form (resident day <-> two launches per hour)
-        chosen = eng.resident
-        eng.resident, eng.graph = not chosen, None
-        run_days(eng, pipe, 1, use_graph)
-        rt.synchronize()
-        t0 = time.perf_counter()
-        run_days(eng, pipe, args.other_days, use_graph)
-        rt.synchronize()
-        odt = time.perf_counter() - t0
-        eng.resident, eng.graph = chosen, None
-        other = dict(form='two launches per hour (sw_hours_forward)' if chosen else 'resident day (sw_day_forward)',
-                     value=P * 24 * args.other_days / odt, ms_per_day=odt / args.other_days * 1e3,
-                     days=args.other_days, headline_over_other=value / (P * 24 * args.other_days / odt))
-        log(f'[bench] other launch form ({other["form"]}): {other["ms_per_day"]:.2f} ms/day '
-            f'({other["value"]:.3e} px-h/s); headline / other = {other["headline_over_other"]:.3f}')
+        # the same days in the launch forms not chosen (autotune.FORMS)
+        from solweig_amd import autotune
+        chosen = 'pipelined' if eng.pipelined else ('resident' if eng.resident else 'hourly')
+        other = {}
+        for f in autotune.FORMS:
+            if f == chosen:
+                continue
+            autotune.set_launch_form(eng, f)
+            run_days(eng, pipe, 1, use_graph)
+            rt.synchronize()
+            t0 = time.perf_counter()
+            run_days(eng, pipe, args.other_days, use_graph)
+            rt.synchronize()
+            odt = time.perf_counter() - t0
+            ov = P * 24 * args.other_days / odt
+            other[f] = dict(value=ov, ms_per_day=odt / args.other_days * 1e3, days=args.other_days,
+                            headline_over_this=value / ov)
+            log(f'[bench] launch form {f}: {other[f]["ms_per_day"]:.2f} ms/day ({ov:.3e} px-h/s); '
+                f'headline / this = {value / ov:.3f}')
+        autotune.set_launch_form(eng, chosen)
     eng.close_table_workers()
 
     plain = None
@@ -627,9 +656,10 @@ def main():
                                     f'cycled), day tables built '
                                     f'{f"by {n_workers} CPU worker processes" if n_workers else "in the main process"}'
                                     f' inside the timed region, '
-                                    f'{"hipGraph replay" if use_graph else "eager launches"} of the '
-                                    f'{"resident day (sw_day_forward)" if resident else "two launches per hour (sw_hours_forward)"}',
-                           graph=use_graph, launch_form=form, resident=resident, static_prep=static, tile=tuned,
+                                    f'{"hipGraph replay" if use_graph else "eager launches"} of '
+                                    f'{FORM_TEXT[form_name]}',
+                           graph=use_graph, launch_form=form if form is not None else form_name,
+                           resident=resident, static_prep=static, tile=tuned,
                            eager=eager, other_launch_form=other,
                            plain_forward=plain,
                            forward_adjoint=adj, c4=c4, end_to_end=e2e),

@@ -253,6 +253,14 @@ int sw_day_shadow(const sw_domain *domains_dev, const int32_t *block_start_dev, 
 int sw_day_pixels(const sw_domain *domains_dev, const int32_t *block_start_dev, int n_domains,
                   int total_blocks, int hour_begin, int hour_end, const int32_t *gvf_offsets_dev,
                   const double *gvf_bounds_dev, int first_steps, int second_steps, void *stream);
+/* sw_day_pixels with at most groups_per_cu (2..8) of its workgroups on one CU (the launch asks
+ * for the LDS that enforces it), for a pixel pass that runs beside another day's sw_day_shadow
+ * on a second stream: the shadow pass's waves then find registers on every CU (the same day
+ * loop, series.py:445-469, with consecutive days overlapped). Results equal sw_day_pixels.   */
+int sw_day_pixels_capped(const sw_domain *domains_dev, const int32_t *block_start_dev, int n_domains,
+                         int total_blocks, int hour_begin, int hour_end, const int32_t *gvf_offsets_dev,
+                         const double *gvf_bounds_dev, int first_steps, int second_steps,
+                         int groups_per_cu, void *stream);
 
 /* ----- adjoint of J = mean over ground pixels of UTCI at one objective hour -------------------
  * (no counterpart in the reference: the differentiable form of the per-step chain

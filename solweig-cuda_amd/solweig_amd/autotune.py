@@ -93,17 +93,33 @@ def tune(engine, tables=None, candidates=rt.TILE_CHOICES, repeats=3, use_cache=T
     return dict(result, cached=False)
 
 
-def tune_launch_form(engine, tables=None, days=10, rounds=3, use_graph=True):
-    """Pick how the engine launches a day: resident (sw_day_forward: the shadow pass of every hour
-    in one launch, then one pixel pass with the lag state in registers) or two launches per hour
-    (sw_hours_forward). ``tables``: a list of day tables (one day's list per entry) cycled with an
+FORMS = ('pipelined', 'resident', 'hourly')
+
+
+def set_launch_form(engine, form):
+    """Set ``engine``'s launch form: 'pipelined' (resident days whose shadow pass overlaps the
+    previous day's pixel pass: Engine.run_days), 'resident' (sw_day_forward) or 'hourly'
+    (sw_hours_forward)."""
+    if form not in FORMS:
+        raise ValueError(f'launch form {form!r} not in {FORMS}')
+    engine.resident, engine.pipelined, engine.graph = form != 'hourly', form == 'pipelined', None
+
+
+def tune_launch_form(engine, tables=None, days=10, rounds=3, use_graph=True, forms=FORMS):
+    """Pick how the engine launches its days: 'pipelined' (resident days, each day's all-hours
+    shadow pass running on a side stream alongside the previous day's pixel pass, Engine.run_days),
+    'resident' (sw_day_forward: the shadow pass of every hour in one launch, then one pixel pass
+    with the lag state in registers, one day after the other) or 'hourly' (two launches per hour,
+    sw_hours_forward). ``tables``: a list of day tables (one day's list per entry) cycled with an
     upload before every timed day, as a run of consecutive days does (default: the loaded day,
-    replayed). Times ``days`` days in each form, alternating the forms for ``rounds`` rounds (so
-    that clock and cache drift falls on both), keeps the faster by the best round and leaves it set
-    on the engine; returns {'resident', 'ms'} (ms per day)."""
+    replayed). Times ``days`` consecutive days in each form, alternating the forms for ``rounds``
+    rounds (so that clock and cache drift falls on all), keeps the fastest by the best round and
+    leaves it set on the engine; returns {'form', 'resident', 'pipelined', 'ms'} (ms per day)."""
     cycle = list(tables) if tables is not None else [None]
+    if cycle == [None] and 'pipelined' in forms:
+        raise ValueError('the pipelined form needs the day tables')
     ev = [rt.Event() for _ in range(2)]
-    best = {True: float('inf'), False: float('inf')}
+    best = {f: float('inf') for f in forms}
 
     def day(k):
         if cycle[k % len(cycle)] is not None:
@@ -112,15 +128,21 @@ def tune_launch_form(engine, tables=None, days=10, rounds=3, use_graph=True):
         engine.forward_day(use_graph=use_graph)
 
     for _ in range(rounds):
-        for resident in (True, False):
-            engine.resident, engine.graph = resident, None
-            day(0)                                          # capture / warm this form
-            ev[0].record(engine.stream)
-            for k in range(days):
-                day(k + 1)
+        for form in forms:
+            set_launch_form(engine, form)
+            if form == 'pipelined':
+                engine.run_days([cycle[0], cycle[1 % len(cycle)]], use_graph=use_graph)   # capture / warm
+                ev[0].record(engine.stream)
+                engine.run_days((cycle[(k + 1) % len(cycle)] for k in range(days)), use_graph=use_graph)
+            else:
+                day(0)                                      # capture / warm this form
+                ev[0].record(engine.stream)
+                for k in range(days):
+                    day(k + 1)
             ev[1].record(engine.stream)
             engine.stream.sync()
-            best[resident] = min(best[resident], ev[0].elapsed_ms(ev[1]) / days)
-    engine.resident, engine.graph = best[True] <= best[False], None
+            best[form] = min(best[form], ev[0].elapsed_ms(ev[1]) / days)
+    form = min(forms, key=lambda f: best[f])
+    set_launch_form(engine, form)
     engine.reset_state()
-    return dict(resident=engine.resident, ms={'resident': best[True], 'hourly': best[False]})
+    return dict(form=form, resident=engine.resident, pipelined=engine.pipelined, ms=best)

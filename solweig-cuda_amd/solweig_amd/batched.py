@@ -26,13 +26,18 @@ from .sun import met_sun, met_sun_device
 
 
 class Engine:
-    def __init__(self, domains, n_hours=24, stream=None, keep_outputs=True, extra_outputs=False, resident=True):
+    def __init__(self, domains, n_hours=24, stream=None, keep_outputs=True, extra_outputs=False, resident=True,
+                 pipelined=False):
         """``extra_outputs``: also keep each hour's Kdown, Kup, Ldown, Lup and Shadow planes
         ([n_hours][5][P] per domain, rt.EXTRA_NAMES order), the optional series outputs.
         ``resident``: run a day as two launches (sw_day_forward: the shadow pass of every hour into
         per-hour record and mask planes, 9 B per pixel-hour, then one pixel pass in which each tile
         runs through the hours with its lag state in registers) instead of two launches per hour
-        (sw_hours_forward)."""
+        (sw_hours_forward); an engine built with ``resident=False`` keeps one record slot and
+        cannot switch to the resident forms.
+        ``pipelined``: multi-day callers (series, tiling, bench) run consecutive days through
+        ``run_days``, each day's shadow pass alongside the previous day's pixel pass (resident
+        form only); ``forward_day`` always runs one day on its own."""
         scales = {float(d.domain.scale if isinstance(d, PreparedDomain) else d.scale) for d in domains}
         if len(scales) != 1:
             # one launch shares one ground-view step table (the step shifts and counts follow the
@@ -65,6 +70,7 @@ class Engine:
             b['masks'] = rt.DeviceArray((self.n_hours if resident else 1, d.P), np.uint8)
             self.buf.append(b)
         self.resident = bool(resident)
+        self.pipelined = bool(pipelined) and self.resident
         self._sun_days = [None] * len(self.domains)
         self.block_start = rt.DeviceArray((len(self.domains) + 1,), np.int32)
         self.structs = (rt.SwDomain * len(self.domains))()
@@ -83,6 +89,14 @@ class Engine:
         self._stage_used = [False, False]
         self._slot = 0
         self.graph = None
+        # pipelined days (day_start / day_next): a second table and record set, a side stream for
+        # the shadow passes, and the set the current day's tables and records are in
+        self._set1 = None
+        self.dom_devs = [self.dom_dev]
+        self._side = None
+        self._pipe_set = None
+        self._pipe_graphs = {}
+        self.groups_beside_shadow = 4
 
     def _write_domains(self):
         """Upload the domain records and the tile prefix (after a change of tile widths)."""
@@ -90,17 +104,22 @@ class Engine:
         starts[1:] = np.cumsum([d.tiles for d in self.domains])
         self.total_blocks = int(starts[-1])
         self.block_start.upload(starts, self.stream)
-        for i, (d, b) in enumerate(zip(self.domains, self.buf)):
-            self.structs[i] = d.struct(hours=b['hours'].ptr, ptab_f=b['ptab_f'].ptr, ptab_d=b['ptab_d'].ptr,
-                                       ptab_s=b['ptab_s'].ptr, ray_off=b['ray_off'].ptr, ray_dz=b['ray_dz'].ptr,
-                                       tmrt=b['tmrt'].ptr if 'tmrt' in b else 0,
-                                       utci=b['utci'].ptr if 'utci' in b else 0,
-                                       full=b['extra'].ptr if 'extra' in b else 0,
-                                       rec=b['rec'].ptr, masks=b['masks'].ptr)
-        rt.call('sw_memcpy_h2d', self.dom_dev.ptr, ctypes.addressof(self.structs),
-                ctypes.sizeof(self.structs), self.stream.handle)
-        self.stream.sync()
+        sets = [(self.buf, self.dom_dev)]
+        if getattr(self, '_set1', None) is not None:
+            sets.append((self._set1, self.dom_devs[1]))
+        for bufs, dev in sets:
+            for i, (d, b, o) in enumerate(zip(self.domains, bufs, self.buf)):
+                self.structs[i] = d.struct(hours=b['hours'].ptr, ptab_f=b['ptab_f'].ptr, ptab_d=b['ptab_d'].ptr,
+                                           ptab_s=b['ptab_s'].ptr, ray_off=b['ray_off'].ptr, ray_dz=b['ray_dz'].ptr,
+                                           tmrt=o['tmrt'].ptr if 'tmrt' in o else 0,
+                                           utci=o['utci'].ptr if 'utci' in o else 0,
+                                           full=o['extra'].ptr if 'extra' in o else 0,
+                                           rec=b['rec'].ptr, masks=b['masks'].ptr)
+            rt.call('sw_memcpy_h2d', dev.ptr, ctypes.addressof(self.structs), ctypes.sizeof(self.structs),
+                    self.stream.handle)
+            self.stream.sync()
         self.graph = None                                # launch geometry changed
+        self._pipe_graphs = {}
 
     def set_tiles(self, tile_cols):
         """Tile width per domain (one value for all, or a list), from rt.TILE_CHOICES."""
@@ -168,11 +187,13 @@ class Engine:
         return _Pending([self._pool.submit(_day_tables, *job)
                          for job in self._jobs(day_index, met, sun, constants, psi)])
 
-    def load(self, tables):
+    def load(self, tables, set_index=0):
         """Queue the upload of one day's tables on the engine's stream (fixed device addresses, so
         a captured graph stays valid). The copies run after every launch queued before them, so
-        the host may build and load the next day while the device still runs this one."""
-        for t, b in zip(tables, self.buf):
+        the host may build and load the next day while the device still runs this one.
+        ``set_index`` 1: into the second table set of the pipelined days (day_start / day_next)."""
+        bufs = self.buf if set_index == 0 else self._set1
+        for t, b in zip(tables, bufs):
             if t.n_hours != self.n_hours:
                 raise ValueError(f'expected {self.n_hours} hours, got {t.n_hours}')
             if t.ray_dz.shape[0] > b['cap']:
@@ -182,7 +203,7 @@ class Engine:
         if self._stage_used[slot]:
             self._stage_event[slot].sync()        # the copies out of this slot have finished
         h = self.stream.handle
-        for t, b, st in zip(tables, self.buf, self._stage[slot]):
+        for t, b, st in zip(tables, bufs, self._stage[slot]):
             n = t.ray_dz.shape[0]
             st['hours'].array[...] = t.hours
             st['ptab_f'].array[...] = t.ptab_f
@@ -202,7 +223,13 @@ class Engine:
             d.dev['lag'].zero(self.stream)
 
     # ------------------------------------------------------------------ launches
+    def _check_resident(self):
+        if self.buf[0]['rec'].shape[0] != self.n_hours:
+            raise RuntimeError('this engine was built with resident=False: its record planes hold one hour')
+
     def _launch_hours(self, h0, h1):
+        if self.resident:
+            self._check_resident()
         rt.call('sw_day_forward' if self.resident else 'sw_hours_forward', self.dom_dev.ptr, self.block_start.ptr,
                 len(self.domains), self.total_blocks, h0, h1, self.gvf_off.ptr, self.gvf_bounds.ptr, self.first,
                 self.second, self.stream.handle)
@@ -221,6 +248,128 @@ class Engine:
             self.graph.launch(self.stream)
         else:
             self._launch_hours(0, self.n_hours)
+
+    # ------------------------------------------------------------------ pipelined days
+    # Shadows depend only on the static rasters and the day's tables, never on the lag state, so
+    # the all-hours shadow pass of day k + 1 can run while the pixel pass of day k runs: the two
+    # kernels are bound differently (the march by scalar window tests and gather latency, the
+    # pixel pass by vector issue) and fill each other's idle issue slots. Days alternate between
+    # two table and record sets; the shadow passes run on a side stream.
+    def _ensure_pipeline(self):
+        self._check_resident()
+        if self._set1 is not None:
+            return
+        self._set1 = []
+        for d, b in zip(self.domains, self.buf):
+            self._set1.append(dict(hours=rt.DeviceArray(b['hours'].shape, np.float64),
+                                   ptab_f=rt.DeviceArray(b['ptab_f'].shape, np.float32),
+                                   ptab_d=rt.DeviceArray(b['ptab_d'].shape, np.float64),
+                                   ptab_s=rt.DeviceArray(b['ptab_s'].shape, np.uint32),
+                                   ray_off=rt.DeviceArray((b['cap'], 2), np.int32),
+                                   ray_dz=rt.DeviceArray((b['cap'],), np.float64), cap=b['cap'],
+                                   rec=rt.DeviceArray((self.n_hours, d.P, 2), np.float32),
+                                   masks=rt.DeviceArray((self.n_hours, d.P), np.uint8)))
+        self.dom_devs = [self.dom_dev, rt.DeviceArray((ctypes.sizeof(self.structs),), np.uint8)]
+        self._side = rt.Stream()
+        self._fork, self._join = rt.Event(), rt.Event()
+        self._write_domains()
+
+    def _args(self, k):
+        return (self.dom_devs[k].ptr, self.block_start.ptr, len(self.domains), self.total_blocks, 0, self.n_hours)
+
+    def _shadow_pass(self, k, stream):
+        rt.call('sw_day_shadow', *self._args(k), stream.handle)
+
+    def _pixel_pass(self, k, stream, beside_shadow=False):
+        """Pixel pass of set k; ``beside_shadow``: a shadow pass runs on the side stream meanwhile,
+        so at most ``groups_beside_shadow`` pixel workgroups share a CU (at 2000^2, 4 per CU
+        leave two shadow waves' registers on every SIMD: 7.81-7.84 ms per pipelined day against
+        7.8-8.4 ms with the 5 that registers and LDS allow)."""
+        if beside_shadow and self.groups_beside_shadow:
+            rt.call('sw_day_pixels_capped', *self._args(k), self.gvf_off.ptr, self.gvf_bounds.ptr, self.first,
+                    self.second, self.groups_beside_shadow, stream.handle)
+        else:
+            rt.call('sw_day_pixels', *self._args(k), self.gvf_off.ptr, self.gvf_bounds.ptr, self.first,
+                    self.second, stream.handle)
+
+    def _overlapped(self, k, shadow_next):
+        """The pixel pass of set k on the engine's stream and, with ``shadow_next``, the shadow pass
+        of set 1 - k on the side stream, joined back into the engine's stream."""
+        if shadow_next:
+            self._fork.record(self.stream)
+            self._side.wait(self._fork)
+            self._shadow_pass(1 - k, self._side)
+            self._join.record(self._side)
+        self._pixel_pass(k, self.stream, beside_shadow=shadow_next)
+        if shadow_next:
+            self.stream.wait(self._join)
+
+    def _capture_pipeline(self):
+        """Capture the four graphs of day_next (set 0 or 1, with or without the next shadow pass)."""
+        self._ensure_pipeline()
+        for k in (0, 1):
+            for more in (True, False):
+                if (k, more) not in self._pipe_graphs:
+                    self._pipe_graphs[(k, more)] = rt.Graph.capture(
+                        self.stream, lambda k=k, more=more: self._overlapped(k, more))
+
+    @property
+    def day_open(self):
+        """True between day_start and the day_next that had no next day."""
+        return self._pipe_set is not None
+
+    def day_start(self, tables):
+        """First day of a run of pipelined days (resident launch form): load its tables and enqueue
+        its shadow pass. Then call ``day_next`` once per day."""
+        self._ensure_pipeline()
+        self._pipe_set = 0
+        self.load(tables, 0)
+        self._shadow_pass(0, self.stream)
+
+    def day_next(self, next_tables=None, use_graph=False):
+        """Enqueue the pixel pass of the current day (lag state cleared first) on the engine's
+        stream; with ``next_tables`` the next day's tables go into the other set and its shadow pass
+        runs alongside, on the side stream. When this returns, the current day's outputs are
+        complete in the engine's stream order, until the next day_next overwrites them. With
+        ``use_graph`` the two passes are replayed from a captured graph (one per set and form)."""
+        if self._pipe_set is None:
+            raise RuntimeError('day_next without day_start')
+        k, more = self._pipe_set, next_tables is not None
+        if more:
+            self.load(next_tables, 1 - k)
+        self.reset_state()
+        if use_graph:
+            self._capture_pipeline()
+            self._pipe_graphs[(k, more)].launch(self.stream)
+        else:
+            self._overlapped(k, more)
+        self._pipe_set = 1 - k if more else None
+
+    def run_days(self, tables, use_graph=False, after_day=None):
+        """Run consecutive days through day_start / day_next. ``tables``: an iterable of day tables
+        (or of objects with ``result()``, e.g. ``tables_async``), consumed one day ahead;
+        ``after_day(i)`` is called when day i's pixel pass is enqueued (its outputs are then
+        complete in the engine's stream order). Returns the number of days run."""
+        it = iter(tables)
+
+        def take():
+            t = next(it, None)
+            return t.result() if hasattr(t, 'result') else t
+
+        if use_graph:
+            self._capture_pipeline()
+        cur, n = take(), 0
+        if cur is None:
+            return 0
+        self.day_start(cur)
+        while True:
+            nxt = take()
+            self.day_next(nxt, use_graph=use_graph)
+            if after_day is not None:
+                after_day(n)
+            n += 1
+            if nxt is None:
+                return n
 
     def hour(self, h, full=False):
         """Both passes of one hour (for step-wise use and tests)."""

@@ -1775,10 +1775,27 @@ size_t pixel_lds_bytes(int cap) {
     return bytes < (size_t)SW_SEARCH_WORDS * 4 ? (size_t)SW_SEARCH_WORDS * 4 : bytes;
 }
 
+// groups_per_cu > 0: the launch asks for enough LDS per workgroup that at most that many of its
+// workgroups share a CU (the LDS is the CU's, so the cap holds whatever the registers allow).
 int launch_day_pixels(const sw_domain *doms, const int32_t *bs, int nd, int total, int h0, int h1,
-                      const int32_t *gofs, const double *gb, int first, int second, hipStream_t s) {
+                      const int32_t *gofs, const double *gb, int first, int second, hipStream_t s,
+                      int groups_per_cu = 0) {
     const int cap = lds_records(second);
-    hipLaunchKernelGGL(k_day_pixels, dim3(total), dim3(BLOCK), pixel_lds_bytes(cap), s, doms, bs, nd, total, h0, h1,
+    size_t bytes = pixel_lds_bytes(cap);
+    if (groups_per_cu > 0) {
+        int dev = 0, per_cu = 0;
+        int rc = hip_check(hipGetDevice(&dev), "hipGetDevice");
+        if (!rc) rc = hip_check(hipDeviceGetAttribute(&per_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev),
+                                "hipDeviceGetAttribute");
+        if (rc) return rc;
+        // one allocation granule above per_cu / (groups_per_cu + 1): groups_per_cu + 1 workgroups
+        // no longer fit, groups_per_cu still do
+        const size_t need = (size_t)per_cu / (size_t)(groups_per_cu + 1) + 1024;
+        if (need * (size_t)groups_per_cu > (size_t)per_cu || need > 65536)
+            return fail(SW_ERR_ARGUMENT, "sw_day_pixels_capped: groups_per_cu out of range for this device");
+        if (need > bytes) bytes = need;
+    }
+    hipLaunchKernelGGL(k_day_pixels, dim3(total), dim3(BLOCK), bytes, s, doms, bs, nd, total, h0, h1,
                        gofs, gb, first, second, cap);
     return hip_check(hipGetLastError(), "k_day_pixels launch");
 }
@@ -3825,6 +3842,19 @@ int sw_day_pixels(const sw_domain *doms, const int32_t *bs, int nd, int total, i
         return fail(SW_ERR_ARGUMENT, "sw_day_pixels: bad arguments");
     if (h1 == h0) return SW_OK;
     return launch_day_pixels(doms, bs, nd, total, h0, h1, gofs, gb, first, second, (hipStream_t)stream);
+}
+
+int sw_day_pixels_capped(const sw_domain *doms, const int32_t *bs, int nd, int total, int h0, int h1,
+                         const int32_t *gofs, const double *gb, int first, int second, int groups_per_cu,
+                         void *stream) {
+    int rc = check_launch(doms, bs, nd, total);
+    if (rc) return rc;
+    if (!gofs || !gb || first < 1 || second < 1 || second > MAX_SECOND || first > second || h0 < 0 || h1 < h0 ||
+        groups_per_cu < 2 || groups_per_cu > 8)
+        return fail(SW_ERR_ARGUMENT, "sw_day_pixels_capped: bad arguments");
+    if (h1 == h0) return SW_OK;
+    return launch_day_pixels(doms, bs, nd, total, h0, h1, gofs, gb, first, second, (hipStream_t)stream,
+                             groups_per_cu);
 }
 
 int sw_day_forward(const sw_domain *doms, const int32_t *bs, int nd, int total, int h0, int h1,

@@ -117,6 +117,7 @@ def lib():
         'sw_day_forward': ([vp, vp, i32, i32, i32, i32, vp, vp, i32, i32, vp], i32),
         'sw_day_shadow': ([vp, vp, i32, i32, i32, i32, vp], i32),
         'sw_day_pixels': ([vp, vp, i32, i32, i32, i32, vp, vp, i32, i32, vp], i32),
+        'sw_day_pixels_capped': ([vp, vp, i32, i32, i32, i32, vp, vp, i32, i32, i32, vp], i32),
         'sw_utci_scalar': ([vp, i64, vp, f64, i32, vp, vp, vp], i32),
         'sw_sun_position': ([vp, i32, i32, i32, f64, f64, f64, f64, vp, vp], i32),
         'sw_copy': ([vp, vp, sz, vp], i32),
@@ -146,7 +147,7 @@ EXPORTED = ('sw_version', 'sw_record_bytes', 'sw_last_error', 'sw_device_count',
             'sw_stream_destroy', 'sw_stream_sync', 'sw_event_create', 'sw_event_destroy',
             'sw_event_record', 'sw_event_elapsed_ms', 'sw_device_sync', 'sw_svf_visibility',
             'sw_svf_reduce', 'sw_sky_sums', 'sw_hour_shadow', 'sw_hour_pixels', 'sw_hours_forward', 'sw_day_forward',
-            'sw_day_shadow', 'sw_day_pixels', 'sw_sun_position', 'sw_utci_scalar',
+            'sw_day_shadow', 'sw_day_pixels', 'sw_day_pixels_capped', 'sw_sun_position', 'sw_utci_scalar',
             'sw_copy', 'sw_graph_begin', 'sw_graph_end', 'sw_graph_launch', 'sw_graph_destroy',
             'sw_adjoint_hour', 'sw_day_adjoint', 'sw_cylindric_wedge', 'sw_wave_delay', 'sw_kup', 'sw_lside',
             'sw_surface_temperature', 'sw_assemble_tmrt', 'sw_gvf', 'sw_pack_visibility', 'sw_longwave',

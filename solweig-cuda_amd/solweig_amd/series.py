@@ -228,7 +228,7 @@ def run_timeseries(building_dsm_path, tree_path, dem_path, met_file, output_path
                     if engine is not None:
                         engine.close_table_workers()
                     engine = batched.Engine([dom], n_hours=n, extra_outputs=bool(extras),
-                                            keep_outputs=not no_write or return_arrays)
+                                            keep_outputs=not no_write or return_arrays, pipelined=True)
                     pending = {}
                     workers = engine.start_table_workers(table_workers) if prefetch else 0
                     if save_svf and not no_write and 'SVF' not in files['static']:
@@ -237,12 +237,15 @@ def run_timeseries(building_dsm_path, tree_path, dem_path, met_file, output_path
                         files['static']['SVF'] = _write_static(out / 'static', 'SVF', svfbuveg, 'SVF', '1')
                     writer = None if no_write else DayOutputs(engine, slots=max(1, int(output_queue_size)))
                 # the tables of the next days are built on the workers while this one runs
-                for j in range(i, min(len(days), i + 1 + (workers if prefetch else 0))):
+                for j in range(i, min(len(days), i + 2 + (workers if prefetch else 0))):
                     if j not in pending and days[j].values.shape[0] == n:
                         pending[j] = engine.tables_async(met=[days[j].values])
-                engine.load(pending.pop(i).result())
-                engine.reset_state()
-                engine.forward_day()
+                # consecutive days of one engine are pipelined: the next day's shadow pass runs
+                # beside this day's pixel pass (Engine.day_start / day_next)
+                if not engine.day_open:
+                    engine.day_start(pending.pop(i).result())
+                nxt = pending.pop(i + 1).result() if i + 1 in pending else None
+                engine.day_next(nxt)
                 if writer is not None:
                     writer.submit(day.date, write)
                     continue

@@ -111,19 +111,19 @@ def tile_jobs(case, settings, tiles_dir):
 # tan(svfalfa) f64), neighbour record, mask byte and lag state 33, static sky sums 40, march canopy
 # plane 4, ground-view planes 80 (step counts, unshadowed albedo sums, in-raster step counts).
 STATIC_BYTES_PER_PIXEL = 278
-# Default pixel budget of one batch: at 24 h and 2 output slots about 107 GB of device memory and
+# Default pixel budget of one batch: at 24 h and 2 output slots about 129 GB of device memory and
 # 38 GB of page-locked host memory (batch_bytes).
 DEFAULT_BATCH_PIXELS = 100_000_000
 
 
 def batch_bytes(pixels, n_hours=24, slots=2):
     """(device bytes, page-locked host bytes) that a batch of ``pixels`` input pixels holds while
-    it runs: the static state, the shadow pass's per-hour records and mask bytes (9 B per hour),
-    the day's Tmrt and UTCI (4 B each per hour) and, per output slot, one device staging copy and
-    one page-locked host copy of them (output.DayOutputs). The sky-view pass adds 128 B per pixel
-    of the domain being prepared while it runs."""
+    it runs: the static state, the shadow pass's per-hour records and mask bytes (9 B per hour, in
+    the two sets of the pipelined days), the day's Tmrt and UTCI (4 B each per hour) and, per output
+    slot, one device staging copy and one page-locked host copy of them (output.DayOutputs). The
+    sky-view pass adds 128 B per pixel of the domain being prepared while it runs."""
     per_day = 8 * int(n_hours)
-    device = int(pixels) * (STATIC_BYTES_PER_PIXEL + 9 * int(n_hours) + per_day * (1 + int(slots)))
+    device = int(pixels) * (STATIC_BYTES_PER_PIXEL + 2 * 9 * int(n_hours) + per_day * (1 + int(slots)))
     host = int(pixels) * per_day * int(slots)
     return device, host
 
@@ -185,19 +185,27 @@ def run_tiles(tiles, met_file, output_path, longitude, latitude, *, utc_offset_h
         writes = [writer_for(t) for t in group]
         engine, outs = None, []
         try:
-            for day in forcing.days:
+            days = forcing.days
+
+            def tables_of(engine, day):
+                sun = met_sun_device(day.values, doms[0].location, utc)
+                return engine.tables(met=[day.values] * len(group), sun=[sun] * len(group))
+
+            for i, day in enumerate(days):
                 n = day.values.shape[0]
                 if progress:
                     print(f'[{datetime.now():%H:%M:%S}] {day.date}: {len(group)} tiles, {n} steps', flush=True)
                 if engine is None or engine.n_hours != n:
                     for o in outs:
                         o.close()
-                    engine = batched.Engine(doms, n_hours=n)
+                    engine = batched.Engine(doms, n_hours=n, pipelined=True)
                     outs = [DayOutputs(engine, k, slots=max(1, int(output_queue_size))) for k in range(len(group))]
-                sun = met_sun_device(day.values, doms[0].location, utc)
-                engine.load(engine.tables(met=[day.values] * len(group), sun=[sun] * len(group)))
-                engine.reset_state()
-                engine.forward_day()
+                # consecutive days of one engine are pipelined: the next day's shadow pass runs
+                # beside this day's pixel pass (Engine.day_start / day_next)
+                if not engine.day_open:
+                    engine.day_start(tables_of(engine, day))
+                nxt = days[i + 1] if i + 1 < len(days) and days[i + 1].values.shape[0] == n else None
+                engine.day_next(None if nxt is None else tables_of(engine, nxt))
                 for o, w in zip(outs, writes):
                     o.submit(day.date, w)
         finally:

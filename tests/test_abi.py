@@ -141,3 +141,16 @@ def test_compiled_spa_terms_match_the_tables():
     spec.loader.exec_module(gen)
     with open(os.path.join(csrc, 'spa_terms.inc')) as fh:
         assert fh.read() == gen.render()
+
+
+@pytest.mark.parametrize('groups', [0, 1, 9])
+def test_capped_pixel_pass_refuses_caps_out_of_range(groups):
+    """sw_day_pixels_capped checks its cap on the host before any device call (no GPU needed)."""
+    if not os.path.exists(rt.LIBRARY):
+        pytest.skip('library not built')
+    L = rt.lib()
+    dummy = ctypes.create_string_buffer(64)
+    a = ctypes.addressof(dummy)
+    rc = L.sw_day_pixels_capped(a, a, 1, 1, 0, 24, a, a, 1, 22, groups, None)
+    assert rc == -1                                            # SW_ERR_ARGUMENT
+    assert b'sw_day_pixels_capped: bad arguments' in L.sw_last_error()

@@ -183,3 +183,35 @@ def test_resident_day_equals_hourly_launches(rt, tile):
     for key, got in runs.items():
         for k, ((t, u), (t0, u0)) in enumerate(zip(got, ref)):
             assert np.array_equal(t, t0) and np.array_equal(u, u0, equal_nan=True), (key, k)
+
+
+@pytest.mark.parametrize('use_graph', [False, True])
+def test_pipelined_days_equal_single_days(rt, use_graph):
+    """Engine.run_days (each day's all-hours shadow pass on a side stream beside the previous day's
+    pixel pass, days alternating between two table and record sets) gives every day the bits of the
+    same day run on its own, over uneven domains; a second run on the same engine, in another day
+    order, starts again from the first set."""
+    from solweig_amd import autotune, batched, synthetic
+    doms = [synthetic.make_c1(seed=70 + i, size=n, days=4) for i, n in enumerate((48, 33, 61))]
+    alone = batched.Engine(doms, n_hours=24)
+    tabs = [alone.tables(day_index=day) for day in range(4)]
+    ref = []
+    for day in range(4):
+        alone.load(tabs[day])
+        alone.reset_state()
+        alone.forward_day()
+        ref.append([alone.outputs(k) for k in range(len(doms))])
+    eng = batched.Engine(doms, n_hours=24, pipelined=True)
+    for order in ([0, 1, 2, 3], [3, 1, 2]):
+        print(f'pipelined days {order}, graph {use_graph}', flush=True)
+        got = []
+        n = eng.run_days((tabs[day] for day in order), use_graph=use_graph,
+                         after_day=lambda i: got.append([eng.outputs(k) for k in range(len(doms))]))
+        assert n == len(order) and not eng.day_open
+        for day, outs in zip(order, got):
+            for k, ((t, u), (t0, u0)) in enumerate(zip(outs, ref[day])):
+                assert np.array_equal(t, t0) and np.array_equal(u, u0, equal_nan=True), (order, day, k)
+    # the launch-form tuner runs all three forms and leaves one set
+    res = autotune.tune_launch_form(eng, tabs, days=3, rounds=1, use_graph=use_graph)
+    assert res['form'] in autotune.FORMS and set(res['ms']) == set(autotune.FORMS)
+    assert eng.pipelined == (res['form'] == 'pipelined')
