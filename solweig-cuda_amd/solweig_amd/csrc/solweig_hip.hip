@@ -413,7 +413,10 @@ __device__ __forceinline__ PlaneRsrc plane_rsrc(const AS1 float *plane, long lon
     return __builtin_amdgcn_make_buffer_rsrc((void *)(uintptr_t)plane, (short)0, (int)(n * 4), 0x00020000);
 }
 
-template <bool MASKED, bool TOPS>
+// SKIP_BLOCKED: the caller needs no canopy terms (topmax, the pergola test) of a lane that is
+// building-shadowed when the chunk starts, nor of a lane past the raster: veg is false for good
+// there, ever keeps its value, and topmax enters the outputs only through veg.
+template <bool MASKED, bool TOPS, bool SKIP_BLOCKED>
 __device__ __forceinline__ void march_chunk(March &m, PlaneRsrc dsm, PlaneRsrc trees,
                                             double raise, double me_a, const PixelAt &at, int rows,
                                             int cols, const AS4 int32_t *off, const AS4 double *dzs,
@@ -447,11 +450,15 @@ __device__ __forceinline__ void march_chunk(March &m, PlaneRsrc dsm, PlaneRsrc t
     }
     if (TOPS) {
         // no canopy top among the chunk's samples of any lane: top and bot are -inf, so topmax
-        // keeps its value and the pergola test fails at every step; only f, veg and ever move
-        bool canopy = false;
+        // keeps its value and the pergola test fails at every step; only f, veg and ever move.
+        // The march plane holds -inf or a top height > 0, so as signed integers the bit patterns
+        // order -inf (0xff800000, negative) below every height: the chunk has a top exactly when
+        // the largest pattern is not -inf's (7 integer maxima instead of a flag per sample).
+        int tmax = __float_as_int(tv[0]);
 #pragma unroll
-        for (int i = 0; i < CHUNK; ++i) canopy = canopy || tv[i] != -__builtin_huge_valf();
-        if (__ballot(canopy) == 0) {
+        for (int i = 1; i < CHUNK; ++i) tmax = max(tmax, __float_as_int(tv[i]));
+        const bool open = !SKIP_BLOCKED || (at.valid && !(m.f > me_a));
+        if (__ballot(open && tmax != (int)0xff800000u) == 0) {
 #pragma unroll
             for (int i = 0; i < CHUNK; ++i) {
                 const bool live = !MASKED || (in[i] && k0 + i < L);
@@ -575,13 +582,13 @@ __global__ __launch_bounds__(BLOCK) void k_hour_shadow(const sw_domain *__restri
         if (__ballot(in_first) == 0) break;          // every lane's ray has left the raster
         const bool full = k0 + CHUNK <= L && __ballot(in_last) == active;
         if (full && tops)
-            march_chunk<false, true>(m, rdsm, mtrees, raise, me.a, at, rows, cols, off, dzs, k0, L);
+            march_chunk<false, true, !FULL>(m, rdsm, mtrees, raise, me.a, at, rows, cols, off, dzs, k0, L);
         else if (full)
-            march_chunk<false, false>(m, rdsm, mtrees, raise, me.a, at, rows, cols, off, dzs, k0, L);
+            march_chunk<false, false, !FULL>(m, rdsm, mtrees, raise, me.a, at, rows, cols, off, dzs, k0, L);
         else if (tops)
-            march_chunk<true, true>(m, rdsm, mtrees, raise, me.a, at, rows, cols, off, dzs, k0, L);
+            march_chunk<true, true, !FULL>(m, rdsm, mtrees, raise, me.a, at, rows, cols, off, dzs, k0, L);
         else
-            march_chunk<true, false>(m, rdsm, mtrees, raise, me.a, at, rows, cols, off, dzs, k0, L);
+            march_chunk<true, false, !FULL>(m, rdsm, mtrees, raise, me.a, at, rows, cols, off, dzs, k0, L);
     }
     if (!valid) return;
     const double f = m.f, topmax = m.topmax;
@@ -611,7 +618,10 @@ __global__ __launch_bounds__(BLOCK) void k_hour_shadow(const sw_domain *__restri
     const double vegsh = 1.0 - vegv;
     const double shadow = sh - (1.0 - vegsh) * (1.0 - H[SWH_PSI]);
     const bool ground = G.ground[p] != 0;
-    const bool sunwall = (wallsun / wall * (ground ? 1.0 : 0.0)) == 1.0;
+    // (wallsun / walls * buildings) == 1 (umep_solweig.py:606): with wall > 0 a correctly rounded
+    // quotient is 1 exactly when wallsun == wall; with wall <= 0, wallsun is 0 or -0 and the
+    // quotient is -0 or NaN
+    const bool sunwall = ground && wall > 0.0 && wallsun == wall;
     // Surface temperature of the hour (umep_solweig.py:144-170) and the sunonsurface Lup field.
     const double amp = (double)G.tgk[p] * H[SWH_ALTMAX] + (double)G.tstart[p];
     double tg = amp * tg_sine(H, G.tmaxlst[p]);
@@ -731,13 +741,13 @@ __global__ __launch_bounds__(BLOCK) void k_svf_march(const float *__restrict__ d
             if (__ballot(in_first) == 0) break;              // every lane's ray has left the raster
             const bool full = k0 + CHUNK <= L && __ballot(in_last) == active;
             if (full && tops)
-                march_chunk<false, true>(m, rdsm, mtrees, raise, me.a, at, rows, cols, off, dzs, k0, L);
+                march_chunk<false, true, true>(m, rdsm, mtrees, raise, me.a, at, rows, cols, off, dzs, k0, L);
             else if (full)
-                march_chunk<false, false>(m, rdsm, mtrees, raise, me.a, at, rows, cols, off, dzs, k0, L);
+                march_chunk<false, false, true>(m, rdsm, mtrees, raise, me.a, at, rows, cols, off, dzs, k0, L);
             else if (tops)
-                march_chunk<true, true>(m, rdsm, mtrees, raise, me.a, at, rows, cols, off, dzs, k0, L);
+                march_chunk<true, true, true>(m, rdsm, mtrees, raise, me.a, at, rows, cols, off, dzs, k0, L);
             else
-                march_chunk<true, false>(m, rdsm, mtrees, raise, me.a, at, rows, cols, off, dzs, k0, L);
+                march_chunk<true, false, true>(m, rdsm, mtrees, raise, me.a, at, rows, cols, off, dzs, k0, L);
         }
         const uint32_t bit = 1u << (k - kb);
         const bool sh = m.f > me.a;
