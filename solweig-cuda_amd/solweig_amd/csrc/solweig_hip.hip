@@ -451,14 +451,14 @@ __device__ __forceinline__ void march_chunk(March &m, PlaneRsrc dsm, PlaneRsrc t
     if (TOPS) {
         // no canopy top among the chunk's samples of any lane: top and bot are -inf, so topmax
         // keeps its value and the pergola test fails at every step; only f, veg and ever move.
-        // The march plane holds -inf or a top height > 0, so as signed integers the bit patterns
-        // order -inf (0xff800000, negative) below every height: the chunk has a top exactly when
-        // the largest pattern is not -inf's (7 integer maxima instead of a flag per sample).
-        int tmax = __float_as_int(tv[0]);
+        // The march plane holds -inf or a top height > 0: the chunk has a top exactly when the
+        // largest sample is not -inf (a chain of three-operand maxima instead of a flag per
+        // sample; the same test on the integer bit patterns holds 7 more registers).
+        float tmax = tv[0];
 #pragma unroll
-        for (int i = 1; i < CHUNK; ++i) tmax = max(tmax, __float_as_int(tv[i]));
+        for (int i = 1; i < CHUNK; ++i) tmax = __builtin_fmaxf(tmax, tv[i]);
         const bool open = !SKIP_BLOCKED || (at.valid && !(m.f > me_a));
-        if (__ballot(open && tmax != (int)0xff800000u) == 0) {
+        if (__ballot(open && tmax != -__builtin_huge_valf()) == 0) {
 #pragma unroll
             for (int i = 0; i < CHUNK; ++i) {
                 const bool live = !MASKED || (in[i] && k0 + i < L);
@@ -860,7 +860,9 @@ __device__ __forceinline__ void stage_tile(const DomainG &G, const PixelAt &at, 
 // the same for every direction: it is added once per half by finish().
 struct GvSums {
     float firstf, secondf, kF1, kF0, kN1, kN0;
-    float hB[4], hL[4], hA[4];     // wall weight, Lup and albedo x shadow sums of E, S, W, N
+    // wall weight, Lup and albedo x shadow sums of the quadrants NE (azimuth < 90), SE, SW and NW;
+    // each half is the sum of its two quadrants (E = NE + SE, S = SE + SW, W = SW + NW, N = NW + NE)
+    float hB[4], hL[4], hA[4];
 
     __device__ __forceinline__ GvSums(int first, int second) {
         firstf = (float)first;
@@ -876,10 +878,11 @@ struct GvSums {
     }
 
     __device__ __forceinline__ void add_to(int dir, float B, float tl, float ta) {
-        const int az = 5 + 20 * dir;
-        const int ew = az < 180 ? 0 : 2, sn = (az >= 90 && az < 270) ? 1 : 3;
-        hB[ew] += B; hL[ew] += tl; hA[ew] += ta;
-        hB[sn] += B; hL[sn] += tl; hA[sn] += ta;
+        const int q = (5 + 20 * dir) / 90;
+        // a constant quadrant in the unrolled walks; the rolled walks select (no indexed registers)
+#pragma unroll
+        for (int g = 0; g < 4; ++g)
+            if (g == q) { hB[g] += B; hL[g] += tl; hA[g] += ta; }
     }
 
     // far field of one direction: lup and albsh are the sums over its L alive steps, kw the
@@ -910,6 +913,14 @@ struct GvSums {
         const float albedo_b = (float)H[SWH_ALBEDO_B];
         const float local_lup = me.x * (1.0f - building);
         const float local_albsh = fabsf(me.y) * (1.0f - building);
+        // the halves E, S, W, N from the quadrants
+        float B[4], Lh[4], Ah[4];
+#pragma unroll
+        for (int g = 0; g < 4; ++g) {
+            B[g] = hB[g] + hB[(g + 1) & 3];
+            Lh[g] = hL[g] + hL[(g + 1) & 3];
+            Ah[g] = hA[g] + hA[(g + 1) & 3];
+        }
         if (near_uniform) {
             // receiver on the ground (L >= 1) and a sunlit wall (its record's sign bit): kw == 0
             const bool f_sunin = building != 0.0f && (__float_as_uint(me.y) >> 31) != 0u;
@@ -918,17 +929,17 @@ struct GvSums {
             const float nL = 9.0f * (kN * (building * me.x));
             const float nA = 9.0f * (kN * (building * fabsf(me.y)));
 #pragma unroll
-            for (int g = 0; g < 4; ++g) { hB[g] += nB; hL[g] += nL; hA[g] += nA; }
+            for (int g = 0; g < 4; ++g) { B[g] += nB; Lh[g] += nL; Ah[g] += nA; }
         }
         GvfOut o;
         Triple *grp[4] = {&o.e, &o.s, &o.w, &o.n};
 #pragma unroll
         for (int g = 0; g < 4; ++g) {
-            grp[g]->lup = fmaf(lwall, hB[g], hL[g]) + 9.0f * local_lup;
-            grp[g]->alb = fmaf(albedo_b, hB[g], hA[g]) + 9.0f * local_albsh;
+            grp[g]->lup = fmaf(lwall, B[g], Lh[g]) + 9.0f * local_lup;
+            grp[g]->alb = fmaf(albedo_b, B[g], Ah[g]) + 9.0f * local_albsh;
         }
-        o.all.lup = fmaf(lwall, hB[0] + hB[2], hL[0] + hL[2]) + 18.0f * local_lup;
-        o.all.alb = fmaf(albedo_b, hB[0] + hB[2], hA[0] + hA[2]) + 18.0f * local_albsh;
+        o.all.lup = fmaf(lwall, B[0] + B[2], Lh[0] + Lh[2]) + 18.0f * local_lup;
+        o.all.alb = fmaf(albedo_b, B[0] + B[2], Ah[0] + Ah[2]) + 18.0f * local_albsh;
         const AS1 float *ns = G.gvf_nosh;
         const long long P = G.pixels, p = at.p;
         o.all.nosh = fmaf(albedo_b, ns[5 * P + p], ns[p]);
@@ -1347,7 +1358,6 @@ __device__ __forceinline__ void day_patch_sums_tab(const AS1 uint32_t *vis, long
                                                    float &a_swsun, float &a_swsh, float &a_lsunc,
                                                    float &a_lsuns, float &a_lshc, float &a_lshs) {
     const AS3 uint32_t *T = (const AS3 uint32_t *)(AS3 void *)gv_lds;
-    const AS3 v2f *nib = (const AS3 v2f *)(T + SWS_NIB);
     const AS3 double *ys = (const AS3 double *)(T + SWS_YS);
     const AS3 uint32_t *mk = T + SWS_MASK;
     const AS3 uint32_t *el = T + SWS_ELIG;
@@ -1356,6 +1366,12 @@ __device__ __forceinline__ void day_patch_sums_tab(const AS1 uint32_t *vis, long
     // ``tb`` (always 0) orders each word's table reads after the previous word's sums: left free,
     // the compiler issues all 78 reads at once (over 200 VGPRs)
     int tb = 0;
+    // the nibble tables through one base register: a read is then one bit-field extract (kept as
+    // such: folded into the address, the extract becomes a shift and a mask) and one shift-add,
+    // with the table's position as the read's immediate offset
+    uint32_t nib_base = (uint32_t)(uintptr_t)(T + SWS_NIB);
+    asm volatile("" : "+v"(nib_base));
+    const AS3 v2f *nib = (const AS3 v2f *)(AS3 void *)(uintptr_t)nib_base;
 #pragma unroll
     for (int w = 0; w < SW_VIS_WORDS; ++w) {
         const uint32_t vsh = vis[(0 * SW_VIS_WORDS + w) * P + p];
@@ -1367,8 +1383,11 @@ __device__ __forceinline__ void day_patch_sums_tab(const AS1 uint32_t *vis, long
         for (int q = 0; q < 8; ++q) {
             const int i = 8 * w + q;
             if (i >= 39) break;
-            const v2f t1 = nib[tb + i * 16 + (int)((sky >> (4 * q)) & 15u)];
-            const v2f t2 = nib[tb + i * 16 + (int)((vegonly >> (4 * q)) & 15u)];
+            uint32_t n1, n2;
+            asm("v_bfe_u32 %0, %1, %2, 4" : "=v"(n1) : "v"(sky), "s"(4 * q));
+            asm("v_bfe_u32 %0, %1, %2, 4" : "=v"(n2) : "v"(vegonly), "s"(4 * q));
+            const v2f t1 = nib[tb + i * 16 + (int)n1];
+            const v2f t2 = nib[tb + i * 16 + (int)n2];
             sl += t1.x; sk += t1.y; vl += t2.x; vk += t2.y;
         }
         asm volatile("" : "+v"(tb), "+v"(sl), "+v"(sk), "+v"(vl), "+v"(vk));
