@@ -8,7 +8,9 @@ in which each tile runs through the hours with its lag state in registers) or as
 one pixel launch per hour (sw_hours_forward), each launch covering the tiles of all domains
 together. Lag state is cleared at each day start (series.py:425-436). Because buffer addresses
 never change, a day's launches can be captured once as a hipGraph and replayed for every further
-day after the tables are refreshed.
+day after the tables are refreshed. Consecutive days can run pipelined (run_days, day_start /
+day_next): each day's all-hours shadow pass on a side stream beside the previous day's pixel pass,
+the days alternating between two table and record sets.
 """
 from __future__ import annotations
 
@@ -191,7 +193,12 @@ class Engine:
         """Queue the upload of one day's tables on the engine's stream (fixed device addresses, so
         a captured graph stays valid). The copies run after every launch queued before them, so
         the host may build and load the next day while the device still runs this one.
-        ``set_index`` 1: into the second table set of the pipelined days (day_start / day_next)."""
+        ``set_index`` 1: into the second table set of the pipelined days (day_start / day_next).
+        Refused while a run of pipelined days is open: its days own both sets."""
+        self._refuse_open()
+        self._load(tables, set_index)
+
+    def _load(self, tables, set_index=0):
         bufs = self.buf if set_index == 0 else self._set1
         for t, b in zip(tables, bufs):
             if t.n_hours != self.n_hours:
@@ -227,7 +234,12 @@ class Engine:
         if self.buf[0]['rec'].shape[0] != self.n_hours:
             raise RuntimeError('this engine was built with resident=False: its record planes hold one hour')
 
+    def _refuse_open(self):
+        if self.day_open:
+            raise RuntimeError('a run of pipelined days is open (day_start without its last day_next)')
+
     def _launch_hours(self, h0, h1):
+        self._refuse_open()
         if self.resident:
             self._check_resident()
         rt.call('sw_day_forward' if self.resident else 'sw_hours_forward', self.dom_dev.ptr, self.block_start.ptr,
@@ -320,10 +332,12 @@ class Engine:
 
     def day_start(self, tables):
         """First day of a run of pipelined days (resident launch form): load its tables and enqueue
-        its shadow pass. Then call ``day_next`` once per day."""
+        its shadow pass. Then call ``day_next`` once per day. While a run is open (``day_open``) the
+        engine's table sets belong to it: ``load``, ``forward_day`` and ``run_hours`` would
+        overwrite the tables or records of a day still in flight."""
         self._ensure_pipeline()
         self._pipe_set = 0
-        self.load(tables, 0)
+        self._load(tables, 0)
         self._shadow_pass(0, self.stream)
 
     def day_next(self, next_tables=None, use_graph=False):
@@ -336,7 +350,7 @@ class Engine:
             raise RuntimeError('day_next without day_start')
         k, more = self._pipe_set, next_tables is not None
         if more:
-            self.load(next_tables, 1 - k)
+            self._load(next_tables, 1 - k)
         self.reset_state()
         if use_graph:
             self._capture_pipeline()
@@ -373,6 +387,7 @@ class Engine:
 
     def hour(self, h, full=False):
         """Both passes of one hour (for step-wise use and tests)."""
+        self._refuse_open()
         rt.call('sw_hour_shadow', self.dom_dev.ptr, self.block_start.ptr, len(self.domains),
                 self.total_blocks, h, int(full), self.stream.handle)
         rt.call('sw_hour_pixels', self.dom_dev.ptr, self.block_start.ptr, len(self.domains),

@@ -211,6 +211,14 @@ def test_pipelined_days_equal_single_days(rt, use_graph):
         for day, outs in zip(order, got):
             for k, ((t, u), (t0, u0)) in enumerate(zip(outs, ref[day])):
                 assert np.array_equal(t, t0) and np.array_equal(u, u0, equal_nan=True), (order, day, k)
+    # while a run is open its days own both table sets: loads and single-day launches are refused
+    eng.day_start(tabs[0])
+    with pytest.raises(RuntimeError, match='pipelined days is open'):
+        eng.load(tabs[1])
+    with pytest.raises(RuntimeError, match='pipelined days is open'):
+        eng.forward_day()
+    eng.day_next(None)
+    assert not eng.day_open
     # the launch-form tuner runs all three forms and leaves one set
     res = autotune.tune_launch_form(eng, tabs, days=3, rounds=1, use_graph=use_graph)
     assert res['form'] in autotune.FORMS and set(res['ms']) == set(autotune.FORMS)
