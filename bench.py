@@ -1,7 +1,8 @@
 """Benchmark: Tmrt+UTCI pixel-hours/s of the batched day forward on one MI355X.
 
 A step is one day (24 hourly steps) of the batched forward over the 2000x2000 c2 workload
-(seeded synthetic morphology, its 30-day forcing cycled): per step the host builds the day's
+(seeded synthetic morphology, its 30-day forcing cycled; the default 30 steps time the whole
+720-hour month, each day once): per step the host builds the day's
 tables (clearness, Perez, patch rows, rays; the solar position comes from the device) in CPU
 worker processes, the engine uploads them, and the device replays the day's launches as captured
 hipGraphs, in the launch form that a timed comparison before the warm-up picks:
@@ -423,7 +424,7 @@ def e2e_leg(repeats=3):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
-    ap.add_argument('--steps', type=int, default=10)
+    ap.add_argument('--steps', type=int, default=30, help='timed days (30: the whole 720-hour c2 month)')
     ap.add_argument('--warmup', type=int, default=1)
     ap.add_argument('--size', type=int, default=2000)
     ap.add_argument('--eager', action='store_true', help='headline with eager launches instead of the hipGraph')
