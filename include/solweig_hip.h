@@ -75,11 +75,17 @@ enum { SW_PATCHES = 153, SW_VIS_WORDS = 5 };
 /* Per hour row, the 32-bit words of the patch search tables (sw_domain.ptab_s):
  *   SWS_NIB   float [39][16][2]: over the 4 patches 4i..4i+3, the sums of (luminance,
  *             lumChi cos omega) of the patches whose bit is set in the 4-bit index;
- *   SWS_YS    double [8][32]: per altitude band its patches' yi in ascending order, +inf after;
+ *   SWS_CSUN  double [8][32]: per altitude band, in ascending yi order of its patches, the
+ *             breakpoint c_j = the smallest double x >= 0 with x + yi_j (FP64) >= the band's
+ *             sunlit threshold: the patch is sunlit from a pixel exactly when tan(asvf) < c_j
+ *             (0 after the band's last patch);
+ *   SWS_CSH   double [8][32]: the same for the band's shaded threshold (shaded iff
+ *             tan(asvf) >= c_j);
  *   SWS_MASK  uint32 [8][33]: per band, the band-local bits (bit i: patch BAND_START + i) of the
  *             j smallest-yi patches, j = 0..32;
  *   SWS_ELIG  uint32 [8]: per band, the band-local sun-test eligibility bits.              */
-enum { SWS_NIB = 0, SWS_YS = 1248, SWS_MASK = 1760, SWS_ELIG = 2024, SW_SEARCH_WORDS = 2048 };
+enum { SWS_NIB = 0, SWS_CSUN = 1248, SWS_CSH = 1760, SWS_MASK = 2272, SWS_ELIG = 2536,
+       SW_SEARCH_WORDS = 2560 };
 
 /* FULL-mode output planes of sw_hour_pixels (the per-step 39-tuple's per-pixel members) */
 enum {

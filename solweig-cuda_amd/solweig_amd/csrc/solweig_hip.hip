@@ -1351,14 +1351,15 @@ __device__ __forceinline__ void stage_search(const AS1 uint32_t *src, int tid) {
 // day_patch_sums with the hour's search tables in LDS (stage_search): the diffuse sums are 8
 // nibble-table reads per visibility word and mask, and per altitude band the sunlit and the
 // shaded wall patches are the patches below a bisected position in the band's yi order (the
-// exact FP64 tests tan(asvf) + yi < threshold, evaluated at the bisection probes only).
+// exact FP64 tests tan(asvf) + yi < threshold, as comparisons of tan(asvf) with the tables'
+// breakpoints, evaluated at the bisection probes only).
 __device__ __forceinline__ void day_patch_sums_tab(const AS1 uint32_t *vis, long long P, long long p,
                                                    const AS4 float *pf, const AS4 double *pd,
                                                    double tan_asvf, float dveg, float &a_lv, float &a_kd,
                                                    float &a_swsun, float &a_swsh, float &a_lsunc,
                                                    float &a_lsuns, float &a_lshc, float &a_lshs) {
     const AS3 uint32_t *T = (const AS3 uint32_t *)(AS3 void *)gv_lds;
-    const AS3 double *ys = (const AS3 double *)(T + SWS_YS);
+    const AS3 double *cs = (const AS3 double *)(T + SWS_CSUN), *ch = (const AS3 double *)(T + SWS_CSH);
     const AS3 uint32_t *mk = T + SWS_MASK;
     const AS3 uint32_t *el = T + SWS_ELIG;
     float sl = 0.f, sk = 0.f, vl = 0.f, vk = 0.f;
@@ -1401,15 +1402,13 @@ __device__ __forceinline__ void day_patch_sums_tab(const AS1 uint32_t *vis, long
         const uint32_t full = n == 32 ? 0xffffffffu : ((1u << n) - 1u);
         const uint32_t bm = (sh + n <= 32 ? (bld[wi] >> sh) : __builtin_amdgcn_alignbit(bld[wi + 1], bld[wi], sh)) & full;
         if (__ballot(bm != 0u) == 0) continue;       // no building patch of the band in this wave
-        const double t_sun = pd[SWP_SUN_BELOW * SW_PATCHES + lo];
-        const double t_sh = pd[SWP_SHADE_FROM * SW_PATCHES + lo];
+        // tan(asvf) + yi_j < threshold exactly when tan(asvf) < c_j (the tables' breakpoints, non-
+        // increasing in j): j counts the band's sunlit (not shaded) patches in yi order
         int js = tb, jh = tb;
 #pragma unroll
         for (int s = 1 << (BAND_STEPS[b] - 1); s > 0; s >>= 1) {
-            const double ya = ys[b * 32 + js + s - 1];
-            const double yb = ys[b * 32 + jh + s - 1];
-            js += (tan_asvf + ya < t_sun) ? s : 0;
-            jh += (tan_asvf + yb < t_sh) ? s : 0;
+            js += (tan_asvf < cs[b * 32 + js + s - 1]) ? s : 0;
+            jh += (tan_asvf < ch[b * 32 + jh + s - 1]) ? s : 0;
         }
         const uint32_t sun = mk[b * 33 + js], shd = full & ~mk[b * 33 + jh], elig = el[b];
         const float wc = pf[lo * SW_PATCH_F + SWP_WCOS];

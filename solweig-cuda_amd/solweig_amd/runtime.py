@@ -20,7 +20,7 @@ SW_PATCH_D = 4
 SW_PATCHES = 153
 SW_VIS_WORDS = 5
 SW_MAX_PIXELS = (1 << 30) - 1   # largest raster of one domain (32-bit byte offsets of the marches)
-SWS_NIB, SWS_YS, SWS_MASK, SWS_ELIG, SW_SEARCH_WORDS = 0, 1248, 1760, 2024, 2048
+SWS_NIB, SWS_CSUN, SWS_CSH, SWS_MASK, SWS_ELIG, SW_SEARCH_WORDS = 0, 1248, 1760, 2272, 2536, 2560
 TILE_X, TILE_Y = 32, 8          # default workgroup tile (256 threads)
 TILE_CHOICES = (16, 32, 64, 128)  # tile widths the hour kernels accept (height = 256 / width)
 

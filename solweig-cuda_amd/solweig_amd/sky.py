@@ -15,7 +15,7 @@ from functools import lru_cache
 import numpy as np
 
 from .runtime import (H, PD, PF, SW_HOUR_STRIDE, SW_PATCH_D, SW_PATCH_F, SW_PATCHES, SW_VIS_WORDS,
-                      SWS_ELIG, SWS_MASK, SWS_NIB, SWS_YS, SW_SEARCH_WORDS)
+                      SWS_CSH, SWS_CSUN, SWS_ELIG, SWS_MASK, SWS_NIB, SW_SEARCH_WORDS)
 from . import utci as _utci
 
 SBC = 5.67051e-8
@@ -426,6 +426,30 @@ def patch_rows(day, esky, Ta, altitude, azimuth, lv=None, radD=0.0, f64=False):
 
 
 _NIB_BITS = ((np.arange(16)[:, None] >> np.arange(4)[None, :]) & 1).astype(np.float64)   # [16][4]
+_INF_BITS = int(np.float64(np.inf).view(np.int64))
+
+
+def breakpoints(y, t):
+    """Elementwise the smallest double c >= 0 with c + y >= t in FP64 (+inf when no finite c
+    reaches t): for every x >= 0, x + y < t exactly when x < c, since rounding is monotonic. Found
+    by bisection over the bit patterns of x >= 0 (which order as the values do), evaluating the
+    sum as the device does, so that the device's wall-patch test tan(asvf) + yi < threshold
+    becomes tan(asvf) < c."""
+    y, t = np.broadcast_arrays(np.asarray(y, dtype=np.float64), np.asarray(t, dtype=np.float64))
+
+    def reached(bits):
+        return (bits.view(np.float64) + y) >= t
+
+    lo = np.zeros(y.shape, np.int64)                      # +0.0: not reached (else c = 0)
+    hi = np.full(y.shape, _INF_BITS, np.int64)            # +inf: reached (else c = +inf)
+    at0, atinf = reached(lo), reached(hi)
+    for _ in range(64):
+        mid = lo + (hi - lo) // 2
+        r = reached(mid)
+        hi = np.where(r, mid, hi)
+        lo = np.where(r, lo, mid)
+    c = hi.view(np.float64)
+    return np.where(at0, 0.0, np.where(atinf, c, np.inf))
 _BAND_BITS = np.left_shift(np.uint32(1), np.arange(32, dtype=np.uint32))
 
 
@@ -436,9 +460,12 @@ def search_row(f, d):
     * nibble sums: for the 4 patches 4i..4i+3 and every 4-bit subset, the FP64 sums (rounded
       once to FP32) of luminance and lumChi cos omega, so that a visibility word's diffuse sums
       are 8 table reads;
-    * per altitude band the patches' yi in ascending order and the band-local bit masks of the
-      j smallest: tan(asvf) + yi is non-decreasing in yi (rounding is monotonic), so the patches
-      of a band below each exact threshold are the first j of that order, j found by bisection;
+    * per altitude band, in the ascending yi order of its patches, the breakpoints of the sunlit
+      and the shaded test (``breakpoints``: tan(asvf) + yi < threshold exactly when
+      tan(asvf) < c) and the band-local bit masks of the j first patches of that order:
+      tan(asvf) + yi is non-decreasing in yi (rounding is monotonic), so the patches of a band
+      below each exact threshold are the first j of the order, j found by bisection over the
+      non-increasing breakpoints;
     * per band the eligibility bits of the sunlit-wall test."""
     out = np.zeros(SW_SEARCH_WORDS, np.uint32)
     w = np.zeros((156, 2))
@@ -447,6 +474,8 @@ def search_row(f, d):
     nib = np.einsum('ej,ijc->iec', _NIB_BITS, w.reshape(39, 4, 2)).astype(np.float32)
     out[SWS_NIB:SWS_NIB + 39 * 16 * 2] = nib.reshape(-1).view(np.uint32)
     ys = np.full((8, 32), np.inf)
+    t_sun = np.zeros((8, 1))
+    t_sh = np.zeros((8, 1))
     masks = np.zeros((8, 33), np.uint32)
     elig = np.zeros(8, np.uint32)
     yi_all = np.asarray(d[:, PD['YI']], dtype=np.float64)
@@ -456,12 +485,15 @@ def search_row(f, d):
         yi = yi_all[lo:lo + n]
         order = np.argsort(yi, kind='stable')
         ys[b, :n] = yi[order]
+        t_sun[b] = d[lo, PD['SUN_BELOW']]
+        t_sh[b] = d[lo, PD['SHADE_FROM']]
         bits = np.left_shift(np.uint32(1), order.astype(np.uint32))
         masks[b, 1:n + 1] = np.bitwise_or.accumulate(bits)
         masks[b, n + 1:] = masks[b, n]
         elig[b] = np.bitwise_or.reduce(np.where(el_all[lo:lo + n], _BAND_BITS[:n], np.uint32(0)))
         lo += n
-    out[SWS_YS:SWS_YS + 512] = ys.reshape(-1).view(np.uint32)
+    out[SWS_CSUN:SWS_CSUN + 512] = breakpoints(ys, t_sun).reshape(-1).view(np.uint32)
+    out[SWS_CSH:SWS_CSH + 512] = breakpoints(ys, t_sh).reshape(-1).view(np.uint32)
     out[SWS_MASK:SWS_MASK + 264] = masks.reshape(-1)
     out[SWS_ELIG:SWS_ELIG + 8] = elig
     return out

@@ -203,3 +203,53 @@ def test_hour_branches_follow_the_piecewise_edges():
     low = sky.hour_branches(alt, azi, zen, jday, 20.0, 50.0, 100.0, P, lat)
     high = sky.hour_branches(alt, azi, zen, jday, 20.0, 50.0, 1200.0, P, lat)
     assert low[1] is False and high[1] is True                         # CI clamped at high radG
+
+
+def test_wall_test_breakpoints_reproduce_the_fp64_sum():
+    """sky.breakpoints: x + y < t in FP64 exactly when x < c, for x at, just below and just above
+    c and at random, over yi values, +inf padding and the band thresholds."""
+    thr = np.array([t for alt in (6.0, 18.0, 42.0, 78.0) for t in sky.sunlit_thresholds(alt)])
+    rng = np.random.default_rng(7)
+    y = np.concatenate([-rng.exponential(2.0, 40), [0.0, -0.0, -1e-300, np.inf]])
+    for t in thr:
+        c = sky.breakpoints(y, t)
+        xs = np.concatenate([np.abs(rng.normal(0, 3, 2000)), [0.0, 1e300]])
+        for x in xs:
+            np.testing.assert_array_equal((x + y) < t, x < c)
+        for xc in (c, np.nextafter(c, -np.inf), np.nextafter(c, np.inf)):
+            ok = np.isfinite(xc) & (xc >= 0)
+            np.testing.assert_array_equal(((xc + y) < t)[ok], (xc < c)[ok])
+
+
+@pytest.mark.parametrize('altitude,azimuth', [(3.5, 70.0), (21.0, 135.0), (48.0, 200.0), (61.0, 300.0)])
+def test_search_tables_give_the_direct_wall_classification(altitude, azimuth):
+    """The device's bisection over the breakpoint tables (search_row) finds, per altitude band and
+    pixel, the patches with tan(asvf) + yi < sunlit threshold and >= shaded threshold, as the
+    direct per-patch FP64 test does."""
+    from solweig_amd.runtime import SWS_CSH, SWS_CSUN, SWS_MASK
+    f, d = sky.patch_rows(1, 0.8, 20.0, altitude, azimuth, lv=np.full(153, 1 / 153), radD=100.0)
+    row = sky.search_row(f, d)
+    cs = row[SWS_CSUN:SWS_CSUN + 512].view(np.float64).reshape(8, 32)
+    ch = row[SWS_CSH:SWS_CSH + 512].view(np.float64).reshape(8, 32)
+    mk = row[SWS_MASK:SWS_MASK + 264].reshape(8, 33)
+    counts, steps = [31, 30, 28, 24, 19, 13, 7, 1], [5, 5, 5, 5, 5, 4, 3, 1]
+    yi = d[:, PD['YI']]
+    rng = np.random.default_rng(int(altitude * 10))
+    xs = np.concatenate([np.abs(rng.normal(0, 2, 300)), cs[np.isfinite(cs)], ch[np.isfinite(ch)]])
+    lo = 0
+    for b, n in enumerate(counts):
+        t_sun, t_sh = d[lo, PD['SUN_BELOW']], d[lo, PD['SHADE_FROM']]
+        bits = np.left_shift(np.uint32(1), np.arange(n, dtype=np.uint32))
+        full = np.uint32(0xffffffff) if n == 32 else np.uint32((1 << n) - 1)
+        for x in xs:
+            js = jh = 0
+            s = 1 << (steps[b] - 1)
+            while s > 0:
+                js += s if x < cs[b, js + s - 1] else 0
+                jh += s if x < ch[b, jh + s - 1] else 0
+                s >>= 1
+            sun = np.bitwise_or.reduce(np.where(x + yi[lo:lo + n] < t_sun, bits, np.uint32(0)))
+            shade = np.bitwise_or.reduce(np.where(x + yi[lo:lo + n] >= t_sh, bits, np.uint32(0)))
+            assert mk[b, js] == sun, (b, x)
+            assert (full & ~mk[b, jh]) == shade, (b, x)
+        lo += n
