@@ -453,7 +453,7 @@ def breakpoints(y, t):
 _BAND_BITS = np.left_shift(np.uint32(1), np.arange(32, dtype=np.uint32))
 
 
-def search_row(f, d):
+def search_row(f, d, _defer=False):
     """The patch search tables of one hour (SWS_* words of sw_domain.ptab_s) from its float
     patch rows ``f`` [153][SW_PATCH_F] and double rows ``d`` [153][SW_PATCH_D]:
 
@@ -492,11 +492,24 @@ def search_row(f, d):
         masks[b, n + 1:] = masks[b, n]
         elig[b] = np.bitwise_or.reduce(np.where(el_all[lo:lo + n], _BAND_BITS[:n], np.uint32(0)))
         lo += n
-    out[SWS_CSUN:SWS_CSUN + 512] = breakpoints(ys, t_sun).reshape(-1).view(np.uint32)
-    out[SWS_CSH:SWS_CSH + 512] = breakpoints(ys, t_sh).reshape(-1).view(np.uint32)
     out[SWS_MASK:SWS_MASK + 264] = masks.reshape(-1)
     out[SWS_ELIG:SWS_ELIG + 8] = elig
-    return out
+    if _defer:                       # build_day fills the breakpoints of all hours at once
+        return out, (ys, t_sun, t_sh)
+    return _with_breakpoints(out[None], [(ys, t_sun, t_sh)])[0]
+
+
+def _with_breakpoints(rows, parts):
+    """Fill the breakpoint tables of search rows [k][SW_SEARCH_WORDS] from their (sorted yi,
+    sunlit and shaded thresholds), in one vectorised bisection for all rows."""
+    if not len(parts):
+        return rows
+    ys = np.stack([p[0] for p in parts])
+    cs = breakpoints(np.stack([ys, ys]), np.stack([np.stack([p[1] for p in parts]),
+                                                   np.stack([p[2] for p in parts])]))
+    rows[:, SWS_CSUN:SWS_CSUN + 512] = cs[0].reshape(len(parts), -1).view(np.uint32)
+    rows[:, SWS_CSH:SWS_CSH + 512] = cs[1].reshape(len(parts), -1).view(np.uint32)
+    return rows
 
 
 # ----------------------------------------------------------------------------- a day
@@ -632,6 +645,7 @@ def build_day(met, sun, latitude, shape, scale, amaxvalue, *, landcover=0, wall_
     pd = np.zeros((n, SW_PATCH_D, SW_PATCHES))
     ps = np.zeros((n, SW_SEARCH_WORDS), np.uint32)
     offs, dzs, meta = [], [], []
+    day_rows, parts = [], []
     start = 0
     for i in range(n):
         if dectime[i] - np.floor(dectime[i]) == 0:
@@ -645,7 +659,9 @@ def build_day(met, sun, latitude, shape, scale, amaxvalue, *, landcover=0, wall_
         m['state_in'] = st_in
         pd[i] = d_i.T
         if h[H['DAY']]:
-            ps[i] = search_row(pf[i], d_i)
+            ps[i], part = search_row(pf[i], d_i, _defer=True)
+            day_rows.append(i)
+            parts.append(part)
         h[H['RAY_START']] = start
         h[H['PATCH_TABLE']] = i
         hours[i] = h
@@ -655,6 +671,8 @@ def build_day(met, sun, latitude, shape, scale, amaxvalue, *, landcover=0, wall_
             dzs.append(dz)
             start += len(dz)
         meta.append(m)
+    if day_rows:
+        ps[day_rows] = _with_breakpoints(ps[day_rows], parts)
     ray_off = np.concatenate(offs) if offs else np.zeros((1, 2), np.int32)
     ray_dz = np.concatenate(dzs) if dzs else np.zeros(1)
     t = DayTables(hours, pf, pd, ray_off, ray_dz, meta, ps)

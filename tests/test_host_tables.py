@@ -253,3 +253,15 @@ def test_search_tables_give_the_direct_wall_classification(altitude, azimuth):
             assert mk[b, js] == sun, (b, x)
             assert (full & ~mk[b, jh]) == shade, (b, x)
         lo += n
+
+
+def test_day_search_rows_equal_the_single_hour_rows():
+    """build_day fills the breakpoints of all daytime hours in one vectorised bisection; each row
+    equals the row search_row builds for its hour alone."""
+    d = synthetic.small_domain(seed=3, size=24)
+    s = psun.met_sun(d.met, d.location, d.utc)
+    tab = sky.build_day(d.met, s, d.latitude, d.dsm.shape, 1.0, 30.0)
+    day = np.flatnonzero(tab.hours[:, H['DAY']] != 0)
+    assert len(day) > 5
+    for i in day:
+        np.testing.assert_array_equal(tab.ptab_s[i], sky.search_row(tab.ptab_f[i], tab.ptab_d[i].T))
