@@ -515,6 +515,20 @@ def main():
     log(f'[bench] c2 {"graph" if use_graph else "eager"}: {dt / args.steps * 1e3:.2f} ms/day, {value:.3e} px-h/s')
     tmrt, utci = eng.outputs(0)
     log(f'[bench] Tmrt range {np.nanmin(tmrt):.2f}..{np.nanmax(tmrt):.2f}, UTCI {np.nanmin(utci):.2f}..{np.nanmax(utci):.2f}')
+    # the last timed day again, alone (one resident day, no overlap): its outputs must be the bits
+    # the timed run left
+    last = (args.warmup + args.steps - 1) % days
+    keep = eng.resident, eng.pipelined
+    eng.resident, eng.pipelined = True, False
+    eng.load(eng.tables(day_index=last))
+    eng.reset_state()
+    eng.forward_day()
+    t1, u1 = eng.outputs(0)
+    eng.resident, eng.pipelined = keep
+    eng.graph = None
+    last_day_check = dict(day=int(last), identical_to_the_day_run_alone=bool(
+        np.array_equal(tmrt, t1, equal_nan=True) and np.array_equal(utci, u1, equal_nan=True)))
+    log(f'[bench] last timed day (day {last}) run alone: identical {last_day_check["identical_to_the_day_run_alone"]}')
 
     # eager leg: per-pass HIP events on the engine's stream (the kernels' launch durations)
     eager = roof = None
@@ -660,6 +674,7 @@ def main():
                                     f'{"hipGraph replay" if use_graph else "eager launches"} of '
                                     f'{FORM_TEXT[form_name]}',
                            graph=use_graph, launch_form=form if form is not None else form_name,
+                           last_day_check=last_day_check,
                            resident=resident, static_prep=static, tile=tuned,
                            eager=eager, other_launch_form=other,
                            plain_forward=plain,
