@@ -60,12 +60,12 @@ HBM_PEAK = 8.0e12      # MI355X HBM3E bytes/s
 # HBM bytes per pixel-pass launch at 2000x2000: rocprofv3 FETCH_SIZE scaled by the k_copy
 # calibration + WRITE_SIZE, from the profile named in TRAFFIC_SOURCE (profile values, not measured by
 # this run): k_day_pixels (one launch for 24 hours), k_hour_pixels (mean over a day's 24 launches).
-TRAFFIC_2000 = {'k_day_pixels': 25.55e9, 'k_hour_pixels': 0.822e9}
+TRAFFIC_2000 = {'k_day_pixels': 25.81e9, 'k_hour_pixels': 0.821e9}
 TRAFFIC_SOURCE = 'profiles/k_day_pixels_c2_2000x2000.md'
 # VALU instructions (wave64) per pixel-pass launch at 2000x2000, SQ_INSTS_VALU x SQ_WAVES from the same
 # profile, and the single-issue VALU rate of MI355X (256 CUs x 2.4 GHz, one wave64 instruction per CU
 # per cycle without the packed FP32 forms): the ceiling the pixel pass actually runs against.
-VALU_2000 = {'k_day_pixels': 54010 * 62500, 'k_hour_pixels': 2169 * 62500}
+VALU_2000 = {'k_day_pixels': 53301 * 62500, 'k_hour_pixels': 2141 * 62500}
 VALU_PEAK = 256 * 2.4e9
 FORM_TEXT = {
     'pipelined': 'resident days pipelined over two streams (each day\'s all-hours shadow pass beside the '
