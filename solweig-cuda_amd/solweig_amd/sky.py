@@ -431,14 +431,43 @@ _INF_BITS = int(np.float64(np.inf).view(np.int64))
 
 def breakpoints(y, t):
     """Elementwise the smallest double c >= 0 with c + y >= t in FP64 (+inf when no finite c
-    reaches t): for every x >= 0, x + y < t exactly when x < c, since rounding is monotonic. Found
-    by bisection over the bit patterns of x >= 0 (which order as the values do), evaluating the
-    sum as the device does, so that the device's wall-patch test tan(asvf) + yi < threshold
-    becomes tan(asvf) < c."""
+    reaches t): for every x >= 0, x + y < t exactly when x < c, since rounding is monotonic, so
+    that the device's wall-patch test tan(asvf) + yi < threshold becomes tan(asvf) < c. Searched
+    over the bit patterns of x >= 0 (which order as the values do), evaluating the sum as the device
+    does: from the estimate t - y a few neighbouring patterns settle almost every element; the rest
+    are bisected."""
     y, t = np.broadcast_arrays(np.asarray(y, dtype=np.float64), np.asarray(t, dtype=np.float64))
+    shape = y.shape
+    y, t = y.ravel(), t.ravel()
 
+    def reached(bits, yy, tt):
+        with np.errstate(invalid='ignore', over='ignore'):
+            return (bits.view(np.float64) + yy) >= tt
+
+    with np.errstate(invalid='ignore', over='ignore'):
+        est = t - y
+    est = np.clip(np.where(np.isnan(est), 0.0, est), 0.0, np.inf)
+    bits = est.view(np.int64).copy()
+    for _ in range(4):                                    # down while the pattern below reaches t
+        below = np.maximum(bits - 1, 0)
+        bits = np.where((bits > 0) & reached(below, y, t), below, bits)
+    for _ in range(4):                                    # up while this pattern does not
+        bits = np.where(~reached(bits, y, t) & (bits < _INF_BITS), bits + 1, bits)
+    hit = reached(bits, y, t)
+    exact = hit & ((bits == 0) | ~reached(np.maximum(bits - 1, 0), y, t))
+    never = (bits == _INF_BITS) & ~hit                    # no finite x reaches t
+    c = np.where(never, np.inf, bits.view(np.float64))
+    rest = ~(exact | never)
+    if rest.any():
+        c[rest] = _breakpoints_bisect(y[rest], t[rest])
+    return c.reshape(shape)
+
+
+def _breakpoints_bisect(y, t):
+    """breakpoints by bisection over all bit patterns of x >= 0 (64 steps)."""
     def reached(bits):
-        return (bits.view(np.float64) + y) >= t
+        with np.errstate(invalid='ignore', over='ignore'):
+            return (bits.view(np.float64) + y) >= t
 
     lo = np.zeros(y.shape, np.int64)                      # +0.0: not reached (else c = 0)
     hi = np.full(y.shape, _INF_BITS, np.int64)            # +inf: reached (else c = +inf)

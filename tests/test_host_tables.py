@@ -213,6 +213,7 @@ def test_wall_test_breakpoints_reproduce_the_fp64_sum():
     y = np.concatenate([-rng.exponential(2.0, 40), [0.0, -0.0, -1e-300, np.inf]])
     for t in thr:
         c = sky.breakpoints(y, t)
+        np.testing.assert_array_equal(c, sky._breakpoints_bisect(y, np.full_like(y, t)))
         xs = np.concatenate([np.abs(rng.normal(0, 3, 2000)), [0.0, 1e300]])
         for x in xs:
             np.testing.assert_array_equal((x + y) < t, x < c)
