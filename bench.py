@@ -57,6 +57,7 @@ B_SHADOW = 16.0
 B_PIXELS = 141.5
 B_ALG = B_SHADOW + B_PIXELS
 HBM_PEAK = 8.0e12      # MI355X HBM3E bytes/s
+DIFF_EVENT = 3 * 24    # per eager day: the events around the diffuse-sum launch (see run_days)
 # HBM bytes per pixel-pass launch at 2000x2000: rocprofv3 FETCH_SIZE scaled by the k_copy
 # calibration + WRITE_SIZE, from the profile named in TRAFFIC_SOURCE (profile values, not measured by
 # this run): k_day_pixels (one launch for 24 hours), k_hour_pixels (mean over a day's 24 launches).
@@ -263,14 +264,22 @@ def run_days(eng, pipe, n, use_graph, events=None):
         elif eng.resident:
             ev = events[k]
             args = (eng.dom_dev.ptr, eng.block_start.ptr, len(eng.domains), eng.total_blocks, 0, eng.n_hours)
+            # shadow pass (events 0 -> 1), diffuse sums (1 -> DIFF_EVENT), pixel pass (DIFF_EVENT -> 2)
             ev[0].record(eng.stream)
             rt.call('sw_day_shadow', *args, eng.stream.handle)
             ev[1].record(eng.stream)
+            eng.diffuse()
+            ev[DIFF_EVENT].record(eng.stream)
             rt.call('sw_day_pixels', *args, eng.gvf_off.ptr, eng.gvf_bounds.ptr, eng.first, eng.second,
                     eng.stream.handle)
             ev[2].record(eng.stream)
         else:
             ev = events[k]
+            # the diffuse sums of the day (events DIFF_EVENT -> DIFF_EVENT + 1), then per hour the
+            # shadow pass (3h -> 3h + 1) and the pixel pass (3h + 1 -> 3h + 2)
+            ev[DIFF_EVENT].record(eng.stream)
+            eng.diffuse()
+            ev[DIFF_EVENT + 1].record(eng.stream)
             for h in range(eng.n_hours):
                 ev[3 * h].record(eng.stream)
                 rt.call('sw_hour_shadow', eng.dom_dev.ptr, eng.block_start.ptr, len(eng.domains), eng.total_blocks,
@@ -533,7 +542,7 @@ def main():
     # eager leg: per-pass HIP events on the engine's stream (the kernels' launch durations)
     eager = roof = None
     if args.eager_days > 0:
-        evs = [[rt.Event() for _ in range(3 * 24 + 1)] for _ in range(args.eager_days)]
+        evs = [[rt.Event() for _ in range(DIFF_EVENT + 2)] for _ in range(args.eager_days)]
         t0 = time.perf_counter()
         etabs = run_days(eng, pipe, args.eager_days, False, events=evs)
         rt.synchronize()
@@ -542,22 +551,24 @@ def main():
         if eng.resident:
             # two launches per day: the shadow pass of all hours, the pixel pass of all hours
             t_sh = sum(evs[k][0].elapsed_ms(evs[k][1]) for k in range(args.eager_days))
-            t_px = sum(evs[k][1].elapsed_ms(evs[k][2]) for k in range(args.eager_days))
+            t_df = sum(evs[k][1].elapsed_ms(evs[k][DIFF_EVENT]) for k in range(args.eager_days))
+            t_px = sum(evs[k][DIFF_EVENT].elapsed_ms(evs[k][2]) for k in range(args.eager_days))
             launches, hours_per_launch = args.eager_days, 24
             names = ('k_hour_shadow (all hours)', 'k_day_pixels')
         else:
             t_sh = t_px = 0.0
+            t_df = sum(evs[k][DIFF_EVENT].elapsed_ms(evs[k][DIFF_EVENT + 1]) for k in range(args.eager_days))
             for k in range(args.eager_days):
                 for h in range(24):
                     t_sh += evs[k][3 * h].elapsed_ms(evs[k][3 * h + 1])
                     t_px += evs[k][3 * h + 1].elapsed_ms(evs[k][3 * h + 2])
             launches, hours_per_launch = 24 * args.eager_days, 1
             names = ('k_hour_shadow', 'k_hour_pixels')
-        sh_ms, px_ms = t_sh / launches, t_px / launches
+        sh_ms, px_ms, df_ms = t_sh / launches, t_px / launches, t_df / args.eager_days
         eager = dict(value=P * 24 * args.eager_days / edt, ms_per_day=edt / args.eager_days * 1e3,
-                     days=args.eager_days, device_ms_per_day=(t_sh + t_px) / args.eager_days,
-                     avg_ms={names[0]: sh_ms, names[1]: px_ms}, hours_per_launch=hours_per_launch,
-                     daytime_hours_per_day=n_day_hours / args.eager_days)
+                     days=args.eager_days, device_ms_per_day=(t_sh + t_df + t_px) / args.eager_days,
+                     avg_ms={names[0]: sh_ms, names[1]: px_ms, 'k_day_diffuse (all hours)': df_ms},
+                     hours_per_launch=hours_per_launch, daytime_hours_per_day=n_day_hours / args.eager_days)
         dom_ms, dom_name, dom_b = (px_ms, names[1], B_PIXELS) if px_ms >= sh_ms else (sh_ms, names[0], B_SHADOW)
         dom_bytes = dom_b * P * hours_per_launch
         achieved = dom_bytes / (dom_ms * 1e-3)
@@ -576,7 +587,7 @@ def main():
                     pair=dict(kernels=f'{names[0]} + {names[1]}', achieved=pair / 1e9, frac=pair / HBM_PEAK,
                               algorithmic_bytes=B_ALG * P * hours_per_launch, avg_ms=sh_ms + px_ms))
         log(f'[bench] eager: {eager["ms_per_day"]:.2f} ms/day; per launch shadow {sh_ms:.4f} ms, pixels {px_ms:.4f} ms '
-            f'({hours_per_launch} h per launch); {dom_name} {achieved / 1e9:.0f} GB/s = '
+            f'({hours_per_launch} h per launch), diffuse sums {df_ms:.4f} ms per day; {dom_name} {achieved / 1e9:.0f} GB/s = '
             f'{100 * achieved / HBM_PEAK:.1f} % of HBM')
     other = None
     if args.other_days > 0:

@@ -87,6 +87,27 @@ enum { SW_PATCHES = 153, SW_VIS_WORDS = 5 };
 enum { SWS_NIB = 0, SWS_CSUN = 1248, SWS_CSH = 1760, SWS_MASK = 2272, SWS_ELIG = 2536,
        SW_SEARCH_WORDS = 2560 };
 
+/* The day's diffuse table (sw_domain.diffuse_tab, read by sw_day_diffuse): per hour the patch
+ * luminance (Perez, umep_solweig.py:148-154) and lumChi cos(altitude) omega (the diffuse Kside
+ * term, shortwave.cu:155-157) as integer rows of a product with the pixels' visibility masks.
+ *   rows:    SW_DIFF_ROWS(n_hours), two per hour: 2h luminance, 2h + 1 lumChi cos omega;
+ *   columns: SW_DIFF_K patch positions: k < 160 patch k of the sky mask (vsh & vveg), 160 + k
+ *            patch k of the vegetation-only mask (vsh & ~vveg), its value times the hour's
+ *            diffsh of vegetation (SWH_DIFF_VEG in FP32, the product in FP64); positions
+ *            153..159 and 313..319 zero;
+ *   layout:  the int8 digits in the order the device's matrix operands take them,
+ *            [rows / 16][SW_DIFF_K / 64][SW_DIFF_DIGITS][64][16]: per block of 16 rows, step of
+ *            64 positions and digit, 64 fragments of 16 bytes, fragment c + 16 g holding row
+ *            16 block + c at positions 64 step + 16 g .. + 15; then double scale [rows].
+ * Row r holds integers q (|q| <= 2^27) with value = q * scale_r exactly, scale_r a power of two
+ * (the row's values rounded to 27 bits of its largest magnitude), as the digits
+ * q = d3 2^21 + d2 2^14 + d1 2^7 + d0, d0..d2 in [0, 127], d3 in [-64, 64]. A negative scale
+ * marks the rows of a night hour (and the rows past the day): no sums are written for them.  */
+enum { SW_DIFF_K = 320, SW_DIFF_DIGITS = 4 };
+#define SW_DIFF_ROWS(n_hours) (16 * (((n_hours) + 7) / 8))
+#define SW_DIFF_TABLE_BYTES(n_hours) \
+    ((size_t)SW_DIFF_ROWS(n_hours) * (SW_DIFF_DIGITS * SW_DIFF_K + sizeof(double)))
+
 /* FULL-mode output planes of sw_hour_pixels (the per-step 39-tuple's per-pixel members) */
 enum {
     SWF_KDOWN = 0, SWF_KUP, SWF_LDOWN, SWF_LUP, SWF_TG, SWF_KEAST, SWF_KSOUTH, SWF_KWEST,
@@ -150,6 +171,12 @@ typedef struct sw_domain {
     const uint32_t *gvf_edge; /* [5][P] or NULL: per direction (byte d % 4 of word d / 4) the
                                  number of ground-view steps whose shift stays inside the raster
                                  (static, sw_gvf_static); later steps repeat the last of them */
+    const void *diffuse_tab;  /* the day's diffuse table (SW_DIFF_K) or NULL                   */
+    float *diffuse;           /* [hours][P][2] or NULL: per pixel-hour the diffuse sums
+                                 (luminance, lumChi cos omega) over the pixel's sky patches plus
+                                 diffsh x those over its vegetation-only patches, written by
+                                 sw_day_diffuse; when set, the batched pixel passes read hour h's
+                                 sums from slot h instead of summing the patches              */
 } sw_domain;
 
 /* Static per-pixel sums over the patches of each visibility class (day patch geometry), the
@@ -267,6 +294,15 @@ int sw_day_pixels_capped(const sw_domain *domains_dev, const int32_t *block_star
                          int total_blocks, int hour_begin, int hour_end, const int32_t *gvf_offsets_dev,
                          const double *gvf_bounds_dev, int first_steps, int second_steps,
                          int groups_per_cu, void *stream);
+/* The diffuse patch sums of every daytime hour of a day, for every pixel of every domain whose
+ * record has diffuse_tab and diffuse set (others are skipped): the per-patch luminance loop of
+ * umep_solweig.py:148-154 / shortwave.cu:99-107 and the diffuse Kside term, shortwave.cu:155-157,
+ * as one launch of int8 matrix products (visibility bits x table digits, exact integer sums, one
+ * rounding to FP32 per sum). Run it after the day's table is in place and before the pixel
+ * passes (sw_hours_forward, sw_day_forward, sw_day_pixels, sw_hour_pixels) that read the sums;
+ * n_hours (1..65535) is the day's hour count the table was built for.                        */
+int sw_day_diffuse(const sw_domain *domains_dev, const int32_t *block_start_dev, int n_domains,
+                   int total_blocks, int n_hours, void *stream);
 
 /* ----- adjoint of J = mean over ground pixels of UTCI at one objective hour -------------------
  * (no counterpart in the reference: the differentiable form of the per-step chain

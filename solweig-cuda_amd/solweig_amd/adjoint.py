@@ -131,6 +131,7 @@ class UtciGradient:
         sun, tabs = self._tables()
         self.eng.load(tabs)
         self.eng.reset_state()
+        self.eng.diffuse()
         lag = self.pd.dev['lag']
         for h0, h1 in ((0, self.h_obj), (self.h_obj, self.h_obj + 1)):
             if h0 == self.h_obj:

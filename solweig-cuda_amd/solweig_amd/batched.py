@@ -59,6 +59,7 @@ class Engine:
                      ptab_f=rt.DeviceArray((self.n_hours, rt.SW_PATCHES, rt.SW_PATCH_F), np.float32),
                      ptab_d=rt.DeviceArray((self.n_hours, rt.SW_PATCH_D, rt.SW_PATCHES), np.float64),
                      ptab_s=rt.DeviceArray((self.n_hours, rt.SW_SEARCH_WORDS), np.uint32),
+                     diff_tab=rt.DeviceArray((rt.diff_table_bytes(self.n_hours),), np.uint8),
                      ray_off=rt.DeviceArray((cap, 2), np.int32), ray_dz=rt.DeviceArray((cap,), np.float64),
                      cap=cap)
             if keep_outputs:
@@ -70,6 +71,8 @@ class Engine:
             # hour; the single-hour launches use the first slot
             b['rec'] = rt.DeviceArray((self.n_hours if resident else 1, d.P, 2), np.float32)
             b['masks'] = rt.DeviceArray((self.n_hours if resident else 1, d.P), np.uint8)
+            # the day's diffuse sums of every pixel-hour (sw_day_diffuse), read by the pixel passes
+            b['diffuse'] = rt.DeviceArray((self.n_hours, d.P, 2), np.float32)
             self.buf.append(b)
         self.resident = bool(resident)
         self.pipelined = bool(pipelined) and self.resident
@@ -84,6 +87,7 @@ class Engine:
                              ptab_f=rt.PinnedArray(b['ptab_f'].shape, np.float32),
                              ptab_d=rt.PinnedArray(b['ptab_d'].shape, np.float64),
                              ptab_s=rt.PinnedArray(b['ptab_s'].shape, np.uint32),
+                             diff_tab=rt.PinnedArray(b['diff_tab'].shape, np.uint8),
                              ray_off=rt.PinnedArray((b['cap'], 2), np.int32),
                              ray_dz=rt.PinnedArray((b['cap'],), np.float64)) for b in self.buf]
                        for _ in range(2)]
@@ -98,7 +102,9 @@ class Engine:
         self._side = None
         self._pipe_set = None
         self._pipe_graphs = {}
-        self.groups_beside_shadow = 4
+        self.groups_beside_shadow = 3
+        # per table set: the diffuse sums (sw_day_diffuse) of the loaded day are enqueued
+        self._diffuse_ok = [False, False]
 
     def _write_domains(self):
         """Upload the domain records and the tile prefix (after a change of tile widths)."""
@@ -113,6 +119,7 @@ class Engine:
             for i, (d, b, o) in enumerate(zip(self.domains, bufs, self.buf)):
                 self.structs[i] = d.struct(hours=b['hours'].ptr, ptab_f=b['ptab_f'].ptr, ptab_d=b['ptab_d'].ptr,
                                            ptab_s=b['ptab_s'].ptr, ray_off=b['ray_off'].ptr, ray_dz=b['ray_dz'].ptr,
+                                           diffuse_tab=b['diff_tab'].ptr, diffuse=b['diffuse'].ptr,
                                            tmrt=o['tmrt'].ptr if 'tmrt' in o else 0,
                                            utci=o['utci'].ptr if 'utci' in o else 0,
                                            full=o['extra'].ptr if 'extra' in o else 0,
@@ -205,6 +212,7 @@ class Engine:
                 raise ValueError(f'expected {self.n_hours} hours, got {t.n_hours}')
             if t.ray_dz.shape[0] > b['cap']:
                 raise ValueError('ray tables exceed their buffer')
+        self._diffuse_ok[set_index] = False
         slot = self._slot
         self._slot ^= 1
         if self._stage_used[slot]:
@@ -216,9 +224,10 @@ class Engine:
             st['ptab_f'].array[...] = t.ptab_f
             st['ptab_d'].array[...] = t.ptab_d
             st['ptab_s'].array[...] = t.ptab_s
+            st['diff_tab'].array[...] = t.diffuse
             st['ray_off'].array[:n] = t.ray_off
             st['ray_dz'].array[:n] = t.ray_dz
-            for key in ('hours', 'ptab_f', 'ptab_d', 'ptab_s'):
+            for key in ('hours', 'ptab_f', 'ptab_d', 'ptab_s', 'diff_tab'):
                 rt.call('sw_memcpy_h2d', b[key].ptr, st[key].ptr, st[key].nbytes, h)
             rt.call('sw_memcpy_h2d', b['ray_off'].ptr, st['ray_off'].ptr, n * 8, h)
             rt.call('sw_memcpy_h2d', b['ray_dz'].ptr, st['ray_dz'].ptr, n * 8, h)
@@ -238,10 +247,18 @@ class Engine:
         if self.day_open:
             raise RuntimeError('a run of pipelined days is open (day_start without its last day_next)')
 
+    def diffuse(self, k=0, stream=None):
+        """Enqueue the diffuse sums of every pixel-hour of the day loaded in table set k
+        (sw_day_diffuse), which the pixel passes of that day read; ``stream``: the engine's."""
+        rt.call('sw_day_diffuse', self.dom_devs[k].ptr, self.block_start.ptr, len(self.domains), self.total_blocks,
+                self.n_hours, (stream or self.stream).handle)
+        self._diffuse_ok[k] = True
+
     def _launch_hours(self, h0, h1):
         self._refuse_open()
         if self.resident:
             self._check_resident()
+        self.diffuse()
         rt.call('sw_day_forward' if self.resident else 'sw_hours_forward', self.dom_dev.ptr, self.block_start.ptr,
                 len(self.domains), self.total_blocks, h0, h1, self.gvf_off.ptr, self.gvf_bounds.ptr, self.first,
                 self.second, self.stream.handle)
@@ -277,10 +294,12 @@ class Engine:
                                    ptab_f=rt.DeviceArray(b['ptab_f'].shape, np.float32),
                                    ptab_d=rt.DeviceArray(b['ptab_d'].shape, np.float64),
                                    ptab_s=rt.DeviceArray(b['ptab_s'].shape, np.uint32),
+                                   diff_tab=rt.DeviceArray(b['diff_tab'].shape, np.uint8),
                                    ray_off=rt.DeviceArray((b['cap'], 2), np.int32),
                                    ray_dz=rt.DeviceArray((b['cap'],), np.float64), cap=b['cap'],
                                    rec=rt.DeviceArray((self.n_hours, d.P, 2), np.float32),
-                                   masks=rt.DeviceArray((self.n_hours, d.P), np.uint8)))
+                                   masks=rt.DeviceArray((self.n_hours, d.P), np.uint8),
+                                   diffuse=rt.DeviceArray((self.n_hours, d.P, 2), np.float32)))
         self.dom_devs = [self.dom_dev, rt.DeviceArray((ctypes.sizeof(self.structs),), np.uint8)]
         self._side = rt.Stream()
         self._fork, self._join = rt.Event(), rt.Event()
@@ -290,7 +309,9 @@ class Engine:
         return (self.dom_devs[k].ptr, self.block_start.ptr, len(self.domains), self.total_blocks, 0, self.n_hours)
 
     def _shadow_pass(self, k, stream):
+        """The all-hours shadow pass of set k and the diffuse sums of its day."""
         rt.call('sw_day_shadow', *self._args(k), stream.handle)
+        self.diffuse(k, stream)
 
     def _pixel_pass(self, k, stream, beside_shadow=False):
         """Pixel pass of set k; ``beside_shadow``: a shadow pass runs on the side stream meanwhile,
@@ -388,6 +409,8 @@ class Engine:
     def hour(self, h, full=False):
         """Both passes of one hour (for step-wise use and tests)."""
         self._refuse_open()
+        if not full and not self._diffuse_ok[0]:
+            self.diffuse()                               # once per loaded day
         rt.call('sw_hour_shadow', self.dom_dev.ptr, self.block_start.ptr, len(self.domains),
                 self.total_blocks, h, int(full), self.stream.handle)
         rt.call('sw_hour_pixels', self.dom_dev.ptr, self.block_start.ptr, len(self.domains),

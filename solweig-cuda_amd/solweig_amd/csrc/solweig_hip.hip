@@ -23,7 +23,7 @@
 
 #include "../../../include/solweig_hip.h"
 
-static_assert(sizeof(sw_domain) == 320, "sw_domain must stay 40 eight-byte fields");
+static_assert(sizeof(sw_domain) == 336, "sw_domain must stay 42 eight-byte fields");
 
 namespace {
 
@@ -106,6 +106,8 @@ struct DomainG {
     const AS1 float *march_trees;
     const AS1 uint32_t *ptab_s;
     const AS1 uint32_t *gvf_edge;
+    const AS1 int8_t *diffuse_tab;
+    AS1 v2f *diffuse;
 };
 
 __device__ __forceinline__ DomainG domain_at(const sw_domain *D) {
@@ -134,6 +136,8 @@ __device__ __forceinline__ DomainG domain_at(const sw_domain *D) {
     g.march_trees = (const AS1 float *)d->march_trees;
     g.ptab_s = (const AS1 uint32_t *)d->ptab_s;
     g.gvf_edge = (const AS1 uint32_t *)d->gvf_edge;
+    g.diffuse_tab = (const AS1 int8_t *)d->diffuse_tab;
+    g.diffuse = (AS1 v2f *)d->diffuse;
     return g;
 }
 
@@ -1278,6 +1282,8 @@ constexpr int BAND_START[9] = {0, 31, 61, 89, 113, 132, 145, 152, 153};
 // diffsh 1 for sky, 1 - (1 - 0.03f) for vegetation-only and 0 for building patches. Building
 // patches are split into sunlit and shaded walls by the exact thresholds of the hour's table;
 // since the trig weights are per band, those sums are band counts times the band weight.
+// DIFF false: the diffuse sums come from sw_day_diffuse; only the wall patches are summed.
+template <bool DIFF>
 __device__ __forceinline__ void day_patch_sums(const AS1 uint32_t *vis, long long P, long long p,
                                                const AS4 float *pf, const AS4 double *pd,
                                                const AS4 double *H, double tan_asvf, float dveg, float &a_lv, float &a_kd,
@@ -1293,7 +1299,7 @@ __device__ __forceinline__ void day_patch_sums(const AS1 uint32_t *vis, long lon
         // rolled loops: this path (no search tables) is the ABI's fallback; unrolled, its per-bit
         // lane masks fill the scalar registers of every kernel that inlines it
 #pragma nounroll
-        for (int j = 0; j < n; ++j) {
+        for (int j = 0; j < (DIFF ? n : 0); ++j) {
             const AS4 float *c = pf + (k0 + j) * SW_PATCH_F;
             const float ds = ((sky >> j) & 1u) ? 1.0f : (((vegonly >> j) & 1u) ? dveg : 0.0f);
             a_lv = fmaf(ds, c[SWP_LV], a_lv);
@@ -1338,13 +1344,14 @@ __device__ __forceinline__ void day_patch_sums(const AS1 uint32_t *vis, long lon
 constexpr int BAND_STEPS[8] = {5, 5, 5, 5, 5, 4, 3, 1};
 
 
-// Copy one hour row of the patch search tables into LDS (every thread of the workgroup calls;
-// the caller has synchronised the workgroup since the LDS was last read).
+// Copy one hour row of the patch search tables, from word ``from`` (a multiple of 4) on, into
+// LDS at the same word positions (every thread of the workgroup calls; the caller has
+// synchronised the workgroup since the LDS was last read).
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
-__device__ __forceinline__ void stage_search(const AS1 uint32_t *src, int tid) {
+__device__ __forceinline__ void stage_search(const AS1 uint32_t *src, int tid, int from) {
     AS3 u32x4 *dst = (AS3 u32x4 *)(AS3 void *)gv_lds;
     const AS1 u32x4 *s4 = (const AS1 u32x4 *)src;
-    for (int i = tid; i < SW_SEARCH_WORDS / 4; i += BLOCK) dst[i] = s4[i];
+    for (int i = tid + from / 4; i < SW_SEARCH_WORDS / 4; i += BLOCK) dst[i] = s4[i];
     __syncthreads();
 }
 
@@ -1352,7 +1359,9 @@ __device__ __forceinline__ void stage_search(const AS1 uint32_t *src, int tid) {
 // nibble-table reads per visibility word and mask, and per altitude band the sunlit and the
 // shaded wall patches are the patches below a bisected position in the band's yi order (the
 // exact FP64 tests tan(asvf) + yi < threshold, as comparisons of tan(asvf) with the tables'
-// breakpoints, evaluated at the bisection probes only).
+// breakpoints, evaluated at the bisection probes only). DIFF false: the diffuse sums come from
+// sw_day_diffuse and the nibble tables are neither staged nor read.
+template <bool DIFF>
 __device__ __forceinline__ void day_patch_sums_tab(const AS1 uint32_t *vis, long long P, long long p,
                                                    const AS4 float *pf, const AS4 double *pd,
                                                    double tan_asvf, float dveg, float &a_lv, float &a_kd,
@@ -1376,10 +1385,11 @@ __device__ __forceinline__ void day_patch_sums_tab(const AS1 uint32_t *vis, long
 #pragma unroll
     for (int w = 0; w < SW_VIS_WORDS; ++w) {
         const uint32_t vsh = vis[(0 * SW_VIS_WORDS + w) * P + p];
-        const uint32_t vveg = vis[(1 * SW_VIS_WORDS + w) * P + p];
         const uint32_t vvb = vis[(2 * SW_VIS_WORDS + w) * P + p];
-        const uint32_t sky = vsh & vveg, vegonly = vsh & ~vveg;
         bld[w] = ~vsh & vvb;
+        if (!DIFF) continue;
+        const uint32_t vveg = vis[(1 * SW_VIS_WORDS + w) * P + p];
+        const uint32_t sky = vsh & vveg, vegonly = vsh & ~vveg;
 #pragma unroll
         for (int q = 0; q < 8; ++q) {
             const int i = 8 * w + q;
@@ -1393,8 +1403,10 @@ __device__ __forceinline__ void day_patch_sums_tab(const AS1 uint32_t *vis, long
         }
         asm volatile("" : "+v"(tb), "+v"(sl), "+v"(sk), "+v"(vl), "+v"(vk));
     }
-    a_lv = fmaf(dveg, vl, sl);
-    a_kd = fmaf(dveg, vk, sk);
+    if (DIFF) {
+        a_lv = fmaf(dveg, vl, sl);
+        a_kd = fmaf(dveg, vk, sk);
+    }
 #pragma unroll
     for (int b = 0; b < 8; ++b) {
         const int lo = BAND_START[b], n = BAND_START[b + 1] - BAND_START[b];
@@ -1539,14 +1551,32 @@ __device__ __forceinline__ void pixel_hour(const DomainG &G, const PixelAt &at, 
             a_lshs = ss[SWS_BLD_SIN * P + p];
         }
     }
+    // the diffuse sums of the hour from sw_day_diffuse (a domain either has them or not)
+    const bool pre = !FULL && day && G.diffuse != nullptr;
+    if (pre) {
+        const v2f dd = G.diffuse[(long long)hour * P + p];
+        a_lv = dd.x;
+        a_kd = dd.y;
+    }
     if (!FULL && day && G.ptab_s) {
         __syncthreads();                              // every wave's walk is done with the LDS
-        stage_search(G.ptab_s + (long long)row * SW_SEARCH_WORDS, at.tid);
-        day_patch_sums_tab(vis, P, p, pf, pd, tan_asvf, dveg, a_lv, a_kd, a_swsun, a_swsh, a_lsunc,
-                           a_lsuns, a_lshc, a_lshs);
+        const AS1 uint32_t *srow = G.ptab_s + (long long)row * SW_SEARCH_WORDS;
+        if (pre) {
+            stage_search(srow, at.tid, SWS_CSUN);
+            day_patch_sums_tab<false>(vis, P, p, pf, pd, tan_asvf, dveg, a_lv, a_kd, a_swsun, a_swsh,
+                                      a_lsunc, a_lsuns, a_lshc, a_lshs);
+        } else {
+            stage_search(srow, at.tid, 0);
+            day_patch_sums_tab<true>(vis, P, p, pf, pd, tan_asvf, dveg, a_lv, a_kd, a_swsun, a_swsh,
+                                     a_lsunc, a_lsuns, a_lshc, a_lshs);
+        }
     } else if (!FULL && day) {
-        day_patch_sums(vis, P, p, pf, pd, H, tan_asvf, dveg, a_lv, a_kd, a_swsun, a_swsh, a_lsunc,
-                       a_lsuns, a_lshc, a_lshs);
+        if (pre)
+            day_patch_sums<false>(vis, P, p, pf, pd, H, tan_asvf, dveg, a_lv, a_kd, a_swsun, a_swsh, a_lsunc,
+                                  a_lsuns, a_lshc, a_lshs);
+        else
+            day_patch_sums<true>(vis, P, p, pf, pd, H, tan_asvf, dveg, a_lv, a_kd, a_swsun, a_swsh, a_lsunc,
+                                 a_lsuns, a_lshc, a_lshs);
     }
     if (FULL) {
         float lv_sky = 0.f, lv_veg = 0.f, kd_sky = 0.f, kd_veg = 0.f;
@@ -1746,6 +1776,142 @@ __global__ __launch_bounds__(BLOCK) void k_day_pixels(const sw_domain *__restric
     PixelAt at = locate(doms, bs, nd, total, G);
     if (clamp_to_raster(G, at))
         for (int k = 0; k < 5; ++k) G.lag[k * G.pixels + at.p] = st[k];
+}
+
+// =====================================================================================
+// Diffuse patch sums of a day on the matrix cores (sw_day_diffuse). Per pixel and hour the
+// reference sums diffsh_k x luminance_k and diffsh_k x lumChi_k cos(alt_k) omega_k over the 153
+// patches (umep_solweig.py:148-154, shortwave.cu:99-107 and 155-157), diffsh 1 on sky patches,
+// the hour's vegetation diffsh on vegetation-only patches and 0 on building patches: over a day,
+// the product of the [rows][320] table of the hour values (two rows per hour) with the
+// [320][pixels] matrix of the pixels' sky | vegetation-only bits. The table is integer (int8
+// digits of 27-bit fixed-point values with a power-of-two scale per row), so the int32 sums of
+// v_mfma_i32_16x16x64_i8 are exact and each result is rounded once, to FP32: the sums do not
+// depend on the launch geometry.
+// Mapping: a workgroup takes the pixel passes' tile; wave w takes the pixels of the tile's lanes
+// 64w .. 64w + 63 as four 16-pixel column blocks (block t: lanes 64w + 16t + c, c = 0..15).
+// Lane c + 16g holds as the A operand row c of an 8-hour block of the table at positions
+// 64s + 16g .. 64s + 16g + 15 of step s (s = 0..4; the table stores these fragments in lane
+// order, so a wave's load is 1 KB in one piece), as the B operand the same positions of column
+// c's pixel, and receives rows 4g .. 4g + 3 of that pixel (the 16 x 16 result layout).
+// =====================================================================================
+typedef int v4i __attribute__((ext_vector_type(4)));
+
+// 16 bits as 16 int8 values 0 / 1, bit j in byte j: a nibble n becomes the word
+// n * 0x204081 & 0x01010101 (its shifted copies n, n << 7, n << 14, n << 21 do not overlap)
+__device__ __forceinline__ v4i bits_to_bytes(uint32_t b16) {
+    v4i v;
+    v.x = (int)(__umul24(b16 & 0xfu, 0x204081u) & 0x01010101u);
+    v.y = (int)(__umul24((b16 >> 4) & 0xfu, 0x204081u) & 0x01010101u);
+    v.z = (int)(__umul24((b16 >> 8) & 0xfu, 0x204081u) & 0x01010101u);
+    v.w = (int)(__umul24((b16 >> 12) & 0xfu, 0x204081u) & 0x01010101u);
+    return v;
+}
+
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(3))) void k_day_diffuse(
+    const sw_domain *__restrict__ doms, const int32_t *__restrict__ bs, int nd, int total, int n_hours) {
+    DomainG G;
+    const PixelAt at = locate(doms, bs, nd, total, G);
+    if (G.diffuse_tab == nullptr || G.diffuse == nullptr) return;   // one domain per workgroup
+    const int lane = (int)threadIdx.x & 63, wave = (int)threadIdx.x >> 6;
+    const int c = lane & 15, g = lane >> 4;
+    const int rows = (int)G.rows, cols = (int)G.cols, sh = G.tile_shift;
+    const long long P = G.pixels;
+    const int nrows = SW_DIFF_ROWS(n_hours);
+    // per block t and step s this lane's 16 bits: word 2s + g / 2 of the pixel's sky (words
+    // 0..4) then vegetation-only (5..9) masks, its half g % 2; steps 2j and 2j + 1 share a word
+    uint32_t part[4][3];
+    // block t's pixel of this lane (recomputed where it is stored: kept, the four 64-bit indices
+    // would cost the registers of a fourth wave per SIMD)
+    auto pixel_of = [&](int t, bool &in) {
+        const int l = 64 * wave + 16 * t + c;
+        const int r = at.tr0 + (l >> sh), cc = at.tc0 + (l & ((1 << sh) - 1));
+        in = r < rows && cc < cols;
+        return in ? (long long)r * cols + cc : 0LL;       // pixel 0 stands in outside the raster
+    };
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+        bool in;
+        const long long q = pixel_of(t, in);
+        const AS1 uint32_t *vq = G.vis + q;
+#pragma unroll
+        for (int s = 0; s < 5; ++s) {
+            const int w = 2 * s + (g >> 1), wi = w < 5 ? w : w - 5;
+            const uint32_t vsh = vq[wi * P], vveg = vq[(SW_VIS_WORDS + wi) * P];
+            const uint32_t word = !in ? 0u : (w < 5 ? (vsh & vveg) : (vsh & ~vveg));
+            const uint32_t half = (word >> (16 * (g & 1))) & 0xffffu;
+            if (s % 2 == 0) part[t][s / 2] = half;
+            else part[t][s / 2] |= half << 16;
+        }
+    }
+    // the table's fragments: per 8-hour block, step and digit the 64 lanes' 16 bytes in lane
+    // order, addressed as the table (uniform) plus a 32-bit byte offset
+    const AS1 int8_t *frag = G.diffuse_tab;
+    const AS1 double *scl = (const AS1 double *)(G.diffuse_tab + (long long)SW_DIFF_DIGITS * nrows * SW_DIFF_K);
+#pragma nounroll
+    for (int mt = 0; mt < nrows / 16; ++mt) {
+        // opaque per block: the byte expansions are then formed step by step, not hoisted out of
+        // the loop for every block (80 more registers)
+#pragma unroll
+        for (int t = 0; t < 4; ++t) asm volatile("" : "+v"(part[t][0]), "+v"(part[t][1]), "+v"(part[t][2]));
+        v4i acc[4][SW_DIFF_DIGITS];
+#pragma unroll
+        for (int t = 0; t < 4; ++t)
+#pragma unroll
+            for (int d = 0; d < SW_DIFF_DIGITS; ++d) acc[t][d] = v4i{0, 0, 0, 0};
+#pragma unroll
+        for (int s = 0; s < 5; ++s) {
+            // the step's fragment index made opaque: its loads are issued in step order, not all
+            // 20 at the start of the block
+            uint32_t fo = (uint32_t)(((mt * 5 + s) * SW_DIFF_DIGITS * 64 + lane) * 16);
+            asm volatile("" : "+v"(fo));
+            v4i a[SW_DIFF_DIGITS];
+#pragma unroll
+            for (int d = 0; d < SW_DIFF_DIGITS; ++d) a[d] = *(const AS1 v4i *)(frag + (fo + (uint32_t)(d * 1024)));
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                const v4i b = bits_to_bytes((part[t][s / 2] >> (16 * (s % 2))) & 0xffffu);
+#pragma unroll
+                for (int d = 0; d < SW_DIFF_DIGITS; ++d)
+                    acc[t][d] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a[d], b, acc[t][d], 0, 0, 0);
+            }
+        }
+        // rows 16 mt + 4 g + i: hour h = 8 mt + 2 g (i = 0, 1) and h + 1 (i = 2, 3). The integer
+        // sum is hi 2^14 + lo with hi = acc3 2^7 + acc2 and lo = acc1 2^7 + acc0, both exact in
+        // int32 and in FP32 (below 2^23): one fma rounds it once, and the power-of-two scale is
+        // exact (a NaN scale gives NaN, as the reference's 0 x NaN terms do)
+        const int r0 = 16 * mt + 4 * g, h = 8 * mt + 2 * g;
+        float sc[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) sc[i] = (float)scl[r0 + i];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            float v[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int hi = acc[t][3][i] * 128 + acc[t][2][i], lo = acc[t][1][i] * 128 + acc[t][0][i];
+                v[i] = fmaf((float)hi, 16384.0f, (float)lo) * sc[i];
+            }
+            // this block's values are final here: without the barrier all 64 sums are converted
+            // at once
+            asm volatile("" : "+v"(v[0]), "+v"(v[1]), "+v"(v[2]), "+v"(v[3]));
+            bool in;
+            const long long q = pixel_of(t, in);
+            if (!in) continue;
+            if (h < n_hours && !(sc[0] < 0.0f)) {
+                v2f o;
+                o.x = v[0];
+                o.y = v[1];
+                G.diffuse[(long long)h * P + q] = o;
+            }
+            if (h + 1 < n_hours && !(sc[2] < 0.0f)) {
+                v2f o;
+                o.x = v[2];
+                o.y = v[3];
+                G.diffuse[(long long)(h + 1) * P + q] = o;
+            }
+        }
+    }
 }
 
 struct Coef7 {
@@ -2204,7 +2370,7 @@ __device__ __forceinline__ void adj_prep_day(const DomainG &G, const AdjG &A, co
         const AS4 double *pd = G.ptab_d + (long long)row * SW_PATCH_D * SW_PATCHES;
         float a_lv = 0.f, a_kd = 0.f, a_swsun = 0.f, a_swsh = 0.f, a_lsunc = 0.f, a_lsuns = 0.f;
         float a_lshc = 0.f, a_lshs = 0.f;
-        day_patch_sums(G.vis, P, p, pf, pd, H, G.tan_asvf[p], (float)H[SWH_DIFF_VEG], a_lv, a_kd,
+        day_patch_sums<true>(G.vis, P, p, pf, pd, H, G.tan_asvf[p], (float)H[SWH_DIFF_VEG], a_lv, a_kd,
                        a_swsun, a_swsh, a_lsunc, a_lsuns, a_lshc, a_lshs);
         const AS1 float *ss = G.skysums;
         const float SA = (float)H[SWH_SKY_A], SB = (float)H[SWH_SKY_B];
@@ -3883,6 +4049,15 @@ int sw_day_pixels_capped(const sw_domain *doms, const int32_t *bs, int nd, int t
     if (h1 == h0) return SW_OK;
     return launch_day_pixels(doms, bs, nd, total, h0, h1, gofs, gb, first, second, (hipStream_t)stream,
                              groups_per_cu);
+}
+
+int sw_day_diffuse(const sw_domain *doms, const int32_t *bs, int nd, int total, int n_hours, void *stream) {
+    const int rc = check_launch(doms, bs, nd, total);
+    if (rc) return rc;
+    if (n_hours < 1 || n_hours > 65535) return fail(SW_ERR_ARGUMENT, "sw_day_diffuse: bad hour count");
+    hipLaunchKernelGGL(k_day_diffuse, dim3((unsigned)total), dim3(BLOCK), 0, (hipStream_t)stream, doms, bs, nd,
+                       total, n_hours);
+    return hip_check(hipGetLastError(), "k_day_diffuse launch");
 }
 
 int sw_day_forward(const sw_domain *doms, const int32_t *bs, int nd, int total, int h0, int h1,

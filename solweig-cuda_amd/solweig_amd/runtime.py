@@ -21,6 +21,17 @@ SW_PATCHES = 153
 SW_VIS_WORDS = 5
 SW_MAX_PIXELS = (1 << 30) - 1   # largest raster of one domain (32-bit byte offsets of the marches)
 SWS_NIB, SWS_CSUN, SWS_CSH, SWS_MASK, SWS_ELIG, SW_SEARCH_WORDS = 0, 1248, 1760, 2272, 2536, 2560
+# the day's diffuse table of sw_day_diffuse: int8 digits in fragment order, then
+# double scale [rows], two rows per hour, rows = diff_rows(n_hours)
+SW_DIFF_K, SW_DIFF_DIGITS = 320, 4
+
+
+def diff_rows(n_hours):
+    return 16 * ((int(n_hours) + 7) // 8)
+
+
+def diff_table_bytes(n_hours):
+    return diff_rows(n_hours) * (SW_DIFF_DIGITS * SW_DIFF_K + 8)
 TILE_X, TILE_Y = 32, 8          # default workgroup tile (256 threads)
 TILE_CHOICES = (16, 32, 64, 128)  # tile widths the hour kernels accept (height = 256 / width)
 
@@ -58,7 +69,8 @@ class SwDomain(ctypes.Structure):
                                         'rec', 'lag', 'tmrt', 'utci', 'hours', 'ptab_f', 'ptab_d',
                                         'ray_off', 'ray_dz', 'masks', 'shadow_planes', 'full', 'skysums',
                                         'hmax')] + [('hmax_cols', ctypes.c_int64)] + \
-        [(n, ctypes.c_uint64) for n in ('gvf_len', 'gvf_nosh', 'march_trees', 'ptab_s', 'gvf_edge')]
+        [(n, ctypes.c_uint64) for n in ('gvf_len', 'gvf_nosh', 'march_trees', 'ptab_s', 'gvf_edge',
+                                        'diffuse_tab', 'diffuse')]
 HMAX_CELL = 16
 
 
@@ -118,6 +130,7 @@ def lib():
         'sw_day_shadow': ([vp, vp, i32, i32, i32, i32, vp], i32),
         'sw_day_pixels': ([vp, vp, i32, i32, i32, i32, vp, vp, i32, i32, vp], i32),
         'sw_day_pixels_capped': ([vp, vp, i32, i32, i32, i32, vp, vp, i32, i32, i32, vp], i32),
+        'sw_day_diffuse': ([vp, vp, i32, i32, i32, vp], i32),
         'sw_utci_scalar': ([vp, i64, vp, f64, i32, vp, vp, vp], i32),
         'sw_sun_position': ([vp, i32, i32, i32, f64, f64, f64, f64, vp, vp], i32),
         'sw_copy': ([vp, vp, sz, vp], i32),
@@ -147,7 +160,8 @@ EXPORTED = ('sw_version', 'sw_record_bytes', 'sw_last_error', 'sw_device_count',
             'sw_stream_destroy', 'sw_stream_sync', 'sw_event_create', 'sw_event_destroy',
             'sw_event_record', 'sw_event_elapsed_ms', 'sw_device_sync', 'sw_svf_visibility',
             'sw_svf_reduce', 'sw_sky_sums', 'sw_hour_shadow', 'sw_hour_pixels', 'sw_hours_forward', 'sw_day_forward',
-            'sw_day_shadow', 'sw_day_pixels', 'sw_day_pixels_capped', 'sw_sun_position', 'sw_utci_scalar',
+            'sw_day_shadow', 'sw_day_pixels', 'sw_day_pixels_capped', 'sw_day_diffuse', 'sw_sun_position',
+            'sw_utci_scalar',
             'sw_copy', 'sw_graph_begin', 'sw_graph_end', 'sw_graph_launch', 'sw_graph_destroy',
             'sw_adjoint_hour', 'sw_day_adjoint', 'sw_cylindric_wedge', 'sw_wave_delay', 'sw_kup', 'sw_lside',
             'sw_surface_temperature', 'sw_assemble_tmrt', 'sw_gvf', 'sw_pack_visibility', 'sw_longwave',

@@ -14,8 +14,8 @@ from functools import lru_cache
 
 import numpy as np
 
-from .runtime import (H, PD, PF, SW_HOUR_STRIDE, SW_PATCH_D, SW_PATCH_F, SW_PATCHES, SW_VIS_WORDS,
-                      SWS_CSH, SWS_CSUN, SWS_ELIG, SWS_MASK, SWS_NIB, SW_SEARCH_WORDS)
+from .runtime import (H, PD, PF, SW_DIFF_DIGITS, SW_DIFF_K, SW_HOUR_STRIDE, SW_PATCH_D, SW_PATCH_F, SW_PATCHES,
+                      SW_VIS_WORDS, SWS_CSH, SWS_CSUN, SWS_ELIG, SWS_MASK, SWS_NIB, SW_SEARCH_WORDS, diff_rows)
 from . import utci as _utci
 
 SBC = 5.67051e-8
@@ -543,6 +543,70 @@ def _with_breakpoints(rows, parts):
 
 # ----------------------------------------------------------------------------- a day
 
+def diffuse_table(hours, ptab_f):
+    """The day's diffuse table (sw_domain.diffuse_tab, the input of sw_day_diffuse) from its hour
+    records [n][SW_HOUR_STRIDE] and float patch rows [rows][153][SW_PATCH_F], as bytes.
+
+    Per daytime hour h, row 2h holds the patch luminance and row 2h + 1 lumChi cos omega (the
+    per-patch terms that umep_solweig.py:148-154 and shortwave.cu:155-157 weight by diffsh): at
+    positions k < 153 for the sky mask, at 160 + k times the hour's vegetation diffsh (as FP32,
+    the product in FP64, exact) for the vegetation-only mask. Each row is rounded to integers q
+    on a power-of-two scale with |q| <= 2^27 (27 bits of the row's largest magnitude) and stored
+    as four int8 digits q = d3 2^21 + d2 2^14 + d1 2^7 + d0 (d0..d2 in [0, 127]); a visibility
+    mask times a row is then an exact integer sum on the device. Rows of night hours and rows
+    past the day carry the scale -1 (not written by the device); a row holding a non-finite value
+    carries the scale NaN."""
+    n = hours.shape[0]
+    rows = diff_rows(n)
+    vals = np.zeros((rows, SW_DIFF_K))
+    day = np.zeros(rows, bool)
+    for h in range(n):
+        if not hours[h, H['DAY']]:
+            continue
+        f = ptab_f[int(hours[h, H['PATCH_TABLE']])]
+        dveg = float(np.float32(hours[h, H['DIFF_VEG']]))
+        for v, col in ((0, PF['LV']), (1, PF['KD'])):
+            x = np.asarray(f[:, col], dtype=np.float32).astype(np.float64)
+            vals[2 * h + v, :SW_PATCHES] = x
+            vals[2 * h + v, 160:160 + SW_PATCHES] = dveg * x
+        day[2 * h:2 * h + 2] = True
+    # a non-finite value makes every pixel's sum NaN, as 0 x NaN does in the reference's
+    # diffsh-weighted sum: the row gets q = 0 and the scale NaN
+    finite = np.isfinite(vals).all(axis=1)
+    vals[~finite] = 0.0
+    top = np.abs(vals).max(axis=1)
+    e = np.frexp(top)[1]                                  # top < 2^e (0 for an all-zero row)
+    scale = np.where(finite, np.ldexp(1.0, e - 27), np.nan)
+    q = np.rint(vals / np.where(finite, scale, 1.0)[:, None]).astype(np.int64)
+    digits = np.empty((SW_DIFF_DIGITS, rows, SW_DIFF_K), np.int8)
+    for d in range(SW_DIFF_DIGITS - 1):
+        digits[d] = q & 127
+        q >>= 7
+    digits[SW_DIFF_DIGITS - 1] = q
+    scale = np.where(day, scale, -1.0)
+    # the device's operand order: per 8-hour block, step of 64 positions and digit, the 64 lanes'
+    # 16-byte fragments (lane c + 16 g: row 16 block + c, positions 64 step + 16 g .. + 15)
+    frag = digits.reshape(SW_DIFF_DIGITS, rows // 16, 16, SW_DIFF_K // 64, 4, 16).transpose(1, 3, 0, 4, 2, 5)
+    return np.concatenate([np.ascontiguousarray(frag).reshape(-1).view(np.uint8), scale.view(np.uint8)])
+
+
+def diffuse_digits(table, n_hours):
+    """The int8 digits [SW_DIFF_DIGITS][rows][SW_DIFF_K] and the row scales of a diffuse table."""
+    rows = diff_rows(n_hours)
+    nd = SW_DIFF_DIGITS * rows * SW_DIFF_K
+    frag = np.asarray(table[:nd]).view(np.int8).reshape(rows // 16, SW_DIFF_K // 64, SW_DIFF_DIGITS, 4, 16, 16)
+    digits = frag.transpose(2, 0, 4, 1, 3, 5).reshape(SW_DIFF_DIGITS, rows, SW_DIFF_K)
+    return digits, np.asarray(table[nd:nd + 8 * rows]).view(np.float64)
+
+
+def diffuse_values(table, n_hours):
+    """The rows of a diffuse table as the integers they hold, [rows][SW_DIFF_K] int64, and the
+    row scales (value = q x |scale|)."""
+    digits, scale = diffuse_digits(table, n_hours)
+    q = sum(digits[d].astype(np.int64) << (7 * d) for d in range(SW_DIFF_DIGITS))
+    return q, scale
+
+
 class DayTables:
     """Host tables of every hour of one day for one domain (the inputs of sw_hours_forward)."""
 
@@ -550,10 +614,18 @@ class DayTables:
         self.hours, self.ptab_f, self.ptab_d = hours, ptab_f, ptab_d
         self.ray_off, self.ray_dz, self.meta = ray_off, ray_dz, meta
         self.ptab_s = ptab_s
+        self._diffuse = None
 
     @property
     def n_hours(self):
         return self.hours.shape[0]
+
+    @property
+    def diffuse(self):
+        """The day's diffuse table (diffuse_table), built on first use."""
+        if getattr(self, '_diffuse', None) is None:
+            self._diffuse = diffuse_table(self.hours, self.ptab_f)
+        return self._diffuse
 
 
 def hour_tables(alt, azi, zen, jday, dectime, altmax, Ta, RH, radG, P, Ws, doy, latitude, shape, scale,
@@ -705,5 +777,6 @@ def build_day(met, sun, latitude, shape, scale, amaxvalue, *, landcover=0, wall_
     ray_off = np.concatenate(offs) if offs else np.zeros((1, 2), np.int32)
     ray_dz = np.concatenate(dzs) if dzs else np.zeros(1)
     t = DayTables(hours, pf, pd, ray_off, ray_dz, meta, ps)
+    t._diffuse = diffuse_table(hours, pf)
     t.end_state = st[:3]
     return t

@@ -111,19 +111,20 @@ def tile_jobs(case, settings, tiles_dir):
 # tan(svfalfa) f64), neighbour record, mask byte and lag state 33, static sky sums 40, march canopy
 # plane 4, ground-view planes 80 (step counts, unshadowed albedo sums, in-raster step counts).
 STATIC_BYTES_PER_PIXEL = 278
-# Default pixel budget of one batch: at 24 h and 2 output slots about 129 GB of device memory and
-# 38 GB of page-locked host memory (batch_bytes).
-DEFAULT_BATCH_PIXELS = 100_000_000
+# Default pixel budget of one batch: at 24 h and 2 output slots about 134 GB of device memory and
+# 31 GB of page-locked host memory (batch_bytes).
+DEFAULT_BATCH_PIXELS = 80_000_000
 
 
 def batch_bytes(pixels, n_hours=24, slots=2):
     """(device bytes, page-locked host bytes) that a batch of ``pixels`` input pixels holds while
-    it runs: the static state, the shadow pass's per-hour records and mask bytes (9 B per hour, in
-    the two sets of the pipelined days), the day's Tmrt and UTCI (4 B each per hour) and, per output
-    slot, one device staging copy and one page-locked host copy of them (output.DayOutputs). The
-    sky-view pass adds 128 B per pixel of the domain being prepared while it runs."""
+    it runs: the static state, the shadow pass's per-hour records and mask bytes and the diffuse
+    sums (17 B per hour, in the two sets of the pipelined days), the day's Tmrt and UTCI (4 B each
+    per hour) and, per output slot, one device staging copy and one page-locked host copy of them
+    (output.DayOutputs). The sky-view pass adds 128 B per pixel of the domain being prepared while
+    it runs."""
     per_day = 8 * int(n_hours)
-    device = int(pixels) * (STATIC_BYTES_PER_PIXEL + 2 * 9 * int(n_hours) + per_day * (1 + int(slots)))
+    device = int(pixels) * (STATIC_BYTES_PER_PIXEL + 2 * 17 * int(n_hours) + per_day * (1 + int(slots)))
     host = int(pixels) * per_day * int(slots)
     return device, host
 

@@ -50,15 +50,15 @@ def test_hour_record_layout_matches_header():
     assert vals['SW_HOUR_STRIDE'] == rt.SW_HOUR_STRIDE
 
 
-def test_domain_struct_is_320_bytes():
-    assert ctypes.sizeof(rt.SwDomain) == 320
+def test_domain_struct_is_336_bytes():
+    assert ctypes.sizeof(rt.SwDomain) == 336
     text = header_text()[header_text().index('typedef struct sw_domain'):header_text().index('} sw_domain;')]
     fields = re.findall(r'\*(\w+)', text)
     names = [f[0] for f in rt.SwDomain._fields_ if f[0] not in ('rows', 'cols', 'pixels', 'landcover',
                                                                 'dsm_raise', 'tile_cols', 'hmax_cols')]
     assert fields == names
-    assert [f[0] for f in rt.SwDomain._fields_][-6:] == ['hmax_cols', 'gvf_len', 'gvf_nosh', 'march_trees',
-                                                         'ptab_s', 'gvf_edge']
+    assert [f[0] for f in rt.SwDomain._fields_][-8:] == ['hmax_cols', 'gvf_len', 'gvf_nosh', 'march_trees',
+                                                         'ptab_s', 'gvf_edge', 'diffuse_tab', 'diffuse']
 
 
 def test_adjoint_slots_and_struct_match_header():
@@ -106,7 +106,7 @@ def test_bindings_refuse_a_library_of_another_record_size(monkeypatch):
 
     monkeypatch.setattr(rt, '_lib', None)
     monkeypatch.setattr(rt, 'SwDomain', Wider)
-    with pytest.raises(rt.SolweigHipError, match='320-byte .* records, these bindings use 328'):
+    with pytest.raises(rt.SolweigHipError, match='336-byte .* records, these bindings use 344'):
         rt.lib()
 
 

@@ -26,7 +26,7 @@ def test_engine_refuses_mixed_pixel_sizes():
 def test_batch_bytes_model():
     dev, host = tiling.batch_bytes(1_000_000, n_hours=24, slots=2)
     # two sets of per-hour records (pipelined days), Tmrt + UTCI and two output staging slots
-    assert dev == 1_000_000 * (tiling.STATIC_BYTES_PER_PIXEL + 2 * 9 * 24 + 192 * 3)
+    assert dev == 1_000_000 * (tiling.STATIC_BYTES_PER_PIXEL + 2 * 17 * 24 + 192 * 3)
     assert host == 1_000_000 * 384
     dev, host = tiling.batch_bytes(tiling.DEFAULT_BATCH_PIXELS)
     assert dev < 140e9 and host < 50e9          # the default batch fits one 288 GB device with room

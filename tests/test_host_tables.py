@@ -266,3 +266,54 @@ def test_day_search_rows_equal_the_single_hour_rows():
     assert len(day) > 5
     for i in day:
         np.testing.assert_array_equal(tab.ptab_s[i], sky.search_row(tab.ptab_f[i], tab.ptab_d[i].T))
+
+
+def test_diffuse_table_holds_each_hour_to_27_bits():
+    """sky.diffuse_table (the input of sw_day_diffuse): per daytime hour the luminance row and the
+    lumChi cos omega row of its patches (sky positions k, vegetation-only positions 160 + k times
+    the hour's vegetation diffsh in FP32), each as integers q on a power-of-two scale, |q| <= 2^27,
+    within half a unit of the value, in int8 digits of base 128; night rows carry the scale -1 and
+    a row with a non-finite value the scale NaN."""
+    from solweig_amd import runtime as rt
+    d = synthetic.small_domain(seed=5, size=24, latitude=40.0)
+    s = psun.met_sun(d.met, d.location, d.utc)
+    t = sky.build_day(d.met[:24], {k: v[:24] for k, v in s.items()}, d.latitude, d.dsm.shape, d.scale, 30.0)
+    rows = rt.diff_rows(24)
+    assert rows == 48 and t.diffuse.nbytes == rt.diff_table_bytes(24) == rows * (4 * 320 + 8)
+    nd = 4 * rows * 320
+    digits, scale = sky.diffuse_digits(t.diffuse, 24)
+    digits = digits.astype(np.int64)
+    # the fragment order of the device: block 1, step 2, digit 3, lane 5 (c 5, g 0) = row 21,
+    # positions 128..143 of digit 3
+    frag = t.diffuse[:nd].view(np.int8).reshape(3, 5, 4, 64, 16)
+    assert np.array_equal(frag[1, 2, 3, 5], digits[3, 21, 128:144])
+    assert np.array_equal(frag[0, 4, 0, 63], digits[0, 15, 304:320])
+    assert digits[:3].min() >= 0 and digits[:3].max() <= 127
+    assert digits[3].min() >= -64 and digits[3].max() <= 64
+    q = digits[0] + 128 * digits[1] + 128 ** 2 * digits[2] + 128 ** 3 * digits[3]
+    n_day = 0
+    for h in range(24):
+        r = slice(2 * h, 2 * h + 2)
+        if not t.hours[h, H['DAY']]:
+            assert np.all(scale[r] == -1.0) and not q[r].any()
+            continue
+        n_day += 1
+        f = t.ptab_f[int(t.hours[h, H['PATCH_TABLE']])].astype(np.float64)
+        dveg = float(np.float32(t.hours[h, H['DIFF_VEG']]))
+        for v, col in ((0, PF['LV']), (1, PF['KD'])):
+            want = np.zeros(320)
+            want[:153] = f[:, col]
+            want[160:313] = dveg * f[:, col]
+            sc = scale[2 * h + v]
+            assert np.frexp(sc)[0] == 0.5                               # a power of two
+            assert np.abs(q[2 * h + v]).max() <= 2 ** 27
+            assert np.abs(want).max() <= 2 ** 27 * sc < 2 * np.abs(want).max()   # 27 bits of the top
+            assert np.all(np.abs(q[2 * h + v] * sc - want) <= sc / 2)
+    assert n_day >= 10
+    # a non-finite patch value: the row's sums are NaN for every pixel (0 x NaN in the reference)
+    pf = t.ptab_f.copy()
+    h = int(np.flatnonzero(t.hours[:, H['DAY']] != 0)[3])
+    pf[int(t.hours[h, H['PATCH_TABLE']]), 7, PF['KD']] = np.nan
+    bad_digits, bs = sky.diffuse_digits(sky.diffuse_table(t.hours, pf), 24)
+    assert np.isnan(bs[2 * h + 1]) and bs[2 * h] == scale[2 * h]
+    assert not bad_digits[:, 2 * h + 1].any() and np.array_equal(bad_digits[:, 2 * h], digits[:, 2 * h])
