@@ -52,7 +52,9 @@ import numpy as np  # noqa: E402
 # Algorithmic bytes per pixel-hour of the forward: the static inputs at their minimal exact size,
 # read once, plus the outputs. Shadow pass: DSM, canopy, walls, aspect (16 B). Pixel pass: ground
 # bit (0.125), 15 SVF planes FP32 (60), tan(asvf) and tan(svfalfa) FP64 (16), 3 x 153 visibility
-# bits (57.375), Tmrt + UTCI FP32 (8): 141.5 B.
+# bits (57.375), Tmrt + UTCI FP32 (8): 141.5 B. (The diffuse sums of sw_day_diffuse, like the shadow
+# pass's records, are intermediates: the pre-pass reads the sky and vegetation bits, the pixel pass
+# the building bits.)
 B_SHADOW = 16.0
 B_PIXELS = 141.5
 B_ALG = B_SHADOW + B_PIXELS
@@ -61,12 +63,12 @@ DIFF_EVENT = 3 * 24    # per eager day: the events around the diffuse-sum launch
 # HBM bytes per pixel-pass launch at 2000x2000: rocprofv3 FETCH_SIZE scaled by the k_copy
 # calibration + WRITE_SIZE, from the profile named in TRAFFIC_SOURCE (profile values, not measured by
 # this run): k_day_pixels (one launch for 24 hours), k_hour_pixels (mean over a day's 24 launches).
-TRAFFIC_2000 = {'k_day_pixels': 25.81e9, 'k_hour_pixels': 0.821e9}
+TRAFFIC_2000 = {'k_day_pixels': 23.98e9, 'k_hour_pixels': 0.796e9}
 TRAFFIC_SOURCE = 'profiles/k_day_pixels_c2_2000x2000.md'
 # VALU instructions (wave64) per pixel-pass launch at 2000x2000, SQ_INSTS_VALU x SQ_WAVES from the same
 # profile, and the single-issue VALU rate of MI355X (256 CUs x 2.4 GHz, one wave64 instruction per CU
 # per cycle without the packed FP32 forms): the ceiling the pixel pass actually runs against.
-VALU_2000 = {'k_day_pixels': 53301 * 62500, 'k_hour_pixels': 2141 * 62500}
+VALU_2000 = {'k_day_pixels': 45554 * 62500, 'k_hour_pixels': 1873 * 62500}
 VALU_PEAK = 256 * 2.4e9
 FORM_TEXT = {
     'pipelined': 'resident days pipelined over two streams (each day\'s all-hours shadow pass beside the '
@@ -677,7 +679,7 @@ def main():
     out = dict(metric='Tmrt+UTCI pixel-hours/s at 2000^2 x24h, 1 GPU; batched forward',
                value=value, unit='pixel-hours/s', n_gpus=1, steps=args.steps, warmup=args.warmup,
                ms_per_step=dt / args.steps * 1e3, higher_is_better=True, scaling='weak', vs_baseline=None,
-               dtype='f32 (f64 masks/lag/wedge/UTCI)', data='synthetic',
+               dtype='f32 (f64 masks/lag/UTCI; int8 x int8 -> int32 diffuse sums)', data='synthetic',
                config=dict(workload=f'c2 batched forward {args.size}x{args.size}, 24 h per step (30-day forcing '
                                     f'cycled), day tables built '
                                     f'{f"by {n_workers} CPU worker processes" if n_workers else "in the main process"}'

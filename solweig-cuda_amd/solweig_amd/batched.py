@@ -314,10 +314,11 @@ class Engine:
         self.diffuse(k, stream)
 
     def _pixel_pass(self, k, stream, beside_shadow=False):
-        """Pixel pass of set k; ``beside_shadow``: a shadow pass runs on the side stream meanwhile,
-        so at most ``groups_beside_shadow`` pixel workgroups share a CU (at 2000^2, 4 per CU
-        leave two shadow waves' registers on every SIMD: 7.81-7.84 ms per pipelined day against
-        7.8-8.4 ms with the 5 that registers and LDS allow)."""
+        """Pixel pass of set k; ``beside_shadow``: the next day's shadow pass and diffuse sums run
+        on the side stream meanwhile, so at most ``groups_beside_shadow`` pixel workgroups share a
+        CU (at 2000^2: 7.31-7.36 ms per pipelined day with 3 per CU, 7.37-7.44 with 4, 7.49-7.50
+        with 5 or no cap, 8.9 with 2). Splitting the day's pixel pass into a capped part beside
+        the shadow pass and an uncapped rest gave no gain (7.32-7.40 ms for splits at hours 8-18)."""
         if beside_shadow and self.groups_beside_shadow:
             rt.call('sw_day_pixels_capped', *self._args(k), self.gvf_off.ptr, self.gvf_bounds.ptr, self.first,
                     self.second, self.groups_beside_shadow, stream.handle)

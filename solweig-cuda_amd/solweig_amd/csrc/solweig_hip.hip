@@ -1258,10 +1258,18 @@ __device__ __forceinline__ float wedge(double ta, double tb) {
     if (v <= 1.0f) return 1.0f - v * INV_PI;
     if (v > 1048576.0f) return 0.5f + (0.5f * INV_PI) * __builtin_amdgcn_rcpf(v);
     const float v2 = v * v;
-    const float R = __builtin_sqrtf(v2 + 0.75f), t = __builtin_sqrtf(v2 - 0.25f);
+    // hardware square roots (v_sqrt_f32, within 1 ulp; both arguments are >= 0.75)
+    const float R = __builtin_amdgcn_sqrtf(v2 + 0.75f), t = __builtin_amdgcn_sqrtf(v2 - 0.25f);
     const float phi = atanf(t);
     const float num = v * __builtin_amdgcn_rcpf(R + t) - 0.25f * __builtin_amdgcn_rcpf(t + v) + phi * (v - 1.0f);
     return 1.0f - num * INV_PI * __builtin_amdgcn_rcpf(R - 1.0f);
+}
+
+// Tmrt from the absorbed radiation (umep_solweig.py:360): the fourth root as two hardware square
+// roots (v_sqrt_f32, within 1 ulp each; the correctly rounded sqrtf takes about 15 instructions
+// more per root)
+__device__ __forceinline__ float tmrt_of(float Sstr, float absL) {
+    return __builtin_amdgcn_sqrtf(__builtin_amdgcn_sqrtf(Sstr / (absL * (float)SBC))) - 273.2f;
 }
 
 __device__ __forceinline__ float lag(float current, float &state, double w, bool first,
@@ -1650,7 +1658,7 @@ __device__ __forceinline__ void pixel_hour(const DomainG &G, const PixelAt &at, 
     const float Lsum = (LupN + LupE + LupS + LupW) * 0.5f;
     const float Sstr = absK * (Kside * Fcyl + (Kdown + Kup) * Fup + Ksum * Fside) +
                        absL * ((Ldown + Lup) * Fup + Lside * Fcyl + Lsum * Fside);
-    const float Tmrt = sqrtf(sqrtf(Sstr / (absL * (float)SBC))) - 273.2f;
+    const float Tmrt = tmrt_of(Sstr, absL);
     // UTCI with the hour's scalar weather (utci.cu:500-523), NaN off the ground mask.
     float u;
     if (!G.ground[p]) {
@@ -2394,7 +2402,7 @@ __device__ __forceinline__ void adj_prep_day(const DomainG &G, const AdjG &A, co
         const float Lsum = (LupN + LupE + LupS + LupW) * 0.5f;
         const float Sstr = absK * (Kside * Fcyl + (Kdown + Kup) * Fup + Ksum * Fside) +
                            absL * ((Ldown + Lup) * Fup + Lside * Fcyl + Lsum * Fside);
-        const float Tmrt = sqrtf(sqrtf(Sstr / (absL * (float)SBC))) - 273.2f;
+        const float Tmrt = tmrt_of(Sstr, absL);
         const bool ground = G.ground[p] != 0;
         // ---- reverse ---------------------------------------------------------------------
         float lU = 0.f, lT = 0.f;
