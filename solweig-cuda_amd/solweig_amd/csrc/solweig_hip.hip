@@ -1887,7 +1887,8 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(3))) void
         // rows 16 mt + 4 g + i: hour h = 8 mt + 2 g (i = 0, 1) and h + 1 (i = 2, 3). The integer
         // sum is hi 2^14 + lo with hi = acc3 2^7 + acc2 and lo = acc1 2^7 + acc0, both exact in
         // int32 and in FP32 (below 2^23): one fma rounds it once, and the power-of-two scale is
-        // exact (a NaN scale gives NaN, as the reference's 0 x NaN terms do)
+        // exact (a NaN scale gives NaN, as the reference's 0 x NaN terms do). A row whose values
+        // all lie below 2^-99 would have a scale under the FP32 range: its sums become 0.
         const int r0 = 16 * mt + 4 * g, h = 8 * mt + 2 * g;
         float sc[4];
 #pragma unroll
