@@ -34,29 +34,30 @@ def _table_integers(table, n_hours):
     return d[0] + d[1] * 128 + d[2] * 128 ** 2 + d[3] * 128 ** 3, scale
 
 
-@pytest.mark.parametrize('tile', [16, 32, 128])
-def test_day_diffuse_sums_are_exact_products(rt, tile):
+@pytest.mark.parametrize('tile,n_hours', [(16, 24), (32, 24), (128, 24), (16, 13)])
+def test_day_diffuse_sums_are_exact_products(rt, tile, n_hours):
+    """13 hours: two 8-hour blocks of table rows, the second reaching past the day."""
     from solweig_amd import batched, synthetic, sun as psun
     doms = [synthetic.small_domain(seed=80 + i, size=n, latitude=lat, maps=(i % 2 == 0))
             for i, (n, lat) in enumerate([(45, 12.0), (31, 52.0), (38, 30.0)])]
-    suns = [psun.met_sun(d.met, d.location, d.utc) for d in doms]
-    eng = batched.Engine(doms, n_hours=24)
+    suns = [{k: v[:n_hours] for k, v in psun.met_sun(d.met, d.location, d.utc).items()} for d in doms]
+    eng = batched.Engine(doms, n_hours=n_hours)
     eng.set_tiles(tile)
-    tabs = eng.tables(sun=suns)
+    tabs = eng.tables(sun=suns, met=[d.met[:n_hours] for d in doms])
     eng.load(tabs)
     for b in eng.buf:                                 # all-ones bits: a NaN left where nothing is written
         rt.call('sw_memset', b['diffuse'].ptr, 0xff, b['diffuse'].nbytes, eng.stream.handle)
-    print(f'tile {tile}: sw_day_diffuse over {len(doms)} domains', flush=True)
+    print(f'tile {tile}, {n_hours} h: sw_day_diffuse over {len(doms)} domains', flush=True)
     eng.diffuse()
     eng.stream.sync()
     n_day = 0
     for k, (pdom, t, b) in enumerate(zip(eng.domains, tabs, eng.buf)):
         P = pdom.P
-        got = b['diffuse'].to_host(eng.stream)                          # [24][P][2]
+        got = b['diffuse'].to_host(eng.stream)                          # [hours][P][2]
         bits = _mask_bits(pdom.dev['vis'].to_host(eng.stream).reshape(3, 5, P), P)
-        q, scale = _table_integers(t.diffuse, 24)
+        q, scale = _table_integers(t.diffuse, n_hours)
         exact = bits @ q.T                                              # [P][rows], exact int64
-        for h in range(24):
+        for h in range(n_hours):
             if not t.hours[h, rt.H['DAY']]:
                 assert np.all(got[h].view(np.uint32) == 0xffffffff), (k, h)
                 continue
