@@ -459,7 +459,7 @@ def main():
     ap.add_argument('--stepwise-days', type=int, default=1,
                     help='also time the reference-decomposition per-step forward for this many days (0 = skip)')
     ap.add_argument('--no-c4', action='store_true', help='skip the 64-domain uneven batch leg')
-    ap.add_argument('--c4-steps', type=int, default=3)
+    ap.add_argument('--c4-steps', type=int, default=8, help='timed days of the c4 batch (each run starts with a lone shadow pass and ends with a lone pixel pass)')
     ap.add_argument('--c4-count', type=int, default=64, help='domains of the c4 batch')
     ap.add_argument('--c4-audit', type=int, default=3,
                     help='c4 domains audited against the FP64 step decomposition (smallest, median, largest, ...)')
