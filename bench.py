@@ -323,10 +323,11 @@ def c4_leg(args, workers):
     t_tab_serial = time.perf_counter() - t0
     warm = max(1, args.warmup)
     pipe = DayPipeline(eng, [0] * (warm + args.c4_steps))
-    run_days(eng, pipe, warm, use_graph=False)
+    use_graph = not args.eager
+    run_days(eng, pipe, warm, use_graph=use_graph)
     rt.synchronize()
     t0 = time.perf_counter()
-    run_days(eng, pipe, args.c4_steps, use_graph=False)
+    run_days(eng, pipe, args.c4_steps, use_graph=use_graph)
     rt.synchronize()
     dt = time.perf_counter() - t0
     value = P * 24 * args.c4_steps / dt
@@ -350,7 +351,9 @@ def c4_leg(args, workers):
     return dict(value=value, unit='pixel-hours/s', ms_per_day=dt / args.c4_steps * 1e3, steps=args.c4_steps,
                 workload=f'c4 uneven batch: {len(doms)} domains, sides {min(sides)}..{max(sides)}, {P} pixels, '
                          f'24 h per step, pipelined resident days (one launch per pass and day over all '
-                         f'domains), day tables built in the timed region',
+                         f'domains{", replayed from hipGraphs" if use_graph else ""}), day tables built in the '
+                         f'timed region',
+                graph=use_graph,
                 tile=args.tile or 16, generation_s=t_gen, static_prep_s=t_prep,
                 host_tables_one_core_ms=t_tab_serial * 1e3, table_workers=eng.workers, checks=checks,
                 fp64_audit=audit)
