@@ -71,11 +71,11 @@ TRAFFIC_SOURCE = 'profiles/k_day_pixels_c2_2000x2000.md'
 VALU_2000 = {'k_day_pixels': 45554 * 62500, 'k_hour_pixels': 1873 * 62500}
 VALU_PEAK = 256 * 2.4e9
 FORM_TEXT = {
-    'pipelined': 'resident days pipelined over two streams (each day\'s all-hours shadow pass beside the '
-                 'previous day\'s pixel pass, batched.Engine.run_days; the timed run starts with a lone shadow '
-                 'pass and ends with a lone pixel pass)',
-    'resident': 'the resident day (sw_day_forward)',
-    'hourly': 'two launches per hour (sw_hours_forward)',
+    'pipelined': 'resident days pipelined over two streams (each day\'s all-hours shadow pass and its diffuse '
+                 'sums on the matrix cores beside the previous day\'s pixel pass, batched.Engine.run_days; the '
+                 'timed run starts with a lone shadow pass and ends with a lone pixel pass)',
+    'resident': 'the resident day (sw_day_diffuse, then sw_day_forward)',
+    'hourly': 'the diffuse sums of the day, then two launches per hour (sw_hours_forward)',
 }
 
 
